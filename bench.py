@@ -1,0 +1,148 @@
+#!/usr/bin/env python3
+"""Throughput of the tracer hot path on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): 1920x1080,
+64 spp, BOUNCE_LIMIT 8, 64 instances of 4 noise-displaced geodesic spheres
+(808,500 triangles, two-level TLAS/BLAS), 2048x1024 HDR environment with
+importance-sampled NEE.  One step = one 64-spp progressive frame (one
+pt_trace call with 64 passes) with the scene resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Prints one JSON line.  Mrays/s = (extension rays + shadow rays) / time, rays
+counted on the device exactly as the reference issues them (one trace_scene
+call = one extension ray, one trace_scene_shadow call = one shadow ray).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s @1080p 8-bounce; equal-time relMSE vs 16k-spp ref (learned vs à-trous)"
+W, H, SPP = 1920, 1080, 64
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(st, sky_w, sky_h):
+    """Bytes the traversal and shading must touch:
+    32 per node pop, +48 per inner pop (two child boxes), 96 per TLAS-leaf
+    object, 36 per triangle test, and per shading event 144 (material) + 96
+    (object) + 80 (winning triangle's normals/uvs) + 2 sky lookups x 4 texels
+    x 8 B + CDF binary-search reads; plus 36 B of accum/output per path."""
+    cdf_reads = (math.ceil(math.log2(max(sky_h, 2))) + math.ceil(math.log2(max(sky_w, 2)))) * 4
+    shade = 144 + 96 + 80 + 2 * 4 * 8 + cdf_reads
+    return (32 * st.node_pops + 48 * st.inner_pops + 96 * st.object_visits + 36 * st.tri_tests +
+            shade * st.shade_events + 36 * st.paths)
+
+
+def cpu_baseline(scene, params, budget_s):
+    """The oracle (a CPU restatement of shader.wgsl `main`) on the same scene,
+    1 spp on every 8th row, repeated until budget_s of work; all host threads."""
+    from oracle.refcpu import RefScene
+    ref = RefScene(scene.desc())
+    threads = os.cpu_count() or 1
+    rays = 0
+    t0 = time.perf_counter()
+    passes = 0
+    while True:
+        params.sample_number = passes
+        params.seed = 1000 + passes
+        _, _, st = ref.render(params, passes=1, threads=threads, row_stride=8, with_output=False)
+        rays += st.extension_rays + st.shadow_rays
+        passes += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": "oracle FP32 restatement, same scene and camera, 1 spp x %d passes over every 8th row "
+                      "of 1920x1080 (%d rays in %.1f s)" % (passes, rays, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-count", action="store_true", help="time the kernel without node/triangle counters")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the frame would shard by sample "
+              "index across GPUs, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    from webgpuraytracer_amd import Renderer, SceneBuilder
+    from webgpuraytracer_amd import _abi as A
+
+    t0 = time.perf_counter()
+    scene = SceneBuilder.config(2, seed=args.seed)
+    build_s = time.perf_counter() - t0
+    r = Renderer(0)
+    t0 = time.perf_counter()
+    r.set_scene(scene)
+    r.resize(W, H)
+    upload_s = time.perf_counter() - t0
+    r.set_option(A.PT_OPT_COUNT_STATS, 0 if args.no_count else 1)
+    d = scene.desc()
+
+    def step(k):
+        seeds = [(args.seed * 0x9E3779B1 + (k * SPP + i) * 0x85EBCA77) & 0xFFFFFFFF for i in range(SPP)]
+        p = scene.params(W, H, sample_number=0, seed=seeds[0])
+        return r.render_section(p, passes=SPP, seeds=seeds)
+
+    for k in range(args.warmup):
+        step(1000 + k)
+    r.lib.pt_synchronize(r.ctx)
+    stats = []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        stats.append(step(k))
+    r.lib.pt_synchronize(r.ctx)
+    wall = time.perf_counter() - t_start
+
+    rays = sum(s.extension_rays + s.shadow_rays for s in stats)
+    kern_ms = sum(s.kernel_ms for s in stats)
+    value = rays / wall / 1e6
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1000 / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "plain tracer, large scene: 1920x1080, 64 spp/step, 8 bounces, 64 instances of 4 "
+                               "displaced geodesic spheres (%d triangles), 2048x1024 env, megakernel"
+                               % d.triangles.count,
+                   "extension_rays": int(sum(s.extension_rays for s in stats)),
+                   "shadow_rays": int(sum(s.shadow_rays for s in stats)),
+                   "paths": int(sum(s.paths for s in stats)),
+                   "kernel_mrays_s": round(rays / (kern_ms / 1000) / 1e6, 2) if kern_ms > 0 else None,
+                   "scene_build_s": round(build_s, 3), "upload_s": round(upload_s, 3)},
+    }
+    if not args.no_count:
+        per_launch = [algorithmic_bytes(s, d.sky_width, d.sky_height) for s in stats]
+        avg_bytes = sum(per_launch) / len(per_launch)
+        avg_ms = kern_ms / len(stats)
+        achieved = avg_bytes / (avg_ms / 1000) / 1e9
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "trace_kernel_c2_traffic.json")
+        if os.path.exists(tf):
+            with open(tf) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                           "algorithmic_bytes_per_launch": int(avg_bytes), "kernel_ms_per_launch": round(avg_ms, 3)}
+    if not args.no_cpu:
+        p = scene.params(W, H, 0, 1)
+        out["cpu_baseline"] = cpu_baseline(scene, p, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    r.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

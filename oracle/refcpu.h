@@ -1,0 +1,82 @@
+/*
+ * refcpu.h — the oracle: a CPU restatement of the reference's hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (libpt, libptscene, the
+ * Python package, the Node addon) links, loads or calls this code; only the
+ * tests, the smoke check and the CPU-baseline leg of the benchmark do, and
+ * only as the checker or as the measured CPU baseline.
+ *
+ * What it restates (file:line into the reference):
+ *   ref_trace_rays        trace_scene            shader.wgsl:754-774 (+ 473-752)
+ *   ref_trace_shadow_rays trace_scene_shadow     shader.wgsl:1206-1229 (+ 777-1184)
+ *   ref_render            main                   shader.wgsl:2273-2492
+ *   ref_trace_hits        main's first bounces, recording each trace_scene hit
+ *   ref_gbuffer           gbuf_main              shader.wgsl:2510-2603
+ *   ref_denoise           prepass/atrous/composite denoiser.wgsl:55-214, pass
+ *                         schedule Denoiser.execute tracer.js:2583-2637
+ *   ref_build_sky_cdf     HDRTexture.buildHDRCDF tracer.js:135-204
+ *   ref64_closest         brute-force closest hit in double precision over every
+ *                         plane, object and triangle (no BVH): the FP64 arbiter
+ *
+ * Arithmetic rules (the WGSL leaves these to the implementation; the oracle
+ * fixes them and the HIP kernels follow the same rules):
+ *   - f32 everywhere, no contraction (compiled with -ffp-contract=off), each
+ *     WGSL expression evaluated left to right;
+ *   - dot(a,b) = (a.x*b.x + a.y*b.y) + a.z*b.z; normalize(v) = v / length(v);
+ *     mat*vec sums the column products in column order;
+ *   - min/max return the non-NaN operand (IEEE minNum/maxNum);
+ *   - pow, exp, log, sin, cos, atan2, asin, acos, tan are the C library's
+ *     (pow of a negative base with an integral exponent is the signed power);
+ *   - textureSampleLevel on the sky is bilinear with fp32 weights, texel
+ *     centres at (i+0.5)/size, clamp-to-edge, f16 texels widened exactly;
+ *   - rgba8unorm stores round(clamp(x,0,1)*255) to nearest, NaN -> 0.
+ */
+#ifndef PT_REFCPU_H
+#define PT_REFCPU_H
+
+#include "pt_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ref_scene ref_scene;
+
+int ref_scene_create(ref_scene** out, const pt_scene_desc* desc);
+void ref_scene_destroy(ref_scene* s);
+
+int ref_trace_rays(const ref_scene* s, const float* rays, uint32_t n, pt_hit* hits, pt_stats* stats);
+int ref_trace_shadow_rays(const ref_scene* s, const float* rays, const int32_t* target_idx,
+                          const float* target_dist, uint32_t n, float* transmittance);
+
+/* n_passes progressive samples into accum (width*height*4 floats, in/out);
+ * out_rgba8 may be NULL.  rows/row_stride restrict the run to rows
+ * row0, row0+row_stride, ... (row_stride 1 = all rows of the section).
+ * threads <= 0 uses every hardware thread. */
+int ref_render(const ref_scene* s, const pt_scene_params* params, uint32_t section_w,
+               uint32_t section_h, uint32_t n_passes, const uint32_t* seeds, float* accum,
+               uint8_t* out_rgba8, pt_stats* stats, int threads, uint32_t row_stride);
+
+/* One sample per pixel (sample_number/seed from params); records the hit of
+ * each of the first n_bounces trace_scene calls per pixel into
+ * hits[pixel * n_bounces + b] (t = -1, ids -1 when the path ended earlier). */
+int ref_trace_hits(const ref_scene* s, const pt_scene_params* params, uint32_t n_bounces,
+                   pt_hit* hits, int threads);
+
+int ref_gbuffer(const ref_scene* s, const pt_scene_params* params, pt_gbuffer_texel* gbuf,
+                uint8_t* out_rgba8, int threads);
+
+int ref_denoise(const pt_denoise_params* params, int stride_mode, const float* accum,
+                const pt_gbuffer_texel* gbuf, float* out_rgba, uint8_t* out_rgba8);
+
+int ref_build_sky_cdf(const uint16_t* rgba16f, uint32_t width, uint32_t height, float exposure,
+                      float* cdf, double* total_weight);
+
+/* rays: 6 doubles; fills *hit with the double-precision closest hit (t in
+ * hit->t rounded to float, exact value in *t_exact). */
+int ref64_closest(const ref_scene* s, const double* ray, pt_hit* hit, double* t_exact);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

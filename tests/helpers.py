@@ -1,0 +1,71 @@
+"""Shared helpers for the parity tests: scenes, rays and hit comparison."""
+import ctypes as C
+
+import numpy as np
+
+from webgpuraytracer_amd import SceneBuilder
+from webgpuraytracer_amd import _abi as A
+
+
+def hits_to_np(hits):
+    a = np.ctypeslib.as_array(hits)
+    return {
+        "t": a["t"].astype(np.float32),
+        "o": a["o_idx"].astype(np.int64),
+        "m": a["m_idx"].astype(np.int64),
+        "prim": a["prim_id"].astype(np.int64),
+        "plane": a["plane_idx"].astype(np.int64),
+        "count": a["count"].astype(np.int64),
+        "n": np.array(a["n"], np.float32),
+        "uv": np.array(a["uv"], np.float32),
+    }
+
+
+def small_mesh_scene(seed=3, freq=(6, 8, 10, 12), n_instances=12, sky=(64, 32), bounces=4):
+    """A config-2-style instanced-mesh scene small enough for the oracle."""
+    return SceneBuilder.config(2, seed=seed, sky=sky, mesh_freq=freq, n_instances=n_instances, bounces=bounces)
+
+
+def random_rays(n, seed, box=3.0, toward=(0.0, 0.0, 0.0), spread=2.5):
+    """Rays from random origins in a box toward random targets near `toward`."""
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(-box, box, size=(n, 3)).astype(np.float32)
+    o[:, 1] = np.abs(o[:, 1]) + 0.5
+    tgt = (np.asarray(toward, np.float32) + rng.uniform(-spread, spread, size=(n, 3))).astype(np.float32)
+    d = (tgt - o).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True).astype(np.float32)
+    return np.concatenate([o, d], axis=1).astype(np.float32)
+
+
+def compare_hits(gpu, ref, rays, refscene, rel_t=1e-5):
+    """Identities must be equal and t within rel_t; a mismatch is accepted only
+    when the FP64 brute-force arbiter finds the two candidates tied (their
+    double-precision distances within rel_t) and so cannot separate them.
+    Returns a summary dict; raises AssertionError on a real mismatch."""
+    g, r = hits_to_np(gpu), hits_to_np(ref)
+    same_id = (g["o"] == r["o"]) & (g["m"] == r["m"]) & (g["prim"] == r["prim"]) & (g["plane"] == r["plane"])
+    both_miss = (r["m"] == -1) & (g["m"] == -1)
+    tdiff = np.abs(g["t"].astype(np.float64) - r["t"].astype(np.float64))
+    t_ok = both_miss | (tdiff <= rel_t * np.maximum(np.abs(r["t"].astype(np.float64)), 1e-6))
+    bad = np.nonzero(~(same_id & t_ok))[0]
+    adjudicated = 0
+    for i in bad:
+        if rays is None:
+            raise AssertionError("hit %d differs: gpu %s ref %s" % (i, {k: g[k][i] for k in ("t", "o", "m", "prim", "plane")},
+                                                                  {k: r[k][i] for k in ("t", "o", "m", "prim", "plane")}))
+        rc, h64, t64 = refscene.closest64([float(x) for x in rays[i]])
+        assert rc == 0, "FP64 arbiter unavailable for ray %d" % i
+        close_g = abs(float(g["t"][i]) - t64) <= rel_t * max(t64, 1e-6)
+        close_r = abs(float(r["t"][i]) - t64) <= rel_t * max(t64, 1e-6)
+        assert close_g and close_r, "hit %d: gpu t=%r (o=%d prim=%d) ref t=%r (o=%d prim=%d) fp64 t=%r (o=%d prim=%d)" % (
+            i, g["t"][i], g["o"][i], g["prim"][i], r["t"][i], r["o"][i], r["prim"][i], t64, h64.o_idx, h64.prim_id)
+        adjudicated += 1
+    exact_t = int(np.sum((g["t"] == r["t"]) & same_id))
+    return {"n": len(g["t"]), "exact_t": exact_t, "adjudicated": adjudicated,
+            "max_rel_t": float(np.max(np.where(both_miss, 0.0, tdiff / np.maximum(np.abs(r["t"]), 1e-6))))}
+
+
+def rel_mse(x, ref):
+    x = np.asarray(x, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.mean((x - ref) ** 2 / (ref ** 2 + 1e-2)))

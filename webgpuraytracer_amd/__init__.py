@@ -1,0 +1,10 @@
+"""MI355X-native path-tracer hot path (shader.wgsl / denoiser.wgsl of
+DragonFireGames/WebGPURaytracer), behind the pt_* C ABI of libpt.so.
+
+    from webgpuraytracer_amd import SceneBuilder, Renderer, Denoiser
+"""
+from . import _abi as abi  # noqa: F401
+from .renderer import Denoiser, Renderer  # noqa: F401
+from .scene import SceneBuilder, build_sky_cdf  # noqa: F401
+
+__all__ = ["SceneBuilder", "Renderer", "Denoiser", "build_sky_cdf", "abi"]

@@ -1,0 +1,402 @@
+// geometry.h — scene layout in HBM and ray/primitive intersection.
+//
+// Device layout built at upload (pt_api.cpp) from the tracer.js arrays:
+//   * BVH (TLAS and every BLAS): 64-byte child-pair nodes.  One CNode per
+//     inner node of the reference tree holds both children's boxes and
+//     their references, so an inner visit is one 64 B fetch instead of the
+//     reference's three 32 B node reads.  Visit order, culling and the
+//     per-pop counter are the reference's (shader.wgsl:473-542, 694-752).
+//   * Triangles: 48 B (v0, v1-v0, v2-v0) per triangle for the test; the
+//     original 128 B records are read only for the winning triangle.
+//   * Objects, materials, planes, lights, list meshes: the tracer.js records.
+#pragma once
+#include "vec.h"
+#include "pt_layouts.h"
+
+namespace pt {
+
+// Child reference encoding:
+//   bit31 = 0           inner node: bits 0..30 index a CNode
+//   bit31 = 1, bit30 = 0 leaf: bits 25..29 = count-1, bits 0..24 = first item
+//                        (triangle index relative to the mesh's tri_offset, or
+//                        absolute object index for the TLAS)
+//   bit31 = 1, bit30 = 1 leaf too large to pack: bits 0..29 index the original
+//                        32 B node, whose count/next give the range
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kIndirectBit = 0x40000000u;
+
+struct __align__(16) CNode {
+    float4 a;  // left.min.xyz, left.max.x
+    float4 b;  // left.max.yz, right.min.xy
+    float4 c;  // right.min.z, right.max.xyz
+    uint4 d;   // left ref, right ref, unused, unused
+};
+static_assert(sizeof(CNode) == 64, "CNode is 64 B");
+
+struct __align__(16) Root {
+    float mn[3];
+    uint32_t ref;
+    float mx[3];
+    uint32_t pad;
+};
+static_assert(sizeof(Root) == 32, "Root is 32 B");
+
+struct DevScene {
+    const pt_material* mats;
+    const pt_object* objs;
+    const uint32_t* obj_root;      // per object: Root slot of its BLAS (meshes)
+    const CNode* tnodes;
+    const CNode* bnodes;
+    const Root* roots;             // slot 0: TLAS; then one per distinct BLAS root
+    const pt_bvh_node* tlas_raw;
+    const pt_bvh_node* bvh_raw;
+    const float4* tri_geo;         // 3 per triangle
+    const pt_triangle* tris;
+    const pt_plane* planes;
+    const pt_mesh_instance* meshes;
+    const uint32_t* mesh_root;     // per list mesh: Root slot
+    const pt_light* lights;
+    const uint16_t* sky;           // rgba16f
+    const float* cdf;
+    uint32_t n_planes, n_meshes, n_lights, n_objs;
+    uint32_t sky_w, sky_h;
+    uint32_t flags;
+    int32_t bounce_limit;
+    uint32_t stack_cap;
+};
+
+// Compile-time flag policy: F == kDynamicFlags reads the uploaded flags at
+// run time (wave-uniform branches); any other value is a specialisation.
+constexpr uint32_t kDynamicFlags = 0xffffffffu;
+template <uint32_t F>
+struct Flags {
+    uint32_t rt;
+    __device__ __forceinline__ bool has(uint32_t f) const { return F == kDynamicFlags ? (rt & f) != 0 : (F & f) != 0; }
+};
+constexpr uint32_t kAnyObject = PT_HAS_SPHERES | PT_HAS_CUBES | PT_HAS_CYLINDERS | PT_HAS_TORI | PT_HAS_MESHES;
+
+// ---------------------------------------------------------------- stack ----
+// Per-lane traversal stack in LDS, interleaved across the block's lanes
+// (entry e of lane l at e*blockDim + l) so that a wave's accesses fall in
+// distinct banks.
+struct LdsStack {
+    uint32_t* base;
+    uint32_t stride;
+    uint32_t cap;
+    int sp;
+    __device__ __forceinline__ void push(uint32_t v) {
+        if (uint32_t(sp) < cap) { base[uint32_t(sp) * stride] = v; ++sp; }
+    }
+    __device__ __forceinline__ uint32_t pop() { --sp; return base[uint32_t(sp) * stride]; }
+};
+
+struct Counters {
+    uint32_t ext, shadow, paths, pops, inner, objs, tris, shade;
+    __device__ void clear() { ext = shadow = paths = pops = inner = objs = tris = shade = 0; }
+};
+
+// -------------------------------------------------------------- boxes -------
+// intersect_aabb, shader.wgsl:256-265
+__device__ __forceinline__ float slab(v3 o, v3 inv, v3 bmin, v3 bmax) {
+    const v3 t0 = (bmin - o) * inv;
+    const v3 t1 = (bmax - o) * inv;
+    const float tn = hmax(vmin(t0, t1));
+    const float tf = hmin(vmax(t0, t1));
+    if (tn > tf || tf < 0.0f) return -1.0f;
+    return tn < 0.0f ? 0.0f : tn;
+}
+__device__ __forceinline__ void cnode_boxes(const CNode& n, v3& lmin, v3& lmax, v3& rmin, v3& rmax) {
+    lmin = mk(n.a.x, n.a.y, n.a.z);
+    lmax = mk(n.a.w, n.b.x, n.b.y);
+    rmin = mk(n.b.z, n.b.w, n.c.x);
+    rmax = mk(n.c.y, n.c.z, n.c.w);
+}
+__device__ __forceinline__ CNode load_cnode(const CNode* p) {
+    CNode n;
+    const float4* f = reinterpret_cast<const float4*>(p);
+    n.a = f[0];
+    n.b = f[1];
+    n.c = f[2];
+    n.d = reinterpret_cast<const uint4*>(p)[3];
+    return n;
+}
+__device__ __forceinline__ void leaf_range(uint32_t ref, const pt_bvh_node* raw, uint32_t& start, uint32_t& count) {
+    if (ref & kIndirectBit) {
+        const pt_bvh_node& n = raw[ref & 0x3fffffffu];
+        start = n.next;
+        count = n.count;
+    } else {
+        start = ref & 0x01ffffffu;
+        count = ((ref >> 25) & 31u) + 1u;
+    }
+}
+
+// ------------------------------------------------------------ triangle ------
+// intersect_triangle, shader.wgsl:267-288, with edges precomputed on the
+// host (v1-v0 and v2-v0 round identically there).  Returns t > 1e-6 or -1;
+// u, v are the barycentrics of v1 and v2.
+__device__ __forceinline__ float tri_hit(v3 o, v3 d, const float4* g, float& u, float& v) {
+    const float4 A = g[0], B = g[1], C = g[2];
+    const v3 v0 = mk(A.x, A.y, A.z), e1 = mk(B.x, B.y, B.z), e2 = mk(C.x, C.y, C.z);
+    const v3 h = cross(d, e2);
+    const float a = dot(e1, h);
+    if (fabsf(a) < 0.000001f) return -1.0f;
+    const float f = 1.0f / a;
+    const v3 s = o - v0;
+    u = f * dot(s, h);
+    if (u < 0.0f || u > 1.0f) return -1.0f;
+    const v3 q = cross(s, e1);
+    v = f * dot(d, q);
+    if (v < 0.0f || u + v > 1.0f) return -1.0f;
+    const float t = f * dot(e2, q);
+    return t > 0.000001f ? t : -1.0f;
+}
+
+// ------------------------------------------------------------ analytic ------
+// hit_unit_sphere, shader.wgsl:290-300
+__device__ __forceinline__ float unit_sphere(v3 o, v3 d) {
+    const float a = dot(d, d);
+    const float hb = dot(o, d);
+    const float c = dot(o, o) - 1.0f;
+    const float disc = hb * hb - a * c;
+    if (disc < 0.0f) return -1.0f;
+    const float sq = sqrtf(disc);
+    float root = (-hb - sq) / a;
+    if (root < 0.001f) root = (-hb + sq) / a;
+    return root >= 0.001f ? root : -1.0f;
+}
+// hit_unit_cube, shader.wgsl:302-312
+__device__ __forceinline__ float unit_cube(v3 o, v3 d) {
+    const v3 inv = splat(1.0f) / d;
+    const v3 t0 = (splat(-1.0f) - o) * inv;
+    const v3 t1 = (splat(1.0f) - o) * inv;
+    const float tn = hmax(vmin(t0, t1)), tf = hmin(vmax(t0, t1));
+    if (tn < tf && tf > 0.0f) return tn > 0.001f ? tn : tf;
+    return -1.0f;
+}
+// hit_plane, shader.wgsl:314-321
+__device__ __forceinline__ float plane_t(v3 n, float dd, v3 o, v3 d) {
+    const float denom = dot(n, d);
+    if (fabsf(denom) > 1e-6f) {
+        const float t = (dd - dot(n, o)) / denom;
+        if (t > 0.001f) return t;
+    }
+    return -1.0f;
+}
+// hit_cylinder, shader.wgsl:323-379.  which: 0 side, 1 bottom cap, 2 top cap.
+__device__ __forceinline__ float cone_cylinder(v3 o, v3 d, float r0, float r1, int& which, v3& p_side) {
+    const float k = (r1 - r0) / 1.0f;
+    const float oer = r0 + k * o.y;
+    const float A = d.x * d.x + d.z * d.z - k * k * d.y * d.y;
+    const float B = 2.0f * (o.x * d.x + o.z * d.z - oer * k * d.y);
+    const float C = o.x * o.x + o.z * o.z - oer * oer;
+    float tmin = 1e10f;
+    which = -1;
+    const float disc = B * B - 4.0f * A * C;
+    if (disc > 0.0f) {
+        const float sq = sqrtf(disc);
+        const float ts0 = (-B - sq) / (2.0f * A);
+        const float ts1 = (-B + sq) / (2.0f * A);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const float t = i == 0 ? ts0 : ts1;
+            const float y = o.y + t * d.y;
+            if (t > 0.001f && t < tmin && y >= 0.0f && y <= 1.0f) { tmin = t; which = 0; p_side = o + t * d; }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const float py = i == 0 ? 0.0f : 1.0f;
+        const float rr = i == 0 ? r0 : r1;
+        const float t = (py - o.y) / d.y;
+        if (t > 0.001f && t < tmin) {
+            const v3 p = o + t * d;
+            if (p.x * p.x + p.z * p.z <= rr * rr) { tmin = t; which = 1 + i; }
+        }
+    }
+    return which >= 0 ? tmin : -1.0f;
+}
+// hit_torus, shader.wgsl:381-471 (major radius Ra, minor ra, unit direction)
+__device__ __noinline__ float torus_t(v3 o, v3 d, float Ra, float ra) {
+    float po = 1.0f;
+    const float Ra2 = Ra * Ra, ra2 = ra * ra;
+    const float m = dot(o, o), n = dot(o, d);
+    const float br = Ra + ra;
+    const float hb = n * n - m + br * br;
+    if (hb < 0.0f) return -1.0f;
+    const float k = (m - ra2 - Ra2) / 2.0f;
+    const float k3 = n;
+    const float k2 = n * n + Ra2 * d.y * d.y + k;
+    const float k1 = k * n + Ra2 * o.y * d.y;
+    const float k0 = k * k + Ra2 * o.y * o.y - Ra2 * ra2;
+    float K3 = k3, K2 = k2, K1 = k1, K0 = k0;
+    if (fabsf(k3 * (k3 * k3 - k2) + k1) < 0.01f) {
+        po = -1.0f;
+        const float ik0 = 1.0f / k0;
+        K1 = k3 * ik0;
+        K2 = k2 * ik0;
+        K3 = k1 * ik0;
+        K0 = ik0;
+    }
+    const float c2 = (2.0f * K2 - 3.0f * K3 * K3) / 3.0f;
+    const float c1 = 2.0f * (K3 * (K3 * K3 - K2) + K1);
+    const float c0 = (K3 * (K3 * (-3.0f * K3 * K3 + 4.0f * K2) - 8.0f * K1) + 4.0f * K0) / 3.0f;
+    const float Q = c2 * c2 + c0;
+    const float R = 3.0f * c0 * c2 - c2 * c2 * c2 - c1 * c1;
+    const float h = R * R - Q * Q * Q;
+    float z;
+    if (h < 0.0f) {
+        const float sQ = sqrtf(Q);
+        z = 2.0f * sQ * cosf(acosf(clampf(R / (sQ * Q), -1.0f, 1.0f)) / 3.0f);
+    } else {
+        const float sQ = powf(sqrtf(h) + fabsf(R), 1.0f / 3.0f);
+        z = sgn(R) * fabsf(sQ + Q / sQ);
+    }
+    z = c2 - z;
+    float d1 = z - 3.0f * c2;
+    float d2 = z * z - 3.0f * c0;
+    if (fabsf(d1) < 1.0e-4f) {
+        if (d2 < 0.0f) return -1.0f;
+        d2 = sqrtf(d2);
+    } else {
+        if (d1 < 0.0f) return -1.0f;
+        d1 = sqrtf(d1 / 2.0f);
+        d2 = c1 / d1;
+    }
+    float result = 1e20f;
+    float h1 = d1 * d1 - z + d2;
+    if (h1 > 0.0f) {
+        h1 = sqrtf(h1);
+        float t1 = -d1 - h1 - K3; if (po < 0.0f) t1 = 2.0f / t1;
+        float t2 = -d1 + h1 - K3; if (po < 0.0f) t2 = 2.0f / t2;
+        if (t1 > 0.001f) result = t1;
+        if (t2 > 0.001f) result = mn(result, t2);
+    }
+    float h2 = d1 * d1 - z - d2;
+    if (h2 > 0.0f) {
+        h2 = sqrtf(h2);
+        float t3 = d1 - h2 - K3; if (po < 0.0f) t3 = 2.0f / t3;
+        float t4 = d1 + h2 - K3; if (po < 0.0f) t4 = 2.0f / t4;
+        if (t3 > 0.001f) result = mn(result, t3);
+        if (t4 > 0.001f) result = mn(result, t4);
+    }
+    return result > 1e10f ? -1.0f : result;
+}
+
+// ----------------------------------------------------- shadow analytics -----
+// trace_sphere_shadow's interval test, shader.wgsl:874-902
+__device__ __forceinline__ bool sphere_blocks(v3 o, v3 d, float target) {
+    const float a = dot(d, d), hb = dot(o, d), c = dot(o, o) - 1.0f;
+    const float disc = hb * hb - a * c;
+    if (disc < 0.0f) return false;
+    const float sq = sqrtf(disc);
+    const float t1 = (-hb - sq) / a, t2 = (-hb + sq) / a;
+    return mx(0.001f, t1) < mn(target, t2);
+}
+// trace_cube_shadow, shader.wgsl:924-953
+__device__ __forceinline__ bool cube_blocks(v3 o, v3 d, float target) {
+    const v3 inv = splat(1.0f) / d;
+    const v3 tn = (splat(-1.0f) - o) * inv;
+    const v3 tf = (splat(1.0f) - o) * inv;
+    const float t1 = hmax(vmin(tn, tf)), t2 = hmin(vmax(tn, tf));
+    return mx(0.001f, t1) < mn(target, t2);
+}
+// trace_cylinder_shadow, shader.wgsl:961-1016
+__device__ __forceinline__ bool cylinder_blocks(v3 o, v3 d, float top, float target) {
+    const float k = (top - 1.0f) / 1.0f;
+    const float oer = 1.0f + k * o.y;
+    const float A = d.x * d.x + d.z * d.z - k * k * d.y * d.y;
+    const float B = 2.0f * (o.x * d.x + o.z * d.z - oer * k * d.y);
+    const float C = o.x * o.x + o.z * o.z - oer * oer;
+    float h1 = 1e10f, h2 = -1e10f;
+    uint32_t hits = 0;
+    const float disc = B * B - 4.0f * A * C;
+    if (disc >= 0.0f) {
+        const float sq = sqrtf(disc);
+        const float ts0 = (-B - sq) / (2.0f * A), ts1 = (-B + sq) / (2.0f * A);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const float t = i == 0 ? ts0 : ts1;
+            const float y = o.y + t * d.y;
+            if (y >= 0.0f && y <= 1.0f) { h1 = mn(h1, t); h2 = mx(h2, t); ++hits; }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const float cy = i == 0 ? 0.0f : 1.0f;
+        const float rr = i == 0 ? 1.0f : top;
+        const float t = (cy - o.y) / d.y;
+        const v3 p = o + t * d;
+        if (p.x * p.x + p.z * p.z <= rr * rr) { h1 = mn(h1, t); h2 = mx(h2, t); ++hits; }
+    }
+    return hits >= 2u && mx(0.001f, h1) < mn(target, h2);
+}
+// solve_cubic_real / solve_quartic_real, shader.wgsl:1019-1087
+__device__ __forceinline__ float cubic_root(float a, float b, float c) {
+    const float q = (a * a - 3.0f * b) / 9.0f;
+    const float r = (2.0f * a * a * a - 9.0f * a * b + 27.0f * c) / 54.0f;
+    const float r2 = r * r, q3 = q * q * q;
+    if (r2 < q3) {
+        const float th = acosf(clampf(r / sqrtf(q3), -1.0f, 1.0f));
+        return -2.0f * sqrtf(q) * cosf(th / 3.0f) - a / 3.0f;
+    }
+    const float aa = -sgn(r) * powf(fabsf(r) + sqrtf(r2 - q3), 1.0f / 3.0f);
+    float bb = 0.0f;
+    if (aa != 0.0f) bb = q / aa;
+    return (aa + bb) - a / 3.0f;
+}
+__device__ __noinline__ bool torus_blocks(v3 wo, v3 wd, const float* inv, float rminor, float target) {
+    v3 o = xform_point(inv, wo), d = xform_dir(inv, wd);
+    const float rs = len(d);
+    d = d / rs;
+    const float R = 1.0f, r = rminor;
+    const float K = dot(o, o) + R * R - r * r;
+    const float G = dot(o, d);
+    const float a = 4.0f * G;
+    const float b = 2.0f * K + 4.0f * G * G - 4.0f * R * R * (1.0f - d.y * d.y);
+    const float c = 4.0f * K * G - 8.0f * R * R * o.y * d.y;
+    const float e = K * K - 4.0f * R * R * (o.x * o.x + o.z * o.z);
+    float roots[4] = {-1.0f, -1.0f, -1.0f, -1.0f};
+    uint32_t count = 0;
+    const float a2 = a * a;
+    const float y = cubic_root(-b, a * c - 4.0f * e, 4.0f * b * e - a2 * e - c * c);
+    const float R2 = 0.25f * a2 - b + y;
+    if (R2 < 0.0f) return false;
+    const float Rr = sqrtf(R2);
+    float D2, E2;
+    if (Rr < 1e-6f) {
+        D2 = 0.75f * a2 - 2.0f * b + 2.0f * sqrtf(y * y - 4.0f * e);
+        E2 = 0.75f * a2 - 2.0f * b - 2.0f * sqrtf(y * y - 4.0f * e);
+    } else {
+        D2 = 0.75f * a2 - R2 - 2.0f * b + (4.0f * a * b - 8.0f * c - a2 * a) / (4.0f * Rr);
+        E2 = 0.75f * a2 - R2 - 2.0f * b - (4.0f * a * b - 8.0f * c - a2 * a) / (4.0f * Rr);
+    }
+    if (D2 >= 0.0f) {
+        const float D = sqrtf(D2);
+        roots[0] = -0.25f * a + 0.5f * Rr + 0.5f * D;
+        roots[1] = -0.25f * a + 0.5f * Rr - 0.5f * D;
+        count += 2u;
+    }
+    if (E2 >= 0.0f) {
+        const float E = sqrtf(E2);
+        const float x0 = -0.25f * a - 0.5f * Rr + 0.5f * E;
+        const float x1 = -0.25f * a - 0.5f * Rr - 0.5f * E;
+        if (count == 0) { roots[0] = x0; roots[1] = x1; } else { roots[2] = x0; roots[3] = x1; }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = i + 1; j < 4; ++j)
+            if (roots[i] > roots[j]) { const float t = roots[i]; roots[i] = roots[j]; roots[j] = t; }
+#pragma unroll
+    for (int i = 0; i < 4; i += 2) {
+        const float tn = roots[i], tf = roots[i + 1];
+        if (tn < 0.0f || tn > target * rs) continue;
+        const float h1 = mx(0.001f, tn) / rs;
+        const float h2 = mn(target * rs, tf) / rs;
+        if (h1 < h2) return true;
+    }
+    return false;
+}
+
+}  // namespace pt

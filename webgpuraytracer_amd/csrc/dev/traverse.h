@@ -1,0 +1,453 @@
+// traverse.h — closest-hit and any-hit traversal of planes, the TLAS, every
+// BLAS and the list meshes.
+//
+// The reference nests a TLAS loop and a per-mesh BLAS loop, each with its own
+// 64-entry private stack (shader.wgsl:473-542, 694-752, 813-872, 1134-1184).
+// Here both levels run in one flattened loop over one LDS stack: each
+// iteration does one unit of work (pop a TLAS node, intersect one TLAS-leaf
+// object, or pop a BLAS node), so lanes of a wave stay in the same loop body
+// whichever level they are at.  The per-lane visit order, the culling against
+// the running closest t / target distance, the strict `<` ties and the
+// per-pop counter are exactly the reference's.
+#pragma once
+#include "geometry.h"
+
+namespace pt {
+
+enum HitKind : int { kMiss = 0, kPlaneHit = 1, kObjectHit = 2, kListMeshHit = 3 };
+
+// Identity of the closest hit; attributes are computed once for the winner
+// (resolve_hit) instead of for every candidate, which gives the same values.
+struct HitRec {
+    float t;
+    uint32_t count;
+    int kind;
+    int idx;    // plane / object / list-mesh index
+    int prim;   // absolute triangle index, -1 otherwise
+    float bu, bv;
+};
+
+__device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
+
+// Push order of shader.wgsl:533-539 / 743-749.
+__device__ __forceinline__ void push_children(LdsStack& st, float lt, float rt, uint32_t lref, uint32_t rref,
+                                              float limit) {
+    if (lt < rt) {
+        if (rt >= 0.0f && rt < limit) st.push(rref);
+        if (lt >= 0.0f && lt < limit) st.push(lref);
+    } else {
+        if (lt >= 0.0f && lt < limit) st.push(lref);
+        if (rt >= 0.0f && rt < limit) st.push(rref);
+    }
+}
+
+template <bool CN>
+__device__ __forceinline__ void visit_inner(const CNode* nodes, uint32_t ref, v3 o, v3 inv, float limit, LdsStack& st,
+                                            Counters& c) {
+    if (CN) c.inner++;
+    const CNode n = load_cnode(nodes + ref);
+    v3 lmin, lmax, rmin, rmax;
+    cnode_boxes(n, lmin, lmax, rmin, rmax);
+    const float lt = slab(o, inv, lmin, lmax);
+    const float rt = slab(o, inv, rmin, rmax);
+    push_children(st, lt, rt, n.d.x, n.d.y, limit);
+}
+
+// BLAS-only closest-hit traversal (list meshes, shader.wgsl:767-771).
+template <bool CN>
+__device__ void blas_closest(const DevScene& S, uint32_t root_slot, const float* inv_m, uint32_t tri_off, v3 o, v3 d,
+                             int idx, int kind, LdsStack& st, HitRec& h, Counters& c) {
+    const v3 lo = xform_point(inv_m, o), ld = xform_dir(inv_m, d);
+    const v3 linv = splat(1.0f) / ld;
+    const Root& R = S.roots[root_slot];
+    const float bt = slab(lo, linv, ld3(R.mn), ld3(R.mx));
+    if (bt < 0.0f || bt >= h.t) return;
+    st.sp = 0;
+    st.push(R.ref);
+    while (st.sp > 0) {
+        const uint32_t ref = st.pop();
+        h.count++;
+        if (CN) c.pops++;
+        if (ref & kLeafBit) {
+            uint32_t s0, n0;
+            leaf_range(ref, S.bvh_raw, s0, n0);
+            for (uint32_t k = tri_off + s0; k < tri_off + s0 + n0; ++k) {
+                if (CN) c.tris++;
+                float u, v;
+                const float t = tri_hit(lo, ld, S.tri_geo + 3u * k, u, v);
+                if (t >= 0.0f && t < h.t) { h.t = t; h.kind = kind; h.idx = idx; h.prim = int(k); h.bu = u; h.bv = v; }
+            }
+        } else {
+            visit_inner<CN>(S.bnodes, ref, lo, linv, h.t, st, c);
+        }
+    }
+}
+
+// trace_scene (shader.wgsl:754-774): planes, then the TLAS, then list meshes.
+template <uint32_t F, bool CN>
+__device__ void closest_hit(const DevScene& S, Flags<F> fl, v3 o, v3 d, LdsStack& st, HitRec& h, Counters& c) {
+    h.t = 1e10f;
+    h.count = 0;
+    h.kind = kMiss;
+    h.idx = -1;
+    h.prim = -1;
+    h.bu = h.bv = 0.0f;
+    if (fl.has(PT_HAS_PLANES)) {
+        for (uint32_t i = 0; i < S.n_planes; ++i) {
+            const pt_plane& p = S.planes[i];
+            const float t = plane_t(ld3(p.normal), p.d, o, d);
+            if (t > 0.001f && t < h.t) { h.t = t; h.kind = kPlaneHit; h.idx = int(i); h.prim = -1; }
+        }
+    }
+    if (fl.has(kAnyObject)) {
+        const v3 inv = splat(1.0f) / d;
+        const Root& R = S.roots[0];
+        const float bt = slab(o, inv, ld3(R.mn), ld3(R.mx));
+        if (!(bt < 0.0f || bt >= h.t)) {
+            st.sp = 0;
+            st.push(R.ref);
+            bool in_blas = false;
+            int blas_base = 0;
+            uint32_t cur = 0, end = 0, tri_off = 0;
+            int obj = -1;
+            v3 lo = o, ld = d, linv = inv;
+            for (;;) {
+                if (in_blas) {
+                    if (st.sp == blas_base) { in_blas = false; continue; }
+                    const uint32_t ref = st.pop();
+                    h.count++;
+                    if (CN) c.pops++;
+                    if (ref & kLeafBit) {
+                        uint32_t s0, n0;
+                        leaf_range(ref, S.bvh_raw, s0, n0);
+                        for (uint32_t k = tri_off + s0; k < tri_off + s0 + n0; ++k) {
+                            if (CN) c.tris++;
+                            float u, v;
+                            const float t = tri_hit(lo, ld, S.tri_geo + 3u * k, u, v);
+                            if (t >= 0.0f && t < h.t) {
+                                h.t = t; h.kind = kObjectHit; h.idx = obj; h.prim = int(k); h.bu = u; h.bv = v;
+                            }
+                        }
+                    } else {
+                        visit_inner<CN>(S.bnodes, ref, lo, linv, h.t, st, c);
+                    }
+                    continue;
+                }
+                if (cur < end) {
+                    const uint32_t i = cur++;
+                    if (CN) c.objs++;
+                    const pt_object& ob = S.objs[i];
+                    const int type = ob.object_type;
+                    if (fl.has(PT_HAS_MESHES) && type == PT_OBJ_MESH) {
+                        lo = xform_point(ob.inv_matrix, o);
+                        ld = xform_dir(ob.inv_matrix, d);
+                        linv = splat(1.0f) / ld;
+                        const Root& br = S.roots[S.obj_root[i]];
+                        const float b2 = slab(lo, linv, ld3(br.mn), ld3(br.mx));
+                        if (!(b2 < 0.0f || b2 >= h.t)) {
+                            tri_off = f2u(ob.param1);
+                            obj = int(i);
+                            blas_base = st.sp;
+                            st.push(br.ref);
+                            in_blas = true;
+                        }
+                    } else if (fl.has(PT_HAS_SPHERES) && type == PT_OBJ_SPHERE) {
+                        const float t = unit_sphere(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d));
+                        if (t > 0.001f && t < h.t) { h.t = t; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
+                    } else if (fl.has(PT_HAS_CUBES) && type == PT_OBJ_CUBE) {
+                        const float t = unit_cube(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d));
+                        if (t > 0.001f && t < h.t) { h.t = t; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
+                    } else if (fl.has(PT_HAS_CYLINDERS) && type == PT_OBJ_CYLINDER) {
+                        int which;
+                        v3 ps;
+                        const float t = cone_cylinder(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), 1.0f,
+                                                      ob.param0, which, ps);
+                        if (which >= 0 && t < h.t) { h.t = t; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
+                    } else if (fl.has(PT_HAS_TORI) && type == PT_OBJ_TORUS) {
+                        v3 tl = xform_dir(ob.inv_matrix, d);
+                        const float rs = len(tl);
+                        tl = tl / rs;
+                        const float t = torus_t(xform_point(ob.inv_matrix, o), tl, 1.0f, ob.param0 * 1.0f);
+                        if (t > 0.0f) {
+                            const float wt = t / rs;
+                            if (wt < h.t) { h.t = wt; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
+                        }
+                    }
+                    continue;
+                }
+                if (st.sp == 0) break;
+                const uint32_t ref = st.pop();
+                h.count++;
+                if (CN) c.pops++;
+                if (ref & kLeafBit) {
+                    uint32_t s0, n0;
+                    leaf_range(ref, S.tlas_raw, s0, n0);
+                    cur = s0;
+                    end = s0 + n0;
+                } else {
+                    visit_inner<CN>(S.tnodes, ref, o, inv, h.t, st, c);
+                }
+            }
+        }
+    }
+    if (fl.has(PT_HAS_LIST_MESHES)) {
+        for (uint32_t m = 0; m < S.n_meshes; ++m) {
+            const pt_mesh_instance& mi = S.meshes[m];
+            blas_closest<CN>(S, S.mesh_root[m], mi.inv_matrix, mi.tri_offset, o, d, int(m), kListMeshHit, st, h, c);
+        }
+    }
+}
+
+template <bool CN>
+__device__ bool blas_any(const DevScene& S, uint32_t root_slot, const float* inv_m, uint32_t tri_off, v3 o, v3 d,
+                         float tdist, LdsStack& st, Counters& c) {
+    const v3 lo = xform_point(inv_m, o), ld = xform_dir(inv_m, d);
+    const v3 linv = splat(1.0f) / ld;
+    const Root& R = S.roots[root_slot];
+    const float bt = slab(lo, linv, ld3(R.mn), ld3(R.mx));
+    if (bt < 0.0f || bt >= tdist) return false;
+    st.sp = 0;
+    st.push(R.ref);
+    while (st.sp > 0) {
+        const uint32_t ref = st.pop();
+        if (CN) c.pops++;
+        if (ref & kLeafBit) {
+            uint32_t s0, n0;
+            leaf_range(ref, S.bvh_raw, s0, n0);
+            for (uint32_t k = tri_off + s0; k < tri_off + s0 + n0; ++k) {
+                if (CN) c.tris++;
+                float u, v;
+                const float t = tri_hit(lo, ld, S.tri_geo + 3u * k, u, v);
+                if (t >= 0.0f && t < tdist) return true;
+            }
+        } else {
+            visit_inner<CN>(S.bnodes, ref, lo, linv, tdist, st, c);
+        }
+    }
+    return false;
+}
+
+// trace_scene_shadow (shader.wgsl:1206-1229).  Without albedo textures
+// every occluder blocks outright (evaluate_shadow_attenuation returns true,
+// shader.wgsl:777-792), so the transmittance is 0 or 1.
+template <uint32_t F, bool CN>
+__device__ bool occluded(const DevScene& S, Flags<F> fl, v3 o, v3 d, int target, float tdist, LdsStack& st,
+                         Counters& c) {
+    if (fl.has(PT_HAS_PLANES)) {
+        for (uint32_t i = 0; i < S.n_planes; ++i) {
+            const pt_plane& p = S.planes[i];
+            const v3 n = ld3(p.normal);
+            const float denom = dot(n, d);
+            if (fabsf(denom) > 1e-6f) {
+                const float t = (p.d - dot(n, o)) / denom;
+                if (t > 0.001f && t < tdist) return true;
+            }
+        }
+    }
+    if (fl.has(kAnyObject)) {
+        const v3 inv = splat(1.0f) / d;
+        st.sp = 0;
+        st.push(S.roots[0].ref);
+        bool in_blas = false;
+        int blas_base = 0;
+        uint32_t cur = 0, end = 0, tri_off = 0;
+        v3 lo = o, ld = d, linv = inv;
+        for (;;) {
+            if (in_blas) {
+                if (st.sp == blas_base) { in_blas = false; continue; }
+                const uint32_t ref = st.pop();
+                if (CN) c.pops++;
+                if (ref & kLeafBit) {
+                    uint32_t s0, n0;
+                    leaf_range(ref, S.bvh_raw, s0, n0);
+                    for (uint32_t k = tri_off + s0; k < tri_off + s0 + n0; ++k) {
+                        if (CN) c.tris++;
+                        float u, v;
+                        const float t = tri_hit(lo, ld, S.tri_geo + 3u * k, u, v);
+                        if (t >= 0.0f && t < tdist) return true;
+                    }
+                } else {
+                    visit_inner<CN>(S.bnodes, ref, lo, linv, tdist, st, c);
+                }
+                continue;
+            }
+            if (cur < end) {
+                const uint32_t i = cur++;
+                if (int(i) == target) continue;
+                if (CN) c.objs++;
+                const pt_object& ob = S.objs[i];
+                const int type = ob.object_type;
+                if (fl.has(PT_HAS_MESHES) && type == PT_OBJ_MESH) {
+                    lo = xform_point(ob.inv_matrix, o);
+                    ld = xform_dir(ob.inv_matrix, d);
+                    linv = splat(1.0f) / ld;
+                    const Root& br = S.roots[S.obj_root[i]];
+                    const float b2 = slab(lo, linv, ld3(br.mn), ld3(br.mx));
+                    if (!(b2 < 0.0f || b2 >= tdist)) {
+                        tri_off = f2u(ob.param1);
+                        blas_base = st.sp;
+                        st.push(br.ref);
+                        in_blas = true;
+                    }
+                } else if (fl.has(PT_HAS_SPHERES) && type == PT_OBJ_SPHERE) {
+                    if (sphere_blocks(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), tdist)) return true;
+                } else if (fl.has(PT_HAS_CUBES) && type == PT_OBJ_CUBE) {
+                    if (cube_blocks(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), tdist)) return true;
+                } else if (fl.has(PT_HAS_CYLINDERS) && type == PT_OBJ_CYLINDER) {
+                    if (cylinder_blocks(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), ob.param0, tdist))
+                        return true;
+                } else if (fl.has(PT_HAS_TORI) && type == PT_OBJ_TORUS) {
+                    if (torus_blocks(o, d, ob.inv_matrix, ob.param0, tdist)) return true;
+                }
+                continue;
+            }
+            if (st.sp == 0) break;
+            const uint32_t ref = st.pop();
+            if (CN) c.pops++;
+            if (ref & kLeafBit) {
+                uint32_t s0, n0;
+                leaf_range(ref, S.tlas_raw, s0, n0);
+                cur = s0;
+                end = s0 + n0;
+            } else {
+                visit_inner<CN>(S.tnodes, ref, o, inv, tdist, st, c);
+            }
+        }
+    }
+    if (fl.has(PT_HAS_LIST_MESHES)) {
+        for (uint32_t m = 0; m < S.n_meshes; ++m) {
+            const pt_mesh_instance& mi = S.meshes[m];
+            if (blas_any<CN>(S, S.mesh_root[m], mi.inv_matrix, mi.tri_offset, o, d, tdist, st, c)) return true;
+        }
+    }
+    return false;
+}
+
+// Attributes of the winning hit (the values the reference leaves in its
+// SurfaceHit after the last accepted candidate).
+struct SurfHit {
+    float t;
+    int m_idx;
+    int o_idx;
+    v3 n;
+    v2 uv;
+};
+
+template <bool WANT_UV>
+__device__ SurfHit resolve_hit(const DevScene& S, v3 o, v3 d, const HitRec& h) {
+    SurfHit r;
+    r.t = h.t;
+    r.m_idx = -1;
+    r.o_idx = -1;
+    r.n = splat(0.0f);
+    r.uv = v2{0.0f, 0.0f};
+    if (h.kind == kMiss) return r;
+    if (h.kind == kPlaneHit) {  // trace_plane, shader.wgsl:622-636
+        const pt_plane& p = S.planes[h.idx];
+        r.m_idx = p.material_idx;
+        r.n = ld3(p.normal);
+        if (WANT_UV) {
+            const v3 hp = o + d * h.t;
+            v3 tg = mk(1.0f, 0.0f, 0.0f);
+            if (fabsf(r.n.x) > 0.9f) tg = mk(0.0f, 0.0f, 1.0f);
+            const v3 T = normalize(cross(r.n, tg));
+            const v3 B = normalize(cross(r.n, T));
+            r.uv = v2{dot(hp, T), dot(hp, B)};
+        }
+        return r;
+    }
+    const float* inv;
+    if (h.kind == kListMeshHit) {
+        inv = S.meshes[h.idx].inv_matrix;
+        r.m_idx = S.meshes[h.idx].material_idx;
+        r.o_idx = -1;
+    } else {
+        const pt_object& ob = S.objs[h.idx];
+        inv = ob.inv_matrix;
+        r.m_idx = ob.material_idx;
+        r.o_idx = h.idx;
+    }
+    if (h.prim >= 0) {  // trace_mesh, shader.wgsl:509-523
+        const pt_triangle& tr = S.tris[h.prim];
+        const float u = h.bu, v = h.bv;
+        const float b0 = 1.0f - u - v;
+        const v3 ln = normalize(ld3(tr.n0) * b0 + ld3(tr.n1) * u + ld3(tr.n2) * v);
+        r.n = normalize(xform_tdir(inv, ln));
+        if (WANT_UV) r.uv = v2{tr.uv0[0] * b0 + tr.uv1[0] * u + tr.uv2[0] * v, tr.uv0[1] * b0 + tr.uv1[1] * u + tr.uv2[1] * v};
+        return r;
+    }
+    const pt_object& ob = S.objs[h.idx];
+    const v3 lo = xform_point(inv, o), ld = xform_dir(inv, d);
+    const float t = h.t;
+    switch (ob.object_type) {
+        case PT_OBJ_SPHERE: {  // shader.wgsl:552-573
+            const v3 ln = lo + ld * t;
+            if (WANT_UV) {
+                const float phi = atan2f(ln.z, ln.x);
+                const float th = asinf(clampf(ln.y, -1.0f, 1.0f));
+                r.uv = v2{0.5f + phi / kTWO_PI, 0.5f + th / kPI};
+            }
+            r.n = normalize(xform_normal(inv, ln));
+            break;
+        }
+        case PT_OBJ_CUBE: {  // shader.wgsl:585-618
+            const v3 lh = lo + ld * t;
+            const v3 a = mk(fabsf(lh.x), fabsf(lh.y), fabsf(lh.z));
+            const float m = hmax(a);
+            v3 ln;
+            if (m == a.x) {
+                const float s = sgn(lh.x);
+                ln = mk(s, 0.0f, 0.0f);
+                r.uv = v2{(-s * lh.z) * 0.5f + 0.5f, lh.y * 0.5f + 0.5f};
+            } else if (m == a.y) {
+                const float s = sgn(lh.y);
+                ln = mk(0.0f, s, 0.0f);
+                r.uv = v2{lh.x * 0.5f + 0.5f, (s * lh.z) * 0.5f + 0.5f};
+            } else {
+                const float s = sgn(lh.z);
+                ln = mk(0.0f, 0.0f, s);
+                r.uv = v2{(s * lh.x) * 0.5f + 0.5f, lh.y * 0.5f + 0.5f};
+            }
+            r.n = normalize(xform_normal(inv, ln));
+            break;
+        }
+        case PT_OBJ_CYLINDER: {  // hit_cylinder normals/uvs, shader.wgsl:345-368
+            int which;
+            v3 ps;
+            cone_cylinder(lo, ld, 1.0f, ob.param0, which, ps);
+            const float r0 = 1.0f, r1 = ob.param0, k = (r1 - r0) / 1.0f;
+            v3 ln;
+            const v3 p = lo + t * ld;
+            if (which == 0) {
+                const float y = lo.y + t * ld.y;
+                const float slant = -k * (r0 + k * y);
+                ln = normalize(mk(p.x, slant, p.z));
+                if (WANT_UV) r.uv = v2{atan2f(p.z, p.x) / kTWO_PI + 0.5f, y};
+            } else {
+                ln = mk(0.0f, float(which - 1) * 2.0f - 1.0f, 0.0f);
+                const float m2 = mx(r0, r1) * 2.0f;
+                r.uv = v2{p.x / m2 + 0.5f, p.z / m2 + 0.5f};
+            }
+            r.n = normalize(xform_normal(inv, ln));
+            break;
+        }
+        case PT_OBJ_TORUS: {  // shader.wgsl:666-690 (normal uses mat3, not its transpose)
+            const float rs = len(ld);
+            const v3 dn = ld / rs;
+            const float tl = torus_t(lo, dn, 1.0f, ob.param0 * 1.0f);
+            const v3 p = lo + dn * tl;
+            const v3 ring = normalize(mk(p.x, 0.0f, p.z));
+            const v3 ln = normalize(p - ring);
+            r.n = normalize(xform_mat3(inv, ln));
+            if (WANT_UV) {
+                r.uv = v2{(atan2f(p.z, p.x) / kTWO_PI) + 0.5f,
+                          (atan2f(p.y, sqrtf(p.x * p.x + p.z * p.z) - 1.0f) / kTWO_PI) + 0.5f};
+            }
+            break;
+        }
+        default: break;
+    }
+    return r;
+}
+
+}  // namespace pt

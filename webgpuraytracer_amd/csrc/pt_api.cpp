@@ -1,0 +1,783 @@
+// pt_api.cpp — the C ABI of libpt (declared in include/pt_abi.h).
+//
+// Host-side responsibilities: validate the tracer.js arrays, build the device
+// layout (child-pair BVH nodes, root table, 48 B triangle records), own every
+// device buffer, launch the kernels on the context's stream and report errors
+// as status codes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pt_abi.h"
+#include "pt_internal.h"
+
+using namespace pt;
+
+namespace {
+
+constexpr uint32_t kMaxStack = 128;  // combined TLAS+BLAS entries per lane
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+}  // namespace
+
+struct pt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    // options
+    int backend = PT_BACKEND_MEGAKERNEL;
+    bool count_stats = true;
+    // scene
+    bool has_scene = false;
+    uint32_t flags = 0;
+    int bounce_limit = 8;
+    DevBuf mats, objs, obj_root, tnodes, bnodes, roots, tlas_raw, bvh_raw, tri_geo, tris, planes, meshes, mesh_root,
+        lights, sky, cdf, cdf_rows;
+    uint64_t n_mats = 0, n_objs = 0, n_tlas = 0, n_planes = 0, n_meshes = 0, n_lights = 0, n_bvh = 0, n_tris = 0;
+    uint32_t sky_w = 1, sky_h = 1;
+    float sky_total_lum = 0.f;
+    double sky_total_weight = 0.0;
+    std::map<uint32_t, uint32_t> blas_slot;   // node_offset -> root slot
+    std::vector<Root> host_roots;
+    uint32_t tlas_depth = 0, blas_depth = 0, stack_cap = 2;
+    DevScene dev{};
+    // frame
+    uint32_t width = 0, height = 0;
+    bool with_gbuffer = false;
+    DevBuf accum, out, gbuf, dn_a, dn_b, dn_fin, dn_guide, stats, seeds, rays_in, rays_aux, rays_aux2, rays_out;
+
+    int fail(int code, const char* fmt, ...) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+    int hip_fail(hipError_t e, const char* what) {
+        return fail(e == hipErrorOutOfMemory ? PT_ERR_OOM : PT_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    }
+    int alloc(DevBuf& b, size_t bytes, const char* what) {
+        if (b.p && b.bytes >= bytes && bytes > 0) return PT_OK;
+        b.release();
+        const size_t n = std::max<size_t>(bytes, 16);
+        hipError_t e = hipMalloc(&b.p, n);
+        if (e != hipSuccess) { b.p = nullptr; return hip_fail(e, what); }
+        b.bytes = n;
+        return PT_OK;
+    }
+    int upload(DevBuf& b, const void* src, size_t bytes, const char* what) {
+        int rc = alloc(b, bytes, what);
+        if (rc) return rc;
+        if (bytes == 0) return PT_OK;
+        hipError_t e = hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) return hip_fail(e, what);
+        return PT_OK;
+    }
+    int sync(const char* what) {
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return hip_fail(e, what);
+        return PT_OK;
+    }
+    void release_all() {
+        DevBuf* all[] = {&mats, &objs, &obj_root, &tnodes, &bnodes, &roots, &tlas_raw, &bvh_raw, &tri_geo, &tris,
+                         &planes, &meshes, &mesh_root, &lights, &sky, &cdf, &cdf_rows, &accum, &out, &gbuf, &dn_a,
+                         &dn_b, &dn_fin, &dn_guide, &stats, &seeds, &rays_in, &rays_aux, &rays_aux2, &rays_out};
+        for (DevBuf* b : all) b->release();
+    }
+};
+
+namespace {
+
+// Converts one reference BVH (32 B nodes, left = i+1) rooted at `root` into
+// child-pair nodes appended to `out`.  For a BLAS the right child is
+// node_offset + next (shader.wgsl:530); for the TLAS it is next
+// (shader.wgsl:740).  Returns PT_OK and fills *r.
+struct TreeConv {
+    const pt_bvh_node* nodes;
+    uint64_t n_nodes;
+    bool blas;
+    uint32_t node_offset;
+    std::vector<CNode>* out;
+    uint32_t max_depth = 0;
+    std::string err;
+
+    bool leaf_ref(uint32_t i, uint32_t* ref) {
+        const pt_bvh_node& n = nodes[i];
+        if (n.count <= 32 && n.next < (1u << 25)) {
+            *ref = kLeafBit | ((n.count - 1u) << 25) | n.next;
+        } else {
+            if (i >= (1u << 30)) { err = "leaf index too large"; return false; }
+            *ref = kLeafBit | kIndirectBit | i;
+        }
+        return true;
+    }
+    // Returns the reference for node i (recursively converting its subtree).
+    bool convert(uint32_t i, uint32_t depth, uint32_t* ref) {
+        if (i >= n_nodes) { err = "BVH child index out of range"; return false; }
+        if (depth > 256) { err = "BVH deeper than 256 levels (cycle?)"; return false; }
+        max_depth = std::max(max_depth, depth);
+        const pt_bvh_node& n = nodes[i];
+        if (n.count > 0) return leaf_ref(i, ref);
+        const uint64_t left = uint64_t(i) + 1u;
+        const uint64_t right = blas ? uint64_t(node_offset) + n.next : uint64_t(n.next);
+        if (left >= n_nodes || right >= n_nodes) { err = "BVH child index out of range"; return false; }
+        const uint32_t k = uint32_t(out->size());
+        if (k >= 0x80000000u) { err = "too many BVH nodes"; return false; }
+        out->push_back(CNode{});
+        uint32_t lref, rref;
+        if (!convert(uint32_t(left), depth + 1, &lref)) return false;
+        if (!convert(uint32_t(right), depth + 1, &rref)) return false;
+        const pt_bvh_node& L = nodes[left];
+        const pt_bvh_node& R = nodes[right];
+        CNode& c = (*out)[k];
+        c.a = make_float4(L.aabb_min[0], L.aabb_min[1], L.aabb_min[2], L.aabb_max[0]);
+        c.b = make_float4(L.aabb_max[1], L.aabb_max[2], R.aabb_min[0], R.aabb_min[1]);
+        c.c = make_float4(R.aabb_min[2], R.aabb_max[0], R.aabb_max[1], R.aabb_max[2]);
+        c.d = make_uint4(lref, rref, 0u, 0u);
+        *ref = k;
+        return true;
+    }
+    bool root(uint32_t i, Root* r) {
+        if (i >= n_nodes) { err = "BVH root index out of range"; return false; }
+        uint32_t ref;
+        if (!convert(i, 0, &ref)) return false;
+        const pt_bvh_node& n = nodes[i];
+        for (int k = 0; k < 3; ++k) { r->mn[k] = n.aabb_min[k]; r->mx[k] = n.aabb_max[k]; }
+        r->ref = ref;
+        r->pad = 0;
+        return true;
+    }
+};
+
+bool check_array(const pt_array& a, uint32_t stride, bool allow_empty) {
+    if (a.stride != stride) return false;
+    if (a.count == 0) return allow_empty;
+    return a.ptr != nullptr;
+}
+
+uint32_t f2u_host(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+
+int validate_desc(pt_ctx* c, const pt_scene_desc* d) {
+    if (!d) return c->fail(PT_ERR_ARG, "scene descriptor is NULL");
+    struct { const pt_array* a; uint32_t stride; const char* name; } arrs[] = {
+        {&d->materials, sizeof(pt_material), "materials"}, {&d->objects, sizeof(pt_object), "objects"},
+        {&d->tlas, sizeof(pt_bvh_node), "tlas"},          {&d->meshes, sizeof(pt_mesh_instance), "meshes"},
+        {&d->bvh, sizeof(pt_bvh_node), "bvh"},            {&d->triangles, sizeof(pt_triangle), "triangles"},
+        {&d->planes, sizeof(pt_plane), "planes"},         {&d->lights, sizeof(pt_light), "lights"}};
+    for (auto& x : arrs)
+        if (!check_array(*x.a, x.stride, false))
+            return c->fail(PT_ERR_LAYOUT, "%s: expected a non-empty array with stride %u (got count %llu stride %u)",
+                           x.name, x.stride, (unsigned long long)x.a->count, x.a->stride);
+    if (d->flags & PT_HAS_HEIGHTMAPS) return c->fail(PT_ERR_UNSUPPORTED, "heightmaps (POM) are not supported");
+    if (d->texture_layers != 0) return c->fail(PT_ERR_UNSUPPORTED, "texture arrays are not supported");
+    const pt_material* m = static_cast<const pt_material*>(d->materials.ptr);
+    for (uint64_t i = 0; i < d->materials.count; ++i)
+        if (m[i].emissive_idx >= 0 || m[i].albedo_idx >= 0 || m[i].normal_idx >= 0 || m[i].height_idx >= 0 ||
+            m[i].roughness_idx >= 0 || m[i].metallic_idx >= 0)
+            return c->fail(PT_ERR_UNSUPPORTED, "material %llu references a texture", (unsigned long long)i);
+    if (d->bounce_limit < 0) return c->fail(PT_ERR_ARG, "bounce_limit < 0");
+    if (!d->sky_rgba16f || d->sky_width == 0 || d->sky_height == 0) return c->fail(PT_ERR_ARG, "sky texels missing");
+    if (d->sky_cdf) {
+        const uint64_t need = uint64_t(d->sky_height) + 1 + uint64_t(d->sky_height) * (d->sky_width + 1);
+        if (d->sky_cdf_count != need)
+            return c->fail(PT_ERR_LAYOUT, "sky_cdf has %llu floats, expected %llu", (unsigned long long)d->sky_cdf_count,
+                           (unsigned long long)need);
+    }
+    // index sanity for everything the kernels dereference
+    const uint64_t nm = d->materials.count, no = d->objects.count;
+    const pt_object* ob = static_cast<const pt_object*>(d->objects.ptr);
+    const uint32_t any_obj = PT_HAS_SPHERES | PT_HAS_CUBES | PT_HAS_CYLINDERS | PT_HAS_TORI | PT_HAS_MESHES;
+    if (d->flags & any_obj) {
+        for (uint64_t i = 0; i < no; ++i) {
+            if (ob[i].material_idx < 0 || uint64_t(ob[i].material_idx) >= nm)
+                return c->fail(PT_ERR_LAYOUT, "object %llu: material index out of range", (unsigned long long)i);
+            if (ob[i].light_idx >= 0 && uint64_t(ob[i].light_idx) >= d->lights.count)
+                return c->fail(PT_ERR_LAYOUT, "object %llu: light index out of range", (unsigned long long)i);
+        }
+    }
+    if (d->flags & PT_HAS_PLANES) {
+        const pt_plane* p = static_cast<const pt_plane*>(d->planes.ptr);
+        for (uint64_t i = 0; i < d->planes.count; ++i)
+            if (p[i].material_idx < 0 || uint64_t(p[i].material_idx) >= nm)
+                return c->fail(PT_ERR_LAYOUT, "plane %llu: material index out of range", (unsigned long long)i);
+    }
+    if (d->flags & PT_HAS_LIST_MESHES) {
+        const pt_mesh_instance* mi = static_cast<const pt_mesh_instance*>(d->meshes.ptr);
+        for (uint64_t i = 0; i < d->meshes.count; ++i)
+            if (mi[i].material_idx < 0 || uint64_t(mi[i].material_idx) >= nm)
+                return c->fail(PT_ERR_LAYOUT, "mesh %llu: material index out of range", (unsigned long long)i);
+    }
+    if (d->flags & PT_HAS_LIGHTS) {
+        const pt_light* L = static_cast<const pt_light*>(d->lights.ptr);
+        for (uint64_t i = 0; i < d->lights.count; ++i)
+            if (L[i].obj_idx >= 0 && uint64_t(L[i].obj_idx) >= no)
+                return c->fail(PT_ERR_LAYOUT, "light %llu: object index out of range", (unsigned long long)i);
+    }
+    return PT_OK;
+}
+
+// Checks that every triangle range a BLAS leaf can reach stays inside the
+// triangle array, for a given tri_offset.
+bool blas_ranges_ok(const pt_bvh_node* nodes, uint64_t n_nodes, uint32_t node_offset, uint32_t tri_offset,
+                    uint64_t n_tris) {
+    std::vector<uint32_t> st{node_offset};
+    uint64_t visited = 0;
+    while (!st.empty()) {
+        const uint32_t i = st.back();
+        st.pop_back();
+        if (i >= n_nodes || ++visited > n_nodes) return false;
+        const pt_bvh_node& n = nodes[i];
+        if (n.count > 0) {
+            if (uint64_t(tri_offset) + n.next + n.count > n_tris) return false;
+        } else {
+            st.push_back(i + 1);
+            st.push_back(node_offset + n.next);
+        }
+    }
+    return true;
+}
+
+// Builds TLAS nodes + root slot 0, and BLAS roots for every mesh object and
+// list mesh, into host vectors.
+int build_bvh_layout(pt_ctx* c, const pt_scene_desc* d, std::vector<CNode>& tn, std::vector<CNode>& bn,
+                     std::vector<Root>& roots, std::vector<uint32_t>& obj_root, std::vector<uint32_t>& mesh_root,
+                     std::map<uint32_t, uint32_t>& slots, uint32_t* tlas_depth, uint32_t* blas_depth) {
+    const uint32_t any_obj = PT_HAS_SPHERES | PT_HAS_CUBES | PT_HAS_CYLINDERS | PT_HAS_TORI | PT_HAS_MESHES;
+    const pt_bvh_node* tl = static_cast<const pt_bvh_node*>(d->tlas.ptr);
+    const pt_bvh_node* bl = static_cast<const pt_bvh_node*>(d->bvh.ptr);
+    const pt_object* ob = static_cast<const pt_object*>(d->objects.ptr);
+    roots.assign(1, Root{});
+    *tlas_depth = 0;
+    *blas_depth = 0;
+    if (d->flags & any_obj) {
+        TreeConv t{tl, d->tlas.count, false, 0, &tn};
+        if (!t.root(0, &roots[0])) return c->fail(PT_ERR_LAYOUT, "tlas: %s", t.err.c_str());
+        *tlas_depth = t.max_depth;
+        // every TLAS leaf range must index real objects
+        for (uint64_t i = 0; i < d->tlas.count; ++i)
+            if (tl[i].count > 0 && uint64_t(tl[i].next) + tl[i].count > d->objects.count)
+                return c->fail(PT_ERR_LAYOUT, "tlas node %llu: object range out of bounds", (unsigned long long)i);
+    }
+    obj_root.assign(d->objects.count, 0u);
+    auto slot_for = [&](uint32_t node_offset, uint32_t tri_offset, uint32_t* slot) -> int {
+        if (!blas_ranges_ok(bl, d->bvh.count, node_offset, tri_offset, d->triangles.count))
+            return c->fail(PT_ERR_LAYOUT, "BLAS at node %u / triangle offset %u indexes outside the arrays", node_offset,
+                           tri_offset);
+        auto it = slots.find(node_offset);
+        if (it != slots.end()) { *slot = it->second; return PT_OK; }
+        TreeConv b{bl, d->bvh.count, true, node_offset, &bn};
+        Root r;
+        if (!b.root(node_offset, &r)) return c->fail(PT_ERR_LAYOUT, "bvh: %s", b.err.c_str());
+        *blas_depth = std::max(*blas_depth, b.max_depth);
+        *slot = uint32_t(roots.size());
+        roots.push_back(r);
+        slots[node_offset] = *slot;
+        return PT_OK;
+    };
+    if (d->flags & PT_HAS_MESHES) {
+        for (uint64_t i = 0; i < d->objects.count; ++i) {
+            if (ob[i].object_type != PT_OBJ_MESH) continue;
+            uint32_t s;
+            int rc = slot_for(f2u_host(ob[i].param0), f2u_host(ob[i].param1), &s);
+            if (rc) return rc;
+            obj_root[i] = s;
+        }
+    }
+    mesh_root.assign(d->meshes.count, 0u);
+    if (d->flags & PT_HAS_LIST_MESHES) {
+        const pt_mesh_instance* mi = static_cast<const pt_mesh_instance*>(d->meshes.ptr);
+        for (uint64_t i = 0; i < d->meshes.count; ++i) {
+            uint32_t s;
+            int rc = slot_for(mi[i].node_offset, mi[i].tri_offset, &s);
+            if (rc) return rc;
+            mesh_root[i] = s;
+        }
+    }
+    return PT_OK;
+}
+
+void refresh_dev(pt_ctx* c) {
+    DevScene& S = c->dev;
+    S.mats = static_cast<const pt_material*>(c->mats.p);
+    S.objs = static_cast<const pt_object*>(c->objs.p);
+    S.obj_root = static_cast<const uint32_t*>(c->obj_root.p);
+    S.tnodes = static_cast<const CNode*>(c->tnodes.p);
+    S.bnodes = static_cast<const CNode*>(c->bnodes.p);
+    S.roots = static_cast<const Root*>(c->roots.p);
+    S.tlas_raw = static_cast<const pt_bvh_node*>(c->tlas_raw.p);
+    S.bvh_raw = static_cast<const pt_bvh_node*>(c->bvh_raw.p);
+    S.tri_geo = static_cast<const float4*>(c->tri_geo.p);
+    S.tris = static_cast<const pt_triangle*>(c->tris.p);
+    S.planes = static_cast<const pt_plane*>(c->planes.p);
+    S.meshes = static_cast<const pt_mesh_instance*>(c->meshes.p);
+    S.mesh_root = static_cast<const uint32_t*>(c->mesh_root.p);
+    S.lights = static_cast<const pt_light*>(c->lights.p);
+    S.sky = static_cast<const uint16_t*>(c->sky.p);
+    S.cdf = static_cast<const float*>(c->cdf.p);
+    S.n_planes = uint32_t(c->n_planes);
+    S.n_meshes = uint32_t(c->n_meshes);
+    S.n_lights = uint32_t(c->n_lights);
+    S.n_objs = uint32_t(c->n_objs);
+    S.sky_w = c->sky_w;
+    S.sky_h = c->sky_h;
+    S.flags = c->flags;
+    S.bounce_limit = c->bounce_limit;
+    S.stack_cap = c->stack_cap;
+}
+
+int upload_dynamic(pt_ctx* c, const pt_scene_desc* d, bool first) {
+    std::vector<CNode> tn, bn;
+    std::vector<Root> roots;
+    std::vector<uint32_t> obj_root, mesh_root;
+    std::map<uint32_t, uint32_t> slots;
+    uint32_t td = 0, bd = 0;
+    if (!first) slots = {};
+    int rc = build_bvh_layout(c, d, tn, bn, roots, obj_root, mesh_root, slots, &td, &bd);
+    if (rc) return rc;
+    if (!first) {
+        // the fast path may only reuse BLAS layouts built at upload
+        for (auto& kv : slots)
+            if (c->blas_slot.find(kv.first) == c->blas_slot.end())
+                return c->fail(PT_ERR_CAPACITY, "objects reference a BLAS root (%u) absent at upload", kv.first);
+        // remap slots to the uploaded numbering, keep the uploaded BLAS nodes
+        std::vector<uint32_t> remap(roots.size(), 0);
+        for (auto& kv : slots) remap[kv.second] = c->blas_slot[kv.first];
+        for (auto& s : obj_root) s = remap[s];
+        for (auto& s : mesh_root) s = remap[s];
+        std::vector<Root> merged = c->host_roots;
+        merged[0] = roots[0];
+        roots = merged;
+        if (td + c->blas_depth + 2 > c->stack_cap && td + c->blas_depth + 2 <= kMaxStack)
+            c->stack_cap = td + c->blas_depth + 2;
+    } else {
+        const uint32_t need = td + bd + 2;
+        if (need > kMaxStack) return c->fail(PT_ERR_UNSUPPORTED, "BVH depth %u+%u exceeds the %u-entry stack", td, bd, kMaxStack);
+        c->stack_cap = need;
+        c->blas_slot = slots;
+        c->blas_depth = bd;
+        if ((rc = c->upload(c->bnodes, bn.data(), bn.size() * sizeof(CNode), "bvh nodes"))) return rc;
+    }
+    c->tlas_depth = td;
+    c->host_roots = roots;
+    if ((rc = c->upload(c->tnodes, tn.data(), tn.size() * sizeof(CNode), "tlas nodes"))) return rc;
+    if ((rc = c->upload(c->roots, roots.data(), roots.size() * sizeof(Root), "roots"))) return rc;
+    if ((rc = c->upload(c->obj_root, obj_root.data(), obj_root.size() * 4, "object roots"))) return rc;
+    if ((rc = c->upload(c->mesh_root, mesh_root.data(), mesh_root.size() * 4, "mesh roots"))) return rc;
+    if ((rc = c->upload(c->mats, d->materials.ptr, d->materials.count * sizeof(pt_material), "materials"))) return rc;
+    if ((rc = c->upload(c->objs, d->objects.ptr, d->objects.count * sizeof(pt_object), "objects"))) return rc;
+    if ((rc = c->upload(c->tlas_raw, d->tlas.ptr, d->tlas.count * sizeof(pt_bvh_node), "tlas"))) return rc;
+    if ((rc = c->upload(c->planes, d->planes.ptr, d->planes.count * sizeof(pt_plane), "planes"))) return rc;
+    if ((rc = c->upload(c->meshes, d->meshes.ptr, d->meshes.count * sizeof(pt_mesh_instance), "meshes"))) return rc;
+    c->n_mats = d->materials.count;
+    c->n_objs = d->objects.count;
+    c->n_tlas = d->tlas.count;
+    c->n_planes = d->planes.count;
+    c->n_meshes = d->meshes.count;
+    return c->sync("upload");
+}
+
+}  // namespace
+
+extern "C" {
+
+int pt_create(pt_ctx** out, int device) {
+    if (!out) return PT_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return PT_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return PT_ERR_ARG;
+    auto* c = new (std::nothrow) pt_ctx();
+    if (!c) return PT_ERR_OOM;
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) { delete c; return PT_ERR_HIP; }
+    *out = c;
+    return PT_OK;
+}
+
+void pt_destroy(pt_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->release_all();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* pt_last_error(const pt_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int pt_set_option(pt_ctx* c, int key, int64_t value) {
+    if (!c) return PT_ERR_ARG;
+    switch (key) {
+        case PT_OPT_BACKEND:
+            if (value != PT_BACKEND_MEGAKERNEL) return c->fail(PT_ERR_UNSUPPORTED, "only the megakernel backend is built");
+            c->backend = int(value);
+            return PT_OK;
+        case PT_OPT_COUNT_STATS: c->count_stats = value != 0; return PT_OK;
+        case PT_OPT_WAVEFRONT_POOL: return value > 0 ? PT_OK : c->fail(PT_ERR_ARG, "pool size must be > 0");
+        default: return c->fail(PT_ERR_ARG, "unknown option %d", key);
+    }
+}
+
+int pt_upload_scene(pt_ctx* c, const pt_scene_desc* d) {
+    if (!c) return PT_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    int rc = validate_desc(c, d);
+    if (rc) return rc;
+    c->has_scene = false;
+    c->flags = d->flags;
+    c->bounce_limit = d->bounce_limit;
+    if ((rc = upload_dynamic(c, d, true))) return rc;
+    // triangles: 48 B geometry records + the original 128 B records
+    const pt_triangle* tr = static_cast<const pt_triangle*>(d->triangles.ptr);
+    std::vector<float4> geo(size_t(d->triangles.count) * 3);
+    for (uint64_t i = 0; i < d->triangles.count; ++i) {
+        const pt_triangle& t = tr[i];
+        geo[3 * i] = make_float4(t.v0[0], t.v0[1], t.v0[2], 0.f);
+        geo[3 * i + 1] = make_float4(t.v1[0] - t.v0[0], t.v1[1] - t.v0[1], t.v1[2] - t.v0[2], 0.f);
+        geo[3 * i + 2] = make_float4(t.v2[0] - t.v0[0], t.v2[1] - t.v0[1], t.v2[2] - t.v0[2], 0.f);
+    }
+    if ((rc = c->upload(c->tri_geo, geo.data(), geo.size() * sizeof(float4), "triangle geometry"))) return rc;
+    if ((rc = c->upload(c->tris, tr, d->triangles.count * sizeof(pt_triangle), "triangles"))) return rc;
+    if ((rc = c->upload(c->bvh_raw, d->bvh.ptr, d->bvh.count * sizeof(pt_bvh_node), "bvh"))) return rc;
+    if ((rc = c->upload(c->lights, d->lights.ptr, d->lights.count * sizeof(pt_light), "lights"))) return rc;
+    c->n_bvh = d->bvh.count;
+    c->n_tris = d->triangles.count;
+    c->n_lights = d->lights.count;
+    // sky + CDF
+    c->sky_w = d->sky_width;
+    c->sky_h = d->sky_height;
+    const size_t texels = size_t(d->sky_width) * d->sky_height;
+    if ((rc = c->upload(c->sky, d->sky_rgba16f, texels * 8, "sky"))) return rc;
+    const size_t cdf_n = size_t(d->sky_height) + 1 + size_t(d->sky_height) * (d->sky_width + 1);
+    if (d->sky_cdf) {
+        if ((rc = c->upload(c->cdf, d->sky_cdf, cdf_n * 4, "sky cdf"))) return rc;
+        c->sky_total_lum = d->sky_total_lum;
+        c->sky_total_weight = d->sky_total_lum;
+    } else {
+        if ((rc = c->alloc(c->cdf, cdf_n * 4, "sky cdf"))) return rc;
+        if ((rc = c->alloc(c->cdf_rows, (size_t(d->sky_height) + 1) * sizeof(double), "sky cdf rows"))) return rc;
+        double* rows = static_cast<double*>(c->cdf_rows.p);
+        hipError_t e = launch_sky_cdf(static_cast<const uint16_t*>(c->sky.p), d->sky_width, d->sky_height,
+                                      d->sky_exposure, static_cast<float*>(c->cdf.p), rows, rows + d->sky_height,
+                                      c->stream);
+        if (e != hipSuccess) return c->hip_fail(e, "sky cdf kernels");
+        double total = 0;
+        e = hipMemcpyAsync(&total, rows + d->sky_height, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e != hipSuccess) return c->hip_fail(e, "sky cdf total");
+        if ((rc = c->sync("sky cdf"))) return rc;
+        c->sky_total_weight = total;
+        c->sky_total_lum = float(total);
+    }
+    if ((rc = c->sync("upload"))) return rc;
+    refresh_dev(c);
+    c->has_scene = true;
+    return PT_OK;
+}
+
+int pt_update_objects(pt_ctx* c, const pt_scene_desc* d) {
+    if (!c) return PT_ERR_ARG;
+    if (!c->has_scene) return c->fail(PT_ERR_STATE, "no scene uploaded");
+    (void)hipSetDevice(c->device);
+    int rc = validate_desc(c, d);
+    if (rc) return rc;
+    if (d->flags != c->flags) return c->fail(PT_ERR_FLAGS, "specialisation flags changed");
+    if (d->objects.count > c->n_objs || d->tlas.count > c->n_tlas || d->planes.count > c->n_planes ||
+        d->meshes.count > c->n_meshes || d->materials.count > c->n_mats || d->bvh.count != c->n_bvh ||
+        d->triangles.count != c->n_tris)
+        return c->fail(PT_ERR_CAPACITY, "scene arrays outgrew the uploaded buffers");
+    c->bounce_limit = d->bounce_limit;
+    if ((rc = upload_dynamic(c, d, false))) return rc;
+    refresh_dev(c);
+    return PT_OK;
+}
+
+int pt_resize(pt_ctx* c, uint32_t w, uint32_t h, int with_gbuffer) {
+    if (!c) return PT_ERR_ARG;
+    if (w == 0 || h == 0 || uint64_t(w) * h > (1ull << 31)) return c->fail(PT_ERR_ARG, "bad frame size %ux%u", w, h);
+    (void)hipSetDevice(c->device);
+    const size_t px = size_t(w) * h;
+    int rc;
+    if ((rc = c->alloc(c->accum, px * 16, "accum"))) return rc;
+    if ((rc = c->alloc(c->out, px * 4, "output"))) return rc;
+    hipError_t e = hipMemsetAsync(c->accum.p, 0, px * 16, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->out.p, 0, px * 4, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "clear frame");
+    if (with_gbuffer) {
+        if ((rc = c->alloc(c->gbuf, px * 48, "gbuffer"))) return rc;
+        e = hipMemsetAsync(c->gbuf.p, 0, px * 48, c->stream);
+        if (e != hipSuccess) return c->hip_fail(e, "clear gbuffer");
+    }
+    c->width = w;
+    c->height = h;
+    c->with_gbuffer = with_gbuffer != 0 || c->with_gbuffer;
+    return c->sync("resize");
+}
+
+static int check_params(pt_ctx* c, const pt_scene_params* p) {
+    if (!p) return c->fail(PT_ERR_ARG, "params is NULL");
+    if (!c->has_scene) return c->fail(PT_ERR_STATE, "no scene uploaded");
+    if (c->width == 0) return c->fail(PT_ERR_STATE, "no frame: call pt_resize first");
+    if (p->width != c->width || p->height != c->height)
+        return c->fail(PT_ERR_ARG, "params frame %ux%u does not match the resized frame %ux%u", p->width, p->height,
+                       c->width, c->height);
+    if ((c->flags & PT_MIS_SKYBOX) && (p->sky_width != c->sky_w || p->sky_height != c->sky_h ||
+                                       p->sky_cdf_index != c->sky_h + 1))
+        return c->fail(PT_ERR_ARG, "params sky fields do not match the uploaded sky (%ux%u, cdf index %u)", c->sky_w,
+                       c->sky_h, c->sky_h + 1);
+    return PT_OK;
+}
+
+static int run_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t sh, uint32_t n_passes,
+                     const uint32_t* seeds, pt_stats* stats, pt_hit* hits_dev, uint32_t n_rec) {
+    int rc;
+    if ((rc = c->alloc(c->stats, kStCount * 8, "stats"))) return rc;
+    hipError_t e = launch_zero_u64(static_cast<unsigned long long*>(c->stats.p), kStCount, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "zero stats");
+    TraceArgs a{};
+    a.P = *p;
+    const uint32_t sx = p->section[0], sy = p->section[1];
+    if (sx >= p->width || sy >= p->height) return c->fail(PT_ERR_ARG, "section origin outside the frame");
+    a.sec_w = sw ? std::min(sw, p->width - sx) : p->width - sx;
+    a.sec_h = sh ? std::min(sh, p->height - sy) : p->height - sy;
+    a.n_passes = n_passes;
+    a.seeds = nullptr;
+    if (seeds && n_passes) {
+        if ((rc = c->upload(c->seeds, seeds, size_t(n_passes) * 4, "seeds"))) return rc;
+        a.seeds = static_cast<const uint32_t*>(c->seeds.p);
+    }
+    a.accum = static_cast<float4*>(c->accum.p);
+    a.out = static_cast<uchar4*>(c->out.p);
+    a.stats = static_cast<unsigned long long*>(c->stats.p);
+    a.hits = hits_dev;
+    a.n_rec = n_rec;
+    e = hipEventRecord(c->ev0, c->stream);
+    if (e == hipSuccess) e = launch_trace(c->dev, a, c->count_stats && !hits_dev, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "trace launch");
+    if ((rc = c->sync("trace"))) return rc;
+    if (stats) {
+        unsigned long long v[kStCount];
+        e = hipMemcpy(v, c->stats.p, sizeof(v), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return c->hip_fail(e, "stats readback");
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        std::memset(stats, 0, sizeof(*stats));
+        stats->extension_rays = v[kStExt];
+        stats->shadow_rays = v[kStShadow];
+        stats->paths = v[kStPaths];
+        stats->node_pops = v[kStPops];
+        stats->inner_pops = v[kStInner];
+        stats->object_visits = v[kStObjs];
+        stats->tri_tests = v[kStTris];
+        stats->shade_events = v[kStShade];
+        stats->kernel_ms = ms;
+        stats->launches = 1;
+    }
+    return PT_OK;
+}
+
+int pt_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t sh, uint32_t n_passes, const uint32_t* seeds,
+             pt_stats* stats) {
+    if (!c) return PT_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    int rc = check_params(c, p);
+    if (rc) return rc;
+    if (n_passes == 0) { if (stats) std::memset(stats, 0, sizeof(*stats)); return PT_OK; }
+    return run_trace(c, p, sw, sh, n_passes, seeds, stats, nullptr, 0);
+}
+
+int pt_trace_hits(pt_ctx* c, const pt_scene_params* p, uint32_t n_bounces, pt_hit* hits) {
+    if (!c) return PT_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    int rc = check_params(c, p);
+    if (rc) return rc;
+    if (!hits || n_bounces == 0 || n_bounces > 64) return c->fail(PT_ERR_ARG, "bad hit buffer / bounce count");
+    const size_t n = size_t(p->width) * p->height * n_bounces;
+    std::vector<pt_hit> init(n);
+    for (auto& h : init) {
+        std::memset(&h, 0, sizeof(h));
+        h.t = -1.0f;
+        h.o_idx = h.m_idx = h.prim_id = h.plane_idx = -1;
+    }
+    if ((rc = c->upload(c->rays_out, init.data(), n * sizeof(pt_hit), "hit records"))) return rc;
+    pt_scene_params q = *p;
+    q.section[0] = q.section[1] = 0;
+    if ((rc = run_trace(c, &q, 0, 0, 1, nullptr, nullptr, static_cast<pt_hit*>(c->rays_out.p), n_bounces))) return rc;
+    hipError_t e = hipMemcpy(hits, c->rays_out.p, n * sizeof(pt_hit), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return c->hip_fail(e, "hit readback");
+    return PT_OK;
+}
+
+int pt_gbuffer(pt_ctx* c, const pt_scene_params* p) {
+    if (!c) return PT_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    int rc = check_params(c, p);
+    if (rc) return rc;
+    if (!c->gbuf.p) return c->fail(PT_ERR_STATE, "frame has no G-buffer: pt_resize(..., with_gbuffer=1)");
+    hipError_t e = launch_gbuffer(c->dev, *p, static_cast<pt_gbuffer_texel*>(c->gbuf.p), static_cast<uchar4*>(c->out.p),
+                                  c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "gbuffer launch");
+    return c->sync("gbuffer");
+}
+
+int pt_denoise(pt_ctx* c, const pt_denoise_params* u, int stride_mode) {
+    if (!c) return PT_ERR_ARG;
+    if (!u) return c->fail(PT_ERR_ARG, "params is NULL");
+    if (stride_mode != 0 && stride_mode != 1) return c->fail(PT_ERR_ARG, "stride_mode must be 0 or 1");
+    if (c->width == 0 || !c->gbuf.p) return c->fail(PT_ERR_STATE, "denoise needs a frame with a G-buffer");
+    if (u->width != c->width || u->height != c->height) return c->fail(PT_ERR_ARG, "denoise size does not match the frame");
+    if (u->passes > 16) return c->fail(PT_ERR_ARG, "at most 16 a-trous passes");
+    (void)hipSetDevice(c->device);
+    const size_t px = size_t(c->width) * c->height;
+    int rc;
+    if ((rc = c->alloc(c->dn_a, px * 16, "denoise A")) || (rc = c->alloc(c->dn_b, px * 16, "denoise B")) ||
+        (rc = c->alloc(c->dn_fin, px * 16, "denoise out")) || (rc = c->alloc(c->dn_guide, px * 48, "denoise guide")))
+        return rc;
+    DenoiseArgs a{};
+    a.u = *u;
+    a.accum = static_cast<const float4*>(c->accum.p);
+    a.gbuf = static_cast<const pt_gbuffer_texel*>(c->gbuf.p);
+    a.guide = static_cast<float4*>(c->dn_guide.p);
+    a.a = static_cast<float4*>(c->dn_a.p);
+    a.b = static_cast<float4*>(c->dn_b.p);
+    a.fin = static_cast<float4*>(c->dn_fin.p);
+    a.out = static_cast<uchar4*>(c->out.p);
+    hipError_t e = launch_denoise(a, stride_mode, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "denoise launch");
+    return c->sync("denoise");
+}
+
+static int copy_out(pt_ctx* c, void* dst, const DevBuf& b, uint64_t bytes, uint64_t need, const char* what) {
+    if (!c) return PT_ERR_ARG;
+    if (!dst || bytes < need) return c->fail(PT_ERR_ARG, "%s: destination smaller than %llu bytes", what, (unsigned long long)need);
+    if (!b.p) return c->fail(PT_ERR_STATE, "%s: buffer not allocated", what);
+    (void)hipSetDevice(c->device);
+    int rc = c->sync(what);
+    if (rc) return rc;
+    hipError_t e = hipMemcpy(dst, b.p, need, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return c->hip_fail(e, what);
+    return PT_OK;
+}
+static int copy_in(pt_ctx* c, DevBuf& b, const void* src, uint64_t bytes, uint64_t need, const char* what) {
+    if (!c) return PT_ERR_ARG;
+    if (!src || bytes != need) return c->fail(PT_ERR_ARG, "%s: expected exactly %llu bytes", what, (unsigned long long)need);
+    if (!b.p) return c->fail(PT_ERR_STATE, "%s: buffer not allocated", what);
+    (void)hipSetDevice(c->device);
+    hipError_t e = hipMemcpy(b.p, src, need, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return c->hip_fail(e, what);
+    return PT_OK;
+}
+
+int pt_read_accum(pt_ctx* c, float* dst, uint64_t bytes) {
+    return copy_out(c, dst, c ? c->accum : DevBuf{}, bytes, c ? uint64_t(c->width) * c->height * 16 : 0, "read accum");
+}
+int pt_write_accum(pt_ctx* c, const float* src, uint64_t bytes) {
+    if (!c) return PT_ERR_ARG;
+    return copy_in(c, c->accum, src, bytes, uint64_t(c->width) * c->height * 16, "write accum");
+}
+int pt_read_output(pt_ctx* c, uint8_t* dst, uint64_t bytes) {
+    return copy_out(c, dst, c ? c->out : DevBuf{}, bytes, c ? uint64_t(c->width) * c->height * 4 : 0, "read output");
+}
+int pt_read_gbuffer(pt_ctx* c, pt_gbuffer_texel* dst, uint64_t bytes) {
+    return copy_out(c, dst, c ? c->gbuf : DevBuf{}, bytes, c ? uint64_t(c->width) * c->height * 48 : 0, "read gbuffer");
+}
+int pt_write_gbuffer(pt_ctx* c, const pt_gbuffer_texel* src, uint64_t bytes) {
+    if (!c) return PT_ERR_ARG;
+    return copy_in(c, c->gbuf, src, bytes, uint64_t(c->width) * c->height * 48, "write gbuffer");
+}
+int pt_read_denoised(pt_ctx* c, float* dst, uint64_t bytes) {
+    return copy_out(c, dst, c ? c->dn_fin : DevBuf{}, bytes, c ? uint64_t(c->width) * c->height * 16 : 0, "read denoised");
+}
+int pt_read_sky_cdf(pt_ctx* c, float* dst, uint64_t count, float* total) {
+    if (!c) return PT_ERR_ARG;
+    const uint64_t need = uint64_t(c->sky_h) + 1 + uint64_t(c->sky_h) * (c->sky_w + 1);
+    if (!c->has_scene) return c->fail(PT_ERR_STATE, "no scene uploaded");
+    int rc = copy_out(c, dst, c->cdf, count * 4, need * 4, "read sky cdf");
+    if (rc) return rc;
+    if (total) *total = c->sky_total_lum;
+    return PT_OK;
+}
+
+int pt_trace_rays(pt_ctx* c, const float* rays, uint32_t n, pt_hit* hits) {
+    if (!c) return PT_ERR_ARG;
+    if (!c->has_scene) return c->fail(PT_ERR_STATE, "no scene uploaded");
+    if (n == 0) return PT_OK;
+    if (!rays || !hits) return c->fail(PT_ERR_ARG, "rays/hits is NULL");
+    (void)hipSetDevice(c->device);
+    int rc;
+    if ((rc = c->upload(c->rays_in, rays, size_t(n) * 24, "rays"))) return rc;
+    if ((rc = c->alloc(c->rays_out, size_t(n) * sizeof(pt_hit), "hits"))) return rc;
+    RayArgs a{};
+    a.rays = static_cast<const float*>(c->rays_in.p);
+    a.n = n;
+    a.hits = static_cast<pt_hit*>(c->rays_out.p);
+    hipError_t e = launch_rays(c->dev, a, false, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "rays launch");
+    if ((rc = c->sync("rays"))) return rc;
+    e = hipMemcpy(hits, c->rays_out.p, size_t(n) * sizeof(pt_hit), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return c->hip_fail(e, "hits readback");
+    return PT_OK;
+}
+
+int pt_trace_shadow_rays(pt_ctx* c, const float* rays, const int32_t* target_idx, const float* target_dist, uint32_t n,
+                         float* trans) {
+    if (!c) return PT_ERR_ARG;
+    if (!c->has_scene) return c->fail(PT_ERR_STATE, "no scene uploaded");
+    if (n == 0) return PT_OK;
+    if (!rays || !target_idx || !target_dist || !trans) return c->fail(PT_ERR_ARG, "NULL argument");
+    (void)hipSetDevice(c->device);
+    int rc;
+    if ((rc = c->upload(c->rays_in, rays, size_t(n) * 24, "rays")) ||
+        (rc = c->upload(c->rays_aux, target_idx, size_t(n) * 4, "targets")) ||
+        (rc = c->upload(c->rays_aux2, target_dist, size_t(n) * 4, "distances")) ||
+        (rc = c->alloc(c->rays_out, size_t(n) * 12, "transmittance")))
+        return rc;
+    RayArgs a{};
+    a.rays = static_cast<const float*>(c->rays_in.p);
+    a.target = static_cast<const int32_t*>(c->rays_aux.p);
+    a.tdist = static_cast<const float*>(c->rays_aux2.p);
+    a.n = n;
+    a.trans = static_cast<float*>(c->rays_out.p);
+    hipError_t e = launch_rays(c->dev, a, true, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "shadow rays launch");
+    if ((rc = c->sync("shadow rays"))) return rc;
+    e = hipMemcpy(trans, c->rays_out.p, size_t(n) * 12, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return c->hip_fail(e, "transmittance readback");
+    return PT_OK;
+}
+
+int pt_synchronize(pt_ctx* c) {
+    if (!c) return PT_ERR_ARG;
+    return c->sync("synchronize");
+}
+
+}  // extern "C"

@@ -1,0 +1,55 @@
+// pt_internal.h — interface between the C ABI (pt_api.cpp) and the kernels
+// (pt_kernels.hip).  Not installed; not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "pt_abi.h"
+#include "dev/geometry.h"
+
+namespace pt {
+
+// Counter slots in the device stats buffer (uint64 each).
+enum { kStExt = 0, kStShadow, kStPaths, kStPops, kStInner, kStObjs, kStTris, kStShade, kStCount };
+
+struct TraceArgs {
+    pt_scene_params P;
+    uint32_t sec_w, sec_h;
+    uint32_t n_passes;
+    const uint32_t* seeds;       // device, n_passes entries (NULL: P.seed)
+    float4* accum;
+    uchar4* out;
+    unsigned long long* stats;   // kStCount counters
+    pt_hit* hits;                // hit recording (NULL: off)
+    uint32_t n_rec;
+};
+
+struct RayArgs {
+    const float* rays;           // 6 floats per ray
+    const int32_t* target;       // shadow: per-ray ignored object
+    const float* tdist;          // shadow: per-ray distance
+    uint32_t n;
+    pt_hit* hits;
+    float* trans;
+};
+
+struct DenoiseArgs {
+    pt_denoise_params u;
+    const float4* accum;
+    const pt_gbuffer_texel* gbuf;
+    float4* guide;               // 3 float4 per pixel (see dn_guide)
+    float4* a;
+    float4* b;
+    float4* fin;                 // final fp32 result
+    uchar4* out;
+};
+
+// Launchers (pt_kernels.hip).  All asynchronous on `stream`.
+hipError_t launch_trace(const DevScene& S, const TraceArgs& a, bool count_nodes, hipStream_t stream);
+hipError_t launch_gbuffer(const DevScene& S, const pt_scene_params& P, pt_gbuffer_texel* gbuf, uchar4* out,
+                          hipStream_t stream);
+hipError_t launch_rays(const DevScene& S, const RayArgs& a, bool shadow, hipStream_t stream);
+hipError_t launch_denoise(const DenoiseArgs& a, int stride_mode, hipStream_t stream);
+hipError_t launch_sky_cdf(const uint16_t* sky, uint32_t w, uint32_t h, float exposure, float* cdf, double* row_tmp,
+                          double* total, hipStream_t stream);
+hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t stream);
+
+}  // namespace pt

@@ -1,0 +1,644 @@
+// pt_kernels.hip — gfx950 kernels of the tracer hot path.
+//
+//   trace_kernel    shader.wgsl `main` (2273-2492): one lane per pixel, all of
+//                   a call's progressive passes in one launch (the running
+//                   mean stays in registers; accum is read and written once)
+//   gbuffer_kernel  shader.wgsl `gbuf_main` (2510-2603)
+//   rays_kernel     trace_scene / trace_scene_shadow on caller rays
+//   dn_*            denoiser.wgsl prepass / atrous / composite (55-214)
+//   cdf_*           HDRTexture.buildHDRCDF (tracer.js:135-204) as a parallel
+//                   double-precision scan, one workgroup per row
+//
+// Work mapping: a 64-lane workgroup (one wave64) shades an 8x8 pixel tile.
+// Workgroup ids are remapped so that each of the 8 XCDs receives a
+// contiguous band of tiles (the dispatcher deals workgroups round-robin over
+// XCDs), which keeps neighbouring tiles — and the BVH nodes they touch — in
+// one XCD's L2.
+#include "pt_internal.h"
+#include "dev/shade.h"
+
+namespace pt {
+
+constexpr int kTile = 8;
+constexpr int kWave = 64;
+constexpr uint32_t kFlagsC1 = PT_HAS_SPHERES | PT_HAS_CUBES | PT_HAS_PLANES | PT_HAS_CYLINDERS | PT_HAS_LIGHTS |
+                              PT_HAS_SKYBOX | PT_MIS_SKYBOX;
+constexpr uint32_t kFlagsC2 = PT_HAS_MESHES | PT_HAS_SKYBOX | PT_MIS_SKYBOX;
+
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
+    const uint32_t q = nb / 8u, r = nb % 8u;
+    const uint32_t xcd = b % 8u, slot = b / 8u;
+    const uint32_t base = xcd < r ? xcd * (q + 1u) : r * (q + 1u) + (xcd - r) * q;
+    return base + slot;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(uint32_t v) {
+    unsigned long long s = v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    return s;
+}
+
+__device__ void flush_counters(const Counters& c, unsigned long long* stats, bool nodes) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const unsigned long long e = wave_sum(c.ext), s = wave_sum(c.shadow), p = wave_sum(c.paths);
+    if (lane == 0) {
+        atomicAdd(stats + kStExt, e);
+        atomicAdd(stats + kStShadow, s);
+        atomicAdd(stats + kStPaths, p);
+    }
+    if (nodes) {
+        const unsigned long long a = wave_sum(c.pops), b = wave_sum(c.inner), o = wave_sum(c.objs),
+                                 t = wave_sum(c.tris), h = wave_sum(c.shade);
+        if (lane == 0) {
+            atomicAdd(stats + kStPops, a);
+            atomicAdd(stats + kStInner, b);
+            atomicAdd(stats + kStObjs, o);
+            atomicAdd(stats + kStTris, t);
+            atomicAdd(stats + kStShade, h);
+        }
+    }
+}
+
+__device__ __forceinline__ v3 camera_dir(const pt_scene_params& P, float su, float sv) {
+    const v3 r00 = ld3(P.ray00), r10 = ld3(P.ray10), r01 = ld3(P.ray01), r11 = ld3(P.ray11);
+    return normalize(lerp(lerp(r00, r10, su), lerp(r01, r11, su), 1.0f - sv));
+}
+
+__device__ void store_hit(pt_hit* o, const HitRec& h, const SurfHit& s) {
+    pt_hit r;
+    r.t = h.t;
+    r.o_idx = s.o_idx;
+    r.m_idx = s.m_idx;
+    r.prim_id = h.prim;
+    r.plane_idx = h.kind == kPlaneHit ? h.idx : -1;
+    r.count = h.count;
+    r.n[0] = s.n.x; r.n[1] = s.n.y; r.n[2] = s.n.z;
+    r.uv[0] = s.uv.x; r.uv[1] = s.uv.y;
+    r._pad[0] = r._pad[1] = r._pad[2] = 0.0f;
+    *o = r;
+}
+
+// evaluate_nee (shader.wgsl:2226-2271) without heightmaps.
+template <uint32_t F, bool CN>
+__device__ v3 nee(const DevScene& S, Flags<F> fl, const LightSample& ls, v3 hp, v3 no, v3 V, const pt_material& m,
+                  const Surface& sc, const Media& md, int ignore, LdsStack& st, Counters& c) {
+    if (ls.pdf <= 0.0f) return splat(0.0f);
+    const float nl = dot(no, ls.dir);
+    if (nl <= 0.0f && m.transmission <= 0.0f) return splat(0.0f);
+    const float bp = pdf_bsdf(V, ls.dir, m, sc, md);
+    const float w = mis_weight(ls.pdf, bp);
+    const v3 f = eval_bsdf(V, ls.dir, m, sc, md);
+    const v3 cw = w * f / mx(ls.pdf, 1e-6f);
+    if (hmax(ls.color * cw) < 1e-5f) return splat(0.0f);
+    const v3 off = nl < 0.0f ? -no * 0.001f : no * 0.001f;
+    c.shadow++;
+    if (occluded<F, CN>(S, fl, hp + off, ls.dir, ignore, ls.dist, st, c)) return splat(0.0f);
+    return ls.color * cw;
+}
+
+// One path of shader.wgsl `main` (2279-2480).  rec: when non-null, the first
+// n_rec trace_scene hits of the path are written there.
+template <uint32_t F, bool CN, bool REC>
+__device__ v3 trace_path(const DevScene& S, Flags<F> fl, const pt_scene_params& PP, const Params& P, uint32_t px,
+                         uint32_t py, uint32_t sample_number, uint32_t seed, LdsStack& st, Counters& c, pt_hit* rec,
+                         uint32_t n_rec) {
+    const uint32_t idx = py * PP.width + px;
+    Pcg rng{(idx * 1973u + sample_number * 9277u + seed * 26699u) | 1u};
+    rng.next();
+    const float jx = rng.next();
+    const float jy = rng.next();
+    v3 rd = camera_dir(PP, (float(px) + jx) / float(PP.width), (float(py) + jy) / float(PP.height));
+    v3 ro = ld3(PP.eye);
+    v3 tp = splat(1.0f), rad = splat(0.0f);
+    float last_pdf = 1.0f, last_wsum = 0.0f;
+    Media md;
+    md.n = 0;
+    c.paths++;
+    for (int bounce = 0; bounce < S.bounce_limit; ++bounce) {
+        HitRec h;
+        c.ext++;
+        closest_hit<F, CN>(S, fl, ro, rd, st, h, c);
+        const SurfHit sh = resolve_hit<REC>(S, ro, rd, h);
+        if (REC && uint32_t(bounce) < n_rec) store_hit(rec + bounce, h, sh);
+        // Beer-Lambert attenuation and medium emission (shader.wgsl:2325-2342)
+        const Medium cm = md.top();
+        if (cm.ior > 1.0f || len(cm.sigma) > 0.0f) {
+            const float dist = sh.m_idx == -1 ? 1000.0f : h.t;
+            const v3 att = mk(expf(-cm.sigma.x * dist), expf(-cm.sigma.y * dist), expf(-cm.sigma.z * dist));
+            const float ex = cm.sigma.x > 1e-4f ? cm.emission.x * (1.0f - att.x) / cm.sigma.x : cm.emission.x * dist;
+            const float ey = cm.sigma.y > 1e-4f ? cm.emission.y * (1.0f - att.y) / cm.sigma.y : cm.emission.y * dist;
+            const float ez = cm.sigma.z > 1e-4f ? cm.emission.z * (1.0f - att.z) / cm.sigma.z : cm.emission.z * dist;
+            rad = rad + tp * firefly(mk(ex, ey, ez), bounce);
+            tp = tp * att;
+        }
+        if (sh.m_idx == -1) {  // miss (shader.wgsl:2344-2358)
+            if (fl.has(PT_HAS_SKYBOX)) {
+                const v3 sky = sample_sky(S, rd);
+                float w = 1.0f;
+                if (bounce > 0 && fl.has(PT_MIS_SKYBOX)) w = mis_weight(last_pdf, sky_pdf(S, P, rd));
+                rad = rad + tp * firefly(sky * w, bounce);
+            } else {
+                rad = rad + tp * firefly(mk(0.02f, 0.03f, 0.05f), bounce);
+            }
+            break;
+        }
+        if (CN) c.shade++;
+        const pt_material& m = S.mats[sh.m_idx];
+        Surface sc;
+        sc.normal = sh.n;
+        sc.snormal = sh.n;
+        sc.albedo = ld3(m.color);
+        sc.alpha = 1.0f;
+        sc.metallic = m.metallic;
+        sc.roughness = m.roughness;
+        sc.emittance = ld3(m.emittance);
+        // implicit light hit (shader.wgsl:2377-2395)
+        if (len(sc.emittance) > 0.0f) {
+            float w = 1.0f;
+            if (fl.has(PT_HAS_LIGHTS) && bounce > 0 && sh.o_idx >= 0) {
+                const pt_object& ob = S.objs[sh.o_idx];
+                if (ob.light_idx >= 0) {
+                    const pt_light& L = S.lights[ob.light_idx];
+                    float raw = 0.0f;
+                    if (fl.has(PT_HAS_SPHERES) && ob.object_type == PT_OBJ_SPHERE) raw = sphere_light_pdf(L, ob, ro, rd);
+                    else if (fl.has(PT_HAS_CUBES) && ob.object_type == PT_OBJ_CUBE) raw = box_light_pdf(L, ro, rd, h.t, sh.n);
+                    const v3 tl = ld3(ob.world_position) - ro;
+                    const float d2 = mx(dot(tl, tl), 0.001f);
+                    const float imp = L.power / d2;
+                    const float sel = last_wsum > 0.0f ? imp / last_wsum : 0.0f;
+                    w = mis_weight(last_pdf, raw * sel);
+                }
+            }
+            rad = rad + tp * firefly(sc.emittance * w, bounce);
+            if (len(sc.emittance) > 1.0f) break;
+        }
+        const v3 hp = ro + rd * h.t;
+        const v3 V = -rd;
+        const v3 no = dot(sc.normal, V) > 0.0f ? sc.normal : -sc.normal;
+        // light NEE with one-pass reservoir selection (shader.wgsl:2402-2446)
+        if (fl.has(PT_HAS_LIGHTS)) {
+            int sel = -1;
+            float wsum = 0.0f, sw = 0.0f;
+            for (uint32_t i = 0; i < S.n_lights; ++i) {
+                const pt_light& L = S.lights[i];
+                if (L.obj_idx < 0) continue;
+                const v3 tl = ld3(S.objs[L.obj_idx].world_position) - hp;
+                const float d2 = mx(dot(tl, tl), 0.001f);
+                const float w = L.power / d2;
+                wsum += w;
+                if (rng.next() < (w / wsum)) { sel = int(i); sw = w; }
+            }
+            if (sel >= 0 && wsum > 0.0f) {
+                const pt_light& L = S.lights[sel];
+                const pt_object& lo = S.objs[L.obj_idx];
+                LightSample ls;
+                ls.dir = splat(0.0f); ls.color = splat(0.0f); ls.dist = 0.0f; ls.pdf = 0.0f;
+                if (fl.has(PT_HAS_SPHERES) && lo.object_type == PT_OBJ_SPHERE) {
+                    const float x0 = rng.next();
+                    const float x1 = rng.next();
+                    ls = sphere_light_sample(S, L, lo, hp, x0, x1);
+                } else if (fl.has(PT_HAS_CUBES) && lo.object_type == PT_OBJ_CUBE) {
+                    const float x0 = rng.next();
+                    const float x1 = rng.next();
+                    ls = box_light_sample(S, L, lo, hp, x0, x1, rng);
+                }
+                if (ls.pdf > 0.0f) {
+                    ls.pdf *= sw / wsum;
+                    const v3 inc = nee<F, CN>(S, fl, ls, hp, no, V, m, sc, md, L.obj_idx, st, c);
+                    rad = rad + tp * firefly(inc, bounce);
+                }
+            }
+            last_wsum = wsum;
+        }
+        // env NEE (shader.wgsl:2449-2453)
+        if (fl.has(PT_HAS_SKYBOX) && fl.has(PT_MIS_SKYBOX)) {
+            const float ux = rng.next();
+            const float uy = rng.next();
+            const LightSample env = sample_env(S, P, ux, uy);
+            const v3 inc = nee<F, CN>(S, fl, env, hp, no, V, m, sc, md, -1, st, c);
+            rad = rad + tp * firefly(inc, bounce);
+        }
+        if (!sample_bsdf(ro, rd, tp, m, sc, hp, md, last_pdf, rng)) break;
+        // Russian roulette from the third bounce (shader.wgsl:2474-2479)
+        if (bounce < 2) continue;
+        const float surv = clampf(hmax(tp), 0.05f, 0.95f);
+        if (rng.next() > surv) break;
+        tp = tp / surv;
+    }
+    return rad;
+}
+
+template <uint32_t F, bool CN, bool REC>
+__global__ void __launch_bounds__(kWave) trace_kernel(DevScene S, TraceArgs A) {
+    extern __shared__ uint32_t lds_stack[];
+    const Flags<F> fl{S.flags};
+    const uint32_t tiles_x = (A.sec_w + kTile - 1) / kTile;
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t px = A.P.section[0] + tx * kTile + (lane & 7u);
+    const uint32_t py = A.P.section[1] + ty * kTile + (lane >> 3);
+    const bool active = px < A.P.section[0] + A.sec_w && py < A.P.section[1] + A.sec_h && px < A.P.width &&
+                        py < A.P.height;
+    Counters c;
+    c.clear();
+    if (active) {
+        LdsStack st{lds_stack + lane, blockDim.x, S.stack_cap, 0};
+        Params P{A.P.sky_total_lum, A.P.sky_width, A.P.sky_height, A.P.sky_cdf_index};
+        const uint32_t idx = py * A.P.width + px;
+        const float4 old = A.accum[idx];
+        v3 acc = mk(old.x, old.y, old.z);
+        for (uint32_t pass = 0; pass < A.n_passes; ++pass) {
+            const uint32_t sn = A.P.sample_number + pass;
+            const uint32_t seed = A.seeds ? A.seeds[pass] : A.P.seed;
+            pt_hit* rec = REC ? A.hits + size_t(idx) * A.n_rec : nullptr;
+            const v3 r = trace_path<F, CN, REC>(S, fl, A.P, P, px, py, sn, seed, st, c, rec, A.n_rec);
+            const float w = 1.0f / float(sn + 1u);
+            acc = lerp(acc, r, w);
+        }
+        if (!REC) {
+            A.accum[idx] = make_float4(acc.x, acc.y, acc.z, 1.0f);
+            v3 o = acc * splat(A.P.exposure);
+            o = o / (o + splat(0.3f));
+            A.out[idx] = make_uchar4(unorm8(powf(o.x, 0.4545f)), unorm8(powf(o.y, 0.4545f)),
+                                     unorm8(powf(o.z, 0.4545f)), 255);
+        }
+    }
+    if (A.stats) flush_counters(c, A.stats, CN);
+}
+
+// gbuf_main, shader.wgsl:2510-2603
+template <uint32_t F>
+__global__ void __launch_bounds__(kWave) gbuffer_kernel(DevScene S, pt_scene_params P, pt_gbuffer_texel* gbuf,
+                                                        uchar4* out) {
+    extern __shared__ uint32_t lds_stack[];
+    const Flags<F> fl{S.flags};
+    const uint32_t tiles_x = (P.width + kTile - 1) / kTile;
+    const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t lane = threadIdx.x;
+    const uint32_t px = (tile % tiles_x) * kTile + (lane & 7u);
+    const uint32_t py = (tile / tiles_x) * kTile + (lane >> 3);
+    if (px >= P.width || py >= P.height) return;
+    LdsStack st{lds_stack + lane, blockDim.x, S.stack_cap, 0};
+    Counters c;
+    c.clear();
+    const uint32_t idx = py * P.width + px;
+    v3 rd = camera_dir(P, (float(px) + 0.5f) / float(P.width), (float(py) + 0.5f) / float(P.height));
+    v3 ro = ld3(P.eye);
+    v3 n = splat(0.0f), albedo = splat(0.0f), emit = splat(0.0f), tint = splat(1.0f);
+    float rough = 0.0f, metal = 0.0f, trans = 0.0f, dist = 0.0f;
+    for (int bounce = 0; bounce < S.bounce_limit; ++bounce) {
+        HitRec h;
+        closest_hit<F, false>(S, fl, ro, rd, st, h, c);
+        const SurfHit sh = resolve_hit<false>(S, ro, rd, h);
+        if (sh.m_idx == -1) {
+            n = splat(0.0f);
+            emit = sample_sky(S, rd);
+            albedo = splat(0.0f);
+            break;
+        }
+        const pt_material& m = S.mats[sh.m_idx];
+        trans = m.transmission;
+        albedo = ld3(m.color);
+        emit = ld3(m.emittance);
+        rough = m.roughness;
+        metal = m.metallic;
+        n = dot(sh.n, -rd) > 0.0f ? sh.n : -sh.n;
+        dist += h.t;
+        // alpha is 1 without albedo textures, so the alpha < 1 branch never runs
+        if (m.metallic >= 0.99f && m.roughness <= 0.01f) {
+            tint = tint * albedo;
+            ro = ro + rd * h.t + n * 0.001f;
+            rd = reflect(rd, n);
+            continue;
+        }
+        break;
+    }
+    const v3 color = (albedo + emit) * tint;
+    pt_gbuffer_texel g;
+    g.normal[0] = n.x; g.normal[1] = n.y; g.normal[2] = n.z;
+    g.roughness = rough;
+    g.albedo[0] = color.x; g.albedo[1] = color.y; g.albedo[2] = color.z;
+    g.metallic = metal;
+    g.motion[0] = g.motion[1] = 0.0f;
+    g.transmission = trans;
+    g.dist = dist;
+    gbuf[idx] = g;
+    v3 o;
+    if (px < P.width * 1u / 4u) {
+        o = splat(dist / (dist + 3.0f));
+    } else if (px < P.width * 2u / 4u) {
+        o = n * 0.5f + splat(0.5f);
+    } else if (px < P.width * 3u / 4u) {
+        v3 t = color * splat(P.exposure);
+        t = t / (t + splat(0.3f));
+        o = mk(powf(t.x, 0.4545f), powf(t.y, 0.4545f), powf(t.z, 0.4545f));
+    } else {
+        o = mk(trans, rough, metal);
+    }
+    out[idx] = make_uchar4(unorm8(o.x), unorm8(o.y), unorm8(o.z), 255);
+}
+
+template <uint32_t F>
+__global__ void __launch_bounds__(kWave) rays_kernel(DevScene S, RayArgs A, int shadow) {
+    extern __shared__ uint32_t lds_stack[];
+    const Flags<F> fl{S.flags};
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    LdsStack st{lds_stack + threadIdx.x, blockDim.x, S.stack_cap, 0};
+    Counters c;
+    c.clear();
+    const float* r = A.rays + size_t(i) * 6;
+    const v3 o = mk(r[0], r[1], r[2]), d = mk(r[3], r[4], r[5]);
+    if (shadow) {
+        const bool b = occluded<F, false>(S, fl, o, d, A.target[i], A.tdist[i], st, c);
+        const float v = b ? 0.0f : 1.0f;
+        A.trans[3 * i] = v; A.trans[3 * i + 1] = v; A.trans[3 * i + 2] = v;
+    } else {
+        HitRec h;
+        closest_hit<F, false>(S, fl, o, d, st, h, c);
+        store_hit(A.hits + i, h, resolve_hit<true>(S, o, d, h));
+    }
+}
+
+// ---------------------------------------------------------------- denoiser --
+__device__ __forceinline__ v3 dn_tonemap(v3 c, float e) {
+    v3 t = c * e;
+    t = t / (t + splat(0.3f));
+    return mk(powf(t.x, 0.4545f), powf(t.y, 0.4545f), powf(t.z, 0.4545f));
+}
+
+// Per-pixel guide values the a-trous taps compare, computed once instead of
+// once per tap: guide[3i] = (normal, |normal| > 0.1), guide[3i+1] =
+// (tonemap(albedo), dist/(dist+3)), guide[3i+2] = (transmission, roughness,
+// metallic, 0).
+__global__ void dn_guide(DenoiseArgs A) {
+    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    if (x >= A.u.width || y >= A.u.height) return;
+    const uint32_t i = y * A.u.width + x;
+    const pt_gbuffer_texel g = A.gbuf[i];
+    const v3 n = ld3(g.normal);
+    const v3 ta = dn_tonemap(ld3(g.albedo), A.u.exposure);
+    A.guide[3 * i] = make_float4(n.x, n.y, n.z, len(n) > 0.1f ? 1.0f : 0.0f);
+    A.guide[3 * i + 1] = make_float4(ta.x, ta.y, ta.z, g.dist / (g.dist + 3.0f));
+    A.guide[3 * i + 2] = make_float4(g.transmission, g.roughness, g.metallic, 0.0f);
+}
+
+// prepass_main, denoiser.wgsl:55-112
+__global__ void dn_prepass(DenoiseArgs A) {
+    const int W = int(A.u.width), H = int(A.u.height);
+    const int x = int(blockIdx.x * 16 + threadIdx.x), y = int(blockIdx.y * 16 + threadIdx.y);
+    if (x >= W || y >= H) return;
+    const uint32_t i = uint32_t(y * W + x);
+    float4 cc;
+    if (A.u.flags & 1u) {
+        float4 col[9];
+        float lum[9];
+        int k = 0;
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int cx = min(max(x + dx, 0), W - 1), cy = min(max(y + dy, 0), H - 1);
+                col[k] = A.accum[cy * W + cx];
+                lum[k] = dot(mk(col[k].x, col[k].y, col[k].z), mk(0.299f, 0.587f, 0.114f));
+                ++k;
+            }
+#pragma unroll
+        for (int a = 0; a < 5; ++a)
+#pragma unroll
+            for (int b = a + 1; b < 9; ++b)
+                if (lum[b] < lum[a]) {
+                    const float tl = lum[a]; lum[a] = lum[b]; lum[b] = tl;
+                    const float4 tc = col[a]; col[a] = col[b]; col[b] = tc;
+                }
+        cc = col[4];
+    } else {
+        cc = A.accum[i];
+    }
+    const bool valid = cc.x >= 0.0f && cc.y >= 0.0f && cc.z >= 0.0f && cc.x <= 10000.0f && cc.y <= 10000.0f &&
+                       cc.z <= 10000.0f;
+    if (!valid) { cc.x = 0.0f; cc.y = 0.0f; cc.z = 0.0f; }
+    v3 mc = dn_tonemap(mk(cc.x, cc.y, cc.z), A.u.exposure);
+    const float4 g0 = A.guide[3 * i], g1 = A.guide[3 * i + 1];
+    const bool sky = len(mk(g0.x, g0.y, g0.z)) < 0.1f;
+    if ((A.u.flags & 2u) && !sky) {
+        const v3 ma = vmax(mk(g1.x, g1.y, g1.z), splat(0.001f));
+        mc = mc / ma;
+        mc = mk(clampf(mc.x, 0.0f, 10.0f), clampf(mc.y, 0.0f, 10.0f), clampf(mc.z, 0.0f, 10.0f));
+    }
+    A.a[i] = make_float4(mc.x, mc.y, mc.z, cc.w);
+}
+
+// atrous_main, denoiser.wgsl:117-189
+__global__ void dn_atrous(DenoiseArgs A, const float4* __restrict__ in, float4* __restrict__ outb, int step) {
+    const int W = int(A.u.width), H = int(A.u.height);
+    const int x = int(blockIdx.x * 16 + threadIdx.x), y = int(blockIdx.y * 16 + threadIdx.y);
+    if (x >= W || y >= H) return;
+    const uint32_t ci = uint32_t(y * W + x);
+    const float4 cc = in[ci];
+    const float4 cg0 = A.guide[3 * ci], cg1 = A.guide[3 * ci + 1], cg2 = A.guide[3 * ci + 2];
+    const v3 cn = mk(cg0.x, cg0.y, cg0.z);
+    const float kern[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
+    float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+    float sw = 0.0f;
+    const float preserve = (1.0f - cg2.y) * mx(cg2.z, cg2.x);
+    const float att = lerp(1.0f, 0.0f, preserve * A.u.glossy_protect);
+    const float sc2 = A.u.sigma_color * A.u.sigma_color + 1e-5f;
+    const float sd2 = A.u.sigma_depth * A.u.sigma_depth + 1e-5f;
+    const float sa2 = A.u.sigma_albedo * A.u.sigma_albedo + 1e-5f;
+    const float sm2 = A.u.sigma_material * A.u.sigma_material + 1e-5f;
+#pragma unroll
+    for (int dy = -2; dy <= 2; ++dy) {
+        const int syp = min(max(y + dy * step, 0), H - 1);
+#pragma unroll
+        for (int dx = -2; dx <= 2; ++dx) {
+            const int sxp = min(max(x + dx * step, 0), W - 1);
+            const uint32_t si = uint32_t(syp * W + sxp);
+            const float4 s = in[si];
+            const float4 g0 = A.guide[3 * si], g1 = A.guide[3 * si + 1], g2 = A.guide[3 * si + 2];
+            const float hk = kern[dx + 2] * kern[dy + 2];
+            const v3 cd = mk(cc.x, cc.y, cc.z) - mk(s.x, s.y, s.z);
+            const float wc = expf(-dot(cd, cd) / sc2);
+            float wn = 1.0f;
+            if (cg0.w != 0.0f || g0.w != 0.0f) wn = powf(mx(0.0f, dot(cn, mk(g0.x, g0.y, g0.z))), A.u.sigma_normal);
+            const float dd = cg1.w - g1.w;
+            const float wd = expf(-(dd * dd) / sd2);
+            const v3 ad = mk(cg1.x, cg1.y, cg1.z) - mk(g1.x, g1.y, g1.z);
+            const float wa = expf(-dot(ad, ad) / sa2);
+            const v3 md_ = mk(cg2.x, cg2.y, cg2.z) - mk(g2.x, g2.y, g2.z);
+            const float wm = expf(-dot(md_, md_) / sm2);
+            float w = hk * wc * wn * wd * wa * wm;
+            if (dx != 0 || dy != 0) w *= att;
+            sum.x = sum.x + s.x * w;
+            sum.y = sum.y + s.y * w;
+            sum.z = sum.z + s.z * w;
+            sum.w = sum.w + s.w * w;
+            sw += w;
+        }
+    }
+    const float den = sw + 1e-5f;
+    outb[ci] = make_float4(sum.x / den, sum.y / den, sum.z / den, sum.w / den);
+}
+
+// composite_main, denoiser.wgsl:194-214
+__global__ void dn_composite(DenoiseArgs A, const float4* __restrict__ in) {
+    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    if (x >= A.u.width || y >= A.u.height) return;
+    const uint32_t i = y * A.u.width + x;
+    float4 f = in[i];
+    const float4 g0 = A.guide[3 * i], g1 = A.guide[3 * i + 1];
+    const bool sky = len(mk(g0.x, g0.y, g0.z)) < 0.1f;
+    if ((A.u.flags & 2u) && !sky) {
+        const v3 ma = vmax(mk(g1.x, g1.y, g1.z), splat(0.001f));
+        f.x = f.x * ma.x;
+        f.y = f.y * ma.y;
+        f.z = f.z * ma.z;
+    }
+    A.fin[i] = f;
+    A.out[i] = make_uchar4(unorm8(f.x), unorm8(f.y), unorm8(f.z), unorm8(f.w));
+}
+
+// --------------------------------------------------------------- sky CDF ----
+constexpr int kCdfThreads = 256;
+
+__device__ __forceinline__ double half_bits_to_double(uint16_t b) {
+    return double(float(__builtin_bit_cast(_Float16, b)));
+}
+
+__device__ double block_exclusive_scan(double v, double* sh, double* total) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kCdfThreads; off <<= 1) {
+        const double add = t >= off ? sh[t - off] : 0.0;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    const double incl = sh[t];
+    *total = sh[kCdfThreads - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+// One workgroup per row: each lane sums a contiguous run of texels in double,
+// the runs are combined by a block scan, then every lane rewrites its run as
+// running sums, stored as f32 and divided (as doubles) by the row total.
+__global__ void __launch_bounds__(kCdfThreads) cdf_rows(const uint16_t* sky, uint32_t W, uint32_t H, float exposure,
+                                                        float* cdf, double* row_tot) {
+    __shared__ double sh[kCdfThreads];
+    const uint32_t y = blockIdx.x;
+    const double thr = 65519.0;
+    const double sin_t = sin(3.141592653589793 * (double(y) + 0.5) / double(H));
+    const uint32_t per = (W + kCdfThreads - 1) / kCdfThreads;
+    const uint32_t x0 = threadIdx.x * per, x1 = min(W, x0 + per);
+    auto lum_at = [&](uint32_t x) {
+        const uint16_t* t = sky + (size_t(y) * W + x) * 4;
+        const double r = fmin(half_bits_to_double(t[0]) * double(exposure), thr);
+        const double g = fmin(half_bits_to_double(t[1]) * double(exposure), thr);
+        const double b = fmin(half_bits_to_double(t[2]) * double(exposure), thr);
+        return (0.2126 * r + 0.7152 * g + 0.0722 * b) * sin_t;
+    };
+    double local = 0.0;
+    for (uint32_t x = x0; x < x1; ++x) local += lum_at(x);
+    double total;
+    double run = block_exclusive_scan(local, sh, &total);
+    float* row = cdf + (H + 1) + size_t(y) * (W + 1);
+    if (threadIdx.x == 0) row[0] = 0.0f;
+    for (uint32_t x = x0; x < x1; ++x) {
+        run += lum_at(x);
+        row[x + 1] = total > 0.0 ? float(double(float(run)) / total) : float(double(x + 1) / double(W));
+    }
+    if (threadIdx.x == 0) row_tot[y] = total;
+}
+
+__global__ void __launch_bounds__(kCdfThreads) cdf_marginal(uint32_t H, float* cdf, const double* row_tot,
+                                                            double* total_out) {
+    __shared__ double sh[kCdfThreads];
+    const uint32_t per = (H + kCdfThreads - 1) / kCdfThreads;
+    const uint32_t y0 = threadIdx.x * per, y1 = min(H, y0 + per);
+    double local = 0.0;
+    for (uint32_t y = y0; y < y1; ++y) local += row_tot[y];
+    double total;
+    double run = block_exclusive_scan(local, sh, &total);
+    if (threadIdx.x == 0) { cdf[0] = 0.0f; *total_out = total; }
+    for (uint32_t y = y0; y < y1; ++y) {
+        run += row_tot[y];
+        cdf[y + 1] = total > 0.0 ? float(double(float(run)) / total) : float(double(y + 1) / double(H));
+    }
+}
+
+__global__ void zero_u64(unsigned long long* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// ---------------------------------------------------------------- launchers --
+static size_t stack_bytes(const DevScene& S, uint32_t threads) { return size_t(S.stack_cap) * threads * 4u; }
+
+template <uint32_t F>
+static hipError_t launch_trace_f(const DevScene& S, const TraceArgs& a, bool cn, hipStream_t s) {
+    const uint32_t tiles = ((a.sec_w + kTile - 1) / kTile) * ((a.sec_h + kTile - 1) / kTile);
+    if (tiles == 0) return hipSuccess;
+    const size_t lds = stack_bytes(S, kWave);
+    if (a.hits) {
+        hipLaunchKernelGGL((trace_kernel<F, false, true>), dim3(tiles), dim3(kWave), lds, s, S, a);
+    } else if (cn) {
+        hipLaunchKernelGGL((trace_kernel<F, true, false>), dim3(tiles), dim3(kWave), lds, s, S, a);
+    } else {
+        hipLaunchKernelGGL((trace_kernel<F, false, false>), dim3(tiles), dim3(kWave), lds, s, S, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_trace(const DevScene& S, const TraceArgs& a, bool count_nodes, hipStream_t s) {
+    if (!a.hits && S.flags == kFlagsC1) return launch_trace_f<kFlagsC1>(S, a, count_nodes, s);
+    if (!a.hits && S.flags == kFlagsC2) return launch_trace_f<kFlagsC2>(S, a, count_nodes, s);
+    return launch_trace_f<kDynamicFlags>(S, a, count_nodes, s);
+}
+
+hipError_t launch_gbuffer(const DevScene& S, const pt_scene_params& P, pt_gbuffer_texel* gbuf, uchar4* out,
+                          hipStream_t s) {
+    const uint32_t tiles = ((P.width + kTile - 1) / kTile) * ((P.height + kTile - 1) / kTile);
+    hipLaunchKernelGGL((gbuffer_kernel<kDynamicFlags>), dim3(tiles), dim3(kWave), stack_bytes(S, kWave), s, S, P,
+                       gbuf, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_rays(const DevScene& S, const RayArgs& a, bool shadow, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    const uint32_t blocks = (a.n + kWave - 1) / kWave;
+    hipLaunchKernelGGL((rays_kernel<kDynamicFlags>), dim3(blocks), dim3(kWave), stack_bytes(S, kWave), s, S, a,
+                       shadow ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_denoise(const DenoiseArgs& a, int stride_mode, hipStream_t s) {
+    const dim3 blk(16, 16), grd((a.u.width + 15) / 16, (a.u.height + 15) / 16);
+    hipLaunchKernelGGL(dn_guide, grd, blk, 0, s, a);
+    hipLaunchKernelGGL(dn_prepass, grd, blk, 0, s, a);
+    for (uint32_t i = 0; i < a.u.passes; ++i) {
+        const int step = stride_mode == 0 ? int(1u << i) : int(1u << (a.u.passes - 1));
+        const float4* in = (i % 2 == 0) ? a.a : a.b;
+        float4* outb = (i % 2 == 0) ? a.b : a.a;
+        hipLaunchKernelGGL(dn_atrous, grd, blk, 0, s, a, in, outb, step);
+    }
+    hipLaunchKernelGGL(dn_composite, grd, blk, 0, s, a, (a.u.passes % 2 == 0) ? a.a : a.b);
+    return hipGetLastError();
+}
+
+hipError_t launch_sky_cdf(const uint16_t* sky, uint32_t w, uint32_t h, float exposure, float* cdf, double* row_tmp,
+                          double* total, hipStream_t s) {
+    hipLaunchKernelGGL(cdf_rows, dim3(h), dim3(kCdfThreads), 0, s, sky, w, h, exposure, cdf, row_tmp);
+    hipLaunchKernelGGL(cdf_marginal, dim3(1), dim3(kCdfThreads), 0, s, h, cdf, row_tmp, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(zero_u64, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
+    return hipGetLastError();
+}
+
+}  // namespace pt

@@ -1,0 +1,192 @@
+"""Renderer / Denoiser: the tracer.js host classes over the libpt C ABI.
+
+Method names and argument meanings follow tracer.js's Renderer
+(tracer.js:1718-2417) and Denoiser (tracer.js:2418-2638); every call goes
+through libpt.so, which fails loudly when the HIP library or a GPU is absent.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+
+
+class Renderer:
+    """new Renderer(canvas) + init() (tracer.js:1719-1770)."""
+
+    def __init__(self, device=0):
+        self.lib = A.load_pt()
+        self.ctx = C.c_void_p()
+        rc = self.lib.pt_create(C.byref(self.ctx), device)
+        if rc:
+            raise A.PTError(rc, "pt_create(device=%d) failed: no usable HIP device" % device)
+        self.scene = None
+        self.frame = 0
+        self.width = self.height = 0
+        self.isgbuf = False
+
+    def _check(self, rc):
+        if rc:
+            raise A.PTError(rc, (self.lib.pt_last_error(self.ctx) or b"").decode())
+        return rc
+
+    def close(self):
+        if self.ctx:
+            self.lib.pt_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_option(self, key, value):
+        return self._check(self.lib.pt_set_option(self.ctx, key, value))
+
+    # -- setScene / updateObjects / resize (tracer.js:1955-2205) --
+    def set_scene(self, scene, isgbuf=False):
+        """scene: a finalized SceneBuilder, or an A.SceneDesc."""
+        desc = scene.desc() if hasattr(scene, "desc") else scene
+        self._check(self.lib.pt_upload_scene(self.ctx, C.byref(desc)))
+        self.scene = scene
+        self.isgbuf = isgbuf
+        if self.width:
+            self.resize(self.width, self.height)
+        return self
+
+    def update_objects(self, scene):
+        """Fast path; falls back to a full set_scene on capacity / flag changes,
+        as Renderer.updateObjects does (tracer.js:2140-2163)."""
+        desc = scene.desc() if hasattr(scene, "desc") else scene
+        rc = self.lib.pt_update_objects(self.ctx, C.byref(desc))
+        if rc in (A.PT_ERR_CAPACITY, A.PT_ERR_FLAGS):
+            return self.set_scene(scene, self.isgbuf)
+        self._check(rc)
+        self.scene = scene
+        return self
+
+    def resize(self, width, height):
+        self._check(self.lib.pt_resize(self.ctx, width, height, int(self.isgbuf)))
+        self.width, self.height = width, height
+        self.reset()
+        return self
+
+    def reset(self):
+        self.frame = 0
+
+    # -- render / renderSection (tracer.js:2242-2309) --
+    def params(self, sample_number=None, seed=0, section=(0, 0)):
+        p = self.scene.params(self.width, self.height, self.frame if sample_number is None else sample_number, seed)
+        p.section[0], p.section[1] = section
+        return p
+
+    def render_section(self, params, section_size=(0, 0), passes=1, seeds=None):
+        st = A.Stats()
+        sp = None
+        if seeds is not None:
+            s = np.ascontiguousarray(seeds, np.uint32)
+            assert s.size == passes
+            sp = s.ctypes.data_as(C.POINTER(A.u32))
+        self._check(self.lib.pt_trace(self.ctx, C.byref(params), section_size[0], section_size[1], passes, sp,
+                                      C.byref(st)))
+        return st
+
+    def render(self, spp=1, seeds=None, base_seed=0):
+        """spp progressive samples over the whole frame, starting at self.frame."""
+        if seeds is None:
+            seeds = [(base_seed * 0x9E3779B1 + (self.frame + k) * 0x85EBCA77) & 0xFFFFFFFF for k in range(spp)]
+        p = self.params(seed=seeds[0])
+        st = self.render_section(p, passes=spp, seeds=seeds)
+        self.frame += spp
+        return st
+
+    def gbuffer(self, params=None):
+        self._check(self.lib.pt_gbuffer(self.ctx, C.byref(params or self.params(sample_number=0))))
+
+    def trace_hits(self, params, n_bounces=2):
+        out = (A.Hit * (self.width * self.height * n_bounces))()
+        self._check(self.lib.pt_trace_hits(self.ctx, C.byref(params), n_bounces, out))
+        return out
+
+    # -- readback --
+    def read_accum(self):
+        a = np.empty((self.height, self.width, 4), np.float32)
+        self._check(self.lib.pt_read_accum(self.ctx, a.ctypes.data, a.nbytes))
+        return a
+
+    def write_accum(self, a):
+        a = np.ascontiguousarray(a, np.float32)
+        self._check(self.lib.pt_write_accum(self.ctx, a.ctypes.data, a.nbytes))
+
+    def read_output(self):
+        a = np.empty((self.height, self.width, 4), np.uint8)
+        self._check(self.lib.pt_read_output(self.ctx, a.ctypes.data, a.nbytes))
+        return a
+
+    def read_gbuffer(self):
+        a = np.empty((self.height, self.width, 12), np.float32)
+        self._check(self.lib.pt_read_gbuffer(self.ctx, a.ctypes.data, a.nbytes))
+        return a
+
+    def write_gbuffer(self, g):
+        g = np.ascontiguousarray(g, np.float32)
+        self._check(self.lib.pt_write_gbuffer(self.ctx, g.ctypes.data, g.nbytes))
+
+    def read_sky_cdf(self, count):
+        a = np.empty(count, np.float32)
+        tot = A.f32()
+        self._check(self.lib.pt_read_sky_cdf(self.ctx, a.ctypes.data, count, C.byref(tot)))
+        return a, tot.value
+
+    # -- ray queries --
+    def trace_rays(self, rays):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        out = (A.Hit * r.shape[0])()
+        self._check(self.lib.pt_trace_rays(self.ctx, r.ctypes.data, r.shape[0], out))
+        return out
+
+    def trace_shadow_rays(self, rays, target_idx, target_dist):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        t = np.ascontiguousarray(target_idx, np.int32)
+        d = np.ascontiguousarray(target_dist, np.float32)
+        out = np.empty((r.shape[0], 3), np.float32)
+        self._check(self.lib.pt_trace_shadow_rays(self.ctx, r.ctypes.data, t.ctypes.data, d.ctypes.data,
+                                                  r.shape[0], out.ctypes.data))
+        return out
+
+
+class Denoiser:
+    """Denoiser (tracer.js:2418-2637) with the reference's default sigmas."""
+
+    def __init__(self, renderer):
+        self.r = renderer
+        self.passes = 4
+        self.sigma_color = 0.30
+        self.sigma_normal = 128.0
+        self.sigma_depth = 0.2
+        self.sigma_albedo = 0.07
+        self.sigma_material = 0.07
+        self.glossy_protect = 0.75
+        self.exposure = 1.0
+        self.use_median = True
+        self.use_demodulate = True
+
+    def uniforms(self):
+        u = A.DenoiseParams()
+        u.width, u.height, u.step_width, u.passes = self.r.width, self.r.height, 1, self.passes
+        u.sigma_color, u.sigma_normal, u.sigma_depth = self.sigma_color, self.sigma_normal, self.sigma_depth
+        u.sigma_albedo, u.sigma_material, u.glossy_protect = self.sigma_albedo, self.sigma_material, self.glossy_protect
+        u.exposure = self.exposure
+        u.flags = (1 if self.use_median else 0) | (2 if self.use_demodulate else 0)
+        return u
+
+    def execute(self, stride_mode=0):
+        """stride_mode 0: steps 1,2,4,...; 1: the reference's constant last step."""
+        self.r._check(self.r.lib.pt_denoise(self.r.ctx, C.byref(self.uniforms()), stride_mode))
+        return self.r.read_output()
+
+    def read_result(self):
+        a = np.empty((self.r.height, self.r.width, 4), np.float32)
+        self.r._check(self.r.lib.pt_read_denoised(self.r.ctx, a.ctypes.data, a.nbytes))
+        return a
