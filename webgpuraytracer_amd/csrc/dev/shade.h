@@ -208,19 +208,37 @@ __device__ __forceinline__ float mis_weight(float pa, float pb) {
 
 // ---------------------------------------------------------------- media ----
 struct Medium { float ior; v3 sigma, emission; };
+
+// MediumStack (shader.wgsl:1661-1697): up to 4 media of (ior, sigma,
+// emission) per lane, kept in LDS and interleaved across the block's lanes
+// (slot s, field f of lane l at ((s*7+f)*blockDim + l)), so the rarely
+// touched stack costs no registers and no scratch traffic.
+constexpr int kMediaFloats = 4 * 7;
 struct Media {
-    Medium m[4];
+    float* base;
+    uint32_t stride;
     int n;
-    __device__ Medium top() const {
-        if (n > 0) return m[n - 1];
+    __device__ __forceinline__ Medium at(int s) const {
+        const float* p = base + uint32_t(s) * 7u * stride;
+        return Medium{p[0], mk(p[stride], p[2 * stride], p[3 * stride]),
+                      mk(p[4 * stride], p[5 * stride], p[6 * stride])};
+    }
+    __device__ __forceinline__ Medium top() const {
+        if (n > 0) return at(n - 1);
         return Medium{1.0f, splat(0.f), splat(0.f)};
     }
-    __device__ Medium outer() const {
-        if (n > 1) return m[n - 2];
-        return Medium{1.0f, splat(0.f), splat(0.f)};
+    __device__ __forceinline__ float top_ior() const { return n > 0 ? base[uint32_t(n - 1) * 7u * stride] : 1.0f; }
+    __device__ __forceinline__ float outer_ior() const { return n > 1 ? base[uint32_t(n - 2) * 7u * stride] : 1.0f; }
+    __device__ __forceinline__ void push(const Medium& x) {
+        if (n < 4) {
+            float* p = base + uint32_t(n) * 7u * stride;
+            p[0] = x.ior;
+            p[stride] = x.sigma.x; p[2 * stride] = x.sigma.y; p[3 * stride] = x.sigma.z;
+            p[4 * stride] = x.emission.x; p[5 * stride] = x.emission.y; p[6 * stride] = x.emission.z;
+            n += 1;
+        }
     }
-    __device__ void push(const Medium& x) { if (n < 4) { m[n] = x; n += 1; } }
-    __device__ void pop() { if (n > 0) n -= 1; }
+    __device__ __forceinline__ void pop() { if (n > 0) n -= 1; }
 };
 
 // ------------------------------------------------------------------ BSDF ---
@@ -287,8 +305,8 @@ __device__ v3 vndf_sample(float x0, float x1, v3 V, v3 N, float ag) {
 }
 
 __device__ __forceinline__ void iors(bool entering, const pt_material& m, const Media& md, float& etai, float& etat) {
-    etai = md.top().ior;
-    etat = entering ? m.ior : md.outer().ior;
+    etai = md.top_ior();
+    etat = entering ? m.ior : md.outer_ior();
 }
 
 // eval_surface, shader.wgsl:1831-1947

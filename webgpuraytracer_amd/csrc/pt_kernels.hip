@@ -101,8 +101,8 @@ __device__ v3 nee(const DevScene& S, Flags<F> fl, const LightSample& ls, v3 hp, 
 // n_rec trace_scene hits of the path are written there.
 template <uint32_t F, bool CN, bool REC>
 __device__ v3 trace_path(const DevScene& S, Flags<F> fl, const pt_scene_params& PP, const Params& P, uint32_t px,
-                         uint32_t py, uint32_t sample_number, uint32_t seed, LdsStack& st, Counters& c, pt_hit* rec,
-                         uint32_t n_rec) {
+                         uint32_t py, uint32_t sample_number, uint32_t seed, LdsStack& st, float* media_lds,
+                         Counters& c, pt_hit* rec, uint32_t n_rec) {
     const uint32_t idx = py * PP.width + px;
     Pcg rng{(idx * 1973u + sample_number * 9277u + seed * 26699u) | 1u};
     rng.next();
@@ -112,8 +112,7 @@ __device__ v3 trace_path(const DevScene& S, Flags<F> fl, const pt_scene_params& 
     v3 ro = ld3(PP.eye);
     v3 tp = splat(1.0f), rad = splat(0.0f);
     float last_pdf = 1.0f, last_wsum = 0.0f;
-    Media md;
-    md.n = 0;
+    Media md{media_lds, blockDim.x, 0};
     c.paths++;
     for (int bounce = 0; bounce < S.bounce_limit; ++bounce) {
         HitRec h;
@@ -245,6 +244,7 @@ __global__ void __launch_bounds__(kWave) trace_kernel(DevScene S, TraceArgs A) {
     c.clear();
     if (active) {
         LdsStack st{lds_stack + lane, blockDim.x, S.stack_cap, 0};
+        float* media = reinterpret_cast<float*>(lds_stack + S.stack_cap * blockDim.x) + lane;
         Params P{A.P.sky_total_lum, A.P.sky_width, A.P.sky_height, A.P.sky_cdf_index};
         const uint32_t idx = py * A.P.width + px;
         const float4 old = A.accum[idx];
@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(kWave) trace_kernel(DevScene S, TraceArgs A) {
             const uint32_t sn = A.P.sample_number + pass;
             const uint32_t seed = A.seeds ? A.seeds[pass] : A.P.seed;
             pt_hit* rec = REC ? A.hits + size_t(idx) * A.n_rec : nullptr;
-            const v3 r = trace_path<F, CN, REC>(S, fl, A.P, P, px, py, sn, seed, st, c, rec, A.n_rec);
+            const v3 r = trace_path<F, CN, REC>(S, fl, A.P, P, px, py, sn, seed, st, media, c, rec, A.n_rec);
             const float w = 1.0f / float(sn + 1u);
             acc = lerp(acc, r, w);
         }
@@ -582,7 +582,7 @@ template <uint32_t F>
 static hipError_t launch_trace_f(const DevScene& S, const TraceArgs& a, bool cn, hipStream_t s) {
     const uint32_t tiles = ((a.sec_w + kTile - 1) / kTile) * ((a.sec_h + kTile - 1) / kTile);
     if (tiles == 0) return hipSuccess;
-    const size_t lds = stack_bytes(S, kWave);
+    const size_t lds = stack_bytes(S, kWave) + size_t(kMediaFloats) * kWave * 4u;
     if (a.hits) {
         hipLaunchKernelGGL((trace_kernel<F, false, true>), dim3(tiles), dim3(kWave), lds, s, S, a);
     } else if (cn) {
