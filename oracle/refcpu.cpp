@@ -1868,6 +1868,67 @@ int ref_denoise(const pt_denoise_params* u, int stride_mode, const float* accum,
     return PT_OK;
 }
 
+static SurfaceContext bsdf_ctx(const pt_material& m, const float n[3]) {
+    SurfaceHit h = empty_hit();
+    h.hit_n = V(n[0], n[1], n[2]);
+    return get_surface_context(h, m);
+}
+
+int ref_bsdf_sample(const pt_material* m, const float n[3], const float v[3], uint32_t n_samples, uint32_t seed,
+                    float* out) {
+    if (!m || !n || !v || !out) return PT_ERR_ARG;
+    const SurfaceContext ctx = bsdf_ctx(*m, n);
+    for (uint32_t i = 0; i < n_samples; ++i) {
+        Rng rng{(seed + i * 7919u) | 1u};
+        Ray ray{V(v[0], v[1], v[2]), -V(v[0], v[1], v[2])};
+        f3 tp = V(1.0f);
+        MediumStack stack;
+        float last_pdf = 1.0f;
+        const bool ok = sample_surface(&ray, &tp, *m, ctx, V(0.f), &stack, &last_pdf, rng);
+        float* o = out + size_t(i) * 8;
+        o[0] = ray.direction.x; o[1] = ray.direction.y; o[2] = ray.direction.z;
+        o[3] = tp.x; o[4] = tp.y; o[5] = tp.z;
+        o[6] = last_pdf;
+        o[7] = ok ? 1.0f : 0.0f;
+    }
+    return PT_OK;
+}
+
+int ref_bsdf_eval(const pt_material* m, const float n[3], const float v[3], const float* L, uint32_t k, float* f,
+                  float* pdf) {
+    if (!m || !n || !v || !L || !f || !pdf) return PT_ERR_ARG;
+    const SurfaceContext ctx = bsdf_ctx(*m, n);
+    MediumStack stack;
+    const f3 Vv = V(v[0], v[1], v[2]);
+    for (uint32_t i = 0; i < k; ++i) {
+        const f3 l = V(L[3 * i], L[3 * i + 1], L[3 * i + 2]);
+        const f3 e = eval_surface(Vv, l, *m, ctx, stack);
+        f[3 * i] = e.x; f[3 * i + 1] = e.y; f[3 * i + 2] = e.z;
+        pdf[i] = pdf_surface(Vv, l, *m, ctx, stack);
+    }
+    return PT_OK;
+}
+
+int ref_env_sample(const ref_scene* s, const pt_scene_params* params, uint32_t n, uint32_t seed, float* out) {
+    if (!s || !params || !out) return PT_ERR_ARG;
+    for (uint32_t i = 0; i < n; ++i) {
+        Rng rng{(seed + i * 7919u) | 1u};
+        float ux = rng.next();
+        float uy = rng.next();
+        LightSample ls = sample_env_cdf(*s, *params, f2{ux, uy});
+        float* o = out + size_t(i) * 7;
+        o[0] = ls.dir.x; o[1] = ls.dir.y; o[2] = ls.dir.z; o[3] = ls.pdf;
+        o[4] = ls.color.x; o[5] = ls.color.y; o[6] = ls.color.z;
+    }
+    return PT_OK;
+}
+
+int ref_env_pdf(const ref_scene* s, const pt_scene_params* params, const float* dirs, uint32_t k, float* pdf) {
+    if (!s || !params || !dirs || !pdf) return PT_ERR_ARG;
+    for (uint32_t i = 0; i < k; ++i) pdf[i] = get_sky_pdf(*s, *params, V(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+    return PT_OK;
+}
+
 // tracer.js:135-204
 int ref_build_sky_cdf(const uint16_t* data, uint32_t W, uint32_t H, float exposure, float* cdf, double* total_weight) {
     if (!data || !cdf || !W || !H) return PT_ERR_ARG;

@@ -72,6 +72,23 @@ int ref_denoise(const pt_denoise_params* params, int stride_mode, const float* a
 int ref_build_sky_cdf(const uint16_t* rgba16f, uint32_t width, uint32_t height, float exposure,
                       float* cdf, double* total_weight);
 
+/* sample_surface (shader.wgsl:2052-2175) n times at a surface point at the
+ * origin with shading normal n, viewer direction v (pointing away from the
+ * surface), material m, outside any medium.  Sample i draws from PCG state
+ * (seed + i*7919)|1.  out[8*i..]: L.xyz, weight.xyz (f/pdf), pdf, status
+ * (1: continued along L, 0: the path ended). */
+int ref_bsdf_sample(const pt_material* m, const float n[3], const float v[3], uint32_t n_samples, uint32_t seed,
+                    float* out);
+/* eval_surface / pdf_surface for k directions L (3 floats each). */
+int ref_bsdf_eval(const pt_material* m, const float n[3], const float v[3], const float* L, uint32_t k, float* f,
+                  float* pdf);
+
+/* sample_env_cdf (shader.wgsl:231-254) n times with uniforms (u.x, u.y) from
+ * PCG state (seed + i*7919)|1: out[7*i..] = dir.xyz, pdf, color.rgb.  And
+ * get_sky_pdf (shader.wgsl:208-229) for k directions. */
+int ref_env_sample(const ref_scene* s, const pt_scene_params* params, uint32_t n, uint32_t seed, float* out);
+int ref_env_pdf(const ref_scene* s, const pt_scene_params* params, const float* dirs, uint32_t k, float* pdf);
+
 /* rays: 6 doubles; fills *hit with the double-precision closest hit (t in
  * hit->t rounded to float, exact value in *t_exact). */
 int ref64_closest(const ref_scene* s, const double* ray, pt_hit* hit, double* t_exact);

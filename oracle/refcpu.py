@@ -24,6 +24,10 @@ FUNCS = {
     "ref_denoise": (C.c_int, [P(A.DenoiseParams), C.c_int, vp, vp, vp, vp]),
     "ref_build_sky_cdf": (C.c_int, [vp, A.u32, A.u32, A.f32, vp, P(C.c_double)]),
     "ref64_closest": (C.c_int, [vp, P(C.c_double), P(A.Hit), P(C.c_double)]),
+    "ref_env_sample": (C.c_int, [vp, P(A.SceneParams), A.u32, A.u32, vp]),
+    "ref_env_pdf": (C.c_int, [vp, P(A.SceneParams), vp, A.u32, vp]),
+    "ref_bsdf_sample": (C.c_int, [P(A.Material), P(A.f32), P(A.f32), A.u32, A.u32, vp]),
+    "ref_bsdf_eval": (C.c_int, [P(A.Material), P(A.f32), P(A.f32), vp, A.u32, vp, vp]),
 }
 _lib = None
 
@@ -100,6 +104,17 @@ class RefScene:
         _ok(self.L.ref_gbuffer(self.h, C.byref(params), g.ctypes.data, o.ctypes.data, threads), "ref_gbuffer")
         return g, o
 
+    def env_sample(self, params, n, seed=1):
+        out = np.zeros((n, 7), np.float32)
+        _ok(self.L.ref_env_sample(self.h, C.byref(params), n, seed, out.ctypes.data), "ref_env_sample")
+        return out
+
+    def env_pdf(self, params, dirs):
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros(d.shape[0], np.float32)
+        _ok(self.L.ref_env_pdf(self.h, C.byref(params), d.ctypes.data, d.shape[0], out.ctypes.data), "ref_env_pdf")
+        return out
+
     def closest64(self, ray):
         r = (C.c_double * 6)(*[float(x) for x in ray])
         hit = A.Hit()
@@ -125,3 +140,19 @@ def build_sky_cdf(rgba16f, exposure=1.0):
     tot = C.c_double()
     _ok(lib().ref_build_sky_cdf(t.ctypes.data, w, h, exposure, cdf.ctypes.data, C.byref(tot)), "ref_build_sky_cdf")
     return cdf, tot.value
+
+
+def bsdf_sample(material, n, v, count, seed=1):
+    out = np.zeros((count, 8), np.float32)
+    _ok(lib().ref_bsdf_sample(C.byref(material), A.fvec(*n), A.fvec(*v), count, seed, out.ctypes.data),
+        "ref_bsdf_sample")
+    return out
+
+
+def bsdf_eval(material, n, v, L):
+    L = np.ascontiguousarray(L, np.float32).reshape(-1, 3)
+    f = np.zeros((L.shape[0], 3), np.float32)
+    pdf = np.zeros(L.shape[0], np.float32)
+    _ok(lib().ref_bsdf_eval(C.byref(material), A.fvec(*n), A.fvec(*v), L.ctypes.data, L.shape[0], f.ctypes.data,
+                            pdf.ctypes.data), "ref_bsdf_eval")
+    return f, pdf
