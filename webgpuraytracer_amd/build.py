@@ -7,6 +7,7 @@ Outputs (git-ignored, shipped with the tree):
     webgpuraytracer_amd/lib/libptscene.so  host scene builder (ptgen_* C ABI)
     webgpuraytracer_amd/js/build/pt_addon.node   Node-API addon over libpt
 """
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -62,12 +63,15 @@ def build_pt():
     hipcc = _hipcc()
     out = os.path.join(LIB, "libpt.so")
     with tempfile.TemporaryDirectory(prefix="ptbuild") as td:
-        k = os.path.join(td, "pt_kernels.o")
-        a = os.path.join(td, "pt_api.o")
-        _run([hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, "pt_kernels.hip"), "-o", k])
-        _run([hipcc] + HIP_FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, "pt_api.cpp"), "-o", a])
+        srcs = ("pt_kernels.hip", "pt_wavefront.hip", "pt_api.cpp")
+        objs = [os.path.join(td, src.split(".")[0] + ".o") for src in srcs]
+        with concurrent.futures.ThreadPoolExecutor(max_workers=len(srcs)) as ex:
+            jobs = [ex.submit(_run, [hipcc] + HIP_FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", o])
+                    for src, o in zip(srcs, objs)]
+            for j in jobs:
+                j.result()
         tmp = out + ".tmp"
-        _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", k, a, "-o", tmp])
+        _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC"] + objs + ["-o", tmp])
         os.replace(tmp, out)
     return out
 

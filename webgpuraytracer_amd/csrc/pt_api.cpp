@@ -62,6 +62,11 @@ struct pt_ctx {
     uint32_t width = 0, height = 0;
     bool with_gbuffer = false;
     DevBuf accum, out, gbuf, dn_a, dn_b, dn_fin, dn_guide, stats, seeds, rays_in, rays_aux, rays_aux2, rays_out;
+    // wavefront path pool
+    uint64_t wf_pool_opt = 1ull << 21;
+    uint32_t wf_pool = 0;
+    DevBuf wf_o, wf_d, wf_t, wf_r, wf_meta, wf_ha, wf_hb, wf_media, wf_q0, wf_q1, wf_sq, wf_so, wf_sd, wf_stp, wf_sfi,
+        wf_cnt;
 
     int fail(int code, const char* fmt, ...) {
         char buf[512];
@@ -100,7 +105,9 @@ struct pt_ctx {
     void release_all() {
         DevBuf* all[] = {&mats, &objs, &obj_root, &tnodes, &bnodes, &roots, &tlas_raw, &bvh_raw, &tri_geo, &tris,
                          &planes, &meshes, &mesh_root, &lights, &sky, &cdf, &cdf_rows, &accum, &out, &gbuf, &dn_a,
-                         &dn_b, &dn_fin, &dn_guide, &stats, &seeds, &rays_in, &rays_aux, &rays_aux2, &rays_out};
+                         &dn_b, &dn_fin, &dn_guide, &stats, &seeds, &rays_in, &rays_aux, &rays_aux2, &rays_out,
+                         &wf_o, &wf_d, &wf_t, &wf_r, &wf_meta, &wf_ha, &wf_hb, &wf_media, &wf_q0, &wf_q1, &wf_sq,
+                         &wf_so, &wf_sd, &wf_stp, &wf_sfi, &wf_cnt};
         for (DevBuf* b : all) b->release();
     }
 };
@@ -438,11 +445,15 @@ int pt_set_option(pt_ctx* c, int key, int64_t value) {
     if (!c) return PT_ERR_ARG;
     switch (key) {
         case PT_OPT_BACKEND:
-            if (value != PT_BACKEND_MEGAKERNEL) return c->fail(PT_ERR_UNSUPPORTED, "only the megakernel backend is built");
+            if (value != PT_BACKEND_MEGAKERNEL && value != PT_BACKEND_WAVEFRONT)
+                return c->fail(PT_ERR_ARG, "unknown backend %lld", (long long)value);
             c->backend = int(value);
             return PT_OK;
         case PT_OPT_COUNT_STATS: c->count_stats = value != 0; return PT_OK;
-        case PT_OPT_WAVEFRONT_POOL: return value > 0 ? PT_OK : c->fail(PT_ERR_ARG, "pool size must be > 0");
+        case PT_OPT_WAVEFRONT_POOL:
+            if (value < 1024 || value > (1ll << 28)) return c->fail(PT_ERR_ARG, "pool size must be in [1024, 2^28]");
+            c->wf_pool_opt = uint64_t(value);
+            return PT_OK;
         default: return c->fail(PT_ERR_ARG, "unknown option %d", key);
     }
 }
@@ -556,6 +567,68 @@ static int check_params(pt_ctx* c, const pt_scene_params* p) {
     return PT_OK;
 }
 
+// Wavefront backend: the call's passes in batches of whole passes, each
+// batch one raygen + bounce_limit x (extend, shade, shadow) + accumulate.
+static int run_wavefront(pt_ctx* c, const TraceArgs& a) {
+    const uint64_t npix = uint64_t(a.sec_w) * a.sec_h;
+    uint64_t per = std::max<uint64_t>(1, c->wf_pool_opt / std::max<uint64_t>(npix, 1));
+    per = std::min<uint64_t>(per, a.n_passes);
+    const uint64_t pool = per * npix;
+    if (pool > (1ull << 30)) return c->fail(PT_ERR_ARG, "wavefront pool too large");
+    const uint32_t P = uint32_t(pool);
+    int rc;
+    if (c->wf_pool < P) {
+        struct { DevBuf* b; size_t each; const char* what; } bufs[] = {
+            {&c->wf_o, 16, "wf origin"}, {&c->wf_d, 16, "wf direction"}, {&c->wf_t, 16, "wf throughput"},
+            {&c->wf_r, 16, "wf radiance"}, {&c->wf_meta, 4, "wf meta"}, {&c->wf_ha, 16, "wf hit"},
+            {&c->wf_hb, 8, "wf bary"}, {&c->wf_media, 28 * 4, "wf media"}, {&c->wf_q0, 4, "wf queue"},
+            {&c->wf_q1, 4, "wf queue"}, {&c->wf_sq, 4, "wf shadow queue"}, {&c->wf_so, 32, "wf shadow o"},
+            {&c->wf_sd, 32, "wf shadow d"}, {&c->wf_stp, 32, "wf shadow tp"}, {&c->wf_sfi, 32, "wf shadow L"}};
+        for (auto& b : bufs) {
+            b.b->release();
+            if ((rc = c->alloc(*b.b, size_t(P) * b.each, b.what))) { c->wf_pool = 0; return rc; }
+        }
+        if ((rc = c->alloc(c->wf_cnt, 64, "wf counters"))) return rc;
+        c->wf_pool = P;
+    }
+    WfArgs w{};
+    w.P = a.P;
+    w.sec_w = a.sec_w;
+    w.sec_h = a.sec_h;
+    w.pool = c->wf_pool;
+    w.seeds = a.seeds;
+    w.accum = a.accum;
+    w.out = a.out;
+    w.stats = a.stats;
+    w.o_pdf = static_cast<float4*>(c->wf_o.p);
+    w.d_wsum = static_cast<float4*>(c->wf_d.p);
+    w.tp_rng = static_cast<float4*>(c->wf_t.p);
+    w.rad = static_cast<float4*>(c->wf_r.p);
+    w.meta = static_cast<uint32_t*>(c->wf_meta.p);
+    w.hit_a = static_cast<int4*>(c->wf_ha.p);
+    w.hit_b = static_cast<float2*>(c->wf_hb.p);
+    w.media = static_cast<float*>(c->wf_media.p);
+    w.queue[0] = static_cast<uint32_t*>(c->wf_q0.p);
+    w.queue[1] = static_cast<uint32_t*>(c->wf_q1.p);
+    w.shadow_queue = static_cast<uint32_t*>(c->wf_sq.p);
+    w.sh_o = static_cast<float4*>(c->wf_so.p);
+    w.sh_d = static_cast<float4*>(c->wf_sd.p);
+    w.sh_tp = static_cast<float4*>(c->wf_stp.p);
+    w.sh_fi = static_cast<float4*>(c->wf_sfi.p);
+    w.counters = static_cast<uint32_t*>(c->wf_cnt.p);
+    hipError_t e = hipEventRecord(c->ev0, c->stream);
+    for (uint32_t first = 0; e == hipSuccess && first < a.n_passes; first += uint32_t(per)) {
+        w.first_pass = first;
+        w.batch_passes = uint32_t(std::min<uint64_t>(per, a.n_passes - first));
+        w.n_items = uint32_t(w.batch_passes * npix);
+        w.write_output = first + w.batch_passes >= a.n_passes ? 1 : 0;
+        e = launch_wavefront_batch(c->dev, w, c->count_stats, c->stream);
+    }
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "wavefront launch");
+    return PT_OK;
+}
+
 static int run_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t sh, uint32_t n_passes,
                      const uint32_t* seeds, pt_stats* stats, pt_hit* hits_dev, uint32_t n_rec) {
     int rc;
@@ -579,10 +652,14 @@ static int run_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t 
     a.stats = static_cast<unsigned long long*>(c->stats.p);
     a.hits = hits_dev;
     a.n_rec = n_rec;
-    e = hipEventRecord(c->ev0, c->stream);
-    if (e == hipSuccess) e = launch_trace(c->dev, a, c->count_stats && !hits_dev, c->stream);
-    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
-    if (e != hipSuccess) return c->hip_fail(e, "trace launch");
+    if (c->backend == PT_BACKEND_WAVEFRONT && !hits_dev) {
+        if ((rc = run_wavefront(c, a))) return rc;
+    } else {
+        e = hipEventRecord(c->ev0, c->stream);
+        if (e == hipSuccess) e = launch_trace(c->dev, a, c->count_stats && !hits_dev, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+        if (e != hipSuccess) return c->hip_fail(e, "trace launch");
+    }
     if ((rc = c->sync("trace"))) return rc;
     if (stats) {
         unsigned long long v[kStCount];

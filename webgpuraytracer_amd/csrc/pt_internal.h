@@ -42,6 +42,36 @@ struct DenoiseArgs {
     uchar4* out;
 };
 
+// Wavefront batch (pt_wavefront.hip): device state of `pool` path slots;
+// slot i of a batch is pass first_pass + i / npix of section pixel i % npix.
+struct WfArgs {
+    pt_scene_params P;
+    uint32_t sec_w, sec_h;
+    uint32_t n_items;
+    uint32_t pool;
+    uint32_t first_pass, batch_passes;
+    int write_output;
+    const uint32_t* seeds;
+    float4* accum;
+    uchar4* out;
+    unsigned long long* stats;
+    float4* o_pdf;      // ray origin, last BSDF pdf
+    float4* d_wsum;     // ray direction, last light weight sum
+    float4* tp_rng;     // throughput, PCG state bits
+    float4* rad;        // radiance
+    uint32_t* meta;     // medium stack depth
+    int4* hit_a;        // t bits, kind<<30 | index, triangle, node pops
+    float2* hit_b;      // barycentrics
+    float* media;       // 28 floats per slot, structure of arrays with stride pool
+    uint32_t* queue[2];
+    uint32_t* shadow_queue;
+    float4* sh_o;       // per slot x2: origin, distance
+    float4* sh_d;       //              direction, ignored object bits
+    float4* sh_tp;      //              throughput, valid flag
+    float4* sh_fi;      //              clamped unoccluded radiance
+    uint32_t* counters; // queue A length, queue B length, shadow queue length
+};
+
 // Launchers (pt_kernels.hip).  All asynchronous on `stream`.
 hipError_t launch_trace(const DevScene& S, const TraceArgs& a, bool count_nodes, hipStream_t stream);
 hipError_t launch_gbuffer(const DevScene& S, const pt_scene_params& P, pt_gbuffer_texel* gbuf, uchar4* out,
@@ -51,5 +81,6 @@ hipError_t launch_denoise(const DenoiseArgs& a, int stride_mode, hipStream_t str
 hipError_t launch_sky_cdf(const uint16_t* sky, uint32_t w, uint32_t h, float exposure, float* cdf, double* row_tmp,
                           double* total, hipStream_t stream);
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t stream);
+hipError_t launch_wavefront_batch(const DevScene& S, const WfArgs& a, bool count_nodes, hipStream_t stream);
 
 }  // namespace pt

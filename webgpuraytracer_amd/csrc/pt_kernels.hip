@@ -15,7 +15,7 @@
 // XCDs), which keeps neighbouring tiles — and the BVH nodes they touch — in
 // one XCD's L2.
 #include "pt_internal.h"
-#include "dev/shade.h"
+#include "dev/path.h"
 
 namespace pt {
 
@@ -60,11 +60,6 @@ __device__ void flush_counters(const Counters& c, unsigned long long* stats, boo
     }
 }
 
-__device__ __forceinline__ v3 camera_dir(const pt_scene_params& P, float su, float sv) {
-    const v3 r00 = ld3(P.ray00), r10 = ld3(P.ray10), r01 = ld3(P.ray01), r11 = ld3(P.ray11);
-    return normalize(lerp(lerp(r00, r10, su), lerp(r01, r11, su), 1.0f - sv));
-}
-
 __device__ void store_hit(pt_hit* o, const HitRec& h, const SurfHit& s) {
     pt_hit r;
     r.t = h.t;
@@ -79,23 +74,24 @@ __device__ void store_hit(pt_hit* o, const HitRec& h, const SurfHit& s) {
     *o = r;
 }
 
-// evaluate_nee (shader.wgsl:2226-2271) without heightmaps.
+// Megakernel NEE sink: traces the shadow ray at once and adds the
+// contribution, exactly where evaluate_nee's result is added
+// (shader.wgsl:2440, 2452).
 template <uint32_t F, bool CN>
-__device__ v3 nee(const DevScene& S, Flags<F> fl, const LightSample& ls, v3 hp, v3 no, v3 V, const pt_material& m,
-                  const Surface& sc, const Media& md, int ignore, LdsStack& st, Counters& c) {
-    if (ls.pdf <= 0.0f) return splat(0.0f);
-    const float nl = dot(no, ls.dir);
-    if (nl <= 0.0f && m.transmission <= 0.0f) return splat(0.0f);
-    const float bp = pdf_bsdf(V, ls.dir, m, sc, md);
-    const float w = mis_weight(ls.pdf, bp);
-    const v3 f = eval_bsdf(V, ls.dir, m, sc, md);
-    const v3 cw = w * f / mx(ls.pdf, 1e-6f);
-    if (hmax(ls.color * cw) < 1e-5f) return splat(0.0f);
-    const v3 off = nl < 0.0f ? -no * 0.001f : no * 0.001f;
-    c.shadow++;
-    if (occluded<F, CN>(S, fl, hp + off, ls.dir, ignore, ls.dist, st, c)) return splat(0.0f);
-    return ls.color * cw;
-}
+struct ImmediateNee {
+    const DevScene& S;
+    Flags<F> fl;
+    LdsStack& st;
+    Counters& c;
+    __device__ void operator()(int /*k*/, bool need_ray, v3 o, v3 d, float dist, int ignore, v3 inc, v3 tp, int bounce,
+                               v3& rad) {
+        if (need_ray) {
+            c.shadow++;
+            if (occluded<F, CN>(S, fl, o, d, ignore, dist, st, c)) inc = splat(0.0f);
+        }
+        rad = rad + tp * firefly(inc, bounce);
+    }
+};
 
 // One path of shader.wgsl `main` (2279-2480).  rec: when non-null, the first
 // n_rec trace_scene hits of the path are written there.
@@ -103,129 +99,20 @@ template <uint32_t F, bool CN, bool REC>
 __device__ v3 trace_path(const DevScene& S, Flags<F> fl, const pt_scene_params& PP, const Params& P, uint32_t px,
                          uint32_t py, uint32_t sample_number, uint32_t seed, LdsStack& st, float* media_lds,
                          Counters& c, pt_hit* rec, uint32_t n_rec) {
-    const uint32_t idx = py * PP.width + px;
-    Pcg rng{(idx * 1973u + sample_number * 9277u + seed * 26699u) | 1u};
-    rng.next();
-    const float jx = rng.next();
-    const float jy = rng.next();
-    v3 rd = camera_dir(PP, (float(px) + jx) / float(PP.width), (float(py) + jy) / float(PP.height));
-    v3 ro = ld3(PP.eye);
-    v3 tp = splat(1.0f), rad = splat(0.0f);
-    float last_pdf = 1.0f, last_wsum = 0.0f;
+    PathState ps;
+    path_begin(PP, px, py, sample_number, seed, ps);
     Media md{media_lds, blockDim.x, 0};
+    ImmediateNee<F, CN> sink{S, fl, st, c};
     c.paths++;
     for (int bounce = 0; bounce < S.bounce_limit; ++bounce) {
         HitRec h;
         c.ext++;
-        closest_hit<F, CN>(S, fl, ro, rd, st, h, c);
-        const SurfHit sh = resolve_hit<REC>(S, ro, rd, h);
+        closest_hit<F, CN>(S, fl, ps.ro, ps.rd, st, h, c);
+        const SurfHit sh = resolve_hit<REC>(S, ps.ro, ps.rd, h);
         if (REC && uint32_t(bounce) < n_rec) store_hit(rec + bounce, h, sh);
-        // Beer-Lambert attenuation and medium emission (shader.wgsl:2325-2342)
-        const Medium cm = md.top();
-        if (cm.ior > 1.0f || len(cm.sigma) > 0.0f) {
-            const float dist = sh.m_idx == -1 ? 1000.0f : h.t;
-            const v3 att = mk(expf(-cm.sigma.x * dist), expf(-cm.sigma.y * dist), expf(-cm.sigma.z * dist));
-            const float ex = cm.sigma.x > 1e-4f ? cm.emission.x * (1.0f - att.x) / cm.sigma.x : cm.emission.x * dist;
-            const float ey = cm.sigma.y > 1e-4f ? cm.emission.y * (1.0f - att.y) / cm.sigma.y : cm.emission.y * dist;
-            const float ez = cm.sigma.z > 1e-4f ? cm.emission.z * (1.0f - att.z) / cm.sigma.z : cm.emission.z * dist;
-            rad = rad + tp * firefly(mk(ex, ey, ez), bounce);
-            tp = tp * att;
-        }
-        if (sh.m_idx == -1) {  // miss (shader.wgsl:2344-2358)
-            if (fl.has(PT_HAS_SKYBOX)) {
-                const v3 sky = sample_sky(S, rd);
-                float w = 1.0f;
-                if (bounce > 0 && fl.has(PT_MIS_SKYBOX)) w = mis_weight(last_pdf, sky_pdf(S, P, rd));
-                rad = rad + tp * firefly(sky * w, bounce);
-            } else {
-                rad = rad + tp * firefly(mk(0.02f, 0.03f, 0.05f), bounce);
-            }
-            break;
-        }
-        if (CN) c.shade++;
-        const pt_material& m = S.mats[sh.m_idx];
-        Surface sc;
-        sc.normal = sh.n;
-        sc.snormal = sh.n;
-        sc.albedo = ld3(m.color);
-        sc.alpha = 1.0f;
-        sc.metallic = m.metallic;
-        sc.roughness = m.roughness;
-        sc.emittance = ld3(m.emittance);
-        // implicit light hit (shader.wgsl:2377-2395)
-        if (len(sc.emittance) > 0.0f) {
-            float w = 1.0f;
-            if (fl.has(PT_HAS_LIGHTS) && bounce > 0 && sh.o_idx >= 0) {
-                const pt_object& ob = S.objs[sh.o_idx];
-                if (ob.light_idx >= 0) {
-                    const pt_light& L = S.lights[ob.light_idx];
-                    float raw = 0.0f;
-                    if (fl.has(PT_HAS_SPHERES) && ob.object_type == PT_OBJ_SPHERE) raw = sphere_light_pdf(L, ob, ro, rd);
-                    else if (fl.has(PT_HAS_CUBES) && ob.object_type == PT_OBJ_CUBE) raw = box_light_pdf(L, ro, rd, h.t, sh.n);
-                    const v3 tl = ld3(ob.world_position) - ro;
-                    const float d2 = mx(dot(tl, tl), 0.001f);
-                    const float imp = L.power / d2;
-                    const float sel = last_wsum > 0.0f ? imp / last_wsum : 0.0f;
-                    w = mis_weight(last_pdf, raw * sel);
-                }
-            }
-            rad = rad + tp * firefly(sc.emittance * w, bounce);
-            if (len(sc.emittance) > 1.0f) break;
-        }
-        const v3 hp = ro + rd * h.t;
-        const v3 V = -rd;
-        const v3 no = dot(sc.normal, V) > 0.0f ? sc.normal : -sc.normal;
-        // light NEE with one-pass reservoir selection (shader.wgsl:2402-2446)
-        if (fl.has(PT_HAS_LIGHTS)) {
-            int sel = -1;
-            float wsum = 0.0f, sw = 0.0f;
-            for (uint32_t i = 0; i < S.n_lights; ++i) {
-                const pt_light& L = S.lights[i];
-                if (L.obj_idx < 0) continue;
-                const v3 tl = ld3(S.objs[L.obj_idx].world_position) - hp;
-                const float d2 = mx(dot(tl, tl), 0.001f);
-                const float w = L.power / d2;
-                wsum += w;
-                if (rng.next() < (w / wsum)) { sel = int(i); sw = w; }
-            }
-            if (sel >= 0 && wsum > 0.0f) {
-                const pt_light& L = S.lights[sel];
-                const pt_object& lo = S.objs[L.obj_idx];
-                LightSample ls;
-                ls.dir = splat(0.0f); ls.color = splat(0.0f); ls.dist = 0.0f; ls.pdf = 0.0f;
-                if (fl.has(PT_HAS_SPHERES) && lo.object_type == PT_OBJ_SPHERE) {
-                    const float x0 = rng.next();
-                    const float x1 = rng.next();
-                    ls = sphere_light_sample(S, L, lo, hp, x0, x1);
-                } else if (fl.has(PT_HAS_CUBES) && lo.object_type == PT_OBJ_CUBE) {
-                    const float x0 = rng.next();
-                    const float x1 = rng.next();
-                    ls = box_light_sample(S, L, lo, hp, x0, x1, rng);
-                }
-                if (ls.pdf > 0.0f) {
-                    ls.pdf *= sw / wsum;
-                    const v3 inc = nee<F, CN>(S, fl, ls, hp, no, V, m, sc, md, L.obj_idx, st, c);
-                    rad = rad + tp * firefly(inc, bounce);
-                }
-            }
-            last_wsum = wsum;
-        }
-        // env NEE (shader.wgsl:2449-2453)
-        if (fl.has(PT_HAS_SKYBOX) && fl.has(PT_MIS_SKYBOX)) {
-            const float ux = rng.next();
-            const float uy = rng.next();
-            const LightSample env = sample_env(S, P, ux, uy);
-            const v3 inc = nee<F, CN>(S, fl, env, hp, no, V, m, sc, md, -1, st, c);
-            rad = rad + tp * firefly(inc, bounce);
-        }
-        if (!sample_bsdf(ro, rd, tp, m, sc, hp, md, last_pdf, rng)) break;
-        // Russian roulette from the third bounce (shader.wgsl:2474-2479)
-        if (bounce < 2) continue;
-        const float surv = clampf(hmax(tp), 0.05f, 0.95f);
-        if (rng.next() > surv) break;
-        tp = tp / surv;
+        if (!shade_bounce<F, CN>(S, fl, P, bounce, h, sh, ps, md, c, sink)) break;
     }
-    return rad;
+    return ps.rad;
 }
 
 template <uint32_t F, bool CN, bool REC>

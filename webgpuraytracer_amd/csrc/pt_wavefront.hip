@@ -1,0 +1,297 @@
+// pt_wavefront.hip — the wavefront form of shader.wgsl `main` (2273-2492).
+//
+// A batch of paths (whole progressive passes over the section) lives in HBM
+// as structure-of-arrays state; each bounce is three launches:
+//   wf_extend  closest hit for every path in the active queue — traversal
+//              only, so it runs at a fraction of the megakernel's registers
+//   wf_shade   the bounce's shading (dev/path.h, shared with the megakernel);
+//              queues the path for the next bounce and its NEE shadow rays
+//   wf_shadow  any-hit test of the queued shadow rays; adds the light NEE then
+//              the env NEE contribution of each path, in the reference's order
+// Queues are compacted with a wave64 ballot + one atomic per wave.  Every
+// kernel reads its queue length from device memory, so a whole batch is
+// launched without a host round trip.  The per-path RNG state and draw order
+// are the megakernel's, so both backends produce the same bits.
+#include "pt_internal.h"
+#include "dev/path.h"
+
+namespace pt {
+
+constexpr int kWfBlock = 256;
+enum { kCntA = 0, kCntB = 1, kCntShadow = 2 };
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Appends the calling lanes with pred set to a queue; returns each such
+// lane's slot.  Must be reached by every active lane of the wave.
+__device__ __forceinline__ uint32_t wave_push(uint32_t* counter, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return 0u;
+    const uint32_t lane = lane_id();
+    const uint32_t leader = uint32_t(__ffsll(static_cast<long long>(mask)) - 1);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+    base = __shfl(base, int(leader), 64);
+    return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+}
+
+__device__ __forceinline__ unsigned long long wsum64(uint32_t v) {
+    unsigned long long s = v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    return s;
+}
+
+__device__ __forceinline__ void add_stat(unsigned long long* stats, int slot, uint32_t v) {
+    const unsigned long long s = wsum64(v);
+    if (lane_id() == 0 && s) atomicAdd(stats + slot, s);
+}
+
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t q = nb / 8u, r = nb % 8u;
+    const uint32_t xcd = b % 8u, slot = b / 8u;
+    return (xcd < r ? xcd * (q + 1u) : r * (q + 1u) + (xcd - r) * q) + slot;
+}
+
+__device__ __forceinline__ void slot_pixel(const WfArgs& A, uint32_t i, uint32_t& px, uint32_t& py, uint32_t& pass) {
+    const uint32_t npix = A.sec_w * A.sec_h;
+    pass = i / npix;
+    const uint32_t lp = i - pass * npix;
+    py = A.P.section[1] + lp / A.sec_w;
+    px = A.P.section[0] + (lp - (lp / A.sec_w) * A.sec_w);
+}
+
+__global__ void __launch_bounds__(kWfBlock) wf_raygen(WfArgs A) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        A.counters[kCntA] = A.n_items;
+        A.counters[kCntB] = 0u;
+        A.counters[kCntShadow] = 0u;
+    }
+    if (i >= A.n_items) return;
+    uint32_t px, py, pass;
+    slot_pixel(A, i, px, py, pass);
+    const uint32_t sn = A.P.sample_number + A.first_pass + pass;
+    const uint32_t seed = A.seeds ? A.seeds[A.first_pass + pass] : A.P.seed;
+    PathState ps;
+    path_begin(A.P, px, py, sn, seed, ps);
+    A.o_pdf[i] = make_float4(ps.ro.x, ps.ro.y, ps.ro.z, ps.last_pdf);
+    A.d_wsum[i] = make_float4(ps.rd.x, ps.rd.y, ps.rd.z, ps.last_wsum);
+    A.tp_rng[i] = make_float4(ps.tp.x, ps.tp.y, ps.tp.z, __uint_as_float(ps.rng.s));
+    A.rad[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    A.meta[i] = 0u;
+    A.queue[0][i] = i;
+    // shadow slots read as empty until a bounce queues a query; wf_shadow
+    // empties them again after use
+    A.sh_tp[2u * i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    A.sh_tp[2u * i + 1u] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+template <uint32_t F, bool CN>
+__global__ void __launch_bounds__(kWfBlock) wf_extend(DevScene S, WfArgs A, int cur) {
+    extern __shared__ uint32_t lds_stack[];
+    const Flags<F> fl{S.flags};
+    const uint32_t n = A.counters[cur];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        A.counters[cur ^ 1] = 0u;
+        A.counters[kCntShadow] = 0u;
+    }
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    if (blk * blockDim.x >= n) return;  // the whole block is past the queue
+    const uint32_t i = blk * blockDim.x + threadIdx.x;
+    Counters c;
+    c.clear();
+    if (i < n) {
+        const uint32_t p = A.queue[cur][i];
+        const float4 o = A.o_pdf[p], d = A.d_wsum[p];
+        LdsStack st{lds_stack + threadIdx.x, blockDim.x, S.stack_cap, 0};
+        HitRec h;
+        closest_hit<F, CN>(S, fl, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), st, h, c);
+        A.hit_a[p] = make_int4(__float_as_int(h.t), (h.kind << 30) | (h.idx & 0x3fffffff), h.prim, int(h.count));
+        A.hit_b[p] = make_float2(h.bu, h.bv);
+        c.ext = 1;
+    }
+    if (A.stats) {
+        add_stat(A.stats, kStExt, c.ext);
+        if (CN) {
+            add_stat(A.stats, kStPops, c.pops);
+            add_stat(A.stats, kStInner, c.inner);
+            add_stat(A.stats, kStObjs, c.objs);
+            add_stat(A.stats, kStTris, c.tris);
+        }
+    }
+}
+
+// Queues the NEE shadow query of slot k of path p (0: light, 1: env).
+struct QueuedNee {
+    const WfArgs& A;
+    uint32_t p;
+    bool any;
+    __device__ void operator()(int k, bool need_ray, v3 o, v3 d, float dist, int ignore, v3 inc, v3 tp, int bounce,
+                               v3& rad) {
+        if (!need_ray) {
+            rad = rad + tp * firefly(inc, bounce);
+            return;
+        }
+        const v3 fi = firefly(inc, bounce);
+        const uint32_t s = 2u * p + uint32_t(k);
+        A.sh_o[s] = make_float4(o.x, o.y, o.z, dist);
+        A.sh_d[s] = make_float4(d.x, d.y, d.z, __int_as_float(ignore));
+        A.sh_tp[s] = make_float4(tp.x, tp.y, tp.z, 1.0f);
+        A.sh_fi[s] = make_float4(fi.x, fi.y, fi.z, 0.0f);
+        any = true;
+    }
+};
+
+template <uint32_t F, bool CN>
+__global__ void __launch_bounds__(kWfBlock) wf_shade(DevScene S, WfArgs A, int cur, int bounce) {
+    const Flags<F> fl{S.flags};
+    const uint32_t n = A.counters[cur];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x * blockDim.x >= n) return;
+    const Params P{A.P.sky_total_lum, A.P.sky_width, A.P.sky_height, A.P.sky_cdf_index};
+    bool alive = false, shadow = false;
+    uint32_t p = 0;
+    Counters c;
+    c.clear();
+    if (i < n) {
+        p = A.queue[cur][i];
+        const float4 o = A.o_pdf[p], d = A.d_wsum[p], t = A.tp_rng[p], r = A.rad[p];
+        const int4 ha = A.hit_a[p];
+        const float2 hb = A.hit_b[p];
+        PathState ps;
+        ps.ro = mk(o.x, o.y, o.z);
+        ps.last_pdf = o.w;
+        ps.rd = mk(d.x, d.y, d.z);
+        ps.last_wsum = d.w;
+        ps.tp = mk(t.x, t.y, t.z);
+        ps.rng.s = __float_as_uint(t.w);
+        ps.rad = mk(r.x, r.y, r.z);
+        HitRec h;
+        h.t = __int_as_float(ha.x);
+        h.kind = (ha.y >> 30) & 3;
+        h.idx = ha.y & 0x3fffffff;
+        if (h.kind == kMiss) h.idx = -1;
+        h.prim = ha.z;
+        h.count = uint32_t(ha.w);
+        h.bu = hb.x;
+        h.bv = hb.y;
+        const uint32_t meta = A.meta[p];
+        Media md{A.media + p, A.pool, int(meta & 7u)};
+        const SurfHit sh = resolve_hit<false>(S, ps.ro, ps.rd, h);
+        QueuedNee sink{A, p, false};
+        alive = shade_bounce<F, CN>(S, fl, P, bounce, h, sh, ps, md, c, sink);
+        alive = alive && bounce + 1 < S.bounce_limit;
+        shadow = sink.any;
+        A.o_pdf[p] = make_float4(ps.ro.x, ps.ro.y, ps.ro.z, ps.last_pdf);
+        A.d_wsum[p] = make_float4(ps.rd.x, ps.rd.y, ps.rd.z, ps.last_wsum);
+        A.tp_rng[p] = make_float4(ps.tp.x, ps.tp.y, ps.tp.z, __uint_as_float(ps.rng.s));
+        A.rad[p] = make_float4(ps.rad.x, ps.rad.y, ps.rad.z, 0.0f);
+        A.meta[p] = uint32_t(md.n);
+    }
+    const uint32_t qa = wave_push(&A.counters[cur ^ 1], alive);
+    if (alive) A.queue[cur ^ 1][qa] = p;
+    const uint32_t qs = wave_push(&A.counters[kCntShadow], shadow);
+    if (shadow) A.shadow_queue[qs] = p;
+    if (A.stats && CN) add_stat(A.stats, kStShade, c.shade);
+}
+
+template <uint32_t F, bool CN>
+__global__ void __launch_bounds__(kWfBlock) wf_shadow(DevScene S, WfArgs A) {
+    extern __shared__ uint32_t lds_stack[];
+    const Flags<F> fl{S.flags};
+    const uint32_t n = A.counters[kCntShadow];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    if (blk * blockDim.x >= n) return;
+    const uint32_t i = blk * blockDim.x + threadIdx.x;
+    Counters c;
+    c.clear();
+    if (i < n) {
+        const uint32_t p = A.shadow_queue[i];
+        LdsStack st{lds_stack + threadIdx.x, blockDim.x, S.stack_cap, 0};
+        float4 r = A.rad[p];
+        v3 rad = mk(r.x, r.y, r.z);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const uint32_t s = 2u * p + uint32_t(k);
+            const float4 tp = A.sh_tp[s];
+            if (tp.w == 0.0f) continue;
+            const float4 o = A.sh_o[s], d = A.sh_d[s], fi = A.sh_fi[s];
+            c.shadow++;
+            const bool occ = occluded<F, CN>(S, fl, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), __float_as_int(d.w), o.w,
+                                             st, c);
+            const v3 add = occ ? splat(0.0f) : mk(fi.x, fi.y, fi.z);
+            rad = rad + mk(tp.x, tp.y, tp.z) * add;
+            A.sh_tp[s].w = 0.0f;
+        }
+        A.rad[p] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+    }
+    if (A.stats) {
+        add_stat(A.stats, kStShadow, c.shadow);
+        if (CN) {
+            add_stat(A.stats, kStPops, c.pops);
+            add_stat(A.stats, kStInner, c.inner);
+            add_stat(A.stats, kStObjs, c.objs);
+            add_stat(A.stats, kStTris, c.tris);
+        }
+    }
+}
+
+// Running mean over the batch's passes in pass order (shader.wgsl:2482-2491).
+__global__ void __launch_bounds__(kWfBlock) wf_accumulate(WfArgs A) {
+    const uint32_t lp = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npix = A.sec_w * A.sec_h;
+    if (lp >= npix) return;
+    const uint32_t py = A.P.section[1] + lp / A.sec_w;
+    const uint32_t px = A.P.section[0] + (lp - (lp / A.sec_w) * A.sec_w);
+    const uint32_t idx = py * A.P.width + px;
+    const float4 old = A.accum[idx];
+    v3 acc = mk(old.x, old.y, old.z);
+    for (uint32_t pass = 0; pass < A.batch_passes; ++pass) {
+        const float4 r = A.rad[pass * npix + lp];
+        const uint32_t sn = A.P.sample_number + A.first_pass + pass;
+        acc = lerp(acc, mk(r.x, r.y, r.z), 1.0f / float(sn + 1u));
+    }
+    A.accum[idx] = make_float4(acc.x, acc.y, acc.z, 1.0f);
+    if (A.write_output) {
+        v3 o = acc * splat(A.P.exposure);
+        o = o / (o + splat(0.3f));
+        A.out[idx] = make_uchar4(unorm8(powf(o.x, 0.4545f)), unorm8(powf(o.y, 0.4545f)), unorm8(powf(o.z, 0.4545f)),
+                                 255);
+    }
+    if (A.stats && lp == 0) atomicAdd(A.stats + kStPaths, (unsigned long long)A.n_items);
+}
+
+template <uint32_t F>
+static hipError_t wf_run(const DevScene& S, const WfArgs& a0, bool cn, hipStream_t s) {
+    const uint32_t blocks = (a0.n_items + kWfBlock - 1) / kWfBlock;
+    const size_t lds = size_t(S.stack_cap) * kWfBlock * 4u;
+    hipLaunchKernelGGL(wf_raygen, dim3(blocks), dim3(kWfBlock), 0, s, a0);
+    int cur = 0;
+    for (int b = 0; b < S.bounce_limit; ++b) {
+        if (cn) {
+            hipLaunchKernelGGL((wf_extend<F, true>), dim3(blocks), dim3(kWfBlock), lds, s, S, a0, cur);
+            hipLaunchKernelGGL((wf_shade<F, true>), dim3(blocks), dim3(kWfBlock), 0, s, S, a0, cur, b);
+            hipLaunchKernelGGL((wf_shadow<F, true>), dim3(blocks), dim3(kWfBlock), lds, s, S, a0);
+        } else {
+            hipLaunchKernelGGL((wf_extend<F, false>), dim3(blocks), dim3(kWfBlock), lds, s, S, a0, cur);
+            hipLaunchKernelGGL((wf_shade<F, false>), dim3(blocks), dim3(kWfBlock), 0, s, S, a0, cur, b);
+            hipLaunchKernelGGL((wf_shadow<F, false>), dim3(blocks), dim3(kWfBlock), lds, s, S, a0);
+        }
+        cur ^= 1;
+    }
+    const uint32_t npix = a0.sec_w * a0.sec_h;
+    hipLaunchKernelGGL(wf_accumulate, dim3((npix + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, s, a0);
+    return hipGetLastError();
+}
+
+hipError_t launch_wavefront_batch(const DevScene& S, const WfArgs& a, bool count_nodes, hipStream_t s) {
+    constexpr uint32_t kC1 = PT_HAS_SPHERES | PT_HAS_CUBES | PT_HAS_PLANES | PT_HAS_CYLINDERS | PT_HAS_LIGHTS |
+                             PT_HAS_SKYBOX | PT_MIS_SKYBOX;
+    constexpr uint32_t kC2 = PT_HAS_MESHES | PT_HAS_SKYBOX | PT_MIS_SKYBOX;
+    if (S.flags == kC1) return wf_run<kC1>(S, a, count_nodes, s);
+    if (S.flags == kC2) return wf_run<kC2>(S, a, count_nodes, s);
+    return wf_run<kDynamicFlags>(S, a, count_nodes, s);
+}
+
+}  // namespace pt
