@@ -73,7 +73,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-count", action="store_true", help="time the kernel without node/triangle counters")
     ap.add_argument("--backend", choices=["megakernel", "wavefront"], default="megakernel")
-    ap.add_argument("--compare", action="store_true", help="also time the other backend (reported as alt_backend)")
+    ap.add_argument("--schedule", choices=["regen", "sync"], default="regen",
+                    help="megakernel pass schedule (same bits either way)")
+    ap.add_argument("--compare", action="store_true",
+                    help="also time the other backend/schedule variants (reported under alt)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the frame would shard by sample "
@@ -93,15 +96,21 @@ def main():
     upload_s = time.perf_counter() - t0
     r.set_option(A.PT_OPT_COUNT_STATS, 0 if args.no_count else 1)
     d = scene.desc()
-    backends = {"megakernel": 0, "wavefront": 1}
+    variants = {"megakernel/regen": [(A.PT_OPT_BACKEND, A.PT_BACKEND_MEGAKERNEL),
+                                     (A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_REGEN)],
+                "megakernel/sync": [(A.PT_OPT_BACKEND, A.PT_BACKEND_MEGAKERNEL),
+                                    (A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_SYNC)],
+                "wavefront": [(A.PT_OPT_BACKEND, A.PT_BACKEND_WAVEFRONT)]}
+    chosen = "wavefront" if args.backend == "wavefront" else "megakernel/" + args.schedule
 
     def step(k):
         seeds = [(args.seed * 0x9E3779B1 + (k * SPP + i) * 0x85EBCA77) & 0xFFFFFFFF for i in range(SPP)]
         p = scene.params(W, H, sample_number=0, seed=seeds[0])
         return r.render_section(p, passes=SPP, seeds=seeds)
 
-    def timed(backend):
-        r.set_option(A.PT_OPT_BACKEND, backends[backend])
+    def timed(variant):
+        for key, val in variants[variant]:
+            r.set_option(key, val)
         for k in range(args.warmup):
             step(1000 + k)
         r.lib.pt_synchronize(r.ctx)
@@ -113,7 +122,7 @@ def main():
         r.lib.pt_synchronize(r.ctx)
         return st, time.perf_counter() - t0
 
-    stats, wall = timed(args.backend)
+    stats, wall = timed(chosen)
 
     rays = sum(s.extension_rays + s.shadow_rays for s in stats)
     kern_ms = sum(s.kernel_ms for s in stats)
@@ -123,8 +132,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(wall * 1000 / args.steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "plain tracer, large scene: 1920x1080, 64 spp/step, 8 bounces, 64 instances of 4 "
-                               "displaced geodesic spheres (%d triangles), 2048x1024 env, %s backend"
-                               % (d.triangles.count, args.backend),
+                               "displaced geodesic spheres (%d triangles), 2048x1024 env, %s"
+                               % (d.triangles.count, chosen),
                    "extension_rays": int(sum(s.extension_rays for s in stats)),
                    "shadow_rays": int(sum(s.shadow_rays for s in stats)),
                    "paths": int(sum(s.paths for s in stats)),
@@ -145,13 +154,16 @@ def main():
                            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                            "algorithmic_bytes_per_launch": int(avg_bytes), "kernel_ms_per_launch": round(avg_ms, 3)}
     if args.compare:
-        other = "wavefront" if args.backend == "megakernel" else "megakernel"
-        st2, wall2 = timed(other)
-        rays2 = sum(x.extension_rays + x.shadow_rays for x in st2)
-        ms2 = sum(x.kernel_ms for x in st2)
-        out["alt_backend"] = {"backend": other, "value": round(rays2 / wall2 / 1e6, 2), "unit": "Mrays/s",
-                              "ms_per_step": round(wall2 * 1000 / args.steps, 3),
-                              "kernel_ms_per_step": round(ms2 / args.steps, 3)}
+        out["alt"] = []
+        for other in variants:
+            if other == chosen:
+                continue
+            st2, wall2 = timed(other)
+            rays2 = sum(x.extension_rays + x.shadow_rays for x in st2)
+            ms2 = sum(x.kernel_ms for x in st2)
+            out["alt"].append({"variant": other, "value": round(rays2 / wall2 / 1e6, 2), "unit": "Mrays/s",
+                               "ms_per_step": round(wall2 * 1000 / args.steps, 3),
+                               "kernel_ms_per_step": round(ms2 / args.steps, 3)})
     if not args.no_cpu:
         p = scene.params(W, H, 0, 1)
         out["cpu_baseline"] = cpu_baseline(scene, p, args.cpu_seconds)

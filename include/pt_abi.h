@@ -114,9 +114,13 @@ PT_LAYOUT_ASSERT(sizeof(pt_hit) == 56, "pt_hit is 56 B");
 enum {
     PT_OPT_BACKEND = 1,     /* PT_BACKEND_* */
     PT_OPT_COUNT_STATS = 2, /* 1: count node/triangle work (default 1) */
-    PT_OPT_WAVEFRONT_POOL = 3 /* rays per wavefront pool (default 1<<21) */
+    PT_OPT_WAVEFRONT_POOL = 3, /* rays per wavefront pool (default 1<<21) */
+    PT_OPT_SCHEDULE = 4        /* megakernel pass schedule, PT_SCHEDULE_* (default REGEN) */
 };
 enum { PT_BACKEND_MEGAKERNEL = 0, PT_BACKEND_WAVEFRONT = 1 };
+/* SYNC: the lanes of a wave run each pass together.  REGEN: a lane whose path
+   ends starts its pixel's next pass at once.  Both produce the same bits. */
+enum { PT_SCHEDULE_SYNC = 0, PT_SCHEDULE_REGEN = 1 };
 
 /* Lifetime: `new Renderer` + Renderer.init (tracer.js:1719-1770). */
 int pt_create(pt_ctx** out, int device);

@@ -193,29 +193,34 @@ def test_gpu_matches_golden_fixture(renderer, name):
 
 
 @pytest.mark.parametrize("which", ["c1", "mesh"])
-def test_wavefront_equals_megakernel_bitwise(renderer, which):
-    """The wavefront backend runs the same shading code with the same RNG
-    order and adds NEE contributions in the same order: identical bits."""
+def test_backends_and_schedules_bitwise(renderer, which):
+    """Both megakernel pass schedules and the wavefront backend run the same
+    shading code with the same per-pixel RNG order and add NEE contributions
+    in the same order: identical bits and identical work counts."""
     scene = SceneBuilder.config(1, seed=81) if which == "c1" else small_mesh_scene(seed=13)
     w, h = 80, 48
     renderer.set_scene(scene)
-    renderer.resize(w, h)
     seeds = [11, 22, 33, 44, 55]
     p = scene.params(w, h, sample_number=0, seed=seeds[0])
-    renderer.set_option(A.PT_OPT_BACKEND, 0)
-    st_m = renderer.render_section(p, passes=5, seeds=seeds)
-    acc_m = renderer.read_accum()
-    out_m = renderer.read_output()
-    renderer.resize(w, h)
-    renderer.set_option(A.PT_OPT_BACKEND, 1)
-    renderer.set_option(A.PT_OPT_WAVEFRONT_POOL, 2 * w * h)  # 2 passes per batch: 3 batches
+    runs = {}
+    variants = {"regen": [(A.PT_OPT_BACKEND, A.PT_BACKEND_MEGAKERNEL), (A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_REGEN)],
+                "sync": [(A.PT_OPT_BACKEND, A.PT_BACKEND_MEGAKERNEL), (A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_SYNC)],
+                # 2 passes per batch: 3 batches
+                "wavefront": [(A.PT_OPT_BACKEND, A.PT_BACKEND_WAVEFRONT), (A.PT_OPT_WAVEFRONT_POOL, 2 * w * h)]}
     try:
-        st_w = renderer.render_section(p, passes=5, seeds=seeds)
-        acc_w = renderer.read_accum()
-        out_w = renderer.read_output()
+        for name, opts in variants.items():
+            for k, v in opts:
+                renderer.set_option(k, v)
+            renderer.resize(w, h)
+            st = renderer.render_section(p, passes=5, seeds=seeds)
+            runs[name] = (st, renderer.read_accum(), renderer.read_output())
     finally:
-        renderer.set_option(A.PT_OPT_BACKEND, 0)
-    assert np.array_equal(acc_m.view(np.uint32), acc_w.view(np.uint32))
-    assert np.array_equal(out_m, out_w)
-    for k in ("extension_rays", "shadow_rays", "paths", "node_pops", "inner_pops", "tri_tests", "shade_events"):
-        assert getattr(st_m, k) == getattr(st_w, k), k
+        renderer.set_option(A.PT_OPT_BACKEND, A.PT_BACKEND_MEGAKERNEL)
+        renderer.set_option(A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_REGEN)
+    st_m, acc_m, out_m = runs["regen"]
+    for name in ("sync", "wavefront"):
+        st_o, acc_o, out_o = runs[name]
+        assert np.array_equal(acc_m.view(np.uint32), acc_o.view(np.uint32)), name
+        assert np.array_equal(out_m, out_o), name
+        for k in ("extension_rays", "shadow_rays", "paths", "node_pops", "inner_pops", "tri_tests", "shade_events"):
+            assert getattr(st_m, k) == getattr(st_o, k), (name, k)

@@ -29,28 +29,32 @@ struct HitRec {
 
 __device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
 
-// Push order of shader.wgsl:533-539 / 743-749.
-__device__ __forceinline__ void push_children(LdsStack& st, float lt, float rt, uint32_t lref, uint32_t rref,
-                                              float limit) {
-    if (lt < rt) {
-        if (rt >= 0.0f && rt < limit) st.push(rref);
-        if (lt >= 0.0f && lt < limit) st.push(lref);
-    } else {
-        if (lt >= 0.0f && lt < limit) st.push(lref);
-        if (rt >= 0.0f && rt < limit) st.push(rref);
-    }
-}
-
+// Inner-node visit.  The reference pushes the accepted children far-first
+// and pops the near one at once (shader.wgsl:533-539, 743-749); here the child
+// it would pop next is returned in `next` (kept in a register) and only the
+// other accepted child goes to the stack.  Same visit order, half the stack
+// traffic.  Returns false when neither child is accepted.
 template <bool CN>
-__device__ __forceinline__ void visit_inner(const CNode* nodes, uint32_t ref, v3 o, v3 inv, float limit, LdsStack& st,
-                                            Counters& c) {
+__device__ __forceinline__ bool visit_inner(const CNode* nodes, uint32_t ref, v3 o, v3 inv, float limit, LdsStack& st,
+                                            Counters& c, uint32_t& next) {
     if (CN) c.inner++;
     const CNode n = load_cnode(nodes + ref);
     v3 lmin, lmax, rmin, rmax;
     cnode_boxes(n, lmin, lmax, rmin, rmax);
     const float lt = slab(o, inv, lmin, lmax);
     const float rt = slab(o, inv, rmin, rmax);
-    push_children(st, lt, rt, n.d.x, n.d.y, limit);
+    const bool lok = lt >= 0.0f && lt < limit;
+    const bool rok = rt >= 0.0f && rt < limit;
+    const bool left_top = lt < rt;  // ties: the right child ends on top
+    const uint32_t top = left_top ? n.d.x : n.d.y, bot = left_top ? n.d.y : n.d.x;
+    const bool top_ok = left_top ? lok : rok, bot_ok = left_top ? rok : lok;
+    if (top_ok) {
+        if (bot_ok) st.push(bot);
+        next = top;
+        return true;
+    }
+    next = bot;
+    return bot_ok;
 }
 
 // BLAS-only closest-hit traversal (list meshes, shader.wgsl:767-771).
@@ -63,11 +67,11 @@ __device__ void blas_closest(const DevScene& S, uint32_t root_slot, const float*
     const float bt = slab(lo, linv, ld3(R.mn), ld3(R.mx));
     if (bt < 0.0f || bt >= h.t) return;
     st.sp = 0;
-    st.push(R.ref);
-    while (st.sp > 0) {
-        const uint32_t ref = st.pop();
+    uint32_t ref = R.ref;
+    for (;;) {
         h.count++;
         if (CN) c.pops++;
+        bool have = false;
         if (ref & kLeafBit) {
             uint32_t s0, n0;
             leaf_range(ref, S.bvh_raw, s0, n0);
@@ -78,12 +82,18 @@ __device__ void blas_closest(const DevScene& S, uint32_t root_slot, const float*
                 if (t >= 0.0f && t < h.t) { h.t = t; h.kind = kind; h.idx = idx; h.prim = int(k); h.bu = u; h.bv = v; }
             }
         } else {
-            visit_inner<CN>(S.bnodes, ref, lo, linv, h.t, st, c);
+            have = visit_inner<CN>(S.bnodes, ref, lo, linv, h.t, st, c, ref);
+        }
+        if (!have) {
+            if (st.sp == 0) break;
+            ref = st.pop();
         }
     }
 }
 
 // trace_scene (shader.wgsl:754-774): planes, then the TLAS, then list meshes.
+// The two BVH levels run in one loop: `ref` is the node visited next, or
+// have == false when the next work is a TLAS-leaf object or a stack pop.
 template <uint32_t F, bool CN>
 __device__ void closest_hit(const DevScene& S, Flags<F> fl, v3 o, v3 d, LdsStack& st, HitRec& h, Counters& c) {
     h.t = 1e10f;
@@ -105,20 +115,72 @@ __device__ void closest_hit(const DevScene& S, Flags<F> fl, v3 o, v3 d, LdsStack
         const float bt = slab(o, inv, ld3(R.mn), ld3(R.mx));
         if (!(bt < 0.0f || bt >= h.t)) {
             st.sp = 0;
-            st.push(R.ref);
+            uint32_t ref = R.ref;
+            bool have = true;
             bool in_blas = false;
             int blas_base = 0;
             uint32_t cur = 0, end = 0, tri_off = 0;
             int obj = -1;
             v3 lo = o, ld = d, linv = inv;
             for (;;) {
-                if (in_blas) {
-                    if (st.sp == blas_base) { in_blas = false; continue; }
-                    const uint32_t ref = st.pop();
-                    h.count++;
-                    if (CN) c.pops++;
-                    if (ref & kLeafBit) {
-                        uint32_t s0, n0;
+                if (!have) {
+                    if (in_blas) {
+                        if (st.sp == blas_base) { in_blas = false; continue; }
+                        ref = st.pop();
+                    } else if (cur < end) {
+                        const uint32_t i = cur++;
+                        if (CN) c.objs++;
+                        const pt_object& ob = S.objs[i];
+                        const int type = ob.object_type;
+                        if (fl.has(PT_HAS_MESHES) && type == PT_OBJ_MESH) {
+                            lo = xform_point(ob.inv_matrix, o);
+                            ld = xform_dir(ob.inv_matrix, d);
+                            linv = splat(1.0f) / ld;
+                            const Root& br = S.roots[S.obj_root[i]];
+                            const float b2 = slab(lo, linv, ld3(br.mn), ld3(br.mx));
+                            if (!(b2 < 0.0f || b2 >= h.t)) {
+                                tri_off = f2u(ob.param1);
+                                obj = int(i);
+                                blas_base = st.sp;
+                                in_blas = true;
+                                ref = br.ref;
+                                have = true;
+                            }
+                        } else if (fl.has(PT_HAS_SPHERES) && type == PT_OBJ_SPHERE) {
+                            const float t = unit_sphere(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d));
+                            if (t > 0.001f && t < h.t) { h.t = t; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
+                        } else if (fl.has(PT_HAS_CUBES) && type == PT_OBJ_CUBE) {
+                            const float t = unit_cube(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d));
+                            if (t > 0.001f && t < h.t) { h.t = t; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
+                        } else if (fl.has(PT_HAS_CYLINDERS) && type == PT_OBJ_CYLINDER) {
+                            int which;
+                            v3 ps;
+                            const float t = cone_cylinder(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d),
+                                                          1.0f, ob.param0, which, ps);
+                            if (which >= 0 && t < h.t) { h.t = t; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
+                        } else if (fl.has(PT_HAS_TORI) && type == PT_OBJ_TORUS) {
+                            v3 tl = xform_dir(ob.inv_matrix, d);
+                            const float rs = len(tl);
+                            tl = tl / rs;
+                            const float t = torus_t(xform_point(ob.inv_matrix, o), tl, 1.0f, ob.param0 * 1.0f);
+                            if (t > 0.0f) {
+                                const float wt = t / rs;
+                                if (wt < h.t) { h.t = wt; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
+                            }
+                        }
+                        continue;
+                    } else {
+                        if (st.sp == 0) break;
+                        ref = st.pop();
+                    }
+                }
+                // visit `ref` (the reference's pop)
+                h.count++;
+                if (CN) c.pops++;
+                have = false;
+                if (ref & kLeafBit) {
+                    uint32_t s0, n0;
+                    if (in_blas) {
                         leaf_range(ref, S.bvh_raw, s0, n0);
                         for (uint32_t k = tri_off + s0; k < tri_off + s0 + n0; ++k) {
                             if (CN) c.tris++;
@@ -129,63 +191,14 @@ __device__ void closest_hit(const DevScene& S, Flags<F> fl, v3 o, v3 d, LdsStack
                             }
                         }
                     } else {
-                        visit_inner<CN>(S.bnodes, ref, lo, linv, h.t, st, c);
+                        leaf_range(ref, S.tlas_raw, s0, n0);
+                        cur = s0;
+                        end = s0 + n0;
                     }
-                    continue;
-                }
-                if (cur < end) {
-                    const uint32_t i = cur++;
-                    if (CN) c.objs++;
-                    const pt_object& ob = S.objs[i];
-                    const int type = ob.object_type;
-                    if (fl.has(PT_HAS_MESHES) && type == PT_OBJ_MESH) {
-                        lo = xform_point(ob.inv_matrix, o);
-                        ld = xform_dir(ob.inv_matrix, d);
-                        linv = splat(1.0f) / ld;
-                        const Root& br = S.roots[S.obj_root[i]];
-                        const float b2 = slab(lo, linv, ld3(br.mn), ld3(br.mx));
-                        if (!(b2 < 0.0f || b2 >= h.t)) {
-                            tri_off = f2u(ob.param1);
-                            obj = int(i);
-                            blas_base = st.sp;
-                            st.push(br.ref);
-                            in_blas = true;
-                        }
-                    } else if (fl.has(PT_HAS_SPHERES) && type == PT_OBJ_SPHERE) {
-                        const float t = unit_sphere(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d));
-                        if (t > 0.001f && t < h.t) { h.t = t; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
-                    } else if (fl.has(PT_HAS_CUBES) && type == PT_OBJ_CUBE) {
-                        const float t = unit_cube(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d));
-                        if (t > 0.001f && t < h.t) { h.t = t; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
-                    } else if (fl.has(PT_HAS_CYLINDERS) && type == PT_OBJ_CYLINDER) {
-                        int which;
-                        v3 ps;
-                        const float t = cone_cylinder(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), 1.0f,
-                                                      ob.param0, which, ps);
-                        if (which >= 0 && t < h.t) { h.t = t; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
-                    } else if (fl.has(PT_HAS_TORI) && type == PT_OBJ_TORUS) {
-                        v3 tl = xform_dir(ob.inv_matrix, d);
-                        const float rs = len(tl);
-                        tl = tl / rs;
-                        const float t = torus_t(xform_point(ob.inv_matrix, o), tl, 1.0f, ob.param0 * 1.0f);
-                        if (t > 0.0f) {
-                            const float wt = t / rs;
-                            if (wt < h.t) { h.t = wt; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
-                        }
-                    }
-                    continue;
-                }
-                if (st.sp == 0) break;
-                const uint32_t ref = st.pop();
-                h.count++;
-                if (CN) c.pops++;
-                if (ref & kLeafBit) {
-                    uint32_t s0, n0;
-                    leaf_range(ref, S.tlas_raw, s0, n0);
-                    cur = s0;
-                    end = s0 + n0;
+                } else if (in_blas) {
+                    have = visit_inner<CN>(S.bnodes, ref, lo, linv, h.t, st, c, ref);
                 } else {
-                    visit_inner<CN>(S.tnodes, ref, o, inv, h.t, st, c);
+                    have = visit_inner<CN>(S.tnodes, ref, o, inv, h.t, st, c, ref);
                 }
             }
         }
@@ -207,10 +220,10 @@ __device__ bool blas_any(const DevScene& S, uint32_t root_slot, const float* inv
     const float bt = slab(lo, linv, ld3(R.mn), ld3(R.mx));
     if (bt < 0.0f || bt >= tdist) return false;
     st.sp = 0;
-    st.push(R.ref);
-    while (st.sp > 0) {
-        const uint32_t ref = st.pop();
+    uint32_t ref = R.ref;
+    for (;;) {
         if (CN) c.pops++;
+        bool have = false;
         if (ref & kLeafBit) {
             uint32_t s0, n0;
             leaf_range(ref, S.bvh_raw, s0, n0);
@@ -221,7 +234,11 @@ __device__ bool blas_any(const DevScene& S, uint32_t root_slot, const float* inv
                 if (t >= 0.0f && t < tdist) return true;
             }
         } else {
-            visit_inner<CN>(S.bnodes, ref, lo, linv, tdist, st, c);
+            have = visit_inner<CN>(S.bnodes, ref, lo, linv, tdist, st, c, ref);
+        }
+        if (!have) {
+            if (st.sp == 0) break;
+            ref = st.pop();
         }
     }
     return false;
@@ -247,18 +264,57 @@ __device__ bool occluded(const DevScene& S, Flags<F> fl, v3 o, v3 d, int target,
     if (fl.has(kAnyObject)) {
         const v3 inv = splat(1.0f) / d;
         st.sp = 0;
-        st.push(S.roots[0].ref);
+        uint32_t ref = S.roots[0].ref;  // no root box test here (shader.wgsl:1134-1144)
+        bool have = true;
         bool in_blas = false;
         int blas_base = 0;
         uint32_t cur = 0, end = 0, tri_off = 0;
         v3 lo = o, ld = d, linv = inv;
         for (;;) {
-            if (in_blas) {
-                if (st.sp == blas_base) { in_blas = false; continue; }
-                const uint32_t ref = st.pop();
-                if (CN) c.pops++;
-                if (ref & kLeafBit) {
-                    uint32_t s0, n0;
+            if (!have) {
+                if (in_blas) {
+                    if (st.sp == blas_base) { in_blas = false; continue; }
+                    ref = st.pop();
+                } else if (cur < end) {
+                    const uint32_t i = cur++;
+                    if (int(i) == target) continue;
+                    if (CN) c.objs++;
+                    const pt_object& ob = S.objs[i];
+                    const int type = ob.object_type;
+                    if (fl.has(PT_HAS_MESHES) && type == PT_OBJ_MESH) {
+                        lo = xform_point(ob.inv_matrix, o);
+                        ld = xform_dir(ob.inv_matrix, d);
+                        linv = splat(1.0f) / ld;
+                        const Root& br = S.roots[S.obj_root[i]];
+                        const float b2 = slab(lo, linv, ld3(br.mn), ld3(br.mx));
+                        if (!(b2 < 0.0f || b2 >= tdist)) {
+                            tri_off = f2u(ob.param1);
+                            blas_base = st.sp;
+                            in_blas = true;
+                            ref = br.ref;
+                            have = true;
+                        }
+                    } else if (fl.has(PT_HAS_SPHERES) && type == PT_OBJ_SPHERE) {
+                        if (sphere_blocks(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), tdist)) return true;
+                    } else if (fl.has(PT_HAS_CUBES) && type == PT_OBJ_CUBE) {
+                        if (cube_blocks(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), tdist)) return true;
+                    } else if (fl.has(PT_HAS_CYLINDERS) && type == PT_OBJ_CYLINDER) {
+                        if (cylinder_blocks(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), ob.param0, tdist))
+                            return true;
+                    } else if (fl.has(PT_HAS_TORI) && type == PT_OBJ_TORUS) {
+                        if (torus_blocks(o, d, ob.inv_matrix, ob.param0, tdist)) return true;
+                    }
+                    continue;
+                } else {
+                    if (st.sp == 0) break;
+                    ref = st.pop();
+                }
+            }
+            if (CN) c.pops++;
+            have = false;
+            if (ref & kLeafBit) {
+                uint32_t s0, n0;
+                if (in_blas) {
                     leaf_range(ref, S.bvh_raw, s0, n0);
                     for (uint32_t k = tri_off + s0; k < tri_off + s0 + n0; ++k) {
                         if (CN) c.tris++;
@@ -267,50 +323,14 @@ __device__ bool occluded(const DevScene& S, Flags<F> fl, v3 o, v3 d, int target,
                         if (t >= 0.0f && t < tdist) return true;
                     }
                 } else {
-                    visit_inner<CN>(S.bnodes, ref, lo, linv, tdist, st, c);
+                    leaf_range(ref, S.tlas_raw, s0, n0);
+                    cur = s0;
+                    end = s0 + n0;
                 }
-                continue;
-            }
-            if (cur < end) {
-                const uint32_t i = cur++;
-                if (int(i) == target) continue;
-                if (CN) c.objs++;
-                const pt_object& ob = S.objs[i];
-                const int type = ob.object_type;
-                if (fl.has(PT_HAS_MESHES) && type == PT_OBJ_MESH) {
-                    lo = xform_point(ob.inv_matrix, o);
-                    ld = xform_dir(ob.inv_matrix, d);
-                    linv = splat(1.0f) / ld;
-                    const Root& br = S.roots[S.obj_root[i]];
-                    const float b2 = slab(lo, linv, ld3(br.mn), ld3(br.mx));
-                    if (!(b2 < 0.0f || b2 >= tdist)) {
-                        tri_off = f2u(ob.param1);
-                        blas_base = st.sp;
-                        st.push(br.ref);
-                        in_blas = true;
-                    }
-                } else if (fl.has(PT_HAS_SPHERES) && type == PT_OBJ_SPHERE) {
-                    if (sphere_blocks(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), tdist)) return true;
-                } else if (fl.has(PT_HAS_CUBES) && type == PT_OBJ_CUBE) {
-                    if (cube_blocks(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), tdist)) return true;
-                } else if (fl.has(PT_HAS_CYLINDERS) && type == PT_OBJ_CYLINDER) {
-                    if (cylinder_blocks(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), ob.param0, tdist))
-                        return true;
-                } else if (fl.has(PT_HAS_TORI) && type == PT_OBJ_TORUS) {
-                    if (torus_blocks(o, d, ob.inv_matrix, ob.param0, tdist)) return true;
-                }
-                continue;
-            }
-            if (st.sp == 0) break;
-            const uint32_t ref = st.pop();
-            if (CN) c.pops++;
-            if (ref & kLeafBit) {
-                uint32_t s0, n0;
-                leaf_range(ref, S.tlas_raw, s0, n0);
-                cur = s0;
-                end = s0 + n0;
+            } else if (in_blas) {
+                have = visit_inner<CN>(S.bnodes, ref, lo, linv, tdist, st, c, ref);
             } else {
-                visit_inner<CN>(S.tnodes, ref, o, inv, tdist, st, c);
+                have = visit_inner<CN>(S.tnodes, ref, o, inv, tdist, st, c, ref);
             }
         }
     }

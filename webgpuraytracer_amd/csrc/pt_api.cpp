@@ -43,6 +43,7 @@ struct pt_ctx {
     std::string err;
     // options
     int backend = PT_BACKEND_MEGAKERNEL;
+    int schedule = PT_SCHEDULE_REGEN;
     bool count_stats = true;
     // scene
     bool has_scene = false;
@@ -454,6 +455,11 @@ int pt_set_option(pt_ctx* c, int key, int64_t value) {
             if (value < 1024 || value > (1ll << 28)) return c->fail(PT_ERR_ARG, "pool size must be in [1024, 2^28]");
             c->wf_pool_opt = uint64_t(value);
             return PT_OK;
+        case PT_OPT_SCHEDULE:
+            if (value != PT_SCHEDULE_SYNC && value != PT_SCHEDULE_REGEN)
+                return c->fail(PT_ERR_ARG, "unknown schedule %lld", (long long)value);
+            c->schedule = int(value);
+            return PT_OK;
         default: return c->fail(PT_ERR_ARG, "unknown option %d", key);
     }
 }
@@ -652,6 +658,7 @@ static int run_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t 
     a.stats = static_cast<unsigned long long*>(c->stats.p);
     a.hits = hits_dev;
     a.n_rec = n_rec;
+    a.schedule = c->schedule;
     if (c->backend == PT_BACKEND_WAVEFRONT && !hits_dev) {
         if ((rc = run_wavefront(c, a))) return rc;
     } else {

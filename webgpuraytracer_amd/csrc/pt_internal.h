@@ -20,6 +20,7 @@ struct TraceArgs {
     unsigned long long* stats;   // kStCount counters
     pt_hit* hits;                // hit recording (NULL: off)
     uint32_t n_rec;
+    int schedule;                // PT_SCHEDULE_*
 };
 
 struct RayArgs {

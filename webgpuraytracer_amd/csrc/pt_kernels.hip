@@ -115,7 +115,17 @@ __device__ v3 trace_path(const DevScene& S, Flags<F> fl, const pt_scene_params& 
     return ps.rad;
 }
 
-template <uint32_t F, bool CN, bool REC>
+// Pass schedules of trace_kernel.
+//   kSync   every lane of a wave runs pass p, then pass p+1: a wave waits
+//           for its longest path in every pass.
+//   kRegen  a lane whose path ends starts its pixel's next pass at once, so
+//           a wave only waits once, for the lane with the longest total.
+//   kRecord kSync plus per-bounce hit recording.
+// Each pixel still runs its passes in order and folds them into the running
+// mean in order, so every schedule writes the same bits.
+enum { kSync = 0, kRegen = 1, kRecord = 2 };
+
+template <uint32_t F, bool CN, int MODE>
 __global__ void __launch_bounds__(kWave) trace_kernel(DevScene S, TraceArgs A) {
     extern __shared__ uint32_t lds_stack[];
     const Flags<F> fl{S.flags};
@@ -129,22 +139,50 @@ __global__ void __launch_bounds__(kWave) trace_kernel(DevScene S, TraceArgs A) {
                         py < A.P.height;
     Counters c;
     c.clear();
-    if (active) {
+    if (active && A.n_passes > 0) {
         LdsStack st{lds_stack + lane, blockDim.x, S.stack_cap, 0};
         float* media = reinterpret_cast<float*>(lds_stack + S.stack_cap * blockDim.x) + lane;
         Params P{A.P.sky_total_lum, A.P.sky_width, A.P.sky_height, A.P.sky_cdf_index};
         const uint32_t idx = py * A.P.width + px;
         const float4 old = A.accum[idx];
         v3 acc = mk(old.x, old.y, old.z);
-        for (uint32_t pass = 0; pass < A.n_passes; ++pass) {
-            const uint32_t sn = A.P.sample_number + pass;
-            const uint32_t seed = A.seeds ? A.seeds[pass] : A.P.seed;
-            pt_hit* rec = REC ? A.hits + size_t(idx) * A.n_rec : nullptr;
-            const v3 r = trace_path<F, CN, REC>(S, fl, A.P, P, px, py, sn, seed, st, media, c, rec, A.n_rec);
-            const float w = 1.0f / float(sn + 1u);
-            acc = lerp(acc, r, w);
+        if (MODE == kRegen) {
+            ImmediateNee<F, CN> sink{S, fl, st, c};
+            Media md{media, blockDim.x, 0};
+            PathState ps;
+            uint32_t pass = 0;
+            int bounce = 0;
+            path_begin(A.P, px, py, A.P.sample_number, A.seeds ? A.seeds[0] : A.P.seed, ps);
+            c.paths++;
+            for (;;) {
+                if (bounce < S.bounce_limit) {
+                    HitRec h;
+                    c.ext++;
+                    closest_hit<F, CN>(S, fl, ps.ro, ps.rd, st, h, c);
+                    const SurfHit sh = resolve_hit<false>(S, ps.ro, ps.rd, h);
+                    const bool more = shade_bounce<F, CN>(S, fl, P, bounce, h, sh, ps, md, c, sink);
+                    ++bounce;
+                    if (more && bounce < S.bounce_limit) continue;
+                }
+                const uint32_t sn = A.P.sample_number + pass;
+                acc = lerp(acc, ps.rad, 1.0f / float(sn + 1u));
+                if (++pass == A.n_passes) break;
+                path_begin(A.P, px, py, sn + 1u, A.seeds ? A.seeds[pass] : A.P.seed, ps);
+                md.n = 0;
+                bounce = 0;
+                c.paths++;
+            }
+        } else {
+            for (uint32_t pass = 0; pass < A.n_passes; ++pass) {
+                const uint32_t sn = A.P.sample_number + pass;
+                const uint32_t seed = A.seeds ? A.seeds[pass] : A.P.seed;
+                pt_hit* rec = MODE == kRecord ? A.hits + size_t(idx) * A.n_rec : nullptr;
+                const v3 r = trace_path<F, CN, MODE == kRecord>(S, fl, A.P, P, px, py, sn, seed, st, media, c, rec,
+                                                                A.n_rec);
+                acc = lerp(acc, r, 1.0f / float(sn + 1u));
+            }
         }
-        if (!REC) {
+        if (MODE != kRecord) {
             A.accum[idx] = make_float4(acc.x, acc.y, acc.z, 1.0f);
             v3 o = acc * splat(A.P.exposure);
             o = o / (o + splat(0.3f));
@@ -465,17 +503,26 @@ __global__ void zero_u64(unsigned long long* p, uint32_t n) {
 // ---------------------------------------------------------------- launchers --
 static size_t stack_bytes(const DevScene& S, uint32_t threads) { return size_t(S.stack_cap) * threads * 4u; }
 
+template <uint32_t F, int MODE>
+static void launch_trace_m(const DevScene& S, const TraceArgs& a, bool cn, uint32_t tiles, size_t lds, hipStream_t s) {
+    if (cn) {
+        hipLaunchKernelGGL((trace_kernel<F, true, MODE>), dim3(tiles), dim3(kWave), lds, s, S, a);
+    } else {
+        hipLaunchKernelGGL((trace_kernel<F, false, MODE>), dim3(tiles), dim3(kWave), lds, s, S, a);
+    }
+}
+
 template <uint32_t F>
 static hipError_t launch_trace_f(const DevScene& S, const TraceArgs& a, bool cn, hipStream_t s) {
     const uint32_t tiles = ((a.sec_w + kTile - 1) / kTile) * ((a.sec_h + kTile - 1) / kTile);
     if (tiles == 0) return hipSuccess;
     const size_t lds = stack_bytes(S, kWave) + size_t(kMediaFloats) * kWave * 4u;
     if (a.hits) {
-        hipLaunchKernelGGL((trace_kernel<F, false, true>), dim3(tiles), dim3(kWave), lds, s, S, a);
-    } else if (cn) {
-        hipLaunchKernelGGL((trace_kernel<F, true, false>), dim3(tiles), dim3(kWave), lds, s, S, a);
+        hipLaunchKernelGGL((trace_kernel<F, false, kRecord>), dim3(tiles), dim3(kWave), lds, s, S, a);
+    } else if (a.schedule == PT_SCHEDULE_REGEN) {
+        launch_trace_m<F, kRegen>(S, a, cn, tiles, lds, s);
     } else {
-        hipLaunchKernelGGL((trace_kernel<F, false, false>), dim3(tiles), dim3(kWave), lds, s, S, a);
+        launch_trace_m<F, kSync>(S, a, cn, tiles, lds, s);
     }
     return hipGetLastError();
 }
