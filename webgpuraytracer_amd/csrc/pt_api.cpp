@@ -68,6 +68,9 @@ struct pt_ctx {
     uint32_t wf_pool = 0;
     DevBuf wf_o, wf_d, wf_t, wf_r, wf_meta, wf_ha, wf_hb, wf_media, wf_q0, wf_q1, wf_sq, wf_so, wf_sd, wf_stp, wf_sfi,
         wf_cnt;
+    // persistent megakernel: per-lane medium stacks, per-XCD tile counters
+    DevBuf mk_media, mk_tiles;
+    int n_cu = 0;
 
     int fail(int code, const char* fmt, ...) {
         char buf[512];
@@ -108,7 +111,7 @@ struct pt_ctx {
                          &planes, &meshes, &mesh_root, &lights, &sky, &cdf, &cdf_rows, &accum, &out, &gbuf, &dn_a,
                          &dn_b, &dn_fin, &dn_guide, &stats, &seeds, &rays_in, &rays_aux, &rays_aux2, &rays_out,
                          &wf_o, &wf_d, &wf_t, &wf_r, &wf_meta, &wf_ha, &wf_hb, &wf_media, &wf_q0, &wf_q1, &wf_sq,
-                         &wf_so, &wf_sd, &wf_stp, &wf_sfi, &wf_cnt};
+                         &wf_so, &wf_sd, &wf_stp, &wf_sfi, &wf_cnt, &mk_media, &mk_tiles};
         for (DevBuf* b : all) b->release();
     }
 };
@@ -424,7 +427,8 @@ int pt_create(pt_ctx** out, int device) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
-    if (e != hipSuccess) { delete c; return PT_ERR_HIP; }
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess || c->n_cu <= 0) { delete c; return PT_ERR_HIP; }
     *out = c;
     return PT_OK;
 }
@@ -662,8 +666,22 @@ static int run_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t 
     if (c->backend == PT_BACKEND_WAVEFRONT && !hits_dev) {
         if ((rc = run_wavefront(c, a))) return rc;
     } else {
-        e = hipEventRecord(c->ev0, c->stream);
-        if (e == hipSuccess) e = launch_trace(c->dev, a, c->count_stats && !hits_dev, c->stream);
+        const bool cn = c->count_stats && !hits_dev;
+        a.n_tiles = ((a.sec_w + 7u) / 8u) * ((a.sec_h + 7u) / 8u);
+        int bpc = 0;
+        e = trace_blocks_per_cu(c->dev, a, cn, &bpc);
+        if (e != hipSuccess) return c->hip_fail(e, "trace occupancy");
+        if (bpc <= 0) return c->fail(PT_ERR_CAPACITY, "trace kernel cannot be resident (LDS stack of %u entries)",
+                                     c->stack_cap);
+        const uint32_t grid = uint32_t(std::min<uint64_t>(a.n_tiles, uint64_t(bpc) * uint64_t(c->n_cu)));
+        a.media_stride = grid * 64u;
+        if ((rc = c->alloc(c->mk_media, size_t(a.media_stride) * kLaneMediaFloats * 4u, "medium stacks"))) return rc;
+        if ((rc = c->alloc(c->mk_tiles, 8 * 4, "tile counters"))) return rc;
+        a.media = static_cast<float*>(c->mk_media.p);
+        a.tile_ctr = static_cast<uint32_t*>(c->mk_tiles.p);
+        e = launch_zero_u64(static_cast<unsigned long long*>(c->mk_tiles.p), 4, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+        if (e == hipSuccess) e = launch_trace(c->dev, a, grid, cn, c->stream);
         if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
         if (e != hipSuccess) return c->hip_fail(e, "trace launch");
     }

@@ -7,6 +7,9 @@
 
 namespace pt {
 
+// Floats of one lane's medium stack (4 media x ior, sigma rgb, emission rgb).
+constexpr uint32_t kLaneMediaFloats = 28;
+
 // Counter slots in the device stats buffer (uint64 each).
 enum { kStExt = 0, kStShadow, kStPaths, kStPops, kStInner, kStObjs, kStTris, kStShade, kStCount };
 
@@ -21,6 +24,10 @@ struct TraceArgs {
     pt_hit* hits;                // hit recording (NULL: off)
     uint32_t n_rec;
     int schedule;                // PT_SCHEDULE_*
+    uint32_t n_tiles;            // 8x8 tiles of the section
+    uint32_t* tile_ctr;          // 8 per-XCD band counters, zero at launch
+    float* media;                // kMediaFloats per lane of the grid, structure of arrays
+    uint32_t media_stride;       // grid workgroups x 64
 };
 
 struct RayArgs {
@@ -74,7 +81,11 @@ struct WfArgs {
 };
 
 // Launchers (pt_kernels.hip).  All asynchronous on `stream`.
-hipError_t launch_trace(const DevScene& S, const TraceArgs& a, bool count_nodes, hipStream_t stream);
+// The trace kernel is persistent: launch_trace takes the grid size, normally
+// trace_blocks_per_cu x CUs (capped by the tile count); a.media must hold
+// kMediaFloats x grid x 64 floats.
+hipError_t trace_blocks_per_cu(const DevScene& S, const TraceArgs& a, bool count_nodes, int* blocks_per_cu);
+hipError_t launch_trace(const DevScene& S, const TraceArgs& a, uint32_t grid, bool count_nodes, hipStream_t stream);
 hipError_t launch_gbuffer(const DevScene& S, const pt_scene_params& P, pt_gbuffer_texel* gbuf, uchar4* out,
                           hipStream_t stream);
 hipError_t launch_rays(const DevScene& S, const RayArgs& a, bool shadow, hipStream_t stream);
