@@ -115,12 +115,20 @@ enum {
     PT_OPT_BACKEND = 1,     /* PT_BACKEND_* */
     PT_OPT_COUNT_STATS = 2, /* 1: count node/triangle work (default 1) */
     PT_OPT_WAVEFRONT_POOL = 3, /* rays per wavefront pool (default 1<<21) */
-    PT_OPT_SCHEDULE = 4        /* megakernel pass schedule, PT_SCHEDULE_* (default REGEN) */
+    PT_OPT_SCHEDULE = 4,       /* megakernel pass schedule, PT_SCHEDULE_* (default REGEN) */
+    PT_OPT_TRAVERSAL = 5       /* BVH traversal, PT_TRAVERSAL_* (default WIDE) */
 };
 enum { PT_BACKEND_MEGAKERNEL = 0, PT_BACKEND_WAVEFRONT = 1 };
 /* SYNC: the lanes of a wave run each pass together.  REGEN: a lane whose path
    ends starts its pixel's next pass at once.  Both produce the same bits. */
 enum { PT_SCHEDULE_SYNC = 0, PT_SCHEDULE_REGEN = 1 };
+/* REFERENCE: the reference's child-pair tree in its visit order; hits and the
+   per-hit node counts match it bit for bit.  WIDE: the same boxes regrouped
+   into 4-wide nodes, nearest child first; the same closest hit except where
+   two candidates tie within rounding (then the visit order picks one), and
+   node counts are of wide nodes.  Work counters (PT_OPT_COUNT_STATS) and
+   pt_trace_hits always use REFERENCE. */
+enum { PT_TRAVERSAL_REFERENCE = 0, PT_TRAVERSAL_WIDE = 1 };
 
 /* Lifetime: `new Renderer` + Renderer.init (tracer.js:1719-1770). */
 int pt_create(pt_ctx** out, int device);

@@ -1,0 +1,193 @@
+"""The kernels' traversal code compiled for the CPU (tests/native/
+host_traverse.cpp over dev/traverse.h and pt_layout.cpp), checked against
+the oracle without a GPU:
+
+* the reference-order traversal over the child-pair nodes must reproduce the
+  oracle's hits bit for bit, node-pop counts included;
+* the wide traversal must find the same closest hit, except where the FP64
+  arbiter finds two candidates tied, and the same shadow answers;
+* the wide stack bound computed at upload must never drop a push.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import compare_hits, hits_to_np, random_rays, small_mesh_scene
+from oracle.refcpu import RefScene
+from webgpuraytracer_amd import SceneBuilder
+from webgpuraytracer_amd import _abi as A
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "webgpuraytracer_amd", "csrc")
+
+
+@pytest.fixture(scope="module")
+def ht(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("host_traverse") / "libhosttraverse.so")
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+           "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
+           os.path.join(ROOT, "tests", "native", "host_traverse.cpp"), os.path.join(CSRC, "pt_layout.cpp"), "-o", out]
+    subprocess.run(cmd, check=True, capture_output=True)
+    L = C.CDLL(out)
+    L.ht_layout_info.argtypes = [C.c_void_p, C.POINTER(C.c_uint32)]
+    L.ht_wide_nodes.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.ht_wide_nodes.restype = C.c_long
+    L.ht_trace_rays.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_uint32, C.c_void_p]
+    L.ht_trace_shadow_rays.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int,
+                                       C.c_void_p]
+    return L
+
+
+def _trace(ht, desc, rays, wide, cap=0):
+    r = np.ascontiguousarray(rays, np.float32)
+    out = (A.Hit * r.shape[0])()
+    assert ht.ht_trace_rays(C.byref(desc), r.ctypes.data, r.shape[0], int(wide), cap, out) == 0
+    return out
+
+
+def _info(ht, desc):
+    info = (C.c_uint32 * 8)()
+    assert ht.ht_layout_info(C.byref(desc), info) == 0
+    keys = ("tlas_cnodes", "blas_cnodes", "tlas_wnodes", "blas_wnodes", "stack_cap", "wstack_cap", "tlas_depth",
+            "blas_depth")
+    return dict(zip(keys, list(info)))
+
+
+def _camera_rays(scene, w, h):
+    """Pixel-centre primary rays of the scene camera (shader.wgsl:2284-2287)."""
+    p = scene.params(w, h, 0, 1)
+    r00, r10, r01, r11 = (np.array(list(getattr(p, k)), np.float64) for k in ("ray00", "ray10", "ray01", "ray11"))
+    eye = np.array(list(p.eye), np.float32)
+    rays = []
+    for y in range(h):
+        for x in range(w):
+            u, v = (x + 0.5) / w, (y + 0.5) / h
+            top, bot = r00 + (r10 - r00) * u, r01 + (r11 - r01) * u
+            d = top + (bot - top) * (1 - v)
+            d = d / np.linalg.norm(d)
+            rays.append(list(eye) + list(d.astype(np.float32)))
+    return np.array(rays, np.float32)
+
+
+SCENES = {
+    "c1": lambda: SceneBuilder.config(1, seed=11),
+    "mesh": lambda: small_mesh_scene(seed=5),
+}
+
+
+@pytest.mark.parametrize("which", ["c1", "mesh"])
+def test_reference_order_matches_oracle_bitwise(ht, which):
+    scene = SCENES[which]()
+    desc = scene.desc()
+    ref = RefScene(desc)
+    rays = np.concatenate([random_rays(6000, seed=42, box=4.0 if which == "c1" else 9.0,
+                                       spread=3.0 if which == "c1" else 8.0), _camera_rays(scene, 48, 32)])
+    g = hits_to_np(_trace(ht, desc, rays, wide=False))
+    r = hits_to_np(ref.trace_rays(rays)[0])
+    for k in ("o", "m", "prim", "plane", "count"):
+        assert np.array_equal(g[k], r[k]), k
+    assert np.array_equal(g["t"].view(np.uint32), r["t"].view(np.uint32))
+
+
+@pytest.mark.parametrize("which", ["c1", "mesh"])
+def test_wide_matches_oracle(ht, which):
+    scene = SCENES[which]()
+    desc = scene.desc()
+    ref = RefScene(desc)
+    rays = np.concatenate([random_rays(6000, seed=7, box=4.0 if which == "c1" else 9.0,
+                                       spread=3.0 if which == "c1" else 8.0), _camera_rays(scene, 48, 32)])
+    g = _trace(ht, desc, rays, wide=True)
+    r, _ = ref.trace_rays(rays)
+    s = compare_hits(g, r, rays, ref)
+    print(which, s)
+    assert s["exact_t"] >= 0.999 * s["n"]
+
+
+def test_wide_matches_oracle_large_mesh_scene(ht):
+    """The benchmark scene (808,500 triangles, 64 instances), 3000 rays."""
+    scene = SceneBuilder.config(2, seed=7, sky=(64, 32))
+    desc = scene.desc()
+    ref = RefScene(desc)
+    rays = random_rays(3000, seed=3, box=9.0, spread=8.0)
+    g = _trace(ht, desc, rays, wide=True)
+    r, _ = ref.trace_rays(rays)
+    s = compare_hits(g, r, rays, ref)
+    info = _info(ht, desc)
+    print(s, info)
+    assert s["exact_t"] >= 0.999 * s["n"]
+    # two levels per wide node: about half the depth, at most 3 pushes a level
+    assert info["wstack_cap"] <= 3 * (info["tlas_depth"] + info["blas_depth"]) + 2
+
+
+@pytest.mark.parametrize("which", ["c1", "mesh"])
+def test_wide_shadow_equals_reference(ht, which):
+    scene = SCENES[which]()
+    desc = scene.desc()
+    ref = RefScene(desc)
+    rays = random_rays(4000, seed=9, box=4.0 if which == "c1" else 9.0, spread=3.0 if which == "c1" else 8.0)
+    rng = np.random.default_rng(1)
+    n_obj = desc.objects.count
+    target = rng.integers(-1, max(n_obj, 1), size=len(rays)).astype(np.int32)
+    dist = rng.uniform(0.5, 20.0, size=len(rays)).astype(np.float32)
+    occ = {}
+    for wide in (0, 1):
+        o = np.zeros(len(rays), np.uint8)
+        assert ht.ht_trace_shadow_rays(C.byref(desc), rays.ctypes.data, target.ctypes.data, dist.ctypes.data,
+                                       len(rays), wide, o.ctypes.data) == 0
+        occ[wide] = o
+    oracle = ref.trace_shadow_rays(rays, target, dist)[:, 0] == 0.0
+    assert np.array_equal(occ[0].astype(bool), oracle)
+    assert np.array_equal(occ[1], occ[0])
+    assert 0.05 < oracle.mean() < 0.95
+
+
+@pytest.mark.parametrize("which", ["c1", "mesh"])
+def test_wide_stack_bound_drops_nothing(ht, which):
+    """Results with the computed stack capacity equal those with a 128-entry
+    stack (a dropped push would lose a subtree)."""
+    scene = SCENES[which]()
+    desc = scene.desc()
+    info = _info(ht, desc)
+    rays = random_rays(6000, seed=21, box=4.0 if which == "c1" else 9.0, spread=3.0 if which == "c1" else 8.0)
+    a = hits_to_np(_trace(ht, desc, rays, wide=True))
+    b = hits_to_np(_trace(ht, desc, rays, wide=True, cap=128))
+    for k in ("o", "m", "prim", "plane", "t"):
+        assert np.array_equal(a[k], b[k]), k
+    assert 2 <= info["wstack_cap"] <= 128
+
+
+def test_wide_nodes_regroup_the_reference_tree(ht):
+    """Every wide node has 2-4 children, unused slots hold the empty +inf
+    box, and the wide trees reach exactly the reference trees' leaves."""
+    scene = small_mesh_scene(seed=5)
+    desc = scene.desc()
+    info = _info(ht, desc)
+    wt = np.zeros((info["tlas_wnodes"], 32), np.uint32)
+    wb = np.zeros((info["blas_wnodes"], 32), np.uint32)
+    n = ht.ht_wide_nodes(C.byref(desc), wt.ctypes.data if len(wt) else None, wb.ctypes.data if len(wb) else None)
+    assert n == len(wt) + len(wb)
+
+    def leaves(nodes):
+        cnt = nodes[:, 28]
+        assert np.all((cnt >= 2) & (cnt <= 4))
+        boxes = nodes[:, :24].view(np.float32).reshape(-1, 6, 4)
+        refs = nodes[:, 24:28]
+        total = 0
+        for i in range(len(nodes)):
+            c = int(cnt[i])
+            assert np.all(np.isinf(boxes[i, :, c:])), i
+            total += int(np.sum(refs[i, :c] >> 31))
+        return total
+
+    tlas = np.ctypeslib.as_array((C.c_uint8 * (desc.tlas.count * 32)).from_address(desc.tlas.ptr)).view(np.uint32)
+    tlas = tlas.reshape(-1, 8)
+    bvh = np.ctypeslib.as_array((C.c_uint8 * (desc.bvh.count * 32)).from_address(desc.bvh.ptr)).view(np.uint32)
+    bvh = bvh.reshape(-1, 8)
+    # node layout (32 B): aabb_min vec3, count u32, aabb_max vec3, next u32
+    tlas_leaves = int(np.sum(tlas[:, 3] > 0))
+    bvh_leaves = int(np.sum(bvh[:, 3] > 0))
+    assert leaves(wt) == tlas_leaves
+    assert leaves(wb) == bvh_leaves  # each distinct BLAS converted once

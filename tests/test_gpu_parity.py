@@ -28,8 +28,20 @@ def _setup(renderer, scene, w, h, gbuf=False):
     return RefScene(scene.desc())
 
 
+@pytest.fixture
+def traversal(renderer, request):
+    """Runs a test under one BVH traversal and restores the default."""
+    renderer.set_option(A.PT_OPT_TRAVERSAL, request.param)
+    yield request.param
+    renderer.set_option(A.PT_OPT_TRAVERSAL, A.PT_TRAVERSAL_WIDE)
+
+
+TRAVERSALS = [A.PT_TRAVERSAL_REFERENCE, A.PT_TRAVERSAL_WIDE]
+
+
+@pytest.mark.parametrize("traversal", TRAVERSALS, indirect=True, ids=["reference", "wide"])
 @pytest.mark.parametrize("which", ["c1", "mesh"])
-def test_trace_rays_match_oracle(renderer, which):
+def test_trace_rays_match_oracle(renderer, which, traversal):
     scene = SceneBuilder.config(1, seed=11) if which == "c1" else small_mesh_scene(seed=5)
     ref = _setup(renderer, scene, 16, 16)
     rays = random_rays(20000, seed=42, box=4.0 if which == "c1" else 9.0, spread=3.0 if which == "c1" else 8.0)
@@ -38,6 +50,8 @@ def test_trace_rays_match_oracle(renderer, which):
     s = compare_hits(g, r, rays, ref)
     print(which, s)
     assert s["exact_t"] >= 0.999 * s["n"]
+    if traversal == A.PT_TRAVERSAL_REFERENCE:  # the reference's visit order: its node counts too
+        assert np.array_equal(hits_to_np(g)["count"], hits_to_np(r)["count"])
 
 
 @pytest.mark.parametrize("which", ["c1", "mesh"])
@@ -69,7 +83,8 @@ def test_primary_secondary_hits_match_oracle(renderer, which):
     assert np.mean((ra["t"][sec] >= 0) == (ga["t"][sec] >= 0)) >= 0.99
 
 
-def test_shadow_rays_match_oracle(renderer):
+@pytest.mark.parametrize("traversal", TRAVERSALS, indirect=True, ids=["reference", "wide"])
+def test_shadow_rays_match_oracle(renderer, traversal):
     scene = SceneBuilder.config(1, seed=13)
     ref = _setup(renderer, scene, 16, 16)
     rays = random_rays(20000, seed=7, box=4.0, spread=3.0)
@@ -224,3 +239,27 @@ def test_backends_and_schedules_bitwise(renderer, which):
         assert np.array_equal(out_m, out_o), name
         for k in ("extension_rays", "shadow_rays", "paths", "node_pops", "inner_pops", "tri_tests", "shade_events"):
             assert getattr(st_m, k) == getattr(st_o, k), (name, k)
+
+
+@pytest.mark.parametrize("which", ["c1", "mesh"])
+def test_wide_traversal_renders_like_reference_order(renderer, which):
+    """The wide traversal picks a different closest hit only on ties within
+    rounding; on these scenes a 5-pass render is pixel-identical to the
+    reference-order traversal's on at least 99.9% of pixels."""
+    scene = SceneBuilder.config(1, seed=82) if which == "c1" else small_mesh_scene(seed=14)
+    w, h = 96, 64
+    renderer.set_scene(scene)
+    seeds = [5, 6, 7, 8, 9]
+    p = scene.params(w, h, sample_number=0, seed=seeds[0])
+    acc = {}
+    try:
+        for t in TRAVERSALS:
+            renderer.set_option(A.PT_OPT_TRAVERSAL, t)
+            renderer.resize(w, h)
+            renderer.render_section(p, passes=5, seeds=seeds)
+            acc[t] = renderer.read_accum()
+    finally:
+        renderer.set_option(A.PT_OPT_TRAVERSAL, A.PT_TRAVERSAL_WIDE)
+    same = np.all(acc[0].view(np.uint32) == acc[1].view(np.uint32), axis=-1)
+    print(which, "identical pixels", same.mean())
+    assert same.mean() >= 0.999

@@ -25,7 +25,7 @@ namespace pt {
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kIndirectBit = 0x40000000u;
 
-struct __align__(16) CNode {
+struct alignas(16) CNode {
     float4 a;  // left.min.xyz, left.max.x
     float4 b;  // left.max.yz, right.min.xy
     float4 c;  // right.min.z, right.max.xyz
@@ -33,11 +33,23 @@ struct __align__(16) CNode {
 };
 static_assert(sizeof(CNode) == 64, "CNode is 64 B");
 
-struct __align__(16) Root {
+// 4-wide node for the wide traversal (PT_TRAVERSAL_WIDE): two levels of
+// the reference tree collapsed into one node, so a ray takes about half the
+// dependent node fetches.  Every child box is a box of the reference tree;
+// child refs use the encoding above, inner refs indexing WNodes.  Child i
+// sits in component i of each field; 128 B, one L2 line.
+struct alignas(16) WNode {
+    float4 lox, loy, loz, hix, hiy, hiz;
+    uint4 ref;
+    uint4 meta;  // x: number of children (2..4)
+};
+static_assert(sizeof(WNode) == 128, "WNode is 128 B");
+
+struct alignas(16) Root {
     float mn[3];
-    uint32_t ref;
+    uint32_t ref;   // CNode tree
     float mx[3];
-    uint32_t pad;
+    uint32_t wref;  // WNode tree
 };
 static_assert(sizeof(Root) == 32, "Root is 32 B");
 
@@ -47,6 +59,8 @@ struct DevScene {
     const uint32_t* obj_root;      // per object: Root slot of its BLAS (meshes)
     const CNode* tnodes;
     const CNode* bnodes;
+    const WNode* wtnodes;
+    const WNode* wbnodes;
     const Root* roots;             // slot 0: TLAS; then one per distinct BLAS root
     const pt_bvh_node* tlas_raw;
     const pt_bvh_node* bvh_raw;
@@ -62,7 +76,8 @@ struct DevScene {
     uint32_t sky_w, sky_h;
     uint32_t flags;
     int32_t bounce_limit;
-    uint32_t stack_cap;
+    uint32_t stack_cap;            // entries per lane, reference-order traversal
+    uint32_t wstack_cap;           // entries per lane, wide traversal
 };
 
 // Compile-time flag policy: F == kDynamicFlags reads the uploaded flags at
@@ -217,7 +232,7 @@ __device__ __forceinline__ float cone_cylinder(v3 o, v3 d, float r0, float r1, i
     return which >= 0 ? tmin : -1.0f;
 }
 // hit_torus, shader.wgsl:381-471 (major radius Ra, minor ra, unit direction)
-__device__ __noinline__ float torus_t(v3 o, v3 d, float Ra, float ra) {
+static __device__ __noinline__ float torus_t(v3 o, v3 d, float Ra, float ra) {
     float po = 1.0f;
     const float Ra2 = Ra * Ra, ra2 = ra * ra;
     const float m = dot(o, o), n = dot(o, d);
@@ -345,7 +360,7 @@ __device__ __forceinline__ float cubic_root(float a, float b, float c) {
     if (aa != 0.0f) bb = q / aa;
     return (aa + bb) - a / 3.0f;
 }
-__device__ __noinline__ bool torus_blocks(v3 wo, v3 wd, const float* inv, float rminor, float target) {
+static __device__ __noinline__ bool torus_blocks(v3 wo, v3 wd, const float* inv, float rminor, float target) {
     v3 o = xform_point(inv, wo), d = xform_dir(inv, wd);
     const float rs = len(d);
     d = d / rs;

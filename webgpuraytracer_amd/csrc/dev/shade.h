@@ -15,7 +15,7 @@ __device__ __forceinline__ v3 sky_texel(const DevScene& S, int x, int y) {
     return mk(float(r), float(g), float(b));
 }
 // textureSampleLevel on the sky: bilinear, fp32 weights, clamp-to-edge.
-__device__ v3 sky_bilinear(const DevScene& S, float u, float v) {
+static __device__ v3 sky_bilinear(const DevScene& S, float u, float v) {
     const float W = float(S.sky_w), H = float(S.sky_h);
     float x = u * W - 0.5f, y = v * H - 0.5f;
     x = mn(mx(x, -1.0f), W);
@@ -32,7 +32,7 @@ __device__ v3 sky_bilinear(const DevScene& S, float u, float v) {
     return top * (1.0f - ay) + bot * ay;
 }
 // sample_sky, shader.wgsl:176-188
-__device__ v3 sample_sky(const DevScene& S, v3 dir) {
+static __device__ v3 sample_sky(const DevScene& S, v3 dir) {
     const float phi = atan2f(dir.z, dir.x);
     const float th = asinf(clampf(dir.y, -1.0f, 1.0f));
     return sky_bilinear(S, 0.5f + phi / kTWO_PI, 0.5f - th / kPI);
@@ -53,7 +53,7 @@ struct Params {
 };
 
 // get_sky_pdf, shader.wgsl:208-229
-__device__ float sky_pdf(const DevScene& S, const Params& P, v3 dir) {
+static __device__ float sky_pdf(const DevScene& S, const Params& P, v3 dir) {
     const float phi = atan2f(dir.z, dir.x);
     const float th = acosf(clampf(dir.y, -1.0f, 1.0f));
     const float u = (phi + kPI) / (2.0f * kPI);
@@ -68,7 +68,7 @@ __device__ float sky_pdf(const DevScene& S, const Params& P, v3 dir) {
 struct LightSample { v3 dir, color; float dist, pdf; };
 
 // sample_env_cdf, shader.wgsl:231-254
-__device__ LightSample sample_env(const DevScene& S, const Params& P, float ux, float uy) {
+static __device__ LightSample sample_env(const DevScene& S, const Params& P, float ux, float uy) {
     LightSample ls;
     const uint32_t vi = cdf_search(S.cdf, 0u, P.sky_height, uy);
     const uint32_t row = P.sky_cdf_index + vi * (P.sky_width + 1u);
@@ -87,7 +87,7 @@ __device__ LightSample sample_env(const DevScene& S, const Params& P, float ux, 
 
 // --------------------------------------------------------------- lights ----
 // light_sphere_pdf, shader.wgsl:1473-1499
-__device__ float sphere_light_pdf(const pt_light& L, const pt_object& sph, v3 hp, v3 dir) {
+static __device__ float sphere_light_pdf(const pt_light& L, const pt_object& sph, v3 hp, v3 dir) {
     const v3 d = ld3(sph.world_position) - hp;
     const float d2 = dot(d, d);
     const float r2 = L.scale * L.scale;
@@ -109,7 +109,7 @@ __device__ __forceinline__ void tangent_space(v3 N, v3& T, v3& B) {
 }
 
 // sample_light_sphere, shader.wgsl:1501-1549
-__device__ LightSample sphere_light_sample(const DevScene& S, const pt_light& L, const pt_object& sph, v3 hp,
+static __device__ LightSample sphere_light_sample(const DevScene& S, const pt_light& L, const pt_object& sph, v3 hp,
                                            float x0, float x1) {
     const v3 d = ld3(sph.world_position) - hp;
     const float d2 = dot(d, d);
@@ -156,7 +156,7 @@ __device__ __forceinline__ void box_areas(const pt_light& L, v3 hp, float s[3], 
 }
 
 // light_box_pdf, shader.wgsl:1551-1573
-__device__ float box_light_pdf(const pt_light& L, v3 hp, v3 rd, float hit_t, v3 hit_n) {
+static __device__ float box_light_pdf(const pt_light& L, v3 hp, v3 rd, float hit_t, v3 hit_n) {
     float s[3], a[3];
     box_areas(L, hp, s, a);
     const float total = a[0] + a[1] + a[2];
@@ -165,7 +165,7 @@ __device__ float box_light_pdf(const pt_light& L, v3 hp, v3 rd, float hit_t, v3 
 }
 
 // sample_light_box, shader.wgsl:1575-1624 (draws the face pick itself)
-__device__ LightSample box_light_sample(const DevScene& S, const pt_light& L, const pt_object& ob, v3 hp, float x0,
+static __device__ LightSample box_light_sample(const DevScene& S, const pt_light& L, const pt_object& ob, v3 hp, float x0,
                                         float x1, Pcg& rng) {
     const v3 v1 = ld3(L.matrix), v2 = ld3(L.matrix + 4), v3_ = ld3(L.matrix + 8), cen = ld3(L.matrix + 12);
     float s[3], a[3];
@@ -248,7 +248,7 @@ __device__ __forceinline__ float d_ggx(float nh, float a2) {
     const float den = (nh * nh * (a2 - 1.0f) + 1.0f);
     return a2 / (kPI * den * den + 1e-10f);
 }
-__device__ float fresnel_diel(float ci, float etai, float etat) {
+static __device__ float fresnel_diel(float ci, float etai, float etat) {
     const float cosi = clampf(fabsf(ci), 0.0f, 1.0f);
     const float sint = (etai / etat) * sqrtf(mx(0.0f, 1.0f - cosi * cosi));
     if (sint >= 1.0f) return 1.0f;
@@ -273,7 +273,7 @@ __device__ __forceinline__ float g1_div_nv(float nv, float a2) {
 __device__ __forceinline__ float pow5(float x) { return powf(x, 5.0f); }
 
 // ImportanceSampleCosine, shader.wgsl:1751-1763
-__device__ v3 cosine_sample(float x0, float x1, v3 N) {
+static __device__ v3 cosine_sample(float x0, float x1, v3 N) {
     const float phi = kTWO_PI * x0;
     const float ct = sqrtf(clampf(x1, 0.0f, 1.0f));
     const float st = sqrtf(mx(0.0f, 1.0f - x1));
@@ -284,7 +284,7 @@ __device__ v3 cosine_sample(float x0, float x1, v3 N) {
     return normalize(T * L.x + B * L.y + N * L.z);
 }
 // ImportanceSampleVNDF_GGX, shader.wgsl:1806-1829
-__device__ v3 vndf_sample(float x0, float x1, v3 V, v3 N, float ag) {
+static __device__ v3 vndf_sample(float x0, float x1, v3 V, v3 N, float ag) {
     const v3 up = fabsf(N.y) > 0.999f ? mk(0.f, 0.f, 1.f) : mk(0.f, 1.f, 0.f);
     const v3 T = normalize(cross(up, N));
     const v3 B = cross(N, T);
@@ -310,7 +310,7 @@ __device__ __forceinline__ void iors(bool entering, const pt_material& m, const 
 }
 
 // eval_surface, shader.wgsl:1831-1947
-__device__ v3 eval_bsdf(v3 V, v3 L, const pt_material& m, const Surface& sc, const Media& md) {
+static __device__ v3 eval_bsdf(v3 V, v3 L, const pt_material& m, const Surface& sc, const Media& md) {
     const bool entering = dot(sc.normal, V) > 0.0f;
     const v3 no = entering ? sc.normal : -sc.normal;
     const float nv = dot(no, V);
@@ -390,7 +390,7 @@ struct Lobes { float cc, spec, trans, diff; };
 
 // Lobe weights of pdf_surface (shader.wgsl:1975-1987) and sample_surface
 // (shader.wgsl:2078-2090), evaluated at cosine cv.
-__device__ Lobes lobes(float cv, bool entering, float etai, float etat, float rv, const pt_material& m, const Surface& sc) {
+static __device__ Lobes lobes(float cv, bool entering, float etai, float etat, float rv, const pt_material& m, const Surface& sc) {
     const float Fcc = fresnel_diel(cv, 1.0f, m.clearcoat_ior);
     const float pcc = entering ? m.clearcoat * Fcc : 0.0f;
     const float Fr = fresnel_diel(cv, etai, etat);
@@ -408,7 +408,7 @@ __device__ Lobes lobes(float cv, bool entering, float etai, float etat, float rv
 }
 
 // pdf_surface, shader.wgsl:1952-2047
-__device__ float pdf_bsdf(v3 V, v3 L, const pt_material& m, const Surface& sc, const Media& md) {
+static __device__ float pdf_bsdf(v3 V, v3 L, const pt_material& m, const Surface& sc, const Media& md) {
     const bool entering = dot(sc.normal, V) > 0.0f;
     const v3 no = entering ? sc.normal : -sc.normal;
     const float nv = dot(no, V);
@@ -472,7 +472,7 @@ __device__ __forceinline__ v3 medium_sigma(const Surface& sc, const pt_material&
 }
 
 // sample_surface, shader.wgsl:2052-2175.  Returns false when the path ends.
-__device__ bool sample_bsdf(v3& ro, v3& rd, v3& tp, const pt_material& m, const Surface& sc, v3 hp, Media& md,
+static __device__ bool sample_bsdf(v3& ro, v3& rd, v3& tp, const pt_material& m, const Surface& sc, v3 hp, Media& md,
                             float& last_pdf, Pcg& rng) {
     const v3 V = -rd;
     const bool entering = dot(sc.normal, V) > 0.0f;

@@ -6,9 +6,19 @@
 // Here both levels run in one flattened loop over one LDS stack: each
 // iteration does one unit of work (pop a TLAS node, intersect one TLAS-leaf
 // object, or pop a BLAS node), so lanes of a wave stay in the same loop body
-// whichever level they are at.  The per-lane visit order, the culling against
-// the running closest t / target distance, the strict `<` ties and the
-// per-pop counter are exactly the reference's.
+// whichever level they are at.
+//
+// Two traversals share the loop (template parameter W):
+//   * reference order (W = false, child-pair nodes): the per-lane visit
+//     order, the culling against the running closest t / target distance,
+//     the strict `<` ties and the per-pop counter are exactly the
+//     reference's, so hits match it bit for bit;
+//   * wide (W = true, 4-wide nodes): children visited nearest first.  Every
+//     box and triangle test is the reference's arithmetic, so the closest
+//     hit is the same except where two candidates tie to within the
+//     rounding of the box and triangle tests, where the visit order picks
+//     the winner.  Shadow rays give the same answer either way (any
+//     occluder ends the query).
 #pragma once
 #include "geometry.h"
 
@@ -27,7 +37,7 @@ struct HitRec {
     float bu, bv;
 };
 
-__device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ uint32_t f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
 
 // Inner-node visit.  The reference pushes the accepted children far-first
 // and pops the near one at once (shader.wgsl:533-539, 743-749); here the child
@@ -57,8 +67,62 @@ __device__ __forceinline__ bool visit_inner(const CNode* nodes, uint32_t ref, v3
     return bot_ok;
 }
 
-// BLAS-only closest-hit traversal (list meshes, shader.wgsl:767-771).
+__device__ __forceinline__ void order2(float& ka, uint32_t& ra, float& kb, uint32_t& rb) {
+    const bool sw = kb < ka;
+    const float k = sw ? kb : ka;
+    const uint32_t r = sw ? rb : ra;
+    kb = sw ? ka : kb;
+    rb = sw ? ra : rb;
+    ka = k;
+    ra = r;
+}
+
+// Wide-node visit: all children's slabs from one 128 B fetch, accepted
+// children sorted by entry distance (a 4-input sorting network; rejected
+// ones sort last as +inf); the nearest is returned in `next`, the others are
+// pushed farthest first.  Returns false when no child is accepted.
 template <bool CN>
+__device__ __forceinline__ bool visit_wide(const WNode* nodes, uint32_t ref, v3 o, v3 inv, float limit, LdsStack& st,
+                                           Counters& c, uint32_t& next) {
+    if (CN) c.inner++;
+    const WNode* p = nodes + ref;
+    const float4 lx = p->lox, ly = p->loy, lz = p->loz, hx = p->hix, hy = p->hiy, hz = p->hiz;
+    const uint4 r = p->ref;
+    const uint32_t cnt = p->meta.x;
+    const float inf = __builtin_inff();
+    float k0 = slab(o, inv, mk(lx.x, ly.x, lz.x), mk(hx.x, hy.x, hz.x));
+    float k1 = slab(o, inv, mk(lx.y, ly.y, lz.y), mk(hx.y, hy.y, hz.y));
+    float k2 = slab(o, inv, mk(lx.z, ly.z, lz.z), mk(hx.z, hy.z, hz.z));
+    float k3 = slab(o, inv, mk(lx.w, ly.w, lz.w), mk(hx.w, hy.w, hz.w));
+    k0 = (k0 >= 0.0f && k0 < limit) ? k0 : inf;
+    k1 = (k1 >= 0.0f && k1 < limit) ? k1 : inf;
+    k2 = (cnt > 2u && k2 >= 0.0f && k2 < limit) ? k2 : inf;
+    k3 = (cnt > 3u && k3 >= 0.0f && k3 < limit) ? k3 : inf;
+    uint32_t r0 = r.x, r1 = r.y, r2 = r.z, r3 = r.w;
+    order2(k0, r0, k1, r1);
+    order2(k2, r2, k3, r3);
+    order2(k0, r0, k2, r2);
+    order2(k1, r1, k3, r3);
+    order2(k1, r1, k2, r2);
+    if (k3 != inf) st.push(r3);
+    if (k2 != inf) st.push(r2);
+    if (k1 != inf) st.push(r1);
+    next = r0;
+    return k0 != inf;
+}
+
+// One inner visit of either tree: W selects the wide nodes.
+template <bool W, bool CN>
+__device__ __forceinline__ bool visit_node(const CNode* cn, const WNode* wn, uint32_t ref, v3 o, v3 inv, float limit,
+                                           LdsStack& st, Counters& c, uint32_t& next) {
+    if (W) return visit_wide<CN>(wn, ref, o, inv, limit, st, c, next);
+    return visit_inner<CN>(cn, ref, o, inv, limit, st, c, next);
+}
+template <bool W>
+__device__ __forceinline__ uint32_t root_ref(const Root& r) { return W ? r.wref : r.ref; }
+
+// BLAS-only closest-hit traversal (list meshes, shader.wgsl:767-771).
+template <bool CN, bool W>
 __device__ void blas_closest(const DevScene& S, uint32_t root_slot, const float* inv_m, uint32_t tri_off, v3 o, v3 d,
                              int idx, int kind, LdsStack& st, HitRec& h, Counters& c) {
     const v3 lo = xform_point(inv_m, o), ld = xform_dir(inv_m, d);
@@ -67,7 +131,7 @@ __device__ void blas_closest(const DevScene& S, uint32_t root_slot, const float*
     const float bt = slab(lo, linv, ld3(R.mn), ld3(R.mx));
     if (bt < 0.0f || bt >= h.t) return;
     st.sp = 0;
-    uint32_t ref = R.ref;
+    uint32_t ref = root_ref<W>(R);
     for (;;) {
         h.count++;
         if (CN) c.pops++;
@@ -82,7 +146,7 @@ __device__ void blas_closest(const DevScene& S, uint32_t root_slot, const float*
                 if (t >= 0.0f && t < h.t) { h.t = t; h.kind = kind; h.idx = idx; h.prim = int(k); h.bu = u; h.bv = v; }
             }
         } else {
-            have = visit_inner<CN>(S.bnodes, ref, lo, linv, h.t, st, c, ref);
+            have = visit_node<W, CN>(S.bnodes, S.wbnodes, ref, lo, linv, h.t, st, c, ref);
         }
         if (!have) {
             if (st.sp == 0) break;
@@ -94,7 +158,7 @@ __device__ void blas_closest(const DevScene& S, uint32_t root_slot, const float*
 // trace_scene (shader.wgsl:754-774): planes, then the TLAS, then list meshes.
 // The two BVH levels run in one loop: `ref` is the node visited next, or
 // have == false when the next work is a TLAS-leaf object or a stack pop.
-template <uint32_t F, bool CN>
+template <uint32_t F, bool CN, bool W>
 __device__ void closest_hit(const DevScene& S, Flags<F> fl, v3 o, v3 d, LdsStack& st, HitRec& h, Counters& c) {
     h.t = 1e10f;
     h.count = 0;
@@ -115,7 +179,7 @@ __device__ void closest_hit(const DevScene& S, Flags<F> fl, v3 o, v3 d, LdsStack
         const float bt = slab(o, inv, ld3(R.mn), ld3(R.mx));
         if (!(bt < 0.0f || bt >= h.t)) {
             st.sp = 0;
-            uint32_t ref = R.ref;
+            uint32_t ref = root_ref<W>(R);
             bool have = true;
             bool in_blas = false;
             int blas_base = 0;
@@ -143,7 +207,7 @@ __device__ void closest_hit(const DevScene& S, Flags<F> fl, v3 o, v3 d, LdsStack
                                 obj = int(i);
                                 blas_base = st.sp;
                                 in_blas = true;
-                                ref = br.ref;
+                                ref = root_ref<W>(br);
                                 have = true;
                             }
                         } else if (fl.has(PT_HAS_SPHERES) && type == PT_OBJ_SPHERE) {
@@ -196,9 +260,9 @@ __device__ void closest_hit(const DevScene& S, Flags<F> fl, v3 o, v3 d, LdsStack
                         end = s0 + n0;
                     }
                 } else if (in_blas) {
-                    have = visit_inner<CN>(S.bnodes, ref, lo, linv, h.t, st, c, ref);
+                    have = visit_node<W, CN>(S.bnodes, S.wbnodes, ref, lo, linv, h.t, st, c, ref);
                 } else {
-                    have = visit_inner<CN>(S.tnodes, ref, o, inv, h.t, st, c, ref);
+                    have = visit_node<W, CN>(S.tnodes, S.wtnodes, ref, o, inv, h.t, st, c, ref);
                 }
             }
         }
@@ -206,12 +270,12 @@ __device__ void closest_hit(const DevScene& S, Flags<F> fl, v3 o, v3 d, LdsStack
     if (fl.has(PT_HAS_LIST_MESHES)) {
         for (uint32_t m = 0; m < S.n_meshes; ++m) {
             const pt_mesh_instance& mi = S.meshes[m];
-            blas_closest<CN>(S, S.mesh_root[m], mi.inv_matrix, mi.tri_offset, o, d, int(m), kListMeshHit, st, h, c);
+            blas_closest<CN, W>(S, S.mesh_root[m], mi.inv_matrix, mi.tri_offset, o, d, int(m), kListMeshHit, st, h, c);
         }
     }
 }
 
-template <bool CN>
+template <bool CN, bool W>
 __device__ bool blas_any(const DevScene& S, uint32_t root_slot, const float* inv_m, uint32_t tri_off, v3 o, v3 d,
                          float tdist, LdsStack& st, Counters& c) {
     const v3 lo = xform_point(inv_m, o), ld = xform_dir(inv_m, d);
@@ -220,7 +284,7 @@ __device__ bool blas_any(const DevScene& S, uint32_t root_slot, const float* inv
     const float bt = slab(lo, linv, ld3(R.mn), ld3(R.mx));
     if (bt < 0.0f || bt >= tdist) return false;
     st.sp = 0;
-    uint32_t ref = R.ref;
+    uint32_t ref = root_ref<W>(R);
     for (;;) {
         if (CN) c.pops++;
         bool have = false;
@@ -234,7 +298,7 @@ __device__ bool blas_any(const DevScene& S, uint32_t root_slot, const float* inv
                 if (t >= 0.0f && t < tdist) return true;
             }
         } else {
-            have = visit_inner<CN>(S.bnodes, ref, lo, linv, tdist, st, c, ref);
+            have = visit_node<W, CN>(S.bnodes, S.wbnodes, ref, lo, linv, tdist, st, c, ref);
         }
         if (!have) {
             if (st.sp == 0) break;
@@ -247,7 +311,7 @@ __device__ bool blas_any(const DevScene& S, uint32_t root_slot, const float* inv
 // trace_scene_shadow (shader.wgsl:1206-1229).  Without albedo textures
 // every occluder blocks outright (evaluate_shadow_attenuation returns true,
 // shader.wgsl:777-792), so the transmittance is 0 or 1.
-template <uint32_t F, bool CN>
+template <uint32_t F, bool CN, bool W>
 __device__ bool occluded(const DevScene& S, Flags<F> fl, v3 o, v3 d, int target, float tdist, LdsStack& st,
                          Counters& c) {
     if (fl.has(PT_HAS_PLANES)) {
@@ -264,7 +328,7 @@ __device__ bool occluded(const DevScene& S, Flags<F> fl, v3 o, v3 d, int target,
     if (fl.has(kAnyObject)) {
         const v3 inv = splat(1.0f) / d;
         st.sp = 0;
-        uint32_t ref = S.roots[0].ref;  // no root box test here (shader.wgsl:1134-1144)
+        uint32_t ref = root_ref<W>(S.roots[0]);  // no root box test here (shader.wgsl:1134-1144)
         bool have = true;
         bool in_blas = false;
         int blas_base = 0;
@@ -291,7 +355,7 @@ __device__ bool occluded(const DevScene& S, Flags<F> fl, v3 o, v3 d, int target,
                             tri_off = f2u(ob.param1);
                             blas_base = st.sp;
                             in_blas = true;
-                            ref = br.ref;
+                            ref = root_ref<W>(br);
                             have = true;
                         }
                     } else if (fl.has(PT_HAS_SPHERES) && type == PT_OBJ_SPHERE) {
@@ -328,16 +392,16 @@ __device__ bool occluded(const DevScene& S, Flags<F> fl, v3 o, v3 d, int target,
                     end = s0 + n0;
                 }
             } else if (in_blas) {
-                have = visit_inner<CN>(S.bnodes, ref, lo, linv, tdist, st, c, ref);
+                have = visit_node<W, CN>(S.bnodes, S.wbnodes, ref, lo, linv, tdist, st, c, ref);
             } else {
-                have = visit_inner<CN>(S.tnodes, ref, o, inv, tdist, st, c, ref);
+                have = visit_node<W, CN>(S.tnodes, S.wtnodes, ref, o, inv, tdist, st, c, ref);
             }
         }
     }
     if (fl.has(PT_HAS_LIST_MESHES)) {
         for (uint32_t m = 0; m < S.n_meshes; ++m) {
             const pt_mesh_instance& mi = S.meshes[m];
-            if (blas_any<CN>(S, S.mesh_root[m], mi.inv_matrix, mi.tri_offset, o, d, tdist, st, c)) return true;
+            if (blas_any<CN, W>(S, S.mesh_root[m], mi.inv_matrix, mi.tri_offset, o, d, tdist, st, c)) return true;
         }
     }
     return false;

@@ -17,6 +17,7 @@
 
 #include "pt_abi.h"
 #include "pt_internal.h"
+#include "pt_layout.h"
 
 using namespace pt;
 
@@ -44,12 +45,13 @@ struct pt_ctx {
     // options
     int backend = PT_BACKEND_MEGAKERNEL;
     int schedule = PT_SCHEDULE_REGEN;
+    int traversal = PT_TRAVERSAL_WIDE;
     bool count_stats = true;
     // scene
     bool has_scene = false;
     uint32_t flags = 0;
     int bounce_limit = 8;
-    DevBuf mats, objs, obj_root, tnodes, bnodes, roots, tlas_raw, bvh_raw, tri_geo, tris, planes, meshes, mesh_root,
+    DevBuf mats, objs, obj_root, tnodes, bnodes, wtnodes, wbnodes, roots, tlas_raw, bvh_raw, tri_geo, tris, planes, meshes, mesh_root,
         lights, sky, cdf, cdf_rows;
     uint64_t n_mats = 0, n_objs = 0, n_tlas = 0, n_planes = 0, n_meshes = 0, n_lights = 0, n_bvh = 0, n_tris = 0;
     uint32_t sky_w = 1, sky_h = 1;
@@ -58,6 +60,7 @@ struct pt_ctx {
     std::map<uint32_t, uint32_t> blas_slot;   // node_offset -> root slot
     std::vector<Root> host_roots;
     uint32_t tlas_depth = 0, blas_depth = 0, stack_cap = 2;
+    uint32_t blas_wneed = 0, wstack_cap = 2;
     DevScene dev{};
     // frame
     uint32_t width = 0, height = 0;
@@ -107,7 +110,7 @@ struct pt_ctx {
         return PT_OK;
     }
     void release_all() {
-        DevBuf* all[] = {&mats, &objs, &obj_root, &tnodes, &bnodes, &roots, &tlas_raw, &bvh_raw, &tri_geo, &tris,
+        DevBuf* all[] = {&mats, &objs, &obj_root, &tnodes, &bnodes, &wtnodes, &wbnodes, &roots, &tlas_raw, &bvh_raw, &tri_geo, &tris,
                          &planes, &meshes, &mesh_root, &lights, &sky, &cdf, &cdf_rows, &accum, &out, &gbuf, &dn_a,
                          &dn_b, &dn_fin, &dn_guide, &stats, &seeds, &rays_in, &rays_aux, &rays_aux2, &rays_out,
                          &wf_o, &wf_d, &wf_t, &wf_r, &wf_meta, &wf_ha, &wf_hb, &wf_media, &wf_q0, &wf_q1, &wf_sq,
@@ -118,77 +121,10 @@ struct pt_ctx {
 
 namespace {
 
-// Converts one reference BVH (32 B nodes, left = i+1) rooted at `root` into
-// child-pair nodes appended to `out`.  For a BLAS the right child is
-// node_offset + next (shader.wgsl:530); for the TLAS it is next
-// (shader.wgsl:740).  Returns PT_OK and fills *r.
-struct TreeConv {
-    const pt_bvh_node* nodes;
-    uint64_t n_nodes;
-    bool blas;
-    uint32_t node_offset;
-    std::vector<CNode>* out;
-    uint32_t max_depth = 0;
-    std::string err;
-
-    bool leaf_ref(uint32_t i, uint32_t* ref) {
-        const pt_bvh_node& n = nodes[i];
-        if (n.count <= 32 && n.next < (1u << 25)) {
-            *ref = kLeafBit | ((n.count - 1u) << 25) | n.next;
-        } else {
-            if (i >= (1u << 30)) { err = "leaf index too large"; return false; }
-            *ref = kLeafBit | kIndirectBit | i;
-        }
-        return true;
-    }
-    // Returns the reference for node i (recursively converting its subtree).
-    bool convert(uint32_t i, uint32_t depth, uint32_t* ref) {
-        if (i >= n_nodes) { err = "BVH child index out of range"; return false; }
-        if (depth > 256) { err = "BVH deeper than 256 levels (cycle?)"; return false; }
-        max_depth = std::max(max_depth, depth);
-        const pt_bvh_node& n = nodes[i];
-        if (n.count > 0) return leaf_ref(i, ref);
-        const uint64_t left = uint64_t(i) + 1u;
-        const uint64_t right = blas ? uint64_t(node_offset) + n.next : uint64_t(n.next);
-        if (left >= n_nodes || right >= n_nodes) { err = "BVH child index out of range"; return false; }
-        const uint32_t k = uint32_t(out->size());
-        if (k >= 0x80000000u) { err = "too many BVH nodes"; return false; }
-        out->push_back(CNode{});
-        uint32_t lref, rref;
-        if (!convert(uint32_t(left), depth + 1, &lref)) return false;
-        if (!convert(uint32_t(right), depth + 1, &rref)) return false;
-        const pt_bvh_node& L = nodes[left];
-        const pt_bvh_node& R = nodes[right];
-        CNode& c = (*out)[k];
-        c.a = make_float4(L.aabb_min[0], L.aabb_min[1], L.aabb_min[2], L.aabb_max[0]);
-        c.b = make_float4(L.aabb_max[1], L.aabb_max[2], R.aabb_min[0], R.aabb_min[1]);
-        c.c = make_float4(R.aabb_min[2], R.aabb_max[0], R.aabb_max[1], R.aabb_max[2]);
-        c.d = make_uint4(lref, rref, 0u, 0u);
-        *ref = k;
-        return true;
-    }
-    bool root(uint32_t i, Root* r) {
-        if (i >= n_nodes) { err = "BVH root index out of range"; return false; }
-        uint32_t ref;
-        if (!convert(i, 0, &ref)) return false;
-        const pt_bvh_node& n = nodes[i];
-        for (int k = 0; k < 3; ++k) { r->mn[k] = n.aabb_min[k]; r->mx[k] = n.aabb_max[k]; }
-        r->ref = ref;
-        r->pad = 0;
-        return true;
-    }
-};
-
 bool check_array(const pt_array& a, uint32_t stride, bool allow_empty) {
     if (a.stride != stride) return false;
     if (a.count == 0) return allow_empty;
     return a.ptr != nullptr;
-}
-
-uint32_t f2u_host(float f) {
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    return u;
 }
 
 int validate_desc(pt_ctx* c, const pt_scene_desc* d) {
@@ -252,84 +188,6 @@ int validate_desc(pt_ctx* c, const pt_scene_desc* d) {
 
 // Checks that every triangle range a BLAS leaf can reach stays inside the
 // triangle array, for a given tri_offset.
-bool blas_ranges_ok(const pt_bvh_node* nodes, uint64_t n_nodes, uint32_t node_offset, uint32_t tri_offset,
-                    uint64_t n_tris) {
-    std::vector<uint32_t> st{node_offset};
-    uint64_t visited = 0;
-    while (!st.empty()) {
-        const uint32_t i = st.back();
-        st.pop_back();
-        if (i >= n_nodes || ++visited > n_nodes) return false;
-        const pt_bvh_node& n = nodes[i];
-        if (n.count > 0) {
-            if (uint64_t(tri_offset) + n.next + n.count > n_tris) return false;
-        } else {
-            st.push_back(i + 1);
-            st.push_back(node_offset + n.next);
-        }
-    }
-    return true;
-}
-
-// Builds TLAS nodes + root slot 0, and BLAS roots for every mesh object and
-// list mesh, into host vectors.
-int build_bvh_layout(pt_ctx* c, const pt_scene_desc* d, std::vector<CNode>& tn, std::vector<CNode>& bn,
-                     std::vector<Root>& roots, std::vector<uint32_t>& obj_root, std::vector<uint32_t>& mesh_root,
-                     std::map<uint32_t, uint32_t>& slots, uint32_t* tlas_depth, uint32_t* blas_depth) {
-    const uint32_t any_obj = PT_HAS_SPHERES | PT_HAS_CUBES | PT_HAS_CYLINDERS | PT_HAS_TORI | PT_HAS_MESHES;
-    const pt_bvh_node* tl = static_cast<const pt_bvh_node*>(d->tlas.ptr);
-    const pt_bvh_node* bl = static_cast<const pt_bvh_node*>(d->bvh.ptr);
-    const pt_object* ob = static_cast<const pt_object*>(d->objects.ptr);
-    roots.assign(1, Root{});
-    *tlas_depth = 0;
-    *blas_depth = 0;
-    if (d->flags & any_obj) {
-        TreeConv t{tl, d->tlas.count, false, 0, &tn};
-        if (!t.root(0, &roots[0])) return c->fail(PT_ERR_LAYOUT, "tlas: %s", t.err.c_str());
-        *tlas_depth = t.max_depth;
-        // every TLAS leaf range must index real objects
-        for (uint64_t i = 0; i < d->tlas.count; ++i)
-            if (tl[i].count > 0 && uint64_t(tl[i].next) + tl[i].count > d->objects.count)
-                return c->fail(PT_ERR_LAYOUT, "tlas node %llu: object range out of bounds", (unsigned long long)i);
-    }
-    obj_root.assign(d->objects.count, 0u);
-    auto slot_for = [&](uint32_t node_offset, uint32_t tri_offset, uint32_t* slot) -> int {
-        if (!blas_ranges_ok(bl, d->bvh.count, node_offset, tri_offset, d->triangles.count))
-            return c->fail(PT_ERR_LAYOUT, "BLAS at node %u / triangle offset %u indexes outside the arrays", node_offset,
-                           tri_offset);
-        auto it = slots.find(node_offset);
-        if (it != slots.end()) { *slot = it->second; return PT_OK; }
-        TreeConv b{bl, d->bvh.count, true, node_offset, &bn};
-        Root r;
-        if (!b.root(node_offset, &r)) return c->fail(PT_ERR_LAYOUT, "bvh: %s", b.err.c_str());
-        *blas_depth = std::max(*blas_depth, b.max_depth);
-        *slot = uint32_t(roots.size());
-        roots.push_back(r);
-        slots[node_offset] = *slot;
-        return PT_OK;
-    };
-    if (d->flags & PT_HAS_MESHES) {
-        for (uint64_t i = 0; i < d->objects.count; ++i) {
-            if (ob[i].object_type != PT_OBJ_MESH) continue;
-            uint32_t s;
-            int rc = slot_for(f2u_host(ob[i].param0), f2u_host(ob[i].param1), &s);
-            if (rc) return rc;
-            obj_root[i] = s;
-        }
-    }
-    mesh_root.assign(d->meshes.count, 0u);
-    if (d->flags & PT_HAS_LIST_MESHES) {
-        const pt_mesh_instance* mi = static_cast<const pt_mesh_instance*>(d->meshes.ptr);
-        for (uint64_t i = 0; i < d->meshes.count; ++i) {
-            uint32_t s;
-            int rc = slot_for(mi[i].node_offset, mi[i].tri_offset, &s);
-            if (rc) return rc;
-            mesh_root[i] = s;
-        }
-    }
-    return PT_OK;
-}
-
 void refresh_dev(pt_ctx* c) {
     DevScene& S = c->dev;
     S.mats = static_cast<const pt_material*>(c->mats.p);
@@ -337,6 +195,8 @@ void refresh_dev(pt_ctx* c) {
     S.obj_root = static_cast<const uint32_t*>(c->obj_root.p);
     S.tnodes = static_cast<const CNode*>(c->tnodes.p);
     S.bnodes = static_cast<const CNode*>(c->bnodes.p);
+    S.wtnodes = static_cast<const WNode*>(c->wtnodes.p);
+    S.wbnodes = static_cast<const WNode*>(c->wbnodes.p);
     S.roots = static_cast<const Root*>(c->roots.p);
     S.tlas_raw = static_cast<const pt_bvh_node*>(c->tlas_raw.p);
     S.bvh_raw = static_cast<const pt_bvh_node*>(c->bvh_raw.p);
@@ -357,46 +217,54 @@ void refresh_dev(pt_ctx* c) {
     S.flags = c->flags;
     S.bounce_limit = c->bounce_limit;
     S.stack_cap = c->stack_cap;
+    S.wstack_cap = c->wstack_cap;
 }
 
 int upload_dynamic(pt_ctx* c, const pt_scene_desc* d, bool first) {
-    std::vector<CNode> tn, bn;
-    std::vector<Root> roots;
-    std::vector<uint32_t> obj_root, mesh_root;
-    std::map<uint32_t, uint32_t> slots;
-    uint32_t td = 0, bd = 0;
-    if (!first) slots = {};
-    int rc = build_bvh_layout(c, d, tn, bn, roots, obj_root, mesh_root, slots, &td, &bd);
-    if (rc) return rc;
+    HostLayout L;
+    std::string err;
+    int rc = build_layout(d, L, err);
+    if (rc) return c->fail(rc, "%s", err.c_str());
+    std::vector<Root>& roots = L.roots;
+    const uint32_t td = L.tlas_depth;
     if (!first) {
         // the fast path may only reuse BLAS layouts built at upload
-        for (auto& kv : slots)
+        for (auto& kv : L.slots)
             if (c->blas_slot.find(kv.first) == c->blas_slot.end())
                 return c->fail(PT_ERR_CAPACITY, "objects reference a BLAS root (%u) absent at upload", kv.first);
         // remap slots to the uploaded numbering, keep the uploaded BLAS nodes
         std::vector<uint32_t> remap(roots.size(), 0);
-        for (auto& kv : slots) remap[kv.second] = c->blas_slot[kv.first];
-        for (auto& s : obj_root) s = remap[s];
-        for (auto& s : mesh_root) s = remap[s];
+        for (auto& kv : L.slots) remap[kv.second] = c->blas_slot[kv.first];
+        for (auto& s : L.obj_root) s = remap[s];
+        for (auto& s : L.mesh_root) s = remap[s];
         std::vector<Root> merged = c->host_roots;
         merged[0] = roots[0];
         roots = merged;
-        if (td + c->blas_depth + 2 > c->stack_cap && td + c->blas_depth + 2 <= kMaxStack)
-            c->stack_cap = td + c->blas_depth + 2;
-    } else {
-        const uint32_t need = td + bd + 2;
-        if (need > kMaxStack) return c->fail(PT_ERR_UNSUPPORTED, "BVH depth %u+%u exceeds the %u-entry stack", td, bd, kMaxStack);
+        const uint32_t need = stack_need(td, c->blas_depth), wneed = stack_need(L.tlas_wneed, c->blas_wneed);
+        if (need > kMaxStack || wneed > kMaxStack)
+            return c->fail(PT_ERR_UNSUPPORTED, "TLAS depth %u exceeds the %u-entry stack", td, kMaxStack);
         c->stack_cap = need;
-        c->blas_slot = slots;
-        c->blas_depth = bd;
-        if ((rc = c->upload(c->bnodes, bn.data(), bn.size() * sizeof(CNode), "bvh nodes"))) return rc;
+        c->wstack_cap = wneed;
+    } else {
+        const uint32_t need = stack_need(td, L.blas_depth), wneed = stack_need(L.tlas_wneed, L.blas_wneed);
+        if (need > kMaxStack || wneed > kMaxStack)
+            return c->fail(PT_ERR_UNSUPPORTED, "BVH depth %u+%u exceeds the %u-entry stack", td, L.blas_depth,
+                           kMaxStack);
+        c->stack_cap = need;
+        c->wstack_cap = wneed;
+        c->blas_slot = L.slots;
+        c->blas_depth = L.blas_depth;
+        c->blas_wneed = L.blas_wneed;
+        if ((rc = c->upload(c->bnodes, L.bn.data(), L.bn.size() * sizeof(CNode), "bvh nodes"))) return rc;
+        if ((rc = c->upload(c->wbnodes, L.wbn.data(), L.wbn.size() * sizeof(WNode), "wide bvh nodes"))) return rc;
     }
     c->tlas_depth = td;
     c->host_roots = roots;
-    if ((rc = c->upload(c->tnodes, tn.data(), tn.size() * sizeof(CNode), "tlas nodes"))) return rc;
+    if ((rc = c->upload(c->tnodes, L.tn.data(), L.tn.size() * sizeof(CNode), "tlas nodes"))) return rc;
+    if ((rc = c->upload(c->wtnodes, L.wtn.data(), L.wtn.size() * sizeof(WNode), "wide tlas nodes"))) return rc;
     if ((rc = c->upload(c->roots, roots.data(), roots.size() * sizeof(Root), "roots"))) return rc;
-    if ((rc = c->upload(c->obj_root, obj_root.data(), obj_root.size() * 4, "object roots"))) return rc;
-    if ((rc = c->upload(c->mesh_root, mesh_root.data(), mesh_root.size() * 4, "mesh roots"))) return rc;
+    if ((rc = c->upload(c->obj_root, L.obj_root.data(), L.obj_root.size() * 4, "object roots"))) return rc;
+    if ((rc = c->upload(c->mesh_root, L.mesh_root.data(), L.mesh_root.size() * 4, "mesh roots"))) return rc;
     if ((rc = c->upload(c->mats, d->materials.ptr, d->materials.count * sizeof(pt_material), "materials"))) return rc;
     if ((rc = c->upload(c->objs, d->objects.ptr, d->objects.count * sizeof(pt_object), "objects"))) return rc;
     if ((rc = c->upload(c->tlas_raw, d->tlas.ptr, d->tlas.count * sizeof(pt_bvh_node), "tlas"))) return rc;
@@ -459,6 +327,11 @@ int pt_set_option(pt_ctx* c, int key, int64_t value) {
             if (value < 1024 || value > (1ll << 28)) return c->fail(PT_ERR_ARG, "pool size must be in [1024, 2^28]");
             c->wf_pool_opt = uint64_t(value);
             return PT_OK;
+        case PT_OPT_TRAVERSAL:
+            if (value != PT_TRAVERSAL_REFERENCE && value != PT_TRAVERSAL_WIDE)
+                return c->fail(PT_ERR_ARG, "unknown traversal %lld", (long long)value);
+            c->traversal = int(value);
+            return PT_OK;
         case PT_OPT_SCHEDULE:
             if (value != PT_SCHEDULE_SYNC && value != PT_SCHEDULE_REGEN)
                 return c->fail(PT_ERR_ARG, "unknown schedule %lld", (long long)value);
@@ -479,13 +352,8 @@ int pt_upload_scene(pt_ctx* c, const pt_scene_desc* d) {
     if ((rc = upload_dynamic(c, d, true))) return rc;
     // triangles: 48 B geometry records + the original 128 B records
     const pt_triangle* tr = static_cast<const pt_triangle*>(d->triangles.ptr);
-    std::vector<float4> geo(size_t(d->triangles.count) * 3);
-    for (uint64_t i = 0; i < d->triangles.count; ++i) {
-        const pt_triangle& t = tr[i];
-        geo[3 * i] = make_float4(t.v0[0], t.v0[1], t.v0[2], 0.f);
-        geo[3 * i + 1] = make_float4(t.v1[0] - t.v0[0], t.v1[1] - t.v0[1], t.v1[2] - t.v0[2], 0.f);
-        geo[3 * i + 2] = make_float4(t.v2[0] - t.v0[0], t.v2[1] - t.v0[1], t.v2[2] - t.v0[2], 0.f);
-    }
+    std::vector<float4> geo;
+    build_tri_geo(d, geo);
     if ((rc = c->upload(c->tri_geo, geo.data(), geo.size() * sizeof(float4), "triangle geometry"))) return rc;
     if ((rc = c->upload(c->tris, tr, d->triangles.count * sizeof(pt_triangle), "triangles"))) return rc;
     if ((rc = c->upload(c->bvh_raw, d->bvh.ptr, d->bvh.count * sizeof(pt_bvh_node), "bvh"))) return rc;
@@ -610,6 +478,7 @@ static int run_wavefront(pt_ctx* c, const TraceArgs& a) {
     w.accum = a.accum;
     w.out = a.out;
     w.stats = a.stats;
+    w.wide = a.wide;
     w.o_pdf = static_cast<float4*>(c->wf_o.p);
     w.d_wsum = static_cast<float4*>(c->wf_d.p);
     w.tp_rng = static_cast<float4*>(c->wf_t.p);
@@ -663,6 +532,7 @@ static int run_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t 
     a.hits = hits_dev;
     a.n_rec = n_rec;
     a.schedule = c->schedule;
+    a.wide = c->traversal == PT_TRAVERSAL_WIDE;
     if (c->backend == PT_BACKEND_WAVEFRONT && !hits_dev) {
         if ((rc = run_wavefront(c, a))) return rc;
     } else {
@@ -842,6 +712,7 @@ int pt_trace_rays(pt_ctx* c, const float* rays, uint32_t n, pt_hit* hits) {
     a.rays = static_cast<const float*>(c->rays_in.p);
     a.n = n;
     a.hits = static_cast<pt_hit*>(c->rays_out.p);
+    a.wide = c->traversal == PT_TRAVERSAL_WIDE;
     hipError_t e = launch_rays(c->dev, a, false, c->stream);
     if (e != hipSuccess) return c->hip_fail(e, "rays launch");
     if ((rc = c->sync("rays"))) return rc;
@@ -869,6 +740,7 @@ int pt_trace_shadow_rays(pt_ctx* c, const float* rays, const int32_t* target_idx
     a.tdist = static_cast<const float*>(c->rays_aux2.p);
     a.n = n;
     a.trans = static_cast<float*>(c->rays_out.p);
+    a.wide = c->traversal == PT_TRAVERSAL_WIDE;
     hipError_t e = launch_rays(c->dev, a, true, c->stream);
     if (e != hipSuccess) return c->hip_fail(e, "shadow rays launch");
     if ((rc = c->sync("shadow rays"))) return rc;

@@ -24,6 +24,7 @@ struct TraceArgs {
     pt_hit* hits;                // hit recording (NULL: off)
     uint32_t n_rec;
     int schedule;                // PT_SCHEDULE_*
+    int wide;                    // 1: wide traversal (PT_TRAVERSAL_WIDE)
     uint32_t n_tiles;            // 8x8 tiles of the section
     uint32_t* tile_ctr;          // 8 per-XCD band counters, zero at launch
     float* media;                // kMediaFloats per lane of the grid, structure of arrays
@@ -37,6 +38,7 @@ struct RayArgs {
     uint32_t n;
     pt_hit* hits;
     float* trans;
+    int wide;                    // 1: wide traversal (PT_TRAVERSAL_WIDE)
 };
 
 struct DenoiseArgs {
@@ -78,6 +80,7 @@ struct WfArgs {
     float4* sh_tp;      //              throughput, valid flag
     float4* sh_fi;      //              clamped unoccluded radiance
     uint32_t* counters; // queue A length, queue B length, shadow queue length
+    int wide;           // 1: wide traversal (PT_TRAVERSAL_WIDE)
 };
 
 // Launchers (pt_kernels.hip).  All asynchronous on `stream`.

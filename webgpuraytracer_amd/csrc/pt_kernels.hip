@@ -78,7 +78,7 @@ __device__ void store_hit(pt_hit* o, const HitRec& h, const SurfHit& s) {
 // Megakernel NEE sink: traces the shadow ray at once and adds the
 // contribution, exactly where evaluate_nee's result is added
 // (shader.wgsl:2440, 2452).
-template <uint32_t F, bool CN>
+template <uint32_t F, bool CN, bool W>
 struct ImmediateNee {
     const DevScene& S;
     Flags<F> fl;
@@ -88,7 +88,7 @@ struct ImmediateNee {
                                v3& rad) {
         if (need_ray) {
             c.shadow++;
-            if (occluded<F, CN>(S, fl, o, d, ignore, dist, st, c)) inc = splat(0.0f);
+            if (occluded<F, CN, W>(S, fl, o, d, ignore, dist, st, c)) inc = splat(0.0f);
         }
         rad = rad + tp * firefly(inc, bounce);
     }
@@ -96,19 +96,19 @@ struct ImmediateNee {
 
 // One path of shader.wgsl `main` (2279-2480).  rec: when non-null, the first
 // n_rec trace_scene hits of the path are written there.
-template <uint32_t F, bool CN, bool REC>
+template <uint32_t F, bool CN, bool REC, bool W>
 __device__ v3 trace_path(const DevScene& S, Flags<F> fl, const pt_scene_params& PP, const Params& P, uint32_t px,
                          uint32_t py, uint32_t sample_number, uint32_t seed, LdsStack& st, float* media,
                          uint32_t media_stride, Counters& c, pt_hit* rec, uint32_t n_rec) {
     PathState ps;
     path_begin(PP, px, py, sample_number, seed, ps);
     Media md{media, media_stride, 0};
-    ImmediateNee<F, CN> sink{S, fl, st, c};
+    ImmediateNee<F, CN, W> sink{S, fl, st, c};
     c.paths++;
     for (int bounce = 0; bounce < S.bounce_limit; ++bounce) {
         HitRec h;
         c.ext++;
-        closest_hit<F, CN>(S, fl, ps.ro, ps.rd, st, h, c);
+        closest_hit<F, CN, W>(S, fl, ps.ro, ps.rd, st, h, c);
         const SurfHit sh = resolve_hit<REC>(S, ps.ro, ps.rd, h);
         if (REC && uint32_t(bounce) < n_rec) store_hit(rec + bounce, h, sh);
         if (!shade_bounce<F, CN>(S, fl, P, bounce, h, sh, ps, md, c, sink)) break;
@@ -165,14 +165,14 @@ __device__ uint32_t next_tile(uint32_t* ctr, uint32_t n_tiles) {
 #ifndef PT_TRACE_WAVES_PER_EU
 #define PT_TRACE_WAVES_PER_EU 4
 #endif
-template <uint32_t F, bool CN, int MODE>
+template <uint32_t F, bool CN, int MODE, bool W>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(PT_TRACE_WAVES_PER_EU)))
 trace_kernel(DevScene S, TraceArgs A) {
     extern __shared__ uint32_t lds_stack[];
     const Flags<F> fl{S.flags};
     const uint32_t tiles_x = (A.sec_w + kTile - 1) / kTile;
     const uint32_t lane = threadIdx.x;
-    LdsStack st{lds_stack + lane, blockDim.x, S.stack_cap, 0};
+    LdsStack st{lds_stack + lane, blockDim.x, W ? S.wstack_cap : S.stack_cap, 0};
     float* media = A.media + blockIdx.x * kWave + lane;
     const Params P{A.P.sky_total_lum, A.P.sky_width, A.P.sky_height, A.P.sky_cdf_index};
     Counters c;
@@ -188,7 +188,7 @@ trace_kernel(DevScene S, TraceArgs A) {
             const float4 old = A.accum[idx];
             v3 acc = mk(old.x, old.y, old.z);
             if (MODE == kRegen) {
-                ImmediateNee<F, CN> sink{S, fl, st, c};
+                ImmediateNee<F, CN, W> sink{S, fl, st, c};
                 Media md{media, A.media_stride, 0};
                 PathState ps;
                 uint32_t pass = 0;
@@ -199,7 +199,7 @@ trace_kernel(DevScene S, TraceArgs A) {
                     if (bounce < S.bounce_limit) {
                         HitRec h;
                         c.ext++;
-                        closest_hit<F, CN>(S, fl, ps.ro, ps.rd, st, h, c);
+                        closest_hit<F, CN, W>(S, fl, ps.ro, ps.rd, st, h, c);
                         const SurfHit sh = resolve_hit<false>(S, ps.ro, ps.rd, h);
                         const bool more = shade_bounce<F, CN>(S, fl, P, bounce, h, sh, ps, md, c, sink);
                         ++bounce;
@@ -218,7 +218,7 @@ trace_kernel(DevScene S, TraceArgs A) {
                     const uint32_t sn = A.P.sample_number + pass;
                     const uint32_t seed = A.seeds ? A.seeds[pass] : A.P.seed;
                     pt_hit* rec = MODE == kRecord ? A.hits + size_t(idx) * A.n_rec : nullptr;
-                    const v3 r = trace_path<F, CN, MODE == kRecord>(S, fl, A.P, P, px, py, sn, seed, st, media,
+                    const v3 r = trace_path<F, CN, MODE == kRecord, W>(S, fl, A.P, P, px, py, sn, seed, st, media,
                                                                     A.media_stride, c, rec, A.n_rec);
                     acc = lerp(acc, r, 1.0f / float(sn + 1u));
                 }
@@ -257,7 +257,7 @@ __global__ void __launch_bounds__(kWave) gbuffer_kernel(DevScene S, pt_scene_par
     float rough = 0.0f, metal = 0.0f, trans = 0.0f, dist = 0.0f;
     for (int bounce = 0; bounce < S.bounce_limit; ++bounce) {
         HitRec h;
-        closest_hit<F, false>(S, fl, ro, rd, st, h, c);
+        closest_hit<F, false, false>(S, fl, ro, rd, st, h, c);
         const SurfHit sh = resolve_hit<false>(S, ro, rd, h);
         if (sh.m_idx == -1) {
             n = splat(0.0f);
@@ -307,24 +307,24 @@ __global__ void __launch_bounds__(kWave) gbuffer_kernel(DevScene S, pt_scene_par
     out[idx] = make_uchar4(unorm8(o.x), unorm8(o.y), unorm8(o.z), 255);
 }
 
-template <uint32_t F>
+template <uint32_t F, bool W>
 __global__ void __launch_bounds__(kWave) rays_kernel(DevScene S, RayArgs A, int shadow) {
     extern __shared__ uint32_t lds_stack[];
     const Flags<F> fl{S.flags};
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
-    LdsStack st{lds_stack + threadIdx.x, blockDim.x, S.stack_cap, 0};
+    LdsStack st{lds_stack + threadIdx.x, blockDim.x, W ? S.wstack_cap : S.stack_cap, 0};
     Counters c;
     c.clear();
     const float* r = A.rays + size_t(i) * 6;
     const v3 o = mk(r[0], r[1], r[2]), d = mk(r[3], r[4], r[5]);
     if (shadow) {
-        const bool b = occluded<F, false>(S, fl, o, d, A.target[i], A.tdist[i], st, c);
+        const bool b = occluded<F, false, W>(S, fl, o, d, A.target[i], A.tdist[i], st, c);
         const float v = b ? 0.0f : 1.0f;
         A.trans[3 * i] = v; A.trans[3 * i + 1] = v; A.trans[3 * i + 2] = v;
     } else {
         HitRec h;
-        closest_hit<F, false>(S, fl, o, d, st, h, c);
+        closest_hit<F, false, W>(S, fl, o, d, st, h, c);
         store_hit(A.hits + i, h, resolve_hit<true>(S, o, d, h));
     }
 }
@@ -543,24 +543,35 @@ __global__ void zero_u64(unsigned long long* p, uint32_t n) {
 }
 
 // ---------------------------------------------------------------- launchers --
-static size_t stack_bytes(const DevScene& S, uint32_t threads) { return size_t(S.stack_cap) * threads * 4u; }
+static size_t stack_bytes(const DevScene& S, uint32_t threads, bool wide) {
+    return size_t(wide ? S.wstack_cap : S.stack_cap) * threads * 4u;
+}
 
-template <uint32_t F, bool CN, int MODE>
+template <uint32_t F, bool CN, int MODE, bool W>
 static hipError_t trace_variant(const DevScene& S, const TraceArgs& a, uint32_t grid, int* blocks_per_cu,
                                 hipStream_t s) {
-    const size_t lds = stack_bytes(S, kWave);
+    const size_t lds = stack_bytes(S, kWave, W);
     if (blocks_per_cu)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, trace_kernel<F, CN, MODE>, kWave, lds);
-    hipLaunchKernelGGL((trace_kernel<F, CN, MODE>), dim3(grid), dim3(kWave), lds, s, S, a);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, trace_kernel<F, CN, MODE, W>, kWave, lds);
+    hipLaunchKernelGGL((trace_kernel<F, CN, MODE, W>), dim3(grid), dim3(kWave), lds, s, S, a);
     return hipGetLastError();
 }
 
+// Variants: hit recording and work counting use the reference-order
+// traversal (they report the reference's per-pop counts); rendering uses the
+// traversal the caller chose.
 template <uint32_t F>
 static hipError_t trace_f(const DevScene& S, const TraceArgs& a, bool cn, uint32_t grid, int* bpc, hipStream_t s) {
-    if (a.hits) return trace_variant<F, false, kRecord>(S, a, grid, bpc, s);
-    if (a.schedule == PT_SCHEDULE_REGEN)
-        return cn ? trace_variant<F, true, kRegen>(S, a, grid, bpc, s) : trace_variant<F, false, kRegen>(S, a, grid, bpc, s);
-    return cn ? trace_variant<F, true, kSync>(S, a, grid, bpc, s) : trace_variant<F, false, kSync>(S, a, grid, bpc, s);
+    if (a.hits) return trace_variant<F, false, kRecord, false>(S, a, grid, bpc, s);
+    const bool regen = a.schedule == PT_SCHEDULE_REGEN;
+    if (cn)
+        return regen ? trace_variant<F, true, kRegen, false>(S, a, grid, bpc, s)
+                     : trace_variant<F, true, kSync, false>(S, a, grid, bpc, s);
+    if (a.wide)
+        return regen ? trace_variant<F, false, kRegen, true>(S, a, grid, bpc, s)
+                     : trace_variant<F, false, kSync, true>(S, a, grid, bpc, s);
+    return regen ? trace_variant<F, false, kRegen, false>(S, a, grid, bpc, s)
+                 : trace_variant<F, false, kSync, false>(S, a, grid, bpc, s);
 }
 
 static hipError_t trace_dispatch(const DevScene& S, const TraceArgs& a, bool cn, uint32_t grid, int* bpc,
@@ -583,16 +594,20 @@ hipError_t launch_trace(const DevScene& S, const TraceArgs& a, uint32_t grid, bo
 hipError_t launch_gbuffer(const DevScene& S, const pt_scene_params& P, pt_gbuffer_texel* gbuf, uchar4* out,
                           hipStream_t s) {
     const uint32_t tiles = ((P.width + kTile - 1) / kTile) * ((P.height + kTile - 1) / kTile);
-    hipLaunchKernelGGL((gbuffer_kernel<kDynamicFlags>), dim3(tiles), dim3(kWave), stack_bytes(S, kWave), s, S, P,
-                       gbuf, out);
+    hipLaunchKernelGGL((gbuffer_kernel<kDynamicFlags>), dim3(tiles), dim3(kWave), stack_bytes(S, kWave, false), s, S,
+                       P, gbuf, out);
     return hipGetLastError();
 }
 
 hipError_t launch_rays(const DevScene& S, const RayArgs& a, bool shadow, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     const uint32_t blocks = (a.n + kWave - 1) / kWave;
-    hipLaunchKernelGGL((rays_kernel<kDynamicFlags>), dim3(blocks), dim3(kWave), stack_bytes(S, kWave), s, S, a,
-                       shadow ? 1 : 0);
+    if (a.wide)
+        hipLaunchKernelGGL((rays_kernel<kDynamicFlags, true>), dim3(blocks), dim3(kWave), stack_bytes(S, kWave, true), s,
+                           S, a, shadow ? 1 : 0);
+    else
+        hipLaunchKernelGGL((rays_kernel<kDynamicFlags, false>), dim3(blocks), dim3(kWave), stack_bytes(S, kWave, false),
+                           s, S, a, shadow ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(kWfBlock) wf_raygen(WfArgs A) {
     A.sh_tp[2u * i + 1u] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-template <uint32_t F, bool CN>
+template <uint32_t F, bool CN, bool W>
 __global__ void __launch_bounds__(kWfBlock) wf_extend(DevScene S, WfArgs A, int cur) {
     extern __shared__ uint32_t lds_stack[];
     const Flags<F> fl{S.flags};
@@ -104,9 +104,9 @@ __global__ void __launch_bounds__(kWfBlock) wf_extend(DevScene S, WfArgs A, int 
     if (i < n) {
         const uint32_t p = A.queue[cur][i];
         const float4 o = A.o_pdf[p], d = A.d_wsum[p];
-        LdsStack st{lds_stack + threadIdx.x, blockDim.x, S.stack_cap, 0};
+        LdsStack st{lds_stack + threadIdx.x, blockDim.x, W ? S.wstack_cap : S.stack_cap, 0};
         HitRec h;
-        closest_hit<F, CN>(S, fl, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), st, h, c);
+        closest_hit<F, CN, W>(S, fl, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), st, h, c);
         A.hit_a[p] = make_int4(__float_as_int(h.t), (h.kind << 30) | (h.idx & 0x3fffffff), h.prim, int(h.count));
         A.hit_b[p] = make_float2(h.bu, h.bv);
         c.ext = 1;
@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(kWfBlock) wf_shade(DevScene S, WfArgs A, int c
     if (A.stats && CN) add_stat(A.stats, kStShade, c.shade);
 }
 
-template <uint32_t F, bool CN>
+template <uint32_t F, bool CN, bool W>
 __global__ void __launch_bounds__(kWfBlock) wf_shadow(DevScene S, WfArgs A) {
     extern __shared__ uint32_t lds_stack[];
     const Flags<F> fl{S.flags};
@@ -208,7 +208,7 @@ __global__ void __launch_bounds__(kWfBlock) wf_shadow(DevScene S, WfArgs A) {
     c.clear();
     if (i < n) {
         const uint32_t p = A.shadow_queue[i];
-        LdsStack st{lds_stack + threadIdx.x, blockDim.x, S.stack_cap, 0};
+        LdsStack st{lds_stack + threadIdx.x, blockDim.x, W ? S.wstack_cap : S.stack_cap, 0};
         float4 r = A.rad[p];
         v3 rad = mk(r.x, r.y, r.z);
 #pragma unroll
@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(kWfBlock) wf_shadow(DevScene S, WfArgs A) {
             if (tp.w == 0.0f) continue;
             const float4 o = A.sh_o[s], d = A.sh_d[s], fi = A.sh_fi[s];
             c.shadow++;
-            const bool occ = occluded<F, CN>(S, fl, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), __float_as_int(d.w), o.w,
+            const bool occ = occluded<F, CN, W>(S, fl, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), __float_as_int(d.w), o.w,
                                              st, c);
             const v3 add = occ ? splat(0.0f) : mk(fi.x, fi.y, fi.z);
             rad = rad + mk(tp.x, tp.y, tp.z) * add;
@@ -262,24 +262,29 @@ __global__ void __launch_bounds__(kWfBlock) wf_accumulate(WfArgs A) {
     if (A.stats && lp == 0) atomicAdd(A.stats + kStPaths, (unsigned long long)A.n_items);
 }
 
+template <uint32_t F, bool CN, bool W>
+static void wf_bounces(const DevScene& S, const WfArgs& a0, uint32_t blocks, hipStream_t s) {
+    const size_t lds = size_t(W ? S.wstack_cap : S.stack_cap) * kWfBlock * 4u;
+    int cur = 0;
+    for (int b = 0; b < S.bounce_limit; ++b) {
+        hipLaunchKernelGGL((wf_extend<F, CN, W>), dim3(blocks), dim3(kWfBlock), lds, s, S, a0, cur);
+        hipLaunchKernelGGL((wf_shade<F, CN>), dim3(blocks), dim3(kWfBlock), 0, s, S, a0, cur, b);
+        hipLaunchKernelGGL((wf_shadow<F, CN, W>), dim3(blocks), dim3(kWfBlock), lds, s, S, a0);
+        cur ^= 1;
+    }
+}
+
+// Work counting runs the reference-order traversal (its per-pop counts).
 template <uint32_t F>
 static hipError_t wf_run(const DevScene& S, const WfArgs& a0, bool cn, hipStream_t s) {
     const uint32_t blocks = (a0.n_items + kWfBlock - 1) / kWfBlock;
-    const size_t lds = size_t(S.stack_cap) * kWfBlock * 4u;
     hipLaunchKernelGGL(wf_raygen, dim3(blocks), dim3(kWfBlock), 0, s, a0);
-    int cur = 0;
-    for (int b = 0; b < S.bounce_limit; ++b) {
-        if (cn) {
-            hipLaunchKernelGGL((wf_extend<F, true>), dim3(blocks), dim3(kWfBlock), lds, s, S, a0, cur);
-            hipLaunchKernelGGL((wf_shade<F, true>), dim3(blocks), dim3(kWfBlock), 0, s, S, a0, cur, b);
-            hipLaunchKernelGGL((wf_shadow<F, true>), dim3(blocks), dim3(kWfBlock), lds, s, S, a0);
-        } else {
-            hipLaunchKernelGGL((wf_extend<F, false>), dim3(blocks), dim3(kWfBlock), lds, s, S, a0, cur);
-            hipLaunchKernelGGL((wf_shade<F, false>), dim3(blocks), dim3(kWfBlock), 0, s, S, a0, cur, b);
-            hipLaunchKernelGGL((wf_shadow<F, false>), dim3(blocks), dim3(kWfBlock), lds, s, S, a0);
-        }
-        cur ^= 1;
-    }
+    if (cn)
+        wf_bounces<F, true, false>(S, a0, blocks, s);
+    else if (a0.wide)
+        wf_bounces<F, false, true>(S, a0, blocks, s);
+    else
+        wf_bounces<F, false, false>(S, a0, blocks, s);
     const uint32_t npix = a0.sec_w * a0.sec_h;
     hipLaunchKernelGGL(wf_accumulate, dim3((npix + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, s, a0);
     return hipGetLastError();
