@@ -116,7 +116,9 @@ enum {
     PT_OPT_COUNT_STATS = 2, /* 1: count node/triangle work (default 1) */
     PT_OPT_WAVEFRONT_POOL = 3, /* rays per wavefront pool (default 1<<21) */
     PT_OPT_SCHEDULE = 4,       /* megakernel pass schedule, PT_SCHEDULE_* (default REGEN) */
-    PT_OPT_TRAVERSAL = 5       /* BVH traversal, PT_TRAVERSAL_* (default WIDE) */
+    PT_OPT_TRAVERSAL = 5,      /* BVH traversal, PT_TRAVERSAL_* (default WIDE) */
+    PT_OPT_LDS_STACK = 6       /* megakernel traversal-stack entries per lane kept in LDS, the rest
+                                  in device memory; 0 = all in LDS (default 16) */
 };
 enum { PT_BACKEND_MEGAKERNEL = 0, PT_BACKEND_WAVEFRONT = 1 };
 /* SYNC: the lanes of a wave run each pass together.  REGEN: a lane whose path

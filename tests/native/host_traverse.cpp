@@ -95,7 +95,7 @@ int ht_trace_rays(const pt_scene_desc* d, const float* rays, uint32_t n, int wid
     for (uint32_t i = 0; i < n; ++i) {
         const float* r = rays + size_t(i) * 6;
         const v3 o = mk(r[0], r[1], r[2]), dd = mk(r[3], r[4], r[5]);
-        LdsStack st{stack.data(), 1, cap, 0};
+        LdsStack st = lds_stack_of(stack.data(), 1, cap);
         Counters c;
         c.clear();
         HitRec hr;
@@ -129,7 +129,7 @@ int ht_trace_shadow_rays(const pt_scene_desc* d, const float* rays, const int32_
     for (uint32_t i = 0; i < n; ++i) {
         const float* r = rays + size_t(i) * 6;
         const v3 o = mk(r[0], r[1], r[2]), dd = mk(r[3], r[4], r[5]);
-        LdsStack st{stack.data(), 1, cap, 0};
+        LdsStack st = lds_stack_of(stack.data(), 1, cap);
         Counters c;
         c.clear();
         const bool b = wide ? occluded<kDynamicFlags, false, true>(h.S, fl, o, dd, target[i], dist[i], st, c)

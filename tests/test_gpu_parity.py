@@ -263,3 +263,25 @@ def test_wide_traversal_renders_like_reference_order(renderer, which):
     same = np.all(acc[0].view(np.uint32) == acc[1].view(np.uint32), axis=-1)
     print(which, "identical pixels", same.mean())
     assert same.mean() >= 0.999
+
+
+@pytest.mark.parametrize("traversal", TRAVERSALS, indirect=True, ids=["reference", "wide"])
+def test_stack_spill_bitwise(renderer, traversal):
+    """Traversal-stack entries beyond the LDS share spill to device memory:
+    a 2-entry LDS share (most pushes spill) renders the same bits as a
+    stack held entirely in LDS."""
+    scene = small_mesh_scene(seed=15)
+    w, h = 64, 48
+    renderer.set_scene(scene)
+    seeds = [3, 4, 5]
+    p = scene.params(w, h, sample_number=0, seed=seeds[0])
+    acc = []
+    try:
+        for entries in (0, 2):
+            renderer.set_option(A.PT_OPT_LDS_STACK, entries)
+            renderer.resize(w, h)
+            renderer.render_section(p, passes=3, seeds=seeds)
+            acc.append(renderer.read_accum())
+    finally:
+        renderer.set_option(A.PT_OPT_LDS_STACK, 16)
+    assert np.array_equal(acc[0].view(np.uint32), acc[1].view(np.uint32))

@@ -91,19 +91,42 @@ struct Flags {
 constexpr uint32_t kAnyObject = PT_HAS_SPHERES | PT_HAS_CUBES | PT_HAS_CYLINDERS | PT_HAS_TORI | PT_HAS_MESHES;
 
 // ---------------------------------------------------------------- stack ----
-// Per-lane traversal stack in LDS, interleaved across the block's lanes
-// (entry e of lane l at e*blockDim + l) so that a wave's accesses fall in
-// distinct banks.
+// Per-lane traversal stack.  The first `cap` entries live in LDS,
+// interleaved across the block's lanes (entry e of lane l at e*blockDim + l)
+// so that a wave's accesses fall in distinct banks; entries cap..total-1, if
+// any, spill to a per-lane column in device memory (`spill`, entries
+// `sstride` apart), so a kernel can keep a short LDS stack for occupancy and
+// still hold the deepest traversal.  Pushes beyond `total` are dropped (the
+// host sizes `total` from the tree depths so that none is).
 struct LdsStack {
     uint32_t* base;
     uint32_t stride;
     uint32_t cap;
     int sp;
+    uint32_t* spill;
+    uint32_t sstride;
+    uint32_t total;
     __device__ __forceinline__ void push(uint32_t v) {
-        if (uint32_t(sp) < cap) { base[uint32_t(sp) * stride] = v; ++sp; }
+        const uint32_t e = uint32_t(sp);
+        if (e < cap) {
+            base[e * stride] = v;
+        } else if (e < total) {
+            spill[(e - cap) * sstride] = v;
+        } else {
+            return;
+        }
+        ++sp;
     }
-    __device__ __forceinline__ uint32_t pop() { --sp; return base[uint32_t(sp) * stride]; }
+    __device__ __forceinline__ uint32_t pop() {
+        --sp;
+        const uint32_t e = uint32_t(sp);
+        return e < cap ? base[e * stride] : spill[(e - cap) * sstride];
+    }
 };
+// A stack held entirely in LDS.
+__device__ __forceinline__ LdsStack lds_stack_of(uint32_t* base, uint32_t stride, uint32_t cap) {
+    return LdsStack{base, stride, cap, 0, nullptr, 0u, cap};
+}
 
 struct Counters {
     uint32_t ext, shadow, paths, pops, inner, objs, tris, shade;

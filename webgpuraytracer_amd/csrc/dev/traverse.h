@@ -189,7 +189,12 @@ __device__ void closest_hit(const DevScene& S, Flags<F> fl, v3 o, v3 d, LdsStack
             for (;;) {
                 if (!have) {
                     if (in_blas) {
-                        if (st.sp == blas_base) { in_blas = false; continue; }
+                        if (st.sp == blas_base) {
+                            in_blas = false;
+                            lo = o;
+                            linv = inv;
+                            continue;
+                        }
                         ref = st.pop();
                     } else if (cur < end) {
                         const uint32_t i = cur++;
@@ -197,12 +202,14 @@ __device__ void closest_hit(const DevScene& S, Flags<F> fl, v3 o, v3 d, LdsStack
                         const pt_object& ob = S.objs[i];
                         const int type = ob.object_type;
                         if (fl.has(PT_HAS_MESHES) && type == PT_OBJ_MESH) {
-                            lo = xform_point(ob.inv_matrix, o);
-                            ld = xform_dir(ob.inv_matrix, d);
-                            linv = splat(1.0f) / ld;
+                            const v3 mo = xform_point(ob.inv_matrix, o), md = xform_dir(ob.inv_matrix, d);
+                            const v3 minv = splat(1.0f) / md;
                             const Root& br = S.roots[S.obj_root[i]];
-                            const float b2 = slab(lo, linv, ld3(br.mn), ld3(br.mx));
+                            const float b2 = slab(mo, minv, ld3(br.mn), ld3(br.mx));
                             if (!(b2 < 0.0f || b2 >= h.t)) {
+                                lo = mo;
+                                ld = md;
+                                linv = minv;
                                 tri_off = f2u(ob.param1);
                                 obj = int(i);
                                 blas_base = st.sp;
@@ -259,10 +266,10 @@ __device__ void closest_hit(const DevScene& S, Flags<F> fl, v3 o, v3 d, LdsStack
                         cur = s0;
                         end = s0 + n0;
                     }
-                } else if (in_blas) {
-                    have = visit_node<W, CN>(S.bnodes, S.wbnodes, ref, lo, linv, h.t, st, c, ref);
                 } else {
-                    have = visit_node<W, CN>(S.tnodes, S.wtnodes, ref, o, inv, h.t, st, c, ref);
+                    // one call site for both levels: (lo, linv) is the ray in the current level's space
+                    have = visit_node<W, CN>(in_blas ? S.bnodes : S.tnodes, in_blas ? S.wbnodes : S.wtnodes, ref, lo,
+                                             linv, h.t, st, c, ref);
                 }
             }
         }
@@ -337,7 +344,12 @@ __device__ bool occluded(const DevScene& S, Flags<F> fl, v3 o, v3 d, int target,
         for (;;) {
             if (!have) {
                 if (in_blas) {
-                    if (st.sp == blas_base) { in_blas = false; continue; }
+                    if (st.sp == blas_base) {
+                        in_blas = false;
+                        lo = o;
+                        linv = inv;
+                        continue;
+                    }
                     ref = st.pop();
                 } else if (cur < end) {
                     const uint32_t i = cur++;
@@ -346,12 +358,14 @@ __device__ bool occluded(const DevScene& S, Flags<F> fl, v3 o, v3 d, int target,
                     const pt_object& ob = S.objs[i];
                     const int type = ob.object_type;
                     if (fl.has(PT_HAS_MESHES) && type == PT_OBJ_MESH) {
-                        lo = xform_point(ob.inv_matrix, o);
-                        ld = xform_dir(ob.inv_matrix, d);
-                        linv = splat(1.0f) / ld;
+                        const v3 mo = xform_point(ob.inv_matrix, o), md = xform_dir(ob.inv_matrix, d);
+                        const v3 minv = splat(1.0f) / md;
                         const Root& br = S.roots[S.obj_root[i]];
-                        const float b2 = slab(lo, linv, ld3(br.mn), ld3(br.mx));
+                        const float b2 = slab(mo, minv, ld3(br.mn), ld3(br.mx));
                         if (!(b2 < 0.0f || b2 >= tdist)) {
+                            lo = mo;
+                            ld = md;
+                            linv = minv;
                             tri_off = f2u(ob.param1);
                             blas_base = st.sp;
                             in_blas = true;
@@ -391,10 +405,9 @@ __device__ bool occluded(const DevScene& S, Flags<F> fl, v3 o, v3 d, int target,
                     cur = s0;
                     end = s0 + n0;
                 }
-            } else if (in_blas) {
-                have = visit_node<W, CN>(S.bnodes, S.wbnodes, ref, lo, linv, tdist, st, c, ref);
             } else {
-                have = visit_node<W, CN>(S.tnodes, S.wtnodes, ref, o, inv, tdist, st, c, ref);
+                have = visit_node<W, CN>(in_blas ? S.bnodes : S.tnodes, in_blas ? S.wbnodes : S.wtnodes, ref, lo, linv,
+                                         tdist, st, c, ref);
             }
         }
     }

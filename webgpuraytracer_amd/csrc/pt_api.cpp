@@ -72,7 +72,8 @@ struct pt_ctx {
     DevBuf wf_o, wf_d, wf_t, wf_r, wf_meta, wf_ha, wf_hb, wf_media, wf_q0, wf_q1, wf_sq, wf_so, wf_sd, wf_stp, wf_sfi,
         wf_cnt;
     // persistent megakernel: per-lane medium stacks, per-XCD tile counters
-    DevBuf mk_media, mk_tiles;
+    DevBuf mk_media, mk_tiles, mk_spill;
+    uint32_t lds_stack_opt = 16;  // traversal stack entries per lane in LDS (0: all)
     int n_cu = 0;
 
     int fail(int code, const char* fmt, ...) {
@@ -114,7 +115,7 @@ struct pt_ctx {
                          &planes, &meshes, &mesh_root, &lights, &sky, &cdf, &cdf_rows, &accum, &out, &gbuf, &dn_a,
                          &dn_b, &dn_fin, &dn_guide, &stats, &seeds, &rays_in, &rays_aux, &rays_aux2, &rays_out,
                          &wf_o, &wf_d, &wf_t, &wf_r, &wf_meta, &wf_ha, &wf_hb, &wf_media, &wf_q0, &wf_q1, &wf_sq,
-                         &wf_so, &wf_sd, &wf_stp, &wf_sfi, &wf_cnt, &mk_media, &mk_tiles};
+                         &wf_so, &wf_sd, &wf_stp, &wf_sfi, &wf_cnt, &mk_media, &mk_tiles, &mk_spill};
         for (DevBuf* b : all) b->release();
     }
 };
@@ -327,6 +328,11 @@ int pt_set_option(pt_ctx* c, int key, int64_t value) {
             if (value < 1024 || value > (1ll << 28)) return c->fail(PT_ERR_ARG, "pool size must be in [1024, 2^28]");
             c->wf_pool_opt = uint64_t(value);
             return PT_OK;
+        case PT_OPT_LDS_STACK:
+            if (value < 0 || value > kMaxStack)
+                return c->fail(PT_ERR_ARG, "LDS stack entries must be in [0, %u]", kMaxStack);
+            c->lds_stack_opt = uint32_t(value);
+            return PT_OK;
         case PT_OPT_TRAVERSAL:
             if (value != PT_TRAVERSAL_REFERENCE && value != PT_TRAVERSAL_WIDE)
                 return c->fail(PT_ERR_ARG, "unknown traversal %lld", (long long)value);
@@ -538,6 +544,9 @@ static int run_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t 
     } else {
         const bool cn = c->count_stats && !hits_dev;
         a.n_tiles = ((a.sec_w + 7u) / 8u) * ((a.sec_h + 7u) / 8u);
+        // the kernel variant: wide unless counting or recording (both use the reference order)
+        const uint32_t total = (a.wide && !cn && !hits_dev) ? c->wstack_cap : c->stack_cap;
+        a.stack_lds = c->lds_stack_opt ? std::min<uint32_t>(total, c->lds_stack_opt) : total;
         int bpc = 0;
         e = trace_blocks_per_cu(c->dev, a, cn, &bpc);
         if (e != hipSuccess) return c->hip_fail(e, "trace occupancy");
@@ -547,6 +556,8 @@ static int run_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t 
         a.media_stride = grid * 64u;
         if ((rc = c->alloc(c->mk_media, size_t(a.media_stride) * kLaneMediaFloats * 4u, "medium stacks"))) return rc;
         if ((rc = c->alloc(c->mk_tiles, 8 * 4, "tile counters"))) return rc;
+        if ((rc = c->alloc(c->mk_spill, size_t(a.media_stride) * (total - a.stack_lds) * 4u, "stack spill"))) return rc;
+        a.stack_spill = static_cast<uint32_t*>(c->mk_spill.p);
         a.media = static_cast<float*>(c->mk_media.p);
         a.tile_ctr = static_cast<uint32_t*>(c->mk_tiles.p);
         e = launch_zero_u64(static_cast<unsigned long long*>(c->mk_tiles.p), 4, c->stream);

@@ -29,6 +29,8 @@ struct TraceArgs {
     uint32_t* tile_ctr;          // 8 per-XCD band counters, zero at launch
     float* media;                // kMediaFloats per lane of the grid, structure of arrays
     uint32_t media_stride;       // grid workgroups x 64
+    uint32_t stack_lds;          // traversal stack entries per lane kept in LDS
+    uint32_t* stack_spill;       // the rest: (stack capacity - stack_lds) per lane, same layout as media
 };
 
 struct RayArgs {

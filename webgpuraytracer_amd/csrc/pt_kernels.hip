@@ -155,15 +155,17 @@ __device__ uint32_t next_tile(uint32_t* ctr, uint32_t n_tiles) {
 
 // Persistent: the grid holds as many workgroups as are resident at once and
 // each loops over tiles from next_tile.  Lanes keep their medium stacks in
-// A.media (structure of arrays, one column per lane of the grid) and their
-// traversal stacks in LDS.
+// A.media (structure of arrays, one column per lane of the grid) and the
+// first A.stack_lds entries of their traversal stacks in LDS, the rest in
+// A.stack_spill (same column layout).
 //
 // Occupancy: the kernel needs ~150-165 VGPRs, which allows 3 waves per SIMD.
-// Capping it at 128 VGPRs (4 waves per SIMD) spills a few cold values but
-// hides more of the traversal's dependent-load latency: 4% faster on the
-// 808k-triangle scene.
+// Capping it at 96 VGPRs (5 waves per SIMD) spills some values of the
+// shading code but hides more of the traversal's dependent-load latency:
+// on the 808k-triangle scene 4 waves/SIMD run 4% faster than 3, and 5 another
+// 5% faster than 4.
 #ifndef PT_TRACE_WAVES_PER_EU
-#define PT_TRACE_WAVES_PER_EU 4
+#define PT_TRACE_WAVES_PER_EU 5
 #endif
 template <uint32_t F, bool CN, int MODE, bool W>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(PT_TRACE_WAVES_PER_EU)))
@@ -172,8 +174,10 @@ trace_kernel(DevScene S, TraceArgs A) {
     const Flags<F> fl{S.flags};
     const uint32_t tiles_x = (A.sec_w + kTile - 1) / kTile;
     const uint32_t lane = threadIdx.x;
-    LdsStack st{lds_stack + lane, blockDim.x, W ? S.wstack_cap : S.stack_cap, 0};
-    float* media = A.media + blockIdx.x * kWave + lane;
+    const uint32_t gid = blockIdx.x * kWave + lane;
+    LdsStack st{lds_stack + lane, kWave, A.stack_lds, 0, A.stack_spill + gid, A.media_stride,
+                W ? S.wstack_cap : S.stack_cap};
+    float* media = A.media + gid;
     const Params P{A.P.sky_total_lum, A.P.sky_width, A.P.sky_height, A.P.sky_cdf_index};
     Counters c;
     c.clear();
@@ -247,7 +251,7 @@ __global__ void __launch_bounds__(kWave) gbuffer_kernel(DevScene S, pt_scene_par
     const uint32_t px = (tile % tiles_x) * kTile + (lane & 7u);
     const uint32_t py = (tile / tiles_x) * kTile + (lane >> 3);
     if (px >= P.width || py >= P.height) return;
-    LdsStack st{lds_stack + lane, blockDim.x, S.stack_cap, 0};
+    LdsStack st = lds_stack_of(lds_stack + lane, blockDim.x, S.stack_cap);
     Counters c;
     c.clear();
     const uint32_t idx = py * P.width + px;
@@ -313,7 +317,7 @@ __global__ void __launch_bounds__(kWave) rays_kernel(DevScene S, RayArgs A, int 
     const Flags<F> fl{S.flags};
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
-    LdsStack st{lds_stack + threadIdx.x, blockDim.x, W ? S.wstack_cap : S.stack_cap, 0};
+    LdsStack st = lds_stack_of(lds_stack + threadIdx.x, blockDim.x, W ? S.wstack_cap : S.stack_cap);
     Counters c;
     c.clear();
     const float* r = A.rays + size_t(i) * 6;
@@ -550,7 +554,7 @@ static size_t stack_bytes(const DevScene& S, uint32_t threads, bool wide) {
 template <uint32_t F, bool CN, int MODE, bool W>
 static hipError_t trace_variant(const DevScene& S, const TraceArgs& a, uint32_t grid, int* blocks_per_cu,
                                 hipStream_t s) {
-    const size_t lds = stack_bytes(S, kWave, W);
+    const size_t lds = size_t(a.stack_lds) * kWave * 4u;
     if (blocks_per_cu)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, trace_kernel<F, CN, MODE, W>, kWave, lds);
     hipLaunchKernelGGL((trace_kernel<F, CN, MODE, W>), dim3(grid), dim3(kWave), lds, s, S, a);

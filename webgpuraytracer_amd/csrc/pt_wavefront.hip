@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(kWfBlock) wf_extend(DevScene S, WfArgs A, int 
     if (i < n) {
         const uint32_t p = A.queue[cur][i];
         const float4 o = A.o_pdf[p], d = A.d_wsum[p];
-        LdsStack st{lds_stack + threadIdx.x, blockDim.x, W ? S.wstack_cap : S.stack_cap, 0};
+        LdsStack st = lds_stack_of(lds_stack + threadIdx.x, blockDim.x, W ? S.wstack_cap : S.stack_cap);
         HitRec h;
         closest_hit<F, CN, W>(S, fl, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), st, h, c);
         A.hit_a[p] = make_int4(__float_as_int(h.t), (h.kind << 30) | (h.idx & 0x3fffffff), h.prim, int(h.count));
@@ -208,7 +208,7 @@ __global__ void __launch_bounds__(kWfBlock) wf_shadow(DevScene S, WfArgs A) {
     c.clear();
     if (i < n) {
         const uint32_t p = A.shadow_queue[i];
-        LdsStack st{lds_stack + threadIdx.x, blockDim.x, W ? S.wstack_cap : S.stack_cap, 0};
+        LdsStack st = lds_stack_of(lds_stack + threadIdx.x, blockDim.x, W ? S.wstack_cap : S.stack_cap);
         float4 r = A.rad[p];
         v3 rad = mk(r.x, r.y, r.z);
 #pragma unroll
