@@ -155,133 +155,6 @@ __device__ void blas_closest(const DevScene& S, uint32_t root_slot, const float*
     }
 }
 
-// trace_scene (shader.wgsl:754-774): planes, then the TLAS, then list meshes.
-// The two BVH levels run in one loop: `ref` is the node visited next, or
-// have == false when the next work is a TLAS-leaf object or a stack pop.
-template <uint32_t F, bool CN, bool W>
-__device__ void closest_hit(const DevScene& S, Flags<F> fl, v3 o, v3 d, LdsStack& st, HitRec& h, Counters& c) {
-    h.t = 1e10f;
-    h.count = 0;
-    h.kind = kMiss;
-    h.idx = -1;
-    h.prim = -1;
-    h.bu = h.bv = 0.0f;
-    if (fl.has(PT_HAS_PLANES)) {
-        for (uint32_t i = 0; i < S.n_planes; ++i) {
-            const pt_plane& p = S.planes[i];
-            const float t = plane_t(ld3(p.normal), p.d, o, d);
-            if (t > 0.001f && t < h.t) { h.t = t; h.kind = kPlaneHit; h.idx = int(i); h.prim = -1; }
-        }
-    }
-    if (fl.has(kAnyObject)) {
-        const v3 inv = splat(1.0f) / d;
-        const Root& R = S.roots[0];
-        const float bt = slab(o, inv, ld3(R.mn), ld3(R.mx));
-        if (!(bt < 0.0f || bt >= h.t)) {
-            st.sp = 0;
-            uint32_t ref = root_ref<W>(R);
-            bool have = true;
-            bool in_blas = false;
-            int blas_base = 0;
-            uint32_t cur = 0, end = 0, tri_off = 0;
-            int obj = -1;
-            v3 lo = o, ld = d, linv = inv;
-            for (;;) {
-                if (!have) {
-                    if (in_blas) {
-                        if (st.sp == blas_base) {
-                            in_blas = false;
-                            lo = o;
-                            linv = inv;
-                            continue;
-                        }
-                        ref = st.pop();
-                    } else if (cur < end) {
-                        const uint32_t i = cur++;
-                        if (CN) c.objs++;
-                        const pt_object& ob = S.objs[i];
-                        const int type = ob.object_type;
-                        if (fl.has(PT_HAS_MESHES) && type == PT_OBJ_MESH) {
-                            const v3 mo = xform_point(ob.inv_matrix, o), md = xform_dir(ob.inv_matrix, d);
-                            const v3 minv = splat(1.0f) / md;
-                            const Root& br = S.roots[S.obj_root[i]];
-                            const float b2 = slab(mo, minv, ld3(br.mn), ld3(br.mx));
-                            if (!(b2 < 0.0f || b2 >= h.t)) {
-                                lo = mo;
-                                ld = md;
-                                linv = minv;
-                                tri_off = f2u(ob.param1);
-                                obj = int(i);
-                                blas_base = st.sp;
-                                in_blas = true;
-                                ref = root_ref<W>(br);
-                                have = true;
-                            }
-                        } else if (fl.has(PT_HAS_SPHERES) && type == PT_OBJ_SPHERE) {
-                            const float t = unit_sphere(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d));
-                            if (t > 0.001f && t < h.t) { h.t = t; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
-                        } else if (fl.has(PT_HAS_CUBES) && type == PT_OBJ_CUBE) {
-                            const float t = unit_cube(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d));
-                            if (t > 0.001f && t < h.t) { h.t = t; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
-                        } else if (fl.has(PT_HAS_CYLINDERS) && type == PT_OBJ_CYLINDER) {
-                            int which;
-                            v3 ps;
-                            const float t = cone_cylinder(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d),
-                                                          1.0f, ob.param0, which, ps);
-                            if (which >= 0 && t < h.t) { h.t = t; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
-                        } else if (fl.has(PT_HAS_TORI) && type == PT_OBJ_TORUS) {
-                            v3 tl = xform_dir(ob.inv_matrix, d);
-                            const float rs = len(tl);
-                            tl = tl / rs;
-                            const float t = torus_t(xform_point(ob.inv_matrix, o), tl, 1.0f, ob.param0 * 1.0f);
-                            if (t > 0.0f) {
-                                const float wt = t / rs;
-                                if (wt < h.t) { h.t = wt; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
-                            }
-                        }
-                        continue;
-                    } else {
-                        if (st.sp == 0) break;
-                        ref = st.pop();
-                    }
-                }
-                // visit `ref` (the reference's pop)
-                h.count++;
-                if (CN) c.pops++;
-                have = false;
-                if (ref & kLeafBit) {
-                    uint32_t s0, n0;
-                    if (in_blas) {
-                        leaf_range(ref, S.bvh_raw, s0, n0);
-                        for (uint32_t k = tri_off + s0; k < tri_off + s0 + n0; ++k) {
-                            if (CN) c.tris++;
-                            float u, v;
-                            const float t = tri_hit(lo, ld, S.tri_geo + 3u * k, u, v);
-                            if (t >= 0.0f && t < h.t) {
-                                h.t = t; h.kind = kObjectHit; h.idx = obj; h.prim = int(k); h.bu = u; h.bv = v;
-                            }
-                        }
-                    } else {
-                        leaf_range(ref, S.tlas_raw, s0, n0);
-                        cur = s0;
-                        end = s0 + n0;
-                    }
-                } else {
-                    // one call site for both levels: (lo, linv) is the ray in the current level's space
-                    have = visit_node<W, CN>(in_blas ? S.bnodes : S.tnodes, in_blas ? S.wbnodes : S.wtnodes, ref, lo,
-                                             linv, h.t, st, c, ref);
-                }
-            }
-        }
-    }
-    if (fl.has(PT_HAS_LIST_MESHES)) {
-        for (uint32_t m = 0; m < S.n_meshes; ++m) {
-            const pt_mesh_instance& mi = S.meshes[m];
-            blas_closest<CN, W>(S, S.mesh_root[m], mi.inv_matrix, mi.tri_offset, o, d, int(m), kListMeshHit, st, h, c);
-        }
-    }
-}
-
 template <bool CN, bool W>
 __device__ bool blas_any(const DevScene& S, uint32_t root_slot, const float* inv_m, uint32_t tri_off, v3 o, v3 d,
                          float tdist, LdsStack& st, Counters& c) {
@@ -315,109 +188,244 @@ __device__ bool blas_any(const DevScene& S, uint32_t root_slot, const float* inv
     return false;
 }
 
-// trace_scene_shadow (shader.wgsl:1206-1229).  Without albedo textures
-// every occluder blocks outright (evaluate_shadow_attenuation returns true,
-// shader.wgsl:777-792), so the transmittance is 0 or 1.
+// trace_scene (shader.wgsl:754-774) and trace_scene_shadow (1206-1229) as
+// one resumable traversal.  begin_closest / begin_any test the planes (and,
+// for closest hits, the TLAS root box); each step() does one unit of
+// TLAS/BLAS work (pop a node, visit it, or intersect one TLAS-leaf object)
+// and returns false when the trees are done; finish() walks the list meshes.
+// A kernel can run other lanes' work between steps; closest_hit runs them
+// back to back.  `ref` is the node visited next, or have == false when the
+// next work is a TLAS-leaf object or a stack pop.
+//
+// Any-hit queries (shadow rays) keep their target distance in h.t, so the
+// culling tests are shared; the first accepted occluder sets `blocked` and
+// ends the query.  Without albedo textures every occluder blocks outright
+// (evaluate_shadow_attenuation, shader.wgsl:777-792), so the transmittance is
+// 0 or 1.
+template <uint32_t F, bool CN, bool W>
+struct Trav {
+    v3 o, d, inv;     // world-space ray
+    v3 lo, ld, linv;  // the ray in the current level's space
+    uint32_t ref, cur, end, tri_off;
+    int blas_base, obj, target;
+    bool have, in_blas, any, blocked;
+
+    __device__ __forceinline__ void start(const DevScene& S, LdsStack& st) {
+        inv = splat(1.0f) / d;
+        st.sp = 0;
+        ref = root_ref<W>(S.roots[0]);
+        have = true;
+        in_blas = false;
+        blas_base = 0;
+        cur = end = tri_off = 0;
+        obj = -1;
+        lo = o;
+        ld = d;
+        linv = inv;
+    }
+
+    // Returns true when the trees must be traversed.
+    __device__ __forceinline__ bool begin_closest(const DevScene& S, Flags<F> fl, v3 o_, v3 d_, LdsStack& st,
+                                                  HitRec& h) {
+        o = o_;
+        d = d_;
+        any = false;
+        blocked = false;
+        target = -1;
+        h.t = 1e10f;
+        h.count = 0;
+        h.kind = kMiss;
+        h.idx = -1;
+        h.prim = -1;
+        h.bu = h.bv = 0.0f;
+        if (fl.has(PT_HAS_PLANES)) {
+            for (uint32_t i = 0; i < S.n_planes; ++i) {
+                const pt_plane& p = S.planes[i];
+                const float t = plane_t(ld3(p.normal), p.d, o, d);
+                if (t > 0.001f && t < h.t) { h.t = t; h.kind = kPlaneHit; h.idx = int(i); h.prim = -1; }
+            }
+        }
+        if (!fl.has(kAnyObject)) return false;
+        const Root& R = S.roots[0];
+        const float bt = slab(o, splat(1.0f) / d, ld3(R.mn), ld3(R.mx));
+        if (bt < 0.0f || bt >= h.t) return false;
+        start(S, st);
+        return true;
+    }
+
+    // Shadow query toward tdist, skipping object `target`.  Returns true
+    // when the trees must be traversed (false: decided, see `blocked`).
+    __device__ __forceinline__ bool begin_any(const DevScene& S, Flags<F> fl, v3 o_, v3 d_, int target_, float tdist,
+                                              LdsStack& st, HitRec& h) {
+        o = o_;
+        d = d_;
+        any = true;
+        blocked = false;
+        target = target_;
+        h.t = tdist;
+        if (fl.has(PT_HAS_PLANES)) {
+            for (uint32_t i = 0; i < S.n_planes; ++i) {
+                const pt_plane& p = S.planes[i];
+                const v3 n = ld3(p.normal);
+                const float denom = dot(n, d);
+                if (fabsf(denom) > 1e-6f) {
+                    const float t = (p.d - dot(n, o)) / denom;
+                    if (t > 0.001f && t < tdist) {
+                        blocked = true;
+                        return false;
+                    }
+                }
+            }
+        }
+        if (!fl.has(kAnyObject)) return false;
+        start(S, st);  // no root box test for shadow rays (shader.wgsl:1134-1144)
+        return true;
+    }
+
+    __device__ __forceinline__ bool step(const DevScene& S, Flags<F> fl, LdsStack& st, HitRec& h, Counters& c) {
+        if (!have) {
+            if (in_blas) {
+                if (st.sp == blas_base) {
+                    in_blas = false;
+                    lo = o;
+                    linv = inv;
+                    return true;
+                }
+                ref = st.pop();
+            } else if (cur < end) {
+                const uint32_t i = cur++;
+                if (any && int(i) == target) return true;
+                if (CN) c.objs++;
+                const pt_object& ob = S.objs[i];
+                const int type = ob.object_type;
+                if (fl.has(PT_HAS_MESHES) && type == PT_OBJ_MESH) {
+                    const v3 mo = xform_point(ob.inv_matrix, o), md = xform_dir(ob.inv_matrix, d);
+                    const v3 minv = splat(1.0f) / md;
+                    const Root& br = S.roots[S.obj_root[i]];
+                    const float b2 = slab(mo, minv, ld3(br.mn), ld3(br.mx));
+                    if (!(b2 < 0.0f || b2 >= h.t)) {
+                        lo = mo;
+                        ld = md;
+                        linv = minv;
+                        tri_off = f2u(ob.param1);
+                        obj = int(i);
+                        blas_base = st.sp;
+                        in_blas = true;
+                        ref = root_ref<W>(br);
+                        have = true;
+                    }
+                } else if (any) {
+                    bool b = false;
+                    if (fl.has(PT_HAS_SPHERES) && type == PT_OBJ_SPHERE)
+                        b = sphere_blocks(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), h.t);
+                    else if (fl.has(PT_HAS_CUBES) && type == PT_OBJ_CUBE)
+                        b = cube_blocks(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), h.t);
+                    else if (fl.has(PT_HAS_CYLINDERS) && type == PT_OBJ_CYLINDER)
+                        b = cylinder_blocks(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), ob.param0, h.t);
+                    else if (fl.has(PT_HAS_TORI) && type == PT_OBJ_TORUS)
+                        b = torus_blocks(o, d, ob.inv_matrix, ob.param0, h.t);
+                    if (b) {
+                        blocked = true;
+                        return false;
+                    }
+                } else if (fl.has(PT_HAS_SPHERES) && type == PT_OBJ_SPHERE) {
+                    const float t = unit_sphere(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d));
+                    if (t > 0.001f && t < h.t) { h.t = t; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
+                } else if (fl.has(PT_HAS_CUBES) && type == PT_OBJ_CUBE) {
+                    const float t = unit_cube(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d));
+                    if (t > 0.001f && t < h.t) { h.t = t; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
+                } else if (fl.has(PT_HAS_CYLINDERS) && type == PT_OBJ_CYLINDER) {
+                    int which;
+                    v3 ps;
+                    const float t = cone_cylinder(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), 1.0f,
+                                                  ob.param0, which, ps);
+                    if (which >= 0 && t < h.t) { h.t = t; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
+                } else if (fl.has(PT_HAS_TORI) && type == PT_OBJ_TORUS) {
+                    v3 tl = xform_dir(ob.inv_matrix, d);
+                    const float rs = len(tl);
+                    tl = tl / rs;
+                    const float t = torus_t(xform_point(ob.inv_matrix, o), tl, 1.0f, ob.param0 * 1.0f);
+                    if (t > 0.0f) {
+                        const float wt = t / rs;
+                        if (wt < h.t) { h.t = wt; h.kind = kObjectHit; h.idx = int(i); h.prim = -1; }
+                    }
+                }
+                return true;
+            } else {
+                if (st.sp == 0) return false;
+                ref = st.pop();
+            }
+        }
+        // visit `ref` (the reference's pop)
+        if (!any) h.count++;
+        if (CN) c.pops++;
+        have = false;
+        if (ref & kLeafBit) {
+            uint32_t s0, n0;
+            if (in_blas) {
+                leaf_range(ref, S.bvh_raw, s0, n0);
+                for (uint32_t k = tri_off + s0; k < tri_off + s0 + n0; ++k) {
+                    if (CN) c.tris++;
+                    float u, v;
+                    const float t = tri_hit(lo, ld, S.tri_geo + 3u * k, u, v);
+                    if (t >= 0.0f && t < h.t) {
+                        if (any) {
+                            blocked = true;
+                            return false;
+                        }
+                        h.t = t; h.kind = kObjectHit; h.idx = obj; h.prim = int(k); h.bu = u; h.bv = v;
+                    }
+                }
+            } else {
+                leaf_range(ref, S.tlas_raw, s0, n0);
+                cur = s0;
+                end = s0 + n0;
+            }
+        } else {
+            // one call site for both levels: (lo, linv) is the ray in the current level's space
+            have = visit_node<W, CN>(in_blas ? S.bnodes : S.tnodes, in_blas ? S.wbnodes : S.wtnodes, ref, lo, linv,
+                                     h.t, st, c, ref);
+        }
+        return true;
+    }
+
+    __device__ __forceinline__ void finish(const DevScene& S, Flags<F> fl, LdsStack& st, HitRec& h, Counters& c) {
+        if (fl.has(PT_HAS_LIST_MESHES)) {
+            for (uint32_t m = 0; m < S.n_meshes; ++m) {
+                const pt_mesh_instance& mi = S.meshes[m];
+                if (any) {
+                    if (blocked) break;
+                    blocked = blas_any<CN, W>(S, S.mesh_root[m], mi.inv_matrix, mi.tri_offset, o, d, h.t, st, c);
+                } else {
+                    blas_closest<CN, W>(S, S.mesh_root[m], mi.inv_matrix, mi.tri_offset, o, d, int(m), kListMeshHit,
+                                        st, h, c);
+                }
+            }
+        }
+    }
+};
+
+template <uint32_t F, bool CN, bool W>
+__device__ void closest_hit(const DevScene& S, Flags<F> fl, v3 o, v3 d, LdsStack& st, HitRec& h, Counters& c) {
+    Trav<F, CN, W> q;
+    if (q.begin_closest(S, fl, o, d, st, h))
+        while (q.step(S, fl, st, h, c)) {
+        }
+    q.finish(S, fl, st, h, c);
+}
+
+// trace_scene_shadow (shader.wgsl:1206-1229): true when an occluder lies
+// before tdist (object `target` skipped).
 template <uint32_t F, bool CN, bool W>
 __device__ bool occluded(const DevScene& S, Flags<F> fl, v3 o, v3 d, int target, float tdist, LdsStack& st,
                          Counters& c) {
-    if (fl.has(PT_HAS_PLANES)) {
-        for (uint32_t i = 0; i < S.n_planes; ++i) {
-            const pt_plane& p = S.planes[i];
-            const v3 n = ld3(p.normal);
-            const float denom = dot(n, d);
-            if (fabsf(denom) > 1e-6f) {
-                const float t = (p.d - dot(n, o)) / denom;
-                if (t > 0.001f && t < tdist) return true;
-            }
+    Trav<F, CN, W> q;
+    HitRec h;
+    if (q.begin_any(S, fl, o, d, target, tdist, st, h))
+        while (q.step(S, fl, st, h, c)) {
         }
-    }
-    if (fl.has(kAnyObject)) {
-        const v3 inv = splat(1.0f) / d;
-        st.sp = 0;
-        uint32_t ref = root_ref<W>(S.roots[0]);  // no root box test here (shader.wgsl:1134-1144)
-        bool have = true;
-        bool in_blas = false;
-        int blas_base = 0;
-        uint32_t cur = 0, end = 0, tri_off = 0;
-        v3 lo = o, ld = d, linv = inv;
-        for (;;) {
-            if (!have) {
-                if (in_blas) {
-                    if (st.sp == blas_base) {
-                        in_blas = false;
-                        lo = o;
-                        linv = inv;
-                        continue;
-                    }
-                    ref = st.pop();
-                } else if (cur < end) {
-                    const uint32_t i = cur++;
-                    if (int(i) == target) continue;
-                    if (CN) c.objs++;
-                    const pt_object& ob = S.objs[i];
-                    const int type = ob.object_type;
-                    if (fl.has(PT_HAS_MESHES) && type == PT_OBJ_MESH) {
-                        const v3 mo = xform_point(ob.inv_matrix, o), md = xform_dir(ob.inv_matrix, d);
-                        const v3 minv = splat(1.0f) / md;
-                        const Root& br = S.roots[S.obj_root[i]];
-                        const float b2 = slab(mo, minv, ld3(br.mn), ld3(br.mx));
-                        if (!(b2 < 0.0f || b2 >= tdist)) {
-                            lo = mo;
-                            ld = md;
-                            linv = minv;
-                            tri_off = f2u(ob.param1);
-                            blas_base = st.sp;
-                            in_blas = true;
-                            ref = root_ref<W>(br);
-                            have = true;
-                        }
-                    } else if (fl.has(PT_HAS_SPHERES) && type == PT_OBJ_SPHERE) {
-                        if (sphere_blocks(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), tdist)) return true;
-                    } else if (fl.has(PT_HAS_CUBES) && type == PT_OBJ_CUBE) {
-                        if (cube_blocks(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), tdist)) return true;
-                    } else if (fl.has(PT_HAS_CYLINDERS) && type == PT_OBJ_CYLINDER) {
-                        if (cylinder_blocks(xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), ob.param0, tdist))
-                            return true;
-                    } else if (fl.has(PT_HAS_TORI) && type == PT_OBJ_TORUS) {
-                        if (torus_blocks(o, d, ob.inv_matrix, ob.param0, tdist)) return true;
-                    }
-                    continue;
-                } else {
-                    if (st.sp == 0) break;
-                    ref = st.pop();
-                }
-            }
-            if (CN) c.pops++;
-            have = false;
-            if (ref & kLeafBit) {
-                uint32_t s0, n0;
-                if (in_blas) {
-                    leaf_range(ref, S.bvh_raw, s0, n0);
-                    for (uint32_t k = tri_off + s0; k < tri_off + s0 + n0; ++k) {
-                        if (CN) c.tris++;
-                        float u, v;
-                        const float t = tri_hit(lo, ld, S.tri_geo + 3u * k, u, v);
-                        if (t >= 0.0f && t < tdist) return true;
-                    }
-                } else {
-                    leaf_range(ref, S.tlas_raw, s0, n0);
-                    cur = s0;
-                    end = s0 + n0;
-                }
-            } else {
-                have = visit_node<W, CN>(in_blas ? S.bnodes : S.tnodes, in_blas ? S.wbnodes : S.wtnodes, ref, lo, linv,
-                                         tdist, st, c, ref);
-            }
-        }
-    }
-    if (fl.has(PT_HAS_LIST_MESHES)) {
-        for (uint32_t m = 0; m < S.n_meshes; ++m) {
-            const pt_mesh_instance& mi = S.meshes[m];
-            if (blas_any<CN, W>(S, S.mesh_root[m], mi.inv_matrix, mi.tri_offset, o, d, tdist, st, c)) return true;
-        }
-    }
-    return false;
+    q.finish(S, fl, st, h, c);
+    return q.blocked;
 }
 
 // Attributes of the winning hit (the values the reference leaves in its

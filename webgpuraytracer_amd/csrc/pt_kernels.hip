@@ -564,18 +564,18 @@ static hipError_t trace_variant(const DevScene& S, const TraceArgs& a, uint32_t 
 // Variants: hit recording and work counting use the reference-order
 // traversal (they report the reference's per-pop counts); rendering uses the
 // traversal the caller chose.
+template <uint32_t F, bool CN, bool W>
+static hipError_t trace_sched(const DevScene& S, const TraceArgs& a, uint32_t grid, int* bpc, hipStream_t s) {
+    if (a.schedule == PT_SCHEDULE_REGEN) return trace_variant<F, CN, kRegen, W>(S, a, grid, bpc, s);
+    return trace_variant<F, CN, kSync, W>(S, a, grid, bpc, s);
+}
+
 template <uint32_t F>
 static hipError_t trace_f(const DevScene& S, const TraceArgs& a, bool cn, uint32_t grid, int* bpc, hipStream_t s) {
     if (a.hits) return trace_variant<F, false, kRecord, false>(S, a, grid, bpc, s);
-    const bool regen = a.schedule == PT_SCHEDULE_REGEN;
-    if (cn)
-        return regen ? trace_variant<F, true, kRegen, false>(S, a, grid, bpc, s)
-                     : trace_variant<F, true, kSync, false>(S, a, grid, bpc, s);
-    if (a.wide)
-        return regen ? trace_variant<F, false, kRegen, true>(S, a, grid, bpc, s)
-                     : trace_variant<F, false, kSync, true>(S, a, grid, bpc, s);
-    return regen ? trace_variant<F, false, kRegen, false>(S, a, grid, bpc, s)
-                 : trace_variant<F, false, kSync, false>(S, a, grid, bpc, s);
+    if (cn) return trace_sched<F, true, false>(S, a, grid, bpc, s);
+    if (a.wide) return trace_sched<F, false, true>(S, a, grid, bpc, s);
+    return trace_sched<F, false, false>(S, a, grid, bpc, s);
 }
 
 static hipError_t trace_dispatch(const DevScene& S, const TraceArgs& a, bool cn, uint32_t grid, int* bpc,
