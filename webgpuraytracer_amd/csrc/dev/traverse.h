@@ -78,9 +78,10 @@ __device__ __forceinline__ void order2(float& ka, uint32_t& ra, float& kb, uint3
 }
 
 // Wide-node visit: all children's slabs from one 128 B fetch, accepted
-// children sorted by entry distance (a 4-input sorting network; rejected
-// ones sort last as +inf); the nearest is returned in `next`, the others are
-// pushed farthest first.  Returns false when no child is accepted.
+// children sorted by entry distance (a 4-input sorting network; rejected ones
+// sort last as +inf); the nearest is returned in `next`, the others are
+// pushed farthest first.  Shadow queries keep the order too: it finds
+// occluders near the origin first.  Returns false when no child is accepted.
 template <bool CN>
 __device__ __forceinline__ bool visit_wide(const WNode* nodes, uint32_t ref, v3 o, v3 inv, float limit, LdsStack& st,
                                            Counters& c, uint32_t& next) {
@@ -90,14 +91,19 @@ __device__ __forceinline__ bool visit_wide(const WNode* nodes, uint32_t ref, v3 
     const uint4 r = p->ref;
     const uint32_t cnt = p->meta.x;
     const float inf = __builtin_inff();
-    float k0 = slab(o, inv, mk(lx.x, ly.x, lz.x), mk(hx.x, hy.x, hz.x));
-    float k1 = slab(o, inv, mk(lx.y, ly.y, lz.y), mk(hx.y, hy.y, hz.y));
-    float k2 = slab(o, inv, mk(lx.z, ly.z, lz.z), mk(hx.z, hy.z, hz.z));
-    float k3 = slab(o, inv, mk(lx.w, ly.w, lz.w), mk(hx.w, hy.w, hz.w));
-    k0 = (k0 >= 0.0f && k0 < limit) ? k0 : inf;
-    k1 = (k1 >= 0.0f && k1 < limit) ? k1 : inf;
-    k2 = (cnt > 2u && k2 >= 0.0f && k2 < limit) ? k2 : inf;
-    k3 = (cnt > 3u && k3 >= 0.0f && k3 < limit) ? k3 : inf;
+    // The reference's slab test on each child box, all four computed
+    // unconditionally (a branch on the child count costs more than the two
+    // extra tests) and then masked.
+    auto key = [&](float ax, float ay, float az, float bx, float by, float bz) {
+        const float t = slab(o, inv, mk(ax, ay, az), mk(bx, by, bz));
+        return (t >= 0.0f && t < limit) ? t : inf;
+    };
+    float k0 = key(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x);
+    float k1 = key(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y);
+    float k2 = key(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z);
+    float k3 = key(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w);
+    k2 = cnt > 2u ? k2 : inf;
+    k3 = cnt > 3u ? k3 : inf;
     uint32_t r0 = r.x, r1 = r.y, r2 = r.z, r3 = r.w;
     order2(k0, r0, k1, r1);
     order2(k2, r2, k3, r3);
