@@ -73,7 +73,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-count", action="store_true",
                     help="skip the extra counted step that measures node/triangle work (no roofline)")
-    ap.add_argument("--backend", choices=["megakernel", "wavefront"], default="megakernel")
+    ap.add_argument("--backend", choices=["megakernel", "wavefront"], default="wavefront")
     ap.add_argument("--schedule", choices=["regen", "sync"], default="regen",
                     help="megakernel pass schedule (same bits either way)")
     ap.add_argument("--traversal", choices=["wide", "reference"], default="wide",
@@ -105,11 +105,12 @@ def main():
     mk = (A.PT_OPT_BACKEND, A.PT_BACKEND_MEGAKERNEL)
     regen, sync = (A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_REGEN), (A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_SYNC)
     wide, refo = (A.PT_OPT_TRAVERSAL, A.PT_TRAVERSAL_WIDE), (A.PT_OPT_TRAVERSAL, A.PT_TRAVERSAL_REFERENCE)
-    variants = {"megakernel/regen/wide": [mk, regen, wide],
+    wf = (A.PT_OPT_BACKEND, A.PT_BACKEND_WAVEFRONT)
+    variants = {"wavefront/wide": [wf, wide],
+                "wavefront/reference": [wf, refo],
+                "megakernel/regen/wide": [mk, regen, wide],
                 "megakernel/regen/reference": [mk, regen, refo],
-                "megakernel/sync/wide": [mk, sync, wide],
-                "wavefront/wide": [(A.PT_OPT_BACKEND, A.PT_BACKEND_WAVEFRONT), wide],
-                "wavefront/reference": [(A.PT_OPT_BACKEND, A.PT_BACKEND_WAVEFRONT), refo]}
+                "megakernel/sync/wide": [mk, sync, wide]}
     if args.backend == "wavefront":
         chosen = "wavefront/" + args.traversal
     else:

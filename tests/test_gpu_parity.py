@@ -207,33 +207,57 @@ def test_gpu_matches_golden_fixture(renderer, name):
     assert np.mean(np.all(rel <= 1e-4, axis=-1)) >= 0.97
 
 
+def _restore_defaults(renderer):
+    renderer.set_option(A.PT_OPT_BACKEND, A.PT_BACKEND_WAVEFRONT)
+    renderer.set_option(A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_REGEN)
+    renderer.set_option(A.PT_OPT_WAVEFRONT_POOL, 1 << 26)
+    renderer.set_option(A.PT_OPT_WAVEFRONT_REFILL, 32)
+
+
+@pytest.mark.parametrize("section", [None, (16, 8, 40, 24)], ids=["frame", "section"])
 @pytest.mark.parametrize("which", ["c1", "mesh"])
-def test_backends_and_schedules_bitwise(renderer, which):
-    """Both megakernel pass schedules and the wavefront backend run the same
-    shading code with the same per-pixel RNG order and add NEE contributions
-    in the same order: identical bits and identical work counts."""
+def test_backends_and_schedules_bitwise(renderer, which, section):
+    """Both megakernel pass schedules and the wavefront backend (several
+    batch sizes and refill thresholds) run the same shading code with the same
+    per-pixel RNG order and add NEE contributions in the same order:
+    identical bits and identical work counts, for a whole frame and for a
+    section of it."""
     scene = SceneBuilder.config(1, seed=81) if which == "c1" else small_mesh_scene(seed=13)
     w, h = 80, 48
     renderer.set_scene(scene)
     seeds = [11, 22, 33, 44, 55]
     p = scene.params(w, h, sample_number=0, seed=seeds[0])
-    runs = {}
-    variants = {"regen": [(A.PT_OPT_BACKEND, A.PT_BACKEND_MEGAKERNEL), (A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_REGEN)],
-                "sync": [(A.PT_OPT_BACKEND, A.PT_BACKEND_MEGAKERNEL), (A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_SYNC)],
+    size = (0, 0)
+    if section:
+        p.section[0], p.section[1] = section[0], section[1]
+        size = section[2:]
+    npix = (size[0] * size[1]) if section else w * h
+    mk = (A.PT_OPT_BACKEND, A.PT_BACKEND_MEGAKERNEL)
+    wf = (A.PT_OPT_BACKEND, A.PT_BACKEND_WAVEFRONT)
+    variants = {"regen": [mk, (A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_REGEN)],
+                "sync": [mk, (A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_SYNC)],
                 # 2 passes per batch: 3 batches
-                "wavefront": [(A.PT_OPT_BACKEND, A.PT_BACKEND_WAVEFRONT), (A.PT_OPT_WAVEFRONT_POOL, 2 * w * h)]}
+                "wavefront_batches": [wf, (A.PT_OPT_WAVEFRONT_POOL, 2 * npix)],
+                "wavefront_refill1": [wf, (A.PT_OPT_WAVEFRONT_POOL, 1 << 26), (A.PT_OPT_WAVEFRONT_REFILL, 1)],
+                "wavefront": [wf, (A.PT_OPT_WAVEFRONT_REFILL, 32)]}
+    runs = {}
     try:
         for name, opts in variants.items():
             for k, v in opts:
                 renderer.set_option(k, v)
             renderer.resize(w, h)
-            st = renderer.render_section(p, passes=5, seeds=seeds)
+            st = renderer.render_section(p, section_size=size, passes=5, seeds=seeds)
             runs[name] = (st, renderer.read_accum(), renderer.read_output())
     finally:
-        renderer.set_option(A.PT_OPT_BACKEND, A.PT_BACKEND_MEGAKERNEL)
-        renderer.set_option(A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_REGEN)
+        _restore_defaults(renderer)
     st_m, acc_m, out_m = runs["regen"]
-    for name in ("sync", "wavefront"):
+    if section:  # pixels outside the section stay untouched
+        x0, y0, sw, sh = section
+        mask = np.ones((h, w), bool)
+        mask[y0:y0 + sh, x0:x0 + sw] = False
+        assert np.all(acc_m[mask] == 0.0)
+        assert np.any(acc_m[~mask] != 0.0)
+    for name in variants:
         st_o, acc_o, out_o = runs[name]
         assert np.array_equal(acc_m.view(np.uint32), acc_o.view(np.uint32)), name
         assert np.array_equal(out_m, out_o), name

@@ -43,7 +43,7 @@ struct pt_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
     // options
-    int backend = PT_BACKEND_MEGAKERNEL;
+    int backend = PT_BACKEND_WAVEFRONT;
     int schedule = PT_SCHEDULE_REGEN;
     int traversal = PT_TRAVERSAL_WIDE;
     bool count_stats = true;
@@ -67,12 +67,13 @@ struct pt_ctx {
     bool with_gbuffer = false;
     DevBuf accum, out, gbuf, dn_a, dn_b, dn_fin, dn_guide, stats, seeds, rays_in, rays_aux, rays_aux2, rays_out;
     // wavefront path pool
-    uint64_t wf_pool_opt = 1ull << 21;
+    uint64_t wf_pool_opt = 1ull << 26;  // paths per wavefront batch (~344 B each)
     uint32_t wf_pool = 0;
     DevBuf wf_o, wf_d, wf_t, wf_r, wf_meta, wf_ha, wf_hb, wf_media, wf_q0, wf_q1, wf_sq, wf_so, wf_sd, wf_stp, wf_sfi,
         wf_cnt;
     // persistent megakernel: per-lane medium stacks, per-XCD tile counters
-    DevBuf mk_media, mk_tiles, mk_spill;
+    DevBuf mk_media, mk_tiles, mk_spill, wf_spill;
+    uint32_t wf_refill = 32;      // wavefront traversal: idle lanes of a wave that trigger a refill
     uint32_t lds_stack_opt = 16;  // traversal stack entries per lane in LDS (0: all)
     int n_cu = 0;
 
@@ -115,7 +116,7 @@ struct pt_ctx {
                          &planes, &meshes, &mesh_root, &lights, &sky, &cdf, &cdf_rows, &accum, &out, &gbuf, &dn_a,
                          &dn_b, &dn_fin, &dn_guide, &stats, &seeds, &rays_in, &rays_aux, &rays_aux2, &rays_out,
                          &wf_o, &wf_d, &wf_t, &wf_r, &wf_meta, &wf_ha, &wf_hb, &wf_media, &wf_q0, &wf_q1, &wf_sq,
-                         &wf_so, &wf_sd, &wf_stp, &wf_sfi, &wf_cnt, &mk_media, &mk_tiles, &mk_spill};
+                         &wf_so, &wf_sd, &wf_stp, &wf_sfi, &wf_cnt, &mk_media, &mk_tiles, &mk_spill, &wf_spill};
         for (DevBuf* b : all) b->release();
     }
 };
@@ -328,6 +329,10 @@ int pt_set_option(pt_ctx* c, int key, int64_t value) {
             if (value < 1024 || value > (1ll << 28)) return c->fail(PT_ERR_ARG, "pool size must be in [1024, 2^28]");
             c->wf_pool_opt = uint64_t(value);
             return PT_OK;
+        case PT_OPT_WAVEFRONT_REFILL:
+            if (value < 1 || value > 64) return c->fail(PT_ERR_ARG, "refill threshold must be in [1, 64]");
+            c->wf_refill = uint32_t(value);
+            return PT_OK;
         case PT_OPT_LDS_STACK:
             if (value < 0 || value > kMaxStack)
                 return c->fail(PT_ERR_ARG, "LDS stack entries must be in [0, %u]", kMaxStack);
@@ -501,7 +506,20 @@ static int run_wavefront(pt_ctx* c, const TraceArgs& a) {
     w.sh_tp = static_cast<float4*>(c->wf_stp.p);
     w.sh_fi = static_cast<float4*>(c->wf_sfi.p);
     w.counters = static_cast<uint32_t*>(c->wf_cnt.p);
-    hipError_t e = hipEventRecord(c->ev0, c->stream);
+    // persistent traversal launches: grid, stack split, refill threshold
+    const bool wide = a.wide && !c->count_stats;
+    const uint32_t total = wide ? c->wstack_cap : c->stack_cap;
+    w.stack_lds = c->lds_stack_opt ? std::min<uint32_t>(total, c->lds_stack_opt) : total;
+    int bpc = 0;
+    hipError_t e = wf_trace_blocks_per_cu(c->dev, c->count_stats, a.wide, w.stack_lds, &bpc);
+    if (e != hipSuccess) return c->hip_fail(e, "wavefront occupancy");
+    if (bpc <= 0) return c->fail(PT_ERR_CAPACITY, "wavefront traversal kernel cannot be resident");
+    w.trace_grid = uint32_t(bpc) * uint32_t(c->n_cu);
+    w.spill_stride = w.trace_grid * 256u;
+    if ((rc = c->alloc(c->wf_spill, size_t(w.spill_stride) * (total - w.stack_lds) * 4u, "wf stack spill"))) return rc;
+    w.stack_spill = static_cast<uint32_t*>(c->wf_spill.p);
+    w.refill = c->wf_refill;
+    e = hipEventRecord(c->ev0, c->stream);
     for (uint32_t first = 0; e == hipSuccess && first < a.n_passes; first += uint32_t(per)) {
         w.first_pass = first;
         w.batch_passes = uint32_t(std::min<uint64_t>(per, a.n_passes - first));

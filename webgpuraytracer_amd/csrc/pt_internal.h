@@ -81,8 +81,13 @@ struct WfArgs {
     float4* sh_d;       //              direction, ignored object bits
     float4* sh_tp;      //              throughput, valid flag
     float4* sh_fi;      //              clamped unoccluded radiance
-    uint32_t* counters; // queue A length, queue B length, shadow queue length
+    uint32_t* counters; // queue A length, queue B length, shadow queue length, two fetch cursors
     int wide;           // 1: wide traversal (PT_TRAVERSAL_WIDE)
+    uint32_t trace_grid;    // persistent traversal launches: workgroups (all resident)
+    uint32_t refill;        // dynamic fetch: idle lanes of a wave that trigger a refill
+    uint32_t stack_lds;     // traversal stack entries per lane in LDS
+    uint32_t* stack_spill;  // the rest, one column per lane of trace_grid x 256
+    uint32_t spill_stride;
 };
 
 // Launchers (pt_kernels.hip).  All asynchronous on `stream`.
@@ -99,5 +104,8 @@ hipError_t launch_sky_cdf(const uint16_t* sky, uint32_t w, uint32_t h, float exp
                           double* total, hipStream_t stream);
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t stream);
 hipError_t launch_wavefront_batch(const DevScene& S, const WfArgs& a, bool count_nodes, hipStream_t stream);
+// Resident workgroups per CU of the wavefront traversal kernels (the larger of the two).
+hipError_t wf_trace_blocks_per_cu(const DevScene& S, bool count_nodes, bool wide, uint32_t stack_lds,
+                                  int* blocks_per_cu);
 
 }  // namespace pt

@@ -2,23 +2,32 @@
 //
 // A batch of paths (whole progressive passes over the section) lives in HBM
 // as structure-of-arrays state; each bounce is three launches:
-//   wf_extend  closest hit for every path in the active queue — traversal
-//              only, so it runs at a fraction of the megakernel's registers
-//   wf_shade   the bounce's shading (dev/path.h, shared with the megakernel);
-//              queues the path for the next bounce and its NEE shadow rays
-//   wf_shadow  any-hit test of the queued shadow rays; adds the light NEE then
-//              the env NEE contribution of each path, in the reference's order
+//   wf_trace<EXT>     closest hit for every path in the active queue —
+//                     traversal only, at a fraction of the megakernel's
+//                     registers
+//   wf_shade          the bounce's shading (dev/path.h, shared with the
+//                     megakernel); queues the path for the next bounce and its
+//                     NEE shadow rays
+//   wf_trace<SHADOW>  any-hit test of the queued shadow rays; adds the light
+//                     NEE then the env NEE contribution of each path, in the
+//                     reference's order
+// The two traversal launches are persistent with dynamic fetch: a wave keeps
+// its lanes busy by taking new queue entries for its idle lanes (one atomic
+// per refill) instead of idling until its longest ray ends.
 // Queues are compacted with a wave64 ballot + one atomic per wave.  Every
 // kernel reads its queue length from device memory, so a whole batch is
 // launched without a host round trip.  The per-path RNG state and draw order
 // are the megakernel's, so both backends produce the same bits.
+#include <algorithm>
+
 #include "pt_internal.h"
 #include "dev/path.h"
 
 namespace pt {
 
 constexpr int kWfBlock = 256;
-enum { kCntA = 0, kCntB = 1, kCntShadow = 2 };
+// counters: queue lengths, and the fetch cursors of the two traversal kernels
+enum { kCntA = 0, kCntB = 1, kCntShadow = 2, kCntFetchExt = 3, kCntFetchShadow = 4 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -67,6 +76,8 @@ __global__ void __launch_bounds__(kWfBlock) wf_raygen(WfArgs A) {
         A.counters[kCntA] = A.n_items;
         A.counters[kCntB] = 0u;
         A.counters[kCntShadow] = 0u;
+        A.counters[kCntFetchExt] = 0u;
+        A.counters[kCntFetchShadow] = 0u;
     }
     if (i >= A.n_items) return;
     uint32_t px, py, pass;
@@ -87,32 +98,120 @@ __global__ void __launch_bounds__(kWfBlock) wf_raygen(WfArgs A) {
     A.sh_tp[2u * i + 1u] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-template <uint32_t F, bool CN, bool W>
-__global__ void __launch_bounds__(kWfBlock) wf_extend(DevScene S, WfArgs A, int cur) {
+// Persistent traversal of a queue: the extension rays of queue[cur]
+// (SHADOW = false; writes each path's hit) or the shadow slots of the paths in
+// the shadow queue (SHADOW = true; traces slot 0 then slot 1 of a path and
+// adds their contributions to its radiance in that order).  Each wave runs
+// until the queue is drained; whenever at least A.refill of its lanes are
+// idle it takes that many entries with one atomic on the fetch cursor
+// (dynamic fetch), so short and long rays share the wave's lanes.
+template <uint32_t F, bool CN, bool W, bool SHADOW>
+__global__ void __launch_bounds__(kWfBlock) wf_trace(DevScene S, WfArgs A, int cur) {
     extern __shared__ uint32_t lds_stack[];
     const Flags<F> fl{S.flags};
-    const uint32_t n = A.counters[cur];
+    const uint32_t n = SHADOW ? A.counters[kCntShadow] : A.counters[cur];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        A.counters[cur ^ 1] = 0u;
-        A.counters[kCntShadow] = 0u;
+        // counters this launch does not read, reset for the launches after it
+        if (SHADOW) {
+            A.counters[kCntFetchExt] = 0u;
+        } else {
+            A.counters[cur ^ 1] = 0u;
+            A.counters[kCntShadow] = 0u;
+            A.counters[kCntFetchShadow] = 0u;
+        }
     }
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    if (blk * blockDim.x >= n) return;  // the whole block is past the queue
-    const uint32_t i = blk * blockDim.x + threadIdx.x;
+    uint32_t* fetch = &A.counters[SHADOW ? kCntFetchShadow : kCntFetchExt];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = lane_id();
+    LdsStack st{lds_stack + threadIdx.x, blockDim.x, A.stack_lds, 0, A.stack_spill + gid, A.spill_stride,
+                W ? S.wstack_cap : S.stack_cap};
     Counters c;
     c.clear();
-    if (i < n) {
-        const uint32_t p = A.queue[cur][i];
-        const float4 o = A.o_pdf[p], d = A.d_wsum[p];
-        LdsStack st = lds_stack_of(lds_stack + threadIdx.x, blockDim.x, W ? S.wstack_cap : S.stack_cap);
-        HitRec h;
-        closest_hit<F, CN, W>(S, fl, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), st, h, c);
+    Trav<F, CN, W> q;
+    HitRec h;
+    uint32_t p = 0;
+    int k = 0;
+    v3 rad = splat(0.0f);
+    bool busy = false, drained = false;
+    auto store_hit = [&]() {
         A.hit_a[p] = make_int4(__float_as_int(h.t), (h.kind << 30) | (h.idx & 0x3fffffff), h.prim, int(h.count));
         A.hit_b[p] = make_float2(h.bu, h.bv);
-        c.ext = 1;
+    };
+    auto apply_slot = [&]() {
+        const uint32_t sl = 2u * p + uint32_t(k);
+        const float4 tp = A.sh_tp[sl], fi = A.sh_fi[sl];
+        const v3 add = q.blocked ? splat(0.0f) : mk(fi.x, fi.y, fi.z);
+        rad = rad + mk(tp.x, tp.y, tp.z) * add;
+        A.sh_tp[sl].w = 0.0f;
+        ++k;
+    };
+    // starts the next shadow slot of path p, or stores its radiance when none is left
+    auto next_slot = [&]() {
+        for (;;) {
+            while (k < 2 && A.sh_tp[2u * p + uint32_t(k)].w == 0.0f) ++k;
+            if (k >= 2) {
+                A.rad[p] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+                busy = false;
+                return;
+            }
+            const uint32_t sl = 2u * p + uint32_t(k);
+            const float4 o = A.sh_o[sl], d = A.sh_d[sl];
+            c.shadow++;
+            if (q.begin_any(S, fl, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), __float_as_int(d.w), o.w, st, h)) {
+                busy = true;
+                return;
+            }
+            q.finish(S, fl, st, h, c);
+            apply_slot();
+        }
+    };
+    for (;;) {
+        const unsigned long long idle = __ballot(!busy);
+        if (!drained && uint32_t(__popcll(idle)) >= A.refill) {
+            const uint32_t cnt = uint32_t(__popcll(idle));
+            const uint32_t leader = uint32_t(__ffsll(static_cast<long long>(idle)) - 1);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(fetch, cnt);
+            base = __shfl(base, int(leader), 64);
+            if (base + cnt >= n) drained = true;
+            if (!busy) {
+                const uint32_t i = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+                if (i < n) {
+                    if (SHADOW) {
+                        p = A.shadow_queue[i];
+                        const float4 r = A.rad[p];
+                        rad = mk(r.x, r.y, r.z);
+                        k = 0;
+                        next_slot();
+                    } else {
+                        p = A.queue[cur][i];
+                        const float4 o = A.o_pdf[p], d = A.d_wsum[p];
+                        c.ext++;
+                        busy = q.begin_closest(S, fl, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), st, h);
+                        if (!busy) {
+                            q.finish(S, fl, st, h, c);
+                            store_hit();
+                        }
+                    }
+                }
+            }
+        }
+        if (drained && __ballot(busy) == 0ull) break;
+        if (busy) {
+            busy = q.step(S, fl, st, h, c);
+            if (!busy) {
+                q.finish(S, fl, st, h, c);
+                if (SHADOW) {
+                    apply_slot();
+                    next_slot();
+                } else {
+                    store_hit();
+                }
+            }
+        }
     }
     if (A.stats) {
-        add_stat(A.stats, kStExt, c.ext);
+        add_stat(A.stats, SHADOW ? kStShadow : kStExt, SHADOW ? c.shadow : c.ext);
         if (CN) {
             add_stat(A.stats, kStPops, c.pops);
             add_stat(A.stats, kStInner, c.inner);
@@ -196,47 +295,6 @@ __global__ void __launch_bounds__(kWfBlock) wf_shade(DevScene S, WfArgs A, int c
     if (A.stats && CN) add_stat(A.stats, kStShade, c.shade);
 }
 
-template <uint32_t F, bool CN, bool W>
-__global__ void __launch_bounds__(kWfBlock) wf_shadow(DevScene S, WfArgs A) {
-    extern __shared__ uint32_t lds_stack[];
-    const Flags<F> fl{S.flags};
-    const uint32_t n = A.counters[kCntShadow];
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    if (blk * blockDim.x >= n) return;
-    const uint32_t i = blk * blockDim.x + threadIdx.x;
-    Counters c;
-    c.clear();
-    if (i < n) {
-        const uint32_t p = A.shadow_queue[i];
-        LdsStack st = lds_stack_of(lds_stack + threadIdx.x, blockDim.x, W ? S.wstack_cap : S.stack_cap);
-        float4 r = A.rad[p];
-        v3 rad = mk(r.x, r.y, r.z);
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const uint32_t s = 2u * p + uint32_t(k);
-            const float4 tp = A.sh_tp[s];
-            if (tp.w == 0.0f) continue;
-            const float4 o = A.sh_o[s], d = A.sh_d[s], fi = A.sh_fi[s];
-            c.shadow++;
-            const bool occ = occluded<F, CN, W>(S, fl, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), __float_as_int(d.w), o.w,
-                                             st, c);
-            const v3 add = occ ? splat(0.0f) : mk(fi.x, fi.y, fi.z);
-            rad = rad + mk(tp.x, tp.y, tp.z) * add;
-            A.sh_tp[s].w = 0.0f;
-        }
-        A.rad[p] = make_float4(rad.x, rad.y, rad.z, 0.0f);
-    }
-    if (A.stats) {
-        add_stat(A.stats, kStShadow, c.shadow);
-        if (CN) {
-            add_stat(A.stats, kStPops, c.pops);
-            add_stat(A.stats, kStInner, c.inner);
-            add_stat(A.stats, kStObjs, c.objs);
-            add_stat(A.stats, kStTris, c.tris);
-        }
-    }
-}
-
 // Running mean over the batch's passes in pass order (shader.wgsl:2482-2491).
 __global__ void __launch_bounds__(kWfBlock) wf_accumulate(WfArgs A) {
     const uint32_t lp = blockIdx.x * blockDim.x + threadIdx.x;
@@ -264,12 +322,12 @@ __global__ void __launch_bounds__(kWfBlock) wf_accumulate(WfArgs A) {
 
 template <uint32_t F, bool CN, bool W>
 static void wf_bounces(const DevScene& S, const WfArgs& a0, uint32_t blocks, hipStream_t s) {
-    const size_t lds = size_t(W ? S.wstack_cap : S.stack_cap) * kWfBlock * 4u;
+    const size_t lds = size_t(a0.stack_lds) * kWfBlock * 4u;
     int cur = 0;
     for (int b = 0; b < S.bounce_limit; ++b) {
-        hipLaunchKernelGGL((wf_extend<F, CN, W>), dim3(blocks), dim3(kWfBlock), lds, s, S, a0, cur);
+        hipLaunchKernelGGL((wf_trace<F, CN, W, false>), dim3(a0.trace_grid), dim3(kWfBlock), lds, s, S, a0, cur);
         hipLaunchKernelGGL((wf_shade<F, CN>), dim3(blocks), dim3(kWfBlock), 0, s, S, a0, cur, b);
-        hipLaunchKernelGGL((wf_shadow<F, CN, W>), dim3(blocks), dim3(kWfBlock), lds, s, S, a0);
+        hipLaunchKernelGGL((wf_trace<F, CN, W, true>), dim3(a0.trace_grid), dim3(kWfBlock), lds, s, S, a0, cur);
         cur ^= 1;
     }
 }
@@ -288,6 +346,38 @@ static hipError_t wf_run(const DevScene& S, const WfArgs& a0, bool cn, hipStream
     const uint32_t npix = a0.sec_w * a0.sec_h;
     hipLaunchKernelGGL(wf_accumulate, dim3((npix + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, s, a0);
     return hipGetLastError();
+}
+
+template <uint32_t F>
+static hipError_t wf_occupancy_f(bool cn, bool wide, size_t lds, int* bpc) {
+    int a = 0, b = 0;
+    hipError_t e;
+    if (cn) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, wf_trace<F, true, false, false>, kWfBlock, lds);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, wf_trace<F, true, false, true>, kWfBlock, lds);
+    } else if (wide) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, wf_trace<F, false, true, false>, kWfBlock, lds);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, wf_trace<F, false, true, true>, kWfBlock, lds);
+    } else {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, wf_trace<F, false, false, false>, kWfBlock, lds);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, wf_trace<F, false, false, true>, kWfBlock, lds);
+    }
+    *bpc = std::max(a, b);
+    return e;
+}
+
+hipError_t wf_trace_blocks_per_cu(const DevScene& S, bool count_nodes, bool wide, uint32_t stack_lds,
+                                  int* blocks_per_cu) {
+    constexpr uint32_t kC1 = PT_HAS_SPHERES | PT_HAS_CUBES | PT_HAS_PLANES | PT_HAS_CYLINDERS | PT_HAS_LIGHTS |
+                             PT_HAS_SKYBOX | PT_MIS_SKYBOX;
+    constexpr uint32_t kC2 = PT_HAS_MESHES | PT_HAS_SKYBOX | PT_MIS_SKYBOX;
+    const size_t lds = size_t(stack_lds) * kWfBlock * 4u;
+    if (S.flags == kC1) return wf_occupancy_f<kC1>(count_nodes, wide, lds, blocks_per_cu);
+    if (S.flags == kC2) return wf_occupancy_f<kC2>(count_nodes, wide, lds, blocks_per_cu);
+    return wf_occupancy_f<kDynamicFlags>(count_nodes, wide, lds, blocks_per_cu);
 }
 
 hipError_t launch_wavefront_batch(const DevScene& S, const WfArgs& a, bool count_nodes, hipStream_t s) {
