@@ -214,7 +214,7 @@ def _restore_defaults(renderer):
     renderer.set_option(A.PT_OPT_WAVEFRONT_REFILL, 32)
 
 
-@pytest.mark.parametrize("section", [None, (16, 8, 40, 24)], ids=["frame", "section"])
+@pytest.mark.parametrize("section", [None, (16, 8, 40, 24), (5, 3, 37, 21)], ids=["frame", "section", "ragged"])
 @pytest.mark.parametrize("which", ["c1", "mesh"])
 def test_backends_and_schedules_bitwise(renderer, which, section):
     """Both megakernel pass schedules and the wavefront backend (several
