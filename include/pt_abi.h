@@ -114,13 +114,13 @@ PT_LAYOUT_ASSERT(sizeof(pt_hit) == 56, "pt_hit is 56 B");
 enum {
     PT_OPT_BACKEND = 1,     /* PT_BACKEND_* (default WAVEFRONT) */
     PT_OPT_COUNT_STATS = 2, /* 1: count node/triangle work (default 1) */
-    PT_OPT_WAVEFRONT_POOL = 3, /* paths per wavefront batch, whole passes (default 1<<26, ~344 B each) */
+    PT_OPT_WAVEFRONT_POOL = 3, /* paths per wavefront batch, whole passes (default 1<<27, ~344 B each) */
     PT_OPT_SCHEDULE = 4,       /* megakernel pass schedule, PT_SCHEDULE_* (default REGEN) */
     PT_OPT_TRAVERSAL = 5,      /* BVH traversal, PT_TRAVERSAL_* (default WIDE) */
     PT_OPT_LDS_STACK = 6,      /* traversal-stack entries per lane kept in LDS, the rest in device
                                   memory; 0 = all in LDS (default 16) */
     PT_OPT_WAVEFRONT_REFILL = 7 /* wavefront traversal: idle lanes of a wave that trigger fetching
-                                   new rays, 1..64 (default 32) */
+                                   new rays, 1..64 (default 56) */
 };
 enum { PT_BACKEND_MEGAKERNEL = 0, PT_BACKEND_WAVEFRONT = 1 };
 /* SYNC: the lanes of a wave run each pass together.  REGEN: a lane whose path

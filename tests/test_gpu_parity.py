@@ -210,8 +210,8 @@ def test_gpu_matches_golden_fixture(renderer, name):
 def _restore_defaults(renderer):
     renderer.set_option(A.PT_OPT_BACKEND, A.PT_BACKEND_WAVEFRONT)
     renderer.set_option(A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_REGEN)
-    renderer.set_option(A.PT_OPT_WAVEFRONT_POOL, 1 << 26)
-    renderer.set_option(A.PT_OPT_WAVEFRONT_REFILL, 32)
+    renderer.set_option(A.PT_OPT_WAVEFRONT_POOL, 1 << 27)
+    renderer.set_option(A.PT_OPT_WAVEFRONT_REFILL, 56)
 
 
 @pytest.mark.parametrize("section", [None, (16, 8, 40, 24), (5, 3, 37, 21)], ids=["frame", "section", "ragged"])
@@ -238,8 +238,8 @@ def test_backends_and_schedules_bitwise(renderer, which, section):
                 "sync": [mk, (A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_SYNC)],
                 # 2 passes per batch: 3 batches
                 "wavefront_batches": [wf, (A.PT_OPT_WAVEFRONT_POOL, 2 * npix)],
-                "wavefront_refill1": [wf, (A.PT_OPT_WAVEFRONT_POOL, 1 << 26), (A.PT_OPT_WAVEFRONT_REFILL, 1)],
-                "wavefront": [wf, (A.PT_OPT_WAVEFRONT_REFILL, 32)]}
+                "wavefront_refill1": [wf, (A.PT_OPT_WAVEFRONT_POOL, 1 << 27), (A.PT_OPT_WAVEFRONT_REFILL, 1)],
+                "wavefront": [wf, (A.PT_OPT_WAVEFRONT_REFILL, 56)]}
     runs = {}
     try:
         for name, opts in variants.items():

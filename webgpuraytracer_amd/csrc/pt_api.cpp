@@ -67,13 +67,13 @@ struct pt_ctx {
     bool with_gbuffer = false;
     DevBuf accum, out, gbuf, dn_a, dn_b, dn_fin, dn_guide, stats, seeds, rays_in, rays_aux, rays_aux2, rays_out;
     // wavefront path pool
-    uint64_t wf_pool_opt = 1ull << 26;  // paths per wavefront batch (~344 B each)
+    uint64_t wf_pool_opt = 1ull << 27;  // paths per wavefront batch (~344 B each)
     uint32_t wf_pool = 0;
     DevBuf wf_o, wf_d, wf_t, wf_r, wf_meta, wf_ha, wf_hb, wf_media, wf_q0, wf_q1, wf_sq, wf_so, wf_sd, wf_stp, wf_sfi,
         wf_cnt;
     // persistent megakernel: per-lane medium stacks, per-XCD tile counters
     DevBuf mk_media, mk_tiles, mk_spill, wf_spill;
-    uint32_t wf_refill = 32;      // wavefront traversal: idle lanes of a wave that trigger a refill
+    uint32_t wf_refill = 56;      // wavefront traversal: idle lanes of a wave that trigger a refill
     uint32_t lds_stack_opt = 16;  // traversal stack entries per lane in LDS (0: all)
     int n_cu = 0;
 
