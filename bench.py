@@ -171,7 +171,7 @@ def main():
         avg_ms = kern_ms / len(stats)
         achieved = avg_bytes / (avg_ms / 1000) / 1e9
         traffic = None
-        tf = os.path.join(ROOT, "profiles", "trace_kernel_c2_traffic.json")
+        tf = os.path.join(ROOT, "profiles", "%s_c2_traffic.json" % args.backend)
         if os.path.exists(tf):
             with open(tf) as f:
                 traffic = json.load(f).get("hbm_bytes_per_launch")
