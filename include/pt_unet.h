@@ -1,0 +1,64 @@
+/* pt_unet.h — the learned denoiser of libpt: a 5-level U-Net (3x3 conv +
+ * ReLU, 2x average pooling / nearest upsampling, skip concatenation;
+ * channels 32/48/64/96/128) over noisy radiance plus albedo, normal and
+ * depth, trained and run on the device in bf16 with fp32 accumulation.
+ *
+ * The reference has no counterpart: its denoiser is the a-trous filter
+ * (denoiser.wgsl, Denoiser.execute tracer.js:2583-2637), which pt_denoise
+ * replaces.  pt_unet_denoise is the learned alternative for the same frame
+ * and writes the same outputs (pt_read_denoised, pt_read_output), so a caller
+ * switches between them per frame.
+ *
+ * Tensors crossing this boundary are host fp32 NHWC: features
+ * [n][h][w][10] = radiance rgb, albedo rgb, normal xyz, hit distance (as the
+ * accumulation buffer and the G-buffer hold them); targets [n][h][w][3] =
+ * reference radiance; predictions [n][h][w][3] in log(1 + radiance).  h and w
+ * are multiples of 16.  Every call returns PT_OK or a negative PT_ERR_*. */
+#ifndef PT_UNET_H
+#define PT_UNET_H
+
+#include <stdint.h>
+
+#include "pt_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Creates (or re-creates) the network with He-initialised weights drawn from
+ * `seed`, and clears the Adam state. */
+int pt_unet_init(pt_ctx* ctx, uint64_t seed);
+
+/* Parameters as one fp32 array: per layer the weights [9][cout][cin] then the
+ * biases [cout], layers in network order (19 convolutions). */
+int pt_unet_param_count(pt_ctx* ctx, uint64_t* count);
+int pt_unet_get_params(pt_ctx* ctx, float* out, uint64_t count);
+int pt_unet_set_params(pt_ctx* ctx, const float* in, uint64_t count); /* also clears the Adam state */
+
+/* Inference on host features: pred [n][h][w][3] (log space). */
+int pt_unet_forward(pt_ctx* ctx, const float* feat, uint32_t n, uint32_t h, uint32_t w, float* pred);
+
+/* L1 loss on log(1 + x) of one batch, mean over pixels and channels, and its
+ * gradient with respect to every parameter (grad: param_count floats, or
+ * NULL). */
+int pt_unet_loss_grad(pt_ctx* ctx, const float* feat, const float* target, uint32_t n, uint32_t h, uint32_t w,
+                      float* loss, float* grad);
+
+/* Keeps a training set on the device: n crops of h x w. */
+int pt_unet_set_dataset(pt_ctx* ctx, const float* feat, const float* target, uint32_t n, uint32_t h, uint32_t w);
+
+/* `steps` Adam steps (lr, beta 0.9 / 0.999, eps 1e-8) on batches of `batch`
+ * crops drawn from the dataset with `seed`; losses[i] (optional) receives
+ * step i's batch loss. */
+int pt_unet_train(pt_ctx* ctx, uint32_t steps, uint32_t batch, float lr, uint64_t seed, float* losses);
+
+/* Denoises the context's current frame (accumulation buffer + G-buffer, as
+ * pt_denoise reads them): the result goes to the buffers pt_read_denoised
+ * (fp32) and pt_read_output (rgba8, tonemapped with `exposure`) read. */
+int pt_unet_denoise(pt_ctx* ctx, float exposure);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PT_UNET_H */

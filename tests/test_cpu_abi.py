@@ -19,7 +19,7 @@ def test_ctypes_layouts_match_header_sizes():
 
 def test_header_static_asserts_hold_in_c_and_cpp(tmp_path):
     src = tmp_path / "t.c"
-    src.write_text('#include "pt_abi.h"\n#include "pt_scene.h"\nint main(void){return 0;}\n')
+    src.write_text('#include "pt_abi.h"\n#include "pt_scene.h"\n#include "pt_unet.h"\nint main(void){return 0;}\n')
     subprocess.run(["gcc", "-std=c11", "-fsyntax-only", "-I" + INC, str(src)], check=True)
     subprocess.run(["g++", "-std=c++17", "-x", "c++", "-fsyntax-only", "-I" + INC, str(src)], check=True)
 
@@ -44,6 +44,14 @@ def test_libpt_exports_every_declared_symbol():
     A.load_pt()  # every binding resolves
 
 
+def test_libpt_exports_the_learned_denoiser():
+    lib = os.path.join(A.LIB_DIR, "libpt.so")
+    names = _declared("pt_unet.h", "pt_")
+    assert "pt_unet_train" in names and "pt_unet_denoise" in names
+    missing = [n for n in names if n not in _exported(lib)]
+    assert not missing, missing
+
+
 def test_libptscene_exports_every_declared_symbol():
     lib = os.path.join(A.LIB_DIR, "libptscene.so")
     names = _declared("pt_scene.h", "ptgen_")
@@ -53,7 +61,7 @@ def test_libptscene_exports_every_declared_symbol():
 
 
 def test_ctypes_table_covers_the_headers():
-    assert set(_declared("pt_abi.h", "pt_")) == set(A.PT_FUNCS)
+    assert set(_declared("pt_abi.h", "pt_")) | set(_declared("pt_unet.h", "pt_")) == set(A.PT_FUNCS)
     assert set(_declared("pt_scene.h", "ptgen_")) == set(A.PTGEN_FUNCS)
 
 
@@ -65,6 +73,9 @@ def test_null_arguments_are_errors_not_crashes():
     assert lib.pt_denoise(None, None, 0) == A.PT_ERR_ARG
     assert lib.pt_read_accum(None, None, 0) == A.PT_ERR_ARG
     assert lib.pt_write_accum(None, None, 0) == A.PT_ERR_ARG
+    assert lib.pt_unet_init(None, 0) == A.PT_ERR_ARG
+    assert lib.pt_unet_train(None, 1, 1, 1e-3, 0, None) == A.PT_ERR_ARG
+    assert lib.pt_unet_denoise(None, 1.0) == A.PT_ERR_ARG
     lib.pt_destroy(None)
 
 

@@ -167,6 +167,16 @@ PT_FUNCS = {
     "pt_trace_rays": (C.c_int, [vp, vp, u32, vp]),
     "pt_trace_shadow_rays": (C.c_int, [vp, vp, vp, vp, u32, vp]),
     "pt_synchronize": (C.c_int, [vp]),
+    # the learned denoiser (pt_unet.h)
+    "pt_unet_init": (C.c_int, [vp, u64]),
+    "pt_unet_param_count": (C.c_int, [vp, P(u64)]),
+    "pt_unet_get_params": (C.c_int, [vp, vp, u64]),
+    "pt_unet_set_params": (C.c_int, [vp, vp, u64]),
+    "pt_unet_forward": (C.c_int, [vp, vp, u32, u32, u32, vp]),
+    "pt_unet_loss_grad": (C.c_int, [vp, vp, vp, u32, u32, u32, P(f32), vp]),
+    "pt_unet_set_dataset": (C.c_int, [vp, vp, vp, u32, u32, u32]),
+    "pt_unet_train": (C.c_int, [vp, u32, u32, f32, u64, vp]),
+    "pt_unet_denoise": (C.c_int, [vp, f32]),
 }
 
 PTGEN_FUNCS = {

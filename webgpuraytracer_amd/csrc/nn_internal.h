@@ -1,0 +1,98 @@
+// nn_internal.h — kernels of the learned denoiser (nn_kernels.hip) as seen
+// from the host code that runs the network (pt_unet.cpp).
+//
+// Activations and activation gradients are NHWC bf16 with a channel stride
+// and offset per tensor (so a layer can read or write a slice of a
+// concatenation buffer); weights are bf16 [tap][cout][cin] with fp32 master
+// copies; every product accumulates in fp32 on MFMA.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pt {
+namespace nn {
+
+typedef __bf16 bf16;
+
+// An NHWC view: element (pixel, c) at p[pixel * stride + off + c].
+struct Tens {
+    bf16* p;
+    uint32_t stride;
+    uint32_t off;
+};
+
+struct Dims {
+    uint32_t n, h, w;
+    __host__ __device__ uint32_t pixels() const { return n * h * w; }
+};
+
+// Largest output-channel count of a 3x3 convolution (per launch).
+constexpr uint32_t kMaxCout = 128;
+
+// out = act(conv3x3(in, w) + bias) with zero padding; cin and cout are
+// multiples of 16 (cout <= kMaxCout); w is bf16 [9][cout][cin].  relu: ReLU;
+// out_f32 (when non-null) receives the first `f32_channels` channels as fp32
+// [pixel][f32_channels] instead of `out`.  bias may be null.
+hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bias, const Tens& out, uint32_t cout,
+                   Dims d, bool relu, float* out_f32, uint32_t f32_channels, hipStream_t s);
+
+// Weight gradient of a 3x3 convolution over an input-channel slice:
+// dw[tap][co][ci_off + ci] (fp32, rows of cin_total, overwritten) = sum over
+// pixels of dz[p][co] * x[p + offset(tap)][ci] for ci < cin (cin <= kMaxCout);
+// partial holds wgrad_partial_floats(cin, cout, d) floats of scratch.
+size_t wgrad_partial_floats(uint32_t cin, uint32_t cout, Dims d);
+hipError_t conv3x3_wgrad(const Tens& dz, uint32_t cout, const Tens& x, uint32_t cin, Dims d, float* partial,
+                         float* dw, uint32_t ci_off, uint32_t cin_total, hipStream_t s);
+
+// db[co] = sum over pixels of dz[p][co] (fp32, overwritten); partial holds
+// bias_partial_floats(cout) floats of scratch.
+size_t bias_partial_floats(uint32_t cout);
+hipError_t bias_grad(const Tens& dz, uint32_t cout, Dims d, float* partial, float* db, hipStream_t s);
+
+// dz = da * (a > 0) for a ReLU layer with output a; dz is contiguous (stride c).
+hipError_t relu_grad(const Tens& da, const Tens& a, uint32_t c, Dims d, bf16* dz, hipStream_t s);
+
+// 2x2 average pooling (d: the input's dims; h and w even).
+hipError_t pool2(const Tens& in, uint32_t c, Dims d, const Tens& out, hipStream_t s);
+// din += dout / 4 spread over each 2x2 block (accumulates into din).
+hipError_t pool2_back_add(const Tens& dout, uint32_t c, Dims d, const Tens& din, hipStream_t s);
+// 2x nearest upsampling (d: the input's dims).
+hipError_t up2(const Tens& in, uint32_t c, Dims d, const Tens& out, hipStream_t s);
+// din = sum of dout over each 2x2 block.
+hipError_t up2_back(const Tens& dout, uint32_t c, Dims d, const Tens& din, hipStream_t s);
+
+// The weights the data-gradient pass uses, for input channels
+// [ci_begin, ci_begin + ci_count): wd[8 - tap][ci - ci_begin][co] = w[tap][co][ci].
+hipError_t transpose_weights(const bf16* w, uint32_t cout, uint32_t cin, uint32_t ci_begin, uint32_t ci_count,
+                             bf16* wd, hipStream_t s);
+
+// fp32 master weights (layout [9][cout][cin]) to bf16.
+hipError_t to_bf16(const float* src, bf16* dst, size_t n, hipStream_t s);
+hipError_t fill_zero_bf16(bf16* p, size_t n, hipStream_t s);
+hipError_t fill_zero_f32(float* p, size_t n, hipStream_t s);
+
+// Adam over n parameters; step t >= 1.
+hipError_t adam(float* w, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2, float eps,
+                uint32_t t, hipStream_t s);
+
+// Network input: 16 bf16 channels per pixel from fp32 features [pixel][10]
+// (radiance rgb, albedo rgb, normal xyz, depth): log(1 + radiance), albedo,
+// normal, depth / (depth + 1), then zeros.
+hipError_t pack_input(const float* feat, Dims d, bf16* out, hipStream_t s);
+// The same input straight from a rendered frame: accum (float4 per pixel),
+// G-buffer (48 B texels), into a frame padded to d (pad pixels zero).
+hipError_t pack_frame(const float4* accum, const void* gbuf, uint32_t fw, uint32_t fh, Dims d, bf16* out,
+                      hipStream_t s);
+
+// L1 loss on log(1 + x): pred [pixel][3] fp32 is in log space already, target
+// [pixel][3] is linear radiance.  Writes d loss / d pred as bf16 [pixel][16]
+// (channels 3..15 zero) and adds the batch loss sum into *loss (fp32).
+hipError_t l1_log_loss(const float* pred, const float* target, Dims d, bf16* dpred, float* loss, hipStream_t s);
+
+// Denoised frame: exp(pred) - 1 (clamped at 0) into fp32 float4 and rgba8
+// (tonemapped with `exposure`), cropping the padded frame to fw x fh.
+hipError_t unpack_frame(const float* pred, Dims d, uint32_t fw, uint32_t fh, float exposure, float4* fin,
+                        uchar4* out, hipStream_t s);
+
+}  // namespace nn
+}  // namespace pt

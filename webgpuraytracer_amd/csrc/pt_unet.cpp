@@ -1,0 +1,539 @@
+// pt_unet.cpp — the learned denoiser on the host side: the layer table,
+// device buffers, forward and backward passes, Adam, and the C ABI of
+// include/pt_unet.h.  The kernels are in nn_kernels.hip.
+//
+// Network (levels L = 0..4 at 1/2^L resolution, C = 32/48/64/96/128):
+//   encoder L < 4:  x -> conv(C_L) -> conv(C_L) = skip_L -> pool -> level L+1
+//   level 4:        conv(128) -> conv(128)
+//   decoder L < 4:  up(level L+1) ++ skip_L -> conv(C_L) -> conv(C_L)
+//   output:         conv(3), linear, in log(1 + radiance)
+// skip_L is written straight into the decoder's concatenation buffer
+// cat_L = [up | skip] (NHWC, so concatenation is a channel offset).
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "nn_internal.h"
+#include "pt_ctx.h"
+#include "pt_unet.h"
+
+using namespace pt;
+using namespace pt::nn;
+
+namespace pt {
+
+namespace {
+
+constexpr uint32_t kC[5] = {32, 48, 64, 96, 128};
+constexpr uint32_t kInC = 16;     // packed input channels (10 used)
+constexpr uint32_t kOutC = 16;    // output convolution channels (3 used)
+constexpr uint32_t kFeat = 10;    // host feature channels
+constexpr uint32_t kLayers = 19;
+
+struct Layer {
+    uint32_t cin, cout;            // padded (multiples of 16)
+    uint32_t cin_real, cout_real;  // fan-in / used outputs for initialisation
+    size_t w_off, b_off;           // into the flat fp32 parameter array
+    size_t wbf_off;                // into the bf16 weight arrays (same size as the weights)
+    bool relu;
+};
+
+enum {
+    E0A, E0B, E1A, E1B, E2A, E2B, E3A, E3B, B4A, B4B, D3A, D3B, D2A, D2B, D1A, D1B, D0A, D0B, OUT
+};
+
+uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double uniform01(uint64_t& s) { return double(splitmix(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+}  // namespace
+
+struct UNet {
+    Layer L[kLayers];
+    size_t n_params = 0, n_wbf = 0;
+    DevBuf params, grads, m, v;    // fp32, n_params
+    DevBuf wbf, wdt;               // bf16 weights [9][cout][cin], transposed slices [9][ci][cout]
+    uint32_t adam_t = 0;
+    // activations for the current dims
+    Dims dims[5] = {};
+    bool have_acts = false;
+    DevBuf in0, tA[4], cat[4], pin[5], t4A, b4, dA[4], dd[4], pred;
+    DevBuf g_tA[4], g_cat[4], g_pin[5], g_t4A, g_b4, g_dA[4], g_dd[4], g_pred, dz;
+    DevBuf target, feat, loss, wpart, bpart;
+    // dataset
+    DevBuf ds_in, ds_tg;
+    uint32_t ds_n = 0, ds_h = 0, ds_w = 0;
+
+    std::vector<DevBuf*> all_bufs() {
+        std::vector<DevBuf*> bufs{&params, &grads, &m, &v, &wbf, &wdt, &in0, &t4A, &b4, &pred, &g_t4A, &g_b4,
+                                  &g_pred, &dz, &target, &feat, &loss, &wpart, &bpart, &ds_in, &ds_tg};
+        for (int i = 0; i < 4; ++i) {
+            for (DevBuf* b : {&tA[i], &cat[i], &dA[i], &dd[i], &g_tA[i], &g_cat[i], &g_dA[i], &g_dd[i]})
+                bufs.push_back(b);
+        }
+        for (int i = 0; i < 5; ++i) {
+            bufs.push_back(&pin[i]);
+            bufs.push_back(&g_pin[i]);
+        }
+        return bufs;
+    }
+    void release_acts() {
+        for (DevBuf* b : all_bufs())
+            if (b != &params && b != &grads && b != &m && b != &v && b != &wbf && b != &wdt && b != &ds_in &&
+                b != &ds_tg)
+                b->release();
+        have_acts = false;
+    }
+};
+
+void unet_free(UNet* u) {
+    if (!u) return;
+    for (DevBuf* b : u->all_bufs()) b->release();
+    delete u;
+}
+
+namespace {
+
+bf16* bfp(DevBuf& b) { return static_cast<bf16*>(b.p); }
+float* fp(DevBuf& b) { return static_cast<float*>(b.p); }
+Tens T(DevBuf& b, uint32_t stride, uint32_t off = 0) { return Tens{bfp(b), stride, off}; }
+
+void build_layers(UNet& u) {
+    struct Spec { uint32_t cin, cout, cin_real, cout_real; bool relu; };
+    const Spec spec[kLayers] = {
+        {kInC, kC[0], kFeat, kC[0], true},  {kC[0], kC[0], kC[0], kC[0], true},
+        {kC[0], kC[1], kC[0], kC[1], true}, {kC[1], kC[1], kC[1], kC[1], true},
+        {kC[1], kC[2], kC[1], kC[2], true}, {kC[2], kC[2], kC[2], kC[2], true},
+        {kC[2], kC[3], kC[2], kC[3], true}, {kC[3], kC[3], kC[3], kC[3], true},
+        {kC[3], kC[4], kC[3], kC[4], true}, {kC[4], kC[4], kC[4], kC[4], true},
+        {kC[4] + kC[3], kC[3], kC[4] + kC[3], kC[3], true}, {kC[3], kC[3], kC[3], kC[3], true},
+        {kC[3] + kC[2], kC[2], kC[3] + kC[2], kC[2], true}, {kC[2], kC[2], kC[2], kC[2], true},
+        {kC[2] + kC[1], kC[1], kC[2] + kC[1], kC[1], true}, {kC[1], kC[1], kC[1], kC[1], true},
+        {kC[1] + kC[0], kC[0], kC[1] + kC[0], kC[0], true}, {kC[0], kC[0], kC[0], kC[0], true},
+        {kC[0], kOutC, kC[0], 3, false},
+    };
+    size_t off = 0, woff = 0;
+    for (uint32_t i = 0; i < kLayers; ++i) {
+        Layer& l = u.L[i];
+        l.cin = spec[i].cin;
+        l.cout = spec[i].cout;
+        l.cin_real = spec[i].cin_real;
+        l.cout_real = spec[i].cout_real;
+        l.relu = spec[i].relu;
+        l.w_off = off;
+        off += size_t(9) * l.cout * l.cin;
+        l.b_off = off;
+        off += l.cout;
+        l.wbf_off = woff;
+        woff += size_t(9) * l.cout * l.cin;
+    }
+    u.n_params = off;
+    u.n_wbf = woff;
+}
+
+// He initialisation; padded input channels and unused outputs stay zero.
+void init_params(const UNet& u, uint64_t seed, std::vector<float>& p) {
+    p.assign(u.n_params, 0.0f);
+    uint64_t s = seed * 0x2545F4914F6CDD1Dull + 1u;
+    for (uint32_t i = 0; i < kLayers; ++i) {
+        const Layer& l = u.L[i];
+        const double sd = std::sqrt(2.0 / (9.0 * l.cin_real));
+        for (uint32_t tap = 0; tap < 9; ++tap)
+            for (uint32_t co = 0; co < l.cout; ++co)
+                for (uint32_t ci = 0; ci < l.cin; ++ci) {
+                    const double u1 = std::max(uniform01(s), 1e-300), u2 = uniform01(s);
+                    const double g = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+                    const bool used = ci < l.cin_real && co < l.cout_real;
+                    p[l.w_off + (size_t(tap) * l.cout + co) * l.cin + ci] = used ? float(g * sd) : 0.0f;
+                }
+    }
+}
+
+// Number of 128-channel slices of a layer's input (the data-gradient and
+// weight-gradient kernels produce at most 128 channels per launch).
+uint32_t slices(const Layer& l) { return (l.cin + kMaxCout - 1u) / kMaxCout; }
+uint32_t slice_begin(uint32_t k) { return k * kMaxCout; }
+uint32_t slice_count(const Layer& l, uint32_t k) { return std::min(kMaxCout, l.cin - k * kMaxCout); }
+
+int hip_rc(pt_ctx* c, hipError_t e, const char* what) { return e == hipSuccess ? PT_OK : c->hip_fail(e, what); }
+
+// bf16 weights and their transposed slices from the fp32 master copy.
+int refresh_weights(pt_ctx* c, UNet& u) {
+    hipStream_t s = c->stream;
+    for (uint32_t i = 0; i < kLayers; ++i) {
+        const Layer& l = u.L[i];
+        hipError_t e = to_bf16(fp(u.params) + l.w_off, bfp(u.wbf) + l.wbf_off, size_t(9) * l.cout * l.cin, s);
+        size_t so = l.wbf_off;
+        for (uint32_t k = 0; e == hipSuccess && k < slices(l); ++k) {
+            e = transpose_weights(bfp(u.wbf) + l.wbf_off, l.cout, l.cin, slice_begin(k), slice_count(l, k),
+                                  bfp(u.wdt) + so, s);
+            so += size_t(9) * slice_count(l, k) * l.cout;
+        }
+        if (e != hipSuccess) return c->hip_fail(e, "unet weights");
+    }
+    return PT_OK;
+}
+
+int ensure_net(pt_ctx* c) {
+    if (c->unet) return PT_OK;
+    return c->fail(PT_ERR_STATE, "the learned denoiser is not initialised (pt_unet_init)");
+}
+
+int alloc_acts(pt_ctx* c, UNet& u, Dims d0) {
+    if (d0.h % 16u || d0.w % 16u || d0.n == 0 || d0.h == 0 || d0.w == 0)
+        return c->fail(PT_ERR_ARG, "network input must be a positive multiple of 16 in both dimensions");
+    if (u.have_acts && u.dims[0].n == d0.n && u.dims[0].h == d0.h && u.dims[0].w == d0.w) return PT_OK;
+    u.release_acts();
+    for (int l = 0; l < 5; ++l) u.dims[l] = Dims{d0.n, d0.h >> l, d0.w >> l};
+    auto P = [&](int l) { return size_t(u.dims[l].pixels()); };
+    int rc;
+    auto A = [&](DevBuf& b, size_t elems, size_t each, const char* what) { return c->alloc(b, elems * each, what); };
+    if ((rc = A(u.in0, P(0) * kInC, 2, "unet input"))) return rc;
+    for (int l = 0; l < 4; ++l) {
+        const size_t cc = kC[l + 1] + kC[l];
+        if ((rc = A(u.tA[l], P(l) * kC[l], 2, "unet act")) || (rc = A(u.cat[l], P(l) * cc, 2, "unet cat")) ||
+            (rc = A(u.dA[l], P(l) * kC[l], 2, "unet act")) || (rc = A(u.dd[l], P(l) * kC[l], 2, "unet act")) ||
+            (rc = A(u.g_tA[l], P(l) * kC[l], 2, "unet grad")) || (rc = A(u.g_cat[l], P(l) * cc, 2, "unet grad")) ||
+            (rc = A(u.g_dA[l], P(l) * kC[l], 2, "unet grad")) || (rc = A(u.g_dd[l], P(l) * kC[l], 2, "unet grad")))
+            return rc;
+        if ((rc = A(u.pin[l + 1], P(l + 1) * kC[l], 2, "unet pool")) ||
+            (rc = A(u.g_pin[l + 1], P(l + 1) * kC[l], 2, "unet grad")))
+            return rc;
+    }
+    if ((rc = A(u.t4A, P(4) * kC[4], 2, "unet act")) || (rc = A(u.b4, P(4) * kC[4], 2, "unet act")) ||
+        (rc = A(u.g_t4A, P(4) * kC[4], 2, "unet grad")) || (rc = A(u.g_b4, P(4) * kC[4], 2, "unet grad")) ||
+        (rc = A(u.pred, P(0) * 3, 4, "unet pred")) || (rc = A(u.g_pred, P(0) * kOutC, 2, "unet grad")) ||
+        (rc = A(u.target, P(0) * 3, 4, "unet target")) || (rc = A(u.feat, P(0) * kFeat, 4, "unet features")))
+        return rc;
+    // scratch: the largest relu-gradient tensor and gradient partial sums over all layers
+    size_t dz = 0, wp = 0, bp = 0;
+    const int lvl[kLayers] = {0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 3, 3, 2, 2, 1, 1, 0, 0, 0};
+    for (uint32_t i = 0; i < kLayers; ++i) {
+        const Layer& l = u.L[i];
+        dz = std::max(dz, P(lvl[i]) * l.cout);
+        for (uint32_t k = 0; k < slices(l); ++k)
+            wp = std::max(wp, wgrad_partial_floats(slice_count(l, k), l.cout, u.dims[lvl[i]]));
+        bp = std::max(bp, bias_partial_floats(l.cout));
+    }
+    if ((rc = A(u.dz, dz, 2, "unet dz")) || (rc = A(u.wpart, wp, 4, "unet wgrad scratch")) ||
+        (rc = A(u.bpart, bp, 4, "unet bias scratch")) || (rc = A(u.loss, 1024, 4, "unet loss")))
+        return rc;
+    u.have_acts = true;
+    return PT_OK;
+}
+
+int conv_fwd(pt_ctx* c, UNet& u, uint32_t li, const Tens& in, const Tens& out, Dims d, float* of32 = nullptr) {
+    const Layer& l = u.L[li];
+    return hip_rc(c,
+                  conv3x3(in, l.cin, bfp(u.wbf) + l.wbf_off, fp(u.params) + l.b_off, out, l.cout, d, l.relu, of32,
+                          of32 ? 3u : 0u, c->stream),
+                  "unet conv");
+}
+
+int forward(pt_ctx* c, UNet& u) {
+    const Dims* D = u.dims;
+    int rc;
+    // encoder
+    Tens x = T(u.in0, kInC);
+    for (int l = 0; l < 4; ++l) {
+        const uint32_t cc = kC[l + 1] + kC[l];
+        const Tens skip = T(u.cat[l], cc, kC[l + 1]);
+        if ((rc = conv_fwd(c, u, E0A + 2 * l, x, T(u.tA[l], kC[l]), D[l]))) return rc;
+        if ((rc = conv_fwd(c, u, E0B + 2 * l, T(u.tA[l], kC[l]), skip, D[l]))) return rc;
+        if ((rc = hip_rc(c, pool2(skip, kC[l], D[l], T(u.pin[l + 1], kC[l]), c->stream), "unet pool"))) return rc;
+        x = T(u.pin[l + 1], kC[l]);
+    }
+    if ((rc = conv_fwd(c, u, B4A, x, T(u.t4A, kC[4]), D[4]))) return rc;
+    if ((rc = conv_fwd(c, u, B4B, T(u.t4A, kC[4]), T(u.b4, kC[4]), D[4]))) return rc;
+    // decoder
+    Tens low = T(u.b4, kC[4]);
+    for (int l = 3; l >= 0; --l) {
+        const uint32_t cc = kC[l + 1] + kC[l];
+        if ((rc = hip_rc(c, up2(low, kC[l + 1], D[l + 1], T(u.cat[l], cc, 0), c->stream), "unet up"))) return rc;
+        if ((rc = conv_fwd(c, u, D3A + 2 * (3 - l), T(u.cat[l], cc), T(u.dA[l], kC[l]), D[l]))) return rc;
+        if ((rc = conv_fwd(c, u, D3B + 2 * (3 - l), T(u.dA[l], kC[l]), T(u.dd[l], kC[l]), D[l]))) return rc;
+        low = T(u.dd[l], kC[l]);
+    }
+    return conv_fwd(c, u, OUT, low, Tens{nullptr, 0, 0}, D[0], fp(u.pred));
+}
+
+// Backward through layer li: x its input, a its output (for the ReLU mask),
+// ga the gradient at its output; writes the weight and bias gradients and,
+// when gx.p is non-null, the gradient at its input.
+int conv_back(pt_ctx* c, UNet& u, uint32_t li, const Tens& x, const Tens& a, const Tens& ga, const Tens& gx, Dims d) {
+    const Layer& l = u.L[li];
+    hipStream_t s = c->stream;
+    Tens dz = ga;
+    hipError_t e = hipSuccess;
+    if (l.relu) {
+        e = relu_grad(ga, a, l.cout, d, bfp(u.dz), s);
+        dz = T(u.dz, l.cout);
+    }
+    size_t so = l.wbf_off;
+    for (uint32_t k = 0; e == hipSuccess && k < slices(l); ++k) {
+        const uint32_t c0 = slice_begin(k), nc = slice_count(l, k);
+        const Tens xs{x.p, x.stride, x.off + c0};
+        e = conv3x3_wgrad(dz, l.cout, xs, nc, d, fp(u.wpart), fp(u.grads) + l.w_off, c0, l.cin, s);
+        if (e == hipSuccess && gx.p) {
+            const Tens gxs{gx.p, gx.stride, gx.off + c0};
+            e = conv3x3(dz, l.cout, bfp(u.wdt) + so, nullptr, gxs, nc, d, false, nullptr, 0, s);
+        }
+        so += size_t(9) * nc * l.cout;
+    }
+    if (e == hipSuccess) e = bias_grad(dz, l.cout, d, fp(u.bpart), fp(u.grads) + l.b_off, s);
+    return hip_rc(c, e, "unet backward");
+}
+
+int backward(pt_ctx* c, UNet& u) {
+    const Dims* D = u.dims;
+    hipStream_t s = c->stream;
+    int rc;
+    const Tens none{nullptr, 0, 0};
+    // output layer (linear): dz = d loss / d pred, already [pixel][16]
+    if ((rc = conv_back(c, u, OUT, T(u.dd[0], kC[0]), none, T(u.g_pred, kOutC), T(u.g_dd[0], kC[0]), D[0])))
+        return rc;
+    // decoder, level 0 up to 3
+    for (int l = 0; l < 4; ++l) {
+        const uint32_t cc = kC[l + 1] + kC[l];
+        const uint32_t la = D3A + 2 * (3 - l), lb = la + 1;
+        if ((rc = conv_back(c, u, lb, T(u.dA[l], kC[l]), T(u.dd[l], kC[l]), T(u.g_dd[l], kC[l]),
+                            T(u.g_dA[l], kC[l]), D[l])))
+            return rc;
+        if ((rc = conv_back(c, u, la, T(u.cat[l], cc), T(u.dA[l], kC[l]), T(u.g_dA[l], kC[l]), T(u.g_cat[l], cc),
+                            D[l])))
+            return rc;
+        // the upsampled half of the concatenation goes back to the level below
+        const Tens g_low = l < 3 ? T(u.g_dd[l + 1], kC[l + 1]) : T(u.g_b4, kC[4]);
+        if ((rc = hip_rc(c, up2_back(T(u.g_cat[l], cc, 0), kC[l + 1], D[l + 1], g_low, s), "unet up back")))
+            return rc;
+    }
+    // bottleneck
+    if ((rc = conv_back(c, u, B4B, T(u.t4A, kC[4]), T(u.b4, kC[4]), T(u.g_b4, kC[4]), T(u.g_t4A, kC[4]), D[4])))
+        return rc;
+    if ((rc = conv_back(c, u, B4A, T(u.pin[4], kC[3]), T(u.t4A, kC[4]), T(u.g_t4A, kC[4]), T(u.g_pin[4], kC[3]),
+                        D[4])))
+        return rc;
+    // encoder, level 3 down to 0: the skip gradient is the concatenation's
+    // skip half plus the pooled path's gradient
+    for (int l = 3; l >= 0; --l) {
+        const uint32_t cc = kC[l + 1] + kC[l];
+        const Tens g_skip = T(u.g_cat[l], cc, kC[l + 1]);
+        if ((rc = hip_rc(c, pool2_back_add(T(u.g_pin[l + 1], kC[l]), kC[l], D[l], g_skip, s), "unet pool back")))
+            return rc;
+        const Tens xin = l == 0 ? T(u.in0, kInC) : T(u.pin[l], kC[l - 1]);
+        const Tens gxin = l == 0 ? none : T(u.g_pin[l], kC[l - 1]);
+        if ((rc = conv_back(c, u, E0B + 2 * l, T(u.tA[l], kC[l]), T(u.cat[l], cc, kC[l + 1]), g_skip,
+                            T(u.g_tA[l], kC[l]), D[l])))
+            return rc;
+        if ((rc = conv_back(c, u, E0A + 2 * l, xin, T(u.tA[l], kC[l]), T(u.g_tA[l], kC[l]), gxin, D[l]))) return rc;
+    }
+    return PT_OK;
+}
+
+// Loss of the batch in u.pred against u.target; adds its sum into loss[slot].
+int loss_and_grad(pt_ctx* c, UNet& u, uint32_t slot) {
+    return hip_rc(c, l1_log_loss(fp(u.pred), fp(u.target), u.dims[0], bfp(u.g_pred), fp(u.loss) + slot, c->stream),
+                  "unet loss");
+}
+
+int upload_batch_features(pt_ctx* c, UNet& u, const float* feat, const float* target) {
+    const size_t P = u.dims[0].pixels();
+    hipError_t e = hipMemcpyAsync(u.feat.p, feat, P * kFeat * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = pack_input(fp(u.feat), u.dims[0], bfp(u.in0), c->stream);
+    if (e == hipSuccess && target) e = hipMemcpyAsync(u.target.p, target, P * 3 * 4, hipMemcpyHostToDevice, c->stream);
+    return hip_rc(c, e, "unet batch upload");
+}
+
+}  // namespace
+}  // namespace pt
+
+extern "C" {
+
+int pt_unet_init(pt_ctx* c, uint64_t seed) {
+    if (!c) return PT_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (!c->unet) {
+        c->unet = new (std::nothrow) UNet();
+        if (!c->unet) return c->fail(PT_ERR_OOM, "unet");
+        build_layers(*c->unet);
+    }
+    UNet& u = *c->unet;
+    std::vector<float> p;
+    init_params(u, seed, p);
+    int rc;
+    if ((rc = c->upload(u.params, p.data(), p.size() * 4, "unet params")) ||
+        (rc = c->alloc(u.grads, u.n_params * 4, "unet grads")) || (rc = c->alloc(u.m, u.n_params * 4, "unet adam")) ||
+        (rc = c->alloc(u.v, u.n_params * 4, "unet adam")) || (rc = c->alloc(u.wbf, u.n_wbf * 2, "unet bf16 weights")) ||
+        (rc = c->alloc(u.wdt, u.n_wbf * 2, "unet bf16 weights")))
+        return rc;
+    hipError_t e = fill_zero_f32(fp(u.m), u.n_params, c->stream);
+    if (e == hipSuccess) e = fill_zero_f32(fp(u.v), u.n_params, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "unet init");
+    u.adam_t = 0;
+    if ((rc = refresh_weights(c, u))) return rc;
+    return c->sync("unet init");
+}
+
+int pt_unet_param_count(pt_ctx* c, uint64_t* count) {
+    if (!c || !count) return PT_ERR_ARG;
+    int rc = ensure_net(c);
+    if (rc) return rc;
+    *count = c->unet->n_params;
+    return PT_OK;
+}
+
+int pt_unet_get_params(pt_ctx* c, float* out, uint64_t count) {
+    if (!c) return PT_ERR_ARG;
+    int rc = ensure_net(c);
+    if (rc) return rc;
+    if (!out || count != c->unet->n_params) return c->fail(PT_ERR_ARG, "expected %llu parameters",
+                                                            (unsigned long long)c->unet->n_params);
+    if ((rc = c->sync("unet params"))) return rc;
+    return hip_rc(c, hipMemcpy(out, c->unet->params.p, count * 4, hipMemcpyDeviceToHost), "unet params");
+}
+
+int pt_unet_set_params(pt_ctx* c, const float* in, uint64_t count) {
+    if (!c) return PT_ERR_ARG;
+    int rc = ensure_net(c);
+    if (rc) return rc;
+    UNet& u = *c->unet;
+    if (!in || count != u.n_params) return c->fail(PT_ERR_ARG, "expected %llu parameters", (unsigned long long)u.n_params);
+    if ((rc = c->upload(u.params, in, count * 4, "unet params"))) return rc;
+    hipError_t e = fill_zero_f32(fp(u.m), u.n_params, c->stream);
+    if (e == hipSuccess) e = fill_zero_f32(fp(u.v), u.n_params, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "unet params");
+    u.adam_t = 0;
+    if ((rc = refresh_weights(c, u))) return rc;
+    return c->sync("unet params");
+}
+
+int pt_unet_forward(pt_ctx* c, const float* feat, uint32_t n, uint32_t h, uint32_t w, float* pred) {
+    if (!c) return PT_ERR_ARG;
+    int rc = ensure_net(c);
+    if (rc) return rc;
+    if (!feat || !pred) return c->fail(PT_ERR_ARG, "NULL tensor");
+    (void)hipSetDevice(c->device);
+    UNet& u = *c->unet;
+    if ((rc = alloc_acts(c, u, Dims{n, h, w})) || (rc = upload_batch_features(c, u, feat, nullptr)) ||
+        (rc = forward(c, u)) || (rc = c->sync("unet forward")))
+        return rc;
+    return hip_rc(c, hipMemcpy(pred, u.pred.p, size_t(u.dims[0].pixels()) * 3 * 4, hipMemcpyDeviceToHost),
+                  "unet forward readback");
+}
+
+int pt_unet_loss_grad(pt_ctx* c, const float* feat, const float* target, uint32_t n, uint32_t h, uint32_t w,
+                      float* loss, float* grad) {
+    if (!c) return PT_ERR_ARG;
+    int rc = ensure_net(c);
+    if (rc) return rc;
+    if (!feat || !target || !loss) return c->fail(PT_ERR_ARG, "NULL tensor");
+    (void)hipSetDevice(c->device);
+    UNet& u = *c->unet;
+    if ((rc = alloc_acts(c, u, Dims{n, h, w})) || (rc = upload_batch_features(c, u, feat, target))) return rc;
+    if ((rc = hip_rc(c, fill_zero_f32(fp(u.loss), 1, c->stream), "unet loss"))) return rc;
+    if ((rc = forward(c, u)) || (rc = loss_and_grad(c, u, 0))) return rc;
+    if (grad && (rc = backward(c, u))) return rc;
+    if ((rc = c->sync("unet loss"))) return rc;
+    float sum = 0.f;
+    if ((rc = hip_rc(c, hipMemcpy(&sum, u.loss.p, 4, hipMemcpyDeviceToHost), "unet loss readback"))) return rc;
+    *loss = sum / (float(u.dims[0].pixels()) * 3.0f);
+    if (grad)
+        return hip_rc(c, hipMemcpy(grad, u.grads.p, u.n_params * 4, hipMemcpyDeviceToHost), "unet grad readback");
+    return PT_OK;
+}
+
+int pt_unet_set_dataset(pt_ctx* c, const float* feat, const float* target, uint32_t n, uint32_t h, uint32_t w) {
+    if (!c) return PT_ERR_ARG;
+    int rc = ensure_net(c);
+    if (rc) return rc;
+    if (!feat || !target || n == 0) return c->fail(PT_ERR_ARG, "empty dataset");
+    if (h % 16u || w % 16u || h == 0 || w == 0) return c->fail(PT_ERR_ARG, "crops must be multiples of 16");
+    (void)hipSetDevice(c->device);
+    UNet& u = *c->unet;
+    const size_t P = size_t(n) * h * w;
+    DevBuf staging;
+    if ((rc = c->alloc(staging, P * kFeat * 4, "unet dataset staging"))) return rc;
+    hipError_t e = hipMemcpyAsync(staging.p, feat, P * kFeat * 4, hipMemcpyHostToDevice, c->stream);
+    if ((rc = c->alloc(u.ds_in, P * kInC * 2, "unet dataset")) || (rc = c->alloc(u.ds_tg, P * 3 * 4, "unet dataset"))) {
+        staging.release();
+        return rc;
+    }
+    if (e == hipSuccess) e = pack_input(static_cast<const float*>(staging.p), Dims{n, h, w}, bfp(u.ds_in), c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(u.ds_tg.p, target, P * 3 * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    staging.release();
+    if (e != hipSuccess) return c->hip_fail(e, "unet dataset");
+    u.ds_n = n;
+    u.ds_h = h;
+    u.ds_w = w;
+    return PT_OK;
+}
+
+int pt_unet_train(pt_ctx* c, uint32_t steps, uint32_t batch, float lr, uint64_t seed, float* losses) {
+    if (!c) return PT_ERR_ARG;
+    int rc = ensure_net(c);
+    if (rc) return rc;
+    UNet& u = *c->unet;
+    if (u.ds_n == 0) return c->fail(PT_ERR_STATE, "no dataset (pt_unet_set_dataset)");
+    if (batch == 0 || steps == 0) return c->fail(PT_ERR_ARG, "steps and batch must be positive");
+    if (steps > 1024) return c->fail(PT_ERR_ARG, "at most 1024 steps per call");
+    (void)hipSetDevice(c->device);
+    if ((rc = alloc_acts(c, u, Dims{batch, u.ds_h, u.ds_w}))) return rc;
+    const size_t crop = size_t(u.ds_h) * u.ds_w;
+    hipStream_t s = c->stream;
+    if ((rc = hip_rc(c, fill_zero_f32(fp(u.loss), steps, s), "unet loss"))) return rc;
+    uint64_t rs = seed ^ 0xD1B54A32D192ED03ull;
+    for (uint32_t step = 0; step < steps; ++step) {
+        for (uint32_t bi = 0; bi < batch; ++bi) {
+            const uint32_t k = uint32_t(splitmix(rs) % u.ds_n);
+            hipError_t e = hipMemcpyAsync(bfp(u.in0) + bi * crop * kInC, bfp(u.ds_in) + k * crop * kInC,
+                                          crop * kInC * 2, hipMemcpyDeviceToDevice, s);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(fp(u.target) + bi * crop * 3, fp(u.ds_tg) + k * crop * 3, crop * 3 * 4,
+                                   hipMemcpyDeviceToDevice, s);
+            if (e != hipSuccess) return c->hip_fail(e, "unet batch");
+        }
+        if ((rc = forward(c, u)) || (rc = loss_and_grad(c, u, step)) || (rc = backward(c, u))) return rc;
+        ++u.adam_t;
+        if ((rc = hip_rc(c, adam(fp(u.params), fp(u.grads), fp(u.m), fp(u.v), u.n_params, lr, 0.9f, 0.999f, 1e-8f,
+                                 u.adam_t, s),
+                         "unet adam")))
+            return rc;
+        if ((rc = refresh_weights(c, u))) return rc;
+    }
+    if ((rc = c->sync("unet train"))) return rc;
+    if (losses) {
+        std::vector<float> l(steps);
+        if ((rc = hip_rc(c, hipMemcpy(l.data(), u.loss.p, steps * 4, hipMemcpyDeviceToHost), "unet loss readback")))
+            return rc;
+        const float norm = 1.0f / (float(u.dims[0].pixels()) * 3.0f);
+        for (uint32_t i = 0; i < steps; ++i) losses[i] = l[i] * norm;
+    }
+    return PT_OK;
+}
+
+int pt_unet_denoise(pt_ctx* c, float exposure) {
+    if (!c) return PT_ERR_ARG;
+    int rc = ensure_net(c);
+    if (rc) return rc;
+    if (c->width == 0 || !c->gbuf.p) return c->fail(PT_ERR_STATE, "denoise needs a frame with a G-buffer");
+    (void)hipSetDevice(c->device);
+    UNet& u = *c->unet;
+    const uint32_t fw = c->width, fh = c->height;
+    const Dims d{1u, (fh + 15u) & ~15u, (fw + 15u) & ~15u};
+    if ((rc = alloc_acts(c, u, d))) return rc;
+    if ((rc = c->alloc(c->dn_fin, size_t(fw) * fh * 16, "denoise out"))) return rc;
+    hipError_t e = pack_frame(static_cast<const float4*>(c->accum.p), c->gbuf.p, fw, fh, d, bfp(u.in0), c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "unet input");
+    if ((rc = forward(c, u))) return rc;
+    e = unpack_frame(fp(u.pred), d, fw, fh, exposure, static_cast<float4*>(c->dn_fin.p),
+                     static_cast<uchar4*>(c->out.p), c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "unet output");
+    return c->sync("unet denoise");
+}
+
+}  // extern "C"

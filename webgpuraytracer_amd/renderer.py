@@ -190,3 +190,89 @@ class Denoiser:
         a = np.empty((self.r.height, self.r.width, 4), np.float32)
         self.r._check(self.r.lib.pt_read_denoised(self.r.ctx, a.ctypes.data, a.nbytes))
         return a
+
+
+class LearnedDenoiser:
+    """The learned alternative to Denoiser on the same frame (pt_unet.h): a
+    5-level U-Net over radiance, albedo, normal and depth, trained on the
+    device.  Features are fp32 [n][h][w][10], targets [n][h][w][3]."""
+
+    N_FEATURES = 10
+
+    def __init__(self, renderer, seed=0):
+        self.r = renderer
+        self.exposure = 1.0
+        self.r._check(self.r.lib.pt_unet_init(self.r.ctx, seed))
+
+    def param_count(self):
+        n = C.c_uint64()
+        self.r._check(self.r.lib.pt_unet_param_count(self.r.ctx, C.byref(n)))
+        return n.value
+
+    def get_params(self):
+        p = np.empty(self.param_count(), np.float32)
+        self.r._check(self.r.lib.pt_unet_get_params(self.r.ctx, p.ctypes.data, p.size))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self.r._check(self.r.lib.pt_unet_set_params(self.r.ctx, p.ctypes.data, p.size))
+
+    @staticmethod
+    def _nhwc(a, c):
+        a = np.ascontiguousarray(a, np.float32)
+        if a.ndim != 4 or a.shape[3] != c:
+            raise ValueError("expected [n][h][w][%d], got %s" % (c, a.shape))
+        return a
+
+    def forward(self, feat):
+        feat = self._nhwc(feat, self.N_FEATURES)
+        n, h, w, _ = feat.shape
+        pred = np.empty((n, h, w, 3), np.float32)
+        self.r._check(self.r.lib.pt_unet_forward(self.r.ctx, feat.ctypes.data, n, h, w, pred.ctypes.data))
+        return pred
+
+    def loss_grad(self, feat, target, with_grad=True):
+        feat, target = self._nhwc(feat, self.N_FEATURES), self._nhwc(target, 3)
+        n, h, w, _ = feat.shape
+        if target.shape[:3] != (n, h, w):
+            raise ValueError("target shape %s does not match features %s" % (target.shape, feat.shape))
+        loss = C.c_float()
+        grad = np.empty(self.param_count(), np.float32) if with_grad else None
+        self.r._check(self.r.lib.pt_unet_loss_grad(self.r.ctx, feat.ctypes.data, target.ctypes.data, n, h, w,
+                                                   C.byref(loss), grad.ctypes.data if with_grad else None))
+        return loss.value, grad
+
+    def set_dataset(self, feat, target):
+        feat, target = self._nhwc(feat, self.N_FEATURES), self._nhwc(target, 3)
+        n, h, w, _ = feat.shape
+        self.r._check(self.r.lib.pt_unet_set_dataset(self.r.ctx, feat.ctypes.data, target.ctypes.data, n, h, w))
+
+    def train(self, steps, batch, lr=1e-3, seed=0):
+        losses = np.empty(steps, np.float32)
+        self.r._check(self.r.lib.pt_unet_train(self.r.ctx, steps, batch, lr, seed, losses.ctypes.data))
+        return losses
+
+    def execute(self):
+        """Denoises the renderer's current frame; returns the rgba8 output."""
+        self.r._check(self.r.lib.pt_unet_denoise(self.r.ctx, self.exposure))
+        return self.r.read_output()
+
+    def read_result(self):
+        a = np.empty((self.r.height, self.r.width, 4), np.float32)
+        self.r._check(self.r.lib.pt_read_denoised(self.r.ctx, a.ctypes.data, a.nbytes))
+        return a
+
+    def frame_features(self):
+        """The renderer's current frame as network features [1][h][w][10]
+        (accumulated radiance, G-buffer albedo, normal and hit distance)."""
+        acc = self.r.read_accum()
+        g = self.r.read_gbuffer()
+        h, w = self.r.height, self.r.width
+        f = np.empty((1, h, w, self.N_FEATURES), np.float32)
+        f[0, :, :, 0:3] = acc.reshape(h, w, 4)[:, :, :3]
+        gv = g.reshape(h, w, 12)
+        f[0, :, :, 3:6] = gv[:, :, 4:7]
+        f[0, :, :, 6:9] = gv[:, :, 0:3]
+        f[0, :, :, 9] = gv[:, :, 11]
+        return f
