@@ -47,6 +47,16 @@ int pt_unet_loss_grad(pt_ctx* ctx, const float* feat, const float* target, uint3
 /* Keeps a training set on the device: n crops of h x w. */
 int pt_unet_set_dataset(pt_ctx* ctx, const float* feat, const float* target, uint32_t n, uint32_t h, uint32_t w);
 
+/* The training set built on the device from rendered frames: room for n
+ * crops of h x w (cleared), then one call per crop and kind copying the h x w
+ * window at (x0, y0) of the current frame: PT_UNET_INPUT packs the
+ * accumulated radiance and the G-buffer (as pt_unet_denoise reads them),
+ * PT_UNET_TARGET the accumulated radiance.  Replaces a pt_unet_set_dataset
+ * set. */
+enum { PT_UNET_INPUT = 0, PT_UNET_TARGET = 1 };
+int pt_unet_dataset_reserve(pt_ctx* ctx, uint32_t n, uint32_t h, uint32_t w);
+int pt_unet_dataset_capture(pt_ctx* ctx, uint32_t index, int what, uint32_t x0, uint32_t y0);
+
 /* `steps` Adam steps (lr, beta 0.9 / 0.999, eps 1e-8) on batches of `batch`
  * crops drawn from the dataset with `seed`; losses[i] (optional) receives
  * step i's batch loss. */

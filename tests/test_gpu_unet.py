@@ -164,3 +164,29 @@ def test_argument_errors(net):
         net.forward(np.zeros((1, 32, 32, 9), np.float32))
     with pytest.raises(A.PTError):  # zero steps
         net.r._check(net.r.lib.pt_unet_train(net.r.ctx, 0, 1, 1e-3, 0, None))
+
+
+def test_dataset_capture_from_frames_matches_host_features(renderer, net):
+    """Crops captured on the device from a rendered frame train on exactly the
+    features and targets the host would build from the same frame."""
+    scene = SceneBuilder.config(1, seed=5)
+    renderer.set_scene(scene, isgbuf=True)
+    renderer.resize(96, 64)
+    renderer.gbuffer()
+    renderer.render(spp=8, base_seed=3)
+    net.dataset_reserve(1, 32, 48)
+    net.capture(0, target=True, x0=40, y0=16)
+    target = renderer.read_accum()[16:48, 40:88, :3][None].copy()
+    renderer.reset()
+    renderer.render(spp=1, base_seed=9)
+    net.capture(0, target=False, x0=40, y0=16)
+    feat = net.frame_features()[:, 16:48, 40:88].copy()
+    with pytest.raises(Exception):
+        net.capture(0, target=False, x0=60, y0=0)  # window past the right edge
+    with pytest.raises(Exception):
+        net.capture(1, target=False)  # crop index outside the set
+    p0 = net.get_params()
+    losses = net.train(steps=2, batch=1, lr=0.0, seed=1)  # lr 0: weights unchanged
+    assert np.array_equal(net.get_params(), p0)
+    host, _ = net.loss_grad(feat, target, with_grad=False)
+    assert losses[0] == pytest.approx(host, rel=1e-6) and losses[1] == pytest.approx(host, rel=1e-6)

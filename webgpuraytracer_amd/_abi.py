@@ -132,6 +132,7 @@ ERR_NAMES = {0: "PT_OK", -1: "PT_ERR_ARG", -2: "PT_ERR_LAYOUT", -3: "PT_ERR_CAPA
 PT_OPT_BACKEND, PT_OPT_COUNT_STATS, PT_OPT_WAVEFRONT_POOL, PT_OPT_SCHEDULE, PT_OPT_TRAVERSAL, PT_OPT_LDS_STACK, \
     PT_OPT_WAVEFRONT_REFILL = 1, 2, 3, 4, 5, 6, 7
 PT_BACKEND_MEGAKERNEL, PT_BACKEND_WAVEFRONT = 0, 1
+PT_UNET_INPUT, PT_UNET_TARGET = 0, 1
 PT_SCHEDULE_SYNC, PT_SCHEDULE_REGEN = 0, 1
 PT_TRAVERSAL_REFERENCE, PT_TRAVERSAL_WIDE = 0, 1
 
@@ -175,6 +176,8 @@ PT_FUNCS = {
     "pt_unet_forward": (C.c_int, [vp, vp, u32, u32, u32, vp]),
     "pt_unet_loss_grad": (C.c_int, [vp, vp, vp, u32, u32, u32, P(f32), vp]),
     "pt_unet_set_dataset": (C.c_int, [vp, vp, vp, u32, u32, u32]),
+    "pt_unet_dataset_reserve": (C.c_int, [vp, u32, u32, u32]),
+    "pt_unet_dataset_capture": (C.c_int, [vp, u32, C.c_int, u32, u32]),
     "pt_unet_train": (C.c_int, [vp, u32, u32, f32, u64, vp]),
     "pt_unet_denoise": (C.c_int, [vp, f32]),
 }

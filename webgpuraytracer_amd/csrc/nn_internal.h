@@ -36,21 +36,32 @@ constexpr uint32_t kMaxCout = 128;
 hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bias, const Tens& out, uint32_t cout,
                    Dims d, bool relu, float* out_f32, uint32_t f32_channels, hipStream_t s);
 
+// Backward staging for a layer with c output channels and output gradient
+// ga: dz = ga * (a > 0) when a is given (ReLU output), else ga; written NHWC
+// contiguous (dz, [pixel][c], the data-gradient input) and channel-major
+// (dzc, [c][row of chw_elems(1, d) pixels, zero past the last], the
+// weight-gradient input); db[c] (fp32, overwritten) = sum over pixels of dz.
+// scratch holds dz_scratch_floats(c, d) floats.
+size_t chw_elems(uint32_t c, Dims d);
+size_t dz_scratch_floats(uint32_t c, Dims d);
+hipError_t dz_prepare(const Tens& ga, const Tens* a, uint32_t c, Dims d, bf16* dz, bf16* dzc, float* scratch,
+                      float* db, hipStream_t s);
+// The layer input channel-major (rows as dzc), x_copies(d) copies of
+// chw_elems(c, d) elements: with w a multiple of 8 one copy per horizontal tap
+// offset, xs[dx][ch][p] = x[p + dx - 1][ch] within the pixel's image row, else
+// 0 (dx = 0..2); otherwise one per tap, xs[tap][ch][p] = x[p + offset(tap)][ch]
+// inside the image, else 0.  c <= kMaxCout.
+uint32_t x_copies(Dims d);
+hipError_t x_prepare(const Tens& x, uint32_t c, Dims d, bf16* xs, hipStream_t s);
+
 // Weight gradient of a 3x3 convolution over an input-channel slice:
 // dw[tap][co][ci_off + ci] (fp32, rows of cin_total, overwritten) = sum over
-// pixels of dz[p][co] * x[p + offset(tap)][ci] for ci < cin (cin <= kMaxCout);
-// partial holds wgrad_partial_floats(cin, cout, d) floats of scratch.
+// pixels of dz[p][co] * x[p + offset(tap)][ci] for ci < cin (cin <= kMaxCout),
+// from dzc and xs as dz_prepare and x_prepare write them; partial holds
+// wgrad_partial_floats(cin, cout, d) floats of scratch.
 size_t wgrad_partial_floats(uint32_t cin, uint32_t cout, Dims d);
-hipError_t conv3x3_wgrad(const Tens& dz, uint32_t cout, const Tens& x, uint32_t cin, Dims d, float* partial,
+hipError_t conv3x3_wgrad(const bf16* dzc, uint32_t cout, const bf16* xs, uint32_t cin, Dims d, float* partial,
                          float* dw, uint32_t ci_off, uint32_t cin_total, hipStream_t s);
-
-// db[co] = sum over pixels of dz[p][co] (fp32, overwritten); partial holds
-// bias_partial_floats(cout) floats of scratch.
-size_t bias_partial_floats(uint32_t cout);
-hipError_t bias_grad(const Tens& dz, uint32_t cout, Dims d, float* partial, float* db, hipStream_t s);
-
-// dz = da * (a > 0) for a ReLU layer with output a; dz is contiguous (stride c).
-hipError_t relu_grad(const Tens& da, const Tens& a, uint32_t c, Dims d, bf16* dz, hipStream_t s);
 
 // 2x2 average pooling (d: the input's dims; h and w even).
 hipError_t pool2(const Tens& in, uint32_t c, Dims d, const Tens& out, hipStream_t s);
@@ -80,9 +91,14 @@ hipError_t adam(float* w, const float* g, float* m, float* v, size_t n, float lr
 // normal, depth / (depth + 1), then zeros.
 hipError_t pack_input(const float* feat, Dims d, bf16* out, hipStream_t s);
 // The same input straight from a rendered frame: accum (float4 per pixel),
-// G-buffer (48 B texels), into a frame padded to d (pad pixels zero).
-hipError_t pack_frame(const float4* accum, const void* gbuf, uint32_t fw, uint32_t fh, Dims d, bf16* out,
-                      hipStream_t s);
+// G-buffer (48 B texels), the d.w x d.h window at (x0, y0) of the fw x fh
+// frame (d.n = 1; pixels outside the frame are zero, which pads a frame up
+// to a multiple of 16).
+hipError_t pack_frame(const float4* accum, const void* gbuf, uint32_t fw, uint32_t fh, uint32_t x0, uint32_t y0,
+                      Dims d, bf16* out, hipStream_t s);
+// Radiance of the d.w x d.h window at (x0, y0) (inside the frame) as fp32 [pixel][3].
+hipError_t crop_radiance(const float4* accum, uint32_t fw, uint32_t fh, uint32_t x0, uint32_t y0, Dims d, float* out,
+                         hipStream_t s);
 
 // L1 loss on log(1 + x): pred [pixel][3] fp32 is in log space already, target
 // [pixel][3] is linear radiance.  Writes d loss / d pred as bf16 [pixel][16]

@@ -10,12 +10,13 @@
 //               rows = 16 pixels per wave, columns = output channels, K = the
 //               9 taps x input channels in chunks of 32; bias + ReLU fused
 //   wgrad_k     weight gradient: rows = output channels, columns = input
-//               channels, K = pixels; dz and the tap-shifted input are staged
-//               through LDS transposed (channel-major), so every fragment is
-//               one 16-byte LDS read; split over pixel chunks, partial sums
-//               reduced by wgrad_reduce_k (deterministic)
-//   the rest    pooling, upsampling, ReLU gradient, Adam, loss, packing
+//               channels, K = pixels, both operands read channel-major
+//               (dz_prepare_k, x_prepare_k stage them); split over pixel
+//               chunks, partial sums reduced in a fixed order
+//   the rest    pooling, upsampling, Adam, loss, packing (16-byte vectors)
 #include "nn_internal.h"
+
+#include <algorithm>
 
 namespace pt {
 namespace nn {
@@ -32,36 +33,66 @@ __device__ __forceinline__ float f(bf16 x) { return float(x); }
 __device__ __forceinline__ bf16 b(float x) { return bf16(x); }
 
 // ------------------------------------------------------------- conv fwd --
-template <int NCT, bool RELU, bool F32OUT>
+// A wave computes MT x 16 consecutive pixels by all NCT x 16 output channels:
+// per K step (one tap, 32 input channels) it loads MT input fragments and NCT
+// weight fragments and issues MT x NCT products.
+template <int NCT>
+struct ConvTiles {
+    static constexpr int MT = NCT <= 2 ? 8 : (NCT <= 4 ? 4 : 2);
+};
+
+template <int NCT, int MT, bool RELU, bool F32OUT>
 __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const bf16* __restrict__ w,
                                                     const float* __restrict__ bias, Tens out, uint32_t cout, Dims d,
                                                     float* __restrict__ of32, uint32_t f32c) {
     const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint32_t npix = d.pixels();
-    const uint32_t p0 = (blockIdx.x * 4u + wv) * 16u;
+    const uint32_t p0 = (blockIdx.x * 4u + wv) * (16u * MT);
     if (p0 >= npix) return;  // the whole wave is past the end
-    const uint32_t pix = p0 + (lane & 15u), kq = lane >> 4;
-    const bool pv = pix < npix;
-    const uint32_t x = pix % d.w, y = (pix / d.w) % d.h, img = pix / (d.w * d.h);
-    f32x4 acc[NCT];
+    const uint32_t kq = lane >> 4;
+    uint32_t px[MT], py[MT], pimg[MT];
+    bool pv[MT];
 #pragma unroll
-    for (int t = 0; t < NCT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int m = 0; m < MT; ++m) {
+        const uint32_t pix = p0 + uint32_t(m) * 16u + (lane & 15u);
+        pv[m] = pix < npix;
+        px[m] = pix % d.w;
+        py[m] = (pix / d.w) % d.h;
+        pimg[m] = pix / (d.w * d.h);
+    }
+    f32x4 acc[MT][NCT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int t = 0; t < NCT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (uint32_t tap = 0; tap < 9u; ++tap) {
-        const int sy = int(y) + int(tap / 3u) - 1, sx = int(x) + int(tap % 3u) - 1;
-        const bool valid = pv && sy >= 0 && sy < int(d.h) && sx >= 0 && sx < int(d.w);
-        const size_t sp = valid ? (size_t(img) * d.h + uint32_t(sy)) * d.w + uint32_t(sx) : 0;
-        const bf16* src = in.p + sp * in.stride + in.off;
+        const int dy = int(tap / 3u) - 1, dx = int(tap % 3u) - 1;
+        const bf16* src[MT];
+        bool valid[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int sy = int(py[m]) + dy, sx = int(px[m]) + dx;
+            valid[m] = pv[m] && sy >= 0 && sy < int(d.h) && sx >= 0 && sx < int(d.w);
+            const size_t sp = valid[m] ? (size_t(pimg[m]) * d.h + uint32_t(sy)) * d.w + uint32_t(sx) : 0;
+            src[m] = in.p + sp * in.stride + in.off;
+        }
         const bf16* wt = w + size_t(tap) * cout * cin;
         for (uint32_t c0 = 0; c0 < cin; c0 += 32u) {
             const uint32_t c = c0 + kq * 8u;
             const bool cv = c < cin;
-            const bf16x8 a = (valid && cv) ? load8(src + c) : zero8();
+            bf16x8 a[MT], bw[NCT];
+#pragma unroll
+            for (int m = 0; m < MT; ++m) a[m] = (valid[m] && cv) ? load8(src[m] + c) : zero8();
 #pragma unroll
             for (int t = 0; t < NCT; ++t) {
                 const uint32_t co = uint32_t(t) * 16u + (lane & 15u);
-                const bf16x8 bw = cv ? load8(wt + size_t(co) * cin + c) : zero8();
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw, acc[t], 0, 0, 0);
+                bw[t] = cv ? load8(wt + size_t(co) * cin + c) : zero8();
             }
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+#pragma unroll
+                for (int t = 0; t < NCT; ++t)
+                    acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], bw[t], acc[m][t], 0, 0, 0);
         }
     }
 #pragma unroll
@@ -69,33 +100,37 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
         const uint32_t co = uint32_t(t) * 16u + (lane & 15u);
         const float bv = bias ? bias[co] : 0.0f;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const uint32_t op = p0 + kq * 4u + uint32_t(r);
-            if (op >= npix) continue;
-            float v = acc[t][r] + bv;
-            if (RELU) v = fmaxf(v, 0.0f);
-            if (F32OUT) {
-                if (co < f32c) of32[size_t(op) * f32c + co] = v;
-            } else {
-                out.p[size_t(op) * out.stride + out.off + co] = b(v);
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t op = p0 + uint32_t(m) * 16u + kq * 4u + uint32_t(r);
+                if (op >= npix) continue;
+                float v = acc[m][t][r] + bv;
+                if (RELU) v = fmaxf(v, 0.0f);
+                if (F32OUT) {
+                    if (co < f32c) of32[size_t(op) * f32c + co] = v;
+                } else {
+                    out.p[size_t(op) * out.stride + out.off + co] = b(v);
+                }
             }
-        }
     }
 }
 
 template <int NCT>
 static void conv_launch(const Tens& in, uint32_t cin, const bf16* w, const float* bias, const Tens& out,
                         uint32_t cout, Dims d, bool relu, float* of32, uint32_t f32c, hipStream_t s) {
-    const uint32_t blocks = (d.pixels() + 63u) / 64u;
+    constexpr int MT = ConvTiles<NCT>::MT;
+    const uint32_t per_block = 4u * 16u * MT;
+    const uint32_t blocks = (d.pixels() + per_block - 1u) / per_block;
     if (of32)
-        hipLaunchKernelGGL((conv3x3_k<NCT, false, true>), dim3(blocks), dim3(kBlock), 0, s, in, cin, w, bias, out,
+        hipLaunchKernelGGL((conv3x3_k<NCT, MT, false, true>), dim3(blocks), dim3(kBlock), 0, s, in, cin, w, bias, out,
                            cout, d, of32, f32c);
     else if (relu)
-        hipLaunchKernelGGL((conv3x3_k<NCT, true, false>), dim3(blocks), dim3(kBlock), 0, s, in, cin, w, bias, out,
+        hipLaunchKernelGGL((conv3x3_k<NCT, MT, true, false>), dim3(blocks), dim3(kBlock), 0, s, in, cin, w, bias, out,
                            cout, d, of32, f32c);
     else
-        hipLaunchKernelGGL((conv3x3_k<NCT, false, false>), dim3(blocks), dim3(kBlock), 0, s, in, cin, w, bias, out,
-                           cout, d, of32, f32c);
+        hipLaunchKernelGGL((conv3x3_k<NCT, MT, false, false>), dim3(blocks), dim3(kBlock), 0, s, in, cin, w, bias,
+                           out, cout, d, of32, f32c);
 }
 
 hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bias, const Tens& out, uint32_t cout,
@@ -117,196 +152,374 @@ hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bia
     return hipGetLastError();
 }
 
-// ---------------------------------------------------------------- wgrad --
-constexpr uint32_t kWgStep = 32;     // pixels per K step
-constexpr uint32_t kWgPad = 40;      // LDS row length (bf16): 80 B rows keep 16-byte alignment
-constexpr uint32_t kWgBlocks = 2048; // target workgroups per launch
+// ------------------------------------------------------- backward staging --
+// The weight gradient reduces over pixels, so its MFMA operands need 8
+// consecutive pixels of one channel per lane: the backward pass stages dz and
+// the layer input channel-major.  The input gets one copy per horizontal tap
+// offset, so every tap's fragment is one aligned 16-byte load (a vertical
+// offset moves by whole rows, and w is a multiple of 16).
+constexpr uint32_t kStagePix = 64;  // pixels per staging block
 
-static uint32_t wgrad_chunks(uint32_t cout, Dims d) {
-    const uint32_t groups = 9u * ((cout + 31u) / 32u);
-    const uint32_t steps = (d.pixels() + kWgStep - 1u) / kWgStep;
-    uint32_t chunks = kWgBlocks / groups;
-    if (chunks < 1u) chunks = 1u;
-    if (chunks > steps) chunks = steps;
-    return chunks;
+__device__ __forceinline__ void store8(bf16* p, bf16x8 v) { *reinterpret_cast<bf16x8*>(p) = v; }
+
+// Channel-major arrays have rows of chw_stride(d) = P rounded up to 64
+// pixels; the pixels past P are zero.
+static __host__ __device__ uint32_t chw_stride_of(uint32_t P) { return (P + kStagePix - 1u) / kStagePix * kStagePix; }
+
+// xs[dx][ch][p] = x[p + dx - 1][ch] when that pixel is in the same image row,
+// else 0 (dx = 0, 1, 2); rows of Ps pixels.  For w a multiple of 8.
+__global__ void __launch_bounds__(kBlock) x_prepare_k(Tens x, uint32_t c, Dims d, uint32_t Ps,
+                                                      bf16* __restrict__ xs) {
+    __shared__ bf16 t[kMaxCout][kStagePix + 8];  // pixels p0 - 1 .. p0 + 64 at index 0 .. 65
+    const uint32_t P = d.pixels();
+    const uint32_t p0 = blockIdx.x * kStagePix;
+    const uint32_t c8 = c / 8u;
+    for (uint32_t piece = threadIdx.x; piece < (kStagePix + 2u) * c8; piece += kBlock) {
+        const uint32_t j = piece / c8, ch = (piece % c8) * 8u;
+        const int p = int(p0) + int(j) - 1;
+        bf16x8 v = zero8();
+        if (p >= 0 && uint32_t(p) < P) v = load8(x.p + size_t(p) * x.stride + x.off + ch);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) t[ch + i][j] = v[i];
+    }
+    __syncthreads();
+    constexpr uint32_t groups = kStagePix / 8u;
+    for (uint32_t item = threadIdx.x; item < 3u * c * groups; item += kBlock) {
+        const uint32_t g = item % groups, ch = (item / groups) % c, dx = item / (groups * c);
+        const uint32_t pb = p0 + g * 8u;
+        bf16x8 v;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int sc = int((pb + uint32_t(i)) % d.w) + int(dx) - 1;
+            v[i] = (pb + uint32_t(i) < P && sc >= 0 && sc < int(d.w)) ? t[ch][g * 8u + uint32_t(i) + dx] : b(0.0f);
+        }
+        store8(xs + (size_t(dx) * c + ch) * Ps + pb, v);
+    }
 }
 
-size_t wgrad_partial_floats(uint32_t cin, uint32_t cout, Dims d) {
-    return size_t(wgrad_chunks(cout, d)) * 9u * cout * cin;
-}
-
-// grid: x = pixel chunk, y = tap, z = group of 32 output channels
-__global__ void __launch_bounds__(kBlock) wgrad_k(Tens dz, uint32_t cout, Tens x, uint32_t cin, Dims d,
-                                                  uint32_t chunk, float* __restrict__ partial) {
-    __shared__ __attribute__((aligned(16))) bf16 sdz[32][kWgPad];    // [co][px]
-    __shared__ __attribute__((aligned(16))) bf16 sx[kMaxCout][kWgPad];  // [ci][px]
-    const uint32_t tap = blockIdx.y, co0 = blockIdx.z * 32u;
+// Any w: one copy per tap, xs[tap][ch][p] = x[p + offset(tap)][ch] inside the
+// image, else 0 (the small coarse levels whose width is not a multiple of 8).
+__global__ void __launch_bounds__(kBlock) x_prepare9_k(Tens x, uint32_t c, Dims d, uint32_t Ps,
+                                                       bf16* __restrict__ xs) {
+    const uint32_t groups = Ps / 8u;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= 9u * c * groups) return;
+    const uint32_t g = i % groups, ch = (i / groups) % c, tap = i / (groups * c);
     const int dy = int(tap / 3u) - 1, dx = int(tap % 3u) - 1;
-    const uint32_t npix = d.pixels();
-    const uint32_t pb = blockIdx.x * chunk;
-    const uint32_t pe = pb + chunk < npix ? pb + chunk : npix;
-    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, kq = lane >> 4;
-    const uint32_t cit = cin / 16u, ntiles = 2u * cit;
-    f32x4 acc[4];
+    const uint32_t P = d.pixels();
+    bf16x8 v;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (uint32_t p = pb; p < pe; p += kWgStep) {
-        // stage dz: 32 px x 32 co, 4 channels per thread
-        {
-            const uint32_t px = threadIdx.x >> 3, c4 = (threadIdx.x & 7u) * 4u;
-            const uint32_t gp = p + px, co = co0 + c4;
-            float v[4] = {0.f, 0.f, 0.f, 0.f};
-            if (gp < pe && co < cout) {
-                const bf16* q = dz.p + size_t(gp) * dz.stride + dz.off + co;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) v[i] = f(q[i]);
-            }
-#pragma unroll
-            for (int i = 0; i < 4; ++i) sdz[c4 + i][px] = b(v[i]);
+    for (int j = 0; j < 8; ++j) {
+        const uint32_t q = g * 8u + uint32_t(j);
+        bf16 val = b(0.0f);
+        if (q < P) {
+            const int sx = int(q % d.w) + dx, sy = int((q / d.w) % d.h) + dy;
+            const uint32_t img = q / (d.w * d.h);
+            if (sx >= 0 && sx < int(d.w) && sy >= 0 && sy < int(d.h))
+                val = x.p[((size_t(img) * d.h + uint32_t(sy)) * d.w + uint32_t(sx)) * x.stride + x.off + ch];
         }
-        // stage the tap-shifted input: 32 px x cin, 8 channels per piece
-        for (uint32_t piece = threadIdx.x; piece < 32u * (cin / 8u); piece += kBlock) {
-            const uint32_t px = piece / (cin / 8u), c8 = (piece % (cin / 8u)) * 8u;
-            const uint32_t gp = p + px;
-            bf16x8 v = zero8();
-            if (gp < pe) {
-                const uint32_t xx = gp % d.w, yy = (gp / d.w) % d.h, img = gp / (d.w * d.h);
-                const int sy = int(yy) + dy, sx2 = int(xx) + dx;
-                if (sy >= 0 && sy < int(d.h) && sx2 >= 0 && sx2 < int(d.w))
-                    v = load8(x.p + ((size_t(img) * d.h + uint32_t(sy)) * d.w + uint32_t(sx2)) * x.stride + x.off + c8);
-            }
-#pragma unroll
-            for (int i = 0; i < 8; ++i) sx[c8 + i][px] = v[i];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const uint32_t t = wv + 4u * uint32_t(i);
-            if (t < ntiles) {
-                const uint32_t cot = t & 1u, cti = t >> 1;
-                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sdz[cot * 16u + (lane & 15u)][kq * 8u]);
-                const bf16x8 bb = *reinterpret_cast<const bf16x8*>(&sx[cti * 16u + (lane & 15u)][kq * 8u]);
-                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc[i], 0, 0, 0);
-            }
-        }
-        __syncthreads();
+        v[j] = val;
     }
-    float* out = partial + size_t(blockIdx.x * 9u + tap) * cout * cin;
+    store8(xs + (size_t(tap) * c + ch) * Ps + g * 8u, v);
+}
+
+// dz = ga * (a > 0) (relu) or ga, for 64 pixels per block: NHWC [p][c] (dz),
+// channel-major [c][p] (dzc) and the block's channel sums bpart[block][c].
+template <bool RELU>
+__global__ void __launch_bounds__(kBlock) dz_prepare_k(Tens ga, Tens a, uint32_t c, uint32_t P, uint32_t Ps,
+                                                       bf16* __restrict__ dz, bf16* __restrict__ dzc,
+                                                       float* __restrict__ bpart) {
+    __shared__ bf16 t[kMaxCout][kStagePix + 8];
+    const uint32_t p0 = blockIdx.x * kStagePix;
+    const uint32_t c8 = c / 8u;
+    for (uint32_t piece = threadIdx.x; piece < kStagePix * c8; piece += kBlock) {
+        const uint32_t j = piece / c8, ch = (piece % c8) * 8u;
+        const uint32_t p = p0 + j;
+        bf16x8 v = zero8();
+        if (p < P) {
+            v = load8(ga.p + size_t(p) * ga.stride + ga.off + ch);
+            if (RELU) {
+                const bf16x8 av = load8(a.p + size_t(p) * a.stride + a.off + ch);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const uint32_t t = wv + 4u * uint32_t(i);
-        if (t >= ntiles) continue;
-        const uint32_t cot = t & 1u, cti = t >> 1;
-        const uint32_t ci = cti * 16u + (lane & 15u);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const uint32_t co = co0 + cot * 16u + kq * 4u + uint32_t(r);
-            if (co < cout) out[size_t(co) * cin + ci] = acc[i][r];
+                for (int i = 0; i < 8; ++i)
+                    if (!(f(av[i]) > 0.0f)) v[i] = b(0.0f);
+            }
+            store8(dz + size_t(p) * c + ch, v);
         }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) t[ch + i][j] = v[i];
+    }
+    __syncthreads();
+    constexpr uint32_t groups = kStagePix / 8u;
+    for (uint32_t item = threadIdx.x; item < c * groups; item += kBlock) {
+        const uint32_t g = item % groups, ch = item / groups;
+        const uint32_t pb = p0 + g * 8u;
+        bf16x8 v;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = t[ch][g * 8u + uint32_t(i)];
+        store8(dzc + size_t(ch) * Ps + pb, v);
+    }
+    for (uint32_t ch = threadIdx.x; ch < c; ch += kBlock) {
+        float sum = 0.0f;
+        for (uint32_t j = 0; j < kStagePix; ++j) sum += f(t[ch][j]);
+        bpart[size_t(blockIdx.x) * c + ch] = sum;
     }
 }
 
-// out[row][col_off + col] (row length out_cols) = sum over chunks of
-// partial[chunk][row][col] (row length cols), for rows x cols = n entries
-__global__ void __launch_bounds__(kBlock) reduce_chunks_k(const float* __restrict__ partial, uint32_t chunks,
-                                                          uint32_t n, uint32_t cols, uint32_t out_cols,
-                                                          uint32_t col_off, float* __restrict__ out) {
+// ----------------------------------------------------------- reductions --
+// Deterministic sums over chunks: out[row * out_cols + col_off + col] =
+// sum over k < chunks of partial[k * n + row * cols + col].  Pass 1 sums
+// segments of at most kRedSeg chunks into tmp[segment][n]; pass 2 sums the
+// segments.
+constexpr uint32_t kRedSeg = 32;
+
+__global__ void __launch_bounds__(kBlock) reduce_seg_k(const float* __restrict__ partial, uint32_t chunks, uint32_t n,
+                                                       float* __restrict__ tmp) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k0 = blockIdx.y * kRedSeg, k1 = min(k0 + kRedSeg, chunks);
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    uint32_t k = k0;
+    for (; k + 4u <= k1; k += 4u) {
+        s0 += partial[size_t(k) * n + i];
+        s1 += partial[size_t(k + 1u) * n + i];
+        s2 += partial[size_t(k + 2u) * n + i];
+        s3 += partial[size_t(k + 3u) * n + i];
+    }
+    for (; k < k1; ++k) s0 += partial[size_t(k) * n + i];
+    tmp[size_t(blockIdx.y) * n + i] = (s0 + s1) + (s2 + s3);
+}
+
+__global__ void __launch_bounds__(kBlock) reduce_fin_k(const float* __restrict__ tmp, uint32_t segs, uint32_t n,
+                                                       uint32_t cols, uint32_t out_cols, uint32_t col_off,
+                                                       float* __restrict__ out) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     float s = 0.0f;
-    for (uint32_t c = 0; c < chunks; ++c) s += partial[size_t(c) * n + i];
+    for (uint32_t k = 0; k < segs; ++k) s += tmp[size_t(k) * n + i];
     const uint32_t row = i / cols, col = i - row * cols;
     out[size_t(row) * out_cols + col_off + col] = s;
 }
 
-hipError_t conv3x3_wgrad(const Tens& dz, uint32_t cout, const Tens& x, uint32_t cin, Dims d, float* partial,
-                         float* dw, uint32_t ci_off, uint32_t cin_total, hipStream_t s) {
-    if (cin == 0 || cin % 16u || cin > kMaxCout || cout % 16u || cout == 0 || cout > kMaxCout || x.stride % 8u ||
-        x.off % 8u || ci_off + cin > cin_total)
+static uint32_t red_segs(uint32_t chunks) { return (chunks + kRedSeg - 1u) / kRedSeg; }
+
+static hipError_t reduce_chunks(const float* partial, uint32_t chunks, uint32_t n, uint32_t cols, uint32_t out_cols,
+                                uint32_t col_off, float* tmp, float* out, hipStream_t s) {
+    const uint32_t segs = red_segs(chunks);
+    hipLaunchKernelGGL(reduce_seg_k, dim3((n + kBlock - 1u) / kBlock, segs), dim3(kBlock), 0, s, partial, chunks, n,
+                       tmp);
+    hipLaunchKernelGGL(reduce_fin_k, dim3((n + kBlock - 1u) / kBlock), dim3(kBlock), 0, s, tmp, segs, n, cols,
+                       out_cols, col_off, out);
+    return hipGetLastError();
+}
+
+size_t dz_scratch_floats(uint32_t c, Dims d) {
+    const uint32_t blocks = (d.pixels() + kStagePix - 1u) / kStagePix;
+    return size_t(blocks) * c + size_t(red_segs(blocks)) * c;
+}
+
+size_t chw_elems(uint32_t c, Dims d) { return size_t(chw_stride_of(d.pixels())) * c; }
+uint32_t x_copies(Dims d) { return d.w % 8u == 0 ? 3u : 9u; }
+
+hipError_t dz_prepare(const Tens& ga, const Tens* a, uint32_t c, Dims d, bf16* dz, bf16* dzc, float* scratch,
+                      float* db, hipStream_t s) {
+    const uint32_t P = d.pixels();
+    if (c == 0 || c % 16u || c > kMaxCout || ga.stride % 8u || ga.off % 8u || (a && (a->stride % 8u || a->off % 8u)) ||
+        P == 0)
         return hipErrorInvalidValue;
-    const uint32_t n = 9u * cout * cin;
-    if (d.pixels() == 0) return hipErrorInvalidValue;
-    const uint32_t chunks = wgrad_chunks(cout, d);
-    const uint32_t steps = (d.pixels() + kWgStep - 1u) / kWgStep;
-    const uint32_t chunk = ((steps + chunks - 1u) / chunks) * kWgStep;
-    const uint32_t used = (d.pixels() + chunk - 1u) / chunk;
-    hipLaunchKernelGGL(wgrad_k, dim3(used, 9, (cout + 31u) / 32u), dim3(kBlock), 0, s, dz, cout, x, cin, d, chunk,
-                       partial);
-    hipLaunchKernelGGL(reduce_chunks_k, dim3((n + kBlock - 1u) / kBlock), dim3(kBlock), 0, s, partial, used, n, cin,
-                       cin_total, ci_off, dw);
-    return hipGetLastError();
+    const uint32_t Ps = chw_stride_of(P);
+    const uint32_t blocks = Ps / kStagePix;
+    float* bpart = scratch;
+    float* tmp = scratch + size_t(blocks) * c;
+    if (a)
+        hipLaunchKernelGGL(dz_prepare_k<true>, dim3(blocks), dim3(kBlock), 0, s, ga, *a, c, P, Ps, dz, dzc, bpart);
+    else
+        hipLaunchKernelGGL(dz_prepare_k<false>, dim3(blocks), dim3(kBlock), 0, s, ga, ga, c, P, Ps, dz, dzc, bpart);
+    return reduce_chunks(bpart, blocks, c, c, c, 0u, tmp, db, s);
 }
 
-// ------------------------------------------------------------ bias grad --
-constexpr uint32_t kBgChunks = 256;
-__global__ void __launch_bounds__(kBlock) bias_grad_k(Tens dz, uint32_t cout, Dims d, uint32_t chunk,
-                                                      float* __restrict__ partial) {
-    __shared__ float red[kBlock];
-    const uint32_t lanes = kBlock / cout;  // >= 2 for cout <= 128; threads past lanes * cout idle
-    const uint32_t co = threadIdx.x % cout, ph = threadIdx.x / cout;
-    const uint32_t npix = d.pixels();
-    const uint32_t pb = blockIdx.x * chunk;
-    const uint32_t pe = pb + chunk < npix ? pb + chunk : npix;
-    float sum = 0.0f;
-    if (ph < lanes)
-        for (uint32_t p = pb + ph; p < pe; p += lanes) sum += f(dz.p[size_t(p) * dz.stride + dz.off + co]);
-    red[threadIdx.x] = sum;
-    __syncthreads();
-    if (threadIdx.x < cout) {
-        float t = 0.0f;
-        for (uint32_t k = 0; k < lanes; ++k) t += red[k * cout + threadIdx.x];
-        partial[size_t(blockIdx.x) * cout + threadIdx.x] = t;
+hipError_t x_prepare(const Tens& x, uint32_t c, Dims d, bf16* xs, hipStream_t s) {
+    const uint32_t P = d.pixels();
+    if (c == 0 || c % 16u || c > kMaxCout || x.stride % 8u || x.off % 8u || P == 0) return hipErrorInvalidValue;
+    const uint32_t Ps = chw_stride_of(P);
+    if (x_copies(d) == 3u) {
+        hipLaunchKernelGGL(x_prepare_k, dim3(Ps / kStagePix), dim3(kBlock), 0, s, x, c, d, Ps, xs);
+    } else {
+        const uint32_t n = 9u * c * (Ps / 8u);
+        hipLaunchKernelGGL(x_prepare9_k, dim3((n + kBlock - 1u) / kBlock), dim3(kBlock), 0, s, x, c, d, Ps, xs);
     }
+    return hipGetLastError();
 }
 
-size_t bias_partial_floats(uint32_t cout) { return size_t(kBgChunks) * cout; }
+// ---------------------------------------------------------------- wgrad --
+// Weight gradient as a GEMM over pixels: dW[tap][co][ci] = sum_p dzc[co][p] *
+// xs[tap % 3][ci][p + (tap / 3 - 1) w].  A wave owns one job (T taps, up to
+// NTX input-channel tiles, all cout / 16 <= MTX output-channel tiles) over a
+// chunk of pixels; the 4 waves of a block take 4 jobs of the same chunk, so
+// their dz fragments are shared through the cache.  Partial sums per chunk
+// are reduced by reduce_chunks.
+constexpr uint32_t kWgStep = 32;  // pixels per K step
 
-hipError_t bias_grad(const Tens& dz, uint32_t cout, Dims d, float* partial, float* db, hipStream_t s) {
-    if (cout == 0 || cout > kMaxCout) return hipErrorInvalidValue;
-    const uint32_t npix = d.pixels();
-    const uint32_t chunk = (npix + kBgChunks - 1u) / kBgChunks;
-    const uint32_t used = chunk ? (npix + chunk - 1u) / chunk : 0u;
-    if (used == 0) return fill_zero_f32(db, cout, s);
-    hipLaunchKernelGGL(bias_grad_k, dim3(used), dim3(kBlock), 0, s, dz, cout, d, chunk, partial);
-    hipLaunchKernelGGL(reduce_chunks_k, dim3((cout + kBlock - 1u) / kBlock), dim3(kBlock), 0, s, partial, used, cout,
-                       cout, cout, 0u, db);
-    return hipGetLastError();
+template <int MTX, int NTX, int T, bool FULL9>
+__global__ void __launch_bounds__(kBlock) wgrad_k(const bf16* __restrict__ dzc, uint32_t cout,
+                                                  const bf16* __restrict__ xs, uint32_t cin, Dims d, uint32_t chunk,
+                                                  uint32_t ci_groups, uint32_t jobs, float* __restrict__ partial) {
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, kq = lane >> 4;
+    const uint32_t job = blockIdx.y * 4u + wv;
+    if (job >= jobs) return;
+    const uint32_t tap0 = (job / ci_groups) * uint32_t(T);
+    const uint32_t cg = job % ci_groups;
+    const uint32_t ci0 = cg * uint32_t(NTX) * 16u;
+    const uint32_t nt = min(uint32_t(NTX), (cin - ci0) / 16u);
+    const uint32_t mt = cout / 16u;
+    const uint32_t P = d.pixels(), Ps = chw_stride_of(P);
+    const uint32_t pb = blockIdx.x * chunk, pe = min(pb + chunk, Ps);
+    f32x4 acc[T][NTX][MTX];
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+        for (int n = 0; n < NTX; ++n)
+#pragma unroll
+            for (int m = 0; m < MTX; ++m) acc[t][n][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const bf16* arow[MTX];
+#pragma unroll
+    for (int m = 0; m < MTX; ++m) arow[m] = dzc + size_t(uint32_t(m) * 16u + (lane & 15u)) * Ps;
+    for (uint32_t p = pb; p < pe; p += kWgStep) {
+        // this lane's 8 pixels (zero past P); with three copies they share one
+        // image row, because w is a multiple of 8
+        const uint32_t pk = p + kq * 8u;
+        const int row = int((pk / d.w) % d.h);
+        bf16x8 a[MTX];
+#pragma unroll
+        for (int m = 0; m < MTX; ++m) a[m] = uint32_t(m) < mt ? load8(arow[m] + pk) : zero8();
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const uint32_t tap = tap0 + uint32_t(t);
+            const bf16* xrow;
+            bool rv;
+            if (FULL9) {
+                rv = true;
+                xrow = xs + size_t(tap * cin + ci0 + (lane & 15u)) * Ps + pk;
+            } else {
+                const int dy = int(tap / 3u) - 1;
+                rv = pk < P && row + dy >= 0 && row + dy < int(d.h);
+                const uint32_t src = rv ? uint32_t(int(pk) + dy * int(d.w)) : 0u;
+                xrow = xs + size_t((tap % 3u) * cin + ci0 + (lane & 15u)) * Ps + src;
+            }
+#pragma unroll
+            for (int n = 0; n < NTX; ++n) {
+                if (uint32_t(n) >= nt) continue;
+                const bf16x8 bx = rv ? load8(xrow + size_t(n) * 16u * Ps) : zero8();
+#pragma unroll
+                for (int m = 0; m < MTX; ++m)
+                    if (uint32_t(m) < mt)
+                        acc[t][n][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], bx, acc[t][n][m], 0, 0, 0);
+            }
+        }
+    }
+    float* out = partial + size_t(blockIdx.x) * 9u * cout * cin;
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+        for (int n = 0; n < NTX; ++n) {
+            if (uint32_t(n) >= nt) continue;
+            const uint32_t ci = ci0 + uint32_t(n) * 16u + (lane & 15u);
+#pragma unroll
+            for (int m = 0; m < MTX; ++m) {
+                if (uint32_t(m) >= mt) continue;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const uint32_t co = uint32_t(m) * 16u + kq * 4u + uint32_t(r);
+                    out[(size_t(tap0 + uint32_t(t)) * cout + co) * cin + ci] = acc[t][n][m][r];
+                }
+            }
+        }
+}
+
+struct WgPlan {
+    int kind;        // 0: <2, 4, 3>, 1: <4, 4, 1>, 2: <8, 2, 1>
+    uint32_t T, NT;  // taps per job, ci tiles per job
+    uint32_t jobs, ci_groups, chunks, chunk;
+};
+
+static WgPlan wgrad_plan(uint32_t cin, uint32_t cout, Dims d) {
+    WgPlan p{};
+    const uint32_t mt = cout / 16u;
+    if (mt <= 2u) { p.kind = 0; p.T = 3; p.NT = 4; }
+    else if (mt <= 4u) { p.kind = 1; p.T = 1; p.NT = 4; }
+    else { p.kind = 2; p.T = 1; p.NT = 2; }
+    const uint32_t it = cin / 16u;
+    p.ci_groups = (it + p.NT - 1u) / p.NT;
+    p.jobs = (9u / p.T) * p.ci_groups;
+    const uint32_t Ps = chw_stride_of(d.pixels());
+    const uint32_t steps = Ps / kWgStep;
+    // about 4096 waves in flight, at least 16 K steps per wave
+    uint32_t chunks = 4096u / p.jobs;
+    chunks = std::max(1u, std::min(chunks, steps / 16u));
+    const uint32_t per = (steps + chunks - 1u) / chunks;
+    p.chunk = per * kWgStep;
+    p.chunks = (Ps + p.chunk - 1u) / p.chunk;
+    return p;
+}
+
+size_t wgrad_partial_floats(uint32_t cin, uint32_t cout, Dims d) {
+    const WgPlan p = wgrad_plan(cin, cout, d);
+    const size_t n = size_t(9) * cout * cin;
+    return size_t(p.chunks) * n + size_t(red_segs(p.chunks)) * n;
+}
+
+hipError_t conv3x3_wgrad(const bf16* dzc, uint32_t cout, const bf16* xs, uint32_t cin, Dims d, float* partial,
+                         float* dw, uint32_t ci_off, uint32_t cin_total, hipStream_t s) {
+    if (cin == 0 || cin % 16u || cin > kMaxCout || cout % 16u || cout == 0 || cout > kMaxCout ||
+        ci_off + cin > cin_total || d.pixels() == 0)
+        return hipErrorInvalidValue;
+    const WgPlan p = wgrad_plan(cin, cout, d);
+    const dim3 grid(p.chunks, (p.jobs + 3u) / 4u);
+    const bool full9 = x_copies(d) == 9u;
+#define PT_WGRAD_LAUNCH(MT_, NT_, T_)                                                                               \
+    if (full9)                                                                                                      \
+        hipLaunchKernelGGL((wgrad_k<MT_, NT_, T_, true>), grid, dim3(kBlock), 0, s, dzc, cout, xs, cin, d, p.chunk, \
+                           p.ci_groups, p.jobs, partial);                                                          \
+    else                                                                                                            \
+        hipLaunchKernelGGL((wgrad_k<MT_, NT_, T_, false>), grid, dim3(kBlock), 0, s, dzc, cout, xs, cin, d,        \
+                           p.chunk, p.ci_groups, p.jobs, partial);
+    switch (p.kind) {
+        case 0: PT_WGRAD_LAUNCH(2, 4, 3) break;
+        case 1: PT_WGRAD_LAUNCH(4, 4, 1) break;
+        default: PT_WGRAD_LAUNCH(8, 2, 1) break;
+    }
+#undef PT_WGRAD_LAUNCH
+    const uint32_t n = 9u * cout * cin;
+    return reduce_chunks(partial, p.chunks, n, cin, cin_total, ci_off, partial + size_t(p.chunks) * n, dw, s);
 }
 
 // --------------------------------------------------------- elementwise --
-__global__ void __launch_bounds__(kBlock) relu_grad_k(Tens da, Tens a, uint32_t c, uint32_t n, bf16* __restrict__ dz) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t p = i / c, ch = i - p * c;
-    const float av = f(a.p[size_t(p) * a.stride + a.off + ch]);
-    dz[i] = av > 0.0f ? da.p[size_t(p) * da.stride + da.off + ch] : b(0.0f);
-}
-
-hipError_t relu_grad(const Tens& da, const Tens& a, uint32_t c, Dims d, bf16* dz, hipStream_t s) {
-    const uint32_t n = d.pixels() * c;
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(relu_grad_k, dim3((n + kBlock - 1u) / kBlock), dim3(kBlock), 0, s, da, a, c, n, dz);
-    return hipGetLastError();
-}
-
+// 8 channels (16 bytes) per thread; c, strides and offsets are multiples of 8.
 // d: the fine (input) dims
 __global__ void __launch_bounds__(kBlock) pool2_k(Tens in, uint32_t c, Dims d, Tens out) {
-    const uint32_t ow = d.w / 2u, oh = d.h / 2u;
-    const uint32_t n = d.n * oh * ow * c;
+    const uint32_t ow = d.w / 2u, oh = d.h / 2u, c8 = c / 8u;
+    const uint32_t n = d.n * oh * ow * c8;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const uint32_t ch = i % c, op = i / c;
+    const uint32_t ch = (i % c8) * 8u, op = i / c8;
     const uint32_t x = op % ow, y = (op / ow) % oh, img = op / (ow * oh);
     const size_t p00 = (size_t(img) * d.h + 2u * y) * d.w + 2u * x;
     const bf16* q = in.p + in.off + ch;
-    const float s = f(q[p00 * in.stride]) + f(q[(p00 + 1u) * in.stride]) + f(q[(p00 + d.w) * in.stride]) +
-                    f(q[(p00 + d.w + 1u) * in.stride]);
-    out.p[size_t(op) * out.stride + out.off + ch] = b(0.25f * s);
+    const bf16x8 v0 = load8(q + p00 * in.stride), v1 = load8(q + (p00 + 1u) * in.stride);
+    const bf16x8 v2 = load8(q + (p00 + d.w) * in.stride), v3 = load8(q + (p00 + d.w + 1u) * in.stride);
+    bf16x8 o;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = b(0.25f * (f(v0[k]) + f(v1[k]) + f(v2[k]) + f(v3[k])));
+    store8(out.p + size_t(op) * out.stride + out.off + ch, o);
+}
+
+static bool views_ok(const Tens& a, const Tens& b2, uint32_t c) {
+    return c % 8u == 0 && a.stride % 8u == 0 && a.off % 8u == 0 && b2.stride % 8u == 0 && b2.off % 8u == 0;
 }
 
 hipError_t pool2(const Tens& in, uint32_t c, Dims d, const Tens& out, hipStream_t s) {
-    if (d.h % 2u || d.w % 2u) return hipErrorInvalidValue;
-    const uint32_t n = d.n * (d.h / 2u) * (d.w / 2u) * c;
+    if (d.h % 2u || d.w % 2u || !views_ok(in, out, c)) return hipErrorInvalidValue;
+    const uint32_t n = d.n * (d.h / 2u) * (d.w / 2u) * (c / 8u);
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(pool2_k, dim3((n + kBlock - 1u) / kBlock), dim3(kBlock), 0, s, in, c, d, out);
     return hipGetLastError();
@@ -314,18 +527,24 @@ hipError_t pool2(const Tens& in, uint32_t c, Dims d, const Tens& out, hipStream_
 
 // d: the fine dims; din (fine) += dout (coarse) / 4
 __global__ void __launch_bounds__(kBlock) pool2_back_k(Tens dout, uint32_t c, Dims d, Tens din) {
-    const uint32_t n = d.pixels() * c;
+    const uint32_t c8 = c / 8u;
+    const uint32_t n = d.pixels() * c8;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const uint32_t ch = i % c, p = i / c;
+    const uint32_t ch = (i % c8) * 8u, p = i / c8;
     const uint32_t x = p % d.w, y = (p / d.w) % d.h, img = p / (d.w * d.h);
     const size_t cp = (size_t(img) * (d.h / 2u) + y / 2u) * (d.w / 2u) + x / 2u;
     bf16* q = din.p + size_t(p) * din.stride + din.off + ch;
-    *q = b(f(*q) + 0.25f * f(dout.p[cp * dout.stride + dout.off + ch]));
+    const bf16x8 g = load8(dout.p + cp * dout.stride + dout.off + ch), v = load8(q);
+    bf16x8 o;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = b(f(v[k]) + 0.25f * f(g[k]));
+    store8(q, o);
 }
 
 hipError_t pool2_back_add(const Tens& dout, uint32_t c, Dims d, const Tens& din, hipStream_t s) {
-    const uint32_t n = d.pixels() * c;
+    if (!views_ok(dout, din, c)) return hipErrorInvalidValue;
+    const uint32_t n = d.pixels() * (c / 8u);
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(pool2_back_k, dim3((n + kBlock - 1u) / kBlock), dim3(kBlock), 0, s, dout, c, d, din);
     return hipGetLastError();
@@ -333,18 +552,19 @@ hipError_t pool2_back_add(const Tens& dout, uint32_t c, Dims d, const Tens& din,
 
 // d: the coarse (input) dims
 __global__ void __launch_bounds__(kBlock) up2_k(Tens in, uint32_t c, Dims d, Tens out) {
-    const uint32_t W = d.w * 2u, H = d.h * 2u;
-    const uint32_t n = d.n * H * W * c;
+    const uint32_t W = d.w * 2u, H = d.h * 2u, c8 = c / 8u;
+    const uint32_t n = d.n * H * W * c8;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const uint32_t ch = i % c, p = i / c;
+    const uint32_t ch = (i % c8) * 8u, p = i / c8;
     const uint32_t x = p % W, y = (p / W) % H, img = p / (W * H);
     const size_t cp = (size_t(img) * d.h + y / 2u) * d.w + x / 2u;
-    out.p[size_t(p) * out.stride + out.off + ch] = in.p[cp * in.stride + in.off + ch];
+    store8(out.p + size_t(p) * out.stride + out.off + ch, load8(in.p + cp * in.stride + in.off + ch));
 }
 
 hipError_t up2(const Tens& in, uint32_t c, Dims d, const Tens& out, hipStream_t s) {
-    const uint32_t n = d.n * d.h * d.w * 4u * c;
+    if (!views_ok(in, out, c)) return hipErrorInvalidValue;
+    const uint32_t n = d.n * d.h * d.w * 4u * (c / 8u);
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(up2_k, dim3((n + kBlock - 1u) / kBlock), dim3(kBlock), 0, s, in, c, d, out);
     return hipGetLastError();
@@ -352,21 +572,26 @@ hipError_t up2(const Tens& in, uint32_t c, Dims d, const Tens& out, hipStream_t 
 
 // d: the coarse (output) dims
 __global__ void __launch_bounds__(kBlock) up2_back_k(Tens dout, uint32_t c, Dims d, Tens din) {
-    const uint32_t n = d.pixels() * c;
+    const uint32_t c8 = c / 8u;
+    const uint32_t n = d.pixels() * c8;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const uint32_t ch = i % c, p = i / c;
+    const uint32_t ch = (i % c8) * 8u, p = i / c8;
     const uint32_t x = p % d.w, y = (p / d.w) % d.h, img = p / (d.w * d.h);
     const uint32_t W = d.w * 2u;
     const size_t f00 = (size_t(img) * d.h * 2u + 2u * y) * W + 2u * x;
     const bf16* q = dout.p + dout.off + ch;
-    const float s = f(q[f00 * dout.stride]) + f(q[(f00 + 1u) * dout.stride]) + f(q[(f00 + W) * dout.stride]) +
-                    f(q[(f00 + W + 1u) * dout.stride]);
-    din.p[size_t(p) * din.stride + din.off + ch] = b(s);
+    const bf16x8 v0 = load8(q + f00 * dout.stride), v1 = load8(q + (f00 + 1u) * dout.stride);
+    const bf16x8 v2 = load8(q + (f00 + W) * dout.stride), v3 = load8(q + (f00 + W + 1u) * dout.stride);
+    bf16x8 o;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = b(f(v0[k]) + f(v1[k]) + f(v2[k]) + f(v3[k]));
+    store8(din.p + size_t(p) * din.stride + din.off + ch, o);
 }
 
 hipError_t up2_back(const Tens& dout, uint32_t c, Dims d, const Tens& din, hipStream_t s) {
-    const uint32_t n = d.pixels() * c;
+    if (!views_ok(dout, din, c)) return hipErrorInvalidValue;
+    const uint32_t n = d.pixels() * (c / 8u);
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(up2_back_k, dim3((n + kBlock - 1u) / kBlock), dim3(kBlock), 0, s, dout, c, d, din);
     return hipGetLastError();
@@ -485,10 +710,11 @@ struct GTexel {  // pt_gbuffer_texel
 };
 
 __global__ void __launch_bounds__(kBlock) pack_frame_k(const float4* __restrict__ accum, const GTexel* __restrict__ g,
-                                                       uint32_t fw, uint32_t fh, Dims d, bf16* __restrict__ out) {
+                                                       uint32_t fw, uint32_t fh, uint32_t x0, uint32_t y0, Dims d,
+                                                       bf16* __restrict__ out) {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= d.pixels()) return;
-    const uint32_t x = p % d.w, y = p / d.w;
+    const uint32_t x = x0 + p % d.w, y = y0 + p / d.w;
     bf16* o = out + size_t(p) * 16u;
     if (x >= fw || y >= fh) {
 #pragma unroll
@@ -502,12 +728,31 @@ __global__ void __launch_bounds__(kBlock) pack_frame_k(const float4* __restrict_
                    t.dist);
 }
 
-hipError_t pack_frame(const float4* accum, const void* gbuf, uint32_t fw, uint32_t fh, Dims d, bf16* out,
-                      hipStream_t s) {
+hipError_t pack_frame(const float4* accum, const void* gbuf, uint32_t fw, uint32_t fh, uint32_t x0, uint32_t y0,
+                      Dims d, bf16* out, hipStream_t s) {
     const uint32_t n = d.pixels();
-    if (n == 0 || d.n != 1u || d.w < fw || d.h < fh) return hipErrorInvalidValue;
+    if (n == 0 || d.n != 1u) return hipErrorInvalidValue;
     hipLaunchKernelGGL(pack_frame_k, dim3((n + kBlock - 1u) / kBlock), dim3(kBlock), 0, s, accum,
-                       static_cast<const GTexel*>(gbuf), fw, fh, d, out);
+                       static_cast<const GTexel*>(gbuf), fw, fh, x0, y0, d, out);
+    return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(kBlock) crop_radiance_k(const float4* __restrict__ accum, uint32_t fw, uint32_t x0,
+                                                          uint32_t y0, Dims d, float* __restrict__ out) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= d.pixels()) return;
+    const float4 a = accum[size_t(y0 + p / d.w) * fw + x0 + p % d.w];
+    out[size_t(p) * 3u] = a.x;
+    out[size_t(p) * 3u + 1u] = a.y;
+    out[size_t(p) * 3u + 2u] = a.z;
+}
+
+hipError_t crop_radiance(const float4* accum, uint32_t fw, uint32_t fh, uint32_t x0, uint32_t y0, Dims d, float* out,
+                         hipStream_t s) {
+    const uint32_t n = d.pixels();
+    if (n == 0 || d.n != 1u || x0 + d.w > fw || y0 + d.h > fh) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(crop_radiance_k, dim3((n + kBlock - 1u) / kBlock), dim3(kBlock), 0, s, accum, fw, x0, y0, d,
+                       out);
     return hipGetLastError();
 }
 

@@ -62,7 +62,7 @@ struct UNet {
     Dims dims[5] = {};
     bool have_acts = false;
     DevBuf in0, tA[4], cat[4], pin[5], t4A, b4, dA[4], dd[4], pred;
-    DevBuf g_tA[4], g_cat[4], g_pin[5], g_t4A, g_b4, g_dA[4], g_dd[4], g_pred, dz;
+    DevBuf g_tA[4], g_cat[4], g_pin[5], g_t4A, g_b4, g_dA[4], g_dd[4], g_pred, dz, dzc, xs;
     DevBuf target, feat, loss, wpart, bpart;
     // dataset
     DevBuf ds_in, ds_tg;
@@ -70,7 +70,8 @@ struct UNet {
 
     std::vector<DevBuf*> all_bufs() {
         std::vector<DevBuf*> bufs{&params, &grads, &m, &v, &wbf, &wdt, &in0, &t4A, &b4, &pred, &g_t4A, &g_b4,
-                                  &g_pred, &dz, &target, &feat, &loss, &wpart, &bpart, &ds_in, &ds_tg};
+                                  &g_pred, &dz, &dzc, &xs, &target, &feat, &loss, &wpart, &bpart, &ds_in,
+                                  &ds_tg};
         for (int i = 0; i < 4; ++i) {
             for (DevBuf* b : {&tA[i], &cat[i], &dA[i], &dd[i], &g_tA[i], &g_cat[i], &g_dA[i], &g_dd[i]})
                 bufs.push_back(b);
@@ -209,18 +210,23 @@ int alloc_acts(pt_ctx* c, UNet& u, Dims d0) {
         (rc = A(u.pred, P(0) * 3, 4, "unet pred")) || (rc = A(u.g_pred, P(0) * kOutC, 2, "unet grad")) ||
         (rc = A(u.target, P(0) * 3, 4, "unet target")) || (rc = A(u.feat, P(0) * kFeat, 4, "unet features")))
         return rc;
-    // scratch: the largest relu-gradient tensor and gradient partial sums over all layers
-    size_t dz = 0, wp = 0, bp = 0;
+    // backward scratch, sized for the largest layer: dz (NHWC and channel-major),
+    // the staged layer input, and the partial sums of the weight and bias gradients
+    size_t dz = 0, xs = 0, wp = 0, bp = 0;
     const int lvl[kLayers] = {0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 3, 3, 2, 2, 1, 1, 0, 0, 0};
     for (uint32_t i = 0; i < kLayers; ++i) {
         const Layer& l = u.L[i];
-        dz = std::max(dz, P(lvl[i]) * l.cout);
-        for (uint32_t k = 0; k < slices(l); ++k)
-            wp = std::max(wp, wgrad_partial_floats(slice_count(l, k), l.cout, u.dims[lvl[i]]));
-        bp = std::max(bp, bias_partial_floats(l.cout));
+        const Dims d = u.dims[lvl[i]];
+        dz = std::max(dz, chw_elems(l.cout, d));
+        for (uint32_t k = 0; k < slices(l); ++k) {
+            xs = std::max(xs, x_copies(d) * chw_elems(slice_count(l, k), d));
+            wp = std::max(wp, wgrad_partial_floats(slice_count(l, k), l.cout, d));
+        }
+        bp = std::max(bp, dz_scratch_floats(l.cout, d));
     }
-    if ((rc = A(u.dz, dz, 2, "unet dz")) || (rc = A(u.wpart, wp, 4, "unet wgrad scratch")) ||
-        (rc = A(u.bpart, bp, 4, "unet bias scratch")) || (rc = A(u.loss, 1024, 4, "unet loss")))
+    if ((rc = A(u.dz, dz, 2, "unet dz")) || (rc = A(u.dzc, dz, 2, "unet dz")) || (rc = A(u.xs, xs, 2, "unet xs")) ||
+        (rc = A(u.wpart, wp, 4, "unet wgrad scratch")) || (rc = A(u.bpart, bp, 4, "unet bias scratch")) ||
+        (rc = A(u.loss, 1024, 4, "unet loss")))
         return rc;
     u.have_acts = true;
     return PT_OK;
@@ -267,24 +273,20 @@ int forward(pt_ctx* c, UNet& u) {
 int conv_back(pt_ctx* c, UNet& u, uint32_t li, const Tens& x, const Tens& a, const Tens& ga, const Tens& gx, Dims d) {
     const Layer& l = u.L[li];
     hipStream_t s = c->stream;
-    Tens dz = ga;
-    hipError_t e = hipSuccess;
-    if (l.relu) {
-        e = relu_grad(ga, a, l.cout, d, bfp(u.dz), s);
-        dz = T(u.dz, l.cout);
-    }
+    hipError_t e = dz_prepare(ga, l.relu ? &a : nullptr, l.cout, d, bfp(u.dz), bfp(u.dzc), fp(u.bpart),
+                              fp(u.grads) + l.b_off, s);
     size_t so = l.wbf_off;
     for (uint32_t k = 0; e == hipSuccess && k < slices(l); ++k) {
         const uint32_t c0 = slice_begin(k), nc = slice_count(l, k);
-        const Tens xs{x.p, x.stride, x.off + c0};
-        e = conv3x3_wgrad(dz, l.cout, xs, nc, d, fp(u.wpart), fp(u.grads) + l.w_off, c0, l.cin, s);
+        e = x_prepare(Tens{x.p, x.stride, x.off + c0}, nc, d, bfp(u.xs), s);
+        if (e == hipSuccess)
+            e = conv3x3_wgrad(bfp(u.dzc), l.cout, bfp(u.xs), nc, d, fp(u.wpart), fp(u.grads) + l.w_off, c0, l.cin, s);
         if (e == hipSuccess && gx.p) {
             const Tens gxs{gx.p, gx.stride, gx.off + c0};
-            e = conv3x3(dz, l.cout, bfp(u.wdt) + so, nullptr, gxs, nc, d, false, nullptr, 0, s);
+            e = conv3x3(T(u.dz, l.cout), l.cout, bfp(u.wdt) + so, nullptr, gxs, nc, d, false, nullptr, 0, s);
         }
         so += size_t(9) * nc * l.cout;
     }
-    if (e == hipSuccess) e = bias_grad(dz, l.cout, d, fp(u.bpart), fp(u.grads) + l.b_off, s);
     return hip_rc(c, e, "unet backward");
 }
 
@@ -473,6 +475,52 @@ int pt_unet_set_dataset(pt_ctx* c, const float* feat, const float* target, uint3
     return PT_OK;
 }
 
+int pt_unet_dataset_reserve(pt_ctx* c, uint32_t n, uint32_t h, uint32_t w) {
+    if (!c) return PT_ERR_ARG;
+    int rc = ensure_net(c);
+    if (rc) return rc;
+    if (n == 0 || h == 0 || w == 0 || h % 16u || w % 16u)
+        return c->fail(PT_ERR_ARG, "crops must be positive multiples of 16");
+    (void)hipSetDevice(c->device);
+    UNet& u = *c->unet;
+    const size_t P = size_t(n) * h * w;
+    u.ds_n = 0;
+    if ((rc = c->alloc(u.ds_in, P * kInC * 2, "unet dataset")) || (rc = c->alloc(u.ds_tg, P * 3 * 4, "unet dataset")))
+        return rc;
+    hipError_t e = fill_zero_bf16(bfp(u.ds_in), P * kInC, c->stream);
+    if (e == hipSuccess) e = fill_zero_f32(fp(u.ds_tg), P * 3, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "unet dataset");
+    u.ds_n = n;
+    u.ds_h = h;
+    u.ds_w = w;
+    return c->sync("unet dataset");
+}
+
+int pt_unet_dataset_capture(pt_ctx* c, uint32_t index, int what, uint32_t x0, uint32_t y0) {
+    if (!c) return PT_ERR_ARG;
+    int rc = ensure_net(c);
+    if (rc) return rc;
+    UNet& u = *c->unet;
+    if (u.ds_n == 0) return c->fail(PT_ERR_STATE, "no dataset (pt_unet_dataset_reserve)");
+    if (index >= u.ds_n) return c->fail(PT_ERR_ARG, "crop %u outside the dataset of %u", index, u.ds_n);
+    if (what != PT_UNET_INPUT && what != PT_UNET_TARGET) return c->fail(PT_ERR_ARG, "unknown capture kind %d", what);
+    if (c->width == 0 || !c->accum.p) return c->fail(PT_ERR_STATE, "no frame");
+    if (what == PT_UNET_INPUT && !c->gbuf.p) return c->fail(PT_ERR_STATE, "input crops need a G-buffer");
+    if (uint64_t(x0) + u.ds_w > c->width || uint64_t(y0) + u.ds_h > c->height)
+        return c->fail(PT_ERR_ARG, "crop window outside the %ux%u frame", c->width, c->height);
+    (void)hipSetDevice(c->device);
+    const Dims d{1u, u.ds_h, u.ds_w};
+    const size_t crop = size_t(u.ds_h) * u.ds_w;
+    const float4* acc = static_cast<const float4*>(c->accum.p);
+    const hipError_t e =
+        what == PT_UNET_INPUT
+            ? pack_frame(acc, c->gbuf.p, c->width, c->height, x0, y0, d, bfp(u.ds_in) + index * crop * kInC,
+                         c->stream)
+            : crop_radiance(acc, c->width, c->height, x0, y0, d, fp(u.ds_tg) + index * crop * 3, c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "unet capture");
+    return c->sync("unet capture");
+}
+
 int pt_unet_train(pt_ctx* c, uint32_t steps, uint32_t batch, float lr, uint64_t seed, float* losses) {
     if (!c) return PT_ERR_ARG;
     int rc = ensure_net(c);
@@ -527,7 +575,8 @@ int pt_unet_denoise(pt_ctx* c, float exposure) {
     const Dims d{1u, (fh + 15u) & ~15u, (fw + 15u) & ~15u};
     if ((rc = alloc_acts(c, u, d))) return rc;
     if ((rc = c->alloc(c->dn_fin, size_t(fw) * fh * 16, "denoise out"))) return rc;
-    hipError_t e = pack_frame(static_cast<const float4*>(c->accum.p), c->gbuf.p, fw, fh, d, bfp(u.in0), c->stream);
+    hipError_t e =
+        pack_frame(static_cast<const float4*>(c->accum.p), c->gbuf.p, fw, fh, 0u, 0u, d, bfp(u.in0), c->stream);
     if (e != hipSuccess) return c->hip_fail(e, "unet input");
     if ((rc = forward(c, u))) return rc;
     e = unpack_frame(fp(u.pred), d, fw, fh, exposure, static_cast<float4*>(c->dn_fin.p),

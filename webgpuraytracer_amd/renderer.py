@@ -248,6 +248,17 @@ class LearnedDenoiser:
         n, h, w, _ = feat.shape
         self.r._check(self.r.lib.pt_unet_set_dataset(self.r.ctx, feat.ctypes.data, target.ctypes.data, n, h, w))
 
+    def dataset_reserve(self, n, h, w):
+        """Device-side training set of n crops of h x w, filled by capture()."""
+        self.r._check(self.r.lib.pt_unet_dataset_reserve(self.r.ctx, n, h, w))
+
+    def capture(self, index, target, x0=0, y0=0):
+        """Copies the window at (x0, y0) of the renderer's current frame into
+        crop `index`: the network input (radiance + G-buffer) or, with
+        target=True, the reference radiance."""
+        what = A.PT_UNET_TARGET if target else A.PT_UNET_INPUT
+        self.r._check(self.r.lib.pt_unet_dataset_capture(self.r.ctx, index, what, x0, y0))
+
     def train(self, steps, batch, lr=1e-3, seed=0):
         losses = np.empty(steps, np.float32)
         self.r._check(self.r.lib.pt_unet_train(self.r.ctx, steps, batch, lr, seed, losses.ctypes.data))
