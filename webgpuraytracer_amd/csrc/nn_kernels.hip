@@ -33,104 +33,151 @@ __device__ __forceinline__ float f(bf16 x) { return float(x); }
 __device__ __forceinline__ bf16 b(float x) { return bf16(x); }
 
 // ------------------------------------------------------------- conv fwd --
-// A wave computes MT x 16 consecutive pixels by all NCT x 16 output channels:
-// per K step (one tap, 32 input channels) it loads MT input fragments and NCT
-// weight fragments and issues MT x NCT products.
+// A block computes a 16 x 16 pixel tile of one image for a group of up to
+// NCT x 16 output channels.  Per 32-channel slice of the input it stages the
+// 18 x 18 halo and the slice's weights in LDS, both stored by 8-channel
+// group ([group][pixel], [group][tap][co]) so the 16 lanes of a fragment row
+// read 256 consecutive bytes; the next slice is fetched into registers while
+// the current one is multiplied.  Wave w owns tile rows 4w .. 4w + 3 (one
+// 16-pixel MFMA row tile each), so per tap it reads 4 input fragments and NCT
+// weight fragments from LDS for 4 x NCT products.
+constexpr uint32_t kTile = 16, kHalo = 18, kHaloPix = kHalo * kHalo;  // 324
+constexpr uint32_t kSlice = 32;                                         // input channels per stage
+constexpr uint32_t kHaloPieces = kHaloPix * (kSlice / 8u);              // 16-byte pieces: 1296
+
 template <int NCT>
-struct ConvTiles {
-    static constexpr int MT = NCT <= 2 ? 8 : (NCT <= 4 ? 4 : 2);
+struct ConvSmem {
+    bf16x8 halo[4 * kHaloPix];    // [8-channel group][pixel]
+    bf16x8 w[4 * 9 * NCT * 16];   // [8-channel group][tap][co]
 };
 
-template <int NCT, int MT, bool RELU, bool F32OUT>
+template <int NCT, bool RELU, bool F32OUT>
 __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const bf16* __restrict__ w,
                                                     const float* __restrict__ bias, Tens out, uint32_t cout, Dims d,
                                                     float* __restrict__ of32, uint32_t f32c) {
-    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
-    const uint32_t npix = d.pixels();
-    const uint32_t p0 = (blockIdx.x * 4u + wv) * (16u * MT);
-    if (p0 >= npix) return;  // the whole wave is past the end
-    const uint32_t kq = lane >> 4;
-    uint32_t px[MT], py[MT], pimg[MT];
-    bool pv[MT];
+    __shared__ ConvSmem<NCT> sm;
+    constexpr uint32_t kWPieces = 9u * NCT * 16u * 4u;
+    constexpr uint32_t kHaloPer = (kHaloPieces + kBlock - 1u) / kBlock;  // 6
+    constexpr uint32_t kWPer = (kWPieces + kBlock - 1u) / kBlock;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6, kq = lane >> 4;
+    const uint32_t tiles_x = (d.w + kTile - 1u) / kTile, tiles_y = (d.h + kTile - 1u) / kTile;
+    const uint32_t img = blockIdx.x / (tiles_x * tiles_y), trem = blockIdx.x % (tiles_x * tiles_y);
+    const int ty0 = int(trem / tiles_x * kTile), tx0 = int(trem % tiles_x * kTile);
+    const uint32_t co0 = blockIdx.y * uint32_t(NCT) * 16u;
+    const uint32_t nct = min(uint32_t(NCT), (cout - co0) / 16u);
+    // the pieces this thread stages: halo pixel j, channels (piece % 4) * 8
+    const bf16* hsrc[kHaloPer];
+    bool hval[kHaloPer];
 #pragma unroll
-    for (int m = 0; m < MT; ++m) {
-        const uint32_t pix = p0 + uint32_t(m) * 16u + (lane & 15u);
-        pv[m] = pix < npix;
-        px[m] = pix % d.w;
-        py[m] = (pix / d.w) % d.h;
-        pimg[m] = pix / (d.w * d.h);
+    for (uint32_t k = 0; k < kHaloPer; ++k) {
+        const uint32_t piece = tid + k * kBlock;
+        const uint32_t j = piece / 4u;
+        const int gy = ty0 + int(j / kHalo) - 1, gx = tx0 + int(j % kHalo) - 1;
+        hval[k] = piece < kHaloPieces && gy >= 0 && gy < int(d.h) && gx >= 0 && gx < int(d.w);
+        const size_t sp = hval[k] ? (size_t(img) * d.h + uint32_t(gy)) * d.w + uint32_t(gx) : 0;
+        hsrc[k] = in.p + sp * in.stride + in.off + (piece % 4u) * 8u;
     }
-    f32x4 acc[MT][NCT];
+    bf16x8 hreg[kHaloPer], wreg[kWPer];
+    auto fetch = [&](uint32_t c0) {
 #pragma unroll
-    for (int m = 0; m < MT; ++m)
+        for (uint32_t k = 0; k < kHaloPer; ++k) {
+            const uint32_t c = c0 + ((tid + k * kBlock) % 4u) * 8u;
+            hreg[k] = (hval[k] && c < cin) ? load8(hsrc[k] + c0) : zero8();
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kWPer; ++k) {
+            const uint32_t piece = tid + k * kBlock;
+            const uint32_t c8 = piece % 4u, co = (piece / 4u) % (NCT * 16u), tap = piece / (4u * NCT * 16u);
+            const uint32_t c = c0 + c8 * 8u;
+            const bool ok = piece < kWPieces && co0 + co < cout && c < cin;
+            wreg[k] = ok ? load8(w + (size_t(tap) * cout + co0 + co) * cin + c) : zero8();
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (uint32_t k = 0; k < kHaloPer; ++k) {
+            const uint32_t piece = tid + k * kBlock;
+            if (piece < kHaloPieces) sm.halo[(piece % 4u) * kHaloPix + piece / 4u] = hreg[k];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kWPer; ++k) {
+            const uint32_t piece = tid + k * kBlock;
+            if (piece < kWPieces) sm.w[(piece % 4u) * (kWPieces / 4u) + piece / 4u] = wreg[k];
+        }
+    };
+    f32x4 acc[4][NCT];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
 #pragma unroll
         for (int t = 0; t < NCT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (uint32_t tap = 0; tap < 9u; ++tap) {
-        const int dy = int(tap / 3u) - 1, dx = int(tap % 3u) - 1;
-        const bf16* src[MT];
-        bool valid[MT];
+    fetch(0);
+    for (uint32_t c0 = 0; c0 < cin; c0 += kSlice) {
+        __syncthreads();  // the previous slice's reads are done
+        stash();
+        __syncthreads();
+        if (c0 + kSlice < cin) fetch(c0 + kSlice);
 #pragma unroll
-        for (int m = 0; m < MT; ++m) {
-            const int sy = int(py[m]) + dy, sx = int(px[m]) + dx;
-            valid[m] = pv[m] && sy >= 0 && sy < int(d.h) && sx >= 0 && sx < int(d.w);
-            const size_t sp = valid[m] ? (size_t(pimg[m]) * d.h + uint32_t(sy)) * d.w + uint32_t(sx) : 0;
-            src[m] = in.p + sp * in.stride + in.off;
-        }
-        const bf16* wt = w + size_t(tap) * cout * cin;
-        for (uint32_t c0 = 0; c0 < cin; c0 += 32u) {
-            const uint32_t c = c0 + kq * 8u;
-            const bool cv = c < cin;
-            bf16x8 a[MT], bw[NCT];
+        for (uint32_t tap = 0; tap < 9u; ++tap) {
+            const uint32_t dy = tap / 3u, dx = tap % 3u;
+            bf16x8 a[4], bw[NCT];
 #pragma unroll
-            for (int m = 0; m < MT; ++m) a[m] = (valid[m] && cv) ? load8(src[m] + c) : zero8();
-#pragma unroll
-            for (int t = 0; t < NCT; ++t) {
-                const uint32_t co = uint32_t(t) * 16u + (lane & 15u);
-                bw[t] = cv ? load8(wt + size_t(co) * cin + c) : zero8();
+            for (int m = 0; m < 4; ++m) {
+                const uint32_t hp = (wv * 4u + uint32_t(m) + dy) * kHalo + (lane & 15u) + dx;
+                a[m] = sm.halo[kq * kHaloPix + hp];
             }
 #pragma unroll
-            for (int m = 0; m < MT; ++m)
+            for (int t = 0; t < NCT; ++t)
+                bw[t] = sm.w[kq * (kWPieces / 4u) + tap * NCT * 16u + uint32_t(t) * 16u + (lane & 15u)];
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
 #pragma unroll
                 for (int t = 0; t < NCT; ++t)
-                    acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], bw[t], acc[m][t], 0, 0, 0);
+                    if (uint32_t(t) < nct)
+                        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], bw[t], acc[m][t], 0, 0, 0);
         }
     }
+    // C[(lane / 16) * 4 + r][lane % 16]: tile column kq * 4 + r, channel lane % 16
 #pragma unroll
     for (int t = 0; t < NCT; ++t) {
-        const uint32_t co = uint32_t(t) * 16u + (lane & 15u);
+        if (uint32_t(t) >= nct) continue;
+        const uint32_t co = co0 + uint32_t(t) * 16u + (lane & 15u);
         const float bv = bias ? bias[co] : 0.0f;
 #pragma unroll
-        for (int m = 0; m < MT; ++m)
+        for (int m = 0; m < 4; ++m) {
+            const int y = ty0 + int(wv * 4u) + m;
+            if (y >= int(d.h)) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const uint32_t op = p0 + uint32_t(m) * 16u + kq * 4u + uint32_t(r);
-                if (op >= npix) continue;
+                const int x = tx0 + int(kq * 4u) + r;
+                if (x >= int(d.w)) continue;
+                const size_t op = (size_t(img) * d.h + uint32_t(y)) * d.w + uint32_t(x);
                 float v = acc[m][t][r] + bv;
                 if (RELU) v = fmaxf(v, 0.0f);
                 if (F32OUT) {
-                    if (co < f32c) of32[size_t(op) * f32c + co] = v;
+                    if (co < f32c) of32[op * f32c + co] = v;
                 } else {
-                    out.p[size_t(op) * out.stride + out.off + co] = b(v);
+                    out.p[op * out.stride + out.off + co] = b(v);
                 }
             }
+        }
     }
 }
 
 template <int NCT>
 static void conv_launch(const Tens& in, uint32_t cin, const bf16* w, const float* bias, const Tens& out,
                         uint32_t cout, Dims d, bool relu, float* of32, uint32_t f32c, hipStream_t s) {
-    constexpr int MT = ConvTiles<NCT>::MT;
-    const uint32_t per_block = 4u * 16u * MT;
-    const uint32_t blocks = (d.pixels() + per_block - 1u) / per_block;
+    const uint32_t tiles = d.n * ((d.h + kTile - 1u) / kTile) * ((d.w + kTile - 1u) / kTile);
+    const uint32_t groups = (cout / 16u + NCT - 1u) / NCT;
+    const dim3 grid(tiles, groups);
     if (of32)
-        hipLaunchKernelGGL((conv3x3_k<NCT, MT, false, true>), dim3(blocks), dim3(kBlock), 0, s, in, cin, w, bias, out,
-                           cout, d, of32, f32c);
+        hipLaunchKernelGGL((conv3x3_k<NCT, false, true>), grid, dim3(kBlock), 0, s, in, cin, w, bias, out, cout, d,
+                           of32, f32c);
     else if (relu)
-        hipLaunchKernelGGL((conv3x3_k<NCT, MT, true, false>), dim3(blocks), dim3(kBlock), 0, s, in, cin, w, bias, out,
-                           cout, d, of32, f32c);
+        hipLaunchKernelGGL((conv3x3_k<NCT, true, false>), grid, dim3(kBlock), 0, s, in, cin, w, bias, out, cout, d,
+                           of32, f32c);
     else
-        hipLaunchKernelGGL((conv3x3_k<NCT, MT, false, false>), dim3(blocks), dim3(kBlock), 0, s, in, cin, w, bias,
-                           out, cout, d, of32, f32c);
+        hipLaunchKernelGGL((conv3x3_k<NCT, false, false>), grid, dim3(kBlock), 0, s, in, cin, w, bias, out, cout, d,
+                           of32, f32c);
 }
 
 hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bias, const Tens& out, uint32_t cout,
@@ -138,16 +185,14 @@ hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bia
     if (cin == 0 || cin % 16u || cout % 16u || cout == 0 || cout > kMaxCout || in.stride % 8u || in.off % 8u)
         return hipErrorInvalidValue;
     if (d.pixels() == 0) return hipSuccess;
-    switch (cout / 16u) {
+    // output-channel tiles per block: all of them up to 4, else groups of 3 or 4
+    const uint32_t ct = cout / 16u;
+    const uint32_t nct = ct <= 4u ? ct : (ct % 4u == 0 ? 4u : (ct % 3u == 0 ? 3u : 4u));
+    switch (nct) {
         case 1: conv_launch<1>(in, cin, w, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
         case 2: conv_launch<2>(in, cin, w, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
         case 3: conv_launch<3>(in, cin, w, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
-        case 4: conv_launch<4>(in, cin, w, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
-        case 5: conv_launch<5>(in, cin, w, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
-        case 6: conv_launch<6>(in, cin, w, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
-        case 7: conv_launch<7>(in, cin, w, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
-        case 8: conv_launch<8>(in, cin, w, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
-        default: return hipErrorInvalidValue;
+        default: conv_launch<4>(in, cin, w, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
     }
     return hipGetLastError();
 }
@@ -295,10 +340,17 @@ __global__ void __launch_bounds__(kBlock) reduce_fin_k(const float* __restrict__
                                                        float* __restrict__ out) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    float s = 0.0f;
-    for (uint32_t k = 0; k < segs; ++k) s += tmp[size_t(k) * n + i];
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    uint32_t k = 0;
+    for (; k + 4u <= segs; k += 4u) {
+        s0 += tmp[size_t(k) * n + i];
+        s1 += tmp[size_t(k + 1u) * n + i];
+        s2 += tmp[size_t(k + 2u) * n + i];
+        s3 += tmp[size_t(k + 3u) * n + i];
+    }
+    for (; k < segs; ++k) s0 += tmp[size_t(k) * n + i];
     const uint32_t row = i / cols, col = i - row * cols;
-    out[size_t(row) * out_cols + col_off + col] = s;
+    out[size_t(row) * out_cols + col_off + col] = (s0 + s1) + (s2 + s3);
 }
 
 static uint32_t red_segs(uint32_t chunks) { return (chunks + kRedSeg - 1u) / kRedSeg; }
@@ -384,12 +436,11 @@ __global__ void __launch_bounds__(kBlock) wgrad_k(const bf16* __restrict__ dzc, 
     const bf16* arow[MTX];
 #pragma unroll
     for (int m = 0; m < MTX; ++m) arow[m] = dzc + size_t(uint32_t(m) * 16u + (lane & 15u)) * Ps;
-    for (uint32_t p = pb; p < pe; p += kWgStep) {
-        // this lane's 8 pixels (zero past P); with three copies they share one
-        // image row, because w is a multiple of 8
+    // fragments of one K step: this lane's 8 pixels (zero past P); with three
+    // copies they share one image row, because w is a multiple of 8
+    auto load_step = [&](uint32_t p, bf16x8(&a)[MTX], bf16x8(&bx)[T][NTX]) {
         const uint32_t pk = p + kq * 8u;
         const int row = int((pk / d.w) % d.h);
-        bf16x8 a[MTX];
 #pragma unroll
         for (int m = 0; m < MTX; ++m) a[m] = uint32_t(m) < mt ? load8(arow[m] + pk) : zero8();
 #pragma unroll
@@ -407,15 +458,32 @@ __global__ void __launch_bounds__(kBlock) wgrad_k(const bf16* __restrict__ dzc, 
                 xrow = xs + size_t((tap % 3u) * cin + ci0 + (lane & 15u)) * Ps + src;
             }
 #pragma unroll
-            for (int n = 0; n < NTX; ++n) {
-                if (uint32_t(n) >= nt) continue;
-                const bf16x8 bx = rv ? load8(xrow + size_t(n) * 16u * Ps) : zero8();
+            for (int n = 0; n < NTX; ++n) bx[t][n] = (rv && uint32_t(n) < nt) ? load8(xrow + size_t(n) * 16u * Ps) : zero8();
+        }
+    };
+    auto mma = [&](const bf16x8(&a)[MTX], const bf16x8(&bx)[T][NTX]) {
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+#pragma unroll
+            for (int n = 0; n < NTX; ++n)
 #pragma unroll
                 for (int m = 0; m < MTX; ++m)
-                    if (uint32_t(m) < mt)
-                        acc[t][n][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], bx, acc[t][n][m], 0, 0, 0);
-            }
-        }
+                    if (uint32_t(n) < nt && uint32_t(m) < mt)
+                        acc[t][n][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], bx[t][n], acc[t][n][m], 0, 0, 0);
+    };
+    // two register sets: the next step's loads are in flight during this step's products
+    bf16x8 a0[MTX], b0[T][NTX], a1[MTX], b1[T][NTX];
+    uint32_t p = pb;
+    if (p < pe) load_step(p, a0, b0);
+    while (p < pe) {
+        const uint32_t pn = p + kWgStep;
+        if (pn < pe) load_step(pn, a1, b1);
+        mma(a0, b0);
+        if (pn >= pe) break;
+        const uint32_t pn2 = pn + kWgStep;
+        if (pn2 < pe) load_step(pn2, a0, b0);
+        mma(a1, b1);
+        p = pn2;
     }
     float* out = partial + size_t(blockIdx.x) * 9u * cout * cin;
 #pragma unroll
@@ -437,7 +505,7 @@ __global__ void __launch_bounds__(kBlock) wgrad_k(const bf16* __restrict__ dzc, 
 }
 
 struct WgPlan {
-    int kind;        // 0: <2, 4, 3>, 1: <4, 4, 1>, 2: <8, 2, 1>
+    int kind;        // 0: <2, 2, 3>, 1: <4, 4, 1>, 2: <8, 2, 1>
     uint32_t T, NT;  // taps per job, ci tiles per job
     uint32_t jobs, ci_groups, chunks, chunk;
 };
@@ -445,7 +513,7 @@ struct WgPlan {
 static WgPlan wgrad_plan(uint32_t cin, uint32_t cout, Dims d) {
     WgPlan p{};
     const uint32_t mt = cout / 16u;
-    if (mt <= 2u) { p.kind = 0; p.T = 3; p.NT = 4; }
+    if (mt <= 2u) { p.kind = 0; p.T = 3; p.NT = 2; }
     else if (mt <= 4u) { p.kind = 1; p.T = 1; p.NT = 4; }
     else { p.kind = 2; p.T = 1; p.NT = 2; }
     const uint32_t it = cin / 16u;
@@ -453,8 +521,8 @@ static WgPlan wgrad_plan(uint32_t cin, uint32_t cout, Dims d) {
     p.jobs = (9u / p.T) * p.ci_groups;
     const uint32_t Ps = chw_stride_of(d.pixels());
     const uint32_t steps = Ps / kWgStep;
-    // about 4096 waves in flight, at least 16 K steps per wave
-    uint32_t chunks = 4096u / p.jobs;
+    // about 2048 waves, at least 16 K steps per wave
+    uint32_t chunks = 2048u / p.jobs;
     chunks = std::max(1u, std::min(chunks, steps / 16u));
     const uint32_t per = (steps + chunks - 1u) / chunks;
     p.chunk = per * kWgStep;
@@ -484,7 +552,7 @@ hipError_t conv3x3_wgrad(const bf16* dzc, uint32_t cout, const bf16* xs, uint32_
         hipLaunchKernelGGL((wgrad_k<MT_, NT_, T_, false>), grid, dim3(kBlock), 0, s, dzc, cout, xs, cin, d,        \
                            p.chunk, p.ci_groups, p.jobs, partial);
     switch (p.kind) {
-        case 0: PT_WGRAD_LAUNCH(2, 4, 3) break;
+        case 0: PT_WGRAD_LAUNCH(2, 2, 3) break;
         case 1: PT_WGRAD_LAUNCH(4, 4, 1) break;
         default: PT_WGRAD_LAUNCH(8, 2, 1) break;
     }

@@ -97,7 +97,8 @@ def _log(msg):
 
 def build_training_set(r, net, scenes, variants, size, frame, target_spp, input_spp, seed0):
     """Renders scenes x variants crops (size x size, at seeded positions in a
-    frame[0] x frame[1] render) into the device training set; returns the
+    frame[0] x frame[1] render) into the device training set; variant v's
+    input has input_spp[v % len(input_spp)] samples per pixel.  Returns the
     render time in seconds."""
     fw, fh = frame
     if fw < size or fh < size:
@@ -118,7 +119,7 @@ def build_training_set(r, net, scenes, variants, size, frame, target_spp, input_
             net.capture(s * variants + v, target=True, x0=x0, y0=y0)
         for v, (x0, y0) in enumerate(pos):
             r.reset()
-            r.render(spp=input_spp, base_seed=(s * variants + v) * 7919 + 13)
+            r.render(spp=input_spp[v % len(input_spp)], base_seed=(s * variants + v) * 7919 + 13)
             net.capture(s * variants + v, target=False, x0=x0, y0=y0)
         if (s + 1) % 16 == 0 or s + 1 == scenes:
             _log("  scenes %d/%d rendered (%.1f s)" % (s + 1, scenes, time.perf_counter() - t0))
@@ -129,7 +130,7 @@ def cmd_train(a):
     r = Renderer(0)
     r.set_option(A.PT_OPT_COUNT_STATS, 0)
     net = LearnedDenoiser(r, seed=a.seed)
-    _log("rendering %d scenes x %d crops of %d^2 from %dx%d frames (targets %d spp, inputs %d spp)"
+    _log("rendering %d scenes x %d crops of %d^2 from %dx%d frames (targets %d spp, inputs %s spp)"
          % (a.scenes, a.variants, a.size, a.frame[0], a.frame[1], a.target_spp, a.input_spp))
     t_data = build_training_set(r, net, a.scenes, a.variants, a.size, a.frame, a.target_spp, a.input_spp,
                                 a.scene_seed)
@@ -264,7 +265,7 @@ def main(argv=None):
     t.add_argument("--size", type=int, default=256)
     t.add_argument("--frame", type=lambda s: [int(x) for x in s.split("x")], default=[256, 256])
     t.add_argument("--target-spp", type=int, default=2048)
-    t.add_argument("--input-spp", type=int, default=4)
+    t.add_argument("--input-spp", type=lambda s: [int(x) for x in s.split(",")], default=[1, 2, 4, 8])
     t.add_argument("--steps", type=int, default=3000)
     t.add_argument("--batch", type=int, default=16)
     t.add_argument("--lr", type=float, default=1e-3)
