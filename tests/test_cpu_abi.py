@@ -19,7 +19,7 @@ def test_ctypes_layouts_match_header_sizes():
 
 def test_header_static_asserts_hold_in_c_and_cpp(tmp_path):
     src = tmp_path / "t.c"
-    src.write_text('#include "pt_abi.h"\n#include "pt_scene.h"\n#include "pt_unet.h"\nint main(void){return 0;}\n')
+    src.write_text('#include "pt_abi.h"\n#include "pt_scene.h"\n#include "pt_unet.h"\n#include "pt_batch.h"\nint main(void){return 0;}\n')
     subprocess.run(["gcc", "-std=c11", "-fsyntax-only", "-I" + INC, str(src)], check=True)
     subprocess.run(["g++", "-std=c++17", "-x", "c++", "-fsyntax-only", "-I" + INC, str(src)], check=True)
 
@@ -44,10 +44,12 @@ def test_libpt_exports_every_declared_symbol():
     A.load_pt()  # every binding resolves
 
 
-def test_libpt_exports_the_learned_denoiser():
+@pytest.mark.parametrize("header,must", [("pt_unet.h", ["pt_unet_train", "pt_unet_denoise"]),
+                                         ("pt_batch.h", ["pt_batch_frame", "pt_batch_add"])])
+def test_libpt_exports_the_other_headers(header, must):
     lib = os.path.join(A.LIB_DIR, "libpt.so")
-    names = _declared("pt_unet.h", "pt_")
-    assert "pt_unet_train" in names and "pt_unet_denoise" in names
+    names = _declared(header, "pt_")
+    assert all(m in names for m in must)
     missing = [n for n in names if n not in _exported(lib)]
     assert not missing, missing
 
@@ -61,7 +63,8 @@ def test_libptscene_exports_every_declared_symbol():
 
 
 def test_ctypes_table_covers_the_headers():
-    assert set(_declared("pt_abi.h", "pt_")) | set(_declared("pt_unet.h", "pt_")) == set(A.PT_FUNCS)
+    declared = set(_declared("pt_abi.h", "pt_")) | set(_declared("pt_unet.h", "pt_")) | set(_declared("pt_batch.h", "pt_"))
+    assert declared == set(A.PT_FUNCS)
     assert set(_declared("pt_scene.h", "ptgen_")) == set(A.PTGEN_FUNCS)
 
 
@@ -76,6 +79,10 @@ def test_null_arguments_are_errors_not_crashes():
     assert lib.pt_unet_init(None, 0) == A.PT_ERR_ARG
     assert lib.pt_unet_train(None, 1, 1, 1e-3, 0, None) == A.PT_ERR_ARG
     assert lib.pt_unet_denoise(None, 1.0) == A.PT_ERR_ARG
+    assert lib.pt_batch_create(None, 0) == A.PT_ERR_ARG
+    assert lib.pt_batch_add(None, None, None) == A.PT_ERR_ARG
+    assert lib.pt_batch_frame(None, None, None, None, None) == A.PT_ERR_ARG
+    lib.pt_batch_destroy(None)
     lib.pt_destroy(None)
 
 

@@ -180,6 +180,14 @@ PT_FUNCS = {
     "pt_unet_dataset_capture": (C.c_int, [vp, u32, C.c_int, u32, u32]),
     "pt_unet_train": (C.c_int, [vp, u32, u32, f32, u64, vp]),
     "pt_unet_denoise": (C.c_int, [vp, f32]),
+    # many jobs per graph launch (pt_batch.h)
+    "pt_batch_create": (C.c_int, [P(vp), C.c_int]),
+    "pt_batch_destroy": (None, [vp]),
+    "pt_batch_last_error": (C.c_char_p, [vp]),
+    "pt_batch_add": (C.c_int, [vp, vp, P(u32)]),
+    "pt_batch_size": (C.c_int, [vp, P(u32)]),
+    "pt_batch_frame": (C.c_int, [vp, vp, P(u32), vp, vp]),
+    "pt_batch_graph_info": (C.c_int, [vp, P(u32), P(u32), P(u32)]),
 }
 
 PTGEN_FUNCS = {

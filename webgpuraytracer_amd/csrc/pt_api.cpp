@@ -367,7 +367,9 @@ static int check_params(pt_ctx* c, const pt_scene_params* p) {
 
 // Wavefront backend: the call's passes in batches of whole passes, each
 // batch one raygen + bounce_limit x (extend, shade, shadow) + accumulate.
-static int run_wavefront(pt_ctx* c, const TraceArgs& a) {
+// Allocates the path pool and fills plan.w; launches nothing.
+static int prepare_wavefront(pt_ctx* c, pt::TracePlan& plan) {
+    const TraceArgs& a = plan.a;
     const uint64_t npix = uint64_t(a.sec_w) * a.sec_h;
     uint64_t per = std::max<uint64_t>(1, c->wf_pool_opt / std::max<uint64_t>(npix, 1));
     per = std::min<uint64_t>(per, a.n_passes);
@@ -389,7 +391,8 @@ static int run_wavefront(pt_ctx* c, const TraceArgs& a) {
         if ((rc = c->alloc(c->wf_cnt, 64, "wf counters"))) return rc;
         c->wf_pool = P;
     }
-    WfArgs w{};
+    WfArgs& w = plan.w;
+    w = WfArgs{};
     w.P = a.P;
     w.sec_w = a.sec_w;
     w.sec_h = a.sec_h;
@@ -423,31 +426,29 @@ static int run_wavefront(pt_ctx* c, const TraceArgs& a) {
     hipError_t e = wf_trace_blocks_per_cu(c->dev, c->count_stats, a.wide, w.stack_lds, &bpc);
     if (e != hipSuccess) return c->hip_fail(e, "wavefront occupancy");
     if (bpc <= 0) return c->fail(PT_ERR_CAPACITY, "wavefront traversal kernel cannot be resident");
-    w.trace_grid = uint32_t(bpc) * uint32_t(c->n_cu);
+    // the resident grid, or fewer workgroups when a batch has fewer paths than lanes
+    const uint64_t resident = uint64_t(bpc) * uint64_t(c->n_cu);
+    w.trace_grid = uint32_t(std::min<uint64_t>(resident, (pool + 255u) / 256u));
     w.spill_stride = w.trace_grid * 256u;
     if ((rc = c->alloc(c->wf_spill, size_t(w.spill_stride) * (total - w.stack_lds) * 4u, "wf stack spill"))) return rc;
     w.stack_spill = static_cast<uint32_t*>(c->wf_spill.p);
     w.refill = c->wf_refill;
-    e = hipEventRecord(c->ev0, c->stream);
-    for (uint32_t first = 0; e == hipSuccess && first < a.n_passes; first += uint32_t(per)) {
-        w.first_pass = first;
-        w.batch_passes = uint32_t(std::min<uint64_t>(per, a.n_passes - first));
-        w.n_items = uint32_t(w.batch_passes * npix);
-        w.write_output = first + w.batch_passes >= a.n_passes ? 1 : 0;
-        e = launch_wavefront_batch(c->dev, w, c->count_stats, c->stream);
-    }
-    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
-    if (e != hipSuccess) return c->hip_fail(e, "wavefront launch");
+    plan.per = per;
     return PT_OK;
 }
 
-static int run_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t sh, uint32_t n_passes,
-                     const uint32_t* seeds, pt_stats* stats, pt_hit* hits_dev, uint32_t n_rec) {
+}  // extern "C"
+
+namespace pt {
+
+int check_trace_params(pt_ctx* c, const pt_scene_params* p) { return check_params(c, p); }
+
+int prepare_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t sh, uint32_t n_passes,
+                  const uint32_t* seeds, pt_hit* hits_dev, uint32_t n_rec, TracePlan& plan) {
     int rc;
     if ((rc = c->alloc(c->stats, kStCount * 8, "stats"))) return rc;
-    hipError_t e = launch_zero_u64(static_cast<unsigned long long*>(c->stats.p), kStCount, c->stream);
-    if (e != hipSuccess) return c->hip_fail(e, "zero stats");
-    TraceArgs a{};
+    plan = TracePlan{};
+    TraceArgs& a = plan.a;
     a.P = *p;
     const uint32_t sx = p->section[0], sy = p->section[1];
     if (sx >= p->width || sy >= p->height) return c->fail(PT_ERR_ARG, "section origin outside the frame");
@@ -466,51 +467,86 @@ static int run_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t 
     a.n_rec = n_rec;
     a.schedule = c->schedule;
     a.wide = c->traversal == PT_TRAVERSAL_WIDE;
-    if (c->backend == PT_BACKEND_WAVEFRONT && !hits_dev) {
-        if ((rc = run_wavefront(c, a))) return rc;
+    plan.wavefront = c->backend == PT_BACKEND_WAVEFRONT && !hits_dev;
+    if (plan.wavefront) return prepare_wavefront(c, plan);
+    plan.cn = c->count_stats && !hits_dev;
+    a.n_tiles = ((a.sec_w + 7u) / 8u) * ((a.sec_h + 7u) / 8u);
+    // the kernel variant: wide unless counting or recording (both use the reference order)
+    const uint32_t total = (a.wide && !plan.cn && !hits_dev) ? c->wstack_cap : c->stack_cap;
+    a.stack_lds = c->lds_stack_opt ? std::min<uint32_t>(total, c->lds_stack_opt) : total;
+    int bpc = 0;
+    hipError_t e = trace_blocks_per_cu(c->dev, a, plan.cn, &bpc);
+    if (e != hipSuccess) return c->hip_fail(e, "trace occupancy");
+    if (bpc <= 0)
+        return c->fail(PT_ERR_CAPACITY, "trace kernel cannot be resident (LDS stack of %u entries)", c->stack_cap);
+    plan.grid = uint32_t(std::min<uint64_t>(a.n_tiles, uint64_t(bpc) * uint64_t(c->n_cu)));
+    a.media_stride = plan.grid * 64u;
+    if ((rc = c->alloc(c->mk_media, size_t(a.media_stride) * kLaneMediaFloats * 4u, "medium stacks"))) return rc;
+    if ((rc = c->alloc(c->mk_tiles, 8 * 4, "tile counters"))) return rc;
+    if ((rc = c->alloc(c->mk_spill, size_t(a.media_stride) * (total - a.stack_lds) * 4u, "stack spill"))) return rc;
+    a.stack_spill = static_cast<uint32_t*>(c->mk_spill.p);
+    a.media = static_cast<float*>(c->mk_media.p);
+    a.tile_ctr = static_cast<uint32_t*>(c->mk_tiles.p);
+    return PT_OK;
+}
+
+hipError_t enqueue_trace(pt_ctx* c, TracePlan& plan, hipStream_t s, bool timed) {
+    hipError_t e = launch_zero_u64(plan.a.stats, kStCount, s);
+    if (plan.wavefront) {
+        WfArgs w = plan.w;
+        const TraceArgs& a = plan.a;
+        const uint64_t npix = uint64_t(a.sec_w) * a.sec_h;
+        if (e == hipSuccess && timed) e = hipEventRecord(c->ev0, s);
+        for (uint32_t first = 0; e == hipSuccess && first < a.n_passes; first += uint32_t(plan.per)) {
+            w.first_pass = first;
+            w.batch_passes = uint32_t(std::min<uint64_t>(plan.per, a.n_passes - first));
+            w.n_items = uint32_t(w.batch_passes * npix);
+            w.write_output = first + w.batch_passes >= a.n_passes ? 1 : 0;
+            e = launch_wavefront_batch(c->dev, w, c->count_stats, s);
+        }
     } else {
-        const bool cn = c->count_stats && !hits_dev;
-        a.n_tiles = ((a.sec_w + 7u) / 8u) * ((a.sec_h + 7u) / 8u);
-        // the kernel variant: wide unless counting or recording (both use the reference order)
-        const uint32_t total = (a.wide && !cn && !hits_dev) ? c->wstack_cap : c->stack_cap;
-        a.stack_lds = c->lds_stack_opt ? std::min<uint32_t>(total, c->lds_stack_opt) : total;
-        int bpc = 0;
-        e = trace_blocks_per_cu(c->dev, a, cn, &bpc);
-        if (e != hipSuccess) return c->hip_fail(e, "trace occupancy");
-        if (bpc <= 0) return c->fail(PT_ERR_CAPACITY, "trace kernel cannot be resident (LDS stack of %u entries)",
-                                     c->stack_cap);
-        const uint32_t grid = uint32_t(std::min<uint64_t>(a.n_tiles, uint64_t(bpc) * uint64_t(c->n_cu)));
-        a.media_stride = grid * 64u;
-        if ((rc = c->alloc(c->mk_media, size_t(a.media_stride) * kLaneMediaFloats * 4u, "medium stacks"))) return rc;
-        if ((rc = c->alloc(c->mk_tiles, 8 * 4, "tile counters"))) return rc;
-        if ((rc = c->alloc(c->mk_spill, size_t(a.media_stride) * (total - a.stack_lds) * 4u, "stack spill"))) return rc;
-        a.stack_spill = static_cast<uint32_t*>(c->mk_spill.p);
-        a.media = static_cast<float*>(c->mk_media.p);
-        a.tile_ctr = static_cast<uint32_t*>(c->mk_tiles.p);
-        e = launch_zero_u64(static_cast<unsigned long long*>(c->mk_tiles.p), 4, c->stream);
-        if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
-        if (e == hipSuccess) e = launch_trace(c->dev, a, grid, cn, c->stream);
-        if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
-        if (e != hipSuccess) return c->hip_fail(e, "trace launch");
+        if (e == hipSuccess) e = launch_zero_u64(reinterpret_cast<unsigned long long*>(plan.a.tile_ctr), 4, s);
+        if (e == hipSuccess && timed) e = hipEventRecord(c->ev0, s);
+        if (e == hipSuccess) e = launch_trace(c->dev, plan.a, plan.grid, plan.cn, s);
     }
+    if (e == hipSuccess && timed) e = hipEventRecord(c->ev1, s);
+    return e;
+}
+
+int read_stats(pt_ctx* c, pt_stats* stats, float ms, uint32_t launches) {
+    unsigned long long v[kStCount];
+    const hipError_t e = hipMemcpy(v, c->stats.p, sizeof(v), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return c->hip_fail(e, "stats readback");
+    std::memset(stats, 0, sizeof(*stats));
+    stats->extension_rays = v[kStExt];
+    stats->shadow_rays = v[kStShadow];
+    stats->paths = v[kStPaths];
+    stats->node_pops = v[kStPops];
+    stats->inner_pops = v[kStInner];
+    stats->object_visits = v[kStObjs];
+    stats->tri_tests = v[kStTris];
+    stats->shade_events = v[kStShade];
+    stats->kernel_ms = ms;
+    stats->launches = launches;
+    return PT_OK;
+}
+
+}  // namespace pt
+
+extern "C" {
+
+static int run_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t sh, uint32_t n_passes,
+                     const uint32_t* seeds, pt_stats* stats, pt_hit* hits_dev, uint32_t n_rec) {
+    pt::TracePlan plan;
+    int rc = pt::prepare_trace(c, p, sw, sh, n_passes, seeds, hits_dev, n_rec, plan);
+    if (rc) return rc;
+    const hipError_t e = pt::enqueue_trace(c, plan, c->stream, true);
+    if (e != hipSuccess) return c->hip_fail(e, "trace launch");
     if ((rc = c->sync("trace"))) return rc;
     if (stats) {
-        unsigned long long v[kStCount];
-        e = hipMemcpy(v, c->stats.p, sizeof(v), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) return c->hip_fail(e, "stats readback");
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
-        std::memset(stats, 0, sizeof(*stats));
-        stats->extension_rays = v[kStExt];
-        stats->shadow_rays = v[kStShadow];
-        stats->paths = v[kStPaths];
-        stats->node_pops = v[kStPops];
-        stats->inner_pops = v[kStInner];
-        stats->object_visits = v[kStObjs];
-        stats->tri_tests = v[kStTris];
-        stats->shade_events = v[kStShade];
-        stats->kernel_ms = ms;
-        stats->launches = 1;
+        return pt::read_stats(c, stats, ms, 1);
     }
     return PT_OK;
 }

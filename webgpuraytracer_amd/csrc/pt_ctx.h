@@ -116,3 +116,26 @@ struct pt_ctx {
         for (DevBuf* b : all) b->release();
     }
 };
+
+namespace pt {
+
+// A trace call split in two: prepare_trace allocates, uploads the seeds and
+// fills the kernel arguments (outside any stream capture); enqueue_trace only
+// launches kernels (zeroing the stats with a kernel), so a batch can record it
+// into a graph.
+struct TracePlan {
+    TraceArgs a{};
+    WfArgs w{};
+    uint64_t per = 0;      // wavefront: passes per batch
+    bool wavefront = false;
+    bool cn = false;       // megakernel: counting variant
+    uint32_t grid = 0;     // megakernel: persistent grid
+};
+int check_trace_params(pt_ctx* c, const pt_scene_params* p);
+int prepare_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t sh, uint32_t n_passes,
+                  const uint32_t* seeds, pt_hit* hits_dev, uint32_t n_rec, TracePlan& plan);
+hipError_t enqueue_trace(pt_ctx* c, TracePlan& plan, hipStream_t s, bool timed);
+int read_stats(pt_ctx* c, pt_stats* stats, float ms, uint32_t launches);
+
+}  // namespace pt
+

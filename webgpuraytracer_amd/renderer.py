@@ -287,3 +287,65 @@ class LearnedDenoiser:
         f[0, :, :, 6:9] = gv[:, :, 0:3]
         f[0, :, :, 9] = gv[:, :, 11]
         return f
+
+
+class Batch:
+    """Several Renderers (each with its own scene, frame and options) rendered
+    one progressive frame at a time as one graph launch (pt_batch.h)."""
+
+    def __init__(self, device=0):
+        self.lib = A.load_pt()
+        self.h = C.c_void_p()
+        rc = self.lib.pt_batch_create(C.byref(self.h), device)
+        if rc:
+            raise A.PTError(rc, "pt_batch_create failed")
+        self.jobs = []
+
+    def _check(self, rc):
+        if rc:
+            raise A.PTError(rc, (self.lib.pt_batch_last_error(self.h) or b"").decode())
+        return rc
+
+    def close(self):
+        if self.h:
+            self.lib.pt_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, renderer):
+        idx = A.u32()
+        self._check(self.lib.pt_batch_add(self.h, renderer.ctx, C.byref(idx)))
+        self.jobs.append(renderer)
+        return idx.value
+
+    def frame(self, spp, base_seeds=None):
+        """One frame: job i renders spp[i] passes from its own progressive
+        sample count (seeds as Renderer.render derives them from base_seeds[i]).
+        Returns the per-job pt_stats."""
+        n = len(self.jobs)
+        params = (A.SceneParams * n)()
+        passes = (A.u32 * n)(*spp)
+        seed_arrays = []
+        seed_ptrs = (C.c_void_p * n)()
+        for i, r in enumerate(self.jobs):
+            bs = 0 if base_seeds is None else base_seeds[i]
+            s = np.array([(bs * 0x9E3779B1 + (r.frame + k) * 0x85EBCA77) & 0xFFFFFFFF for k in range(spp[i])],
+                         np.uint32)
+            params[i] = r.params(seed=int(s[0]) if spp[i] else 0)
+            seed_arrays.append(s)
+            seed_ptrs[i] = s.ctypes.data if spp[i] else None
+        stats = (A.Stats * n)()
+        self._check(self.lib.pt_batch_frame(self.h, params, passes, seed_ptrs, stats))
+        for i, r in enumerate(self.jobs):
+            r.frame += spp[i]
+        return list(stats)
+
+    def graph_info(self):
+        f, i, u = A.u32(), A.u32(), A.u32()
+        self._check(self.lib.pt_batch_graph_info(self.h, C.byref(f), C.byref(i), C.byref(u)))
+        return f.value, i.value, u.value
