@@ -110,12 +110,15 @@ int ptgen_scene_params(const ptgen_scene* s, uint32_t width, uint32_t height,
  *            (radiance 15, NEE), 512x256 procedural sky (sun peak/mean 1e3).
  *  config 2: 4 displaced geodesic spheres of frequency freq[i] (20 n^2
  *            triangles), 64 instances, 2048x1024 sky (sun dynamic range 1e5).
+ *  config 3: config 1 plus n_instances (16) instances of 4 displaced
+ *            geodesic spheres (frequency freq[i], default 8) placed and
+ *            scaled like config 1's primitives (the batched-jobs scenes).
  * opts: NULL for the defaults; otherwise overrides (0 keeps a default). */
 typedef struct ptgen_config_opts {
     uint32_t sky_width;
     uint32_t sky_height;
-    uint32_t mesh_freq[4];   /* config 2 geodesic frequencies (64, 90, 110, 127) */
-    uint32_t n_instances;    /* config 2 instance count (64) */
+    uint32_t mesh_freq[4];   /* config 2/3 geodesic frequencies (64, 90, 110, 127 / 8) */
+    uint32_t n_instances;    /* config 2/3 instance count (64 / 16) */
     int32_t bounces;         /* BOUNCE_LIMIT (4 for config 1, 8 for config 2) */
 } ptgen_config_opts;
 int ptgen_make_config(ptgen_scene** out, int config, uint64_t seed, const ptgen_config_opts* opts);

@@ -116,6 +116,24 @@ def test_render_1spp_matches_oracle(renderer):
     assert np.mean(np.abs(out.astype(int) - ro.astype(int)) <= 1) >= 0.97
 
 
+def test_render_1spp_mixed_scene_matches_oracle(renderer):
+    """Analytic primitives and instanced meshes in one scene: the kernels'
+    run-time-flags variant (the batched-jobs scenes)."""
+    scene = SceneBuilder.config(3, seed=12, mesh_freq=[6, 7, 8, 9], n_instances=9, bounces=5)
+    w, h = 96, 64
+    ref = _setup(renderer, scene, w, h)
+    p = scene.params(w, h, sample_number=0, seed=4242)
+    st = renderer.render_section(p, passes=1)
+    ga = renderer.read_accum()
+    ra, _, rst = ref.render(p, passes=1)
+    rel = np.abs(ga[..., :3] - ra[..., :3]) / np.maximum(np.abs(ra[..., :3]), 1e-3)
+    frac = float(np.mean(np.all(rel <= 1e-4, axis=-1)))
+    print("mixed-scene 1spp pixel agreement", frac, "rays gpu", st.extension_rays, "ref", rst.extension_rays)
+    assert frac >= 0.97
+    assert abs(int(st.extension_rays) - int(rst.extension_rays)) <= 0.01 * rst.extension_rays
+    assert rst.tri_tests > 0 and st.shadow_rays > 0  # meshes are hit and lights are sampled
+
+
 def test_render_converged_relmse(renderer):
     scene = SceneBuilder.config(1, seed=41)
     w, h = 48, 32

@@ -4,9 +4,10 @@
 
 Job j (seeded): a square frame whose pixel count is log-uniform in
 [128^2, 1024^2]; spp per frame from {1, 4, 16, 64}; bounce limit uniform in
-2..12; an instanced-mesh scene with unique triangles log-uniform in [1k, 1M]
-(four displaced geodesic spheres of equal frequency) and instances
-log-uniform in 1..128, over a 512x256 procedural sky.
+2..12; the small analytic scene (12 primitives, emissive quad, ground plane,
+512x256 procedural sky) plus instanced meshes: unique triangles
+log-uniform in [1k, 1M] (four displaced geodesic spheres of equal
+frequency), instances log-uniform in 1..128, placed among the primitives.
 
 Measured: the same frames rendered job after job with pt_trace (one host
 wait per job), then as batch frames (pt_batch_frame: one graph launch and one
@@ -45,7 +46,7 @@ def job_specs(n, seed):
 
 
 def build(spec):
-    scene = SceneBuilder.config(2, seed=spec["scene_seed"], sky=(512, 256), mesh_freq=[spec["freq"]] * 4,
+    scene = SceneBuilder.config(3, seed=spec["scene_seed"], mesh_freq=[spec["freq"]] * 4,
                                 n_instances=spec["instances"], bounces=spec["bounces"])
     r = Renderer(0)
     r.set_option(A.PT_OPT_COUNT_STATS, 0)
@@ -61,6 +62,8 @@ def main(argv=None):
     ap.add_argument("--frames", type=int, default=4)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--tune", action="store_true")
+    ap.add_argument("--mode", choices=["both", "batched", "job-after-job"], default="both",
+                    help="which way to render (a profile of one way takes one)")
     a = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     specs = job_specs(a.jobs, a.seed)
@@ -80,12 +83,13 @@ def main(argv=None):
             opts = cache.get(r, scene.desc().flags, s["triangles"], s["spp"], log=log)
             tuned.append(describe(opts))
     # job after job: pt_trace per job, one host wait each
-    for r, s in zip(rs, specs):
-        r.reset()
-        r.render(spp=s["spp"], base_seed=11)
     seq_t, seq_rays = [], 0
     per_job, seq_job_rays = [0.0] * len(rs), [0] * len(rs)
-    for f in range(a.frames):
+    for r, s in zip(rs, specs):
+        r.reset()
+        if a.mode != "batched":
+            r.render(spp=s["spp"], base_seed=11)
+    for f in range(a.frames if a.mode != "batched" else 0):
         t = time.perf_counter()
         for i, (r, s) in enumerate(zip(rs, specs)):
             tj = time.perf_counter()
@@ -99,9 +103,10 @@ def main(argv=None):
     for r in rs:
         r.reset()
         b.add(r)
-    b.frame(spp, base_seeds=[11] * len(rs))  # records and instantiates the graph
+    if a.mode != "job-after-job":
+        b.frame(spp, base_seeds=[11] * len(rs))  # records and instantiates the graph
     bat_t, bat_rays, job_rays = [], 0, [0] * len(rs)
-    for f in range(a.frames):
+    for f in range(a.frames if a.mode != "job-after-job" else 0):
         t = time.perf_counter()
         st = b.frame(spp, base_seeds=[11] * len(rs))
         bat_t.append(time.perf_counter() - t)
@@ -109,18 +114,20 @@ def main(argv=None):
             job_rays[i] += x.extension_rays + x.shadow_rays
         bat_rays += sum(x.extension_rays + x.shadow_rays for x in st)
     frames, inst, upd = b.graph_info()
+    rate = lambda rays, t: rays / sum(t) * 1e-6 if t else None  # noqa: E731
+    per_frame = lambda t: 1e3 * sum(t) / len(t) if t else None  # noqa: E731
     out = {
         "metric": "aggregate Mrays/s, batched jobs", "jobs": len(rs), "frames": a.frames,
-        "mrays_per_s_batched": bat_rays / sum(bat_t) * 1e-6,
-        "mrays_per_s_job_after_job": seq_rays / sum(seq_t) * 1e-6,
-        "ms_per_frame_batched": 1e3 * sum(bat_t) / a.frames,
-        "ms_per_frame_job_after_job": 1e3 * sum(seq_t) / a.frames,
+        "mrays_per_s_batched": rate(bat_rays, bat_t),
+        "mrays_per_s_job_after_job": rate(seq_rays, seq_t),
+        "ms_per_frame_batched": per_frame(bat_t),
+        "ms_per_frame_job_after_job": per_frame(seq_t),
         "graph": {"frames": frames, "instantiations": inst, "updates": upd},
         "tuned": a.tune,
         "per_job": [{"job": s["job"], "res": s["width"], "spp": s["spp"], "bounces": s["bounces"],
                      "triangles": s["triangles"], "instances": s["instances"],
-                     "mrays_per_frame": job_rays[i] / a.frames * 1e-6,
-                     "mrays_per_s_alone": seq_job_rays[i] / per_job[i] * 1e-6,
+                     "mrays_per_frame": (job_rays[i] or seq_job_rays[i]) / max(a.frames, 1) * 1e-6,
+                     "mrays_per_s_alone": seq_job_rays[i] / per_job[i] * 1e-6 if per_job[i] else None,
                      "options": tuned[i] if tuned else None} for i, s in enumerate(specs)],
     }
     print(json.dumps(out))

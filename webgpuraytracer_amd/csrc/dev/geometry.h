@@ -83,6 +83,12 @@ struct DevScene {
 // Compile-time flag policy: F == kDynamicFlags reads the uploaded flags at
 // run time (wave-uniform branches); any other value is a specialisation.
 constexpr uint32_t kDynamicFlags = 0xffffffffu;
+// Flag sets compiled as constants: the small analytic scene family, the
+// instanced-mesh family, and the two together.
+constexpr uint32_t kFlagsC1 = PT_HAS_SPHERES | PT_HAS_CUBES | PT_HAS_PLANES | PT_HAS_CYLINDERS | PT_HAS_LIGHTS |
+                              PT_HAS_SKYBOX | PT_MIS_SKYBOX;
+constexpr uint32_t kFlagsC2 = PT_HAS_MESHES | PT_HAS_SKYBOX | PT_MIS_SKYBOX;
+constexpr uint32_t kFlagsC3 = kFlagsC1 | PT_HAS_MESHES;
 template <uint32_t F>
 struct Flags {
     uint32_t rt;

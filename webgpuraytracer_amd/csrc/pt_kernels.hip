@@ -22,9 +22,7 @@ namespace pt {
 
 constexpr int kTile = 8;
 constexpr int kWave = 64;
-constexpr uint32_t kFlagsC1 = PT_HAS_SPHERES | PT_HAS_CUBES | PT_HAS_PLANES | PT_HAS_CYLINDERS | PT_HAS_LIGHTS |
-                              PT_HAS_SKYBOX | PT_MIS_SKYBOX;
-constexpr uint32_t kFlagsC2 = PT_HAS_MESHES | PT_HAS_SKYBOX | PT_MIS_SKYBOX;
+
 
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
     const uint32_t q = nb / 8u, r = nb % 8u;
@@ -582,6 +580,7 @@ static hipError_t trace_dispatch(const DevScene& S, const TraceArgs& a, bool cn,
                                  hipStream_t s) {
     if (!a.hits && S.flags == kFlagsC1) return trace_f<kFlagsC1>(S, a, cn, grid, bpc, s);
     if (!a.hits && S.flags == kFlagsC2) return trace_f<kFlagsC2>(S, a, cn, grid, bpc, s);
+    if (!a.hits && S.flags == kFlagsC3) return trace_f<kFlagsC3>(S, a, cn, grid, bpc, s);
     return trace_f<kDynamicFlags>(S, a, cn, grid, bpc, s);
 }
 

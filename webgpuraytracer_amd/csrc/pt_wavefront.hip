@@ -404,21 +404,17 @@ static hipError_t wf_occupancy_f(bool cn, bool wide, size_t lds, int* bpc) {
 
 hipError_t wf_trace_blocks_per_cu(const DevScene& S, bool count_nodes, bool wide, uint32_t stack_lds,
                                   int* blocks_per_cu) {
-    constexpr uint32_t kC1 = PT_HAS_SPHERES | PT_HAS_CUBES | PT_HAS_PLANES | PT_HAS_CYLINDERS | PT_HAS_LIGHTS |
-                             PT_HAS_SKYBOX | PT_MIS_SKYBOX;
-    constexpr uint32_t kC2 = PT_HAS_MESHES | PT_HAS_SKYBOX | PT_MIS_SKYBOX;
     const size_t lds = size_t(stack_lds) * kWfBlock * 4u;
-    if (S.flags == kC1) return wf_occupancy_f<kC1>(count_nodes, wide, lds, blocks_per_cu);
-    if (S.flags == kC2) return wf_occupancy_f<kC2>(count_nodes, wide, lds, blocks_per_cu);
+    if (S.flags == kFlagsC1) return wf_occupancy_f<kFlagsC1>(count_nodes, wide, lds, blocks_per_cu);
+    if (S.flags == kFlagsC2) return wf_occupancy_f<kFlagsC2>(count_nodes, wide, lds, blocks_per_cu);
+    if (S.flags == kFlagsC3) return wf_occupancy_f<kFlagsC3>(count_nodes, wide, lds, blocks_per_cu);
     return wf_occupancy_f<kDynamicFlags>(count_nodes, wide, lds, blocks_per_cu);
 }
 
 hipError_t launch_wavefront_batch(const DevScene& S, const WfArgs& a, bool count_nodes, hipStream_t s) {
-    constexpr uint32_t kC1 = PT_HAS_SPHERES | PT_HAS_CUBES | PT_HAS_PLANES | PT_HAS_CYLINDERS | PT_HAS_LIGHTS |
-                             PT_HAS_SKYBOX | PT_MIS_SKYBOX;
-    constexpr uint32_t kC2 = PT_HAS_MESHES | PT_HAS_SKYBOX | PT_MIS_SKYBOX;
-    if (S.flags == kC1) return wf_run<kC1>(S, a, count_nodes, s);
-    if (S.flags == kC2) return wf_run<kC2>(S, a, count_nodes, s);
+    if (S.flags == kFlagsC1) return wf_run<kFlagsC1>(S, a, count_nodes, s);
+    if (S.flags == kFlagsC2) return wf_run<kFlagsC2>(S, a, count_nodes, s);
+    if (S.flags == kFlagsC3) return wf_run<kFlagsC3>(S, a, count_nodes, s);
     return wf_run<kDynamicFlags>(S, a, count_nodes, s);
 }
 

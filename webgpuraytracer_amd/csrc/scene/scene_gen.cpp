@@ -1196,7 +1196,9 @@ ptgen_material plain_material() {
     return m;
 }
 
-int make_config1(ptgen_scene* s, uint64_t seed, const ptgen_config_opts& o) {
+// The configuration-1 content (primitives, emissive quad, plane, sky,
+// bounce limit) without finalizing the scene.
+void add_config1_content(ptgen_scene* s, uint64_t seed, const ptgen_config_opts& o) {
     Rng r(mix_seed(seed, 1));
     auto disney = [&]() {
         ptgen_material m = plain_material();
@@ -1251,6 +1253,49 @@ int make_config1(ptgen_scene* s, uint64_t seed, const ptgen_config_opts& o) {
     }
     ptgen_set_sky_procedural(s, o.sky_width ? o.sky_width : 512, o.sky_height ? o.sky_height : 256, seed, 1e3f);
     ptgen_set_bounces(s, o.bounces > 0 ? o.bounces : 4);
+}
+
+int make_config1(ptgen_scene* s, uint64_t seed, const ptgen_config_opts& o) {
+    add_config1_content(s, seed, o);
+    return ptgen_finalize(s);
+}
+
+// Configuration 1 plus instanced meshes: four displaced geodesic spheres of
+// frequency mesh_freq[i] (default 8), n_instances instances (default 16)
+// placed like the primitives (centres U[-2,2]^3, uniform scale U[0.2,0.8],
+// random rotation), each with its own Disney material drawn like config 1's.
+int make_config3(ptgen_scene* s, uint64_t seed, const ptgen_config_opts& o) {
+    add_config1_content(s, seed, o);
+    int models[4];
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t n = o.mesh_freq[i] ? o.mesh_freq[i] : 8;
+        if (n > 255) return s->fail("geodesic frequency above 255");
+        std::vector<float> pos, nrm, uv;
+        std::vector<uint32_t> idx;
+        geodesic_sphere(n, mix_seed(seed, 300 + i), 0.1, pos, nrm, uv, idx);
+        models[i] = ptgen_add_model(s, pos.data(), nrm.data(), uv.data(), uint32_t(pos.size() / 3), idx.data(),
+                                    uint32_t(idx.size() / 3));
+        if (models[i] < 0) return models[i];
+    }
+    Rng r(mix_seed(seed, 3));
+    const uint32_t n_inst = o.n_instances ? o.n_instances : 16;
+    for (uint32_t i = 0; i < n_inst; ++i) {
+        ptgen_material m = plain_material();
+        for (int k = 0; k < 3; ++k) m.color[k] = f32(r.uniform());
+        m.roughness = f32(r.uniform());
+        m.metallic = r.uniform() < 0.3 ? 1.f : 0.f;
+        const bool trans = r.uniform() < 0.15;
+        const float ior = f32(r.uniform(1.3, 1.7));
+        m.transmission = trans ? 1.f : 0.f;
+        m.ior = trans ? ior : 1.5f;
+        const int mi = ptgen_add_material(s, &m);
+        float p[3], q[4];
+        for (int k = 0; k < 3; ++k) p[k] = f32(r.uniform(-2, 2));
+        random_quat(r, q);
+        const float sc1 = f32(r.uniform(0.2, 0.8));
+        const float sc[3] = {sc1, sc1, sc1};
+        ptgen_add_instance(s, models[i % 4], mi, p, q, sc, 1);
+    }
     return ptgen_finalize(s);
 }
 
@@ -1305,13 +1350,13 @@ int make_config2(ptgen_scene* s, uint64_t seed, const ptgen_config_opts& o) {
 }  // namespace
 
 extern "C" int ptgen_make_config(ptgen_scene** out, int config, uint64_t seed, const ptgen_config_opts* opts) {
-    if (!out || (config != 1 && config != 2)) return PT_ERR_ARG;
+    if (!out || config < 1 || config > 3) return PT_ERR_ARG;
     ptgen_config_opts o;
     std::memset(&o, 0, sizeof(o));
     if (opts) o = *opts;
     int rc = ptgen_new(out);
     if (rc) return rc;
-    rc = config == 1 ? make_config1(*out, seed, o) : make_config2(*out, seed, o);
+    rc = config == 1 ? make_config1(*out, seed, o) : config == 2 ? make_config2(*out, seed, o) : make_config3(*out, seed, o);
     if (rc < 0) { ptgen_destroy(*out); *out = nullptr; }
     return rc < 0 ? rc : PT_OK;
 }

@@ -40,7 +40,8 @@ class SceneBuilder:
 
     @classmethod
     def config(cls, config, seed=1, sky=None, mesh_freq=None, n_instances=0, bounces=0):
-        """The seeded synthetic scenes (1: small analytic, 2: large instanced meshes)."""
+        """The seeded synthetic scenes (1: small analytic, 2: large instanced
+        meshes, 3: small analytic plus instanced meshes)."""
         lib = A.load_scene_lib()
         o = A.ConfigOpts()
         if sky:
