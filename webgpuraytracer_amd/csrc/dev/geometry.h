@@ -179,8 +179,7 @@ __device__ __forceinline__ void leaf_range(uint32_t ref, const pt_bvh_node* raw,
 // intersect_triangle, shader.wgsl:267-288, with edges precomputed on the
 // host (v1-v0 and v2-v0 round identically there).  Returns t > 1e-6 or -1;
 // u, v are the barycentrics of v1 and v2.
-__device__ __forceinline__ float tri_hit(v3 o, v3 d, const float4* g, float& u, float& v) {
-    const float4 A = g[0], B = g[1], C = g[2];
+__device__ __forceinline__ float tri_hit_v(v3 o, v3 d, float4 A, float4 B, float4 C, float& u, float& v) {
     const v3 v0 = mk(A.x, A.y, A.z), e1 = mk(B.x, B.y, B.z), e2 = mk(C.x, C.y, C.z);
     const v3 h = cross(d, e2);
     const float a = dot(e1, h);
@@ -194,6 +193,9 @@ __device__ __forceinline__ float tri_hit(v3 o, v3 d, const float4* g, float& u, 
     if (v < 0.0f || u + v > 1.0f) return -1.0f;
     const float t = f * dot(e2, q);
     return t > 0.000001f ? t : -1.0f;
+}
+__device__ __forceinline__ float tri_hit(v3 o, v3 d, const float4* g, float& u, float& v) {
+    return tri_hit_v(o, d, g[0], g[1], g[2], u, v);
 }
 
 // ------------------------------------------------------------ analytic ------

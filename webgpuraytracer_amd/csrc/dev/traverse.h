@@ -370,16 +370,34 @@ struct Trav {
             uint32_t s0, n0;
             if (in_blas) {
                 leaf_range(ref, S.bvh_raw, s0, n0);
-                for (uint32_t k = tri_off + s0; k < tri_off + s0 + n0; ++k) {
+                const uint32_t kb = tri_off + s0, ke = kb + n0;
+                for (uint32_t k = kb; k < ke; k += 2u) {
+                    // both triangles' vertices are requested before either test
+                    const bool two = k + 1u < ke;
+                    const float4* g0 = S.tri_geo + 3u * k;
+                    const float4* g1 = two ? g0 + 3 : g0;
+                    const float4 a0 = g0[0], b0 = g0[1], c0 = g0[2];
+                    const float4 a1 = g1[0], b1 = g1[1], c1 = g1[2];
                     if (CN) c.tris++;
                     float u, v;
-                    const float t = tri_hit(lo, ld, S.tri_geo + 3u * k, u, v);
+                    float t = tri_hit_v(lo, ld, a0, b0, c0, u, v);
                     if (t >= 0.0f && t < h.t) {
                         if (any) {
                             blocked = true;
                             return false;
                         }
                         h.t = t; h.kind = kObjectHit; h.idx = obj; h.prim = int(k); h.bu = u; h.bv = v;
+                    }
+                    if (two) {
+                        if (CN) c.tris++;
+                        t = tri_hit_v(lo, ld, a1, b1, c1, u, v);
+                        if (t >= 0.0f && t < h.t) {
+                            if (any) {
+                                blocked = true;
+                                return false;
+                            }
+                            h.t = t; h.kind = kObjectHit; h.idx = obj; h.prim = int(k + 1u); h.bu = u; h.bv = v;
+                        }
                     }
                 }
             } else {
