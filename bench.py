@@ -78,6 +78,8 @@ def main():
                     help="megakernel pass schedule (same bits either way)")
     ap.add_argument("--traversal", choices=["wide", "reference"], default="wide",
                     help="BVH traversal: 4-wide nodes, or the reference's child-pair order")
+    ap.add_argument("--blas-build", choices=["reference", "device"], default="reference",
+                    help="bottom-level trees of the wide traversal: the uploaded BVH, or rebuilt on the GPU")
     ap.add_argument("--compare", action="store_true",
                     help="also time the other backend/schedule variants (reported under alt)")
     args = ap.parse_args()
@@ -94,6 +96,8 @@ def main():
     scene = SceneBuilder.config(2, seed=args.seed)
     build_s = time.perf_counter() - t0
     r = Renderer(0)
+    if args.blas_build == "device":
+        r.set_option(A.PT_OPT_BLAS_BUILD, A.PT_BLAS_BUILD_DEVICE)
     t0 = time.perf_counter()
     r.set_scene(scene)
     r.resize(W, H)
@@ -148,7 +152,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "plain tracer, large scene: 1920x1080, 64 spp/step, 8 bounces, 64 instances of 4 "
                                "displaced geodesic spheres (%d triangles), 2048x1024 env, %s"
-                               % (d.triangles.count, chosen),
+                               % (d.triangles.count, chosen + ("" if args.blas_build == "reference"
+                                                               else ", BLAS built on the device")),
                    "extension_rays": int(sum(s.extension_rays for s in stats)),
                    "shadow_rays": int(sum(s.shadow_rays for s in stats)),
                    "paths": int(sum(s.paths for s in stats)),

@@ -119,9 +119,16 @@ enum {
     PT_OPT_TRAVERSAL = 5,      /* BVH traversal, PT_TRAVERSAL_* (default WIDE) */
     PT_OPT_LDS_STACK = 6,      /* traversal-stack entries per lane kept in LDS, the rest in device
                                   memory; 0 = all in LDS (default 16) */
-    PT_OPT_WAVEFRONT_REFILL = 7 /* wavefront traversal: idle lanes of a wave that trigger fetching
+    PT_OPT_WAVEFRONT_REFILL = 7, /* wavefront traversal: idle lanes of a wave that trigger fetching
                                    new rays, 1..64 (default 56) */
+    PT_OPT_BLAS_BUILD = 8        /* PT_BLAS_BUILD_*: where the wide traversal's bottom-level trees
+                                   come from; read by the next pt_upload_scene */
 };
+/* REFERENCE: the uploaded tracer.js BVH, widened on the host (default).
+   DEVICE: rebuilt on the GPU from the triangles (linear BVH over Morton
+   codes, 4-wide); the same closest hits, different node counts.  The
+   child-pair (reference-order) traversal always uses the uploaded BVH. */
+enum { PT_BLAS_BUILD_REFERENCE = 0, PT_BLAS_BUILD_DEVICE = 1 };
 enum { PT_BACKEND_MEGAKERNEL = 0, PT_BACKEND_WAVEFRONT = 1 };
 /* SYNC: the lanes of a wave run each pass together.  REGEN: a lane whose path
    ends starts its pixel's next pass at once.  Both produce the same bits. */

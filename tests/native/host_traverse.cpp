@@ -35,6 +35,7 @@ bool make_scene(const pt_scene_desc* d, HostScene& h, std::string& err) {
     S.tlas_raw = static_cast<const pt_bvh_node*>(d->tlas.ptr);
     S.bvh_raw = static_cast<const pt_bvh_node*>(d->bvh.ptr);
     S.tri_geo = h.geo.data();
+    S.tri_geo_w = h.geo.data();
     S.tris = static_cast<const pt_triangle*>(d->triangles.ptr);
     S.planes = static_cast<const pt_plane*>(d->planes.ptr);
     S.meshes = static_cast<const pt_mesh_instance*>(d->meshes.ptr);

@@ -327,3 +327,58 @@ def test_stack_spill_bitwise(renderer, traversal):
     finally:
         renderer.set_option(A.PT_OPT_LDS_STACK, 16)
     assert np.array_equal(acc[0].view(np.uint32), acc[1].view(np.uint32))
+
+
+@pytest.fixture
+def device_blas(renderer):
+    """Bottom-level trees rebuilt on the device at the next upload."""
+    renderer.set_option(A.PT_OPT_BLAS_BUILD, A.PT_BLAS_BUILD_DEVICE)
+    yield
+    renderer.set_option(A.PT_OPT_BLAS_BUILD, A.PT_BLAS_BUILD_REFERENCE)
+
+
+@pytest.mark.parametrize("which", ["mesh", "mixed"])
+def test_device_built_blas_finds_the_same_hits(renderer, device_blas, which):
+    """A linear BVH built on the device from the triangles: the same closest
+    hits (primitive ids of the uploaded order, distances) as the oracle on the
+    uploaded BVH; the reference-order traversal still reads the uploaded BVH."""
+    if which == "mesh":
+        scene = small_mesh_scene(seed=5, freq=(9, 12, 15, 20), n_instances=20)
+        box, spread = 9.0, 8.0
+    else:
+        scene = SceneBuilder.config(3, seed=21, mesh_freq=[10, 12, 14, 16], n_instances=12, bounces=4)
+        box, spread = 4.0, 3.0
+    ref = _setup(renderer, scene, 16, 16)
+    rays = random_rays(20000, seed=43, box=box, spread=spread)
+    r, _ = ref.trace_rays(rays)
+    g = renderer.trace_rays(rays)
+    s = compare_hits(g, r, rays, ref)
+    print(which, "device BLAS", s)
+    assert s["exact_t"] >= 0.999 * s["n"]
+    assert hits_to_np(r)["prim"].max() > 0  # triangles are hit
+    renderer.set_option(A.PT_OPT_TRAVERSAL, A.PT_TRAVERSAL_REFERENCE)
+    try:
+        g2 = renderer.trace_rays(rays)
+    finally:
+        renderer.set_option(A.PT_OPT_TRAVERSAL, A.PT_TRAVERSAL_WIDE)
+    assert np.array_equal(hits_to_np(g2)["count"], hits_to_np(r)["count"])
+
+
+def test_device_built_blas_renders_like_uploaded_blas(renderer):
+    scene = small_mesh_scene(seed=14, freq=(9, 12, 15, 20), n_instances=20)
+    w, h = 96, 64
+    seeds = [5, 6, 7, 8, 9]
+    p = scene.params(w, h, sample_number=0, seed=seeds[0])
+    acc = {}
+    try:
+        for build in (A.PT_BLAS_BUILD_REFERENCE, A.PT_BLAS_BUILD_DEVICE):
+            renderer.set_option(A.PT_OPT_BLAS_BUILD, build)
+            renderer.set_scene(scene)
+            renderer.resize(w, h)
+            renderer.render_section(p, passes=len(seeds), seeds=seeds)
+            acc[build] = renderer.read_accum()
+    finally:
+        renderer.set_option(A.PT_OPT_BLAS_BUILD, A.PT_BLAS_BUILD_REFERENCE)
+    same = np.all(acc[0] == acc[1], axis=-1).mean()
+    print("device vs uploaded BLAS: identical pixels", same)
+    assert same >= 0.999

@@ -64,7 +64,8 @@ struct DevScene {
     const Root* roots;             // slot 0: TLAS; then one per distinct BLAS root
     const pt_bvh_node* tlas_raw;
     const pt_bvh_node* bvh_raw;
-    const float4* tri_geo;         // 3 per triangle
+    const float4* tri_geo;         // 3 per triangle, in the uploaded order (child-pair trees)
+    const float4* tri_geo_w;       // the same records in the wide trees' leaf order
     const pt_triangle* tris;
     const pt_plane* planes;
     const pt_mesh_instance* meshes;

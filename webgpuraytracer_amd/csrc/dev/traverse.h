@@ -148,8 +148,12 @@ __device__ void blas_closest(const DevScene& S, uint32_t root_slot, const float*
             for (uint32_t k = tri_off + s0; k < tri_off + s0 + n0; ++k) {
                 if (CN) c.tris++;
                 float u, v;
-                const float t = tri_hit(lo, ld, S.tri_geo + 3u * k, u, v);
-                if (t >= 0.0f && t < h.t) { h.t = t; h.kind = kind; h.idx = idx; h.prim = int(k); h.bu = u; h.bv = v; }
+                const float4* g = (W ? S.tri_geo_w : S.tri_geo) + 3u * k;
+                const float4 a = g[0];
+                const float t = tri_hit_v(lo, ld, a, g[1], g[2], u, v);
+                if (t >= 0.0f && t < h.t) {
+                    h.t = t; h.kind = kind; h.idx = idx; h.prim = int(f2u(a.w)); h.bu = u; h.bv = v;
+                }
             }
         } else {
             have = visit_node<W, CN>(S.bnodes, S.wbnodes, ref, lo, linv, h.t, st, c, ref);
@@ -180,7 +184,7 @@ __device__ bool blas_any(const DevScene& S, uint32_t root_slot, const float* inv
             for (uint32_t k = tri_off + s0; k < tri_off + s0 + n0; ++k) {
                 if (CN) c.tris++;
                 float u, v;
-                const float t = tri_hit(lo, ld, S.tri_geo + 3u * k, u, v);
+                const float t = tri_hit(lo, ld, (W ? S.tri_geo_w : S.tri_geo) + 3u * k, u, v);
                 if (t >= 0.0f && t < tdist) return true;
             }
         } else {
@@ -374,7 +378,7 @@ struct Trav {
                 for (uint32_t k = kb; k < ke; k += 2u) {
                     // both triangles' vertices are requested before either test
                     const bool two = k + 1u < ke;
-                    const float4* g0 = S.tri_geo + 3u * k;
+                    const float4* g0 = (W ? S.tri_geo_w : S.tri_geo) + 3u * k;
                     const float4* g1 = two ? g0 + 3 : g0;
                     const float4 a0 = g0[0], b0 = g0[1], c0 = g0[2];
                     const float4 a1 = g1[0], b1 = g1[1], c1 = g1[2];
@@ -386,7 +390,7 @@ struct Trav {
                             blocked = true;
                             return false;
                         }
-                        h.t = t; h.kind = kObjectHit; h.idx = obj; h.prim = int(k); h.bu = u; h.bv = v;
+                        h.t = t; h.kind = kObjectHit; h.idx = obj; h.prim = int(f2u(a0.w)); h.bu = u; h.bv = v;
                     }
                     if (two) {
                         if (CN) c.tris++;
@@ -396,7 +400,7 @@ struct Trav {
                                 blocked = true;
                                 return false;
                             }
-                            h.t = t; h.kind = kObjectHit; h.idx = obj; h.prim = int(k + 1u); h.bu = u; h.bv = v;
+                            h.t = t; h.kind = kObjectHit; h.idx = obj; h.prim = int(f2u(a1.w)); h.bu = u; h.bv = v;
                         }
                     }
                 }

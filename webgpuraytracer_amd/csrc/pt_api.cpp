@@ -110,6 +110,7 @@ void refresh_dev(pt_ctx* c) {
     S.tlas_raw = static_cast<const pt_bvh_node*>(c->tlas_raw.p);
     S.bvh_raw = static_cast<const pt_bvh_node*>(c->bvh_raw.p);
     S.tri_geo = static_cast<const float4*>(c->tri_geo.p);
+    S.tri_geo_w = static_cast<const float4*>(c->tri_geo_w.p ? c->tri_geo_w.p : c->tri_geo.p);
     S.tris = static_cast<const pt_triangle*>(c->tris.p);
     S.planes = static_cast<const pt_plane*>(c->planes.p);
     S.meshes = static_cast<const pt_mesh_instance*>(c->meshes.p);
@@ -168,6 +169,7 @@ int upload_dynamic(pt_ctx* c, const pt_scene_desc* d, bool first) {
         if ((rc = c->upload(c->wbnodes, L.wbn.data(), L.wbn.size() * sizeof(WNode), "wide bvh nodes"))) return rc;
     }
     c->tlas_depth = td;
+    c->tlas_wneed = L.tlas_wneed;
     c->host_roots = roots;
     if ((rc = c->upload(c->tnodes, L.tn.data(), L.tn.size() * sizeof(CNode), "tlas nodes"))) return rc;
     if ((rc = c->upload(c->wtnodes, L.wtn.data(), L.wtn.size() * sizeof(WNode), "wide tlas nodes"))) return rc;
@@ -185,6 +187,111 @@ int upload_dynamic(pt_ctx* c, const pt_scene_desc* d, bool first) {
     c->n_planes = d->planes.count;
     c->n_meshes = d->meshes.count;
     return c->sync("upload");
+}
+
+// The wide traversal's bottom-level trees rebuilt on the device from the
+// uploaded triangles (PT_BLAS_BUILD_DEVICE); the child-pair trees and the
+// uploaded triangle order stay for the reference-order traversal.  Each BLAS
+// must cover one contiguous triangle range, reached through one triangle
+// offset.
+uint32_t float_bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+
+int device_blas(pt_ctx* c, const pt_scene_desc* d) {
+    if (c->blas_slot.empty()) return PT_OK;
+    const pt_bvh_node* bl = static_cast<const pt_bvh_node*>(d->bvh.ptr);
+    const pt_object* ob = static_cast<const pt_object*>(d->objects.ptr);
+    const pt_mesh_instance* mi = static_cast<const pt_mesh_instance*>(d->meshes.ptr);
+    std::map<uint32_t, uint32_t> tri_off;  // slot -> triangle offset
+    auto note = [&](uint32_t node_offset, uint32_t off) -> int {
+        const uint32_t slot = c->blas_slot.at(node_offset);
+        auto it = tri_off.find(slot);
+        if (it != tri_off.end() && it->second != off)
+            return c->fail(PT_ERR_UNSUPPORTED, "BLAS at node %u is used with two triangle offsets", node_offset);
+        tri_off[slot] = off;
+        return PT_OK;
+    };
+    int rc;
+    if (d->flags & PT_HAS_MESHES)
+        for (uint64_t i = 0; i < d->objects.count; ++i)
+            if (ob[i].object_type == PT_OBJ_MESH && (rc = note(float_bits(ob[i].param0), float_bits(ob[i].param1))))
+                return rc;
+    if (d->flags & PT_HAS_LIST_MESHES)
+        for (uint64_t i = 0; i < d->meshes.count; ++i)
+            if ((rc = note(mi[i].node_offset, mi[i].tri_offset))) return rc;
+    std::vector<BlasBuildJob> jobs;
+    std::vector<uint32_t> job_slot;
+    uint64_t total = 0;
+    for (const auto& kv : c->blas_slot) {
+        const uint32_t slot = kv.second;
+        if (!tri_off.count(slot)) continue;
+        // the leaves' triangle range (starts relative to the offset)
+        uint64_t lo = UINT64_MAX, hi = 0, count = 0;
+        std::vector<uint32_t> st{kv.first};
+        while (!st.empty()) {
+            const uint32_t i = st.back();
+            st.pop_back();
+            const pt_bvh_node& n = bl[i];
+            if (n.count > 0) {
+                lo = std::min<uint64_t>(lo, n.next);
+                hi = std::max<uint64_t>(hi, uint64_t(n.next) + n.count);
+                count += n.count;
+            } else {
+                st.push_back(i + 1);
+                st.push_back(kv.first + n.next);
+            }
+        }
+        if (count == 0 || count != hi - lo || hi - lo >= (1u << 25))
+            return c->fail(PT_ERR_UNSUPPORTED, "BLAS at node %u does not cover one contiguous triangle range", kv.first);
+        jobs.push_back(BlasBuildJob{uint32_t(tri_off[slot] + lo), uint32_t(hi - lo), uint32_t(lo)});
+        job_slot.push_back(slot);
+        total += hi - lo;
+    }
+    if (jobs.empty()) return PT_OK;
+    uint32_t max_n = 0;
+    for (const auto& j : jobs) max_n = std::max(max_n, j.n);
+    DevBuf scratch, ctr, wn;
+    const size_t sb = blas_build_scratch_bytes(max_n);
+    if ((rc = c->alloc(scratch, sb, "bvh build scratch")) || (rc = c->alloc(ctr, 64, "bvh build counters")) ||
+        (rc = c->alloc(wn, size_t(total) * sizeof(WNode), "device bvh nodes")) ||
+        (rc = c->alloc(c->tri_geo_w, size_t(d->triangles.count) * 3 * sizeof(float4), "triangle geometry (wide)"))) {
+        scratch.release();
+        ctr.release();
+        wn.release();
+        return rc;
+    }
+    std::vector<BlasBuildResult> res(jobs.size());
+    hipError_t e = hipMemcpyAsync(c->tri_geo_w.p, c->tri_geo.p, size_t(d->triangles.count) * 3 * sizeof(float4),
+                                  hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+        e = build_blas_device(static_cast<const float4*>(c->tri_geo.p), static_cast<const pt_triangle*>(c->tris.p),
+                              jobs.data(), uint32_t(jobs.size()), scratch.p, sb, static_cast<WNode*>(wn.p),
+                              static_cast<float4*>(c->tri_geo_w.p), static_cast<uint32_t*>(ctr.p), res.data(),
+                              c->stream);
+    scratch.release();
+    ctr.release();
+    if (e != hipSuccess) {
+        wn.release();
+        return c->hip_fail(e, "device bvh build");
+    }
+    uint32_t need = 0;
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        c->host_roots[job_slot[j]].wref = res[j].root_ref;
+        need = std::max(need, res[j].need);
+    }
+    const uint32_t wneed = stack_need(c->tlas_wneed, need);
+    if (wneed > kMaxStack) {
+        wn.release();
+        return c->fail(PT_ERR_UNSUPPORTED, "device-built BVH needs %u stack entries (at most %u)", wneed, kMaxStack);
+    }
+    c->wbnodes.release();
+    c->wbnodes = wn;
+    c->blas_wneed = need;
+    c->wstack_cap = wneed;
+    return c->upload(c->roots, c->host_roots.data(), c->host_roots.size() * sizeof(Root), "roots");
 }
 
 }  // namespace
@@ -252,6 +359,11 @@ int pt_set_option(pt_ctx* c, int key, int64_t value) {
                 return c->fail(PT_ERR_ARG, "unknown traversal %lld", (long long)value);
             c->traversal = int(value);
             return PT_OK;
+        case PT_OPT_BLAS_BUILD:
+            if (value != PT_BLAS_BUILD_REFERENCE && value != PT_BLAS_BUILD_DEVICE)
+                return c->fail(PT_ERR_ARG, "unknown BLAS build %lld", (long long)value);
+            c->blas_build = int(value);
+            return PT_OK;
         case PT_OPT_SCHEDULE:
             if (value != PT_SCHEDULE_SYNC && value != PT_SCHEDULE_REGEN)
                 return c->fail(PT_ERR_ARG, "unknown schedule %lld", (long long)value);
@@ -277,6 +389,8 @@ int pt_upload_scene(pt_ctx* c, const pt_scene_desc* d) {
     if ((rc = c->upload(c->tri_geo, geo.data(), geo.size() * sizeof(float4), "triangle geometry"))) return rc;
     if ((rc = c->upload(c->tris, tr, d->triangles.count * sizeof(pt_triangle), "triangles"))) return rc;
     if ((rc = c->upload(c->bvh_raw, d->bvh.ptr, d->bvh.count * sizeof(pt_bvh_node), "bvh"))) return rc;
+    c->tri_geo_w.release();
+    if (c->blas_build == PT_BLAS_BUILD_DEVICE && (rc = device_blas(c, d))) return rc;
     if ((rc = c->upload(c->lights, d->lights.ptr, d->lights.count * sizeof(pt_light), "lights"))) return rc;
     c->n_bvh = d->bvh.count;
     c->n_tris = d->triangles.count;

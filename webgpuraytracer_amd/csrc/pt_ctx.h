@@ -55,7 +55,9 @@ struct pt_ctx {
     std::map<uint32_t, uint32_t> blas_slot;   // node_offset -> root slot
     std::vector<pt::Root> host_roots;
     uint32_t tlas_depth = 0, blas_depth = 0, stack_cap = 2;
-    uint32_t blas_wneed = 0, wstack_cap = 2;
+    uint32_t blas_wneed = 0, wstack_cap = 2, tlas_wneed = 0;
+    int blas_build = PT_BLAS_BUILD_REFERENCE;  // where the wide BLAS trees come from
+    DevBuf tri_geo_w;                          // device-built trees: triangle records in their leaf order
     pt::DevScene dev{};
     // frame
     uint32_t width = 0, height = 0;
@@ -109,7 +111,7 @@ struct pt_ctx {
     pt::UNet* unet = nullptr;  // the learned denoiser, created on first use
     void release_all() {
         DevBuf* all[] = {&mats, &objs, &obj_root, &tnodes, &bnodes, &wtnodes, &wbnodes, &roots, &tlas_raw, &bvh_raw, &tri_geo, &tris,
-                         &planes, &meshes, &mesh_root, &lights, &sky, &cdf, &cdf_rows, &accum, &out, &gbuf, &dn_a,
+                         &planes, &meshes, &mesh_root, &lights, &sky, &cdf, &cdf_rows, &tri_geo_w, &accum, &out, &gbuf, &dn_a,
                          &dn_b, &dn_fin, &dn_guide, &stats, &seeds, &rays_in, &rays_aux, &rays_aux2, &rays_out,
                          &wf_o, &wf_d, &wf_t, &wf_r, &wf_meta, &wf_ha, &wf_hb, &wf_media, &wf_q0, &wf_q1, &wf_sq,
                          &wf_so, &wf_sd, &wf_stp, &wf_sfi, &wf_cnt, &mk_media, &mk_tiles, &mk_spill, &wf_spill};

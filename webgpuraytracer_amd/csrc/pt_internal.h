@@ -103,6 +103,26 @@ hipError_t launch_denoise(const DenoiseArgs& a, int stride_mode, hipStream_t str
 hipError_t launch_sky_cdf(const uint16_t* sky, uint32_t w, uint32_t h, float exposure, float* cdf, double* row_tmp,
                           double* total, hipStream_t stream);
 hipError_t launch_zero_u64(unsigned long long* p, uint32_t n, hipStream_t stream);
+
+// Bottom-level trees built on the device (pt_build.hip).  Job j covers the
+// triangles [t0, t0 + n) of the scene; its leaves store starts relative to
+// the triangle offset its objects use (leaf_base = t0 - that offset).
+struct BlasBuildJob {
+    uint32_t t0, n, leaf_base;
+};
+struct BlasBuildResult {
+    uint32_t root_ref;  // wide node index, or a leaf reference
+    uint32_t need;      // largest sum of (children - 1) along a path
+    bool leaf_root;
+};
+size_t blas_build_scratch_bytes(uint32_t max_n);
+// Writes the wide nodes of every job into wn (at most the jobs' total
+// triangle count of them, allocated from index 0) and the triangle records
+// of every job's range, in leaf order, into geo_out; ctr: 4 words of device
+// scratch.  Synchronous.
+hipError_t build_blas_device(const float4* geo, const pt_triangle* tris, const BlasBuildJob* jobs, uint32_t n_jobs,
+                             void* scratch, size_t scratch_bytes, WNode* wn, float4* geo_out, uint32_t* ctr,
+                             BlasBuildResult* res, hipStream_t stream);
 hipError_t launch_wavefront_batch(const DevScene& S, const WfArgs& a, bool count_nodes, hipStream_t stream);
 // Resident workgroups per CU of the wavefront traversal kernels (the larger of the two).
 hipError_t wf_trace_blocks_per_cu(const DevScene& S, bool count_nodes, bool wide, uint32_t stack_lds,

@@ -260,7 +260,11 @@ void build_tri_geo(const pt_scene_desc* d, std::vector<float4>& geo) {
     geo.resize(size_t(d->triangles.count) * 3);
     for (uint64_t i = 0; i < d->triangles.count; ++i) {
         const pt_triangle& t = tr[i];
-        geo[3 * i] = make_float4(t.v0[0], t.v0[1], t.v0[2], 0.f);
+        // w of the first record: the triangle's index (bits), the prim id of a hit
+        float idx;
+        const uint32_t u = uint32_t(i);
+        std::memcpy(&idx, &u, 4);
+        geo[3 * i] = make_float4(t.v0[0], t.v0[1], t.v0[2], idx);
         geo[3 * i + 1] = make_float4(t.v1[0] - t.v0[0], t.v1[1] - t.v0[1], t.v1[2] - t.v0[2], 0.f);
         geo[3 * i + 2] = make_float4(t.v2[0] - t.v0[0], t.v2[1] - t.v0[1], t.v2[2] - t.v0[2], 0.f);
     }
