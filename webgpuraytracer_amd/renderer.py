@@ -5,6 +5,7 @@ Method names and argument meanings follow tracer.js's Renderer
 through libpt.so, which fails loudly when the HIP library or a GPU is absent.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -198,11 +199,17 @@ class LearnedDenoiser:
     device.  Features are fp32 [n][h][w][10], targets [n][h][w][3]."""
 
     N_FEATURES = 10
+    # Weights trained by `python -m webgpuraytracer_amd.unet_train train --scenes 500 --variants 40
+    # --frame 960x540 --target-spp 1024 --input-spp 1,2,4,8 --steps 20000` on the small analytic
+    # scene family (12 primitives, emissive quad, procedural sky).
+    PRETRAINED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unet_small_scenes.npy")
 
-    def __init__(self, renderer, seed=0):
+    def __init__(self, renderer, seed=0, pretrained=False):
         self.r = renderer
         self.exposure = 1.0
         self.r._check(self.r.lib.pt_unet_init(self.r.ctx, seed))
+        if pretrained:
+            self.set_params(np.load(self.PRETRAINED, allow_pickle=False))
 
     def param_count(self):
         n = C.c_uint64()
