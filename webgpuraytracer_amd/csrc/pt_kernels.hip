@@ -418,7 +418,9 @@ __global__ void dn_atrous(DenoiseArgs A, const float4* __restrict__ in, float4* 
     const float sd2 = A.u.sigma_depth * A.u.sigma_depth + 1e-5f;
     const float sa2 = A.u.sigma_albedo * A.u.sigma_albedo + 1e-5f;
     const float sm2 = A.u.sigma_material * A.u.sigma_material + 1e-5f;
-#pragma unroll
+    // one row of taps per iteration: unrolling all 25 taps hoists their loads
+    // past the register file (scratch spills)
+#pragma unroll 1
     for (int dy = -2; dy <= 2; ++dy) {
         const int syp = min(max(y + dy * step, 0), H - 1);
 #pragma unroll
