@@ -110,7 +110,8 @@ def test_node_addon_loads_and_exports():
         pytest.skip("addon not built")
     js = ("const t=require('%s');"
           "const want=['create','uploadScene','updateObjects','resize','trace','gbuffer','denoise','readAccum',"
-          "'writeAccum','readOutput','readGBuffer','lastError'];"
+          "'writeAccum','readOutput','readGBuffer','readDenoised','lastError','setOption','unetInit','unetSetParams',"
+          "'unetDenoise','batchCreate','batchAdd','batchFrame','readSkyCdf'];"
           "for (const k of want) if (typeof t.addon[k] !== 'function') { console.log('missing '+k); process.exit(3); }"
           "const p=t.packParams({eye:[1,2,3],ray00:[0,0,0],ray10:[0,0,0],ray01:[0,0,0],ray11:[0,0,0],"
           "sampleNumber:5,width:7,height:9,seed:11,skyWidth:4,skyHeight:2,skyTotalLum:1.5,skyCdfIndex:3});"
@@ -118,6 +119,7 @@ def test_node_addon_loads_and_exports():
           "const dv=new DataView(p); if (dv.getUint32(12,true)!==5||dv.getUint32(76,true)!==11||"
           "dv.getUint32(104,true)!==3) process.exit(5);"
           "if (t.flagsWord({hasMeshes:true,hasSkybox:true,misSkybox:true})!==(32|512|1024)) process.exit(6);"
+          "const w=t.loadNpyF32(t.PRETRAINED); if (w.length!==1021904||!(Math.abs(w[0])<10)) process.exit(7);"
           "console.log('ok');") % os.path.join(ROOT, "webgpuraytracer_amd", "js", "tracer_native.js")
     r = subprocess.run(["node", "-e", js], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     assert r.returncode == 0 and "ok" in r.stdout, r.stdout

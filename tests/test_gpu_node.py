@@ -1,0 +1,108 @@
+"""The Node-API binding end to end on the device: the JavaScript host
+(tracer_native.js over pt_addon.node) renders, runs both denoisers and a
+batch frame, and every buffer it reads back is bit-identical to the same
+calls through the ctypes binding."""
+import ctypes as C
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from webgpuraytracer_amd import Batch, Denoiser, LearnedDenoiser, Renderer, SceneBuilder
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SHIM = os.path.join(ROOT, "webgpuraytracer_amd", "js", "tracer_native.js")
+ADDON = os.path.join(ROOT, "webgpuraytracer_amd", "js", "build", "pt_addon.node")
+
+JS = r"""
+const fs = require('fs');
+const t = require(process.argv[1]);
+const dir = process.argv[2];
+const cfg = JSON.parse(fs.readFileSync(dir + '/cfg.json'));
+const rd = (name, T) => { const b = fs.readFileSync(dir + '/' + name + '.bin'); return new T(b.buffer, b.byteOffset, b.byteLength / T.BYTES_PER_ELEMENT); };
+const sceneData = { mat: rd('materials', Uint8Array), object: rd('objects', Uint8Array), tlas: rd('tlas', Uint8Array),
+  mesh: rd('meshes', Uint8Array), bvh: rd('bvh', Uint8Array), triangle: rd('triangles', Uint8Array),
+  plane: rd('planes', Uint8Array), light: rd('lights', Uint8Array), flags: cfg.flags };
+const sky = { data: rd('sky', Uint16Array), width: cfg.skyWidth, height: cfg.skyHeight, exposure: cfg.skyExposure };
+const cdf = { skyCDF: rd('cdf', Float32Array), totalWeight: cfg.skyTotalLum };
+const out = (name, a) => fs.writeFileSync(dir + '/js_' + name + '.bin', Buffer.from(a.buffer, a.byteOffset, a.byteLength));
+const r = new t.NativeRenderer(0);
+r.setScene(sceneData, sky, cdf, cfg.bounces, true);
+r.resize(cfg.width, cfg.height);
+r.gbuffer(cfg.camera);
+r.render(cfg.camera, 4, 0);
+out('accum', r.readAccum());
+const dn = new t.NativeDenoiser(r);
+out('atrous', dn.execute(0));
+const ln = new t.NativeLearnedDenoiser(r);
+out('unet8', ln.execute());
+out('unet', ln.readResult());
+const r2 = new t.NativeRenderer(0), r3 = new t.NativeRenderer(0);
+r2.setScene(sceneData, sky, cdf, cfg.bounces); r2.resize(40, 24);
+r3.setScene(sceneData, sky, cdf, cfg.bounces); r3.resize(24, 40);
+const b = new t.NativeBatch(0);
+b.add(r2); b.add(r3);
+const st = b.frame([cfg.camera2, cfg.camera3], [2, 3], 0);
+out('b2', r2.readAccum()); out('b3', r3.readAccum());
+console.log(JSON.stringify(st.map((s) => s.extensionRays)));
+"""
+
+
+def _camera(p):
+    return {"eye": list(p.eye), "ray00": list(p.ray00), "ray10": list(p.ray10), "ray01": list(p.ray01),
+            "ray11": list(p.ray11), "exposure": p.exposure, "totalLightPower": p.total_light_power}
+
+
+def test_node_binding_matches_ctypes(tmp_path):
+    if not shutil.which("node") or not os.path.exists(ADDON):
+        pytest.skip("node or the addon is not available")
+    scene = SceneBuilder.config(1, seed=19)
+    d = scene.desc()
+    for name in ("materials", "objects", "tlas", "meshes", "bvh", "triangles", "planes", "lights"):
+        a = getattr(d, name)
+        (tmp_path / (name + ".bin")).write_bytes(C.string_at(a.ptr, a.count * a.stride) if a.count else b"")
+    (tmp_path / "sky.bin").write_bytes(C.string_at(d.sky_rgba16f, d.sky_width * d.sky_height * 8))
+    (tmp_path / "cdf.bin").write_bytes(C.string_at(d.sky_cdf, d.sky_cdf_count * 4))
+    w, h = 64, 48
+    cfg = {"flags": d.flags, "bounces": d.bounce_limit, "skyWidth": d.sky_width, "skyHeight": d.sky_height,
+           "skyExposure": d.sky_exposure, "skyTotalLum": d.sky_total_lum, "width": w, "height": h,
+           "camera": _camera(scene.params(w, h, 0, 0)), "camera2": _camera(scene.params(40, 24, 0, 0)),
+           "camera3": _camera(scene.params(24, 40, 0, 0))}
+    (tmp_path / "cfg.json").write_text(json.dumps(cfg))
+    js = subprocess.run(["node", "-e", JS, SHIM, str(tmp_path)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                        text=True, timeout=120)
+    assert js.returncode == 0, js.stdout
+    js_rays = json.loads(js.stdout.strip().splitlines()[-1])
+
+    def js_out(name, dtype):
+        return np.frombuffer((tmp_path / ("js_" + name + ".bin")).read_bytes(), dtype)
+
+    r = Renderer(0)
+    r.set_scene(scene, isgbuf=True)
+    r.resize(w, h)
+    r.gbuffer()
+    r.render(spp=4)
+    assert np.array_equal(js_out("accum", np.float32), r.read_accum().ravel())
+    assert np.array_equal(js_out("atrous", np.uint8), Denoiser(r).execute(0).ravel())
+    ln = LearnedDenoiser(r, pretrained=True)
+    assert np.array_equal(js_out("unet8", np.uint8), ln.execute().ravel())
+    assert np.array_equal(js_out("unet", np.float32), ln.read_result().ravel())
+    r2, r3 = Renderer(0), Renderer(0)
+    r2.set_scene(scene)
+    r2.resize(40, 24)
+    r3.set_scene(scene)
+    r3.resize(24, 40)
+    b = Batch(0)
+    b.add(r2)
+    b.add(r3)
+    st = b.frame([2, 3], base_seeds=[0, 0])
+    assert js_rays == [s.extension_rays for s in st]
+    assert np.array_equal(js_out("b2", np.float32), r2.read_accum().ravel())
+    assert np.array_equal(js_out("b3", np.float32), r3.read_accum().ravel())
+    for x in (b, r2, r3, r):
+        x.close()

@@ -16,6 +16,8 @@
 #include <vector>
 
 #include "pt_abi.h"
+#include "pt_batch.h"
+#include "pt_unet.h"
 
 namespace {
 
@@ -306,9 +308,190 @@ napi_value read_into(napi_env env, napi_callback_info info) {
     return argv[1];
 }
 int rd_accum(pt_ctx* c, void* d, uint64_t n) { return pt_read_accum(c, static_cast<float*>(d), n); }
+int rd_denoised(pt_ctx* c, void* d, uint64_t n) { return pt_read_denoised(c, static_cast<float*>(d), n); }
 int rd_output(pt_ctx* c, void* d, uint64_t n) { return pt_read_output(c, static_cast<uint8_t*>(d), n); }
 int rd_gbuf(pt_ctx* c, void* d, uint64_t n) { return pt_read_gbuffer(c, static_cast<pt_gbuffer_texel*>(d), n); }
 int wr_accum(pt_ctx* c, void* d, uint64_t n) { return pt_write_accum(c, static_cast<const float*>(d), n); }
+
+// readSkyCdf(ctx, Float32Array of (h + 1) + h (w + 1) floats) -> total weight:
+// the CDF the device built when the scene came without one
+napi_value read_sky_cdf(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    if (!get_args(env, info, 2, argv)) return nullptr;
+    pt_ctx* ctx = get_ctx(env, argv[0]);
+    if (!ctx) return nullptr;
+    void* data;
+    size_t n;
+    if (!get_bytes(env, argv[1], &data, &n) || n % 4) {
+        napi_throw_type_error(env, nullptr, "destination must be a Float32Array");
+        return nullptr;
+    }
+    float total = 0.f;
+    int rc = pt_read_sky_cdf(ctx, static_cast<float*>(data), n / 4, &total);
+    if (rc) return throw_pt(env, ctx, rc, "pt_read_sky_cdf");
+    napi_value r;
+    napi_create_double(env, double(total), &r);
+    return r;
+}
+
+// setOption(ctx, key, value): pt_set_option (the PT_OPT_* keys)
+napi_value set_option(napi_env env, napi_callback_info info) {
+    napi_value argv[3];
+    if (!get_args(env, info, 3, argv)) return nullptr;
+    pt_ctx* ctx = get_ctx(env, argv[0]);
+    if (!ctx) return nullptr;
+    int32_t key = 0;
+    int64_t value = 0;
+    napi_get_value_int32(env, argv[1], &key);
+    napi_get_value_int64(env, argv[2], &value);
+    int rc = pt_set_option(ctx, key, value);
+    if (rc) return throw_pt(env, ctx, rc, "pt_set_option");
+    return nullptr;
+}
+
+// The learned denoiser (pt_unet.h): unetInit(ctx, seed), unetSetParams(ctx,
+// Float32Array), unetDenoise(ctx, exposure); the result is read with
+// readOutput / readDenoised like the a-trous filter's.
+napi_value unet_init(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    if (!get_args(env, info, 2, argv)) return nullptr;
+    pt_ctx* ctx = get_ctx(env, argv[0]);
+    if (!ctx) return nullptr;
+    int64_t seed = 0;
+    napi_get_value_int64(env, argv[1], &seed);
+    int rc = pt_unet_init(ctx, uint64_t(seed));
+    if (rc) return throw_pt(env, ctx, rc, "pt_unet_init");
+    return nullptr;
+}
+
+napi_value unet_set_params(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    if (!get_args(env, info, 2, argv)) return nullptr;
+    pt_ctx* ctx = get_ctx(env, argv[0]);
+    if (!ctx) return nullptr;
+    void* data;
+    size_t n;
+    if (!get_bytes(env, argv[1], &data, &n) || n % 4) {
+        napi_throw_type_error(env, nullptr, "parameters must be a Float32Array");
+        return nullptr;
+    }
+    int rc = pt_unet_set_params(ctx, static_cast<const float*>(data), n / 4);
+    if (rc) return throw_pt(env, ctx, rc, "pt_unet_set_params");
+    return nullptr;
+}
+
+napi_value unet_denoise(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    if (!get_args(env, info, 2, argv)) return nullptr;
+    pt_ctx* ctx = get_ctx(env, argv[0]);
+    if (!ctx) return nullptr;
+    double exposure = 1.0;
+    napi_get_value_double(env, argv[1], &exposure);
+    int rc = pt_unet_denoise(ctx, float(exposure));
+    if (rc) return throw_pt(env, ctx, rc, "pt_unet_denoise");
+    return nullptr;
+}
+
+// Batches (pt_batch.h): batchCreate(device) -> External, batchAdd(batch, ctx),
+// batchFrame(batch, [params ArrayBuffer...], [passes...], [seeds Uint32Array|null...])
+// -> [{extensionRays, shadowRays, paths, kernelMs}...]
+void batch_cleanup(napi_env, void* data, void*) { pt_batch_destroy(static_cast<pt_batch*>(data)); }
+
+pt_batch* get_batch(napi_env env, napi_value v) {
+    void* p = nullptr;
+    if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+        napi_throw_type_error(env, nullptr, "first argument must be a pt batch");
+        return nullptr;
+    }
+    return static_cast<pt_batch*>(p);
+}
+
+napi_value throw_batch(napi_env env, pt_batch* b, int rc, const char* what) {
+    std::string msg = std::string(what) + " failed (" + std::to_string(rc) + "): " + pt_batch_last_error(b);
+    napi_throw_error(env, nullptr, msg.c_str());
+    return nullptr;
+}
+
+napi_value batch_create(napi_env env, napi_callback_info info) {
+    napi_value argv[1];
+    size_t argc = 1;
+    CHECK_NAPI(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+    int32_t device = 0;
+    if (argc >= 1) napi_get_value_int32(env, argv[0], &device);
+    pt_batch* b = nullptr;
+    int rc = pt_batch_create(&b, device);
+    if (rc) {
+        napi_throw_error(env, nullptr, ("pt_batch_create failed (" + std::to_string(rc) + ")").c_str());
+        return nullptr;
+    }
+    napi_value ext;
+    CHECK_NAPI(napi_create_external(env, b, batch_cleanup, nullptr, &ext));
+    return ext;
+}
+
+napi_value batch_add(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    if (!get_args(env, info, 2, argv)) return nullptr;
+    pt_batch* b = get_batch(env, argv[0]);
+    if (!b) return nullptr;
+    pt_ctx* ctx = get_ctx(env, argv[1]);
+    if (!ctx) return nullptr;
+    uint32_t idx = 0;
+    int rc = pt_batch_add(b, ctx, &idx);
+    if (rc) return throw_batch(env, b, rc, "pt_batch_add");
+    napi_value r;
+    napi_create_uint32(env, idx, &r);
+    return r;
+}
+
+napi_value batch_frame(napi_env env, napi_callback_info info) {
+    napi_value argv[4];
+    if (!get_args(env, info, 4, argv)) return nullptr;
+    pt_batch* b = get_batch(env, argv[0]);
+    if (!b) return nullptr;
+    uint32_t n = 0;
+    pt_batch_size(b, &n);
+    std::vector<pt_scene_params> params(n);
+    std::vector<uint32_t> passes(n, 0);
+    std::vector<const uint32_t*> seeds(n, nullptr);
+    for (uint32_t i = 0; i < n; ++i) {
+        napi_value pv, nv, sv;
+        void* data;
+        size_t len;
+        if (napi_get_element(env, argv[1], i, &pv) != napi_ok || !get_bytes(env, pv, &data, &len) ||
+            len != sizeof(pt_scene_params)) {
+            napi_throw_type_error(env, nullptr, "params must hold one 112-byte SceneParams per job");
+            return nullptr;
+        }
+        std::memcpy(&params[i], data, sizeof(pt_scene_params));
+        if (napi_get_element(env, argv[2], i, &nv) != napi_ok || napi_get_value_uint32(env, nv, &passes[i]) != napi_ok) {
+            napi_throw_type_error(env, nullptr, "passes must hold one count per job");
+            return nullptr;
+        }
+        if (napi_get_element(env, argv[3], i, &sv) == napi_ok && get_bytes(env, sv, &data, &len)) {
+            if (len < size_t(passes[i]) * 4) {
+                napi_throw_type_error(env, nullptr, "seeds must hold one u32 per pass");
+                return nullptr;
+            }
+            seeds[i] = static_cast<const uint32_t*>(data);
+        }
+    }
+    std::vector<pt_stats> st(n);
+    int rc = pt_batch_frame(b, params.data(), passes.data(), seeds.data(), st.data());
+    if (rc) return throw_batch(env, b, rc, "pt_batch_frame");
+    napi_value arr;
+    napi_create_array_with_length(env, n, &arr);
+    for (uint32_t i = 0; i < n; ++i) {
+        napi_value o;
+        napi_create_object(env, &o);
+        set_num(env, o, "extensionRays", double(st[i].extension_rays));
+        set_num(env, o, "shadowRays", double(st[i].shadow_rays));
+        set_num(env, o, "paths", double(st[i].paths));
+        set_num(env, o, "kernelMs", st[i].kernel_ms);
+        napi_set_element(env, arr, i, o);
+    }
+    return arr;
+}
 
 napi_value last_error(napi_env env, napi_callback_info info) {
     napi_value argv[1];
@@ -333,7 +516,16 @@ napi_value init(napi_env env, napi_value exports) {
         {"writeAccum", nullptr, read_into<wr_accum>, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"readOutput", nullptr, read_into<rd_output>, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"readGBuffer", nullptr, read_into<rd_gbuf>, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"readDenoised", nullptr, read_into<rd_denoised>, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"lastError", nullptr, last_error, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"setOption", nullptr, set_option, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"readSkyCdf", nullptr, read_sky_cdf, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"unetInit", nullptr, unet_init, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"unetSetParams", nullptr, unet_set_params, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"unetDenoise", nullptr, unet_denoise, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"batchCreate", nullptr, batch_create, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"batchAdd", nullptr, batch_add, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"batchFrame", nullptr, batch_frame, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
     };
     napi_define_properties(env, exports, sizeof(props) / sizeof(props[0]), props);
     return exports;

@@ -10,9 +10,18 @@
 // `sceneData` is exactly what Renderer.packSceneData returns (tracer.js:1819-1917);
 // the skybox is an HDRTexture-like {data, width, height, exposure} whose data
 // holds rgba16float bit patterns (a Uint16Array, or a Float16Array's buffer).
+const fs = require('fs');
 const path = require('path');
 
 const addon = require(path.join(__dirname, 'build', 'pt_addon.node'));
+
+// pt_set_option keys and values (pt_abi.h)
+const OPT = {
+  BACKEND: 1, COUNT_STATS: 2, WAVEFRONT_POOL: 3, SCHEDULE: 4, TRAVERSAL: 5, LDS_STACK: 6, WAVEFRONT_REFILL: 7,
+  BLAS_BUILD: 8,
+};
+const BACKEND = { MEGAKERNEL: 0, WAVEFRONT: 1 };
+const BLAS_BUILD = { REFERENCE: 0, DEVICE: 1 };
 
 // override constants shader.wgsl:2-12 <- packSceneData flags (tracer.js:1901-1913)
 const FLAG_BITS = {
@@ -81,8 +90,10 @@ class NativeRenderer {
   setScene(sceneData, sky, cdf, bounces, isgbuf = false) {
     this.isgbuf = isgbuf;
     addon.uploadScene(this.ctx, this._desc(sceneData, sky, cdf, bounces));
-    this.sky = { width: sky.width, height: sky.height, totalLum: cdf ? cdf.totalWeight : NaN,
-                 cdfIndex: sky.height + 1 };
+    // without a host CDF the device built one: its total weight goes into the uniforms
+    const total = cdf ? cdf.totalWeight
+      : addon.readSkyCdf(this.ctx, new Float32Array(sky.height + 1 + sky.height * (sky.width + 1)));
+    this.sky = { width: sky.width, height: sky.height, totalLum: total, cdfIndex: sky.height + 1 };
     this.lastScene = [sceneData, sky, cdf, bounces];
     if (this.width) this.resize(this.width, this.height);
   }
@@ -132,9 +143,12 @@ class NativeRenderer {
 
   gbuffer(camera) { addon.gbuffer(this.ctx, this.params(camera, 0)); }
 
+  setOption(key, value) { addon.setOption(this.ctx, key, value); }
+
   readAccum() { return addon.readAccum(this.ctx, new Float32Array(this.width * this.height * 4)); }
   readOutput() { return addon.readOutput(this.ctx, new Uint8Array(this.width * this.height * 4)); }
   readGBuffer() { return addon.readGBuffer(this.ctx, new Float32Array(this.width * this.height * 12)); }
+  readDenoised() { return addon.readDenoised(this.ctx, new Float32Array(this.width * this.height * 4)); }
 }
 
 // Denoiser (tracer.js:2418-2637) with the reference defaults.
@@ -172,4 +186,72 @@ class NativeDenoiser {
   }
 }
 
-module.exports = { NativeRenderer, NativeDenoiser, packParams, flagsWord, FLAG_BITS, addon };
+// A little-endian float32 .npy file (the package's trained weights) as a Float32Array.
+function loadNpyF32(file) {
+  const buf = fs.readFileSync(file);
+  if (buf.toString('latin1', 1, 6) !== 'NUMPY') throw new Error(file + ': not a .npy file');
+  const major = buf[6];
+  const hlen = major >= 2 ? buf.readUInt32LE(8) : buf.readUInt16LE(8);
+  const start = (major >= 2 ? 12 : 10) + hlen;
+  const header = buf.toString('latin1', major >= 2 ? 12 : 10, start);
+  if (!/'descr':\s*'<f4'/.test(header) || /'fortran_order':\s*True/.test(header))
+    throw new Error(file + ': expected a C-order little-endian float32 array');
+  const n = (buf.length - start) / 4;
+  const out = new Float32Array(n);
+  for (let i = 0; i < n; i++) out[i] = buf.readFloatLE(start + 4 * i);
+  return out;
+}
+
+const PRETRAINED = path.join(__dirname, '..', 'weights', 'unet_small_scenes.npy');
+
+// The learned alternative to NativeDenoiser for the same frame (pt_unet.h):
+// the renderer needs the G-buffer (isgbuf) and a gbuffer() pass, as the
+// a-trous filter does.
+class NativeLearnedDenoiser {
+  constructor(renderer, weights = PRETRAINED, seed = 0) {
+    this.r = renderer;
+    this.exposure = 1.0;
+    addon.unetInit(this.r.ctx, seed);
+    if (weights) addon.unetSetParams(this.r.ctx, weights instanceof Float32Array ? weights : loadNpyF32(weights));
+  }
+
+  execute() {
+    addon.unetDenoise(this.r.ctx, this.exposure);
+    return this.r.readOutput();
+  }
+
+  readResult() { return this.r.readDenoised(); }
+}
+
+// Several NativeRenderers rendered one progressive frame at a time as one
+// graph launch (pt_batch.h); each job keeps its own scene, frame and options.
+class NativeBatch {
+  constructor(device = 0) {
+    this.h = addon.batchCreate(device);
+    this.jobs = [];
+  }
+
+  add(renderer) {
+    this.jobs.push(renderer);
+    return addon.batchAdd(this.h, renderer.ctx);
+  }
+
+  // cameras[i], passes[i] for job i (0 skips it); seeds as NativeRenderer.render derives them
+  frame(cameras, passes, seedBase = 0) {
+    const params = [], seeds = [];
+    this.jobs.forEach((r, i) => {
+      const s = new Uint32Array(passes[i]);
+      for (let k = 0; k < passes[i]; k++) s[k] = (Math.imul(r.frame + k, 0x85EBCA77) + seedBase) >>> 0;
+      params.push(r.params(cameras[i], passes[i] ? s[0] : 0));
+      seeds.push(passes[i] ? s : null);
+    });
+    const st = addon.batchFrame(this.h, params, passes, seeds);
+    this.jobs.forEach((r, i) => { r.frame += passes[i]; });
+    return st;
+  }
+}
+
+module.exports = {
+  NativeRenderer, NativeDenoiser, NativeLearnedDenoiser, NativeBatch, packParams, flagsWord, loadNpyF32,
+  FLAG_BITS, OPT, BACKEND, BLAS_BUILD, PRETRAINED, addon,
+};
