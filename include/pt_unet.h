@@ -67,6 +67,29 @@ int pt_unet_train(pt_ctx* ctx, uint32_t steps, uint32_t batch, float lr, uint64_
  * (fp32) and pt_read_output (rgba8, tonemapped with `exposure`) read. */
 int pt_unet_denoise(pt_ctx* ctx, float exposure);
 
+/* Inference precision of pt_unet_denoise and pt_unet_forward.  PT_UNET_FP8
+ * calibrates on the context's current frame (accumulation buffer + G-buffer,
+ * as pt_unet_denoise reads it) and from then on runs the 17 inner
+ * convolutions on fp8 products (OCP e4m3, fp32 accumulation; the first and
+ * the output convolution stay bf16): each layer input is quantized
+ * per channel as x / s_a[ci], s_a = the calibration frame's max |x| / 224
+ * (one binade of headroom; larger values saturate at 448); the weights are
+ * stored in e4m3 as w * s_a[ci] / s_w[co], s_w = that row's max / 448.
+ * Activations stay bf16 between layers.  Training (pt_unet_train,
+ * pt_unet_loss_grad) always runs in bf16; pt_unet_init, pt_unet_set_params
+ * and pt_unet_train return inference to PT_UNET_BF16. */
+enum { PT_UNET_BF16 = 0, PT_UNET_FP8 = 1 };
+int pt_unet_set_precision(pt_ctx* ctx, int precision);
+
+/* The fp8 state (PT_UNET_FP8 only), all 19 layers in network order (the
+ * first and last layers' entries are computed but not used): the input
+ * scales s_a (sum of the layers' input channels, 1472 floats), the weight
+ * scales s_w (sum of output channels, 1232 floats) and the e4m3 weights
+ * [9][cout][cin] per layer (1,020,672 bytes), with the padded channel counts
+ * (input 16, output 16). */
+int pt_unet_fp8_state(pt_ctx* ctx, float* act_scales, uint64_t n_act, float* w_scales, uint64_t n_w,
+                      uint8_t* weights, uint64_t n_bytes);
+
 #ifdef __cplusplus
 }
 #endif

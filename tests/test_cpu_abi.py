@@ -111,7 +111,7 @@ def test_node_addon_loads_and_exports():
     js = ("const t=require('%s');"
           "const want=['create','uploadScene','updateObjects','resize','trace','gbuffer','denoise','readAccum',"
           "'writeAccum','readOutput','readGBuffer','readDenoised','lastError','setOption','unetInit','unetSetParams',"
-          "'unetDenoise','batchCreate','batchAdd','batchFrame','readSkyCdf'];"
+          "'unetDenoise','unetSetPrecision','batchCreate','batchAdd','batchFrame','readSkyCdf'];"
           "for (const k of want) if (typeof t.addon[k] !== 'function') { console.log('missing '+k); process.exit(3); }"
           "const p=t.packParams({eye:[1,2,3],ray00:[0,0,0],ray10:[0,0,0],ray01:[0,0,0],ray11:[0,0,0],"
           "sampleNumber:5,width:7,height:9,seed:11,skyWidth:4,skyHeight:2,skyTotalLum:1.5,skyCdfIndex:3});"

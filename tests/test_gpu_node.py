@@ -42,6 +42,10 @@ out('atrous', dn.execute(0));
 const ln = new t.NativeLearnedDenoiser(r);
 out('unet8', ln.execute());
 out('unet', ln.readResult());
+ln.setPrecision(true);
+ln.execute();
+out('fp8', ln.readResult());
+ln.setPrecision(false);
 const r2 = new t.NativeRenderer(0), r3 = new t.NativeRenderer(0);
 r2.setScene(sceneData, sky, cdf, cfg.bounces); r2.resize(40, 24);
 r3.setScene(sceneData, sky, cdf, cfg.bounces); r3.resize(24, 40);
@@ -92,6 +96,10 @@ def test_node_binding_matches_ctypes(tmp_path):
     ln = LearnedDenoiser(r, pretrained=True)
     assert np.array_equal(js_out("unet8", np.uint8), ln.execute().ravel())
     assert np.array_equal(js_out("unet", np.float32), ln.read_result().ravel())
+    ln.set_precision(True)
+    ln.execute()
+    assert np.array_equal(js_out("fp8", np.float32), ln.read_result().ravel())
+    ln.set_precision(False)
     r2, r3 = Renderer(0), Renderer(0)
     r2.set_scene(scene)
     r2.resize(40, 24)

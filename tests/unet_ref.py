@@ -109,3 +109,67 @@ def layer_slices():
         out.append((name, slice(o, o + 9 * co * ci), slice(o + 9 * co * ci, o + 9 * co * ci + co)))
         o += 9 * co * ci + co
     return out
+
+
+def _bf16(t):
+    return t.to(torch.bfloat16).float()
+
+
+def _e4m3(t):
+    """Quantize to OCP e4m3 (round to nearest even, saturating at +-448) and back."""
+    return torch.clamp(t, -448.0, 448.0).to(torch.float8_e4m3fn).float()
+
+
+def forward_fp8(params, feat, act_scales, w_scales, w8):
+    """The fp8 inference path as the device runs it: every inner convolution's
+    input (bf16) quantized per channel as e4m3(clamp(x * (1 / s_a), +-448)), the
+    e4m3 weights (bytes, [9][cout][cin] per layer), the fp32 product times
+    s_w[co] plus the bias, ReLU, stored as bf16 (pooling averages bf16 values
+    into bf16); the first and the output convolution run on bf16 weights and
+    the last layer's output stays fp32.  params: unflatten()
+    output (for the biases); returns [n][3][h][w] (log space)."""
+    table = layer_table()
+    w8 = torch.from_numpy(np.asarray(w8, np.uint8).copy())
+    sa = torch.as_tensor(np.asarray(act_scales, np.float32))
+    sw = torch.as_tensor(np.asarray(w_scales, np.float32))
+    offs = [0, 0, 0]
+    li = 0
+
+    def conv(t):
+        nonlocal li
+        ci, co, relu = table[li]
+        inv = 1.0 / sa[offs[0]:offs[0] + ci]
+        wq = w8[offs[2]:offs[2] + 9 * co * ci].view(torch.float8_e4m3fn).float()
+        wq = wq.reshape(3, 3, co, ci).permute(2, 3, 0, 1).contiguous()
+        s = sw[offs[1]:offs[1] + co]
+        offs[0] += ci
+        offs[1] += co
+        offs[2] += 9 * co * ci
+        last = li == len(table) - 1
+        if li == 0 or last:  # the first and the output convolution stay bf16
+            y = F.conv2d(t, _bf16(params[li][0]), params[li][1], padding=1)
+        else:
+            xq = _e4m3(t * inv[None, :, None, None])
+            y = F.conv2d(xq, wq, None, padding=1) * s[None, :, None, None] + params[li][1][None, :, None, None]
+        li += 1
+        if relu:
+            y = F.relu(y)
+        return y if last else _bf16(y)
+
+    x = _bf16(pack_input(feat))
+    skips = []
+    for _ in range(4):
+        x = conv(conv(x))
+        skips.append(x)
+        x = _bf16(F.avg_pool2d(x, 2))
+    x = conv(conv(x))
+    for lvl in (3, 2, 1, 0):
+        up = x.repeat_interleave(2, dim=2).repeat_interleave(2, dim=3)
+        x = conv(conv(torch.cat([up, skips[lvl]], dim=1)))
+    return conv(x)[:, :3]
+
+
+def fp8_sizes():
+    """(input scales, output scales, weight bytes) of the fp8 state."""
+    t = layer_table()
+    return sum(ci for ci, _, _ in t), sum(co for _, co, _ in t), sum(9 * ci * co for ci, co, _ in t)

@@ -134,6 +134,7 @@ PT_OPT_BACKEND, PT_OPT_COUNT_STATS, PT_OPT_WAVEFRONT_POOL, PT_OPT_SCHEDULE, PT_O
 PT_BLAS_BUILD_REFERENCE, PT_BLAS_BUILD_DEVICE = 0, 1
 PT_BACKEND_MEGAKERNEL, PT_BACKEND_WAVEFRONT = 0, 1
 PT_UNET_INPUT, PT_UNET_TARGET = 0, 1
+PT_UNET_BF16, PT_UNET_FP8 = 0, 1
 PT_SCHEDULE_SYNC, PT_SCHEDULE_REGEN = 0, 1
 PT_TRAVERSAL_REFERENCE, PT_TRAVERSAL_WIDE = 0, 1
 
@@ -181,6 +182,8 @@ PT_FUNCS = {
     "pt_unet_dataset_capture": (C.c_int, [vp, u32, C.c_int, u32, u32]),
     "pt_unet_train": (C.c_int, [vp, u32, u32, f32, u64, vp]),
     "pt_unet_denoise": (C.c_int, [vp, f32]),
+    "pt_unet_set_precision": (C.c_int, [vp, C.c_int]),
+    "pt_unet_fp8_state": (C.c_int, [vp, vp, u64, vp, u64, vp, u64]),
     # many jobs per graph launch (pt_batch.h)
     "pt_batch_create": (C.c_int, [P(vp), C.c_int]),
     "pt_batch_destroy": (None, [vp]),

@@ -36,6 +36,20 @@ constexpr uint32_t kMaxCout = 128;
 hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bias, const Tens& out, uint32_t cout,
                    Dims d, bool relu, float* out_f32, uint32_t f32_channels, hipStream_t s);
 
+// The same convolution for inference on fp8 (OCP e4m3) operands with fp32
+// accumulation: the input (bf16, as above) is quantized per input channel as
+// clamp(x * inv_scale[ci], -448, 448) rounded to nearest even; w is e4m3
+// [9][cout][cin] (cin-contiguous rows, 16-byte aligned, with the input
+// scales folded in), and the accumulated product is multiplied by
+// wscale[co] before the bias.  inv_scale has cin floats (16-byte aligned).
+hipError_t conv3x3_fp8(const Tens& in, uint32_t cin, const uint8_t* w, const float* inv_scale, const float* wscale,
+                       const float* bias, const Tens& out, uint32_t cout, Dims d, bool relu, float* out_f32,
+                       uint32_t f32_channels, hipStream_t s);
+
+// amax[ch] = max(amax[ch], max over P pixels of |x[p][ch]|) for ch < c
+// (c a multiple of 8, at most 256), the maxima as float bit patterns.
+hipError_t chan_amax(const Tens& x, uint32_t c, uint32_t P, uint32_t* amax, hipStream_t s);
+
 // Backward staging for a layer with c output channels and output gradient
 // ga: dz = ga * (a > 0) when a is given (ReLU output), else ga; written NHWC
 // contiguous (dz, [pixel][c], the data-gradient input) and channel-major

@@ -197,6 +197,264 @@ hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bia
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------- conv fp8 --
+// The inference variant on v_mfma_scale_f32_16x16x128_f8f6f4 (OCP e4m3
+// operands, unit block scales, fp32 accumulation).  The input stays bf16 in
+// memory and is quantized while it is staged: x * inv_scale[c], clamped to
+// +-448 (v_cvt_pk_fp8_f32 turns larger magnitudes into NaN rather than
+// saturating), rounded to nearest even.  The weights arrive quantized
+// ([tap][cout][cin] e4m3 with the input scales folded in) with one scale per
+// output channel, applied in the epilogue with the bias.
+//
+// Staged per slice of SL = 32 or 64 input channels (32 when the layer has
+// no more): the halo as [16-channel group][pixel] pieces of 16 bytes, the
+// weights as [slot][half][co].  A slot is one tap of one 32-channel group of
+// the slice (9 or 18 per slice); a K-step of 128 is four slots, one per
+// 16-lane group, so lane group g of step s holds slot 4s + g: 32 bytes = 32
+// channels of that tap, in the same order for both operands (the instruction
+// pairs A's and B's K positions the same way, checked on the device).  Three
+// or five steps per slice, the last one partly empty (zero operands).
+
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+template <int NCT, int SL>
+struct ConvSmem8 {
+    i32x4 halo[(SL / 16) * kHaloPix];   // [16-channel group][pixel]
+    i32x4 w[9 * (SL / 16) * NCT * 16];  // [slot][half][co]
+};
+
+__device__ __forceinline__ uint32_t e4m3x4(float a, float b, float c, float d) {
+    a = fminf(fmaxf(a, -448.0f), 448.0f);
+    b = fminf(fmaxf(b, -448.0f), 448.0f);
+    c = fminf(fmaxf(c, -448.0f), 448.0f);
+    d = fminf(fmaxf(d, -448.0f), 448.0f);
+    const int lo = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+    return uint32_t(__builtin_amdgcn_cvt_pk_fp8_f32(c, d, lo, true));
+}
+
+template <int NCT, int SL, bool RELU, bool F32OUT>
+__global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, const uint8_t* __restrict__ wq,
+                                                       const float* __restrict__ inv_scale,
+                                                       const float* __restrict__ wscale,
+                                                       const float* __restrict__ bias, Tens out, uint32_t cout, Dims d,
+                                                       float* __restrict__ of32, uint32_t f32c) {
+    __shared__ ConvSmem8<NCT, SL> sm;
+    constexpr uint32_t kG16 = uint32_t(SL) / 16u, kG8 = uint32_t(SL) / 8u;  // 16- and 8-channel groups per slice
+    constexpr uint32_t kSlots = 9u * (uint32_t(SL) / 32u), kSteps = (kSlots + 3u) / 4u;
+    constexpr uint32_t kHalo8Pieces = kHaloPix * kG8;                        // 8-channel pieces of the halo
+    constexpr uint32_t kCo = uint32_t(NCT) * 16u;
+    constexpr uint32_t kWPieces = 9u * kCo * kG16;                           // 16-byte pieces: (tap, co, 16-ch group)
+    constexpr uint32_t kHaloPer = (kHalo8Pieces + kBlock - 1u) / kBlock;
+    constexpr uint32_t kWPer = (kWPieces + kBlock - 1u) / kBlock;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6, kq = lane >> 4;
+    const uint32_t tiles_x = (d.w + kTile - 1u) / kTile, tiles_y = (d.h + kTile - 1u) / kTile;
+    const uint32_t img = blockIdx.x / (tiles_x * tiles_y), trem = blockIdx.x % (tiles_x * tiles_y);
+    const int ty0 = int(trem / tiles_x * kTile), tx0 = int(trem % tiles_x * kTile);
+    const uint32_t co0 = blockIdx.y * kCo;
+    const uint32_t nct = min(uint32_t(NCT), (cout - co0) / 16u);
+    // halo piece k of this thread: pixel piece / 8, channels (piece % 8) * 8 .. + 7 of the slice
+    const bf16* hsrc[kHaloPer];
+    bool hval[kHaloPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kHaloPer; ++k) {
+        const uint32_t piece = tid + k * kBlock;
+        const uint32_t j = piece / kG8;
+        const int gy = ty0 + int(j / kHalo) - 1, gx = tx0 + int(j % kHalo) - 1;
+        hval[k] = piece < kHalo8Pieces && gy >= 0 && gy < int(d.h) && gx >= 0 && gx < int(d.w);
+        const size_t sp = hval[k] ? (size_t(img) * d.h + uint32_t(gy)) * d.w + uint32_t(gx) : 0;
+        hsrc[k] = in.p + sp * in.stride + in.off + (piece % kG8) * 8u;
+    }
+    u32x2 hreg[kHaloPer];
+    i32x4 wreg[kWPer];
+    auto fetch = [&](uint32_t c0) {
+#pragma unroll
+        for (uint32_t k = 0; k < kHaloPer; ++k) {
+            const uint32_t c = c0 + ((tid + k * kBlock) % kG8) * 8u;
+            hreg[k] = u32x2{0u, 0u};
+            if (hval[k] && c < cin) {
+                const bf16x8 v = load8(hsrc[k] + c0);
+                const f32x4 s0 = *reinterpret_cast<const f32x4*>(inv_scale + c);
+                const f32x4 s1 = *reinterpret_cast<const f32x4*>(inv_scale + c + 4u);
+                hreg[k] = u32x2{e4m3x4(f(v[0]) * s0[0], f(v[1]) * s0[1], f(v[2]) * s0[2], f(v[3]) * s0[3]),
+                                e4m3x4(f(v[4]) * s1[0], f(v[5]) * s1[1], f(v[6]) * s1[2], f(v[7]) * s1[3])};
+            }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kWPer; ++k) {
+            const uint32_t piece = tid + k * kBlock;
+            const uint32_t g16 = piece % kG16, co = (piece / kG16) % kCo, tap = piece / (kG16 * kCo);
+            const uint32_t c = c0 + g16 * 16u;
+            const bool ok = piece < kWPieces && co0 + co < cout && c < cin;
+            wreg[k] = ok ? *reinterpret_cast<const i32x4*>(wq + (size_t(tap) * cout + co0 + co) * cin + c)
+                         : i32x4{0, 0, 0, 0};
+        }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (uint32_t k = 0; k < kHaloPer; ++k) {
+            const uint32_t piece = tid + k * kBlock;
+            if (piece < kHalo8Pieces) {
+                const uint32_t e = piece % kG8;  // 8-channel group: 16-channel group e / 2, half e % 2
+                u32x2* dst = reinterpret_cast<u32x2*>(&sm.halo[(e / 2u) * kHaloPix + piece / kG8]) + (e % 2u);
+                *dst = hreg[k];
+            }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kWPer; ++k) {
+            const uint32_t piece = tid + k * kBlock;
+            if (piece < kWPieces) {
+                const uint32_t g16 = piece % kG16, co = (piece / kG16) % kCo, tap = piece / (kG16 * kCo);
+                const uint32_t slot = tap * (kG16 / 2u) + g16 / 2u;
+                sm.w[(slot * 2u + g16 % 2u) * kCo + co] = wreg[k];
+            }
+        }
+    };
+    f32x4 acc[4][NCT];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int t = 0; t < NCT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const i32x8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
+    fetch(0);
+    for (uint32_t c0 = 0; c0 < cin; c0 += uint32_t(SL)) {
+        __syncthreads();  // the previous slice's reads are done
+        stash();
+        __syncthreads();
+        if (c0 + uint32_t(SL) < cin) fetch(c0 + uint32_t(SL));
+#pragma unroll
+        for (uint32_t s = 0; s < kSteps; ++s) {
+            const uint32_t q = s * 4u + kq;                 // this lane group's slot
+            const bool valid = q < kSlots;
+            const uint32_t qa = valid ? q : 0u;             // address a real slot either way
+            const uint32_t tap = qa / (kG16 / 2u), g32 = qa % (kG16 / 2u);
+            const uint32_t dy = tap / 3u, dx = tap % 3u;
+            i32x8 a[4], bw[NCT];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const uint32_t hp = (wv * 4u + uint32_t(m) + dy) * kHalo + (lane & 15u) + dx;
+                const i32x4 lo = sm.halo[(2u * g32) * kHaloPix + hp], hi = sm.halo[(2u * g32 + 1u) * kHaloPix + hp];
+                a[m] = valid ? __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7) : zero;
+            }
+#pragma unroll
+            for (int t = 0; t < NCT; ++t) {
+                const uint32_t co = uint32_t(t) * 16u + (lane & 15u);
+                const i32x4 lo = sm.w[(qa * 2u) * kCo + co], hi = sm.w[(qa * 2u + 1u) * kCo + co];
+                bw[t] = valid ? __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7) : zero;
+            }
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int t = 0; t < NCT; ++t)
+                    if (uint32_t(t) < nct)
+                        acc[m][t] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a[m], bw[t], acc[m][t], 0, 0, 0,
+                                                                                     127, 0, 127);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NCT; ++t) {
+        if (uint32_t(t) >= nct) continue;
+        const uint32_t co = co0 + uint32_t(t) * 16u + (lane & 15u);
+        const float sc = wscale[co], bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int y = ty0 + int(wv * 4u) + m;
+            if (y >= int(d.h)) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int x = tx0 + int(kq * 4u) + r;
+                if (x >= int(d.w)) continue;
+                const size_t op = (size_t(img) * d.h + uint32_t(y)) * d.w + uint32_t(x);
+                float v = fmaf(acc[m][t][r], sc, bv);
+                if (RELU) v = fmaxf(v, 0.0f);
+                if (F32OUT) {
+                    if (co < f32c) of32[op * f32c + co] = v;
+                } else {
+                    out.p[op * out.stride + out.off + co] = b(v);
+                }
+            }
+        }
+    }
+}
+
+template <int NCT, int SL>
+static void conv8_launch_sl(const Tens& in, uint32_t cin, const uint8_t* w, const float* inv_scale,
+                            const float* wscale, const float* bias, const Tens& out, uint32_t cout, Dims d, bool relu,
+                            float* of32, uint32_t f32c, hipStream_t s) {
+    const uint32_t tiles = d.n * ((d.h + kTile - 1u) / kTile) * ((d.w + kTile - 1u) / kTile);
+    const uint32_t groups = (cout / 16u + NCT - 1u) / NCT;
+    const dim3 grid(tiles, groups);
+    if (of32)
+        hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, false, true>), grid, dim3(kBlock), 0, s, in, cin, w, inv_scale,
+                           wscale, bias, out, cout, d, of32, f32c);
+    else if (relu)
+        hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, true, false>), grid, dim3(kBlock), 0, s, in, cin, w, inv_scale,
+                           wscale, bias, out, cout, d, of32, f32c);
+    else
+        hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, false, false>), grid, dim3(kBlock), 0, s, in, cin, w, inv_scale,
+                           wscale, bias, out, cout, d, of32, f32c);
+}
+
+template <int NCT>
+static void conv8_launch(const Tens& in, uint32_t cin, const uint8_t* w, const float* inv_scale, const float* wscale,
+                         const float* bias, const Tens& out, uint32_t cout, Dims d, bool relu, float* of32,
+                         uint32_t f32c, hipStream_t s) {
+    if (cin <= 32u)
+        conv8_launch_sl<NCT, 32>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, of32, f32c, s);
+    else
+        conv8_launch_sl<NCT, 64>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, of32, f32c, s);
+}
+
+hipError_t conv3x3_fp8(const Tens& in, uint32_t cin, const uint8_t* w, const float* inv_scale, const float* wscale,
+                       const float* bias, const Tens& out, uint32_t cout, Dims d, bool relu, float* out_f32,
+                       uint32_t f32_channels, hipStream_t s) {
+    if (cin == 0 || cin % 16u || cout % 16u || cout == 0 || cout > kMaxCout || in.stride % 8u || in.off % 8u ||
+        !w || !inv_scale || !wscale || (reinterpret_cast<uintptr_t>(w) | reinterpret_cast<uintptr_t>(inv_scale)) % 16u)
+        return hipErrorInvalidValue;
+    if (d.pixels() == 0) return hipSuccess;
+    const uint32_t ct = cout / 16u;
+    const uint32_t nct = ct <= 4u ? ct : (ct % 4u == 0 ? 4u : (ct % 3u == 0 ? 3u : 4u));
+    switch (nct) {
+        case 1: conv8_launch<1>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
+        case 2: conv8_launch<2>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
+        case 3: conv8_launch<3>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
+        default: conv8_launch<4>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
+    }
+    return hipGetLastError();
+}
+
+// amax[ch] = max(amax[ch], max over pixels of |x[p][ch]|), as float bits (a
+// non-negative float orders like its bit pattern); per block in LDS, then one
+// device atomic per channel.
+constexpr uint32_t kMaxAmaxC = 256;
+
+__global__ void __launch_bounds__(kBlock) chan_amax_k(Tens x, uint32_t c, uint32_t P, uint32_t* __restrict__ amax) {
+    __shared__ uint32_t sm[kMaxAmaxC];
+    for (uint32_t i = threadIdx.x; i < c; i += kBlock) sm[i] = 0u;
+    __syncthreads();
+    const uint32_t groups = c / 8u;
+    const size_t n = size_t(P) * groups;
+    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += size_t(gridDim.x) * kBlock) {
+        const size_t p = i / groups;
+        const uint32_t g = uint32_t(i % groups);
+        const bf16x8 v = load8(x.p + p * x.stride + x.off + g * 8u);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) atomicMax(&sm[g * 8u + uint32_t(j)], __float_as_uint(fabsf(f(v[j]))));
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < c; i += kBlock) atomicMax(&amax[i], sm[i]);
+}
+
+hipError_t chan_amax(const Tens& x, uint32_t c, uint32_t P, uint32_t* amax, hipStream_t s) {
+    if (c == 0 || c % 8u || c > kMaxAmaxC || x.stride % 8u || x.off % 8u) return hipErrorInvalidValue;
+    if (P == 0) return hipSuccess;
+    const size_t n = size_t(P) * (c / 8u);
+    const uint32_t blocks = uint32_t(std::min<size_t>(1024, (n + kBlock - 1u) / kBlock));
+    hipLaunchKernelGGL(chan_amax_k, dim3(blocks), dim3(kBlock), 0, s, x, c, P, amax);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------- backward staging --
 // The weight gradient reduces over pixels, so its MFMA operands need 8
 // consecutive pixels of one channel per lane: the backward pass stages dz and

@@ -9,10 +9,13 @@
 //   output:         conv(3), linear, in log(1 + radiance)
 // skip_L is written straight into the decoder's concatenation buffer
 // cat_L = [up | skip] (NHWC, so concatenation is a channel offset).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <vector>
 
+#include "fp8.h"
 #include "nn_internal.h"
 #include "pt_ctx.h"
 #include "pt_unet.h"
@@ -35,6 +38,7 @@ struct Layer {
     uint32_t cin_real, cout_real;  // fan-in / used outputs for initialisation
     size_t w_off, b_off;           // into the flat fp32 parameter array
     size_t wbf_off;                // into the bf16 weight arrays (same size as the weights)
+    size_t a_off, s_off;           // into the per-input-channel and per-output-channel fp8 scale arrays
     bool relu;
 };
 
@@ -58,6 +62,13 @@ struct UNet {
     DevBuf params, grads, m, v;    // fp32, n_params
     DevBuf wbf, wdt;               // bf16 weights [9][cout][cin], transposed slices [9][ci][cout]
     uint32_t adam_t = 0;
+    // fp8 inference: e4m3 weights (layout of wbf), input-channel inverse scales, output-channel scales
+    size_t n_act = 0, n_out = 0;   // sums of cin and cout over the layers
+    bool fp8 = false;              // pt_unet_denoise / pt_unet_forward run on fp8
+    uint32_t* calib = nullptr;     // set while a forward pass records per-channel input maxima
+    DevBuf w8, inv_sa, sw, amax;
+    std::vector<float> h_sa, h_sw;
+    std::vector<uint8_t> h_w8;
     // activations for the current dims
     Dims dims[5] = {};
     bool have_acts = false;
@@ -71,7 +82,7 @@ struct UNet {
     std::vector<DevBuf*> all_bufs() {
         std::vector<DevBuf*> bufs{&params, &grads, &m, &v, &wbf, &wdt, &in0, &t4A, &b4, &pred, &g_t4A, &g_b4,
                                   &g_pred, &dz, &dzc, &xs, &target, &feat, &loss, &wpart, &bpart, &ds_in,
-                                  &ds_tg};
+                                  &ds_tg, &w8, &inv_sa, &sw, &amax};
         for (int i = 0; i < 4; ++i) {
             for (DevBuf* b : {&tA[i], &cat[i], &dA[i], &dd[i], &g_tA[i], &g_cat[i], &g_dA[i], &g_dd[i]})
                 bufs.push_back(b);
@@ -85,7 +96,7 @@ struct UNet {
     void release_acts() {
         for (DevBuf* b : all_bufs())
             if (b != &params && b != &grads && b != &m && b != &v && b != &wbf && b != &wdt && b != &ds_in &&
-                b != &ds_tg)
+                b != &ds_tg && b != &w8 && b != &inv_sa && b != &sw && b != &amax)
                 b->release();
         have_acts = false;
     }
@@ -117,9 +128,13 @@ void build_layers(UNet& u) {
         {kC[1] + kC[0], kC[0], kC[1] + kC[0], kC[0], true}, {kC[0], kC[0], kC[0], kC[0], true},
         {kC[0], kOutC, kC[0], 3, false},
     };
-    size_t off = 0, woff = 0;
+    size_t off = 0, woff = 0, aoff = 0, soff = 0;
     for (uint32_t i = 0; i < kLayers; ++i) {
         Layer& l = u.L[i];
+        l.a_off = aoff;
+        aoff += spec[i].cin;
+        l.s_off = soff;
+        soff += spec[i].cout;
         l.cin = spec[i].cin;
         l.cout = spec[i].cout;
         l.cin_real = spec[i].cin_real;
@@ -134,6 +149,8 @@ void build_layers(UNet& u) {
     }
     u.n_params = off;
     u.n_wbf = woff;
+    u.n_act = aoff;
+    u.n_out = soff;
 }
 
 // He initialisation; padded input channels and unused outputs stay zero.
@@ -232,15 +249,30 @@ int alloc_acts(pt_ctx* c, UNet& u, Dims d0) {
     return PT_OK;
 }
 
-int conv_fwd(pt_ctx* c, UNet& u, uint32_t li, const Tens& in, const Tens& out, Dims d, float* of32 = nullptr) {
+int conv_fwd(pt_ctx* c, UNet& u, bool q8, uint32_t li, const Tens& in, const Tens& out, Dims d,
+             float* of32 = nullptr) {
     const Layer& l = u.L[li];
+    if (u.calib) {
+        const hipError_t e = chan_amax(in, l.cin, d.pixels(), u.calib + l.a_off, c->stream);
+        if (e != hipSuccess) return c->hip_fail(e, "unet calibration");
+    }
+    // the first and last convolutions stay bf16: in e4m3 the input features
+    // and the output radiance lose the most (each alone measured at +12% and
+    // +3..9% relMSE of the denoised frame, the 17 others at +3% or less)
+    if (q8 && li != E0A && li != OUT)
+        return hip_rc(c,
+                      conv3x3_fp8(in, l.cin, static_cast<const uint8_t*>(u.w8.p) + l.wbf_off, fp(u.inv_sa) + l.a_off,
+                                  fp(u.sw) + l.s_off, fp(u.params) + l.b_off, out, l.cout, d, l.relu, of32,
+                                  of32 ? 3u : 0u, c->stream),
+                      "unet conv fp8");
     return hip_rc(c,
                   conv3x3(in, l.cin, bfp(u.wbf) + l.wbf_off, fp(u.params) + l.b_off, out, l.cout, d, l.relu, of32,
                           of32 ? 3u : 0u, c->stream),
                   "unet conv");
 }
 
-int forward(pt_ctx* c, UNet& u) {
+// The forward pass over u.in0; q8: the fp8 convolutions (inference only).
+int forward(pt_ctx* c, UNet& u, bool q8 = false) {
     const Dims* D = u.dims;
     int rc;
     // encoder
@@ -248,23 +280,23 @@ int forward(pt_ctx* c, UNet& u) {
     for (int l = 0; l < 4; ++l) {
         const uint32_t cc = kC[l + 1] + kC[l];
         const Tens skip = T(u.cat[l], cc, kC[l + 1]);
-        if ((rc = conv_fwd(c, u, E0A + 2 * l, x, T(u.tA[l], kC[l]), D[l]))) return rc;
-        if ((rc = conv_fwd(c, u, E0B + 2 * l, T(u.tA[l], kC[l]), skip, D[l]))) return rc;
+        if ((rc = conv_fwd(c, u, q8, E0A + 2 * l, x, T(u.tA[l], kC[l]), D[l]))) return rc;
+        if ((rc = conv_fwd(c, u, q8, E0B + 2 * l, T(u.tA[l], kC[l]), skip, D[l]))) return rc;
         if ((rc = hip_rc(c, pool2(skip, kC[l], D[l], T(u.pin[l + 1], kC[l]), c->stream), "unet pool"))) return rc;
         x = T(u.pin[l + 1], kC[l]);
     }
-    if ((rc = conv_fwd(c, u, B4A, x, T(u.t4A, kC[4]), D[4]))) return rc;
-    if ((rc = conv_fwd(c, u, B4B, T(u.t4A, kC[4]), T(u.b4, kC[4]), D[4]))) return rc;
+    if ((rc = conv_fwd(c, u, q8, B4A, x, T(u.t4A, kC[4]), D[4]))) return rc;
+    if ((rc = conv_fwd(c, u, q8, B4B, T(u.t4A, kC[4]), T(u.b4, kC[4]), D[4]))) return rc;
     // decoder
     Tens low = T(u.b4, kC[4]);
     for (int l = 3; l >= 0; --l) {
         const uint32_t cc = kC[l + 1] + kC[l];
         if ((rc = hip_rc(c, up2(low, kC[l + 1], D[l + 1], T(u.cat[l], cc, 0), c->stream), "unet up"))) return rc;
-        if ((rc = conv_fwd(c, u, D3A + 2 * (3 - l), T(u.cat[l], cc), T(u.dA[l], kC[l]), D[l]))) return rc;
-        if ((rc = conv_fwd(c, u, D3B + 2 * (3 - l), T(u.dA[l], kC[l]), T(u.dd[l], kC[l]), D[l]))) return rc;
+        if ((rc = conv_fwd(c, u, q8, D3A + 2 * (3 - l), T(u.cat[l], cc), T(u.dA[l], kC[l]), D[l]))) return rc;
+        if ((rc = conv_fwd(c, u, q8, D3B + 2 * (3 - l), T(u.dA[l], kC[l]), T(u.dd[l], kC[l]), D[l]))) return rc;
         low = T(u.dd[l], kC[l]);
     }
-    return conv_fwd(c, u, OUT, low, Tens{nullptr, 0, 0}, D[0], fp(u.pred));
+    return conv_fwd(c, u, q8, OUT, low, Tens{nullptr, 0, 0}, D[0], fp(u.pred));
 }
 
 // Backward through layer li: x its input, a its output (for the ReLU mask),
@@ -350,6 +382,72 @@ int upload_batch_features(pt_ctx* c, UNet& u, const float* feat, const float* ta
     return hip_rc(c, e, "unet batch upload");
 }
 
+// The context's current frame (accumulation buffer + G-buffer), padded to
+// multiples of 16, through the network into u.pred.
+int frame_forward(pt_ctx* c, UNet& u, bool q8) {
+    const Dims d{1u, (c->height + 15u) & ~15u, (c->width + 15u) & ~15u};
+    int rc;
+    if ((rc = alloc_acts(c, u, d))) return rc;
+    const hipError_t e = pack_frame(static_cast<const float4*>(c->accum.p), c->gbuf.p, c->width, c->height, 0u, 0u,
+                                    d, bfp(u.in0), c->stream);
+    if (e != hipSuccess) return c->hip_fail(e, "unet input");
+    return forward(c, u, q8);
+}
+
+// Calibrated amax of a layer input maps to this e4m3 value: one binade of
+// headroom below the largest finite 448 for frames brighter than the
+// calibration frame (beyond it values saturate).
+constexpr float kFp8ActTarget = 224.0f;
+
+// fp8 inference state from the per-channel input maxima of one bf16 forward
+// pass: per layer s_a[ci] = amax[ci] / 224 (1 for a channel that stayed 0),
+// w'[tap][co][ci] = w * s_a[ci], s_w[co] = max |w'| / 448 (1 for an all-zero
+// row), weights e4m3(w' / s_w[co]); the kernels multiply the input by
+// 1 / s_a and the product by s_w.
+int quantize_fp8(pt_ctx* c, UNet& u) {
+    std::vector<uint32_t> am(u.n_act);
+    std::vector<float> p(u.n_params);
+    int rc;
+    if ((rc = hip_rc(c, hipMemcpy(am.data(), u.amax.p, am.size() * 4, hipMemcpyDeviceToHost), "unet calibration")) ||
+        (rc = hip_rc(c, hipMemcpy(p.data(), u.params.p, p.size() * 4, hipMemcpyDeviceToHost), "unet params")))
+        return rc;
+    u.h_sa.assign(u.n_act, 1.0f);
+    u.h_sw.assign(u.n_out, 1.0f);
+    u.h_w8.assign(u.n_wbf, 0);
+    std::vector<float> inv(u.n_act), wr;
+    for (uint32_t i = 0; i < kLayers; ++i) {
+        const Layer& l = u.L[i];
+        float* sa = u.h_sa.data() + l.a_off;
+        for (uint32_t ci = 0; ci < l.cin; ++ci) {
+            float a;
+            std::memcpy(&a, &am[l.a_off + ci], 4);
+            sa[ci] = (a > 0.0f && std::isfinite(a)) ? a / kFp8ActTarget : 1.0f;
+            inv[l.a_off + ci] = 1.0f / sa[ci];
+        }
+        wr.resize(size_t(9) * l.cin);
+        for (uint32_t co = 0; co < l.cout; ++co) {
+            float m = 0.0f;
+            for (uint32_t tap = 0; tap < 9u; ++tap)
+                for (uint32_t ci = 0; ci < l.cin; ++ci) {
+                    const float w = p[l.w_off + (size_t(tap) * l.cout + co) * l.cin + ci] * sa[ci];
+                    wr[size_t(tap) * l.cin + ci] = w;
+                    m = std::max(m, std::fabs(w));
+                }
+            const float s = m > 0.0f ? m / 448.0f : 1.0f;
+            u.h_sw[l.s_off + co] = s;
+            for (uint32_t tap = 0; tap < 9u; ++tap)
+                for (uint32_t ci = 0; ci < l.cin; ++ci)
+                    u.h_w8[l.wbf_off + (size_t(tap) * l.cout + co) * l.cin + ci] =
+                        e4m3_encode(wr[size_t(tap) * l.cin + ci] / s);
+        }
+    }
+    if ((rc = c->upload(u.w8, u.h_w8.data(), u.h_w8.size(), "unet fp8 weights")) ||
+        (rc = c->upload(u.inv_sa, inv.data(), inv.size() * 4, "unet fp8 scales")) ||
+        (rc = c->upload(u.sw, u.h_sw.data(), u.h_sw.size() * 4, "unet fp8 scales")))
+        return rc;
+    return c->sync("unet fp8");
+}
+
 }  // namespace
 }  // namespace pt
 
@@ -376,6 +474,7 @@ int pt_unet_init(pt_ctx* c, uint64_t seed) {
     if (e == hipSuccess) e = fill_zero_f32(fp(u.v), u.n_params, c->stream);
     if (e != hipSuccess) return c->hip_fail(e, "unet init");
     u.adam_t = 0;
+    u.fp8 = false;
     if ((rc = refresh_weights(c, u))) return rc;
     return c->sync("unet init");
 }
@@ -409,6 +508,7 @@ int pt_unet_set_params(pt_ctx* c, const float* in, uint64_t count) {
     if (e == hipSuccess) e = fill_zero_f32(fp(u.v), u.n_params, c->stream);
     if (e != hipSuccess) return c->hip_fail(e, "unet params");
     u.adam_t = 0;
+    u.fp8 = false;
     if ((rc = refresh_weights(c, u))) return rc;
     return c->sync("unet params");
 }
@@ -421,7 +521,7 @@ int pt_unet_forward(pt_ctx* c, const float* feat, uint32_t n, uint32_t h, uint32
     (void)hipSetDevice(c->device);
     UNet& u = *c->unet;
     if ((rc = alloc_acts(c, u, Dims{n, h, w})) || (rc = upload_batch_features(c, u, feat, nullptr)) ||
-        (rc = forward(c, u)) || (rc = c->sync("unet forward")))
+        (rc = forward(c, u, u.fp8)) || (rc = c->sync("unet forward")))
         return rc;
     return hip_rc(c, hipMemcpy(pred, u.pred.p, size_t(u.dims[0].pixels()) * 3 * 4, hipMemcpyDeviceToHost),
                   "unet forward readback");
@@ -531,6 +631,7 @@ int pt_unet_train(pt_ctx* c, uint32_t steps, uint32_t batch, float lr, uint64_t 
     if (steps > 1024) return c->fail(PT_ERR_ARG, "at most 1024 steps per call");
     (void)hipSetDevice(c->device);
     if ((rc = alloc_acts(c, u, Dims{batch, u.ds_h, u.ds_w}))) return rc;
+    u.fp8 = false;  // the weights change
     const size_t crop = size_t(u.ds_h) * u.ds_w;
     hipStream_t s = c->stream;
     if ((rc = hip_rc(c, fill_zero_f32(fp(u.loss), steps, s), "unet loss"))) return rc;
@@ -572,17 +673,51 @@ int pt_unet_denoise(pt_ctx* c, float exposure) {
     (void)hipSetDevice(c->device);
     UNet& u = *c->unet;
     const uint32_t fw = c->width, fh = c->height;
-    const Dims d{1u, (fh + 15u) & ~15u, (fw + 15u) & ~15u};
-    if ((rc = alloc_acts(c, u, d))) return rc;
     if ((rc = c->alloc(c->dn_fin, size_t(fw) * fh * 16, "denoise out"))) return rc;
-    hipError_t e =
-        pack_frame(static_cast<const float4*>(c->accum.p), c->gbuf.p, fw, fh, 0u, 0u, d, bfp(u.in0), c->stream);
-    if (e != hipSuccess) return c->hip_fail(e, "unet input");
-    if ((rc = forward(c, u))) return rc;
-    e = unpack_frame(fp(u.pred), d, fw, fh, exposure, static_cast<float4*>(c->dn_fin.p),
-                     static_cast<uchar4*>(c->out.p), c->stream);
+    if ((rc = frame_forward(c, u, u.fp8))) return rc;
+    const hipError_t e = unpack_frame(fp(u.pred), u.dims[0], fw, fh, exposure, static_cast<float4*>(c->dn_fin.p),
+                                      static_cast<uchar4*>(c->out.p), c->stream);
     if (e != hipSuccess) return c->hip_fail(e, "unet output");
     return c->sync("unet denoise");
+}
+
+int pt_unet_set_precision(pt_ctx* c, int precision) {
+    if (!c) return PT_ERR_ARG;
+    int rc = ensure_net(c);
+    if (rc) return rc;
+    UNet& u = *c->unet;
+    if (precision == PT_UNET_BF16) {
+        u.fp8 = false;
+        return PT_OK;
+    }
+    if (precision != PT_UNET_FP8) return c->fail(PT_ERR_ARG, "unknown precision %d", precision);
+    if (c->width == 0 || !c->gbuf.p) return c->fail(PT_ERR_STATE, "fp8 calibration needs a frame with a G-buffer");
+    (void)hipSetDevice(c->device);
+    u.fp8 = false;
+    if ((rc = c->alloc(u.amax, u.n_act * 4, "unet calibration"))) return rc;
+    if ((rc = hip_rc(c, fill_zero_f32(fp(u.amax), u.n_act, c->stream), "unet calibration"))) return rc;
+    u.calib = static_cast<uint32_t*>(u.amax.p);
+    rc = frame_forward(c, u, false);
+    u.calib = nullptr;
+    if (rc || (rc = c->sync("unet calibration")) || (rc = quantize_fp8(c, u))) return rc;
+    u.fp8 = true;
+    return PT_OK;
+}
+
+int pt_unet_fp8_state(pt_ctx* c, float* act_scales, uint64_t n_act, float* w_scales, uint64_t n_w, uint8_t* weights,
+                      uint64_t n_bytes) {
+    if (!c) return PT_ERR_ARG;
+    int rc = ensure_net(c);
+    if (rc) return rc;
+    const UNet& u = *c->unet;
+    if (!u.fp8) return c->fail(PT_ERR_STATE, "the network is not in fp8 mode (pt_unet_set_precision)");
+    if (!act_scales || !w_scales || !weights || n_act != u.n_act || n_w != u.n_out || n_bytes != u.n_wbf)
+        return c->fail(PT_ERR_ARG, "expected %llu input scales, %llu output scales and %llu weight bytes",
+                       (unsigned long long)u.n_act, (unsigned long long)u.n_out, (unsigned long long)u.n_wbf);
+    std::memcpy(act_scales, u.h_sa.data(), n_act * 4);
+    std::memcpy(w_scales, u.h_sw.data(), n_w * 4);
+    std::memcpy(weights, u.h_w8.data(), n_bytes);
+    return PT_OK;
 }
 
 }  // extern "C"

@@ -350,7 +350,8 @@ napi_value set_option(napi_env env, napi_callback_info info) {
 }
 
 // The learned denoiser (pt_unet.h): unetInit(ctx, seed), unetSetParams(ctx,
-// Float32Array), unetDenoise(ctx, exposure); the result is read with
+// Float32Array), unetDenoise(ctx, exposure), unetSetPrecision(ctx, 0 bf16 |
+// 1 fp8, calibrated on the current frame); the result is read with
 // readOutput / readDenoised like the a-trous filter's.
 napi_value unet_init(napi_env env, napi_callback_info info) {
     napi_value argv[2];
@@ -389,6 +390,18 @@ napi_value unet_denoise(napi_env env, napi_callback_info info) {
     napi_get_value_double(env, argv[1], &exposure);
     int rc = pt_unet_denoise(ctx, float(exposure));
     if (rc) return throw_pt(env, ctx, rc, "pt_unet_denoise");
+    return nullptr;
+}
+
+napi_value unet_set_precision(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    if (!get_args(env, info, 2, argv)) return nullptr;
+    pt_ctx* ctx = get_ctx(env, argv[0]);
+    if (!ctx) return nullptr;
+    int32_t precision = 0;
+    napi_get_value_int32(env, argv[1], &precision);
+    int rc = pt_unet_set_precision(ctx, precision);
+    if (rc) return throw_pt(env, ctx, rc, "pt_unet_set_precision");
     return nullptr;
 }
 
@@ -523,6 +536,7 @@ napi_value init(napi_env env, napi_value exports) {
         {"unetInit", nullptr, unet_init, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"unetSetParams", nullptr, unet_set_params, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"unetDenoise", nullptr, unet_denoise, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"unetSetPrecision", nullptr, unet_set_precision, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"batchCreate", nullptr, batch_create, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"batchAdd", nullptr, batch_add, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"batchFrame", nullptr, batch_frame, nullptr, nullptr, nullptr, napi_enumerable, nullptr},

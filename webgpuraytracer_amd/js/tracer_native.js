@@ -221,6 +221,9 @@ class NativeLearnedDenoiser {
   }
 
   readResult() { return this.r.readDenoised(); }
+
+  // fp8 inference (true: calibrated on the renderer's current frame) or bf16
+  setPrecision(fp8) { addon.unetSetPrecision(this.r.ctx, fp8 ? 1 : 0); }
 }
 
 // Several NativeRenderers rendered one progressive frame at a time as one

@@ -281,6 +281,19 @@ class LearnedDenoiser:
         self.r._check(self.r.lib.pt_read_denoised(self.r.ctx, a.ctypes.data, a.nbytes))
         return a
 
+    def set_precision(self, fp8):
+        """fp8=True: calibrate on the renderer's current frame and run
+        inference on fp8 products (pt_unet_set_precision); False: bf16."""
+        self.r._check(self.r.lib.pt_unet_set_precision(self.r.ctx, A.PT_UNET_FP8 if fp8 else A.PT_UNET_BF16))
+
+    def fp8_state(self, n_act, n_out, n_weights):
+        """(input scales, weight scales, e4m3 weight bytes) of the fp8 mode."""
+        sa, sw = np.empty(n_act, np.float32), np.empty(n_out, np.float32)
+        w8 = np.empty(n_weights, np.uint8)
+        self.r._check(self.r.lib.pt_unet_fp8_state(self.r.ctx, sa.ctypes.data, n_act, sw.ctypes.data, n_out,
+                                                    w8.ctypes.data, n_weights))
+        return sa, sw, w8
+
     def frame_features(self):
         """The renderer's current frame as network features [1][h][w][10]
         (accumulated radiance, G-buffer albedo, normal and hit distance)."""

@@ -210,6 +210,23 @@ def cmd_eval(a):
                "ssim_unet": ssim(un, ref)}
         rows.append(row)
         _log(json.dumps(row))
+    if a.fp8:
+        # fp8 inference, calibrated on a 1-spp frame of a training-family scene (not the evaluated
+        # one), then the same seeded frames again
+        r.set_scene(SceneBuilder.config(1, seed=a.calib_seed), isgbuf=True)
+        r.gbuffer()
+        r.reset()
+        r.render(spp=1, base_seed=0xCA1B)
+        net.set_precision(True)
+        r.set_scene(scene, isgbuf=True)
+        r.gbuffer()
+        for row in rows:
+            r.reset()
+            r.render(spp=row["spp"], base_seed=0xBEEF + row["spp"])
+            row["ms_unet_fp8"] = _timed(lambda: r._check(r.lib.pt_unet_denoise(r.ctx, net.exposure)))
+            un8 = tonemap(net.read_result()[..., :3])
+            row["relmse_unet_fp8"], row["ssim_unet_fp8"] = rel_mse(un8, ref), ssim(un8, ref)
+            _log(json.dumps({k: row[k] for k in ("spp", "ms_unet", "ms_unet_fp8", "relmse_unet", "relmse_unet_fp8")}))
     # equal quality: the fewest U-Net spp whose relMSE is at or below the à-trous relMSE at each spp
     eq = []
     for ra in rows:
@@ -282,6 +299,8 @@ def main(argv=None):
     e.add_argument("--scene-seed", type=int, default=7)
     e.add_argument("--atrous-passes", type=int, default=5)
     e.add_argument("--json", default=None)
+    e.add_argument("--fp8", action="store_true", help="also denoise with fp8 inference (pt_unet_set_precision)")
+    e.add_argument("--calib-seed", type=int, default=1000, help="scene seed of the fp8 calibration frame")
     m = sub.add_parser("timing")
     m.add_argument("--batch", type=int, default=16)
     m.add_argument("--size", type=int, default=256)
