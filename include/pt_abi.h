@@ -121,14 +121,24 @@ enum {
                                   memory; 0 = all in LDS (default 16) */
     PT_OPT_WAVEFRONT_REFILL = 7, /* wavefront traversal: idle lanes of a wave that trigger fetching
                                    new rays, 1..64 (default 56) */
-    PT_OPT_BLAS_BUILD = 8        /* PT_BLAS_BUILD_*: where the wide traversal's bottom-level trees
+    PT_OPT_BLAS_BUILD = 8,       /* PT_BLAS_BUILD_*: where the wide traversal's bottom-level trees
                                    come from; read by the next pt_upload_scene */
+    PT_OPT_TLAS_BUILD = 9        /* PT_TLAS_BUILD_*: where the top-level tree comes from; read by
+                                   every pt_upload_scene and pt_update_objects */
 };
 /* REFERENCE: the uploaded tracer.js BVH, widened on the host (default).
    DEVICE: rebuilt on the GPU from the triangles (linear BVH over Morton
    codes, 4-wide); the same closest hits, different node counts.  The
    child-pair (reference-order) traversal always uses the uploaded BVH. */
 enum { PT_BLAS_BUILD_REFERENCE = 0, PT_BLAS_BUILD_DEVICE = 1 };
+/* REFERENCE: the descriptor's TLAS (SceneBVHBuilder, tracer.js:1509-1619),
+   converted on the host (default).  DEVICE: both forms of the top-level tree
+   (child-pair and 4-wide) built on the GPU over the objects' world boxes,
+   one object per leaf (a linear BVH over Morton codes), at every upload and
+   update; the descriptor's `tlas` array is ignored and may be empty, so a
+   caller that moves objects sends only the objects.  The same closest hits;
+   node counts are of the device-built tree. */
+enum { PT_TLAS_BUILD_REFERENCE = 0, PT_TLAS_BUILD_DEVICE = 1 };
 enum { PT_BACKEND_MEGAKERNEL = 0, PT_BACKEND_WAVEFRONT = 1 };
 /* SYNC: the lanes of a wave run each pass together.  REGEN: a lane whose path
    ends starts its pixel's next pass at once.  Both produce the same bits. */

@@ -130,10 +130,12 @@ ERR_NAMES = {0: "PT_OK", -1: "PT_ERR_ARG", -2: "PT_ERR_LAYOUT", -3: "PT_ERR_CAPA
              -9: "PT_ERR_NO_DEVICE"}
 
 PT_OPT_BACKEND, PT_OPT_COUNT_STATS, PT_OPT_WAVEFRONT_POOL, PT_OPT_SCHEDULE, PT_OPT_TRAVERSAL, PT_OPT_LDS_STACK, \
-    PT_OPT_WAVEFRONT_REFILL, PT_OPT_BLAS_BUILD = 1, 2, 3, 4, 5, 6, 7, 8
+    PT_OPT_WAVEFRONT_REFILL, PT_OPT_BLAS_BUILD, PT_OPT_TLAS_BUILD = 1, 2, 3, 4, 5, 6, 7, 8, 9
 PT_BLAS_BUILD_REFERENCE, PT_BLAS_BUILD_DEVICE = 0, 1
+PT_TLAS_BUILD_REFERENCE, PT_TLAS_BUILD_DEVICE = 0, 1
 PT_BACKEND_MEGAKERNEL, PT_BACKEND_WAVEFRONT = 0, 1
 PT_UNET_INPUT, PT_UNET_TARGET = 0, 1
+PT_OBJ_MESH, PT_OBJ_SPHERE, PT_OBJ_CUBE, PT_OBJ_CYLINDER, PT_OBJ_TORUS = 0, 1, 2, 3, 4
 PT_UNET_BF16, PT_UNET_FP8 = 0, 1
 PT_SCHEDULE_SYNC, PT_SCHEDULE_REGEN = 0, 1
 PT_TRAVERSAL_REFERENCE, PT_TRAVERSAL_WIDE = 0, 1

@@ -43,8 +43,9 @@ int validate_desc(pt_ctx* c, const pt_scene_desc* d) {
         {&d->tlas, sizeof(pt_bvh_node), "tlas"},          {&d->meshes, sizeof(pt_mesh_instance), "meshes"},
         {&d->bvh, sizeof(pt_bvh_node), "bvh"},            {&d->triangles, sizeof(pt_triangle), "triangles"},
         {&d->planes, sizeof(pt_plane), "planes"},         {&d->lights, sizeof(pt_light), "lights"}};
-    for (auto& x : arrs)
-        if (!check_array(*x.a, x.stride, false))
+    for (auto& x : arrs)  // every array holds at least one record, as WebGPU buffers do; with the
+                          // device-built TLAS the descriptor's TLAS is not read and may be empty
+        if (!check_array(*x.a, x.stride, x.a == &d->tlas && c->tlas_build == PT_TLAS_BUILD_DEVICE))
             return c->fail(PT_ERR_LAYOUT, "%s: expected a non-empty array with stride %u (got count %llu stride %u)",
                            x.name, x.stride, (unsigned long long)x.a->count, x.a->stride);
     if (d->flags & PT_HAS_HEIGHTMAPS) return c->fail(PT_ERR_UNSUPPORTED, "heightmaps (POM) are not supported");
@@ -133,7 +134,7 @@ void refresh_dev(pt_ctx* c) {
 int upload_dynamic(pt_ctx* c, const pt_scene_desc* d, bool first) {
     HostLayout L;
     std::string err;
-    int rc = build_layout(d, L, err);
+    int rc = build_layout(d, L, err, c->tlas_build == PT_TLAS_BUILD_DEVICE);
     if (rc) return c->fail(rc, "%s", err.c_str());
     std::vector<Root>& roots = L.roots;
     const uint32_t td = L.tlas_depth;
@@ -294,6 +295,47 @@ int device_blas(pt_ctx* c, const pt_scene_desc* d) {
     return c->upload(c->roots, c->host_roots.data(), c->host_roots.size() * sizeof(Root), "roots");
 }
 
+// Both forms of the top-level tree built on the device from the uploaded
+// objects (PT_TLAS_BUILD_DEVICE), after upload_dynamic has uploaded the
+// objects, their BLAS root slots and the BLAS roots.
+int device_tlas(pt_ctx* c, const pt_scene_desc* d) {
+    const uint32_t any_obj = PT_HAS_SPHERES | PT_HAS_CUBES | PT_HAS_CYLINDERS | PT_HAS_TORI | PT_HAS_MESHES;
+    if (!(d->flags & any_obj)) return PT_OK;
+    const uint64_t n = d->objects.count;
+    if (n == 0) return c->fail(PT_ERR_LAYOUT, "the scene flags name objects but there are none");
+    if (n >= (1u << 25)) return c->fail(PT_ERR_UNSUPPORTED, "the device-built TLAS takes fewer than 2^25 objects");
+    const size_t nodes = std::max<uint64_t>(n - 1, 1);
+    const size_t sb = tlas_build_scratch_bytes(uint32_t(n));
+    DevBuf scratch, ctr;
+    int rc;
+    if ((rc = c->alloc(scratch, sb, "tlas build scratch")) || (rc = c->alloc(ctr, 64, "tlas build counters")) ||
+        (rc = c->alloc(c->tnodes, nodes * sizeof(CNode), "tlas nodes")) ||
+        (rc = c->alloc(c->wtnodes, nodes * sizeof(WNode), "wide tlas nodes"))) {
+        scratch.release();
+        ctr.release();
+        return rc;
+    }
+    TlasBuildResult res{};
+    const hipError_t e = build_tlas_device(static_cast<const pt_object*>(c->objs.p), uint32_t(n),
+                                           static_cast<const uint32_t*>(c->obj_root.p),
+                                           static_cast<const Root*>(c->roots.p), scratch.p, sb,
+                                           static_cast<CNode*>(c->tnodes.p), static_cast<WNode*>(c->wtnodes.p),
+                                           static_cast<uint32_t*>(ctr.p), &res, c->stream);
+    scratch.release();
+    ctr.release();
+    if (e != hipSuccess) return c->hip_fail(e, "device tlas build");
+    const uint32_t need = stack_need(res.depth, c->blas_depth), wneed = stack_need(res.need, c->blas_wneed);
+    if (need > kMaxStack || wneed > kMaxStack)
+        return c->fail(PT_ERR_UNSUPPORTED, "device-built TLAS depth %u needs more than %u stack entries", res.depth,
+                       kMaxStack);
+    c->host_roots[0] = res.root;
+    c->tlas_depth = res.depth;
+    c->tlas_wneed = res.need;
+    c->stack_cap = need;
+    c->wstack_cap = wneed;
+    return c->upload(c->roots, c->host_roots.data(), c->host_roots.size() * sizeof(Root), "roots");
+}
+
 }  // namespace
 
 extern "C" {
@@ -364,6 +406,11 @@ int pt_set_option(pt_ctx* c, int key, int64_t value) {
                 return c->fail(PT_ERR_ARG, "unknown BLAS build %lld", (long long)value);
             c->blas_build = int(value);
             return PT_OK;
+        case PT_OPT_TLAS_BUILD:
+            if (value != PT_TLAS_BUILD_REFERENCE && value != PT_TLAS_BUILD_DEVICE)
+                return c->fail(PT_ERR_ARG, "unknown TLAS build %lld", (long long)value);
+            c->tlas_build = int(value);
+            return PT_OK;
         case PT_OPT_SCHEDULE:
             if (value != PT_SCHEDULE_SYNC && value != PT_SCHEDULE_REGEN)
                 return c->fail(PT_ERR_ARG, "unknown schedule %lld", (long long)value);
@@ -390,6 +437,7 @@ int pt_upload_scene(pt_ctx* c, const pt_scene_desc* d) {
     if ((rc = c->upload(c->tris, tr, d->triangles.count * sizeof(pt_triangle), "triangles"))) return rc;
     if ((rc = c->upload(c->bvh_raw, d->bvh.ptr, d->bvh.count * sizeof(pt_bvh_node), "bvh"))) return rc;
     c->tri_geo_w.release();
+    if (c->tlas_build == PT_TLAS_BUILD_DEVICE && (rc = device_tlas(c, d))) return rc;
     if (c->blas_build == PT_BLAS_BUILD_DEVICE && (rc = device_blas(c, d))) return rc;
     if ((rc = c->upload(c->lights, d->lights.ptr, d->lights.count * sizeof(pt_light), "lights"))) return rc;
     c->n_bvh = d->bvh.count;
@@ -439,8 +487,9 @@ int pt_update_objects(pt_ctx* c, const pt_scene_desc* d) {
         return c->fail(PT_ERR_CAPACITY, "scene arrays outgrew the uploaded buffers");
     c->bounce_limit = d->bounce_limit;
     if ((rc = upload_dynamic(c, d, false))) return rc;
+    if (c->tlas_build == PT_TLAS_BUILD_DEVICE && (rc = device_tlas(c, d))) return rc;
     refresh_dev(c);
-    return PT_OK;
+    return c->sync("update");
 }
 
 int pt_resize(pt_ctx* c, uint32_t w, uint32_t h, int with_gbuffer) {

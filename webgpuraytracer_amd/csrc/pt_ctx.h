@@ -57,6 +57,7 @@ struct pt_ctx {
     uint32_t tlas_depth = 0, blas_depth = 0, stack_cap = 2;
     uint32_t blas_wneed = 0, wstack_cap = 2, tlas_wneed = 0;
     int blas_build = PT_BLAS_BUILD_REFERENCE;  // where the wide BLAS trees come from
+    int tlas_build = PT_TLAS_BUILD_REFERENCE;  // where both TLAS forms come from
     DevBuf tri_geo_w;                          // device-built trees: triangle records in their leaf order
     pt::DevScene dev{};
     // frame

@@ -123,6 +123,20 @@ size_t blas_build_scratch_bytes(uint32_t max_n);
 hipError_t build_blas_device(const float4* geo, const pt_triangle* tris, const BlasBuildJob* jobs, uint32_t n_jobs,
                              void* scratch, size_t scratch_bytes, WNode* wn, float4* geo_out, uint32_t* ctr,
                              BlasBuildResult* res, hipStream_t stream);
+// The top-level tree built on the device (pt_build.hip) over the objects'
+// world boxes (from their inverse matrices and, for meshes, the BLAS root
+// boxes in roots[obj_root[i]]), one object per leaf referenced by its own
+// index: child-pair nodes into cn (n - 1 of them) and 4-wide nodes into wn
+// (at most n - 1); ctr: 4 words of device scratch.  n <= 2^25.  Synchronous.
+struct TlasBuildResult {
+    Root root;       // box, child-pair ref and wide ref of the root
+    uint32_t depth;  // child-pair tree depth
+    uint32_t need;   // largest sum of (children - 1) along a wide path
+};
+size_t tlas_build_scratch_bytes(uint32_t n);
+hipError_t build_tlas_device(const pt_object* objs, uint32_t n, const uint32_t* obj_root, const Root* roots,
+                             void* scratch, size_t scratch_bytes, CNode* cn, WNode* wn, uint32_t* ctr,
+                             TlasBuildResult* res, hipStream_t stream);
 hipError_t launch_wavefront_batch(const DevScene& S, const WfArgs& a, bool count_nodes, hipStream_t stream);
 // Resident workgroups per CU of the wavefront traversal kernels (the larger of the two).
 hipError_t wf_trace_blocks_per_cu(const DevScene& S, bool count_nodes, bool wide, uint32_t stack_lds,

@@ -175,7 +175,7 @@ bool blas_ranges_ok(const pt_bvh_node* nodes, uint64_t n_nodes, uint32_t node_of
 
 }  // namespace
 
-int build_layout(const pt_scene_desc* d, HostLayout& L, std::string& err) {
+int build_layout(const pt_scene_desc* d, HostLayout& L, std::string& err, bool skip_tlas) {
     auto fail = [&](const char* fmt, ...) {
         char buf[512];
         va_list ap;
@@ -200,7 +200,7 @@ int build_layout(const pt_scene_desc* d, HostLayout& L, std::string& err) {
     roots.assign(1, Root{});
     *tlas_depth = 0;
     *blas_depth = 0;
-    if (d->flags & any_obj) {
+    if ((d->flags & any_obj) && !skip_tlas) {
         TreeConv t{tl, d->tlas.count, false, 0, &tn};
         if (!t.root(0, &roots[0])) return fail("tlas: %s", t.err.c_str());
         *tlas_depth = t.max_depth;

@@ -25,7 +25,9 @@ struct HostLayout {
 
 // Builds every tree of the scene.  Returns PT_OK, or PT_ERR_LAYOUT with a
 // message in err (indices out of range, cycles, oversize trees).
-int build_layout(const pt_scene_desc* d, HostLayout& L, std::string& err);
+// skip_tlas: the top-level tree is built elsewhere (the device build);
+// tn and wtn stay empty and roots[0] is a placeholder.
+int build_layout(const pt_scene_desc* d, HostLayout& L, std::string& err, bool skip_tlas = false);
 
 // 3 float4 per triangle: v0, v1 - v0, v2 - v0.
 void build_tri_geo(const pt_scene_desc* d, std::vector<float4>& geo);

@@ -18,10 +18,11 @@ const addon = require(path.join(__dirname, 'build', 'pt_addon.node'));
 // pt_set_option keys and values (pt_abi.h)
 const OPT = {
   BACKEND: 1, COUNT_STATS: 2, WAVEFRONT_POOL: 3, SCHEDULE: 4, TRAVERSAL: 5, LDS_STACK: 6, WAVEFRONT_REFILL: 7,
-  BLAS_BUILD: 8,
+  BLAS_BUILD: 8, TLAS_BUILD: 9,
 };
 const BACKEND = { MEGAKERNEL: 0, WAVEFRONT: 1 };
 const BLAS_BUILD = { REFERENCE: 0, DEVICE: 1 };
+const TLAS_BUILD = { REFERENCE: 0, DEVICE: 1 };
 
 // override constants shader.wgsl:2-12 <- packSceneData flags (tracer.js:1901-1913)
 const FLAG_BITS = {
@@ -256,5 +257,5 @@ class NativeBatch {
 
 module.exports = {
   NativeRenderer, NativeDenoiser, NativeLearnedDenoiser, NativeBatch, packParams, flagsWord, loadNpyF32,
-  FLAG_BITS, OPT, BACKEND, BLAS_BUILD, PRETRAINED, addon,
+  FLAG_BITS, OPT, BACKEND, BLAS_BUILD, TLAS_BUILD, PRETRAINED, addon,
 };
