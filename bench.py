@@ -80,6 +80,9 @@ def main():
                     help="BVH traversal: 4-wide nodes, or the reference's child-pair order")
     ap.add_argument("--blas-build", choices=["reference", "device"], default="reference",
                     help="bottom-level trees of the wide traversal: the uploaded BVH, or rebuilt on the GPU")
+    ap.add_argument("--tlas-build", choices=["reference", "device"], default="reference",
+                    help="top-level tree: the uploaded TLAS, or built on the GPU from the objects "
+                         "(counted work is then of that tree)")
     ap.add_argument("--compare", action="store_true",
                     help="also time the other backend/schedule variants (reported under alt)")
     args = ap.parse_args()
@@ -98,6 +101,8 @@ def main():
     r = Renderer(0)
     if args.blas_build == "device":
         r.set_option(A.PT_OPT_BLAS_BUILD, A.PT_BLAS_BUILD_DEVICE)
+    if args.tlas_build == "device":
+        r.set_option(A.PT_OPT_TLAS_BUILD, A.PT_TLAS_BUILD_DEVICE)
     t0 = time.perf_counter()
     r.set_scene(scene)
     r.resize(W, H)
@@ -153,7 +158,8 @@ def main():
         "config": {"workload": "plain tracer, large scene: 1920x1080, 64 spp/step, 8 bounces, 64 instances of 4 "
                                "displaced geodesic spheres (%d triangles), 2048x1024 env, %s"
                                % (d.triangles.count, chosen + ("" if args.blas_build == "reference"
-                                                               else ", BLAS built on the device")),
+                                                               else ", BLAS built on the device")
+                                  + ("" if args.tlas_build == "reference" else ", TLAS built on the device")),
                    "extension_rays": int(sum(s.extension_rays for s in stats)),
                    "shadow_rays": int(sum(s.shadow_rays for s in stats)),
                    "paths": int(sum(s.paths for s in stats)),

@@ -28,6 +28,7 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 constexpr uint32_t kBlock = 256;
 
 __device__ __forceinline__ bf16x8 load8(const bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+__device__ __forceinline__ void store8(bf16* p, bf16x8 v) { *reinterpret_cast<bf16x8*>(p) = v; }
 __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, u32x4{0u, 0u, 0u, 0u}); }
 __device__ __forceinline__ float f(bf16 x) { return float(x); }
 __device__ __forceinline__ bf16 b(float x) { return bf16(x); }
@@ -51,7 +52,11 @@ struct ConvSmem {
     bf16x8 w[4 * 9 * NCT * 16];   // [8-channel group][tap][co]
 };
 
-template <int NCT, bool RELU, bool F32OUT>
+// FULL: every block has all NCT output tiles (cout a multiple of NCT x 16)
+// and the frame is a multiple of the 16 x 16 tile, so no guards are needed;
+// the bf16 output then goes through LDS ([pixel][channel], rows padded by 16
+// bytes against bank conflicts) and leaves as 16-byte stores.
+template <int NCT, bool RELU, bool F32OUT, bool FULL>
 __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const bf16* __restrict__ w,
                                                     const float* __restrict__ bias, Tens out, uint32_t cout, Dims d,
                                                     float* __restrict__ of32, uint32_t f32c) {
@@ -64,7 +69,7 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
     const uint32_t img = blockIdx.x / (tiles_x * tiles_y), trem = blockIdx.x % (tiles_x * tiles_y);
     const int ty0 = int(trem / tiles_x * kTile), tx0 = int(trem % tiles_x * kTile);
     const uint32_t co0 = blockIdx.y * uint32_t(NCT) * 16u;
-    const uint32_t nct = min(uint32_t(NCT), (cout - co0) / 16u);
+    const uint32_t nct = FULL ? uint32_t(NCT) : min(uint32_t(NCT), (cout - co0) / 16u);
     // the pieces this thread stages: halo pixel j, channels (piece % 4) * 8
     const bf16* hsrc[kHaloPer];
     bool hval[kHaloPer];
@@ -137,6 +142,36 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
         }
     }
     // C[(lane / 16) * 4 + r][lane % 16]: tile column kq * 4 + r, channel lane % 16
+    if (FULL && !F32OUT) {
+        constexpr uint32_t kCo = uint32_t(NCT) * 16u, kPitch = kCo * 2u + 16u;  // bytes per staged pixel
+        static_assert(kTile * kTile * kPitch <= sizeof(ConvSmem<NCT>), "output stage fits the LDS tile");
+        char* stage = reinterpret_cast<char*>(&sm);
+        __syncthreads();  // every wave is done with the last slice
+#pragma unroll
+        for (int t = 0; t < NCT; ++t) {
+            const uint32_t ch = uint32_t(t) * 16u + (lane & 15u);
+            const float bv = bias ? bias[co0 + ch] : 0.0f;
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    float v = acc[m][t][r] + bv;
+                    if (RELU) v = fmaxf(v, 0.0f);
+                    const uint32_t p = (wv * 4u + uint32_t(m)) * kTile + kq * 4u + uint32_t(r);
+                    *reinterpret_cast<bf16*>(stage + p * kPitch + ch * 2u) = b(v);
+                }
+        }
+        __syncthreads();
+        constexpr uint32_t kChunks = kTile * kTile * kCo / 8u;  // 16-byte pieces of the tile
+#pragma unroll
+        for (uint32_t i = tid; i < kChunks; i += kBlock) {
+            const uint32_t p = i / (kCo / 8u), c8 = i % (kCo / 8u);
+            const size_t op = (size_t(img) * d.h + uint32_t(ty0) + p / kTile) * d.w + uint32_t(tx0) + p % kTile;
+            store8(out.p + op * out.stride + out.off + co0 + c8 * 8u,
+                   *reinterpret_cast<const bf16x8*>(stage + p * kPitch + c8 * 16u));
+        }
+        return;
+    }
 #pragma unroll
     for (int t = 0; t < NCT; ++t) {
         if (uint32_t(t) >= nct) continue;
@@ -169,15 +204,23 @@ static void conv_launch(const Tens& in, uint32_t cin, const bf16* w, const float
     const uint32_t tiles = d.n * ((d.h + kTile - 1u) / kTile) * ((d.w + kTile - 1u) / kTile);
     const uint32_t groups = (cout / 16u + NCT - 1u) / NCT;
     const dim3 grid(tiles, groups);
+    const bool full = cout % (uint32_t(NCT) * 16u) == 0 && d.h % kTile == 0 && d.w % kTile == 0 &&
+                      out.stride % 8u == 0 && out.off % 8u == 0;
     if (of32)
-        hipLaunchKernelGGL((conv3x3_k<NCT, false, true>), grid, dim3(kBlock), 0, s, in, cin, w, bias, out, cout, d,
-                           of32, f32c);
+        hipLaunchKernelGGL((conv3x3_k<NCT, false, true, false>), grid, dim3(kBlock), 0, s, in, cin, w, bias, out,
+                           cout, d, of32, f32c);
+    else if (relu && full)
+        hipLaunchKernelGGL((conv3x3_k<NCT, true, false, true>), grid, dim3(kBlock), 0, s, in, cin, w, bias, out, cout,
+                           d, of32, f32c);
     else if (relu)
-        hipLaunchKernelGGL((conv3x3_k<NCT, true, false>), grid, dim3(kBlock), 0, s, in, cin, w, bias, out, cout, d,
-                           of32, f32c);
+        hipLaunchKernelGGL((conv3x3_k<NCT, true, false, false>), grid, dim3(kBlock), 0, s, in, cin, w, bias, out,
+                           cout, d, of32, f32c);
+    else if (full)
+        hipLaunchKernelGGL((conv3x3_k<NCT, false, false, true>), grid, dim3(kBlock), 0, s, in, cin, w, bias, out,
+                           cout, d, of32, f32c);
     else
-        hipLaunchKernelGGL((conv3x3_k<NCT, false, false>), grid, dim3(kBlock), 0, s, in, cin, w, bias, out, cout, d,
-                           of32, f32c);
+        hipLaunchKernelGGL((conv3x3_k<NCT, false, false, false>), grid, dim3(kBlock), 0, s, in, cin, w, bias, out,
+                           cout, d, of32, f32c);
 }
 
 hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bias, const Tens& out, uint32_t cout,
@@ -463,7 +506,6 @@ hipError_t chan_amax(const Tens& x, uint32_t c, uint32_t P, uint32_t* amax, hipS
 // offset moves by whole rows, and w is a multiple of 16).
 constexpr uint32_t kStagePix = 64;  // pixels per staging block
 
-__device__ __forceinline__ void store8(bf16* p, bf16x8 v) { *reinterpret_cast<bf16x8*>(p) = v; }
 
 // Channel-major arrays have rows of chw_stride(d) = P rounded up to 64
 // pixels; the pixels past P are zero.
