@@ -267,6 +267,12 @@ struct ConvSmem8 {
     i32x4 halo[(SL / 16) * kHaloPix];   // [16-channel group][pixel]
     i32x4 w[9 * (SL / 16) * NCT * 16];  // [slot][half][co]
 };
+// the same LDS reused for the output tile: [pixel][channel] bf16, rows padded by 16 bytes
+template <int NCT, int SL>
+union ConvLds8 {
+    ConvSmem8<NCT, SL> m;
+    char stage[kTile * kTile * (NCT * 32 + 16)];
+};
 
 __device__ __forceinline__ uint32_t e4m3x4(float a, float b, float c, float d) {
     a = fminf(fmaxf(a, -448.0f), 448.0f);
@@ -277,13 +283,14 @@ __device__ __forceinline__ uint32_t e4m3x4(float a, float b, float c, float d) {
     return uint32_t(__builtin_amdgcn_cvt_pk_fp8_f32(c, d, lo, true));
 }
 
-template <int NCT, int SL, bool RELU, bool F32OUT>
+template <int NCT, int SL, bool RELU, bool F32OUT, bool FULL>
 __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, const uint8_t* __restrict__ wq,
                                                        const float* __restrict__ inv_scale,
                                                        const float* __restrict__ wscale,
                                                        const float* __restrict__ bias, Tens out, uint32_t cout, Dims d,
                                                        float* __restrict__ of32, uint32_t f32c) {
-    __shared__ ConvSmem8<NCT, SL> sm;
+    __shared__ ConvLds8<NCT, SL> lds;
+    ConvSmem8<NCT, SL>& sm = lds.m;
     constexpr uint32_t kG16 = uint32_t(SL) / 16u, kG8 = uint32_t(SL) / 8u;  // 16- and 8-channel groups per slice
     constexpr uint32_t kSlots = 9u * (uint32_t(SL) / 32u), kSteps = (kSlots + 3u) / 4u;
     constexpr uint32_t kHalo8Pieces = kHaloPix * kG8;                        // 8-channel pieces of the halo
@@ -296,7 +303,7 @@ __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, co
     const uint32_t img = blockIdx.x / (tiles_x * tiles_y), trem = blockIdx.x % (tiles_x * tiles_y);
     const int ty0 = int(trem / tiles_x * kTile), tx0 = int(trem % tiles_x * kTile);
     const uint32_t co0 = blockIdx.y * kCo;
-    const uint32_t nct = min(uint32_t(NCT), (cout - co0) / 16u);
+    const uint32_t nct = FULL ? uint32_t(NCT) : min(uint32_t(NCT), (cout - co0) / 16u);
     // halo piece k of this thread: pixel piece / 8, channels (piece % 8) * 8 .. + 7 of the slice
     const bf16* hsrc[kHaloPer];
     bool hval[kHaloPer];
@@ -395,6 +402,36 @@ __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, co
                                                                                      127, 0, 127);
         }
     }
+    if (FULL && !F32OUT) {  // the bf16 tile through LDS and out as 16-byte stores, as in conv3x3_k
+        constexpr uint32_t kPitch = kCo * 2u + 16u;
+        static_assert(kTile * kTile * kPitch <= sizeof(lds.stage), "output stage fits the LDS tile");
+        char* stage = lds.stage;
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < NCT; ++t) {
+            const uint32_t ch = uint32_t(t) * 16u + (lane & 15u);
+            const float sc = wscale[co0 + ch], bv = bias ? bias[co0 + ch] : 0.0f;
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    float v = fmaf(acc[m][t][r], sc, bv);
+                    if (RELU) v = fmaxf(v, 0.0f);
+                    const uint32_t p = (wv * 4u + uint32_t(m)) * kTile + kq * 4u + uint32_t(r);
+                    *reinterpret_cast<bf16*>(stage + p * kPitch + ch * 2u) = b(v);
+                }
+        }
+        __syncthreads();
+        constexpr uint32_t kChunks = kTile * kTile * kCo / 8u;
+#pragma unroll
+        for (uint32_t i = tid; i < kChunks; i += kBlock) {
+            const uint32_t p = i / (kCo / 8u), c8 = i % (kCo / 8u);
+            const size_t op = (size_t(img) * d.h + uint32_t(ty0) + p / kTile) * d.w + uint32_t(tx0) + p % kTile;
+            store8(out.p + op * out.stride + out.off + co0 + c8 * 8u,
+                   *reinterpret_cast<const bf16x8*>(stage + p * kPitch + c8 * 16u));
+        }
+        return;
+    }
 #pragma unroll
     for (int t = 0; t < NCT; ++t) {
         if (uint32_t(t) >= nct) continue;
@@ -428,15 +465,23 @@ static void conv8_launch_sl(const Tens& in, uint32_t cin, const uint8_t* w, cons
     const uint32_t tiles = d.n * ((d.h + kTile - 1u) / kTile) * ((d.w + kTile - 1u) / kTile);
     const uint32_t groups = (cout / 16u + NCT - 1u) / NCT;
     const dim3 grid(tiles, groups);
+    const bool full = cout % (uint32_t(NCT) * 16u) == 0 && d.h % kTile == 0 && d.w % kTile == 0 &&
+                      out.stride % 8u == 0 && out.off % 8u == 0;
     if (of32)
-        hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, false, true>), grid, dim3(kBlock), 0, s, in, cin, w, inv_scale,
-                           wscale, bias, out, cout, d, of32, f32c);
+        hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, false, true, false>), grid, dim3(kBlock), 0, s, in, cin, w,
+                           inv_scale, wscale, bias, out, cout, d, of32, f32c);
+    else if (relu && full)
+        hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, true, false, true>), grid, dim3(kBlock), 0, s, in, cin, w,
+                           inv_scale, wscale, bias, out, cout, d, of32, f32c);
     else if (relu)
-        hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, true, false>), grid, dim3(kBlock), 0, s, in, cin, w, inv_scale,
-                           wscale, bias, out, cout, d, of32, f32c);
+        hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, true, false, false>), grid, dim3(kBlock), 0, s, in, cin, w,
+                           inv_scale, wscale, bias, out, cout, d, of32, f32c);
+    else if (full)
+        hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, false, false, true>), grid, dim3(kBlock), 0, s, in, cin, w,
+                           inv_scale, wscale, bias, out, cout, d, of32, f32c);
     else
-        hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, false, false>), grid, dim3(kBlock), 0, s, in, cin, w, inv_scale,
-                           wscale, bias, out, cout, d, of32, f32c);
+        hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, false, false, false>), grid, dim3(kBlock), 0, s, in, cin, w,
+                           inv_scale, wscale, bias, out, cout, d, of32, f32c);
 }
 
 template <int NCT>
