@@ -93,6 +93,32 @@ int ptgen_set_sky(ptgen_scene* s, uint32_t width, uint32_t height, const float* 
 int ptgen_set_sky_procedural(ptgen_scene* s, uint32_t width, uint32_t height,
                              uint64_t seed, float sun_peak);
 
+/* Asset ingest.  ptgen_parse_hdr reads a Radiance RGBE (.hdr) file as
+ * lib/parse-hdr.js does (parse-hdr.js:20-211, as HDRTexture._load uses it,
+ * tracer.js:53-73): the header's first 20 lines (FORMAT=32-bit_rle_rgbe
+ * required, "-Y H +X W" size), new-style run-length scanlines or flat RGBE,
+ * each channel decoded as byte / 255 * 2^(e - 128) in double and stored as
+ * float, alpha 1, rows top first.  rgba == NULL asks for the size only;
+ * otherwise rgba_count must be W * H * 4.  `exposure` receives the header's
+ * EXPOSURE (the reference does not apply it; ptgen_set_sky's exposure is
+ * HDRTexture's).
+ * ptgen_obj_triangles / ptgen_obj_read read a Wavefront OBJ text as
+ * ModelData.parseOBJ + generateTriangles do (tracer.js:949-1022): v, vt, vn
+ * and f lines (v, v/vt, v/vt/vn, v//vn; 1-based or negative = relative
+ * indices), faces fan-triangulated around their first corner, one output
+ * vertex per corner (positions and normals 9 floats per triangle, uvs 6).  A
+ * corner without a normal gets its face's geometric normal and one without
+ * texture coordinates (0, 0), where the reference would read undefined (NaN).
+ * ptgen_add_obj adds such a mesh as a model (ptgen_add_model) and returns its
+ * index.  ptgen_asset_error describes the last failure on this thread. */
+int ptgen_parse_hdr(const uint8_t* bytes, uint64_t n, uint32_t* width, uint32_t* height, float* exposure,
+                    float* rgba, uint64_t rgba_count);
+int ptgen_obj_triangles(const char* text, uint64_t n, uint32_t* n_triangles);
+int ptgen_obj_read(const char* text, uint64_t n, float* positions, float* normals, float* uvs,
+                   uint32_t n_triangles);
+int ptgen_add_obj(ptgen_scene* s, const char* text, uint64_t n);
+const char* ptgen_asset_error(void);
+
 int ptgen_set_camera(ptgen_scene* s, const float eye[3], const float target[3], float fov_deg,
                      float exposure);
 int ptgen_set_bounces(ptgen_scene* s, int bounces);

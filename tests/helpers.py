@@ -69,3 +69,43 @@ def rel_mse(x, ref):
     x = np.asarray(x, np.float64)
     ref = np.asarray(ref, np.float64)
     return float(np.mean((x - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+
+def _rle_channel(vals):
+    """Encode one channel of a scanline: runs of >= 3 equal bytes as runs,
+    the rest as literal packets (at most 128 / 127 bytes each)."""
+    out, i, n = bytearray(), 0, len(vals)
+    while i < n:
+        j = i
+        while j < n and vals[j] == vals[i] and j - i < 127:
+            j += 1
+        if j - i >= 3:
+            out += bytes([128 + (j - i), vals[i]])
+            i = j
+            continue
+        k = i
+        while k < n and k - i < 128:
+            if k + 2 < n and vals[k] == vals[k + 1] == vals[k + 2]:
+                break
+            k += 1
+        k = max(k, i + 1)
+        out += bytes([k - i]) + bytes(vals[i:k])
+        i = k
+    return bytes(out)
+
+
+def hdr_bytes(rgbe, rle=True, header=None, flat_from=None):
+    """A Radiance RGBE file of the [h][w][4] byte image: new-style run-length
+    scanlines (flat from scanline `flat_from`, or all flat without rle)."""
+    h, w = rgbe.shape[:2]
+    head = header if header is not None else ["#?RADIANCE", "# written by the test", "FORMAT=32-bit_rle_rgbe",
+                                              "EXPOSURE= 1.50", "", "-Y %d +X %d" % (h, w)]
+    out = bytearray(("\n".join(head) + "\n").encode())
+    for y in range(h):
+        if not rle or (flat_from is not None and y >= flat_from):
+            out += rgbe[y:].astype(np.uint8).tobytes()
+            break
+        out += bytes([2, 2, w >> 8, w & 255])
+        for c in range(4):
+            out += _rle_channel(bytes(rgbe[y, :, c].astype(np.uint8)))
+    return bytes(out)

@@ -207,6 +207,11 @@ PTGEN_FUNCS = {
     "ptgen_add_torus": (C.c_int, [vp, C.c_int, P(f32), P(f32), f32, f32]),
     "ptgen_add_plane": (C.c_int, [vp, C.c_int, P(f32), f32]),
     "ptgen_add_model": (C.c_int, [vp, vp, vp, vp, u32, vp, u32]),
+    "ptgen_parse_hdr": (C.c_int, [vp, u64, P(u32), P(u32), P(f32), vp, u64]),
+    "ptgen_obj_triangles": (C.c_int, [C.c_char_p, u64, P(u32)]),
+    "ptgen_obj_read": (C.c_int, [C.c_char_p, u64, vp, vp, vp, u32]),
+    "ptgen_add_obj": (C.c_int, [vp, C.c_char_p, u64]),
+    "ptgen_asset_error": (C.c_char_p, []),
     "ptgen_add_instance": (C.c_int, [vp, C.c_int, C.c_int, P(f32), P(f32), P(f32), C.c_int]),
     "ptgen_set_sky": (C.c_int, [vp, u32, u32, vp, f32]),
     "ptgen_set_sky_procedural": (C.c_int, [vp, u32, u32, u64, f32]),

@@ -53,7 +53,7 @@ def build_scene():
     out = os.path.join(LIB, "libptscene.so")
     tmp = out + ".tmp"
     _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-I" + INC,
-          os.path.join(CSRC, "scene", "scene_gen.cpp"), "-o", tmp])
+          os.path.join(CSRC, "scene", "scene_gen.cpp"), os.path.join(CSRC, "scene", "assets.cpp"), "-o", tmp])
     os.replace(tmp, out)
     return out
 

@@ -5,6 +5,7 @@ newCylinder / newTorus / newPlane / newModel, Material, HDRTexture, Camera)
 and produces the packed buffers Renderer.setScene uploads.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -118,6 +119,18 @@ class SceneBuilder:
         self._keep.append(a)
         return self._check(self.lib.ptgen_set_sky(self.h, w, h, a.ctypes.data, exposure))
 
+    def sky_from_hdr(self, source, exposure=1.0):
+        """The sky from a Radiance .hdr file (a path or its bytes), as
+        HDRTexture loads one (tracer.js:53-87)."""
+        rgba, _ = read_hdr(source)
+        return self.sky(rgba[..., :3], exposure)
+
+    def model_from_obj(self, source):
+        """A model from a Wavefront OBJ text (a path, str or bytes), as
+        ModelData.parseOBJ reads one (tracer.js:949-1022); returns its index."""
+        text = _obj_bytes(source)
+        return self._check(self.lib.ptgen_add_obj(self.h, text, len(text)))
+
     def procedural_sky(self, width, height, seed=1, sun_peak=1e3):
         return self._check(self.lib.ptgen_set_sky_procedural(self.h, width, height, seed, sun_peak))
 
@@ -178,3 +191,52 @@ def build_sky_cdf(rgba16f, exposure=1.0):
     if rc:
         raise A.PTError(rc, "ptgen_build_sky_cdf failed")
     return cdf, tot.value
+
+
+def _asset_error(lib, rc, what):
+    raise A.PTError(rc, "%s: %s" % (what, (lib.ptgen_asset_error() or b"").decode()))
+
+
+def read_hdr(source):
+    """A Radiance RGBE image (a path or its bytes) as parse-hdr.js decodes it:
+    (float32 [height][width][4] with alpha 1, rows top first; the header's
+    EXPOSURE, which the reference does not apply)."""
+    data = source if isinstance(source, (bytes, bytearray)) else open(source, "rb").read()
+    lib = A.load_scene_lib()
+    buf = (C.c_uint8 * len(data)).from_buffer_copy(data)
+    w, h, e = A.u32(), A.u32(), A.f32()
+    rc = lib.ptgen_parse_hdr(buf, len(data), C.byref(w), C.byref(h), C.byref(e), None, 0)
+    if rc:
+        _asset_error(lib, rc, "hdr header")
+    out = np.empty((h.value, w.value, 4), np.float32)
+    rc = lib.ptgen_parse_hdr(buf, len(data), C.byref(w), C.byref(h), C.byref(e), out.ctypes.data, out.size)
+    if rc:
+        _asset_error(lib, rc, "hdr pixels")
+    return out, float(e.value)
+
+
+def _obj_bytes(source):
+    if isinstance(source, (bytes, bytearray)):
+        return bytes(source)
+    if isinstance(source, str) and ("\n" in source or not os.path.exists(source)):
+        return source.encode()
+    return open(source, "rb").read()
+
+
+def read_obj(source):
+    """A Wavefront OBJ mesh (a path, str or bytes) as parseOBJ +
+    generateTriangles read it, one vertex per triangle corner:
+    (positions [3T][3], normals [3T][3], uvs [3T][2]) float32."""
+    text = _obj_bytes(source)
+    lib = A.load_scene_lib()
+    n = A.u32()
+    rc = lib.ptgen_obj_triangles(text, len(text), C.byref(n))
+    if rc:
+        _asset_error(lib, rc, "obj")
+    t = n.value
+    pos, nrm, uv = (np.empty((3 * t, 3), np.float32), np.empty((3 * t, 3), np.float32),
+                    np.empty((3 * t, 2), np.float32))
+    rc = lib.ptgen_obj_read(text, len(text), pos.ctypes.data, nrm.ctypes.data, uv.ctypes.data, t)
+    if rc:
+        _asset_error(lib, rc, "obj")
+    return pos, nrm, uv
