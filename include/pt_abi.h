@@ -33,7 +33,7 @@ enum {
     PT_ERR_HIP = -5,         /* a HIP runtime call failed */
     PT_ERR_OOM = -6,         /* device allocation failed */
     PT_ERR_STATE = -7,       /* no scene uploaded / no frame allocated */
-    PT_ERR_UNSUPPORTED = -8, /* feature outside this build (textures, POM) */
+    PT_ERR_UNSUPPORTED = -8, /* feature outside this build (normal maps, POM, translucent albedo) */
     PT_ERR_NO_DEVICE = -9    /* no HIP device visible */
 };
 
@@ -69,8 +69,11 @@ typedef struct pt_scene_desc {
     uint64_t sky_cdf_count;
     float sky_exposure;
     float sky_total_lum;    /* ignored when the CDF is built on the device */
-    /* Texture array, 512x512 rgba8 layers (binding 11).  Must be empty in
-     * this build: materials with a texture index >= 0 are rejected. */
+    /* Texture array, 512x512 rgba8 layers, rows top first (binding 11),
+     * sampled as tracer.js sets it up (linear, repeat, level 0).  Materials
+     * may use albedo (opaque: every alpha 255), roughness, metallic and
+     * emissive maps; normal and height maps (POM) are rejected with
+     * PT_ERR_UNSUPPORTED.  pt_update_objects keeps the uploaded texels. */
     const uint8_t* textures;
     uint32_t texture_layers;
     uint32_t flags;         /* PT_HAS_* / PT_MIS_SKYBOX */

@@ -93,6 +93,15 @@ int ptgen_set_sky(ptgen_scene* s, uint32_t width, uint32_t height, const float* 
 int ptgen_set_sky_procedural(ptgen_scene* s, uint32_t width, uint32_t height,
                              uint64_t seed, float sun_peak);
 
+/* Material textures: ptgen_add_texture appends one 512x512 rgba8 layer
+ * (rows top first) and returns its index; ptgen_set_material_maps sets a
+ * material's texture indices in Material.getSchema order (emissive, albedo,
+ * normal, height, roughness, metallic; -1 = none) and its uv_scale (NULL
+ * keeps (1, 1)).  A material with a height map sets PT_HAS_HEIGHTMAPS, as
+ * packSceneData does (tracer.js:1825). */
+int ptgen_add_texture(ptgen_scene* s, const uint8_t* rgba);
+int ptgen_set_material_maps(ptgen_scene* s, int material, const int32_t maps[6], const float uv_scale[2]);
+
 /* Asset ingest.  ptgen_parse_hdr reads a Radiance RGBE (.hdr) file as
  * lib/parse-hdr.js does (parse-hdr.js:20-211, as HDRTexture._load uses it,
  * tracer.js:53-73): the header's first 20 lines (FORMAT=32-bit_rle_rgbe

@@ -127,6 +127,8 @@ struct ref_scene {
     std::vector<float> cdf;
     uint32_t flags = 0;
     int bounce_limit = 8;
+    std::vector<uint8_t> tex;  // texture array: layers of 512 x 512 rgba8
+    uint32_t tex_layers = 0;
     bool has(uint32_t f) const { return (flags & f) != 0; }
 };
 
@@ -900,8 +902,37 @@ struct Rng {
 
 struct SurfaceContext { f3 normal, surface_normal, albedo; float roughness, metallic; f3 emittance; float alpha; };
 
-// shader.wgsl:1393-1440 without textures
-SurfaceContext get_surface_context(const SurfaceHit& hit, const pt_material& mat) {
+// sample_texture (shader.wgsl:170-173): textureSampleLevel on the material
+// texture array, level 0 of a 512 x 512 rgba8unorm layer, linear filtering,
+// repeat addressing (tracer.js:1965-1968); the filter is restated with fp32
+// weights (texel centres at uv * 512 - 0.5).  idx < 0 gives 1.
+constexpr int kTexSize = 512;
+struct f4 { float x, y, z, w; };
+f4 sample_texture(const ref_scene& S, int idx, f2 uv) {
+    if (idx < 0 || uint32_t(idx) >= S.tex_layers) return f4{1.f, 1.f, 1.f, 1.f};
+    const float x = uv.x * float(kTexSize) - 0.5f, y = uv.y * float(kTexSize) - 0.5f;
+    const float x0 = std::floor(x), y0 = std::floor(y);
+    const float fx = x - x0, fy = y - y0;
+    const long ix = long(x0), iy = long(y0);
+    const uint8_t* L = S.tex.data() + size_t(idx) * kTexSize * kTexSize * 4;
+    auto texel = [&](long tx, long ty, int c) {
+        const size_t xx = size_t(tx & (kTexSize - 1)), yy = size_t(ty & (kTexSize - 1));
+        return float(L[(yy * kTexSize + xx) * 4 + size_t(c)]) / 255.0f;
+    };
+    float o[4];
+    for (int c = 0; c < 4; ++c) {
+        const float t00 = texel(ix, iy, c), t10 = texel(ix + 1, iy, c);
+        const float t01 = texel(ix, iy + 1, c), t11 = texel(ix + 1, iy + 1, c);
+        const float a = t00 + (t10 - t00) * fx, b = t01 + (t11 - t01) * fx;
+        o[c] = a + (b - a) * fy;
+    }
+    return f4{o[0], o[1], o[2], o[3]};
+}
+
+// shader.wgsl:1393-1440 with albedo (its alpha too), roughness, metallic
+// and emissive maps at uv = hit_uv * uv_scale; normal maps are not supported
+// (rejected at upload).
+SurfaceContext get_surface_context(const ref_scene& S, const SurfaceHit& hit, const pt_material& mat) {
     SurfaceContext ctx;
     ctx.normal = hit.hit_n;
     ctx.surface_normal = hit.hit_n;
@@ -910,6 +941,21 @@ SurfaceContext get_surface_context(const SurfaceHit& hit, const pt_material& mat
     ctx.metallic = mat.metallic;
     ctx.roughness = mat.roughness;
     ctx.emittance = ld3(mat.emittance);
+    if (S.tex_layers == 0) return ctx;
+    const f2 uv{hit.hit_uv.x * mat.uv_scale[0], hit.hit_uv.y * mat.uv_scale[1]};
+    if (mat.albedo_idx >= 0) {
+        const f4 t = sample_texture(S, mat.albedo_idx, uv);
+        ctx.alpha = t.w;
+        ctx.albedo = V(ctx.albedo.x * std::pow(t.x, 2.2f), ctx.albedo.y * std::pow(t.y, 2.2f),
+                       ctx.albedo.z * std::pow(t.z, 2.2f));
+    }
+    if (mat.roughness_idx >= 0) ctx.roughness *= sample_texture(S, mat.roughness_idx, uv).y;
+    if (mat.metallic_idx >= 0) ctx.metallic *= sample_texture(S, mat.metallic_idx, uv).z;
+    if (mat.emissive_idx >= 0) {
+        const f4 t = sample_texture(S, mat.emissive_idx, uv);
+        ctx.emittance = V(ctx.emittance.x * std::pow(t.x, 2.2f), ctx.emittance.y * std::pow(t.y, 2.2f),
+                          ctx.emittance.z * std::pow(t.z, 2.2f));
+    }
     return ctx;
 }
 
@@ -1482,7 +1528,7 @@ f3 trace_path(const ref_scene& S, const pt_scene_params& P, uint32_t x, uint32_t
         }
         c->shade++;
         const pt_material& mat = S.mats[hit.m_idx];
-        SurfaceContext ctx = get_surface_context(hit, mat);
+        SurfaceContext ctx = get_surface_context(S, hit, mat);
         if (length(ctx.emittance) > 0.0f) {
             float weight = 1.0f;
             if (S.has(PT_HAS_LIGHTS) && bounce > 0 && hit.o_idx >= 0 && obj.light_idx >= 0) {
@@ -1602,6 +1648,11 @@ int ref_scene_create(ref_scene** out, const pt_scene_desc* d) {
     s->cdf.assign(d->sky_cdf, d->sky_cdf + d->sky_cdf_count);
     s->flags = d->flags;
     s->bounce_limit = d->bounce_limit;
+    if (d->texture_layers > 0) {
+        if (!d->textures) { delete s; return PT_ERR_LAYOUT; }
+        s->tex.assign(d->textures, d->textures + size_t(d->texture_layers) * kTexSize * kTexSize * 4);
+        s->tex_layers = d->texture_layers;
+    }
     *out = s;
     return PT_OK;
 }
@@ -1718,7 +1769,7 @@ int ref_gbuffer(const ref_scene* s, const pt_scene_params* params, pt_gbuffer_te
                     break;
                 }
                 mat = S.mats[hit.m_idx];
-                ctx = get_surface_context(hit, mat);
+                ctx = get_surface_context(S, hit, mat);
                 ctx.normal = dot(ctx.normal, -ray.direction) > 0.0f ? ctx.normal : -ctx.normal;
                 dist += hit.t;
                 if (ctx.alpha < 1.0f) {
@@ -1871,7 +1922,8 @@ int ref_denoise(const pt_denoise_params* u, int stride_mode, const float* accum,
 static SurfaceContext bsdf_ctx(const pt_material& m, const float n[3]) {
     SurfaceHit h = empty_hit();
     h.hit_n = V(n[0], n[1], n[2]);
-    return get_surface_context(h, m);
+    static const ref_scene kNoTextures;
+    return get_surface_context(kNoTextures, h, m);
 }
 
 int ref_bsdf_sample(const pt_material* m, const float n[3], const float v[3], uint32_t n_samples, uint32_t seed,

@@ -627,3 +627,133 @@ def test_render_scene_from_ingested_assets_matches_oracle(renderer):
     print("ingested-assets 1spp pixel agreement", frac)
     assert frac >= 0.97
     assert rst.tri_tests > 0 and abs(int(st.extension_rays) - int(rst.extension_rays)) <= 0.01 * rst.extension_rays
+
+
+# ---- material textures ----
+
+def _tex_layer(kind, seed):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:512, 0:512]
+    t = np.empty((512, 512, 4), np.uint8)
+    if kind == "checker":
+        c = ((xx // 16 + yy // 16) % 2).astype(np.uint8)
+        t[..., 0], t[..., 1], t[..., 2] = 40 + 200 * c, 90 + 120 * c, 200 - 150 * c
+    elif kind == "ramp":
+        t[..., 0] = (xx // 2).astype(np.uint8)
+        t[..., 1] = (yy // 2).astype(np.uint8)
+        t[..., 2] = ((xx + yy) // 4).astype(np.uint8)
+    else:
+        t[..., :3] = rng.integers(0, 256, (512, 512, 3)).astype(np.uint8)
+    t[..., 3] = 255
+    return t
+
+
+def _textured_scene():
+    """Albedo, roughness, metallic and emissive maps on every primitive type
+    (the light's emission is textured too), with uv scales."""
+    b = SceneBuilder()
+    b.procedural_sky(128, 64, seed=5)
+    checker, ramp, noise = b.texture(_tex_layer("checker", 1)), b.texture(_tex_layer("ramp", 2)), \
+        b.texture(_tex_layer("noise", 3))
+    m_plane = b.material(color=(0.9, 0.9, 0.9), roughness=0.8)
+    b.material_maps(m_plane, albedo=checker, roughness=noise, uv_scale=(0.25, 0.25))
+    m_sph = b.material(color=(1.0, 0.9, 0.8), roughness=0.3, metallic=0.8)
+    b.material_maps(m_sph, albedo=ramp, metallic=noise, roughness=checker, uv_scale=(2.0, 1.0))
+    m_box = b.material(color=(0.7, 0.8, 0.9), roughness=0.5)
+    b.material_maps(m_box, albedo=noise)
+    m_light = b.material(color=(1.0, 1.0, 1.0), emittance=(1.0, 0.9, 0.8), emission_intensity=12.0)
+    b.material_maps(m_light, emissive=ramp)
+    m_plain = b.material(color=(0.5, 0.6, 0.3), roughness=0.4)
+    m_mesh = b.material(color=(0.8, 0.5, 0.5), roughness=0.3)
+    b.material_maps(m_mesh, albedo=checker, metallic=ramp)
+    b.plane(m_plane, (0.0, 1.0, 0.0), -0.5)
+    for k in range(4):
+        b.sphere(m_sph, (float(k) - 1.5, 0.2, -0.5), 0.35)
+    b.cube(m_box, (1.2, 0.1, 0.8), (0.3, 0.3, 0.3))
+    b.cube(m_light, (0.0, 2.2, 0.0), (0.8, 0.01, 0.8), enable_nee=True)
+    b.cylinder(m_plain, (-1.3, -0.5, 0.9), 0.3, 0.8)
+    b.torus(m_plain, (0.3, 0.0, 1.2), 0.4, 0.12)
+    mesh = b.model_from_obj(_octahedron_obj())
+    b.instance(mesh, m_mesh, (-0.4, 0.3, 1.0), scale=(0.35, 0.35, 0.35))
+    b.camera()
+    b.bounces(5)
+    return b.finalize()
+
+
+def test_textured_scene_matches_oracle(renderer):
+    """Textured materials (runs on the kernels' run-time-flags variant): the
+    1-spp render and the G-buffer agree with the oracle, both backends give
+    the same bits, and the textures change the image."""
+    scene = _textured_scene()
+    d = scene.desc()
+    assert d.texture_layers == 3
+    w, h = 96, 64
+    ref = _setup(renderer, scene, w, h, gbuf=True)
+    p = scene.params(w, h, sample_number=0, seed=2024)
+    st = renderer.render_section(p, passes=1)
+    ga = renderer.read_accum()
+    ra, _, rst = ref.render(p, passes=1)
+    rel = np.abs(ga[..., :3] - ra[..., :3]) / np.maximum(np.abs(ra[..., :3]), 1e-3)
+    frac = float(np.mean(np.all(rel <= 1e-4, axis=-1)))
+    print("textured 1spp pixel agreement", frac)
+    assert frac >= 0.97
+    assert abs(int(st.extension_rays) - int(rst.extension_rays)) <= 0.01 * rst.extension_rays
+    # G-buffer (textured albedo, roughness, metallic)
+    renderer.gbuffer(scene.params(w, h))
+    gg = renderer.read_gbuffer()
+    rg, _ = ref.gbuffer(scene.params(w, h))
+    gerr = np.abs(gg - rg) / np.maximum(np.abs(rg), 1e-3)
+    print("gbuffer max rel err", float(gerr.max()))
+    assert float(np.mean(np.all(gerr <= 1e-4, axis=-1))) >= 0.99
+    # both backends, same bits
+    renderer.reset()
+    accs = []
+    for be in (A.PT_BACKEND_WAVEFRONT, A.PT_BACKEND_MEGAKERNEL):
+        renderer.set_option(A.PT_OPT_BACKEND, be)
+        renderer.resize(w, h)
+        renderer.render_section(p, passes=2, seeds=[11, 12])
+        accs.append(renderer.read_accum())
+    renderer.set_option(A.PT_OPT_BACKEND, A.PT_BACKEND_WAVEFRONT)
+    assert np.array_equal(accs[0], accs[1])
+    # without the maps the image differs
+    plain = scene.desc()
+    mats = (A.Material * plain.materials.count).from_address(plain.materials.ptr)
+    saved = [(m.albedo_idx, m.roughness_idx, m.metallic_idx, m.emissive_idx) for m in mats]
+    try:
+        for m in mats:
+            m.albedo_idx = m.roughness_idx = m.metallic_idx = m.emissive_idx = -1
+        renderer.set_scene(plain, isgbuf=True)
+        renderer.resize(w, h)
+        renderer.render_section(p, passes=1)
+        assert np.mean(np.abs(renderer.read_accum() - ga)) > 0.02 * np.mean(np.abs(ga))
+    finally:
+        for m, s in zip(mats, saved):
+            m.albedo_idx, m.roughness_idx, m.metallic_idx, m.emissive_idx = s
+
+
+@pytest.mark.parametrize("bad", ["normal", "height", "translucent", "index"])
+def test_texture_validation(renderer, bad):
+    b = SceneBuilder()
+    layer = _tex_layer("noise", 7)
+    if bad == "translucent":
+        layer[10, 20, 3] = 128
+    t = b.texture(layer)
+    m = b.material(color=(0.5, 0.5, 0.5))
+    if bad == "normal":
+        b.material_maps(m, normal=t)
+    elif bad == "height":
+        b.material_maps(m, height=t)
+    else:
+        b.material_maps(m, albedo=t)
+    b.sphere(m, (0.0, 0.0, 0.0), 0.5)
+    b.sky()
+    b.camera()
+    scene = b.finalize()
+    d = scene.desc()
+    if bad == "index":
+        mats = (A.Material * d.materials.count).from_address(d.materials.ptr)
+        mats[m].albedo_idx = 5
+    with pytest.raises(A.PTError) as e:
+        renderer.set_scene(d)
+    want = A.PT_ERR_LAYOUT if bad == "index" else A.PT_ERR_UNSUPPORTED
+    assert e.value.code == want, e.value

@@ -73,6 +73,8 @@ struct DevScene {
     const pt_light* lights;
     const uint16_t* sky;           // rgba16f
     const float* cdf;
+    const uint32_t* tex;           // material texture array: layers of 512 x 512 rgba8
+    uint32_t tex_layers;
     uint32_t n_planes, n_meshes, n_lights, n_objs;
     uint32_t sky_w, sky_h;
     uint32_t flags;
@@ -96,6 +98,10 @@ struct Flags {
     __device__ __forceinline__ bool has(uint32_t f) const { return F == kDynamicFlags ? (rt & f) != 0 : (F & f) != 0; }
 };
 constexpr uint32_t kAnyObject = PT_HAS_SPHERES | PT_HAS_CUBES | PT_HAS_CYLINDERS | PT_HAS_TORI | PT_HAS_MESHES;
+// Not a scene flag of the ABI: set in DevScene::flags when a material uses a
+// texture, so the kernels specialised for the configuration flag sets (which
+// lack it) compile the texture code out.
+constexpr uint32_t kTexturesBit = 1u << 20;
 
 // ---------------------------------------------------------------- stack ----
 // Per-lane traversal stack.  The first `cap` entries live in LDS,

@@ -77,6 +77,7 @@ __device__ bool shade_bounce(const DevScene& S, Flags<F> fl, const Params& P, in
     sc.metallic = m.metallic;
     sc.roughness = m.roughness;
     sc.emittance = ld3(m.emittance);
+    if (fl.has(kTexturesBit)) apply_textures(S, m, sh.uv, sc);
     // implicit light hit (shader.wgsl:2377-2395)
     if (len(sc.emittance) > 0.0f) {
         float w = 1.0f;

@@ -583,4 +583,16 @@ __device__ SurfHit resolve_hit(const DevScene& S, v3 o, v3 d, const HitRec& h) {
     return r;
 }
 
+// The hit as shading needs it: texture coordinates only where a material
+// may use a texture (never in the kernels specialised for the configuration
+// flag sets).
+template <uint32_t F>
+__device__ __forceinline__ SurfHit resolve_for_shading(const DevScene& S, Flags<F> fl, v3 o, v3 d, const HitRec& h) {
+    if constexpr (F != kDynamicFlags && !(F & kTexturesBit)) {
+        return resolve_hit<false>(S, o, d, h);
+    } else {
+        return fl.has(kTexturesBit) ? resolve_hit<true>(S, o, d, h) : resolve_hit<false>(S, o, d, h);
+    }
+}
+
 }  // namespace pt

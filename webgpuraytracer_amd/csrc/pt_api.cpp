@@ -48,13 +48,31 @@ int validate_desc(pt_ctx* c, const pt_scene_desc* d) {
         if (!check_array(*x.a, x.stride, x.a == &d->tlas && c->tlas_build == PT_TLAS_BUILD_DEVICE))
             return c->fail(PT_ERR_LAYOUT, "%s: expected a non-empty array with stride %u (got count %llu stride %u)",
                            x.name, x.stride, (unsigned long long)x.a->count, x.a->stride);
-    if (d->flags & PT_HAS_HEIGHTMAPS) return c->fail(PT_ERR_UNSUPPORTED, "heightmaps (POM) are not supported");
-    if (d->texture_layers != 0) return c->fail(PT_ERR_UNSUPPORTED, "texture arrays are not supported");
+    if (d->texture_layers > 0 && !d->textures) return c->fail(PT_ERR_ARG, "texture_layers > 0 without texels");
+    if (d->texture_layers > 4096) return c->fail(PT_ERR_ARG, "at most 4096 texture layers");
     const pt_material* m = static_cast<const pt_material*>(d->materials.ptr);
-    for (uint64_t i = 0; i < d->materials.count; ++i)
-        if (m[i].emissive_idx >= 0 || m[i].albedo_idx >= 0 || m[i].normal_idx >= 0 || m[i].height_idx >= 0 ||
-            m[i].roughness_idx >= 0 || m[i].metallic_idx >= 0)
-            return c->fail(PT_ERR_UNSUPPORTED, "material %llu references a texture", (unsigned long long)i);
+    const uint8_t* tex = d->textures;
+    for (uint64_t i = 0; i < d->materials.count; ++i) {
+        // normal and height maps (and parallax mapping) need the hit's tangent frame, which the reference
+        // sets only on sphere, cube and plane hits
+        if (m[i].normal_idx >= 0 || m[i].height_idx >= 0)
+            return c->fail(PT_ERR_UNSUPPORTED, "material %llu: normal and height maps are not supported",
+                           (unsigned long long)i);
+        const int32_t idx[4] = {m[i].albedo_idx, m[i].roughness_idx, m[i].metallic_idx, m[i].emissive_idx};
+        for (int32_t t : idx)
+            if (t >= 0 && uint32_t(t) >= d->texture_layers)
+                return c->fail(PT_ERR_LAYOUT, "material %llu: texture %d outside the %u layers",
+                               (unsigned long long)i, t, d->texture_layers);
+        // a translucent albedo texture would make shadow rays attenuate instead of stop
+        // (evaluate_shadow_attenuation, shader.wgsl:777-811)
+        if (m[i].albedo_idx >= 0) {
+            const uint8_t* L = tex + size_t(m[i].albedo_idx) * 512 * 512 * 4;
+            for (size_t k = 3; k < size_t(512) * 512 * 4; k += 4)
+                if (L[k] != 255)
+                    return c->fail(PT_ERR_UNSUPPORTED, "material %llu: albedo texture %d is not opaque",
+                                   (unsigned long long)i, m[i].albedo_idx);
+        }
+    }
     if (d->bounce_limit < 0) return c->fail(PT_ERR_ARG, "bounce_limit < 0");
     if (!d->sky_rgba16f || d->sky_width == 0 || d->sky_height == 0) return c->fail(PT_ERR_ARG, "sky texels missing");
     if (d->sky_cdf) {
@@ -125,10 +143,20 @@ void refresh_dev(pt_ctx* c) {
     S.n_objs = uint32_t(c->n_objs);
     S.sky_w = c->sky_w;
     S.sky_h = c->sky_h;
-    S.flags = c->flags;
+    S.flags = c->flags | (c->has_textures ? kTexturesBit : 0u);
+    S.tex = static_cast<const uint32_t*>(c->tex.p);
+    S.tex_layers = c->tex_layers;
     S.bounce_limit = c->bounce_limit;
     S.stack_cap = c->stack_cap;
     S.wstack_cap = c->wstack_cap;
+}
+
+bool uses_textures(const pt_scene_desc* d) {
+    const pt_material* m = static_cast<const pt_material*>(d->materials.ptr);
+    for (uint64_t i = 0; i < d->materials.count; ++i)
+        if (m[i].albedo_idx >= 0 || m[i].roughness_idx >= 0 || m[i].metallic_idx >= 0 || m[i].emissive_idx >= 0)
+            return true;
+    return false;
 }
 
 int upload_dynamic(pt_ctx* c, const pt_scene_desc* d, bool first) {
@@ -437,6 +465,13 @@ int pt_upload_scene(pt_ctx* c, const pt_scene_desc* d) {
     if ((rc = c->upload(c->tris, tr, d->triangles.count * sizeof(pt_triangle), "triangles"))) return rc;
     if ((rc = c->upload(c->bvh_raw, d->bvh.ptr, d->bvh.count * sizeof(pt_bvh_node), "bvh"))) return rc;
     c->tri_geo_w.release();
+    c->tex_layers = d->texture_layers;
+    c->has_textures = uses_textures(d);
+    if (d->texture_layers > 0) {
+        if ((rc = c->upload(c->tex, d->textures, size_t(d->texture_layers) * 512 * 512 * 4, "textures"))) return rc;
+    } else {
+        c->tex.release();
+    }
     if (c->tlas_build == PT_TLAS_BUILD_DEVICE && (rc = device_tlas(c, d))) return rc;
     if (c->blas_build == PT_BLAS_BUILD_DEVICE && (rc = device_blas(c, d))) return rc;
     if ((rc = c->upload(c->lights, d->lights.ptr, d->lights.count * sizeof(pt_light), "lights"))) return rc;
@@ -483,10 +518,11 @@ int pt_update_objects(pt_ctx* c, const pt_scene_desc* d) {
     if (d->flags != c->flags) return c->fail(PT_ERR_FLAGS, "specialisation flags changed");
     if (d->objects.count > c->n_objs || d->tlas.count > c->n_tlas || d->planes.count > c->n_planes ||
         d->meshes.count > c->n_meshes || d->materials.count > c->n_mats || d->bvh.count != c->n_bvh ||
-        d->triangles.count != c->n_tris)
+        d->triangles.count != c->n_tris || d->texture_layers != c->tex_layers)
         return c->fail(PT_ERR_CAPACITY, "scene arrays outgrew the uploaded buffers");
     c->bounce_limit = d->bounce_limit;
     if ((rc = upload_dynamic(c, d, false))) return rc;
+    c->has_textures = uses_textures(d);  // the texels stay those of the upload
     if (c->tlas_build == PT_TLAS_BUILD_DEVICE && (rc = device_tlas(c, d))) return rc;
     refresh_dev(c);
     return c->sync("update");

@@ -58,6 +58,9 @@ struct pt_ctx {
     uint32_t blas_wneed = 0, wstack_cap = 2, tlas_wneed = 0;
     int blas_build = PT_BLAS_BUILD_REFERENCE;  // where the wide BLAS trees come from
     int tlas_build = PT_TLAS_BUILD_REFERENCE;  // where both TLAS forms come from
+    DevBuf tex;                      // material texture array (512 x 512 rgba8 layers)
+    uint32_t tex_layers = 0;
+    bool has_textures = false;       // some material samples a texture
     DevBuf tri_geo_w;                          // device-built trees: triangle records in their leaf order
     pt::DevScene dev{};
     // frame
@@ -115,7 +118,8 @@ struct pt_ctx {
                          &planes, &meshes, &mesh_root, &lights, &sky, &cdf, &cdf_rows, &tri_geo_w, &accum, &out, &gbuf, &dn_a,
                          &dn_b, &dn_fin, &dn_guide, &stats, &seeds, &rays_in, &rays_aux, &rays_aux2, &rays_out,
                          &wf_o, &wf_d, &wf_t, &wf_r, &wf_meta, &wf_ha, &wf_hb, &wf_media, &wf_q0, &wf_q1, &wf_sq,
-                         &wf_so, &wf_sd, &wf_stp, &wf_sfi, &wf_cnt, &mk_media, &mk_tiles, &mk_spill, &wf_spill};
+                         &wf_so, &wf_sd, &wf_stp, &wf_sfi, &wf_cnt, &mk_media, &mk_tiles, &mk_spill, &wf_spill,
+                         &tex};
         for (DevBuf* b : all) b->release();
     }
 };

@@ -202,7 +202,7 @@ trace_kernel(DevScene S, TraceArgs A) {
                         HitRec h;
                         c.ext++;
                         closest_hit<F, CN, W>(S, fl, ps.ro, ps.rd, st, h, c);
-                        const SurfHit sh = resolve_hit<false>(S, ps.ro, ps.rd, h);
+                        const SurfHit sh = resolve_for_shading<F>(S, fl, ps.ro, ps.rd, h);
                         const bool more = shade_bounce<F, CN>(S, fl, P, bounce, h, sh, ps, md, c, sink);
                         ++bounce;
                         if (more && bounce < S.bounce_limit) continue;
@@ -260,7 +260,7 @@ __global__ void __launch_bounds__(kWave) gbuffer_kernel(DevScene S, pt_scene_par
     for (int bounce = 0; bounce < S.bounce_limit; ++bounce) {
         HitRec h;
         closest_hit<F, false, false>(S, fl, ro, rd, st, h, c);
-        const SurfHit sh = resolve_hit<false>(S, ro, rd, h);
+        const SurfHit sh = resolve_for_shading<F>(S, fl, ro, rd, h);
         if (sh.m_idx == -1) {
             n = splat(0.0f);
             emit = sample_sky(S, rd);
@@ -273,6 +273,19 @@ __global__ void __launch_bounds__(kWave) gbuffer_kernel(DevScene S, pt_scene_par
         emit = ld3(m.emittance);
         rough = m.roughness;
         metal = m.metallic;
+        if (fl.has(kTexturesBit)) {  // get_surface_context's maps (opaque albedo: the alpha < 1 branch never runs)
+            Surface sc;
+            sc.albedo = albedo;
+            sc.emittance = emit;
+            sc.roughness = rough;
+            sc.metallic = metal;
+            sc.alpha = 1.0f;
+            apply_textures(S, m, sh.uv, sc);
+            albedo = sc.albedo;
+            emit = sc.emittance;
+            rough = sc.roughness;
+            metal = sc.metallic;
+        }
         n = dot(sh.n, -rd) > 0.0f ? sh.n : -sh.n;
         dist += h.t;
         // alpha is 1 without albedo textures, so the alpha < 1 branch never runs

@@ -310,7 +310,7 @@ __global__ void __launch_bounds__(kWfBlock) wf_shade(DevScene S, WfArgs A, int c
         h.bv = hb.y;
         const uint32_t meta = A.meta[p];
         Media md{A.media + p, A.pool, int(meta & 7u)};
-        const SurfHit sh = resolve_hit<false>(S, ps.ro, ps.rd, h);
+        const SurfHit sh = resolve_for_shading<F>(S, fl, ps.ro, ps.rd, h);
         QueuedNee sink{A, p, false};
         alive = shade_bounce<F, CN>(S, fl, P, bounce, h, sh, ps, md, c, sink);
         alive = alive && bounce + 1 < S.bounce_limit;

@@ -373,6 +373,8 @@ struct ptgen_scene {
     std::vector<pt_plane> planes;
     std::vector<pt_light> lights;
     std::vector<float> sky_cdf;
+    std::vector<uint8_t> textures;   // 512 x 512 rgba8 layers
+    uint32_t texture_layers = 0;
     double total_lum = 0;
     float total_light_power = 0;
     uint32_t flags = 0;
@@ -548,6 +550,33 @@ int ptgen_add_material(ptgen_scene* s, const ptgen_material* m) {
     s->material_src.push_back(*m);
     s->materials.push_back(pack_material(*m));
     return int(s->materials.size() - 1);
+}
+
+int ptgen_add_texture(ptgen_scene* s, const uint8_t* rgba) {
+    if (!s || !rgba) return PT_ERR_ARG;
+    if (s->texture_layers >= 4096) return s->fail("at most 4096 texture layers");
+    s->textures.insert(s->textures.end(), rgba, rgba + size_t(512) * 512 * 4);
+    return int(s->texture_layers++);
+}
+
+int ptgen_set_material_maps(ptgen_scene* s, int material, const int32_t maps[6], const float uv_scale[2]) {
+    if (!s || !maps) return PT_ERR_ARG;
+    if (material < 0 || size_t(material) >= s->materials.size()) return s->fail("bad material index");
+    for (int k = 0; k < 6; ++k)
+        if (maps[k] >= int32_t(s->texture_layers)) return s->fail("texture index beyond the added layers");
+    pt_material& m = s->materials[size_t(material)];
+    m.emissive_idx = maps[0] < 0 ? -1 : maps[0];
+    m.albedo_idx = maps[1] < 0 ? -1 : maps[1];
+    m.normal_idx = maps[2] < 0 ? -1 : maps[2];
+    m.height_idx = maps[3] < 0 ? -1 : maps[3];
+    m.roughness_idx = maps[4] < 0 ? -1 : maps[4];
+    m.metallic_idx = maps[5] < 0 ? -1 : maps[5];
+    if (uv_scale) {
+        m.uv_scale[0] = uv_scale[0];
+        m.uv_scale[1] = uv_scale[1];
+    }
+    s->finalized = false;
+    return PT_OK;
 }
 
 static int add_prim(ptgen_scene* s, Prim p) {
@@ -964,6 +993,8 @@ int ptgen_finalize(ptgen_scene* s) {
     if (total_power > 0 && !s->lights.empty() && s->lights[0].obj_idx >= 0) f |= PT_HAS_LIGHTS;
     f |= PT_HAS_SKYBOX;
     if (!s->sky_plain) f |= PT_MIS_SKYBOX;
+    for (const pt_material& m : s->materials)  // packSceneData, tracer.js:1825
+        if (m.height_idx >= 0) f |= PT_HAS_HEIGHTMAPS;
     s->flags = f;
 
     s->sky_cdf.assign(size_t(s->sky_h + 1) + size_t(s->sky_h) * (s->sky_w + 1), 0.f);
@@ -992,8 +1023,8 @@ int ptgen_get_desc(const ptgen_scene* s, pt_scene_desc* d) {
     d->sky_cdf_count = s->sky_cdf.size();
     d->sky_exposure = s->sky_exposure;
     d->sky_total_lum = f32(s->total_lum);
-    d->textures = nullptr;
-    d->texture_layers = 0;
+    d->textures = s->texture_layers ? s->textures.data() : nullptr;
+    d->texture_layers = s->texture_layers;
     d->flags = s->flags;
     d->bounce_limit = s->bounces;
     return PT_OK;

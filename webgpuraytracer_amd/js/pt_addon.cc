@@ -133,7 +133,8 @@ napi_value create(napi_env env, napi_callback_info info) {
 
 // desc: {materials, objects, tlas, meshes, bvh, triangles, planes, lights,
 //        sky (Uint16Array rgba16f), skyWidth, skyHeight, skyCDF (Float32Array|null),
-//        skyExposure, skyTotalLum, flags, bounceLimit}
+//        skyExposure, skyTotalLum, flags, bounceLimit,
+//        textures (Uint8Array of 512x512 rgba8 layers, optional)}
 bool read_desc(napi_env env, napi_value o, pt_scene_desc* d) {
     std::memset(d, 0, sizeof(*d));
     if (!array_field(env, o, "materials", sizeof(pt_material), &d->materials) ||
@@ -167,6 +168,15 @@ bool read_desc(napi_env env, napi_value o, pt_scene_desc* d) {
     d->sky_total_lum = float(prop_f64(env, o, "skyTotalLum", 0.0));
     d->flags = prop_u32(env, o, "flags", 0);
     d->bounce_limit = int32_t(prop_u32(env, o, "bounceLimit", 8));
+    if (prop(env, o, "textures", &v) && get_bytes(env, v, &p, &n) && n > 0) {
+        const size_t layer = size_t(512) * 512 * 4;
+        if (n % layer) {
+            napi_throw_type_error(env, nullptr, "textures must hold whole 512x512 rgba8 layers");
+            return false;
+        }
+        d->textures = static_cast<const uint8_t*>(p);
+        d->texture_layers = uint32_t(n / layer);
+    }
     return true;
 }
 

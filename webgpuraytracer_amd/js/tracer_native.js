@@ -82,6 +82,8 @@ class NativeRenderer {
       sky: sky.data, skyWidth: sky.width, skyHeight: sky.height,
       skyExposure: sky.exposure === undefined ? 1 : sky.exposure,
       skyCDF: cdf ? cdf.skyCDF : null, skyTotalLum: cdf ? cdf.totalWeight : 0,
+      // the texture array setScene builds (tracer.js:1919-1953): 512x512 rgba8 layers
+      textures: sceneData.textures ? bytesOf(sceneData.textures) : null,
     };
     return d;
   }

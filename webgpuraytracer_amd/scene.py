@@ -119,6 +119,20 @@ class SceneBuilder:
         self._keep.append(a)
         return self._check(self.lib.ptgen_set_sky(self.h, w, h, a.ctypes.data, exposure))
 
+    def texture(self, rgba):
+        """Adds a 512x512 rgba8 layer to the material texture array; returns its index."""
+        a = np.ascontiguousarray(rgba, np.uint8)
+        if a.shape != (512, 512, 4):
+            raise ValueError("a texture layer is 512 x 512 x 4 bytes")
+        return self._check(self.lib.ptgen_add_texture(self.h, a.ctypes.data))
+
+    def material_maps(self, material, albedo=-1, roughness=-1, metallic=-1, emissive=-1, normal=-1, height=-1,
+                      uv_scale=None):
+        """Texture indices of a material (Material.albedoTex etc., tracer.js:234-246)."""
+        maps = (C.c_int32 * 6)(emissive, albedo, normal, height, roughness, metallic)
+        uv = A.fvec(*uv_scale) if uv_scale is not None else None
+        return self._check(self.lib.ptgen_set_material_maps(self.h, material, maps, uv))
+
     def sky_from_hdr(self, source, exposure=1.0):
         """The sky from a Radiance .hdr file (a path or its bytes), as
         HDRTexture loads one (tracer.js:53-87)."""
