@@ -117,7 +117,12 @@ hipError_t crop_radiance(const float4* accum, uint32_t fw, uint32_t fh, uint32_t
 // L1 loss on log(1 + x): pred [pixel][3] fp32 is in log space already, target
 // [pixel][3] is linear radiance.  Writes d loss / d pred as bf16 [pixel][16]
 // (channels 3..15 zero) and adds the batch loss sum into *loss (fp32).
-hipError_t l1_log_loss(const float* pred, const float* target, Dims d, bf16* dpred, float* loss, hipStream_t s);
+// L1 loss of pred against log1p(max(target, 0)): its gradient sign into
+// dpred, per-block partial sums into part (loss_partials(pixels) floats),
+// and their sum added to *loss.
+uint32_t loss_partials(uint32_t pixels);
+hipError_t l1_log_loss(const float* pred, const float* target, Dims d, bf16* dpred, float* part, float* loss,
+                       hipStream_t s);
 
 // Denoised frame: exp(pred) - 1 (clamped at 0) into fp32 float4 and rgba8
 // (tonemapped with `exposure`), cropping the padded frame to fw x fh.

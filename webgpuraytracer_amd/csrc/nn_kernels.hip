@@ -1171,7 +1171,7 @@ hipError_t crop_radiance(const float4* accum, uint32_t fw, uint32_t fh, uint32_t
 
 __global__ void __launch_bounds__(kBlock) l1_log_loss_k(const float* __restrict__ pred,
                                                         const float* __restrict__ target, uint32_t n, float scale,
-                                                        bf16* __restrict__ dpred, float* __restrict__ loss) {
+                                                        bf16* __restrict__ dpred, float* __restrict__ part) {
     __shared__ float red[kBlock];
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     float l = 0.0f;
@@ -1193,15 +1193,35 @@ __global__ void __launch_bounds__(kBlock) l1_log_loss_k(const float* __restrict_
         if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-hipError_t l1_log_loss(const float* pred, const float* target, Dims d, bf16* dpred, float* loss, hipStream_t s) {
+// loss += the block partials, summed in a fixed order (the same bits on
+// every run, which float atomics would not give)
+__global__ void __launch_bounds__(kBlock) loss_sum_k(const float* __restrict__ part, uint32_t nb,
+                                                     float* __restrict__ loss) {
+    __shared__ float red[kBlock];
+    float l = 0.0f;
+    for (uint32_t i = threadIdx.x; i < nb; i += kBlock) l += part[i];
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (uint32_t off = kBlock / 2u; off > 0u; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *loss += red[0];
+}
+
+uint32_t loss_partials(uint32_t pixels) { return (pixels + kBlock - 1u) / kBlock; }
+
+hipError_t l1_log_loss(const float* pred, const float* target, Dims d, bf16* dpred, float* part, float* loss,
+                       hipStream_t s) {
     const uint32_t n = d.pixels();
     if (n == 0) return hipSuccess;
     const float scale = 1.0f / (float(n) * 3.0f);
-    hipLaunchKernelGGL(l1_log_loss_k, dim3((n + kBlock - 1u) / kBlock), dim3(kBlock), 0, s, pred, target, n, scale,
-                       dpred, loss);
+    const uint32_t nb = loss_partials(n);
+    hipLaunchKernelGGL(l1_log_loss_k, dim3(nb), dim3(kBlock), 0, s, pred, target, n, scale, dpred, part);
+    hipLaunchKernelGGL(loss_sum_k, dim3(1), dim3(kBlock), 0, s, part, nb, loss);
     return hipGetLastError();
 }
 

@@ -74,7 +74,7 @@ struct UNet {
     bool have_acts = false;
     DevBuf in0, tA[4], cat[4], pin[5], t4A, b4, dA[4], dd[4], pred;
     DevBuf g_tA[4], g_cat[4], g_pin[5], g_t4A, g_b4, g_dA[4], g_dd[4], g_pred, dz, dzc, xs;
-    DevBuf target, feat, loss, wpart, bpart;
+    DevBuf target, feat, loss, lpart, wpart, bpart;
     // dataset
     DevBuf ds_in, ds_tg;
     uint32_t ds_n = 0, ds_h = 0, ds_w = 0;
@@ -243,7 +243,8 @@ int alloc_acts(pt_ctx* c, UNet& u, Dims d0) {
     }
     if ((rc = A(u.dz, dz, 2, "unet dz")) || (rc = A(u.dzc, dz, 2, "unet dz")) || (rc = A(u.xs, xs, 2, "unet xs")) ||
         (rc = A(u.wpart, wp, 4, "unet wgrad scratch")) || (rc = A(u.bpart, bp, 4, "unet bias scratch")) ||
-        (rc = A(u.loss, 1024, 4, "unet loss")))
+        (rc = A(u.loss, 1024, 4, "unet loss")) ||
+        (rc = A(u.lpart, loss_partials(uint32_t(P(0))), 4, "unet loss partials")))
         return rc;
     u.have_acts = true;
     return PT_OK;
@@ -370,7 +371,8 @@ int backward(pt_ctx* c, UNet& u) {
 
 // Loss of the batch in u.pred against u.target; adds its sum into loss[slot].
 int loss_and_grad(pt_ctx* c, UNet& u, uint32_t slot) {
-    return hip_rc(c, l1_log_loss(fp(u.pred), fp(u.target), u.dims[0], bfp(u.g_pred), fp(u.loss) + slot, c->stream),
+    return hip_rc(c, l1_log_loss(fp(u.pred), fp(u.target), u.dims[0], bfp(u.g_pred), fp(u.lpart), fp(u.loss) + slot,
+                                   c->stream),
                   "unet loss");
 }
 
