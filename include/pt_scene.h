@@ -101,6 +101,10 @@ int ptgen_set_sky_procedural(ptgen_scene* s, uint32_t width, uint32_t height,
  * packSceneData does (tracer.js:1825). */
 int ptgen_add_texture(ptgen_scene* s, const uint8_t* rgba);
 int ptgen_set_material_maps(ptgen_scene* s, int material, const int32_t maps[6], const float uv_scale[2]);
+/* Material.height_params (tracer.js:247-250, 295): normalMultiplier,
+ * heightMultiplier (> 0 depth map, < 0 height map), heightSamp (parallax
+ * layers, > 0) and heightOffset; defaults 1, 0.05, 32, 0. */
+int ptgen_set_height_params(ptgen_scene* s, int material, const float params[4]);
 
 /* Asset ingest.  ptgen_parse_hdr reads a Radiance RGBE (.hdr) file as
  * lib/parse-hdr.js does (parse-hdr.js:20-211, as HDRTexture._load uses it,

@@ -145,12 +145,14 @@ struct SurfaceHit {
     int count;
     int prim_id;   // not in the WGSL record: which triangle won
     int plane_idx; // not in the WGSL record: which plane won
+    f3 tangent, bitangent;  // set by sphere, cube and plane hits only
 };
 
 SurfaceHit empty_hit() {
     SurfaceHit h;
     h.t = 1e10f; h.m_idx = -1; h.hit_n = V(0.f); h.hit_uv = f2{0.f, 0.f}; h.o_idx = -1; h.count = 0;
     h.prim_id = -1; h.plane_idx = -1;
+    h.tangent = V(0.f); h.bitangent = V(0.f);
     return h;
 }
 
@@ -487,7 +489,11 @@ void trace_sphere(const Ray& rw, const pt_object& o, SurfaceHit* hit, int idx) {
         float phi = std::atan2(ln.z, ln.x);
         float theta = std::asin(clampf(ln.y, -1.0f, 1.0f));
         hit->hit_uv = f2{0.5f + phi / TWO_PI, 0.5f + theta / PI};
+        f3 lt = std::fabs(ln.y) > 0.9999f ? V(1.0f, 0.0f, 0.0f) : V(-ln.z, 0.0f, ln.x);
+        lt = normalize(lt);
         hit->hit_n = normalize(mul_normal(o.inv_matrix, ln));
+        hit->tangent = normalize(mul_normal(o.inv_matrix, lt));
+        hit->bitangent = normalize(mul_normal(o.inv_matrix, cross(ln, lt)));
     }
 }
 
@@ -500,21 +506,27 @@ void trace_cube(const Ray& rw, const pt_object& o, SurfaceHit* hit, int idx) {
         f3 lh = lr.origin + lr.direction * t;
         f3 d = V(std::fabs(lh.x), std::fabs(lh.y), std::fabs(lh.z));
         float max_d = max_component(d);
-        f3 ln;
+        f3 ln, lt, lb = V(0.0f, 1.0f, 0.0f);
         if (max_d == d.x) {
             float s = signf(lh.x);
             ln = V(s, 0.0f, 0.0f);
+            lt = V(0.0f, 0.0f, -s);
             hit->hit_uv = f2{(-s * lh.z) * 0.5f + 0.5f, lh.y * 0.5f + 0.5f};
         } else if (max_d == d.y) {
             float s = signf(lh.y);
             ln = V(0.0f, s, 0.0f);
+            lt = V(1.0f, 0.0f, 0.0f);
+            lb = V(0.0f, 0.0f, s);
             hit->hit_uv = f2{lh.x * 0.5f + 0.5f, (s * lh.z) * 0.5f + 0.5f};
         } else {
             float s = signf(lh.z);
             ln = V(0.0f, 0.0f, s);
+            lt = V(s, 0.0f, 0.0f);
             hit->hit_uv = f2{(s * lh.x) * 0.5f + 0.5f, lh.y * 0.5f + 0.5f};
         }
         hit->hit_n = normalize(mul_normal(o.inv_matrix, ln));
+        hit->tangent = normalize(mul_normal(o.inv_matrix, lt));
+        hit->bitangent = normalize(mul_normal(o.inv_matrix, lb));
     }
 }
 
@@ -532,6 +544,8 @@ void trace_plane(const Ray& rw, const pt_plane& p, SurfaceHit* hit, int plane_id
         f3 tg = normalize(cross(n, tangent));
         f3 bt = normalize(cross(n, tg));
         hit->hit_uv = f2{dot(hp, tg), dot(hp, bt)};
+        hit->tangent = tg;
+        hit->bitangent = bt;
         hit->prim_id = -1;
         hit->plane_idx = plane_idx;
     }
@@ -929,10 +943,11 @@ f4 sample_texture(const ref_scene& S, int idx, f2 uv) {
     return f4{o[0], o[1], o[2], o[3]};
 }
 
-// shader.wgsl:1393-1440 with albedo (its alpha too), roughness, metallic
-// and emissive maps at uv = hit_uv * uv_scale; normal maps are not supported
-// (rejected at upload).
-SurfaceContext get_surface_context(const ref_scene& S, const SurfaceHit& hit, const pt_material& mat) {
+// shader.wgsl:1393-1440 at texture coordinates uv (hit_uv * uv_scale, moved
+// by parallax mapping): the normal map in the frame (tangent, bitangent,
+// hit_n) with x and y scaled by height_params.x, albedo (its alpha too),
+// roughness, metallic and emissive maps.
+SurfaceContext get_surface_context(const ref_scene& S, const SurfaceHit& hit, const pt_material& mat, f2 uv) {
     SurfaceContext ctx;
     ctx.normal = hit.hit_n;
     ctx.surface_normal = hit.hit_n;
@@ -942,7 +957,12 @@ SurfaceContext get_surface_context(const ref_scene& S, const SurfaceHit& hit, co
     ctx.roughness = mat.roughness;
     ctx.emittance = ld3(mat.emittance);
     if (S.tex_layers == 0) return ctx;
-    const f2 uv{hit.hit_uv.x * mat.uv_scale[0], hit.hit_uv.y * mat.uv_scale[1]};
+    if (mat.normal_idx >= 0) {
+        const f4 t = sample_texture(S, mat.normal_idx, uv);
+        const float k = mat.height_params[0];
+        f3 nm = normalize(V((t.x * 2.0f - 1.0f) * k, (t.y * 2.0f - 1.0f) * k, t.z * 2.0f - 1.0f));
+        ctx.normal = normalize(hit.tangent * nm.x + hit.bitangent * nm.y + hit.hit_n * nm.z);
+    }
     if (mat.albedo_idx >= 0) {
         const f4 t = sample_texture(S, mat.albedo_idx, uv);
         ctx.alpha = t.w;
@@ -958,6 +978,80 @@ SurfaceContext get_surface_context(const ref_scene& S, const SurfaceHit& hit, co
     }
     return ctx;
 }
+
+f2 material_uv(const SurfaceHit& hit, const pt_material& mat) {
+    return f2{hit.hit_uv.x * mat.uv_scale[0], hit.hit_uv.y * mat.uv_scale[1]};
+}
+
+struct PomResult { f2 uv; float height; bool hit; };
+
+// Depth under the surface the height map gives at uv (height_params.y < 0:
+// a height map, white high; else a depth map, white deep).
+float pom_depth(const ref_scene& S, const pt_material& mat, f2 uv) {
+    float s = sample_texture(S, mat.height_idx, uv).x;
+    return mat.height_params[1] < 0.0f ? 1.0f - s : s;
+}
+
+// shader.wgsl:1238-1278 with start depth 0.  The early return leaves the
+// zero-initialised result: uv (0, 0), height 0.
+PomResult calculate_pom(const ref_scene& S, f2 uv0, f3 view_ts, const pt_material& mat) {
+    PomResult res{f2{0.0f, 0.0f}, 0.0f, false};
+    float scale = std::fabs(mat.height_params[1]);
+    float num_layers = mat.height_params[2];
+    float layer_depth = 1.0f / num_layers;
+    float cur_depth = 0.0f;
+    f2 P{-(view_ts.x / view_ts.z) * scale, -(view_ts.y / view_ts.z) * scale};
+    f2 delta{P.x / num_layers, P.y / num_layers};
+    f2 cur{uv0.x - P.x * mat.height_params[3], uv0.y - P.y * mat.height_params[3]};
+    float map_depth = pom_depth(S, mat, cur);
+    for (int i = 0; i < 64; ++i) {
+        if (float(i) >= num_layers || cur_depth >= map_depth) break;
+        if (cur_depth > 1.0f || cur_depth < 0.0f) return res;
+        cur = f2{cur.x + delta.x, cur.y + delta.y};
+        map_depth = pom_depth(S, mat, cur);
+        cur_depth += layer_depth;
+    }
+    f2 prev{cur.x - delta.x, cur.y - delta.y};
+    float prev_depth = pom_depth(S, mat, prev);
+    float after = map_depth - cur_depth;
+    float before = prev_depth - cur_depth + layer_depth;
+    float w = after / (after - before);
+    res.uv = f2{cur.x * (1.0f - w) + prev.x * w, cur.y * (1.0f - w) + prev.y * w};
+    res.height = cur_depth * (1.0f - w) + (cur_depth - layer_depth) * w;
+    res.hit = true;
+    return res;
+}
+
+// shader.wgsl:1280-1329
+PomResult calculate_shadow_pom(const ref_scene& S, f2 uv, float height, f3 light_ts, const pt_material& mat) {
+    PomResult res{uv, height, false};
+    if (light_ts.z <= 0.0f) {
+        res.hit = true;
+        return res;
+    }
+    float scale = std::fabs(mat.height_params[1]);
+    float num_layers = mat.height_params[2];
+    float layer_depth = 1.0f / num_layers;
+    f2 p{(light_ts.x / light_ts.z) * scale * layer_depth, (light_ts.y / light_ts.z) * scale * layer_depth};
+    f2 su = uv;
+    float sd = height;
+    for (int i = 0; i < 32; ++i) {
+        if (float(i) / 2.0f >= num_layers || sd <= 0.0f) break;
+        su = f2{su.x + p.x, su.y + p.y};
+        sd -= layer_depth;
+        float md = pom_depth(S, mat, su);
+        if (md < sd) {
+            res.hit = true;
+            res.uv = su;
+            res.height = md;
+            return res;
+        }
+    }
+    return res;
+}
+
+// transpose(tbn) * v
+f3 tangent_space(const SurfaceHit& hit, f3 v) { return V(dot(hit.tangent, v), dot(hit.bitangent, v), dot(hit.hit_n, v)); }
 
 // shader.wgsl:1463-1468
 void build_tangent_space(f3 N, f3* T, f3* B) {
@@ -1444,9 +1538,10 @@ f3 clamp_firefly(f3 rad, int bounce) {
     return rad;
 }
 
-// shader.wgsl:2226-2271 (no heightmaps)
+// shader.wgsl:2226-2271
 f3 evaluate_nee(const ref_scene& S, const LightSample& ls, f3 hit_pos, f3 n_orient, f3 Vv, const pt_material& mat,
-                const SurfaceContext& ctx, const MediumStack& stack, int ignore_obj_idx, Counters* c) {
+                const SurfaceContext& ctx, const SurfaceHit& hit, float cur_height, f2 final_uv,
+                const MediumStack& stack, int ignore_obj_idx, Counters* c) {
     if (ls.pdf <= 0.0f) return V(0.0f);
     float dotNL = dot(n_orient, ls.dir);
     if (dotNL <= 0.0f && mat.transmission <= 0.0f) return V(0.0f);
@@ -1456,6 +1551,10 @@ f3 evaluate_nee(const ref_scene& S, const LightSample& ls, f3 hit_pos, f3 n_orie
     f3 sample_color = ls.color;
     f3 c_weight = weight * bsdf_val / fmx(ls.pdf, 1e-6f);
     if (max_component(sample_color * c_weight) < 1e-5f) return V(0.0f);
+    if (S.has(PT_HAS_HEIGHTMAPS) && mat.height_idx >= 0) {  // shader.wgsl:2254-2260
+        f3 light_ts = normalize(tangent_space(hit, ls.dir));
+        if (calculate_shadow_pom(S, final_uv, cur_height, light_ts, mat).hit) return V(0.0f);
+    }
     f3 ray_offset = dotNL < 0.0f ? -n_orient * 0.001f : n_orient * 0.001f;
     Ray shadow_ray{hit_pos + ray_offset, ls.dir};
     f3 shadow_hit = trace_scene_shadow(S, shadow_ray, ignore_obj_idx, ls.dist, c);
@@ -1528,7 +1627,17 @@ f3 trace_path(const ref_scene& S, const pt_scene_params& P, uint32_t x, uint32_t
         }
         c->shade++;
         const pt_material& mat = S.mats[hit.m_idx];
-        SurfaceContext ctx = get_surface_context(S, hit, mat);
+        f2 final_uv = material_uv(hit, mat);
+        float cur_height = 0.0f;
+        if (S.has(PT_HAS_HEIGHTMAPS) && mat.height_idx >= 0) {  // shader.wgsl:2365-2373
+            f3 view_ts = normalize(tangent_space(hit, -ray.direction));
+            if (view_ts.z > 0.0f) {
+                PomResult pom = calculate_pom(S, final_uv, view_ts, mat);
+                final_uv = pom.uv;
+                cur_height = pom.height;
+            }
+        }
+        SurfaceContext ctx = get_surface_context(S, hit, mat, final_uv);
         if (length(ctx.emittance) > 0.0f) {
             float weight = 1.0f;
             if (S.has(PT_HAS_LIGHTS) && bounce > 0 && hit.o_idx >= 0 && obj.light_idx >= 0) {
@@ -1567,7 +1676,8 @@ f3 trace_path(const ref_scene& S, const pt_scene_params& P, uint32_t x, uint32_t
                 if (ls.pdf > 0.0f) {
                     float selection_pdf = selected_weight / weight_sum;
                     ls.pdf *= selection_pdf;
-                    f3 inc = evaluate_nee(S, ls, hit_pos, n_orient, Vv, mat, ctx, stack, light.obj_idx, c);
+                    f3 inc = evaluate_nee(S, ls, hit_pos, n_orient, Vv, mat, ctx, hit, cur_height, final_uv, stack,
+                                          light.obj_idx, c);
                     radiance = radiance + throughput * clamp_firefly(inc, bounce);
                 }
             }
@@ -1577,10 +1687,21 @@ f3 trace_path(const ref_scene& S, const pt_scene_params& P, uint32_t x, uint32_t
             float ux = rng.next();
             float uy = rng.next();
             LightSample env = sample_env_cdf(S, P, f2{ux, uy});
-            f3 inc = evaluate_nee(S, env, hit_pos, n_orient, Vv, mat, ctx, stack, -1, c);
+            f3 inc = evaluate_nee(S, env, hit_pos, n_orient, Vv, mat, ctx, hit, cur_height, final_uv, stack, -1, c);
             radiance = radiance + throughput * clamp_firefly(inc, bounce);
         }
         if (!sample_surface(&ray, &throughput, mat, ctx, hit_pos, &stack, &last_surface_pdf, rng)) break;
+        if (S.has(PT_HAS_HEIGHTMAPS) && mat.height_idx >= 0) {  // shader.wgsl:2458-2472
+            f3 light_ts = normalize(tangent_space(hit, ray.direction));
+            PomResult sr = calculate_shadow_pom(S, final_uv, cur_height, light_ts, mat);
+            if (sr.hit) {
+                final_uv = sr.uv;
+                SurfaceContext ctx_pom = get_surface_context(S, hit, mat, final_uv);
+                radiance = radiance + throughput * clamp_firefly(ctx_pom.emittance, bounce);
+                if (length(ctx_pom.emittance) > 1.0f) break;
+                if (!sample_surface(&ray, &throughput, mat, ctx_pom, hit_pos, &stack, &last_surface_pdf, rng)) break;
+            }
+        }
         if (bounce < 2) continue;
         float p = max_component(throughput);
         float survival_prob = clampf(p, 0.05f, 0.95f);
@@ -1769,7 +1890,7 @@ int ref_gbuffer(const ref_scene* s, const pt_scene_params* params, pt_gbuffer_te
                     break;
                 }
                 mat = S.mats[hit.m_idx];
-                ctx = get_surface_context(S, hit, mat);
+                ctx = get_surface_context(S, hit, mat, material_uv(hit, mat));
                 ctx.normal = dot(ctx.normal, -ray.direction) > 0.0f ? ctx.normal : -ctx.normal;
                 dist += hit.t;
                 if (ctx.alpha < 1.0f) {
@@ -1923,7 +2044,7 @@ static SurfaceContext bsdf_ctx(const pt_material& m, const float n[3]) {
     SurfaceHit h = empty_hit();
     h.hit_n = V(n[0], n[1], n[2]);
     static const ref_scene kNoTextures;
-    return get_surface_context(kNoTextures, h, m);
+    return get_surface_context(kNoTextures, h, m, material_uv(h, m));
 }
 
 int ref_bsdf_sample(const pt_material* m, const float n[3], const float v[3], uint32_t n_samples, uint32_t seed,

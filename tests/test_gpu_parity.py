@@ -731,8 +731,120 @@ def test_textured_scene_matches_oracle(renderer):
             m.albedo_idx, m.roughness_idx, m.metallic_idx, m.emissive_idx = s
 
 
-@pytest.mark.parametrize("bad", ["normal", "height", "translucent", "index"])
+def _bump_layers(seed):
+    """A smooth height field and its tangent-space normal map (rgb = n * 0.5 + 0.5)."""
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:512, 0:512].astype(np.float64) * (2 * np.pi / 512)
+    hgt = np.zeros((512, 512))
+    for _ in range(4):
+        fx, fy = rng.integers(1, 6, 2)
+        ph = rng.uniform(0, 2 * np.pi)
+        hgt += np.sin(fx * xx + fy * yy + ph) / 4
+    hgt = (hgt - hgt.min()) / (hgt.max() - hgt.min())
+    gy, gx = np.gradient(hgt * 40.0)
+    n = np.stack([-gx, -gy, np.ones_like(gx)], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    nt = np.empty((512, 512, 4), np.uint8)
+    nt[..., :3] = np.round((n * 0.5 + 0.5) * 255)
+    nt[..., 3] = 255
+    ht = np.empty((512, 512, 4), np.uint8)
+    ht[..., :3] = np.round(hgt * 255)[..., None]
+    ht[..., 3] = 255
+    return nt, ht
+
+
+def _pom_scene():
+    """Normal and height maps (parallax occlusion, its shadow march in NEE
+    and after each BSDF sample) on planes, spheres and cubes, beside plain
+    cylinders and tori, a light box and an importance-sampled sky."""
+    b = SceneBuilder()
+    b.procedural_sky(128, 64, seed=9)
+    nt, ht = _bump_layers(4)
+    nrm, hgt, checker = b.texture(nt), b.texture(ht), b.texture(_tex_layer("checker", 1))
+    m_plane = b.material(color=(0.9, 0.9, 0.9), roughness=0.6)
+    b.material_maps(m_plane, albedo=checker, normal=nrm, height=hgt, uv_scale=(0.5, 0.5))
+    b.height_params(m_plane, normal_multiplier=1.0, height_multiplier=0.08, layers=24.0)
+    m_n = b.material(color=(0.8, 0.6, 0.4), roughness=0.25, metallic=0.5)
+    b.material_maps(m_n, normal=nrm)
+    b.height_params(m_n, normal_multiplier=2.0)
+    m_h = b.material(color=(0.5, 0.7, 0.9), roughness=0.5)
+    b.material_maps(m_h, height=hgt, albedo=checker, uv_scale=(2.0, 1.0))
+    b.height_params(m_h, height_multiplier=-0.06, layers=16.0, offset=0.3)
+    m_nh = b.material(color=(0.9, 0.9, 0.9), roughness=0.15, transmission=0.7, ior=1.4)
+    b.material_maps(m_nh, normal=nrm, height=hgt)
+    b.height_params(m_nh, normal_multiplier=0.7, height_multiplier=0.05, layers=8.0)
+    m_light = b.material(color=(1.0, 1.0, 1.0), emittance=(1.0, 0.9, 0.8), emission_intensity=12.0)
+    m_plain = b.material(color=(0.5, 0.6, 0.3), roughness=0.4)
+    b.plane(m_plane, (0.0, 1.0, 0.0), -0.5)
+    b.sphere(m_n, (-1.2, 0.2, -0.4), 0.4)
+    b.sphere(m_h, (0.0, 0.2, -0.6), 0.45)
+    b.sphere(m_nh, (1.1, 0.15, -0.3), 0.35)
+    b.cube(m_h, (1.0, 0.0, 0.9), (0.5, 0.5, 0.5))
+    b.cube(m_n, (-1.0, 0.0, 0.9), (0.4, 0.6, 0.4))
+    b.cube(m_light, (0.0, 2.2, 0.0), (0.8, 0.01, 0.8), enable_nee=True)
+    b.cylinder(m_plain, (-0.2, -0.5, 1.1), 0.25, 0.6)
+    b.torus(m_plain, (0.4, 0.0, 1.4), 0.3, 0.1)
+    b.camera()
+    b.bounces(5)
+    return b.finalize()
+
+
+def test_pom_scene_matches_oracle(renderer):
+    """Normal maps and parallax occlusion mapping: the 1-spp render and the
+    G-buffer agree with the oracle, both backends give the same bits, and
+    each kind of map changes the image."""
+    scene = _pom_scene()
+    d = scene.desc()
+    assert d.flags & A.HAS_HEIGHTMAPS
+    w, h = 96, 64
+    ref = _setup(renderer, scene, w, h, gbuf=True)
+    p = scene.params(w, h, sample_number=0, seed=4242)
+    st = renderer.render_section(p, passes=1)
+    ga = renderer.read_accum()
+    ra, _, rst = ref.render(p, passes=1)
+    rel = np.abs(ga[..., :3] - ra[..., :3]) / np.maximum(np.abs(ra[..., :3]), 1e-3)
+    frac = float(np.mean(np.all(rel <= 1e-4, axis=-1)))
+    print("pom 1spp pixel agreement", frac)
+    assert frac >= 0.97
+    assert abs(int(st.extension_rays) - int(rst.extension_rays)) <= 0.01 * rst.extension_rays
+    assert abs(int(st.shadow_rays) - int(rst.shadow_rays)) <= 0.01 * rst.shadow_rays
+    renderer.gbuffer(scene.params(w, h))
+    gg = renderer.read_gbuffer()
+    rg, _ = ref.gbuffer(scene.params(w, h))
+    gerr = np.abs(gg - rg) / np.maximum(np.abs(rg), 1e-3)
+    print("pom gbuffer max rel err", float(gerr.max()))
+    assert float(np.mean(np.all(gerr <= 1e-4, axis=-1))) >= 0.99
+    renderer.reset()
+    accs = []
+    for be in (A.PT_BACKEND_WAVEFRONT, A.PT_BACKEND_MEGAKERNEL):
+        renderer.set_option(A.PT_OPT_BACKEND, be)
+        renderer.resize(w, h)
+        renderer.render_section(p, passes=2, seeds=[21, 22])
+        accs.append(renderer.read_accum())
+    renderer.set_option(A.PT_OPT_BACKEND, A.PT_BACKEND_WAVEFRONT)
+    assert np.array_equal(accs[0], accs[1])
+    # each map matters: the image without it differs (the oracle agrees on that image too)
+    mats = (A.Material * d.materials.count).from_address(d.materials.ptr)
+    for field in ("normal_idx", "height_idx"):
+        saved = [getattr(m, field) for m in mats]
+        try:
+            for m in mats:
+                setattr(m, field, -1)
+            renderer.set_scene(d, isgbuf=True)
+            renderer.resize(w, h)
+            renderer.render_section(p, passes=1)
+            gb = renderer.read_accum()
+            assert np.mean(np.abs(gb - ga)) > 0.01 * np.mean(np.abs(ga)), field
+        finally:
+            for m, v in zip(mats, saved):
+                setattr(m, field, v)
+
+
+@pytest.mark.parametrize("bad", ["normal", "height", "translucent", "index", "layers"])
 def test_texture_validation(renderer, bad):
+    """Rejected: normal and height maps on hits without the reference's own
+    tangent frame (cylinders, tori, meshes), translucent albedo, texture
+    indices beyond the layers and height maps with no parallax layers."""
     b = SceneBuilder()
     layer = _tex_layer("noise", 7)
     if bad == "translucent":
@@ -741,8 +853,11 @@ def test_texture_validation(renderer, bad):
     m = b.material(color=(0.5, 0.5, 0.5))
     if bad == "normal":
         b.material_maps(m, normal=t)
-    elif bad == "height":
+        b.cylinder(m, (0.0, 0.0, 0.0), 0.3, 0.8)
+    elif bad in ("height", "layers"):
         b.material_maps(m, height=t)
+        if bad == "height":
+            b.torus(m, (0.0, 0.0, 0.0), 0.4, 0.1)
     else:
         b.material_maps(m, albedo=t)
     b.sphere(m, (0.0, 0.0, 0.0), 0.5)
@@ -750,10 +865,12 @@ def test_texture_validation(renderer, bad):
     b.camera()
     scene = b.finalize()
     d = scene.desc()
+    mats = (A.Material * d.materials.count).from_address(d.materials.ptr)
     if bad == "index":
-        mats = (A.Material * d.materials.count).from_address(d.materials.ptr)
         mats[m].albedo_idx = 5
+    if bad == "layers":
+        mats[m].height_params[2] = 0.0
     with pytest.raises(A.PTError) as e:
         renderer.set_scene(d)
-    want = A.PT_ERR_LAYOUT if bad == "index" else A.PT_ERR_UNSUPPORTED
+    want = {"index": A.PT_ERR_LAYOUT, "layers": A.PT_ERR_ARG}.get(bad, A.PT_ERR_UNSUPPORTED)
     assert e.value.code == want, e.value

@@ -209,6 +209,7 @@ PTGEN_FUNCS = {
     "ptgen_add_model": (C.c_int, [vp, vp, vp, vp, u32, vp, u32]),
     "ptgen_add_texture": (C.c_int, [vp, vp]),
     "ptgen_set_material_maps": (C.c_int, [vp, C.c_int, vp, vp]),
+    "ptgen_set_height_params": (C.c_int, [vp, C.c_int, vp]),
     "ptgen_parse_hdr": (C.c_int, [vp, u64, P(u32), P(u32), P(f32), vp, u64]),
     "ptgen_obj_triangles": (C.c_int, [C.c_char_p, u64, P(u32)]),
     "ptgen_obj_read": (C.c_int, [C.c_char_p, u64, vp, vp, vp, u32]),

@@ -69,15 +69,24 @@ __device__ bool shade_bounce(const DevScene& S, Flags<F> fl, const Params& P, in
     }
     if (CN) c.shade++;
     const pt_material& m = S.mats[sh.m_idx];
-    Surface sc;
-    sc.normal = sh.n;
-    sc.snormal = sh.n;
-    sc.albedo = ld3(m.color);
-    sc.alpha = 1.0f;
-    sc.metallic = m.metallic;
-    sc.roughness = m.roughness;
-    sc.emittance = ld3(m.emittance);
-    if (fl.has(kTexturesBit)) apply_textures(S, m, sh.uv, sc);
+    Surface sc = base_surface(m, sh.n);
+    // texture coordinates, moved by parallax mapping (shader.wgsl:2361-2373)
+    v2 fuv{0.0f, 0.0f};
+    float fdepth = 0.0f;
+    bool pom = false;
+    if (fl.has(kTexturesBit)) {
+        fuv = v2{sh.uv.x * m.uv_scale[0], sh.uv.y * m.uv_scale[1]};
+        pom = fl.has(PT_HAS_HEIGHTMAPS) && m.height_idx >= 0;
+        if (pom) {
+            const v3 vts = normalize(to_tangent(sh.tg, sh.bt, sh.n, -ps.rd));
+            if (vts.z > 0.0f) {
+                const PomRes pr = pom_march(S, m, fuv, vts);
+                fuv = pr.uv;
+                fdepth = pr.h;
+            }
+        }
+        apply_textures(S, m, fuv, sh.tg, sh.bt, sc);
+    }
     // implicit light hit (shader.wgsl:2377-2395)
     if (len(sc.emittance) > 0.0f) {
         float w = 1.0f;
@@ -114,7 +123,9 @@ __device__ bool shade_bounce(const DevScene& S, Flags<F> fl, const Params& P, in
                 const float w = mis_weight(ls.pdf, bp);
                 const v3 f = eval_bsdf(V, ls.dir, m, sc, md);
                 const v3 cw = w * f / mx(ls.pdf, 1e-6f);
-                if (!(hmax(ls.color * cw) < 1e-5f)) {
+                // a light the height map hides gets no shadow ray (shader.wgsl:2254-2260)
+                if (!(hmax(ls.color * cw) < 1e-5f) &&
+                    !(pom && pom_shadow(S, m, fuv, fdepth, normalize(to_tangent(sh.tg, sh.bt, sh.n, ls.dir))).hit)) {
                     o = hp + (nl < 0.0f ? -no * 0.001f : no * 0.001f);
                     inc = ls.color * cw;
                     need = true;
@@ -164,6 +175,18 @@ __device__ bool shade_bounce(const DevScene& S, Flags<F> fl, const Params& P, in
         nee_eval(sample_env(S, P, ux, uy), -1, 1);
     }
     if (!sample_bsdf(ps.ro, ps.rd, ps.tp, m, sc, hp, md, ps.last_pdf, ps.rng)) return false;
+    // a sampled direction the height map blocks: the surface where it is
+    // blocked adds its emission and is sampled again (shader.wgsl:2458-2472)
+    if (pom) {
+        const PomRes sr = pom_shadow(S, m, fuv, fdepth, normalize(to_tangent(sh.tg, sh.bt, sh.n, ps.rd)));
+        if (sr.hit) {
+            Surface s2 = base_surface(m, sh.n);
+            apply_textures(S, m, sr.uv, sh.tg, sh.bt, s2);
+            ps.rad = ps.rad + ps.tp * firefly(s2.emittance, bounce);
+            if (len(s2.emittance) > 1.0f) return false;
+            if (!sample_bsdf(ps.ro, ps.rd, ps.tp, m, s2, hp, md, ps.last_pdf, ps.rng)) return false;
+        }
+    }
     // Russian roulette from the third bounce (shader.wgsl:2474-2479)
     if (bounce < 2) return true;
     const float surv = clampf(hmax(ps.tp), 0.05f, 0.95f);

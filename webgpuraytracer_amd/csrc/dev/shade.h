@@ -270,11 +270,20 @@ __device__ __forceinline__ float4 sample_texture(const DevScene& S, int idx, v2 
     return make_float4(o[0], o[1], o[2], o[3]);
 }
 
-// get_surface_context's maps (shader.wgsl:1393-1440) at uv = hit_uv *
-// uv_scale: albedo (and its alpha) and emittance times texel^2.2, roughness
-// times green, metallic times blue.  Normal maps are rejected at upload.
-__device__ __forceinline__ void apply_textures(const DevScene& S, const pt_material& m, v2 huv, Surface& sc) {
-    const v2 uv{huv.x * m.uv_scale[0], huv.y * m.uv_scale[1]};
+// get_surface_context's maps (shader.wgsl:1393-1440) at the texture
+// coordinates uv (hit_uv * uv_scale, moved by parallax mapping where the
+// material has a height map): albedo (and its alpha) and emittance times
+// texel^2.2, roughness times green, metallic times blue, and the normal map
+// in the hit's tangent frame (T, B, N) with its x and y scaled by
+// height_params.x.  sc.normal must hold N on entry.
+__device__ __forceinline__ void apply_textures(const DevScene& S, const pt_material& m, v2 uv, v3 T, v3 B,
+                                               Surface& sc) {
+    if (m.normal_idx >= 0) {
+        const float4 t = sample_texture(S, m.normal_idx, uv);
+        const float k = m.height_params[0];
+        const v3 nm = normalize(mk((t.x * 2.0f - 1.0f) * k, (t.y * 2.0f - 1.0f) * k, t.z * 2.0f - 1.0f));
+        sc.normal = normalize(T * nm.x + B * nm.y + sc.normal * nm.z);
+    }
     if (m.albedo_idx >= 0) {
         const float4 t = sample_texture(S, m.albedo_idx, uv);
         sc.alpha = t.w;
@@ -287,6 +296,97 @@ __device__ __forceinline__ void apply_textures(const DevScene& S, const pt_mater
         sc.emittance = sc.emittance * mk(powf(t.x, 2.2f), powf(t.y, 2.2f), powf(t.z, 2.2f));
     }
 }
+
+// The material's own surface before any map.
+__device__ __forceinline__ Surface base_surface(const pt_material& m, v3 n) {
+    Surface sc;
+    sc.normal = n;
+    sc.snormal = n;
+    sc.albedo = ld3(m.color);
+    sc.alpha = 1.0f;
+    sc.metallic = m.metallic;
+    sc.roughness = m.roughness;
+    sc.emittance = ld3(m.emittance);
+    return sc;
+}
+
+struct PomRes {
+    v2 uv;
+    float h;
+    bool hit;
+};
+
+// Depth (0 at the surface, 1 at the bottom) the height map gives at uv:
+// height_params.y < 0 marks a height map (white high), else a depth map.
+__device__ __forceinline__ float pom_depth(const DevScene& S, const pt_material& m, v2 uv) {
+    const float s = sample_texture(S, m.height_idx, uv).x;
+    return m.height_params[1] < 0.0f ? 1.0f - s : s;
+}
+
+// calculate_pom (shader.wgsl:1238-1278) from depth 0: march the view ray in
+// height_params.z layers (at most 64) until it falls below the map, then
+// interpolate between the last two layers.  A march that leaves [0, 1]
+// returns uv (0, 0) and depth 0, as the reference's zero-initialised result.
+__device__ PomRes pom_march(const DevScene& S, const pt_material& m, v2 uv0, v3 vts) {
+    PomRes r{v2{0.0f, 0.0f}, 0.0f, false};
+    const float scale = fabsf(m.height_params[1]);
+    const float nl = m.height_params[2];
+    const float ld = 1.0f / nl;
+    float cur = 0.0f;
+    const v2 P{-(vts.x / vts.z) * scale, -(vts.y / vts.z) * scale};
+    const v2 dt{P.x / nl, P.y / nl};
+    v2 tc{uv0.x - P.x * m.height_params[3], uv0.y - P.y * m.height_params[3]};
+    float dv = pom_depth(S, m, tc);
+    for (int i = 0; i < 64; ++i) {
+        if (float(i) >= nl || cur >= dv) break;
+        if (cur > 1.0f || cur < 0.0f) return r;
+        tc = v2{tc.x + dt.x, tc.y + dt.y};
+        dv = pom_depth(S, m, tc);
+        cur += ld;
+    }
+    const v2 pc{tc.x - dt.x, tc.y - dt.y};
+    const float pdv = pom_depth(S, m, pc);
+    const float after = dv - cur;
+    const float before = pdv - cur + ld;
+    const float w = after / (after - before);
+    r.uv = v2{tc.x * (1.0f - w) + pc.x * w, tc.y * (1.0f - w) + pc.y * w};
+    r.h = cur * (1.0f - w) + (cur - ld) * w;
+    r.hit = true;
+    return r;
+}
+
+// calculate_shadow_pom (shader.wgsl:1280-1329): march from (uv, depth)
+// toward the surface along lts (tangent space); hit when the map rises
+// above the ray or when lts points into the surface.
+__device__ PomRes pom_shadow(const DevScene& S, const pt_material& m, v2 uv, float h, v3 lts) {
+    PomRes r{uv, h, false};
+    if (lts.z <= 0.0f) {
+        r.hit = true;
+        return r;
+    }
+    const float scale = fabsf(m.height_params[1]);
+    const float nl = m.height_params[2];
+    const float ld = 1.0f / nl;
+    const v2 p{(lts.x / lts.z) * scale * ld, (lts.y / lts.z) * scale * ld};
+    v2 su = uv;
+    float sd = h;
+    for (int i = 0; i < 32; ++i) {
+        if (float(i) / 2.0f >= nl || sd <= 0.0f) break;
+        su = v2{su.x + p.x, su.y + p.y};
+        sd -= ld;
+        const float md = pom_depth(S, m, su);
+        if (md < sd) {
+            r.hit = true;
+            r.uv = su;
+            r.h = md;
+            return r;
+        }
+    }
+    return r;
+}
+
+// Tangent-space coordinates of a world direction, (T, B, N)^T * v.
+__device__ __forceinline__ v3 to_tangent(v3 T, v3 B, v3 N, v3 v) { return mk(dot(T, v), dot(B, v), dot(N, v)); }
 
 __device__ __forceinline__ float d_ggx(float nh, float a2) {
     const float den = (nh * nh * (a2 - 1.0f) + 1.0f);

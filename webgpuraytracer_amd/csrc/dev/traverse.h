@@ -464,6 +464,7 @@ struct SurfHit {
     int o_idx;
     v3 n;
     v2 uv;
+    v3 tg, bt;  // tangent frame of sphere, cube and plane hits (with uv only)
 };
 
 template <bool WANT_UV>
@@ -474,6 +475,8 @@ __device__ SurfHit resolve_hit(const DevScene& S, v3 o, v3 d, const HitRec& h) {
     r.o_idx = -1;
     r.n = splat(0.0f);
     r.uv = v2{0.0f, 0.0f};
+    r.tg = splat(0.0f);
+    r.bt = splat(0.0f);
     if (h.kind == kMiss) return r;
     if (h.kind == kPlaneHit) {  // trace_plane, shader.wgsl:622-636
         const pt_plane& p = S.planes[h.idx];
@@ -486,6 +489,8 @@ __device__ SurfHit resolve_hit(const DevScene& S, v3 o, v3 d, const HitRec& h) {
             const v3 T = normalize(cross(r.n, tg));
             const v3 B = normalize(cross(r.n, T));
             r.uv = v2{dot(hp, T), dot(hp, B)};
+            r.tg = T;
+            r.bt = B;
         }
         return r;
     }
@@ -513,12 +518,17 @@ __device__ SurfHit resolve_hit(const DevScene& S, v3 o, v3 d, const HitRec& h) {
     const v3 lo = xform_point(inv, o), ld = xform_dir(inv, d);
     const float t = h.t;
     switch (ob.object_type) {
-        case PT_OBJ_SPHERE: {  // shader.wgsl:552-573
+        case PT_OBJ_SPHERE: {  // shader.wgsl:552-572
             const v3 ln = lo + ld * t;
             if (WANT_UV) {
                 const float phi = atan2f(ln.z, ln.x);
                 const float th = asinf(clampf(ln.y, -1.0f, 1.0f));
                 r.uv = v2{0.5f + phi / kTWO_PI, 0.5f + th / kPI};
+                v3 lt = mk(-ln.z, 0.0f, ln.x);
+                if (fabsf(ln.y) > 0.9999f) lt = mk(1.0f, 0.0f, 0.0f);
+                lt = normalize(lt);
+                r.tg = normalize(xform_normal(inv, lt));
+                r.bt = normalize(xform_normal(inv, cross(ln, lt)));
             }
             r.n = normalize(xform_normal(inv, ln));
             break;
@@ -527,21 +537,31 @@ __device__ SurfHit resolve_hit(const DevScene& S, v3 o, v3 d, const HitRec& h) {
             const v3 lh = lo + ld * t;
             const v3 a = mk(fabsf(lh.x), fabsf(lh.y), fabsf(lh.z));
             const float m = hmax(a);
-            v3 ln;
+            v3 ln, lt, lb;
             if (m == a.x) {
                 const float s = sgn(lh.x);
                 ln = mk(s, 0.0f, 0.0f);
+                lt = mk(0.0f, 0.0f, -s);
+                lb = mk(0.0f, 1.0f, 0.0f);
                 r.uv = v2{(-s * lh.z) * 0.5f + 0.5f, lh.y * 0.5f + 0.5f};
             } else if (m == a.y) {
                 const float s = sgn(lh.y);
                 ln = mk(0.0f, s, 0.0f);
+                lt = mk(1.0f, 0.0f, 0.0f);
+                lb = mk(0.0f, 0.0f, s);
                 r.uv = v2{lh.x * 0.5f + 0.5f, (s * lh.z) * 0.5f + 0.5f};
             } else {
                 const float s = sgn(lh.z);
                 ln = mk(0.0f, 0.0f, s);
+                lt = mk(s, 0.0f, 0.0f);
+                lb = mk(0.0f, 1.0f, 0.0f);
                 r.uv = v2{(s * lh.x) * 0.5f + 0.5f, lh.y * 0.5f + 0.5f};
             }
             r.n = normalize(xform_normal(inv, ln));
+            if (WANT_UV) {
+                r.tg = normalize(xform_normal(inv, lt));
+                r.bt = normalize(xform_normal(inv, lb));
+            }
             break;
         }
         case PT_OBJ_CYLINDER: {  // hit_cylinder normals/uvs, shader.wgsl:345-368

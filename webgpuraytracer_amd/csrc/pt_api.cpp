@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -52,13 +53,31 @@ int validate_desc(pt_ctx* c, const pt_scene_desc* d) {
     if (d->texture_layers > 4096) return c->fail(PT_ERR_ARG, "at most 4096 texture layers");
     const pt_material* m = static_cast<const pt_material*>(d->materials.ptr);
     const uint8_t* tex = d->textures;
+    const pt_object* objs = static_cast<const pt_object*>(d->objects.ptr);
+    const pt_mesh_instance* mi = static_cast<const pt_mesh_instance*>(d->meshes.ptr);
     for (uint64_t i = 0; i < d->materials.count; ++i) {
         // normal and height maps (and parallax mapping) need the hit's tangent frame, which the reference
-        // sets only on sphere, cube and plane hits
-        if (m[i].normal_idx >= 0 || m[i].height_idx >= 0)
-            return c->fail(PT_ERR_UNSUPPORTED, "material %llu: normal and height maps are not supported",
-                           (unsigned long long)i);
-        const int32_t idx[4] = {m[i].albedo_idx, m[i].roughness_idx, m[i].metallic_idx, m[i].emissive_idx};
+        // sets only on sphere, cube and plane hits (shader.wgsl:560-572, 598-617, 630-634): other hits
+        // keep the frame of an earlier candidate, so materials with these maps may be used by spheres,
+        // cubes and planes only
+        if (m[i].normal_idx >= 0 || m[i].height_idx >= 0) {
+            for (uint64_t k = 0; k < d->objects.count; ++k)
+                if (objs[k].material_idx == int32_t(i) && objs[k].object_type != PT_OBJ_SPHERE &&
+                    objs[k].object_type != PT_OBJ_CUBE)
+                    return c->fail(PT_ERR_UNSUPPORTED,
+                                   "material %llu: normal and height maps on object %llu, which is not a sphere, "
+                                   "cube or plane", (unsigned long long)i, (unsigned long long)k);
+            if (d->flags & PT_HAS_LIST_MESHES)
+                for (uint64_t k = 0; k < d->meshes.count; ++k)
+                    if (mi[k].material_idx == int32_t(i))
+                        return c->fail(PT_ERR_UNSUPPORTED, "material %llu: normal and height maps on mesh %llu",
+                                       (unsigned long long)i, (unsigned long long)k);
+            if (m[i].height_idx >= 0 && !(m[i].height_params[2] > 0.0f && std::isfinite(m[i].height_params[2])))
+                return c->fail(PT_ERR_ARG, "material %llu: height map with %g layers", (unsigned long long)i,
+                               double(m[i].height_params[2]));
+        }
+        const int32_t idx[6] = {m[i].albedo_idx,   m[i].roughness_idx, m[i].metallic_idx,
+                                m[i].emissive_idx, m[i].normal_idx,    m[i].height_idx};
         for (int32_t t : idx)
             if (t >= 0 && uint32_t(t) >= d->texture_layers)
                 return c->fail(PT_ERR_LAYOUT, "material %llu: texture %d outside the %u layers",
@@ -154,7 +173,8 @@ void refresh_dev(pt_ctx* c) {
 bool uses_textures(const pt_scene_desc* d) {
     const pt_material* m = static_cast<const pt_material*>(d->materials.ptr);
     for (uint64_t i = 0; i < d->materials.count; ++i)
-        if (m[i].albedo_idx >= 0 || m[i].roughness_idx >= 0 || m[i].metallic_idx >= 0 || m[i].emissive_idx >= 0)
+        if (m[i].albedo_idx >= 0 || m[i].roughness_idx >= 0 || m[i].metallic_idx >= 0 || m[i].emissive_idx >= 0 ||
+            m[i].normal_idx >= 0 || m[i].height_idx >= 0)
             return true;
     return false;
 }

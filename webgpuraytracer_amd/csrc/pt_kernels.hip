@@ -273,20 +273,17 @@ __global__ void __launch_bounds__(kWave) gbuffer_kernel(DevScene S, pt_scene_par
         emit = ld3(m.emittance);
         rough = m.roughness;
         metal = m.metallic;
-        if (fl.has(kTexturesBit)) {  // get_surface_context's maps (opaque albedo: the alpha < 1 branch never runs)
-            Surface sc;
-            sc.albedo = albedo;
-            sc.emittance = emit;
-            sc.roughness = rough;
-            sc.metallic = metal;
-            sc.alpha = 1.0f;
-            apply_textures(S, m, sh.uv, sc);
+        v3 sn = sh.n;
+        if (fl.has(kTexturesBit)) {  // get_surface_context's maps, no parallax (opaque albedo: the alpha < 1 branch never runs)
+            Surface sc = base_surface(m, sh.n);
+            apply_textures(S, m, v2{sh.uv.x * m.uv_scale[0], sh.uv.y * m.uv_scale[1]}, sh.tg, sh.bt, sc);
             albedo = sc.albedo;
             emit = sc.emittance;
             rough = sc.roughness;
             metal = sc.metallic;
+            sn = sc.normal;
         }
-        n = dot(sh.n, -rd) > 0.0f ? sh.n : -sh.n;
+        n = dot(sn, -rd) > 0.0f ? sn : -sn;
         dist += h.t;
         // alpha is 1 without albedo textures, so the alpha < 1 branch never runs
         if (m.metallic >= 0.99f && m.roughness <= 0.01f) {

@@ -579,6 +579,16 @@ int ptgen_set_material_maps(ptgen_scene* s, int material, const int32_t maps[6],
     return PT_OK;
 }
 
+int ptgen_set_height_params(ptgen_scene* s, int material, const float params[4]) {
+    if (!s || !params) return PT_ERR_ARG;
+    if (material < 0 || size_t(material) >= s->materials.size()) return s->fail("bad material index");
+    if (!(params[2] > 0.0f && std::isfinite(params[2]))) return s->fail("height map layers must be positive");
+    pt_material& m = s->materials[size_t(material)];
+    for (int k = 0; k < 4; ++k) m.height_params[k] = params[k];
+    s->finalized = false;
+    return PT_OK;
+}
+
 static int add_prim(ptgen_scene* s, Prim p) {
     s->finalized = false;
     p.update_matrix();

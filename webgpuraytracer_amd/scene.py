@@ -133,6 +133,13 @@ class SceneBuilder:
         uv = A.fvec(*uv_scale) if uv_scale is not None else None
         return self._check(self.lib.ptgen_set_material_maps(self.h, material, maps, uv))
 
+    def height_params(self, material, normal_multiplier=1.0, height_multiplier=0.05, layers=32.0, offset=0.0):
+        """Material.height_params (tracer.js:247-250): the normal map's x/y
+        scale, the parallax depth (negative: a height map, white high), the
+        number of parallax layers and the start offset."""
+        p = (C.c_float * 4)(normal_multiplier, height_multiplier, layers, offset)
+        return self._check(self.lib.ptgen_set_height_params(self.h, material, p))
+
     def sky_from_hdr(self, source, exposure=1.0):
         """The sky from a Radiance .hdr file (a path or its bytes), as
         HDRTexture loads one (tracer.js:53-87)."""
