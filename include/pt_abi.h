@@ -33,7 +33,7 @@ enum {
     PT_ERR_HIP = -5,         /* a HIP runtime call failed */
     PT_ERR_OOM = -6,         /* device allocation failed */
     PT_ERR_STATE = -7,       /* no scene uploaded / no frame allocated */
-    PT_ERR_UNSUPPORTED = -8, /* feature outside this build (translucent albedo; normal/height maps off spheres, cubes, planes) */
+    PT_ERR_UNSUPPORTED = -8, /* feature outside this build (normal/height maps off spheres, cubes, planes) */
     PT_ERR_NO_DEVICE = -9    /* no HIP device visible */
 };
 
@@ -71,8 +71,9 @@ typedef struct pt_scene_desc {
     float sky_total_lum;    /* ignored when the CDF is built on the device */
     /* Texture array, 512x512 rgba8 layers, rows top first (binding 11),
      * sampled as tracer.js sets it up (linear, repeat, level 0).  Materials
-     * may use albedo (opaque: every alpha 255), roughness, metallic,
-     * emissive, normal and height (parallax occlusion) maps; a material with a
+     * may use albedo (an alpha below 1 makes shadow rays attenuate through
+     * the surface), roughness, metallic, emissive, normal and height
+     * (parallax occlusion) maps; a material with a
      * normal or height map used by a cylinder, torus or mesh is rejected with
      * PT_ERR_UNSUPPORTED (only sphere, cube and plane hits carry a tangent
      * frame), a height map with height_params[2] <= 0 layers with

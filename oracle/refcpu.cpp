@@ -654,8 +654,68 @@ SurfaceHit trace_scene(const ref_scene& S, const Ray& ray, Counters* c) {
 // reduces to "is there an intersection inside (0.001 | 1e-6, target_dist)".
 
 // shader.wgsl:813-872
+// sample_texture (shader.wgsl:170-173): textureSampleLevel on the material
+// texture array, level 0 of a 512 x 512 rgba8unorm layer, linear filtering,
+// repeat addressing (tracer.js:1965-1968); the filter is restated with fp32
+// weights (texel centres at uv * 512 - 0.5).  idx < 0 gives 1.
+constexpr int kTexSize = 512;
+struct f4 { float x, y, z, w; };
+f4 sample_texture(const ref_scene& S, int idx, f2 uv) {
+    if (idx < 0 || uint32_t(idx) >= S.tex_layers) return f4{1.f, 1.f, 1.f, 1.f};
+    const float x = uv.x * float(kTexSize) - 0.5f, y = uv.y * float(kTexSize) - 0.5f;
+    const float x0 = std::floor(x), y0 = std::floor(y);
+    const float fx = x - x0, fy = y - y0;
+    const long ix = long(x0), iy = long(y0);
+    const uint8_t* L = S.tex.data() + size_t(idx) * kTexSize * kTexSize * 4;
+    auto texel = [&](long tx, long ty, int c) {
+        const size_t xx = size_t(tx & (kTexSize - 1)), yy = size_t(ty & (kTexSize - 1));
+        return float(L[(yy * kTexSize + xx) * 4 + size_t(c)]) / 255.0f;
+    };
+    float o[4];
+    for (int c = 0; c < 4; ++c) {
+        const float t00 = texel(ix, iy, c), t10 = texel(ix + 1, iy, c);
+        const float t01 = texel(ix, iy + 1, c), t11 = texel(ix + 1, iy + 1, c);
+        const float a = t00 + (t10 - t00) * fx, b = t01 + (t11 - t01) * fx;
+        o[c] = a + (b - a) * fy;
+    }
+    return f4{o[0], o[1], o[2], o[3]};
+}
+
+// evaluate_shadow_attenuation (shader.wgsl:777-811): a material without an
+// albedo texture stops the light (true); with one, the shadow is scaled by
+// (1 - alpha) of the texel at uv * uv_scale and stopped above alpha 0.9999.
+bool evaluate_shadow_attenuation(const ref_scene& S, const pt_material& mat, f2 uv, f3* shadow) {
+    if (mat.albedo_idx < 0) return true;
+    f4 t = sample_texture(S, mat.albedo_idx, f2{uv.x * mat.uv_scale[0], uv.y * mat.uv_scale[1]});
+    *shadow = *shadow * (1.0f - t.w);
+    return t.w > 0.9999f;
+}
+
+// shader.wgsl:893-901 (sphere), 904-922 (get_cube_uv), 958-962
+// (get_frustum_uv), 1090-1094 (get_torus_uv)
+f2 sphere_shadow_uv(f3 p) {
+    return f2{0.5f + std::atan2(p.z, p.x) / TWO_PI, 0.5f + std::asin(clampf(p.y, -1.0f, 1.0f)) / PI};
+}
+f2 get_cube_uv(f3 p) {
+    f3 d = V(std::fabs(p.x), std::fabs(p.y), std::fabs(p.z));
+    float max_d = max_component(d);
+    if (max_d == d.x) { float s = signf(p.x); return f2{(-s * p.z) * 0.5f + 0.5f, p.y * 0.5f + 0.5f}; }
+    if (max_d == d.y) { float s = signf(p.y); return f2{p.x * 0.5f + 0.5f, (s * p.z) * 0.5f + 0.5f}; }
+    float s = signf(p.z);
+    return f2{(s * p.x) * 0.5f + 0.5f, p.y * 0.5f + 0.5f};
+}
+f2 get_frustum_uv(f3 p, float r0, float r1) {
+    if (p.y > 0.999f) return f2{p.x / (r1 * 2.0f) + 0.5f, p.z / (r1 * 2.0f) + 0.5f};
+    if (p.y < 0.001f) return f2{p.x / (r0 * 2.0f) + 0.5f, p.z / (r0 * 2.0f) + 0.5f};
+    return f2{std::atan2(p.z, p.x) / TWO_PI + 0.5f, p.y};
+}
+f2 get_torus_uv(f3 p) {
+    return f2{(std::atan2(p.z, p.x) / TWO_PI) + 0.5f,
+              (std::atan2(p.y, std::sqrt(p.x * p.x + p.z * p.z) - 1.0f) / TWO_PI) + 0.5f};
+}
+
 bool trace_mesh_shadow(const ref_scene& S, const Ray& rw, const float* inv, uint32_t node_offset, uint32_t tri_offset,
-                       float target_dist, Counters* c) {
+                       int material, float target_dist, f3* shadow, Counters* c) {
     Ray rl{mul_point(inv, rw.origin), mul_dir(inv, rw.direction)};
     f3 inv_dir = V(1.0f) / rl.direction;
     uint32_t stack[STACK_SIZE];
@@ -673,8 +733,13 @@ bool trace_mesh_shadow(const ref_scene& S, const Ray& rw, const float* inv, uint
                 float t_tri = 0.0f;
                 f3 bary;
                 c->tris++;
-                if (intersect_triangle(rl, S.tris[i], &t_tri, &bary))
-                    if (t_tri >= 0.0f && t_tri < target_dist) return true;
+                const pt_triangle& tri = S.tris[i];
+                if (intersect_triangle(rl, tri, &t_tri, &bary))
+                    if (t_tri >= 0.0f && t_tri < target_dist) {
+                        f2 uv{tri.uv0[0] * bary.x + tri.uv1[0] * bary.y + tri.uv2[0] * bary.z,
+                              tri.uv0[1] * bary.x + tri.uv1[1] * bary.y + tri.uv2[1] * bary.z};
+                        if (evaluate_shadow_attenuation(S, S.mats[material], uv, shadow)) return true;
+                    }
             }
         } else {
             c->inner++;
@@ -694,7 +759,7 @@ bool trace_mesh_shadow(const ref_scene& S, const Ray& rw, const float* inv, uint
 }
 
 // shader.wgsl:874-902
-bool trace_sphere_shadow(const Ray& rw, const pt_object& o, float target_dist) {
+bool trace_sphere_shadow(const ref_scene& S, const Ray& rw, const pt_object& o, float target_dist, f3* shadow) {
     Ray lr{mul_point(o.inv_matrix, rw.origin), mul_dir(o.inv_matrix, rw.direction)};
     float a = dot(lr.direction, lr.direction);
     float half_b = dot(lr.origin, lr.direction);
@@ -706,11 +771,14 @@ bool trace_sphere_shadow(const Ray& rw, const pt_object& o, float target_dist) {
     float t2 = (-half_b + sqrtd) / a;
     float h1 = fmx(0.001f, t1);
     float h2 = fmn(target_dist, t2);
-    return h1 < h2;
+    if (!(h1 < h2)) return false;
+    const pt_material& mat = S.mats[o.material_idx];
+    if (evaluate_shadow_attenuation(S, mat, sphere_shadow_uv(lr.origin + lr.direction * h1), shadow)) return true;
+    return evaluate_shadow_attenuation(S, mat, sphere_shadow_uv(lr.origin + lr.direction * h2), shadow);
 }
 
 // shader.wgsl:924-953
-bool trace_cube_shadow(const Ray& rw, const pt_object& o, float target_dist) {
+bool trace_cube_shadow(const ref_scene& S, const Ray& rw, const pt_object& o, float target_dist, f3* shadow) {
     Ray lr{mul_point(o.inv_matrix, rw.origin), mul_dir(o.inv_matrix, rw.direction)};
     f3 inv_dir = V(1.0f) / lr.direction;
     f3 tn = (V(-1.0f) - lr.origin) * inv_dir;
@@ -718,11 +786,14 @@ bool trace_cube_shadow(const Ray& rw, const pt_object& o, float target_dist) {
     f3 t_min = fmn(tn, tf), t_max = fmx(tn, tf);
     float t1 = max_component(t_min), t2 = min_component(t_max);
     float h1 = fmx(0.001f, t1), h2 = fmn(target_dist, t2);
-    return h1 < h2;
+    if (!(h1 < h2)) return false;
+    const pt_material& mat = S.mats[o.material_idx];
+    if (evaluate_shadow_attenuation(S, mat, get_cube_uv(lr.origin + lr.direction * h1), shadow)) return true;
+    return evaluate_shadow_attenuation(S, mat, get_cube_uv(lr.origin + lr.direction * h2), shadow);
 }
 
 // shader.wgsl:961-1016
-bool trace_cylinder_shadow(const Ray& rw, const pt_object& o, float target_dist) {
+bool trace_cylinder_shadow(const ref_scene& S, const Ray& rw, const pt_object& o, float target_dist, f3* shadow) {
     Ray r{mul_point(o.inv_matrix, rw.origin), mul_dir(o.inv_matrix, rw.direction)};
     float k = (o.param0 - 1.0f) / 1.0f;
     float origin_eff_r = 1.0f + k * r.origin.y;
@@ -750,7 +821,13 @@ bool trace_cylinder_shadow(const Ray& rw, const pt_object& o, float target_dist)
     }
     if (hit_count >= 2u) {
         float fh1 = fmx(0.001f, h1), fh2 = fmn(target_dist, h2);
-        if (fh1 < fh2) return true;
+        if (fh1 < fh2) {
+            const pt_material& mat = S.mats[o.material_idx];
+            if (evaluate_shadow_attenuation(S, mat, get_frustum_uv(r.origin + r.direction * fh1, 1.0f, o.param0), shadow))
+                return true;
+            return evaluate_shadow_attenuation(S, mat, get_frustum_uv(r.origin + r.direction * fh2, 1.0f, o.param0),
+                                               shadow);
+        }
     }
     return false;
 }
@@ -805,7 +882,7 @@ void solve_quartic_real(float a, float b, float c, float d, float roots[4]) {
 }
 
 // shader.wgsl:1096-1132
-bool trace_torus_shadow(const Ray& rw, const pt_object& o, float target_dist) {
+bool trace_torus_shadow(const ref_scene& S, const Ray& rw, const pt_object& o, float target_dist, f3* shadow) {
     Ray l{mul_point(o.inv_matrix, rw.origin), mul_dir(o.inv_matrix, rw.direction)};
     float ray_scale = length(l.direction);
     l.direction = l.direction / ray_scale;
@@ -823,13 +900,19 @@ bool trace_torus_shadow(const Ray& rw, const pt_object& o, float target_dist) {
         if (t_near < 0.0f || t_near > target_dist * ray_scale) continue;
         float h1 = fmx(0.001f, t_near) / ray_scale;
         float h2 = fmn(target_dist * ray_scale, t_far) / ray_scale;
-        if (h1 < h2) return true;
+        if (h1 < h2) {
+            const pt_material& mat = S.mats[o.material_idx];
+            if (evaluate_shadow_attenuation(S, mat, get_torus_uv(l.origin + l.direction * (h1 * ray_scale)), shadow))
+                return true;
+            if (evaluate_shadow_attenuation(S, mat, get_torus_uv(l.origin + l.direction * (h2 * ray_scale)), shadow))
+                return true;
+        }
     }
     return false;
 }
 
 // shader.wgsl:1134-1184
-bool trace_tlas_shadow(const ref_scene& S, const Ray& ray, int target_idx, float target_dist, Counters* c) {
+bool trace_tlas_shadow(const ref_scene& S, const Ray& ray, int target_idx, float target_dist, f3* shadow, Counters* c) {
     f3 inv_dir = V(1.0f) / ray.direction;
     uint32_t stack[STACK_SIZE];
     int sp = 0;
@@ -848,15 +931,16 @@ bool trace_tlas_shadow(const ref_scene& S, const Ray& ray, int target_idx, float
                     uint32_t no, to;
                     std::memcpy(&no, &obj.param0, 4);
                     std::memcpy(&to, &obj.param1, 4);
-                    if (trace_mesh_shadow(S, ray, obj.inv_matrix, no, to, target_dist, c)) return true;
+                    if (trace_mesh_shadow(S, ray, obj.inv_matrix, no, to, obj.material_idx, target_dist, shadow, c))
+                        return true;
                 } else if (S.has(PT_HAS_SPHERES) && obj.object_type == 1) {
-                    if (trace_sphere_shadow(ray, obj, target_dist)) return true;
+                    if (trace_sphere_shadow(S, ray, obj, target_dist, shadow)) return true;
                 } else if (S.has(PT_HAS_CUBES) && obj.object_type == 2) {
-                    if (trace_cube_shadow(ray, obj, target_dist)) return true;
+                    if (trace_cube_shadow(S, ray, obj, target_dist, shadow)) return true;
                 } else if (S.has(PT_HAS_CYLINDERS) && obj.object_type == 3) {
-                    if (trace_cylinder_shadow(ray, obj, target_dist)) return true;
+                    if (trace_cylinder_shadow(S, ray, obj, target_dist, shadow)) return true;
                 } else if (S.has(PT_HAS_TORI) && obj.object_type == 4) {
-                    if (trace_torus_shadow(ray, obj, target_dist)) return true;
+                    if (trace_torus_shadow(S, ray, obj, target_dist, shadow)) return true;
                 }
             }
         } else {
@@ -877,12 +961,19 @@ bool trace_tlas_shadow(const ref_scene& S, const Ray& ray, int target_idx, float
 }
 
 // shader.wgsl:1186-1204
-bool trace_plane_shadow(const Ray& ray, const pt_plane& p, float target_dist) {
+bool trace_plane_shadow(const ref_scene& S, const Ray& ray, const pt_plane& p, float target_dist, f3* shadow) {
     f3 n = ld3(p.normal);
     float denom = dot(n, ray.direction);
     if (std::fabs(denom) > 1e-6f) {
         float t = (p.d - dot(n, ray.origin)) / denom;
-        if (t > 0.001f && t < target_dist) return true;
+        if (t > 0.001f && t < target_dist) {
+            f3 hit_pos = ray.origin + ray.direction * t;
+            f3 tangent = std::fabs(n.x) > 0.9999f ? V(0.0f, 0.0f, 1.0f) : V(1.0f, 0.0f, 0.0f);
+            tangent = normalize(cross(n, tangent));
+            f3 bitangent = normalize(cross(n, tangent));
+            f2 uv{dot(hit_pos, tangent), dot(hit_pos, bitangent)};
+            return evaluate_shadow_attenuation(S, S.mats[p.material_idx], uv, shadow);
+        }
     }
     return false;
 }
@@ -890,15 +981,18 @@ bool trace_plane_shadow(const Ray& ray, const pt_plane& p, float target_dist) {
 // shader.wgsl:1206-1229
 f3 trace_scene_shadow(const ref_scene& S, const Ray& ray, int target_idx, float target_dist, Counters* c) {
     c->shadow++;
+    f3 shadow = V(1.0f);
     if (S.has(PT_HAS_PLANES))
         for (const auto& p : S.planes)
-            if (trace_plane_shadow(ray, p, target_dist)) return V(0.0f);
+            if (trace_plane_shadow(S, ray, p, target_dist, &shadow)) return V(0.0f);
     if (S.has(PT_HAS_SPHERES | PT_HAS_CUBES | PT_HAS_CYLINDERS | PT_HAS_TORI | PT_HAS_MESHES))
-        if (trace_tlas_shadow(S, ray, target_idx, target_dist, c)) return V(0.0f);
+        if (trace_tlas_shadow(S, ray, target_idx, target_dist, &shadow, c)) return V(0.0f);
     if (S.has(PT_HAS_LIST_MESHES))
         for (const auto& m : S.meshes)
-            if (trace_mesh_shadow(S, ray, m.inv_matrix, m.node_offset, m.tri_offset, target_dist, c)) return V(0.0f);
-    return V(1.0f);
+            if (trace_mesh_shadow(S, ray, m.inv_matrix, m.node_offset, m.tri_offset, m.material_idx, target_dist,
+                                  &shadow, c))
+                return V(0.0f);
+    return shadow;
 }
 
 // ------------------------------------------------------------ shading -------
@@ -916,32 +1010,6 @@ struct Rng {
 
 struct SurfaceContext { f3 normal, surface_normal, albedo; float roughness, metallic; f3 emittance; float alpha; };
 
-// sample_texture (shader.wgsl:170-173): textureSampleLevel on the material
-// texture array, level 0 of a 512 x 512 rgba8unorm layer, linear filtering,
-// repeat addressing (tracer.js:1965-1968); the filter is restated with fp32
-// weights (texel centres at uv * 512 - 0.5).  idx < 0 gives 1.
-constexpr int kTexSize = 512;
-struct f4 { float x, y, z, w; };
-f4 sample_texture(const ref_scene& S, int idx, f2 uv) {
-    if (idx < 0 || uint32_t(idx) >= S.tex_layers) return f4{1.f, 1.f, 1.f, 1.f};
-    const float x = uv.x * float(kTexSize) - 0.5f, y = uv.y * float(kTexSize) - 0.5f;
-    const float x0 = std::floor(x), y0 = std::floor(y);
-    const float fx = x - x0, fy = y - y0;
-    const long ix = long(x0), iy = long(y0);
-    const uint8_t* L = S.tex.data() + size_t(idx) * kTexSize * kTexSize * 4;
-    auto texel = [&](long tx, long ty, int c) {
-        const size_t xx = size_t(tx & (kTexSize - 1)), yy = size_t(ty & (kTexSize - 1));
-        return float(L[(yy * kTexSize + xx) * 4 + size_t(c)]) / 255.0f;
-    };
-    float o[4];
-    for (int c = 0; c < 4; ++c) {
-        const float t00 = texel(ix, iy, c), t10 = texel(ix + 1, iy, c);
-        const float t01 = texel(ix, iy + 1, c), t11 = texel(ix + 1, iy + 1, c);
-        const float a = t00 + (t10 - t00) * fx, b = t01 + (t11 - t01) * fx;
-        o[c] = a + (b - a) * fy;
-    }
-    return f4{o[0], o[1], o[2], o[3]};
-}
 
 // shader.wgsl:1393-1440 at texture coordinates uv (hit_uv * uv_scale, moved
 // by parallax mapping): the normal map in the frame (tangent, bitangent,

@@ -840,15 +840,97 @@ def test_pom_scene_matches_oracle(renderer):
                 setattr(m, field, v)
 
 
-@pytest.mark.parametrize("bad", ["normal", "height", "translucent", "index", "layers"])
+def _alpha_layer(seed):
+    """rgb noise with an alpha pattern holding opaque (255), clear (0) and
+    partial regions."""
+    t = _tex_layer("noise", seed)
+    yy, xx = np.mgrid[0:512, 0:512]
+    t[..., 3] = np.clip(128 + 220 * np.sin(xx / 37.0 + seed) * np.cos(yy / 29.0), 0, 255).astype(np.uint8)
+    return t
+
+
+def _translucent_scene():
+    """Translucent albedo textures (cut-outs with partial alpha) on a plane,
+    spheres, cubes, a cylinder, a torus and a mesh, beside opaque objects, a
+    light box and an importance-sampled sky."""
+    b = SceneBuilder()
+    b.procedural_sky(128, 64, seed=3)
+    leaf, lace = b.texture(_alpha_layer(1)), b.texture(_alpha_layer(2))
+    m_a = b.material(color=(0.9, 0.8, 0.7), roughness=0.5)
+    b.material_maps(m_a, albedo=leaf, uv_scale=(2.0, 1.0))
+    m_b = b.material(color=(0.6, 0.8, 0.9), roughness=0.3, metallic=0.3)
+    b.material_maps(m_b, albedo=lace)
+    m_floor = b.material(color=(0.8, 0.8, 0.8), roughness=0.7)
+    b.material_maps(m_floor, albedo=lace, uv_scale=(0.5, 0.5))
+    m_plain = b.material(color=(0.5, 0.6, 0.3), roughness=0.4)
+    m_light = b.material(color=(1.0, 1.0, 1.0), emittance=(1.0, 0.9, 0.8), emission_intensity=12.0)
+    b.plane(m_floor, (0.0, 1.0, 0.0), -0.5)
+    b.plane(m_plain, (0.0, 0.0, 1.0), -3.0)
+    b.sphere(m_a, (-1.0, 0.3, 0.0), 0.5)
+    b.sphere(m_plain, (1.4, 0.0, -1.0), 0.4)
+    b.cube(m_b, (0.9, 0.2, 0.3), (0.5, 0.5, 0.5))
+    b.cylinder(m_a, (-0.2, -0.5, 0.9), 0.35, 0.9)
+    b.torus(m_b, (0.3, 0.6, -0.6), 0.45, 0.15)
+    b.instance(b.model_from_obj(_octahedron_obj()), m_a, (-0.4, 0.9, 0.6), scale=(0.35, 0.35, 0.35))
+    b.cube(m_light, (0.0, 2.2, 0.0), (0.8, 0.01, 0.8), enable_nee=True)
+    b.camera()
+    b.bounces(5)
+    return b.finalize()
+
+
+def test_translucent_shadows_match_oracle(renderer):
+    """Albedo textures with alpha below 1: shadow rays attenuate by (1 - alpha)
+    at every surface they cross (exact transmittance per ray against the
+    oracle), and the 1-spp render, the G-buffer (seen-through surfaces) and
+    both backends agree."""
+    scene = _translucent_scene()
+    w, h = 96, 64
+    ref = _setup(renderer, scene, w, h, gbuf=True)
+    rays = random_rays(20000, seed=11, box=2.5, spread=2.0)
+    rng = np.random.default_rng(5)
+    tgt = rng.integers(-1, 8, size=len(rays)).astype(np.int32)
+    dist = rng.uniform(0.5, 8.0, size=len(rays)).astype(np.float32)
+    g = renderer.trace_shadow_rays(rays, tgt, dist)
+    r = ref.trace_shadow_rays(rays, tgt, dist)
+    close = np.abs(g - r) <= 1e-5 + 1e-4 * np.abs(r)
+    partial = (r[:, 0] > 0.0) & (r[:, 0] < 1.0)
+    print("transmittance agreement", float(close.all(-1).mean()), "partial", float(partial.mean()),
+          "blocked", float((r[:, 0] == 0).mean()))
+    assert close.all(-1).mean() >= 0.999
+    assert partial.mean() > 0.05 and (r[:, 0] == 0).mean() > 0.05
+    p = scene.params(w, h, sample_number=0, seed=99)
+    st = renderer.render_section(p, passes=1)
+    ga = renderer.read_accum()
+    ra, _, rst = ref.render(p, passes=1)
+    rel = np.abs(ga[..., :3] - ra[..., :3]) / np.maximum(np.abs(ra[..., :3]), 1e-3)
+    frac = float(np.mean(np.all(rel <= 1e-4, axis=-1)))
+    print("translucent 1spp pixel agreement", frac)
+    assert frac >= 0.97
+    assert abs(int(st.extension_rays) - int(rst.extension_rays)) <= 0.01 * rst.extension_rays
+    renderer.gbuffer(scene.params(w, h))
+    gg = renderer.read_gbuffer()
+    rg, _ = ref.gbuffer(scene.params(w, h))
+    gerr = np.abs(gg - rg) / np.maximum(np.abs(rg), 1e-3)
+    print("translucent gbuffer max rel err", float(gerr.max()))
+    assert float(np.mean(np.all(gerr <= 1e-4, axis=-1))) >= 0.99
+    renderer.reset()
+    accs = []
+    for be in (A.PT_BACKEND_WAVEFRONT, A.PT_BACKEND_MEGAKERNEL):
+        renderer.set_option(A.PT_OPT_BACKEND, be)
+        renderer.resize(w, h)
+        renderer.render_section(p, passes=2, seeds=[31, 32])
+        accs.append(renderer.read_accum())
+    renderer.set_option(A.PT_OPT_BACKEND, A.PT_BACKEND_WAVEFRONT)
+    assert np.array_equal(accs[0], accs[1])
+
+
+@pytest.mark.parametrize("bad", ["normal", "height", "index", "layers"])
 def test_texture_validation(renderer, bad):
     """Rejected: normal and height maps on hits without the reference's own
-    tangent frame (cylinders, tori, meshes), translucent albedo, texture
-    indices beyond the layers and height maps with no parallax layers."""
+    tangent frame (cylinders, tori, meshes), texture indices beyond the
+    layers and height maps with no parallax layers."""
     b = SceneBuilder()
     layer = _tex_layer("noise", 7)
-    if bad == "translucent":
-        layer[10, 20, 3] = 128
     t = b.texture(layer)
     m = b.material(color=(0.5, 0.5, 0.5))
     if bad == "normal":

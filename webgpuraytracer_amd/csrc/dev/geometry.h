@@ -102,6 +102,37 @@ constexpr uint32_t kAnyObject = PT_HAS_SPHERES | PT_HAS_CUBES | PT_HAS_CYLINDERS
 // texture, so the kernels specialised for the configuration flag sets (which
 // lack it) compile the texture code out.
 constexpr uint32_t kTexturesBit = 1u << 20;
+// Also internal: set when an albedo texture has an alpha below 1, so that
+// shadow queries attenuate through such surfaces instead of stopping at the
+// first one.
+constexpr uint32_t kTranslucentBit = 1u << 21;
+
+// sample_texture (shader.wgsl:170-173): textureSampleLevel on the material
+// texture array, level 0 of a 512 x 512 rgba8unorm layer, linear filtering,
+// repeat addressing (tracer.js:1965-1968), restated with fp32 weights
+// (texel centres at uv * 512 - 0.5).
+constexpr int kTexSize = 512;
+__device__ __forceinline__ float4 sample_texture(const DevScene& S, int idx, v2 uv) {
+    const float x = uv.x * float(kTexSize) - 0.5f, y = uv.y * float(kTexSize) - 0.5f;
+    const float x0 = floorf(x), y0 = floorf(y);
+    const float fx = x - x0, fy = y - y0;
+    const int ix = int(x0), iy = int(y0);
+    const uint32_t* L = S.tex + size_t(idx) * kTexSize * kTexSize;
+    const uint32_t xa = uint32_t(ix) & (kTexSize - 1), xb = uint32_t(ix + 1) & (kTexSize - 1);
+    const uint32_t ya = uint32_t(iy) & (kTexSize - 1), yb = uint32_t(iy + 1) & (kTexSize - 1);
+    const uint32_t q00 = L[ya * kTexSize + xa], q10 = L[ya * kTexSize + xb];
+    const uint32_t q01 = L[yb * kTexSize + xa], q11 = L[yb * kTexSize + xb];
+    float o[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float t00 = float((q00 >> (8 * c)) & 255u) / 255.0f, t10 = float((q10 >> (8 * c)) & 255u) / 255.0f;
+        const float t01 = float((q01 >> (8 * c)) & 255u) / 255.0f, t11 = float((q11 >> (8 * c)) & 255u) / 255.0f;
+        const float a = t00 + (t10 - t00) * fx, b = t01 + (t11 - t01) * fx;
+        o[c] = a + (b - a) * fy;
+    }
+    return make_float4(o[0], o[1], o[2], o[3]);
+}
+
 
 // ---------------------------------------------------------------- stack ----
 // Per-lane traversal stack.  The first `cap` entries live in LDS,
@@ -384,6 +415,111 @@ __device__ __forceinline__ bool cylinder_blocks(v3 o, v3 d, float top, float tar
     }
     return hits >= 2u && mx(0.001f, h1) < mn(target, h2);
 }
+// ------------------------------------------- translucent shadow surfaces --
+// evaluate_shadow_attenuation (shader.wgsl:777-811): a surface whose material
+// has no albedo texture stops the light; one with a texture passes (1 - alpha)
+// of it (alpha of the texel at uv * uv_scale) and stops it above alpha 0.9999.
+// Returns true when the light is stopped; tr accumulates the product.
+__device__ __forceinline__ bool shadow_surface(const DevScene& S, int mat, v2 uv, float& tr) {
+    const pt_material& m = S.mats[mat];
+    if (m.albedo_idx < 0) return true;
+    const float a = sample_texture(S, m.albedo_idx, v2{uv.x * m.uv_scale[0], uv.y * m.uv_scale[1]}).w;
+    tr *= 1.0f - a;
+    return a > 0.9999f;
+}
+// the texture coordinates trace_*_shadow evaluates at (shader.wgsl:893-901,
+// 904-922 get_cube_uv, 958-962 get_frustum_uv, 1090-1094 get_torus_uv)
+__device__ __forceinline__ v2 sphere_uv(v3 p) {
+    return v2{0.5f + atan2f(p.z, p.x) / kTWO_PI, 0.5f + asinf(clampf(p.y, -1.0f, 1.0f)) / kPI};
+}
+__device__ __forceinline__ v2 cube_uv(v3 p) {
+    const v3 a = mk(fabsf(p.x), fabsf(p.y), fabsf(p.z));
+    const float m = hmax(a);
+    if (m == a.x) {
+        const float s = sgn(p.x);
+        return v2{(-s * p.z) * 0.5f + 0.5f, p.y * 0.5f + 0.5f};
+    }
+    if (m == a.y) {
+        const float s = sgn(p.y);
+        return v2{p.x * 0.5f + 0.5f, (s * p.z) * 0.5f + 0.5f};
+    }
+    const float s = sgn(p.z);
+    return v2{(s * p.x) * 0.5f + 0.5f, p.y * 0.5f + 0.5f};
+}
+__device__ __forceinline__ v2 frustum_uv(v3 p, float r0, float r1) {
+    if (p.y > 0.999f) return v2{p.x / (r1 * 2.0f) + 0.5f, p.z / (r1 * 2.0f) + 0.5f};
+    if (p.y < 0.001f) return v2{p.x / (r0 * 2.0f) + 0.5f, p.z / (r0 * 2.0f) + 0.5f};
+    return v2{atan2f(p.z, p.x) / kTWO_PI + 0.5f, p.y};
+}
+__device__ __forceinline__ v2 torus_uv(v3 p) {
+    return v2{(atan2f(p.z, p.x) / kTWO_PI) + 0.5f, (atan2f(p.y, sqrtf(p.x * p.x + p.z * p.z) - 1.0f) / kTWO_PI) + 0.5f};
+}
+// entry and exit of the clipped interval [max(0.001, t1), min(target, t2)]
+__device__ __forceinline__ bool interval_shadow(const DevScene& S, int mat, v3 o, v3 d, float h1, float h2, int kind,
+                                                float top, float& tr) {
+    if (!(h1 < h2)) return false;
+    const v3 p1 = o + d * h1, p2 = o + d * h2;
+    const v2 u1 = kind == 0 ? sphere_uv(p1) : kind == 1 ? cube_uv(p1) : frustum_uv(p1, 1.0f, top);
+    if (shadow_surface(S, mat, u1, tr)) return true;
+    const v2 u2 = kind == 0 ? sphere_uv(p2) : kind == 1 ? cube_uv(p2) : frustum_uv(p2, 1.0f, top);
+    return shadow_surface(S, mat, u2, tr);
+}
+// trace_sphere_shadow with attenuation (shader.wgsl:874-902)
+__device__ __forceinline__ bool sphere_shadow(const DevScene& S, int mat, v3 o, v3 d, float target, float& tr) {
+    const float a = dot(d, d), hb = dot(o, d), c = dot(o, o) - 1.0f;
+    const float disc = hb * hb - a * c;
+    if (disc < 0.0f) return false;
+    const float sq = sqrtf(disc);
+    const float t1 = (-hb - sq) / a, t2 = (-hb + sq) / a;
+    return interval_shadow(S, mat, o, d, mx(0.001f, t1), mn(target, t2), 0, 0.0f, tr);
+}
+// trace_cube_shadow with attenuation (shader.wgsl:926-954)
+__device__ __forceinline__ bool cube_shadow(const DevScene& S, int mat, v3 o, v3 d, float target, float& tr) {
+    const v3 inv = splat(1.0f) / d;
+    const v3 tn = (splat(-1.0f) - o) * inv;
+    const v3 tf = (splat(1.0f) - o) * inv;
+    const float t1 = hmax(vmin(tn, tf)), t2 = hmin(vmax(tn, tf));
+    return interval_shadow(S, mat, o, d, mx(0.001f, t1), mn(target, t2), 1, 0.0f, tr);
+}
+// trace_cylinder_shadow with attenuation (shader.wgsl:964-1016)
+__device__ __forceinline__ bool cylinder_shadow(const DevScene& S, int mat, v3 o, v3 d, float top, float target,
+                                                float& tr) {
+    const float k = (top - 1.0f) / 1.0f;
+    const float oer = 1.0f + k * o.y;
+    const float A = d.x * d.x + d.z * d.z - k * k * d.y * d.y;
+    const float B = 2.0f * (o.x * d.x + o.z * d.z - oer * k * d.y);
+    const float C = o.x * o.x + o.z * o.z - oer * oer;
+    float h1 = 1e10f, h2 = -1e10f;
+    uint32_t hits = 0;
+    const float disc = B * B - 4.0f * A * C;
+    if (disc >= 0.0f) {
+        const float sq = sqrtf(disc);
+        const float ts0 = (-B - sq) / (2.0f * A), ts1 = (-B + sq) / (2.0f * A);
+        for (int i = 0; i < 2; ++i) {
+            const float t = i == 0 ? ts0 : ts1;
+            const float y = o.y + t * d.y;
+            if (y >= 0.0f && y <= 1.0f) { h1 = mn(h1, t); h2 = mx(h2, t); ++hits; }
+        }
+    }
+    for (int i = 0; i < 2; ++i) {
+        const float cy = i == 0 ? 0.0f : 1.0f;
+        const float rr = i == 0 ? 1.0f : top;
+        const float t = (cy - o.y) / d.y;
+        const v3 p = o + t * d;
+        if (p.x * p.x + p.z * p.z <= rr * rr) { h1 = mn(h1, t); h2 = mx(h2, t); ++hits; }
+    }
+    if (hits < 2u) return false;
+    return interval_shadow(S, mat, o, d, mx(0.001f, h1), mn(target, h2), 2, top, tr);
+}
+// a triangle of trace_mesh_shadow (shader.wgsl:840-852) at barycentrics (u, v):
+// its interpolated texture coordinates
+__device__ __forceinline__ bool tri_shadow(const DevScene& S, int mat, uint32_t prim, float u, float v, float& tr) {
+    const pt_triangle& t = S.tris[prim];
+    const float b0 = 1.0f - u - v;
+    return shadow_surface(S, mat, v2{t.uv0[0] * b0 + t.uv1[0] * u + t.uv2[0] * v, t.uv0[1] * b0 + t.uv1[1] * u + t.uv2[1] * v},
+                          tr);
+}
+
 // solve_cubic_real / solve_quartic_real, shader.wgsl:1019-1087
 __device__ __forceinline__ float cubic_root(float a, float b, float c) {
     const float q = (a * a - 3.0f * b) / 9.0f;
@@ -398,7 +534,12 @@ __device__ __forceinline__ float cubic_root(float a, float b, float c) {
     if (aa != 0.0f) bb = q / aa;
     return (aa + bb) - a / 3.0f;
 }
-static __device__ __noinline__ bool torus_blocks(v3 wo, v3 wd, const float* inv, float rminor, float target) {
+// trace_torus_shadow (shader.wgsl:1096-1132); with ATT each interval's entry
+// and exit attenuate as evaluate_shadow_attenuation says (S, mat, tr used
+// only then).
+template <bool ATT>
+static __device__ __noinline__ bool torus_shadow(v3 wo, v3 wd, const float* inv, float rminor, float target,
+                                                 const DevScene* S, int mat, float* tr) {
     v3 o = xform_point(inv, wo), d = xform_dir(inv, wd);
     const float rs = len(d);
     d = d / rs;
@@ -447,9 +588,16 @@ static __device__ __noinline__ bool torus_blocks(v3 wo, v3 wd, const float* inv,
         if (tn < 0.0f || tn > target * rs) continue;
         const float h1 = mx(0.001f, tn) / rs;
         const float h2 = mn(target * rs, tf) / rs;
-        if (h1 < h2) return true;
+        if (h1 < h2) {
+            if (!ATT) return true;
+            if (shadow_surface(*S, mat, torus_uv(o + d * (h1 * rs)), *tr)) return true;
+            if (shadow_surface(*S, mat, torus_uv(o + d * (h2 * rs)), *tr)) return true;
+        }
     }
     return false;
+}
+__device__ __forceinline__ bool torus_blocks(v3 wo, v3 wd, const float* inv, float rminor, float target) {
+    return torus_shadow<false>(wo, wd, inv, rminor, target, nullptr, 0, nullptr);
 }
 
 }  // namespace pt

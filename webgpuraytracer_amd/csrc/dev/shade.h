@@ -244,32 +244,6 @@ struct Media {
 // ------------------------------------------------------------------ BSDF ---
 struct Surface { v3 normal, snormal, albedo; float roughness, metallic; v3 emittance; float alpha; };
 
-// sample_texture (shader.wgsl:170-173): textureSampleLevel on the material
-// texture array, level 0 of a 512 x 512 rgba8unorm layer, linear filtering,
-// repeat addressing (tracer.js:1965-1968), restated with fp32 weights
-// (texel centres at uv * 512 - 0.5).
-constexpr int kTexSize = 512;
-__device__ __forceinline__ float4 sample_texture(const DevScene& S, int idx, v2 uv) {
-    const float x = uv.x * float(kTexSize) - 0.5f, y = uv.y * float(kTexSize) - 0.5f;
-    const float x0 = floorf(x), y0 = floorf(y);
-    const float fx = x - x0, fy = y - y0;
-    const int ix = int(x0), iy = int(y0);
-    const uint32_t* L = S.tex + size_t(idx) * kTexSize * kTexSize;
-    const uint32_t xa = uint32_t(ix) & (kTexSize - 1), xb = uint32_t(ix + 1) & (kTexSize - 1);
-    const uint32_t ya = uint32_t(iy) & (kTexSize - 1), yb = uint32_t(iy + 1) & (kTexSize - 1);
-    const uint32_t q00 = L[ya * kTexSize + xa], q10 = L[ya * kTexSize + xb];
-    const uint32_t q01 = L[yb * kTexSize + xa], q11 = L[yb * kTexSize + xb];
-    float o[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const float t00 = float((q00 >> (8 * c)) & 255u) / 255.0f, t10 = float((q10 >> (8 * c)) & 255u) / 255.0f;
-        const float t01 = float((q01 >> (8 * c)) & 255u) / 255.0f, t11 = float((q11 >> (8 * c)) & 255u) / 255.0f;
-        const float a = t00 + (t10 - t00) * fx, b = t01 + (t11 - t01) * fx;
-        o[c] = a + (b - a) * fy;
-    }
-    return make_float4(o[0], o[1], o[2], o[3]);
-}
-
 // get_surface_context's maps (shader.wgsl:1393-1440) at the texture
 // coordinates uv (hit_uv * uv_scale, moved by parallax mapping where the
 // material has a height map): albedo (and its alpha) and emittance times

@@ -167,7 +167,7 @@ __device__ void blas_closest(const DevScene& S, uint32_t root_slot, const float*
 
 template <bool CN, bool W>
 __device__ bool blas_any(const DevScene& S, uint32_t root_slot, const float* inv_m, uint32_t tri_off, v3 o, v3 d,
-                         float tdist, LdsStack& st, Counters& c) {
+                         float tdist, LdsStack& st, Counters& c, float* tr = nullptr, int mat = -1) {
     const v3 lo = xform_point(inv_m, o), ld = xform_dir(inv_m, d);
     const v3 linv = splat(1.0f) / ld;
     const Root& R = S.roots[root_slot];
@@ -184,8 +184,9 @@ __device__ bool blas_any(const DevScene& S, uint32_t root_slot, const float* inv
             for (uint32_t k = tri_off + s0; k < tri_off + s0 + n0; ++k) {
                 if (CN) c.tris++;
                 float u, v;
-                const float t = tri_hit(lo, ld, (W ? S.tri_geo_w : S.tri_geo) + 3u * k, u, v);
-                if (t >= 0.0f && t < tdist) return true;
+                const float4* g = (W ? S.tri_geo_w : S.tri_geo) + 3u * k;
+                const float t = tri_hit(lo, ld, g, u, v);
+                if (t >= 0.0f && t < tdist && (!tr || tri_shadow(S, mat, f2u(g[0].w), u, v, *tr))) return true;
             }
         } else {
             have = visit_node<W, CN>(S.bnodes, S.wbnodes, ref, lo, linv, tdist, st, c, ref);
@@ -208,10 +209,11 @@ __device__ bool blas_any(const DevScene& S, uint32_t root_slot, const float* inv
 // next work is a TLAS-leaf object or a stack pop.
 //
 // Any-hit queries (shadow rays) keep their target distance in h.t, so the
-// culling tests are shared; the first accepted occluder sets `blocked` and
-// ends the query.  Without albedo textures every occluder blocks outright
-// (evaluate_shadow_attenuation, shader.wgsl:777-792), so the transmittance is
-// 0 or 1.
+// culling tests are shared; an occluder that stops the light sets `blocked`
+// and ends the query.  Without translucent albedo textures every occluder
+// stops it (evaluate_shadow_attenuation, shader.wgsl:777-792), so the
+// transmittance is 0 or 1; with them (kTranslucentBit) a textured surface
+// multiplies `trans` by (1 - alpha) and the walk goes on.
 template <uint32_t F, bool CN, bool W>
 struct Trav {
     v3 o, d, inv;     // world-space ray
@@ -219,6 +221,12 @@ struct Trav {
     uint32_t ref, cur, end, tri_off;
     int blas_base, obj, target;
     bool have, in_blas, any, blocked;
+    float trans;
+
+    // an any-hit triangle of object `obj`: true when it stops the light
+    __device__ __forceinline__ bool tri_stops(const DevScene& S, Flags<F> fl, uint32_t prim, float u, float v) {
+        return !fl.has(kTranslucentBit) || tri_shadow(S, S.objs[obj].material_idx, prim, u, v, trans);
+    }
 
     __device__ __forceinline__ void start(const DevScene& S, LdsStack& st) {
         inv = splat(1.0f) / d;
@@ -271,6 +279,7 @@ struct Trav {
         d = d_;
         any = true;
         blocked = false;
+        trans = 1.0f;
         target = target_;
         h.t = tdist;
         if (fl.has(PT_HAS_PLANES)) {
@@ -281,6 +290,14 @@ struct Trav {
                 if (fabsf(denom) > 1e-6f) {
                     const float t = (p.d - dot(n, o)) / denom;
                     if (t > 0.001f && t < tdist) {
+                        if (fl.has(kTranslucentBit)) {  // trace_plane_shadow's frame (shader.wgsl:1186-1203)
+                            const v3 hp = o + d * t;
+                            v3 a = mk(1.0f, 0.0f, 0.0f);
+                            if (fabsf(n.x) > 0.9999f) a = mk(0.0f, 0.0f, 1.0f);
+                            const v3 T = normalize(cross(n, a));
+                            const v3 B = normalize(cross(n, T));
+                            if (!shadow_surface(S, p.material_idx, v2{dot(hp, T), dot(hp, B)}, trans)) continue;
+                        }
                         blocked = true;
                         return false;
                     }
@@ -323,6 +340,22 @@ struct Trav {
                         in_blas = true;
                         ref = root_ref<W>(br);
                         have = true;
+                    }
+                } else if (any && fl.has(kTranslucentBit)) {
+                    const int mt = ob.material_idx;
+                    bool b = false;
+                    if (fl.has(PT_HAS_SPHERES) && type == PT_OBJ_SPHERE)
+                        b = sphere_shadow(S, mt, xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), h.t, trans);
+                    else if (fl.has(PT_HAS_CUBES) && type == PT_OBJ_CUBE)
+                        b = cube_shadow(S, mt, xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d), h.t, trans);
+                    else if (fl.has(PT_HAS_CYLINDERS) && type == PT_OBJ_CYLINDER)
+                        b = cylinder_shadow(S, mt, xform_point(ob.inv_matrix, o), xform_dir(ob.inv_matrix, d),
+                                            ob.param0, h.t, trans);
+                    else if (fl.has(PT_HAS_TORI) && type == PT_OBJ_TORUS)
+                        b = torus_shadow<true>(o, d, ob.inv_matrix, ob.param0, h.t, &S, mt, &trans);
+                    if (b) {
+                        blocked = true;
+                        return false;
                     }
                 } else if (any) {
                     bool b = false;
@@ -387,20 +420,27 @@ struct Trav {
                     float t = tri_hit_v(lo, ld, a0, b0, c0, u, v);
                     if (t >= 0.0f && t < h.t) {
                         if (any) {
-                            blocked = true;
-                            return false;
+                            if (tri_stops(S, fl, f2u(a0.w), u, v)) {
+                                blocked = true;
+                                return false;
+                            }
+                        } else {
+                            h.t = t; h.kind = kObjectHit; h.idx = obj; h.prim = int(f2u(a0.w)); h.bu = u; h.bv = v;
                         }
-                        h.t = t; h.kind = kObjectHit; h.idx = obj; h.prim = int(f2u(a0.w)); h.bu = u; h.bv = v;
                     }
                     if (two) {
                         if (CN) c.tris++;
                         t = tri_hit_v(lo, ld, a1, b1, c1, u, v);
                         if (t >= 0.0f && t < h.t) {
                             if (any) {
-                                blocked = true;
-                                return false;
+                                if (tri_stops(S, fl, f2u(a1.w), u, v)) {
+                                    blocked = true;
+                                    return false;
+                                }
+                            } else {
+                                h.t = t; h.kind = kObjectHit; h.idx = obj; h.prim = int(f2u(a1.w)); h.bu = u;
+                                h.bv = v;
                             }
-                            h.t = t; h.kind = kObjectHit; h.idx = obj; h.prim = int(f2u(a1.w)); h.bu = u; h.bv = v;
                         }
                     }
                 }
@@ -423,7 +463,8 @@ struct Trav {
                 const pt_mesh_instance& mi = S.meshes[m];
                 if (any) {
                     if (blocked) break;
-                    blocked = blas_any<CN, W>(S, S.mesh_root[m], mi.inv_matrix, mi.tri_offset, o, d, h.t, st, c);
+                    blocked = blas_any<CN, W>(S, S.mesh_root[m], mi.inv_matrix, mi.tri_offset, o, d, h.t, st, c,
+                                              fl.has(kTranslucentBit) ? &trans : nullptr, mi.material_idx);
                 } else {
                     blas_closest<CN, W>(S, S.mesh_root[m], mi.inv_matrix, mi.tri_offset, o, d, int(m), kListMeshHit,
                                         st, h, c);
@@ -440,6 +481,21 @@ __device__ void closest_hit(const DevScene& S, Flags<F> fl, v3 o, v3 d, LdsStack
         while (q.step(S, fl, st, h, c)) {
         }
     q.finish(S, fl, st, h, c);
+}
+
+// trace_scene_shadow (shader.wgsl:1206-1229): the light that reaches tdist
+// past the occluders (object `target` skipped): 0 when one stops it, else the
+// product of the translucent surfaces' (1 - alpha), 1 without any.
+template <uint32_t F, bool CN, bool W>
+__device__ float transmittance(const DevScene& S, Flags<F> fl, v3 o, v3 d, int target, float tdist, LdsStack& st,
+                               Counters& c) {
+    Trav<F, CN, W> q;
+    HitRec h;
+    if (q.begin_any(S, fl, o, d, target, tdist, st, h))
+        while (q.step(S, fl, st, h, c)) {
+        }
+    q.finish(S, fl, st, h, c);
+    return q.blocked ? 0.0f : q.trans;
 }
 
 // trace_scene_shadow (shader.wgsl:1206-1229): true when an occluder lies

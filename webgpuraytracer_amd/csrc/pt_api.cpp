@@ -52,7 +52,6 @@ int validate_desc(pt_ctx* c, const pt_scene_desc* d) {
     if (d->texture_layers > 0 && !d->textures) return c->fail(PT_ERR_ARG, "texture_layers > 0 without texels");
     if (d->texture_layers > 4096) return c->fail(PT_ERR_ARG, "at most 4096 texture layers");
     const pt_material* m = static_cast<const pt_material*>(d->materials.ptr);
-    const uint8_t* tex = d->textures;
     const pt_object* objs = static_cast<const pt_object*>(d->objects.ptr);
     const pt_mesh_instance* mi = static_cast<const pt_mesh_instance*>(d->meshes.ptr);
     for (uint64_t i = 0; i < d->materials.count; ++i) {
@@ -82,15 +81,6 @@ int validate_desc(pt_ctx* c, const pt_scene_desc* d) {
             if (t >= 0 && uint32_t(t) >= d->texture_layers)
                 return c->fail(PT_ERR_LAYOUT, "material %llu: texture %d outside the %u layers",
                                (unsigned long long)i, t, d->texture_layers);
-        // a translucent albedo texture would make shadow rays attenuate instead of stop
-        // (evaluate_shadow_attenuation, shader.wgsl:777-811)
-        if (m[i].albedo_idx >= 0) {
-            const uint8_t* L = tex + size_t(m[i].albedo_idx) * 512 * 512 * 4;
-            for (size_t k = 3; k < size_t(512) * 512 * 4; k += 4)
-                if (L[k] != 255)
-                    return c->fail(PT_ERR_UNSUPPORTED, "material %llu: albedo texture %d is not opaque",
-                                   (unsigned long long)i, m[i].albedo_idx);
-        }
     }
     if (d->bounce_limit < 0) return c->fail(PT_ERR_ARG, "bounce_limit < 0");
     if (!d->sky_rgba16f || d->sky_width == 0 || d->sky_height == 0) return c->fail(PT_ERR_ARG, "sky texels missing");
@@ -162,12 +152,26 @@ void refresh_dev(pt_ctx* c) {
     S.n_objs = uint32_t(c->n_objs);
     S.sky_w = c->sky_w;
     S.sky_h = c->sky_h;
-    S.flags = c->flags | (c->has_textures ? kTexturesBit : 0u);
+    S.flags = c->flags | (c->has_textures ? kTexturesBit : 0u) | (c->translucent ? kTranslucentBit : 0u);
     S.tex = static_cast<const uint32_t*>(c->tex.p);
     S.tex_layers = c->tex_layers;
     S.bounce_limit = c->bounce_limit;
     S.stack_cap = c->stack_cap;
     S.wstack_cap = c->wstack_cap;
+}
+
+// Some material's albedo texture has an alpha below 1: shadow rays then
+// attenuate through such surfaces (evaluate_shadow_attenuation,
+// shader.wgsl:777-811) instead of stopping at the first occluder.
+bool translucent_albedo(const pt_scene_desc* d) {
+    const pt_material* m = static_cast<const pt_material*>(d->materials.ptr);
+    for (uint64_t i = 0; i < d->materials.count; ++i) {
+        if (m[i].albedo_idx < 0) continue;
+        const uint8_t* L = d->textures + size_t(m[i].albedo_idx) * 512 * 512 * 4;
+        for (size_t k = 3; k < size_t(512) * 512 * 4; k += 4)
+            if (L[k] != 255) return true;
+    }
+    return false;
 }
 
 bool uses_textures(const pt_scene_desc* d) {
@@ -487,6 +491,7 @@ int pt_upload_scene(pt_ctx* c, const pt_scene_desc* d) {
     c->tri_geo_w.release();
     c->tex_layers = d->texture_layers;
     c->has_textures = uses_textures(d);
+    c->translucent = translucent_albedo(d);
     if (d->texture_layers > 0) {
         if ((rc = c->upload(c->tex, d->textures, size_t(d->texture_layers) * 512 * 512 * 4, "textures"))) return rc;
     } else {
@@ -543,6 +548,7 @@ int pt_update_objects(pt_ctx* c, const pt_scene_desc* d) {
     c->bounce_limit = d->bounce_limit;
     if ((rc = upload_dynamic(c, d, false))) return rc;
     c->has_textures = uses_textures(d);  // the texels stay those of the upload
+    c->translucent = translucent_albedo(d);
     if (c->tlas_build == PT_TLAS_BUILD_DEVICE && (rc = device_tlas(c, d))) return rc;
     refresh_dev(c);
     return c->sync("update");

@@ -61,6 +61,7 @@ struct pt_ctx {
     DevBuf tex;                      // material texture array (512 x 512 rgba8 layers)
     uint32_t tex_layers = 0;
     bool has_textures = false;       // some material samples a texture
+    bool translucent = false;        // some albedo texture has an alpha below 1
     DevBuf tri_geo_w;                          // device-built trees: triangle records in their leaf order
     pt::DevScene dev{};
     // frame

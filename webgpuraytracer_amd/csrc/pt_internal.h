@@ -80,7 +80,7 @@ struct WfArgs {
     float4* sh_o;       // per slot x2: origin, distance
     float4* sh_d;       //              direction, ignored object bits
     float4* sh_tp;      //              throughput, valid flag
-    float4* sh_fi;      //              clamped unoccluded radiance
+    float4* sh_fi;      //              clamped unoccluded radiance (unclamped, bounce in w, with translucent occluders)
     uint32_t* counters; // queue A length, queue B length, shadow queue length, two fetch cursors
     int wide;           // 1: wide traversal (PT_TRAVERSAL_WIDE)
     uint32_t trace_grid;    // persistent traversal launches: workgroups (all resident)

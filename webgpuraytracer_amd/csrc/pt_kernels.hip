@@ -86,7 +86,12 @@ struct ImmediateNee {
                                v3& rad) {
         if (need_ray) {
             c.shadow++;
-            if (occluded<F, CN, W>(S, fl, o, d, ignore, dist, st, c)) inc = splat(0.0f);
+            if (fl.has(kTranslucentBit)) {  // sample_color *= shadow_hit (shader.wgsl:2262-2267)
+                const float tr = transmittance<F, CN, W>(S, fl, o, d, ignore, dist, st, c);
+                inc = tr < 0.001f ? splat(0.0f) : inc * tr;
+            } else if (occluded<F, CN, W>(S, fl, o, d, ignore, dist, st, c)) {
+                inc = splat(0.0f);
+            }
         }
         rad = rad + tp * firefly(inc, bounce);
     }
@@ -274,7 +279,8 @@ __global__ void __launch_bounds__(kWave) gbuffer_kernel(DevScene S, pt_scene_par
         rough = m.roughness;
         metal = m.metallic;
         v3 sn = sh.n;
-        if (fl.has(kTexturesBit)) {  // get_surface_context's maps, no parallax (opaque albedo: the alpha < 1 branch never runs)
+        float alpha = 1.0f;
+        if (fl.has(kTexturesBit)) {  // get_surface_context's maps, no parallax
             Surface sc = base_surface(m, sh.n);
             apply_textures(S, m, v2{sh.uv.x * m.uv_scale[0], sh.uv.y * m.uv_scale[1]}, sh.tg, sh.bt, sc);
             albedo = sc.albedo;
@@ -282,10 +288,15 @@ __global__ void __launch_bounds__(kWave) gbuffer_kernel(DevScene S, pt_scene_par
             rough = sc.roughness;
             metal = sc.metallic;
             sn = sc.normal;
+            alpha = sc.alpha;
         }
         n = dot(sn, -rd) > 0.0f ? sn : -sn;
         dist += h.t;
-        // alpha is 1 without albedo textures, so the alpha < 1 branch never runs
+        if (alpha < 1.0f) {  // seen through (shader.wgsl:2553-2557)
+            tint = tint * (splat(1.0f) * (1.0f - alpha) + albedo * alpha);
+            ro = ro + rd * h.t - n * 0.001f;
+            continue;
+        }
         if (m.metallic >= 0.99f && m.roughness <= 0.01f) {
             tint = tint * albedo;
             ro = ro + rd * h.t + n * 0.001f;
@@ -331,8 +342,7 @@ __global__ void __launch_bounds__(kWave) rays_kernel(DevScene S, RayArgs A, int 
     const float* r = A.rays + size_t(i) * 6;
     const v3 o = mk(r[0], r[1], r[2]), d = mk(r[3], r[4], r[5]);
     if (shadow) {
-        const bool b = occluded<F, false, W>(S, fl, o, d, A.target[i], A.tdist[i], st, c);
-        const float v = b ? 0.0f : 1.0f;
+        const float v = transmittance<F, false, W>(S, fl, o, d, A.target[i], A.tdist[i], st, c);
         A.trans[3 * i] = v; A.trans[3 * i + 1] = v; A.trans[3 * i + 2] = v;
     } else {
         HitRec h;

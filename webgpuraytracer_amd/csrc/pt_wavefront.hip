@@ -173,7 +173,11 @@ wf_trace(DevScene S, WfArgs A, int cur) {
     auto apply_slot = [&]() {
         const uint32_t sl = 2u * p + uint32_t(k);
         const float4 tp = A.sh_tp[sl], fi = A.sh_fi[sl];
-        const v3 add = q.blocked ? splat(0.0f) : mk(fi.x, fi.y, fi.z);
+        v3 add = q.blocked ? splat(0.0f) : mk(fi.x, fi.y, fi.z);
+        if (fl.has(kTranslucentBit)) {  // fi holds the unclamped contribution and the bounce
+            add = q.blocked || q.trans < 0.001f ? splat(0.0f)
+                                                : firefly(mk(fi.x, fi.y, fi.z) * q.trans, int(fi.w));
+        }
         rad = rad + mk(tp.x, tp.y, tp.z) * add;
         A.sh_tp[sl].w = 0.0f;
         ++k;
@@ -259,18 +263,20 @@ struct QueuedNee {
     const WfArgs& A;
     uint32_t p;
     bool any;
+    bool att;  // translucent occluders possible
     __device__ void operator()(int k, bool need_ray, v3 o, v3 d, float dist, int ignore, v3 inc, v3 tp, int bounce,
                                v3& rad) {
         if (!need_ray) {
             rad = rad + tp * firefly(inc, bounce);
             return;
         }
-        const v3 fi = firefly(inc, bounce);
+        // clamped now, unless a translucent occluder may scale it first
+        const v3 fi = att ? inc : firefly(inc, bounce);
         const uint32_t s = 2u * p + uint32_t(k);
         A.sh_o[s] = make_float4(o.x, o.y, o.z, dist);
         A.sh_d[s] = make_float4(d.x, d.y, d.z, __int_as_float(ignore));
         A.sh_tp[s] = make_float4(tp.x, tp.y, tp.z, 1.0f);
-        A.sh_fi[s] = make_float4(fi.x, fi.y, fi.z, 0.0f);
+        A.sh_fi[s] = make_float4(fi.x, fi.y, fi.z, att ? float(bounce) : 0.0f);
         any = true;
     }
 };
@@ -311,7 +317,7 @@ __global__ void __launch_bounds__(kWfBlock) wf_shade(DevScene S, WfArgs A, int c
         const uint32_t meta = A.meta[p];
         Media md{A.media + p, A.pool, int(meta & 7u)};
         const SurfHit sh = resolve_for_shading<F>(S, fl, ps.ro, ps.rd, h);
-        QueuedNee sink{A, p, false};
+        QueuedNee sink{A, p, false, fl.has(kTranslucentBit)};
         alive = shade_bounce<F, CN>(S, fl, P, bounce, h, sh, ps, md, c, sink);
         alive = alive && bounce + 1 < S.bounce_limit;
         shadow = sink.any;
