@@ -114,3 +114,74 @@ def test_node_binding_matches_ctypes(tmp_path):
     assert np.array_equal(js_out("b3", np.float32), r3.read_accum().ravel())
     for x in (b, r2, r3, r):
         x.close()
+
+
+JS_TEX = r"""
+const fs = require('fs');
+const t = require(process.argv[1]);
+const dir = process.argv[2];
+const cfg = JSON.parse(fs.readFileSync(dir + '/cfg.json'));
+const rd = (name, T) => { const b = fs.readFileSync(dir + '/' + name + '.bin'); return new T(b.buffer, b.byteOffset, b.byteLength / T.BYTES_PER_ELEMENT); };
+const sceneData = { mat: rd('materials', Uint8Array), object: rd('objects', Uint8Array), tlas: rd('tlas', Uint8Array),
+  mesh: rd('meshes', Uint8Array), bvh: rd('bvh', Uint8Array), triangle: rd('triangles', Uint8Array),
+  plane: rd('planes', Uint8Array), light: rd('lights', Uint8Array), flags: cfg.flags,
+  textures: rd('textures', Uint8Array) };
+const sky = { data: rd('sky', Uint16Array), width: cfg.skyWidth, height: cfg.skyHeight, exposure: cfg.skyExposure };
+const cdf = { skyCDF: rd('cdf', Float32Array), totalWeight: cfg.skyTotalLum };
+const r = new t.NativeRenderer(0);
+r.setScene(sceneData, sky, cdf, cfg.bounces);
+r.resize(cfg.width, cfg.height);
+r.render(cfg.camera, 3, 0);
+const a = r.readAccum();
+fs.writeFileSync(dir + '/js_accum.bin', Buffer.from(a.buffer, a.byteOffset, a.byteLength));
+"""
+
+
+def test_node_binding_textured_scene(tmp_path):
+    """Textures through the Node binding (sceneData.textures): normal and
+    height maps with parallax occlusion, and translucent albedo, render the
+    same bits as through ctypes."""
+    if not shutil.which("node") or not os.path.exists(ADDON):
+        pytest.skip("node or the addon is not available")
+    from test_gpu_parity import _alpha_layer, _bump_layers
+    b = SceneBuilder()
+    b.procedural_sky(64, 32, seed=2)
+    nt, ht = _bump_layers(6)
+    nrm, hgt, lace = b.texture(nt), b.texture(ht), b.texture(_alpha_layer(4))
+    m_pom = b.material(color=(0.8, 0.7, 0.6), roughness=0.4)
+    b.material_maps(m_pom, normal=nrm, height=hgt, uv_scale=(0.5, 0.5))
+    b.height_params(m_pom, height_multiplier=0.08, layers=16.0)
+    m_lace = b.material(color=(0.6, 0.8, 0.9), roughness=0.3)
+    b.material_maps(m_lace, albedo=lace)
+    m_light = b.material(color=(1.0, 1.0, 1.0), emittance=(1.0, 0.9, 0.8), emission_intensity=10.0)
+    b.plane(m_pom, (0.0, 1.0, 0.0), -0.5)
+    b.sphere(m_pom, (-0.6, 0.2, 0.0), 0.45)
+    b.cube(m_lace, (0.6, 0.2, 0.2), (0.5, 0.5, 0.5))
+    b.cylinder(m_lace, (0.0, -0.5, 0.9), 0.3, 0.7)
+    b.cube(m_light, (0.0, 2.0, 0.0), (0.6, 0.01, 0.6), enable_nee=True)
+    b.camera()
+    b.bounces(4)
+    scene = b.finalize()
+    d = scene.desc()
+    for name in ("materials", "objects", "tlas", "meshes", "bvh", "triangles", "planes", "lights"):
+        a = getattr(d, name)
+        (tmp_path / (name + ".bin")).write_bytes(C.string_at(a.ptr, a.count * a.stride) if a.count else b"")
+    (tmp_path / "sky.bin").write_bytes(C.string_at(d.sky_rgba16f, d.sky_width * d.sky_height * 8))
+    (tmp_path / "cdf.bin").write_bytes(C.string_at(d.sky_cdf, d.sky_cdf_count * 4))
+    (tmp_path / "textures.bin").write_bytes(C.string_at(d.textures, d.texture_layers * 512 * 512 * 4))
+    w, h = 48, 32
+    cfg = {"flags": d.flags, "bounces": d.bounce_limit, "skyWidth": d.sky_width, "skyHeight": d.sky_height,
+           "skyExposure": d.sky_exposure, "skyTotalLum": d.sky_total_lum, "width": w, "height": h,
+           "camera": _camera(scene.params(w, h, 0, 0))}
+    (tmp_path / "cfg.json").write_text(json.dumps(cfg))
+    js = subprocess.run(["node", "-e", JS_TEX, SHIM, str(tmp_path)], stdout=subprocess.PIPE,
+                        stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert js.returncode == 0, js.stdout
+    r = Renderer(0)
+    r.set_scene(scene)
+    r.resize(w, h)
+    r.render(spp=3)
+    acc = r.read_accum()
+    assert np.array_equal(np.frombuffer((tmp_path / "js_accum.bin").read_bytes(), np.float32), acc.ravel())
+    assert float(acc[..., :3].mean()) > 0.0
+    r.close()
