@@ -140,4 +140,29 @@ int ht_trace_shadow_rays(const pt_scene_desc* d, const float* rays, const int32_
     return 0;
 }
 
+// Transmittance of shadow queries (0 when blocked), with the scene's
+// textures: albedo alphas below 1 make the queries attenuate.
+int ht_shadow_transmittance(const pt_scene_desc* d, const float* rays, const int32_t* target, const float* dist,
+                            uint32_t n, int wide, float* out) {
+    HostScene h;
+    std::string err;
+    if (!make_scene(d, h, err)) return -1;
+    h.S.tex = reinterpret_cast<const uint32_t*>(d->textures);
+    h.S.tex_layers = d->texture_layers;
+    h.S.flags |= kTexturesBit | kTranslucentBit;
+    const uint32_t cap = wide ? h.S.wstack_cap : h.S.stack_cap;
+    std::vector<uint32_t> stack(cap + 1);
+    const Flags<kDynamicFlags> fl{h.S.flags};
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + size_t(i) * 6;
+        const v3 o = mk(r[0], r[1], r[2]), dd = mk(r[3], r[4], r[5]);
+        LdsStack st = lds_stack_of(stack.data(), 1, cap);
+        Counters c;
+        c.clear();
+        out[i] = wide ? transmittance<kDynamicFlags, false, true>(h.S, fl, o, dd, target[i], dist[i], st, c)
+                      : transmittance<kDynamicFlags, false, false>(h.S, fl, o, dd, target[i], dist[i], st, c);
+    }
+    return 0;
+}
+
 }  // extern "C"

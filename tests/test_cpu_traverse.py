@@ -38,6 +38,8 @@ def ht(tmp_path_factory):
     L.ht_trace_rays.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_uint32, C.c_void_p]
     L.ht_trace_shadow_rays.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int,
                                        C.c_void_p]
+    L.ht_shadow_transmittance.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int,
+                                          C.c_void_p]
     return L
 
 
@@ -191,3 +193,71 @@ def test_wide_nodes_regroup_the_reference_tree(ht):
     bvh_leaves = int(np.sum(bvh[:, 3] > 0))
     assert leaves(wt) == tlas_leaves
     assert leaves(wb) == bvh_leaves  # each distinct BLAS converted once
+
+
+_OCTA_OBJ = b"""v 1 0 0
+v -1 0 0
+v 0 1 0
+v 0 -1 0
+v 0 0 1
+v 0 0 -1
+vt 0 0
+vt 1 0
+vt 0.5 1
+f 1/1 3/2 5/3
+f 3/1 2/2 5/3
+f 2/1 4/2 5/3
+f 4/1 1/2 5/3
+f 3/1 1/2 6/3
+f 2/1 3/2 6/3
+f 4/1 2/2 6/3
+f 1/1 4/2 6/3
+"""
+
+
+def _alpha_texture(seed):
+    rng = np.random.default_rng(seed)
+    t = rng.integers(0, 256, (512, 512, 4)).astype(np.uint8)
+    yy, xx = np.mgrid[0:512, 0:512]
+    t[..., 3] = np.clip(128 + 220 * np.sin(xx / 31.0 + seed) * np.cos(yy / 23.0), 0, 255).astype(np.uint8)
+    return t
+
+
+def test_translucent_shadows_equal_oracle(ht):
+    """Shadow queries through translucent albedo textures on every primitive
+    type (the mesh both as a TLAS object and as a list mesh): both traversal
+    orders give the oracle's transmittance, up to the order of the product."""
+    b = SceneBuilder()
+    lace = b.texture(_alpha_texture(1))
+    m = b.material(color=(0.8, 0.8, 0.8))
+    b.material_maps(m, albedo=lace, uv_scale=(1.5, 1.0))
+    m_opaque = b.material(color=(0.5, 0.5, 0.5))
+    b.plane(m, (0.0, 1.0, 0.0), -1.5)
+    for k in range(6):
+        b.sphere(m, (float(k) - 2.5, 0.0, -1.0), 0.45)
+        b.cube(m, (float(k) - 2.5, 0.0, 1.0), (0.35, 0.35, 0.35))
+    b.cylinder(m, (-1.0, -1.0, 0.0), 0.4, 1.2)
+    b.torus(m, (1.0, 0.0, 0.0), 0.5, 0.15)
+    b.sphere(m_opaque, (0.0, 1.5, 0.0), 0.3)
+    tri = b.model_from_obj(_OCTA_OBJ)
+    b.instance(tri, m, (0.0, 0.8, 0.5), scale=(0.5, 0.5, 0.5))
+    b.instance(tri, m, (-2.0, 0.8, 0.0), scale=(0.4, 0.4, 0.4), in_scene_bvh=False)
+    b.sky()
+    b.camera()
+    scene = b.finalize()
+    desc = scene.desc()
+    assert desc.flags & A.HAS_LIST_MESHES
+    rays = random_rays(6000, seed=4, box=3.0, spread=2.5)
+    rng = np.random.default_rng(8)
+    target = rng.integers(-1, desc.objects.count, size=len(rays)).astype(np.int32)
+    dist = rng.uniform(0.5, 8.0, size=len(rays)).astype(np.float32)
+    want = RefScene(desc).trace_shadow_rays(rays, target, dist)[:, 0]
+    for wide in (0, 1):
+        got = np.zeros(len(rays), np.float32)
+        assert ht.ht_shadow_transmittance(C.byref(desc), rays.ctypes.data, target.ctypes.data, dist.ctypes.data,
+                                          len(rays), wide, got.ctypes.data) == 0
+        close = np.abs(got - want) <= 1e-6 + 1e-4 * want
+        print("wide", wide, "agreement", float(close.mean()))
+        assert close.mean() >= 0.999
+    partial = (want > 0.0) & (want < 1.0)
+    assert partial.mean() > 0.1 and (want == 0.0).mean() > 0.02
