@@ -106,10 +106,13 @@ constexpr uint32_t kTexturesBit = 1u << 20;
 // shadow queries attenuate through such surfaces instead of stopping at the
 // first one.
 constexpr uint32_t kTranslucentBit = 1u << 21;
-// Configuration-1 scenes with textured materials (and with height maps):
-// specialised like the configuration flag sets, with the texture code kept.
+// Configuration-1 scenes with textured materials (and with height maps, and
+// either with translucent albedo): specialised like the configuration flag
+// sets, with the texture code kept.
 constexpr uint32_t kFlagsC1T = kFlagsC1 | kTexturesBit;
 constexpr uint32_t kFlagsC1H = kFlagsC1 | kTexturesBit | PT_HAS_HEIGHTMAPS;
+constexpr uint32_t kFlagsC1TA = kFlagsC1T | kTranslucentBit;
+constexpr uint32_t kFlagsC1HA = kFlagsC1H | kTranslucentBit;
 
 // sample_texture (shader.wgsl:170-173): textureSampleLevel on the material
 // texture array, level 0 of a 512 x 512 rgba8unorm layer, linear filtering,

@@ -605,6 +605,8 @@ static hipError_t trace_dispatch(const DevScene& S, const TraceArgs& a, bool cn,
     if (!a.hits && S.flags == kFlagsC3) return trace_f<kFlagsC3>(S, a, cn, grid, bpc, s);
     if (!a.hits && S.flags == kFlagsC1T) return trace_f<kFlagsC1T>(S, a, cn, grid, bpc, s);
     if (!a.hits && S.flags == kFlagsC1H) return trace_f<kFlagsC1H>(S, a, cn, grid, bpc, s);
+    if (!a.hits && S.flags == kFlagsC1TA) return trace_f<kFlagsC1TA>(S, a, cn, grid, bpc, s);
+    if (!a.hits && S.flags == kFlagsC1HA) return trace_f<kFlagsC1HA>(S, a, cn, grid, bpc, s);
     return trace_f<kDynamicFlags>(S, a, cn, grid, bpc, s);
 }
 

@@ -416,6 +416,8 @@ hipError_t wf_trace_blocks_per_cu(const DevScene& S, bool count_nodes, bool wide
     if (S.flags == kFlagsC3) return wf_occupancy_f<kFlagsC3>(count_nodes, wide, lds, blocks_per_cu);
     if (S.flags == kFlagsC1T) return wf_occupancy_f<kFlagsC1T>(count_nodes, wide, lds, blocks_per_cu);
     if (S.flags == kFlagsC1H) return wf_occupancy_f<kFlagsC1H>(count_nodes, wide, lds, blocks_per_cu);
+    if (S.flags == kFlagsC1TA) return wf_occupancy_f<kFlagsC1TA>(count_nodes, wide, lds, blocks_per_cu);
+    if (S.flags == kFlagsC1HA) return wf_occupancy_f<kFlagsC1HA>(count_nodes, wide, lds, blocks_per_cu);
     return wf_occupancy_f<kDynamicFlags>(count_nodes, wide, lds, blocks_per_cu);
 }
 
@@ -425,6 +427,8 @@ hipError_t launch_wavefront_batch(const DevScene& S, const WfArgs& a, bool count
     if (S.flags == kFlagsC3) return wf_run<kFlagsC3>(S, a, count_nodes, s);
     if (S.flags == kFlagsC1T) return wf_run<kFlagsC1T>(S, a, count_nodes, s);
     if (S.flags == kFlagsC1H) return wf_run<kFlagsC1H>(S, a, count_nodes, s);
+    if (S.flags == kFlagsC1TA) return wf_run<kFlagsC1TA>(S, a, count_nodes, s);
+    if (S.flags == kFlagsC1HA) return wf_run<kFlagsC1HA>(S, a, count_nodes, s);
     return wf_run<kDynamicFlags>(S, a, count_nodes, s);
 }
 
