@@ -998,15 +998,36 @@ f3 trace_scene_shadow(const ref_scene& S, const Ray& ray, int target_idx, float 
 // ------------------------------------------------------------ shading -------
 struct Rng {
     uint32_t state;
-    // shader.wgsl:147-154
-    float next() {
+    // shader.wgsl:147-154: the output permutation of the current state, then
+    // one LCG step.
+    uint32_t next_word() {
         uint32_t s = state;
         state = s * 747796405u + 2891336453u;
         uint32_t word = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
         word = (word >> 22u) ^ word;
-        return float(word) * 2.3283064365386963e-10f;
+        return word;
     }
+    float next() { return float(next_word()) * 2.3283064365386963e-10f; }
 };
+
+// shader.wgsl:2280-2287 (rng == nullptr: gbuf_main's pixel centre,
+// shader.wgsl:2517-2522): seed the pixel's stream, draw the warm-up value and
+// the two jitter values, and build the camera ray through the jittered point.
+Ray camera_ray(const pt_scene_params& P, uint32_t x, uint32_t y, Rng* rng, f2* jitter) {
+    float jx = 0.5f, jy = 0.5f;
+    if (rng) {
+        const uint32_t idx = y * P.width + x;
+        rng->state = (idx * 1973u + P.sample_number * 9277u + P.seed * 26699u) | 1u;
+        rng->next();
+        jx = rng->next();
+        jy = rng->next();
+    }
+    if (jitter) *jitter = f2{jx, jy};
+    f2 screen_uv{(float(x) + jx) / float(P.width), (float(y) + jy) / float(P.height)};
+    f3 r00 = ld3(P.ray00), r10 = ld3(P.ray10), r01 = ld3(P.ray01), r11 = ld3(P.ray11);
+    f3 ray_dir = normalize(mix3(mix3(r00, r10, screen_uv.x), mix3(r01, r11, screen_uv.x), 1.0f - screen_uv.y));
+    return Ray{ld3(P.eye), ray_dir};
+}
 
 struct SurfaceContext { f3 normal, surface_normal, albedo; float roughness, metallic; f3 emittance; float alpha; };
 
@@ -1647,15 +1668,8 @@ void store_hit(pt_hit* out, const SurfaceHit& h) {
 
 // shader.wgsl:2273-2480: radiance of one path through pixel (x, y).
 f3 trace_path(const ref_scene& S, const pt_scene_params& P, uint32_t x, uint32_t y, Counters* c, HitRecorder* rec) {
-    const uint32_t idx = y * P.width + x;
-    Rng rng{(idx * 1973u + P.sample_number * 9277u + P.seed * 26699u) | 1u};
-    rng.next();
-    float jx = rng.next();
-    float jy = rng.next();
-    f2 screen_uv{(float(x) + jx) / float(P.width), (float(y) + jy) / float(P.height)};
-    f3 r00 = ld3(P.ray00), r10 = ld3(P.ray10), r01 = ld3(P.ray01), r11 = ld3(P.ray11);
-    f3 ray_dir = normalize(mix3(mix3(r00, r10, screen_uv.x), mix3(r01, r11, screen_uv.x), 1.0f - screen_uv.y));
-    Ray ray{ld3(P.eye), ray_dir};
+    Rng rng{0u};
+    Ray ray = camera_ray(P, x, y, &rng, nullptr);
     f3 throughput = V(1.0f), radiance = V(0.0f);
     float last_surface_pdf = 1.0f, last_weight_sum = 0.0f;
     MediumStack stack;
@@ -1939,10 +1953,7 @@ int ref_gbuffer(const ref_scene* s, const pt_scene_params* params, pt_gbuffer_te
         Counters c;
         for (uint32_t x = 0; x < P.width; ++x) {
             const uint32_t idx = y * P.width + x;
-            f2 screen_uv{(float(x) + 0.5f) / float(P.width), (float(y) + 0.5f) / float(P.height)};
-            f3 r00 = ld3(P.ray00), r10 = ld3(P.ray10), r01 = ld3(P.ray01), r11 = ld3(P.ray11);
-            f3 ray_dir = normalize(mix3(mix3(r00, r10, screen_uv.x), mix3(r01, r11, screen_uv.x), 1.0f - screen_uv.y));
-            Ray ray{ld3(P.eye), ray_dir};
+            Ray ray = camera_ray(P, x, y, nullptr, nullptr);
             SurfaceContext ctx;
             std::memset(&ctx, 0, sizeof(ctx));
             pt_material mat;
@@ -2167,6 +2178,80 @@ int ref_env_sample(const ref_scene* s, const pt_scene_params* params, uint32_t n
 int ref_env_pdf(const ref_scene* s, const pt_scene_params* params, const float* dirs, uint32_t k, float* pdf) {
     if (!s || !params || !dirs || !pdf) return PT_ERR_ARG;
     for (uint32_t i = 0; i < k; ++i) pdf[i] = get_sky_pdf(*s, *params, V(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+    return PT_OK;
+}
+
+int ref_pcg(uint32_t state, uint32_t n, uint32_t* words, float* values) {
+    if (!words && !values) return PT_ERR_ARG;
+    Rng rng{state};
+    for (uint32_t i = 0; i < n; ++i) {
+        Rng probe = rng;
+        const uint32_t w = rng.next_word();
+        if (words) words[i] = w;
+        if (values) values[i] = probe.next();
+    }
+    return PT_OK;
+}
+
+int ref_camera_rays(const pt_scene_params* params, uint32_t n, const uint32_t* pixels, int centre, float* rays,
+                    float* jitter) {
+    if (!params || !pixels || !rays) return PT_ERR_ARG;
+    for (uint32_t i = 0; i < n; ++i) {
+        Rng rng{0u};
+        f2 j;
+        const Ray r = camera_ray(*params, pixels[2 * i], pixels[2 * i + 1], centre ? nullptr : &rng, &j);
+        float* o = rays + size_t(i) * 6;
+        o[0] = r.origin.x; o[1] = r.origin.y; o[2] = r.origin.z;
+        o[3] = r.direction.x; o[4] = r.direction.y; o[5] = r.direction.z;
+        if (jitter) { jitter[2 * i] = j.x; jitter[2 * i + 1] = j.y; }
+    }
+    return PT_OK;
+}
+
+int ref_bsdf_terms(int term, const float* args, uint32_t n, float* out) {
+    if (!args || !out) return PT_ERR_ARG;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float a = args[3 * i], b = args[3 * i + 1], c = args[3 * i + 2];
+        switch (term) {
+            case 0: out[i] = D_GGX(a, b); break;                              // (NdotH, alpha2)
+            case 1: out[i] = fresnel_dielectric(a, b, c); break;              // (cos, etai, etat)
+            case 2: out[i] = G_Smith_GGX_div_NV(a, b, c); break;              // (NdotV, NdotL, alpha2)
+            case 3: out[i] = G_Smith_GGX_Uncorrelated_div_NV(a, b, c); break;  // (NdotV, NdotL, alpha2)
+            case 4: out[i] = G1_GGX_div_NV(a, b); break;                      // (NdotV, alpha2)
+            case 5: out[i] = mis_weight(a, b); break;                         // (pdf_a, pdf_b)
+            default: return PT_ERR_ARG;
+        }
+    }
+    return PT_OK;
+}
+
+int ref_light_sample(const ref_scene* s, uint32_t light, const float pos[3], uint32_t n, uint32_t seed, float* out) {
+    if (!s || !pos || !out || light >= s->lights.size()) return PT_ERR_ARG;
+    const pt_light& L = s->lights[light];
+    if (L.obj_idx < 0 || size_t(L.obj_idx) >= s->objs.size()) return PT_ERR_ARG;
+    const pt_object& lo = s->objs[L.obj_idx];
+    const f3 p = V(pos[0], pos[1], pos[2]);
+    for (uint32_t i = 0; i < n; ++i) {
+        Rng rng{(seed + i * 7919u) | 1u};
+        const LightSample ls = sample_light(*s, L, lo, p, rng);
+        float* o = out + size_t(i) * 8;
+        o[0] = ls.dir.x; o[1] = ls.dir.y; o[2] = ls.dir.z; o[3] = ls.dist; o[4] = ls.pdf;
+        o[5] = ls.color.x; o[6] = ls.color.y; o[7] = ls.color.z;
+    }
+    return PT_OK;
+}
+
+int ref_light_pdf(const ref_scene* s, uint32_t light, const float pos[3], const float* dirs, uint32_t k, float* pdf) {
+    if (!s || !pos || !dirs || !pdf || light >= s->lights.size()) return PT_ERR_ARG;
+    const pt_light& L = s->lights[light];
+    if (L.obj_idx < 0 || size_t(L.obj_idx) >= s->objs.size()) return PT_ERR_ARG;
+    const pt_object& lo = s->objs[L.obj_idx];
+    for (uint32_t i = 0; i < k; ++i) {
+        const Ray r{V(pos[0], pos[1], pos[2]), V(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2])};
+        Counters c;
+        const SurfaceHit h = trace_scene(*s, r, &c);
+        pdf[i] = h.o_idx == L.obj_idx ? get_light_pdf(*s, L, lo, r.origin, r.direction, h) : 0.0f;
+    }
     return PT_OK;
 }
 

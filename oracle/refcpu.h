@@ -89,6 +89,34 @@ int ref_bsdf_eval(const pt_material* m, const float n[3], const float v[3], cons
 int ref_env_sample(const ref_scene* s, const pt_scene_params* params, uint32_t n, uint32_t seed, float* out);
 int ref_env_pdf(const ref_scene* s, const pt_scene_params* params, const float* dirs, uint32_t k, float* pdf);
 
+/* rand_pcg (shader.wgsl:146-154) n times from `state`: the output words
+ * (before the float conversion) and the f32 values. */
+int ref_pcg(uint32_t state, uint32_t n, uint32_t* words, float* values);
+
+/* The camera ray of shader.wgsl `main` (2280-2287: seeded stream, warm-up
+ * draw, jitter) for n pixels (x, y pairs), or with centre != 0 the pixel-centre
+ * ray of gbuf_main (2517-2522).  rays: 6 floats each (origin, direction);
+ * jitter (may be NULL): the 2 jitter values. */
+int ref_camera_rays(const pt_scene_params* params, uint32_t n, const uint32_t* pixels, int centre, float* rays,
+                    float* jitter);
+
+/* BSDF building blocks on n argument triples:
+ *   0 D_GGX(NdotH, alpha2)                       shader.wgsl:1700-1704
+ *   1 fresnel_dielectric(cos, etai, etat)        :1721-1733
+ *   2 G_Smith_GGX_div_NV(NdotV, NdotL, alpha2)   :1787-1791
+ *   3 G_Smith_GGX_Uncorrelated_div_NV(..)        :1794-1798
+ *   4 G1_GGX_div_NV(NdotV, alpha2)               :1801-1803
+ *   5 mis_weight(pdf_a, pdf_b)                   :1648-1659 */
+int ref_bsdf_terms(int term, const float* args, uint32_t n, float* out);
+
+/* sample_light (shader.wgsl:1636-1646, sphere :1501-1549, box :1575-1624)
+ * for light `light` seen from pos, n times, sample i from PCG state
+ * (seed + i*7919)|1: out[8*i..] = dir.xyz, dist, pdf, color.rgb.  And the
+ * pdf the renderer gives a BSDF direction that reaches the light
+ * (get_light_pdf :1626-1634, 0 when the ray's closest hit is not the light). */
+int ref_light_sample(const ref_scene* s, uint32_t light, const float pos[3], uint32_t n, uint32_t seed, float* out);
+int ref_light_pdf(const ref_scene* s, uint32_t light, const float pos[3], const float* dirs, uint32_t k, float* pdf);
+
 /* rays: 6 doubles; fills *hit with the double-precision closest hit (t in
  * hit->t rounded to float, exact value in *t_exact). */
 int ref64_closest(const ref_scene* s, const double* ray, pt_hit* hit, double* t_exact);

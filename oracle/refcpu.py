@@ -28,6 +28,11 @@ FUNCS = {
     "ref_env_pdf": (C.c_int, [vp, P(A.SceneParams), vp, A.u32, vp]),
     "ref_bsdf_sample": (C.c_int, [P(A.Material), P(A.f32), P(A.f32), A.u32, A.u32, vp]),
     "ref_bsdf_eval": (C.c_int, [P(A.Material), P(A.f32), P(A.f32), vp, A.u32, vp, vp]),
+    "ref_pcg": (C.c_int, [A.u32, A.u32, vp, vp]),
+    "ref_camera_rays": (C.c_int, [P(A.SceneParams), A.u32, vp, C.c_int, vp, vp]),
+    "ref_bsdf_terms": (C.c_int, [C.c_int, vp, A.u32, vp]),
+    "ref_light_sample": (C.c_int, [vp, A.u32, P(A.f32), A.u32, A.u32, vp]),
+    "ref_light_pdf": (C.c_int, [vp, A.u32, P(A.f32), vp, A.u32, vp]),
 }
 _lib = None
 
@@ -115,6 +120,18 @@ class RefScene:
         _ok(self.L.ref_env_pdf(self.h, C.byref(params), d.ctypes.data, d.shape[0], out.ctypes.data), "ref_env_pdf")
         return out
 
+    def light_sample(self, light, pos, n, seed=1):
+        out = np.zeros((n, 8), np.float32)
+        _ok(self.L.ref_light_sample(self.h, light, A.fvec(*pos), n, seed, out.ctypes.data), "ref_light_sample")
+        return out
+
+    def light_pdf(self, light, pos, dirs):
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros(d.shape[0], np.float32)
+        _ok(self.L.ref_light_pdf(self.h, light, A.fvec(*pos), d.ctypes.data, d.shape[0], out.ctypes.data),
+            "ref_light_pdf")
+        return out
+
     def closest64(self, ray):
         r = (C.c_double * 6)(*[float(x) for x in ray])
         hit = A.Hit()
@@ -156,3 +173,38 @@ def bsdf_eval(material, n, v, L):
     _ok(lib().ref_bsdf_eval(C.byref(material), A.fvec(*n), A.fvec(*v), L.ctypes.data, L.shape[0], f.ctypes.data,
                             pdf.ctypes.data), "ref_bsdf_eval")
     return f, pdf
+
+
+def pcg(state, n):
+    """rand_pcg from `state`, n draws: (output words uint32, values float32)."""
+    w = np.zeros(n, np.uint32)
+    v = np.zeros(n, np.float32)
+    _ok(lib().ref_pcg(state & 0xFFFFFFFF, n, w.ctypes.data, v.ctypes.data), "ref_pcg")
+    return w, v
+
+
+def camera_rays(params, pixels, centre=False):
+    """Camera rays of `main` (jittered) or gbuf_main (centre) for (x, y)
+    pixel pairs: (rays [n][6] float32, jitter [n][2] float32)."""
+    px = np.ascontiguousarray(pixels, np.uint32).reshape(-1, 2)
+    rays = np.zeros((px.shape[0], 6), np.float32)
+    jit = np.zeros((px.shape[0], 2), np.float32)
+    _ok(lib().ref_camera_rays(C.byref(params), px.shape[0], px.ctypes.data, int(centre), rays.ctypes.data,
+                              jit.ctypes.data), "ref_camera_rays")
+    return rays, jit
+
+
+BSDF_TERMS = {"D_GGX": 0, "fresnel_dielectric": 1, "G_Smith_GGX_div_NV": 2, "G_Smith_GGX_Uncorrelated_div_NV": 3,
+              "G1_GGX_div_NV": 4, "mis_weight": 5}
+
+
+def bsdf_term(name, *args):
+    """One of the oracle's BSDF building blocks on broadcast float32 arguments."""
+    cols = np.broadcast_arrays(*[np.asarray(a, np.float32) for a in args])
+    shape = cols[0].shape
+    a = np.zeros((cols[0].size, 3), np.float32)
+    for i, c in enumerate(cols):
+        a[:, i] = c.ravel()
+    out = np.zeros(a.shape[0], np.float32)
+    _ok(lib().ref_bsdf_terms(BSDF_TERMS[name], a.ctypes.data, a.shape[0], out.ctypes.data), "ref_bsdf_terms")
+    return out.reshape(shape)
