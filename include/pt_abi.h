@@ -130,8 +130,13 @@ enum {
                                    new rays, 1..64 (default 56) */
     PT_OPT_BLAS_BUILD = 8,       /* PT_BLAS_BUILD_*: where the wide traversal's bottom-level trees
                                    come from; read by the next pt_upload_scene */
-    PT_OPT_TLAS_BUILD = 9        /* PT_TLAS_BUILD_*: where the top-level tree comes from; read by
+    PT_OPT_TLAS_BUILD = 9,       /* PT_TLAS_BUILD_*: where the top-level tree comes from; read by
                                    every pt_upload_scene and pt_update_objects */
+    /* read-only (pt_get_option), describing the uploaded scene: */
+    PT_OPT_KERNEL_FLAGS = 10,    /* the flag word the trace kernels run with: desc->flags plus internal
+                                   bits (bit 20 material textures, bit 21 albedo alpha below 1) */
+    PT_OPT_SPECIALISED = 11      /* 1 when a trace-kernel variant is compiled for that flag word, 0 when
+                                   the run-time-flags variant runs */
 };
 /* REFERENCE: the uploaded tracer.js BVH, widened on the host (default).
    DEVICE: rebuilt on the GPU from the triangles (linear BVH over Morton
@@ -163,6 +168,9 @@ int pt_create(pt_ctx** out, int device);
 void pt_destroy(pt_ctx* ctx);
 const char* pt_last_error(const pt_ctx* ctx);
 int pt_set_option(pt_ctx* ctx, int key, int64_t value);
+/* The current value of an option, or of a read-only PT_OPT_KERNEL_FLAGS /
+ * PT_OPT_SPECIALISED query (PT_ERR_STATE before the first upload). */
+int pt_get_option(pt_ctx* ctx, int key, int64_t* value);
 
 /* Renderer.setScene (tracer.js:1955-2102): copy all scene arrays, the sky and
  * its CDF to the device and specialise the trace kernels on desc->flags. */
@@ -170,9 +178,12 @@ int pt_upload_scene(pt_ctx* ctx, const pt_scene_desc* desc);
 
 /* Renderer.updateObjects fast path (tracer.js:2132-2164): rewrite objects,
  * TLAS, planes, list meshes and materials in place.  Returns PT_ERR_CAPACITY
- * when an array outgrew its upload and PT_ERR_FLAGS when the flags changed;
- * the caller then calls pt_upload_scene, as the reference falls back to a
- * full setScene. */
+ * when an array outgrew the last pt_upload_scene's (earlier updates that
+ * shrank it do not lower the capacity) and PT_ERR_FLAGS when the flags
+ * changed; the caller then calls pt_upload_scene, as the reference falls back
+ * to a full setScene.  As in the reference, lights, triangles, BVH nodes, the
+ * sky and the texels stay those of the upload: a caller that moves or removes
+ * an emitter re-uploads. */
 int pt_update_objects(pt_ctx* ctx, const pt_scene_desc* desc);
 
 /* Renderer.resize (tracer.js:2166-2205): (re)allocate the fp32 RGBA
@@ -221,6 +232,13 @@ int pt_read_sky_cdf(pt_ctx* ctx, float* dst, uint64_t count, float* total_lum);
  * Primary (b = 0) and secondary (b = 1) hits are what the parity tests
  * compare. */
 int pt_trace_hits(pt_ctx* ctx, const pt_scene_params* params, uint32_t n_bounces, pt_hit* hits);
+
+/* pt_trace_hits that also returns the ray each recorded trace_scene call
+ * traced: rays[(pixel * n_bounces + b) * 6 ..] = origin.xyz, direction.xyz
+ * (zeros after the path has ended; rays may be NULL).  The parity tests trace
+ * the device's own secondary rays on the oracle, so a hit comparison does not
+ * depend on how the two sides round sin/cos/pow while sampling the BSDF. */
+int pt_trace_hits_rays(pt_ctx* ctx, const pt_scene_params* params, uint32_t n_bounces, pt_hit* hits, float* rays);
 
 /* Closest hit (trace_scene, shader.wgsl:754-774) for n rays given as
  * origin.xyz, dir.xyz pairs (6 floats per ray). */

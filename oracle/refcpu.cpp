@@ -1652,7 +1652,7 @@ f3 evaluate_nee(const ref_scene& S, const LightSample& ls, f3 hit_pos, f3 n_orie
     return sample_color * c_weight;
 }
 
-struct HitRecorder { pt_hit* out; uint32_t n; };
+struct HitRecorder { pt_hit* out; uint32_t n; float* rays; };
 
 void store_hit(pt_hit* out, const SurfaceHit& h) {
     std::memset(out, 0, sizeof(*out));
@@ -1677,6 +1677,11 @@ f3 trace_path(const ref_scene& S, const pt_scene_params& P, uint32_t x, uint32_t
     const bool has_prims = S.has(PT_HAS_SPHERES | PT_HAS_CUBES | PT_HAS_CYLINDERS | PT_HAS_TORI | PT_HAS_MESHES);
     (void)has_prims;
     for (int bounce = 0; bounce < S.bounce_limit; ++bounce) {
+        if (rec && rec->rays && uint32_t(bounce) < rec->n) {
+            float* r = rec->rays + size_t(bounce) * 6;
+            r[0] = ray.origin.x; r[1] = ray.origin.y; r[2] = ray.origin.z;
+            r[3] = ray.direction.x; r[4] = ray.direction.y; r[5] = ray.direction.z;
+        }
         SurfaceHit hit = trace_scene(S, ray, c);
         if (rec && uint32_t(bounce) < rec->n) store_hit(&rec->out[bounce], hit);
         pt_object obj;
@@ -1928,16 +1933,23 @@ int ref_render(const ref_scene* s, const pt_scene_params* params, uint32_t secti
 }
 
 int ref_trace_hits(const ref_scene* s, const pt_scene_params* params, uint32_t n_bounces, pt_hit* hits, int threads) {
+    return ref_trace_hits_rays(s, params, n_bounces, hits, nullptr, threads);
+}
+
+int ref_trace_hits_rays(const ref_scene* s, const pt_scene_params* params, uint32_t n_bounces, pt_hit* hits,
+                        float* rays, int threads) {
     if (!s || !params || !hits || n_bounces == 0) return PT_ERR_ARG;
     const pt_scene_params& P = *params;
     for (size_t i = 0; i < size_t(P.width) * P.height * n_bounces; ++i) {
         std::memset(&hits[i], 0, sizeof(pt_hit));
         hits[i].t = -1.0f; hits[i].o_idx = hits[i].m_idx = hits[i].prim_id = hits[i].plane_idx = -1;
     }
+    if (rays) std::memset(rays, 0, size_t(P.width) * P.height * n_bounces * 6 * sizeof(float));
     parallel_rows(P.height, threads, [&](uint32_t y, int) {
         Counters c;
         for (uint32_t x = 0; x < P.width; ++x) {
-            HitRecorder rec{hits + (size_t(y) * P.width + x) * n_bounces, n_bounces};
+            const size_t base = (size_t(y) * P.width + x) * n_bounces;
+            HitRecorder rec{hits + base, n_bounces, rays ? rays + base * 6 : nullptr};
             trace_path(*s, P, x, y, &c, &rec);
         }
     });
@@ -2363,6 +2375,43 @@ double d_cyl(d3 o, d3 d, double r1) {
 }
 }  // namespace
 
+// The torus (axis y, major radius 1, minor radius r) along o + t d in its
+// local frame: the first root of (|p|^2 + 1 - r^2)^2 - 4 (p.x^2 + p.z^2) past
+// the reference's 0.001 in local units along the normalised direction,
+// bracketed by a sign change on a fine grid over the bounding sphere's chord
+// and bisected to double precision (a grazing double root that does not
+// change sign is missed, like any tangency).
+double d_torus(d3 o, d3 d, double r) {
+    const double dl = std::sqrt(ddot(d, d));
+    const double b = ddot(o, d) / (dl * dl), c = (ddot(o, o) - (1.0 + r) * (1.0 + r)) / (dl * dl);
+    const double disc = b * b - c;
+    if (disc <= 0.0) return -1.0;
+    const double sq = std::sqrt(disc);
+    const double t0 = std::max(-b - sq, 0.001 / dl), t1 = -b + sq;
+    if (t1 <= t0) return -1.0;
+    auto f = [&](double t) {
+        const d3 p = o + d * t;
+        const double q = ddot(p, p) + 1.0 - r * r;
+        return q * q - 4.0 * (p.x * p.x + p.z * p.z);
+    };
+    const int N = 8192;
+    double a = t0, fa = f(a);
+    for (int i = 1; i <= N; ++i) {
+        double e = t0 + (t1 - t0) * double(i) / N, fe = f(e);
+        if ((fa > 0.0) != (fe > 0.0)) {
+            for (int k = 0; k < 200 && e - a > 0.0; ++k) {
+                const double m = 0.5 * (a + e), fm = f(m);
+                if ((fa > 0.0) != (fm > 0.0)) e = m; else { a = m; fa = fm; }
+                if (m == a || m == e) break;
+            }
+            return 0.5 * (a + e);
+        }
+        a = e;
+        fa = fe;
+    }
+    return -1.0;
+}
+
 extern "C" int ref64_closest(const ref_scene* s, const double* ray, pt_hit* hit, double* t_exact) {
     if (!s || !ray || !hit) return PT_ERR_ARG;
     const ref_scene& S = *s;
@@ -2411,8 +2460,8 @@ extern "C" int ref64_closest(const ref_scene* s, const double* ray, pt_hit* hit,
         if (ob.object_type == 1) t = d_sphere(lo, ldir);
         else if (ob.object_type == 2) t = d_cube(lo, ldir);
         else if (ob.object_type == 3) t = d_cyl(lo, ldir, ob.param0);
-        else return PT_ERR_UNSUPPORTED;  // torus: no FP64 arbiter
-        if (t > 0.001 && t < best) { best = t; hit->m_idx = ob.material_idx; hit->o_idx = int(i); hit->prim_id = -1; hit->plane_idx = -1; }
+        else t = d_torus(lo, ldir, ob.param0);
+        if (t > (ob.object_type == 4 ? 0.0 : 0.001) && t < best) { best = t; hit->m_idx = ob.material_idx; hit->o_idx = int(i); hit->prim_id = -1; hit->plane_idx = -1; }
     }
     hit->t = float(best);
     if (t_exact) *t_exact = best;

@@ -62,6 +62,10 @@ int ref_render(const ref_scene* s, const pt_scene_params* params, uint32_t secti
  * hits[pixel * n_bounces + b] (t = -1, ids -1 when the path ended earlier). */
 int ref_trace_hits(const ref_scene* s, const pt_scene_params* params, uint32_t n_bounces,
                    pt_hit* hits, int threads);
+/* ref_trace_hits plus each recorded call's ray (6 floats, zeros after the
+ * path ended) in rays[(pixel * n_bounces + b) * 6 ..]; rays may be NULL. */
+int ref_trace_hits_rays(const ref_scene* s, const pt_scene_params* params, uint32_t n_bounces, pt_hit* hits,
+                        float* rays, int threads);
 
 int ref_gbuffer(const ref_scene* s, const pt_scene_params* params, pt_gbuffer_texel* gbuf,
                 uint8_t* out_rgba8, int threads);

@@ -20,6 +20,7 @@ FUNCS = {
     "ref_trace_shadow_rays": (C.c_int, [vp, vp, vp, vp, A.u32, vp]),
     "ref_render": (C.c_int, [vp, P(A.SceneParams), A.u32, A.u32, A.u32, vp, vp, vp, P(A.Stats), C.c_int, A.u32]),
     "ref_trace_hits": (C.c_int, [vp, P(A.SceneParams), A.u32, vp, C.c_int]),
+    "ref_trace_hits_rays": (C.c_int, [vp, P(A.SceneParams), A.u32, vp, vp, C.c_int]),
     "ref_gbuffer": (C.c_int, [vp, P(A.SceneParams), vp, vp, C.c_int]),
     "ref_denoise": (C.c_int, [P(A.DenoiseParams), C.c_int, vp, vp, vp, vp]),
     "ref_build_sky_cdf": (C.c_int, [vp, A.u32, A.u32, A.f32, vp, P(C.c_double)]),
@@ -98,10 +99,17 @@ class RefScene:
             "ref_render")
         return acc, out, st
 
-    def trace_hits(self, params, n_bounces=2, threads=0):
+    def trace_hits(self, params, n_bounces=2, threads=0, rays=False):
+        """Hits of the first n_bounces trace_scene calls per pixel; with
+        rays=True also the rays traced ([pixels * n_bounces][6] float32)."""
         out = (A.Hit * (params.width * params.height * n_bounces))()
-        _ok(self.L.ref_trace_hits(self.h, C.byref(params), n_bounces, out, threads), "ref_trace_hits")
-        return out
+        if not rays:
+            _ok(self.L.ref_trace_hits(self.h, C.byref(params), n_bounces, out, threads), "ref_trace_hits")
+            return out
+        r = np.zeros((params.width * params.height * n_bounces, 6), np.float32)
+        _ok(self.L.ref_trace_hits_rays(self.h, C.byref(params), n_bounces, out, r.ctypes.data, threads),
+            "ref_trace_hits_rays")
+        return out, r
 
     def gbuffer(self, params, threads=0):
         g = np.zeros((params.height, params.width, 12), np.float32)

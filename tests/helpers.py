@@ -37,16 +37,23 @@ def random_rays(n, seed, box=3.0, toward=(0.0, 0.0, 0.0), spread=2.5):
     return np.concatenate([o, d], axis=1).astype(np.float32)
 
 
-def compare_hits(gpu, ref, rays, refscene, rel_t=1e-5):
+def compare_hits(gpu, ref, rays, refscene, rel_t=1e-5, torus=None):
     """Identities must be equal and t within rel_t; a mismatch is accepted only
     when the FP64 brute-force arbiter finds the two candidates tied (their
     double-precision distances within rel_t) and so cannot separate them.
+    Hits on the same torus (object ids in `torus`) are held to t within 1e-3:
+    both sides run the reference's f32 quartic solver, whose conditioning
+    turns last-ulp differences in the built-ins into larger t differences.
     Returns a summary dict; raises AssertionError on a real mismatch."""
     g, r = hits_to_np(gpu), hits_to_np(ref)
     same_id = (g["o"] == r["o"]) & (g["m"] == r["m"]) & (g["prim"] == r["prim"]) & (g["plane"] == r["plane"])
     both_miss = (r["m"] == -1) & (g["m"] == -1)
     tdiff = np.abs(g["t"].astype(np.float64) - r["t"].astype(np.float64))
-    t_ok = both_miss | (tdiff <= rel_t * np.maximum(np.abs(r["t"].astype(np.float64)), 1e-6))
+    tref = np.maximum(np.abs(r["t"].astype(np.float64)), 1e-6)
+    t_ok = both_miss | (tdiff <= rel_t * tref)
+    if torus:
+        tor = same_id & (r["plane"] < 0) & np.isin(r["o"], sorted(torus))
+        t_ok |= tor & (tdiff <= 1e-3 * tref)
     bad = np.nonzero(~(same_id & t_ok))[0]
     adjudicated = 0
     for i in bad:
@@ -61,7 +68,7 @@ def compare_hits(gpu, ref, rays, refscene, rel_t=1e-5):
             i, g["t"][i], g["o"][i], g["prim"][i], r["t"][i], r["o"][i], r["prim"][i], t64, h64.o_idx, h64.prim_id)
         adjudicated += 1
     exact_t = int(np.sum((g["t"] == r["t"]) & same_id))
-    return {"n": len(g["t"]), "exact_t": exact_t, "adjudicated": adjudicated,
+    return {"n": len(g["t"]), "exact_t": exact_t, "within_t": int(np.sum(same_id & t_ok)), "adjudicated": adjudicated,
             "max_rel_t": float(np.max(np.where(both_miss, 0.0, tdiff / np.maximum(np.abs(r["t"]), 1e-6))))}
 
 

@@ -119,6 +119,40 @@ def test_bvh_traversal_equals_fp64_brute_force(which):
     assert ties <= 2
 
 
+def test_torus_hits_equal_fp64_root_finding():
+    """Tori (the reference's resolvent-cubic quartic solver in f32,
+    shader.wgsl:381-471) against the FP64 arbiter's independent root
+    bracketing of the implicit torus, with rotations: the same torus on every
+    ray.  The f32 solver's distances are the reference's own approximation:
+    most within 1e-4 of the true root, a few per cent of them (near double
+    roots, far origins) off by up to a few 1e-2, so the arbiter settles torus
+    identities and the GPU tests hold torus distances to 1e-3."""
+    rng = np.random.default_rng(4)
+    b = SceneBuilder()
+    m = b.material(color=(0.5, 0.5, 0.5))
+    for k in range(6):
+        q = rng.normal(size=4)
+        b.torus(m, tuple(float(x) for x in rng.uniform(-2, 2, 3)), float(rng.uniform(0.4, 1.0)),
+                float(rng.uniform(0.08, 0.3)), rotation=tuple(float(x) for x in q / np.linalg.norm(q)))
+    b.sky()
+    b.camera()
+    s = b.finalize()
+    ref = RefScene(s.desc())
+    rays = random_rays(3000, seed=12, box=4.0, spread=2.5)
+    h = hits_to_np(ref.trace_rays(rays)[0])
+    rel = []
+    for i in range(len(rays)):
+        rc, h64, t64 = ref.closest64(rays[i])
+        assert rc == 0
+        if h64.m_idx == -1 and h["m"][i] == -1:
+            continue
+        assert h64.o_idx == h["o"][i], (i, h64.o_idx, h["o"][i], h["t"][i], t64)
+        rel.append(abs(h["t"][i] - t64) / t64)
+    rel = np.array(rel)
+    assert len(rel) > 300
+    assert np.mean(rel <= 1e-4) > 0.8 and np.mean(rel <= 1e-3) > 0.95 and rel.max() < 5e-2
+
+
 def test_hit_count_counts_pops():
     scene = SceneBuilder.config(1, seed=2)
     ref = RefScene(scene.desc())

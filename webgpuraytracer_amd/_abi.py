@@ -131,6 +131,8 @@ ERR_NAMES = {0: "PT_OK", -1: "PT_ERR_ARG", -2: "PT_ERR_LAYOUT", -3: "PT_ERR_CAPA
 
 PT_OPT_BACKEND, PT_OPT_COUNT_STATS, PT_OPT_WAVEFRONT_POOL, PT_OPT_SCHEDULE, PT_OPT_TRAVERSAL, PT_OPT_LDS_STACK, \
     PT_OPT_WAVEFRONT_REFILL, PT_OPT_BLAS_BUILD, PT_OPT_TLAS_BUILD = 1, 2, 3, 4, 5, 6, 7, 8, 9
+PT_OPT_KERNEL_FLAGS, PT_OPT_SPECIALISED = 10, 11
+KERNEL_TEXTURES_BIT, KERNEL_TRANSLUCENT_BIT = 1 << 20, 1 << 21
 PT_BLAS_BUILD_REFERENCE, PT_BLAS_BUILD_DEVICE = 0, 1
 PT_TLAS_BUILD_REFERENCE, PT_TLAS_BUILD_DEVICE = 0, 1
 PT_BACKEND_MEGAKERNEL, PT_BACKEND_WAVEFRONT = 0, 1
@@ -160,6 +162,8 @@ PT_FUNCS = {
     "pt_resize": (C.c_int, [vp, u32, u32, C.c_int]),
     "pt_trace": (C.c_int, [vp, P(SceneParams), u32, u32, u32, P(u32), P(Stats)]),
     "pt_trace_hits": (C.c_int, [vp, P(SceneParams), u32, P(Hit)]),
+    "pt_get_option": (C.c_int, [vp, C.c_int, P(C.c_int64)]),
+    "pt_trace_hits_rays": (C.c_int, [vp, P(SceneParams), u32, P(Hit), vp]),
     "pt_gbuffer": (C.c_int, [vp, P(SceneParams)]),
     "pt_denoise": (C.c_int, [vp, P(DenoiseParams), C.c_int]),
     "pt_read_accum": (C.c_int, [vp, vp, u64]),

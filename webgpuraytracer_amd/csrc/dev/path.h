@@ -4,8 +4,9 @@
 // Next-event estimation hands each shadow query to a sink: the megakernel
 // traces it at once, the wavefront tracer queues it for the shadow kernel.
 // Either way the contribution is added to the path radiance in the
-// reference's order (light NEE, then env NEE, before the next bounce), so
-// both backends produce the same bits.
+// reference's order (light NEE, then env NEE, then the emission of a surface
+// the height field blocks the sampled direction at, before the next bounce),
+// so both backends produce the same bits.
 #pragma once
 #include "shade.h"
 
@@ -182,7 +183,7 @@ __device__ bool shade_bounce(const DevScene& S, Flags<F> fl, const Params& P, in
         if (sr.hit) {
             Surface s2 = base_surface(m, sh.n);
             apply_textures(S, m, sr.uv, sh.tg, sh.bt, s2);
-            ps.rad = ps.rad + ps.tp * firefly(s2.emittance, bounce);
+            sink.emit(ps.tp * firefly(s2.emittance, bounce), ps.rad);  // after the NEE contributions
             if (len(s2.emittance) > 1.0f) return false;
             if (!sample_bsdf(ps.ro, ps.rd, ps.tp, m, s2, hp, md, ps.last_pdf, ps.rng)) return false;
         }

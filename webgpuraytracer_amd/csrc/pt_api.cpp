@@ -160,17 +160,25 @@ void refresh_dev(pt_ctx* c) {
     S.wstack_cap = c->wstack_cap;
 }
 
+// Per texture layer: does any texel have an alpha below 255?  Scanned once at
+// upload; pt_update_objects keeps the uploaded texels and this table.
+void scan_layer_alpha(pt_ctx* c, const pt_scene_desc* d) {
+    c->layer_alpha.assign(d->texture_layers, 0);
+    for (uint32_t l = 0; l < d->texture_layers; ++l) {
+        const uint8_t* L = d->textures + size_t(l) * 512 * 512 * 4;
+        for (size_t k = 3; k < size_t(512) * 512 * 4; k += 4)
+            if (L[k] != 255) { c->layer_alpha[l] = 1; break; }
+    }
+}
+
 // Some material's albedo texture has an alpha below 1: shadow rays then
 // attenuate through such surfaces (evaluate_shadow_attenuation,
 // shader.wgsl:777-811) instead of stopping at the first occluder.
-bool translucent_albedo(const pt_scene_desc* d) {
+bool translucent_albedo(const pt_ctx* c, const pt_scene_desc* d) {
     const pt_material* m = static_cast<const pt_material*>(d->materials.ptr);
-    for (uint64_t i = 0; i < d->materials.count; ++i) {
-        if (m[i].albedo_idx < 0) continue;
-        const uint8_t* L = d->textures + size_t(m[i].albedo_idx) * 512 * 512 * 4;
-        for (size_t k = 3; k < size_t(512) * 512 * 4; k += 4)
-            if (L[k] != 255) return true;
-    }
+    for (uint64_t i = 0; i < d->materials.count; ++i)
+        if (m[i].albedo_idx >= 0 && size_t(m[i].albedo_idx) < c->layer_alpha.size() && c->layer_alpha[m[i].albedo_idx])
+            return true;
     return false;
 }
 
@@ -472,6 +480,28 @@ int pt_set_option(pt_ctx* c, int key, int64_t value) {
     }
 }
 
+int pt_get_option(pt_ctx* c, int key, int64_t* value) {
+    if (!c) return PT_ERR_ARG;
+    if (!value) return c->fail(PT_ERR_ARG, "value is NULL");
+    switch (key) {
+        case PT_OPT_BACKEND: *value = c->backend; return PT_OK;
+        case PT_OPT_COUNT_STATS: *value = c->count_stats ? 1 : 0; return PT_OK;
+        case PT_OPT_WAVEFRONT_POOL: *value = int64_t(c->wf_pool_opt); return PT_OK;
+        case PT_OPT_SCHEDULE: *value = c->schedule; return PT_OK;
+        case PT_OPT_TRAVERSAL: *value = c->traversal; return PT_OK;
+        case PT_OPT_LDS_STACK: *value = c->lds_stack_opt; return PT_OK;
+        case PT_OPT_WAVEFRONT_REFILL: *value = c->wf_refill; return PT_OK;
+        case PT_OPT_BLAS_BUILD: *value = c->blas_build; return PT_OK;
+        case PT_OPT_TLAS_BUILD: *value = c->tlas_build; return PT_OK;
+        case PT_OPT_KERNEL_FLAGS:
+        case PT_OPT_SPECIALISED:
+            if (!c->has_scene) return c->fail(PT_ERR_STATE, "no scene uploaded");
+            *value = key == PT_OPT_KERNEL_FLAGS ? int64_t(c->dev.flags) : (pt::specialised_flag_set(c->dev.flags) ? 1 : 0);
+            return PT_OK;
+        default: return c->fail(PT_ERR_ARG, "unknown option %d", key);
+    }
+}
+
 int pt_upload_scene(pt_ctx* c, const pt_scene_desc* d) {
     if (!c) return PT_ERR_ARG;
     (void)hipSetDevice(c->device);
@@ -481,6 +511,13 @@ int pt_upload_scene(pt_ctx* c, const pt_scene_desc* d) {
     c->flags = d->flags;
     c->bounce_limit = d->bounce_limit;
     if ((rc = upload_dynamic(c, d, true))) return rc;
+    // what pt_update_objects may rewrite in place, however its counts shrink and regrow meanwhile
+    c->cap_mats = d->materials.count;
+    c->cap_objs = d->objects.count;
+    c->cap_tlas = d->tlas.count;
+    c->cap_planes = d->planes.count;
+    c->cap_meshes = d->meshes.count;
+
     // triangles: 48 B geometry records + the original 128 B records
     const pt_triangle* tr = static_cast<const pt_triangle*>(d->triangles.ptr);
     std::vector<float4> geo;
@@ -491,7 +528,8 @@ int pt_upload_scene(pt_ctx* c, const pt_scene_desc* d) {
     c->tri_geo_w.release();
     c->tex_layers = d->texture_layers;
     c->has_textures = uses_textures(d);
-    c->translucent = translucent_albedo(d);
+    scan_layer_alpha(c, d);
+    c->translucent = translucent_albedo(c, d);
     if (d->texture_layers > 0) {
         if ((rc = c->upload(c->tex, d->textures, size_t(d->texture_layers) * 512 * 512 * 4, "textures"))) return rc;
     } else {
@@ -541,14 +579,14 @@ int pt_update_objects(pt_ctx* c, const pt_scene_desc* d) {
     int rc = validate_desc(c, d);
     if (rc) return rc;
     if (d->flags != c->flags) return c->fail(PT_ERR_FLAGS, "specialisation flags changed");
-    if (d->objects.count > c->n_objs || d->tlas.count > c->n_tlas || d->planes.count > c->n_planes ||
-        d->meshes.count > c->n_meshes || d->materials.count > c->n_mats || d->bvh.count != c->n_bvh ||
+    if (d->objects.count > c->cap_objs || d->tlas.count > c->cap_tlas || d->planes.count > c->cap_planes ||
+        d->meshes.count > c->cap_meshes || d->materials.count > c->cap_mats || d->bvh.count != c->n_bvh ||
         d->triangles.count != c->n_tris || d->texture_layers != c->tex_layers)
         return c->fail(PT_ERR_CAPACITY, "scene arrays outgrew the uploaded buffers");
     c->bounce_limit = d->bounce_limit;
     if ((rc = upload_dynamic(c, d, false))) return rc;
     c->has_textures = uses_textures(d);  // the texels stay those of the upload
-    c->translucent = translucent_albedo(d);
+    c->translucent = translucent_albedo(c, d);
     if (c->tlas_build == PT_TLAS_BUILD_DEVICE && (rc = device_tlas(c, d))) return rc;
     refresh_dev(c);
     return c->sync("update");
@@ -614,7 +652,16 @@ static int prepare_wavefront(pt_ctx* c, pt::TracePlan& plan) {
             if ((rc = c->alloc(*b.b, size_t(P) * b.each, b.what))) { c->wf_pool = 0; return rc; }
         }
         if ((rc = c->alloc(c->wf_cnt, 64, "wf counters"))) return rc;
+        c->wf_em.release();
         c->wf_pool = P;
+    }
+    // deferred emission of height-field-blocked surfaces: height-mapped scenes only, zeroed once
+    // (the shadow kernel clears each entry it applies)
+    if ((c->dev.flags & PT_HAS_HEIGHTMAPS) && c->wf_em.bytes < size_t(c->wf_pool) * 16u) {
+        c->wf_em.release();
+        if ((rc = c->alloc(c->wf_em, size_t(c->wf_pool) * 16u, "wf deferred emission"))) return rc;
+        hipError_t e0 = pt::launch_zero_u64(static_cast<unsigned long long*>(c->wf_em.p), c->wf_pool * 2u, c->stream);
+        if (e0 != hipSuccess) return c->hip_fail(e0, "wf deferred emission clear");
     }
     WfArgs& w = plan.w;
     w = WfArgs{};
@@ -642,6 +689,7 @@ static int prepare_wavefront(pt_ctx* c, pt::TracePlan& plan) {
     w.sh_d = static_cast<float4*>(c->wf_sd.p);
     w.sh_tp = static_cast<float4*>(c->wf_stp.p);
     w.sh_fi = static_cast<float4*>(c->wf_sfi.p);
+    w.sh_em = (c->dev.flags & PT_HAS_HEIGHTMAPS) ? static_cast<float4*>(c->wf_em.p) : nullptr;
     w.counters = static_cast<uint32_t*>(c->wf_cnt.p);
     // persistent traversal launches: grid, stack split, refill threshold
     const bool wide = a.wide && !c->count_stats;
@@ -761,10 +809,12 @@ int read_stats(pt_ctx* c, pt_stats* stats, float ms, uint32_t launches) {
 extern "C" {
 
 static int run_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t sh, uint32_t n_passes,
-                     const uint32_t* seeds, pt_stats* stats, pt_hit* hits_dev, uint32_t n_rec) {
+                     const uint32_t* seeds, pt_stats* stats, pt_hit* hits_dev, uint32_t n_rec,
+                     float* rays_dev = nullptr) {
     pt::TracePlan plan;
     int rc = pt::prepare_trace(c, p, sw, sh, n_passes, seeds, hits_dev, n_rec, plan);
     if (rc) return rc;
+    plan.a.rec_rays = rays_dev;
     const hipError_t e = pt::enqueue_trace(c, plan, c->stream, true);
     if (e != hipSuccess) return c->hip_fail(e, "trace launch");
     if ((rc = c->sync("trace"))) return rc;
@@ -787,6 +837,10 @@ int pt_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t sh, uint
 }
 
 int pt_trace_hits(pt_ctx* c, const pt_scene_params* p, uint32_t n_bounces, pt_hit* hits) {
+    return pt_trace_hits_rays(c, p, n_bounces, hits, nullptr);
+}
+
+int pt_trace_hits_rays(pt_ctx* c, const pt_scene_params* p, uint32_t n_bounces, pt_hit* hits, float* rays) {
     if (!c) return PT_ERR_ARG;
     (void)hipSetDevice(c->device);
     int rc = check_params(c, p);
@@ -800,11 +854,19 @@ int pt_trace_hits(pt_ctx* c, const pt_scene_params* p, uint32_t n_bounces, pt_hi
         h.o_idx = h.m_idx = h.prim_id = h.plane_idx = -1;
     }
     if ((rc = c->upload(c->rays_out, init.data(), n * sizeof(pt_hit), "hit records"))) return rc;
+    if (rays) {  // bounces a path never reached keep zero rays
+        std::vector<float> zero(n * 6, 0.0f);
+        if ((rc = c->upload(c->rays_aux2, zero.data(), n * 6 * sizeof(float), "ray records"))) return rc;
+    }
     pt_scene_params q = *p;
     q.section[0] = q.section[1] = 0;
-    if ((rc = run_trace(c, &q, 0, 0, 1, nullptr, nullptr, static_cast<pt_hit*>(c->rays_out.p), n_bounces))) return rc;
+    if ((rc = run_trace(c, &q, 0, 0, 1, nullptr, nullptr, static_cast<pt_hit*>(c->rays_out.p), n_bounces,
+                        rays ? static_cast<float*>(c->rays_aux2.p) : nullptr)))
+        return rc;
     hipError_t e = hipMemcpy(hits, c->rays_out.p, n * sizeof(pt_hit), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return c->hip_fail(e, "hit readback");
+    if (rays && (e = hipMemcpy(rays, c->rays_aux2.p, n * 6 * sizeof(float), hipMemcpyDeviceToHost)) != hipSuccess)
+        return c->hip_fail(e, "ray readback");
     return PT_OK;
 }
 

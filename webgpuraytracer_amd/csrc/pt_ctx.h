@@ -49,6 +49,8 @@ struct pt_ctx {
     DevBuf mats, objs, obj_root, tnodes, bnodes, wtnodes, wbnodes, roots, tlas_raw, bvh_raw, tri_geo, tris, planes, meshes, mesh_root,
         lights, sky, cdf, cdf_rows;
     uint64_t n_mats = 0, n_objs = 0, n_tlas = 0, n_planes = 0, n_meshes = 0, n_lights = 0, n_bvh = 0, n_tris = 0;
+    // counts at the last pt_upload_scene: the most pt_update_objects accepts
+    uint64_t cap_mats = 0, cap_objs = 0, cap_tlas = 0, cap_planes = 0, cap_meshes = 0;
     uint32_t sky_w = 1, sky_h = 1;
     float sky_total_lum = 0.f;
     double sky_total_weight = 0.0;
@@ -62,6 +64,7 @@ struct pt_ctx {
     uint32_t tex_layers = 0;
     bool has_textures = false;       // some material samples a texture
     bool translucent = false;        // some albedo texture has an alpha below 1
+    std::vector<uint8_t> layer_alpha;  // per uploaded texture layer: some alpha below 255
     DevBuf tri_geo_w;                          // device-built trees: triangle records in their leaf order
     pt::DevScene dev{};
     // frame
@@ -72,7 +75,7 @@ struct pt_ctx {
     uint64_t wf_pool_opt = 1ull << 27;  // paths per wavefront batch (~344 B each)
     uint32_t wf_pool = 0;
     DevBuf wf_o, wf_d, wf_t, wf_r, wf_meta, wf_ha, wf_hb, wf_media, wf_q0, wf_q1, wf_sq, wf_so, wf_sd, wf_stp, wf_sfi,
-        wf_cnt;
+        wf_cnt, wf_em;
     // persistent megakernel: per-lane medium stacks, per-XCD tile counters
     DevBuf mk_media, mk_tiles, mk_spill, wf_spill;
     uint32_t wf_refill = 56;      // wavefront traversal: idle lanes of a wave that trigger a refill
@@ -119,7 +122,7 @@ struct pt_ctx {
                          &planes, &meshes, &mesh_root, &lights, &sky, &cdf, &cdf_rows, &tri_geo_w, &accum, &out, &gbuf, &dn_a,
                          &dn_b, &dn_fin, &dn_guide, &stats, &seeds, &rays_in, &rays_aux, &rays_aux2, &rays_out,
                          &wf_o, &wf_d, &wf_t, &wf_r, &wf_meta, &wf_ha, &wf_hb, &wf_media, &wf_q0, &wf_q1, &wf_sq,
-                         &wf_so, &wf_sd, &wf_stp, &wf_sfi, &wf_cnt, &mk_media, &mk_tiles, &mk_spill, &wf_spill,
+                         &wf_so, &wf_sd, &wf_stp, &wf_sfi, &wf_cnt, &wf_em, &mk_media, &mk_tiles, &mk_spill, &wf_spill,
                          &tex};
         for (DevBuf* b : all) b->release();
     }

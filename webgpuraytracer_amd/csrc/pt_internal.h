@@ -22,6 +22,7 @@ struct TraceArgs {
     uchar4* out;
     unsigned long long* stats;   // kStCount counters
     pt_hit* hits;                // hit recording (NULL: off)
+    float* rec_rays;             // with hits: each recorded bounce's ray, 6 floats (NULL: not kept)
     uint32_t n_rec;
     int schedule;                // PT_SCHEDULE_*
     int wide;                    // 1: wide traversal (PT_TRAVERSAL_WIDE)
@@ -81,6 +82,8 @@ struct WfArgs {
     float4* sh_d;       //              direction, ignored object bits
     float4* sh_tp;      //              throughput, valid flag
     float4* sh_fi;      //              clamped unoccluded radiance (unclamped, bounce in w, with translucent occluders)
+    float4* sh_em;      // per path, height-mapped scenes only (else NULL): emission of a surface the height
+                        // field blocked, to be added after the shadow slots; w = 1 while pending
     uint32_t* counters; // queue A length, queue B length, shadow queue length, two fetch cursors
     int wide;           // 1: wide traversal (PT_TRAVERSAL_WIDE)
     uint32_t trace_grid;    // persistent traversal launches: workgroups (all resident)
@@ -138,6 +141,10 @@ hipError_t build_tlas_device(const pt_object* objs, uint32_t n, const uint32_t* 
                              void* scratch, size_t scratch_bytes, CNode* cn, WNode* wn, uint32_t* ctr,
                              TlasBuildResult* res, hipStream_t stream);
 hipError_t launch_wavefront_batch(const DevScene& S, const WfArgs& a, bool count_nodes, hipStream_t stream);
+// True when the trace kernels have a variant compiled for this kernel flag
+// word (both backends dispatch on the same sets); otherwise the run-time-flags
+// variant runs.
+bool specialised_flag_set(uint32_t flags);
 // Resident workgroups per CU of the wavefront traversal kernels (the larger of the two).
 hipError_t wf_trace_blocks_per_cu(const DevScene& S, bool count_nodes, bool wide, uint32_t stack_lds,
                                   int* blocks_per_cu);

@@ -95,14 +95,16 @@ struct ImmediateNee {
         }
         rad = rad + tp * firefly(inc, bounce);
     }
+    __device__ void emit(v3 add, v3& rad) { rad = rad + add; }
 };
 
 // One path of shader.wgsl `main` (2279-2480).  rec: when non-null, the first
-// n_rec trace_scene hits of the path are written there.
+// n_rec trace_scene hits of the path are written there, and when rec_rays is
+// non-null the ray each of them traced (origin, direction).
 template <uint32_t F, bool CN, bool REC, bool W>
 __device__ v3 trace_path(const DevScene& S, Flags<F> fl, const pt_scene_params& PP, const Params& P, uint32_t px,
                          uint32_t py, uint32_t sample_number, uint32_t seed, LdsStack& st, float* media,
-                         uint32_t media_stride, Counters& c, pt_hit* rec, uint32_t n_rec) {
+                         uint32_t media_stride, Counters& c, pt_hit* rec, float* rec_rays, uint32_t n_rec) {
     PathState ps;
     path_begin(PP, px, py, sample_number, seed, ps);
     Media md{media, media_stride, 0};
@@ -111,6 +113,11 @@ __device__ v3 trace_path(const DevScene& S, Flags<F> fl, const pt_scene_params& 
     for (int bounce = 0; bounce < S.bounce_limit; ++bounce) {
         HitRec h;
         c.ext++;
+        if (REC && rec_rays && uint32_t(bounce) < n_rec) {
+            float* r = rec_rays + size_t(bounce) * 6u;
+            r[0] = ps.ro.x; r[1] = ps.ro.y; r[2] = ps.ro.z;
+            r[3] = ps.rd.x; r[4] = ps.rd.y; r[5] = ps.rd.z;
+        }
         closest_hit<F, CN, W>(S, fl, ps.ro, ps.rd, st, h, c);
         const SurfHit sh = resolve_hit<REC>(S, ps.ro, ps.rd, h);
         if (REC && uint32_t(bounce) < n_rec) store_hit(rec + bounce, h, sh);
@@ -225,8 +232,9 @@ trace_kernel(DevScene S, TraceArgs A) {
                     const uint32_t sn = A.P.sample_number + pass;
                     const uint32_t seed = A.seeds ? A.seeds[pass] : A.P.seed;
                     pt_hit* rec = MODE == kRecord ? A.hits + size_t(idx) * A.n_rec : nullptr;
+                    float* rec_rays = MODE == kRecord && A.rec_rays ? A.rec_rays + size_t(idx) * A.n_rec * 6u : nullptr;
                     const v3 r = trace_path<F, CN, MODE == kRecord, W>(S, fl, A.P, P, px, py, sn, seed, st, media,
-                                                                    A.media_stride, c, rec, A.n_rec);
+                                                                    A.media_stride, c, rec, rec_rays, A.n_rec);
                     acc = lerp(acc, r, 1.0f / float(sn + 1u));
                 }
             }

@@ -187,6 +187,13 @@ wf_trace(DevScene S, WfArgs A, int cur) {
         for (;;) {
             while (k < 2 && A.sh_tp[2u * p + uint32_t(k)].w == 0.0f) ++k;
             if (k >= 2) {
+                if (A.sh_em) {
+                    const float4 e = A.sh_em[p];
+                    if (e.w != 0.0f) {
+                        rad = rad + mk(e.x, e.y, e.z);
+                        A.sh_em[p].w = 0.0f;
+                    }
+                }
                 A.rad[p] = make_float4(rad.x, rad.y, rad.z, 0.0f);
                 busy = false;
                 return;
@@ -278,6 +285,12 @@ struct QueuedNee {
         A.sh_tp[s] = make_float4(tp.x, tp.y, tp.z, 1.0f);
         A.sh_fi[s] = make_float4(fi.x, fi.y, fi.z, att ? float(bounce) : 0.0f);
         any = true;
+    }
+    // radiance that must follow the queued NEE contributions: deferred to the
+    // shadow kernel while a slot is pending
+    __device__ void emit(v3 add, v3& rad) {
+        if (any && A.sh_em) A.sh_em[p] = make_float4(add.x, add.y, add.z, 1.0f);
+        else rad = rad + add;
     }
 };
 
@@ -419,6 +432,11 @@ hipError_t wf_trace_blocks_per_cu(const DevScene& S, bool count_nodes, bool wide
     if (S.flags == kFlagsC1TA) return wf_occupancy_f<kFlagsC1TA>(count_nodes, wide, lds, blocks_per_cu);
     if (S.flags == kFlagsC1HA) return wf_occupancy_f<kFlagsC1HA>(count_nodes, wide, lds, blocks_per_cu);
     return wf_occupancy_f<kDynamicFlags>(count_nodes, wide, lds, blocks_per_cu);
+}
+
+bool specialised_flag_set(uint32_t flags) {
+    return flags == kFlagsC1 || flags == kFlagsC2 || flags == kFlagsC3 || flags == kFlagsC1T || flags == kFlagsC1H ||
+           flags == kFlagsC1TA || flags == kFlagsC1HA;
 }
 
 hipError_t launch_wavefront_batch(const DevScene& S, const WfArgs& a, bool count_nodes, hipStream_t s) {

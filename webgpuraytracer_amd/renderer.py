@@ -45,6 +45,11 @@ class Renderer:
     def set_option(self, key, value):
         return self._check(self.lib.pt_set_option(self.ctx, key, value))
 
+    def get_option(self, key):
+        v = C.c_int64()
+        self._check(self.lib.pt_get_option(self.ctx, key, C.byref(v)))
+        return v.value
+
     # -- setScene / updateObjects / resize (tracer.js:1955-2205) --
     def set_scene(self, scene, isgbuf=False):
         """scene: a finalized SceneBuilder, or an A.SceneDesc."""
@@ -105,10 +110,16 @@ class Renderer:
     def gbuffer(self, params=None):
         self._check(self.lib.pt_gbuffer(self.ctx, C.byref(params or self.params(sample_number=0))))
 
-    def trace_hits(self, params, n_bounces=2):
+    def trace_hits(self, params, n_bounces=2, rays=False):
+        """Hits of each path's first n_bounces trace_scene calls (one sample per
+        pixel); with rays=True also the rays traced ([pixels * n_bounces][6])."""
         out = (A.Hit * (self.width * self.height * n_bounces))()
-        self._check(self.lib.pt_trace_hits(self.ctx, C.byref(params), n_bounces, out))
-        return out
+        if not rays:
+            self._check(self.lib.pt_trace_hits(self.ctx, C.byref(params), n_bounces, out))
+            return out
+        r = np.zeros((self.width * self.height * n_bounces, 6), np.float32)
+        self._check(self.lib.pt_trace_hits_rays(self.ctx, C.byref(params), n_bounces, out, r.ctypes.data))
+        return out, r
 
     # -- readback --
     def read_accum(self):

@@ -6,10 +6,12 @@ maps switched on in stages:
 Scene: a ground plane, 48 spheres and 16 cubes on a grid, an untextured
 cylinder, a light box and a procedural sky (4 textures, 512x512).  Stages, each adding to the previous:
   plain        no maps (the kernels specialised for the scene's flag set)
-  albedo       albedo, roughness and metallic maps (run-time-flags kernels)
+  albedo       albedo, roughness and metallic maps (kernels specialised for textures)
   normal       normal maps
   pom          height maps with parallax occlusion and its shadow march
   translucent  albedo alpha below 1 on the cubes (shadow rays attenuate)
+and, on its own, alpha: the albedo stage with the cubes' alpha below 1 (no
+normal or height maps).
 Prints one JSON line: Mrays/s (extension + shadow rays per second of the
 synchronous render call) per stage.
 """
@@ -55,7 +57,8 @@ def _layers(seed):
 
 def scene(stage, bounces):
     order = ["plain", "albedo", "normal", "pom", "translucent"]
-    k = order.index(stage)
+    alpha_only = stage == "alpha"
+    k = 1 if alpha_only else order.index(stage)
     b = SceneBuilder()
     b.procedural_sky(512, 256, seed=4)
     L = _layers(1)
@@ -68,7 +71,7 @@ def scene(stage, bounces):
     for m in (m_floor, m_sph, m_box):
         maps = {}
         if k >= 1:
-            maps.update(albedo=tex["alpha"] if (k >= 4 and m == m_box) else tex["albedo"],
+            maps.update(albedo=tex["alpha"] if ((k >= 4 or alpha_only) and m == m_box) else tex["albedo"],
                         roughness=tex["albedo"], metallic=tex["albedo"])
         if k >= 2:
             maps["normal"] = tex["normal"]
@@ -98,7 +101,7 @@ def main(argv=None):
     w, h = (int(x) for x in a.size.split("x"))
     rows = []
     r = Renderer(0)
-    for stage in ("plain", "albedo", "normal", "pom", "translucent"):
+    for stage in ("plain", "albedo", "normal", "pom", "translucent", "alpha"):
         s = scene(stage, a.bounces)
         r.set_scene(s)
         r.resize(w, h)
