@@ -132,6 +132,7 @@ enum {
                                    come from; read by the next pt_upload_scene */
     PT_OPT_TLAS_BUILD = 9,       /* PT_TLAS_BUILD_*: where the top-level tree comes from; read by
                                    every pt_upload_scene and pt_update_objects */
+    PT_OPT_DENOISE_MATH = 12,    /* PT_DENOISE_MATH_*: how pt_denoise evaluates the a-trous weights */
     /* read-only (pt_get_option), describing the uploaded scene: */
     PT_OPT_KERNEL_FLAGS = 10,    /* the flag word the trace kernels run with: desc->flags plus internal
                                    bits (bit 20 material textures, bit 21 albedo alpha below 1) */
@@ -152,6 +153,12 @@ enum { PT_BLAS_BUILD_REFERENCE = 0, PT_BLAS_BUILD_DEVICE = 1 };
    node counts are of the device-built tree. */
 enum { PT_TLAS_BUILD_REFERENCE = 0, PT_TLAS_BUILD_DEVICE = 1 };
 enum { PT_BACKEND_MEGAKERNEL = 0, PT_BACKEND_WAVEFRONT = 1 };
+/* FUSED (default): the four Gaussian weights of a tap summed into one base-2
+   exponent with precomputed reciprocals, and pow(dot(n, n'), sigma_normal)
+   joined to it as sigma_normal * log2: one hardware exp2 and log2 per tap,
+   within 1e-5 relative of EXACT on the output.  EXACT: each weight with the
+   library exp / pow and a division, as denoiser.wgsl:117-189 writes it. */
+enum { PT_DENOISE_MATH_FUSED = 0, PT_DENOISE_MATH_EXACT = 1 };
 /* SYNC: the lanes of a wave run each pass together.  REGEN: a lane whose path
    ends starts its pixel's next pass at once.  Both produce the same bits. */
 enum { PT_SCHEDULE_SYNC = 0, PT_SCHEDULE_REGEN = 1 };

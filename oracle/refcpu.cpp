@@ -2035,8 +2035,9 @@ int ref_denoise(const pt_denoise_params* u, int stride_mode, const float* accum,
     };
     auto gn = [&](size_t i) { return ld3(gbuf[i].normal); };
     auto ga = [&](size_t i) { return ld3(gbuf[i].albedo); };
-    // prepass
-    for (int y = 0; y < H; ++y)
+    // prepass (rows in parallel; each pixel's arithmetic is unchanged)
+    parallel_rows(uint32_t(H), 0, [&](uint32_t yy, int) {
+        const int y = int(yy);
         for (int x = 0; x < W; ++x) {
             const size_t idx = size_t(y) * W + x;
             float cc[4];
@@ -2073,12 +2074,14 @@ int ref_denoise(const pt_denoise_params* u, int stride_mode, const float* accum,
             }
             A[idx * 4] = mapped.x; A[idx * 4 + 1] = mapped.y; A[idx * 4 + 2] = mapped.z; A[idx * 4 + 3] = cc[3];
         }
+    });
     const float kernel[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
     for (uint32_t pass = 0; pass < u->passes; ++pass) {
         const uint32_t step = stride_mode == 0 ? (1u << pass) : (1u << (u->passes - 1));
         const std::vector<float>& in = (pass % 2 == 0) ? A : B;
         std::vector<float>& out = (pass % 2 == 0) ? B : A;
-        for (int y = 0; y < H; ++y)
+        parallel_rows(uint32_t(H), 0, [&](uint32_t yy, int) {
+            const int y = int(yy);
             for (int x = 0; x < W; ++x) {
                 const size_t ci = size_t(y) * W + x;
                 const float* cc = &in[ci * 4];
@@ -2116,6 +2119,7 @@ int ref_denoise(const pt_denoise_params* u, int stride_mode, const float* accum,
                     }
                 for (int q = 0; q < 4; ++q) out[ci * 4 + q] = sum[q] / (sum_w + 1e-5f);
             }
+        });
     }
     const std::vector<float>& fin = (u->passes % 2 == 0) ? A : B;
     for (size_t i = 0; i < n; ++i) {

@@ -319,8 +319,20 @@ def test_gbuffer_matches_oracle(renderer):
     assert np.mean(np.abs(renderer.read_output().astype(int) - ro.astype(int)) <= 1) >= 0.999
 
 
+@pytest.fixture(params=[A.PT_DENOISE_MATH_EXACT, A.PT_DENOISE_MATH_FUSED], ids=["exact", "fused"])
+def denoise_math(renderer, request):
+    renderer.set_option(A.PT_OPT_DENOISE_MATH, request.param)
+    yield request.param
+    renderer.set_option(A.PT_OPT_DENOISE_MATH, A.PT_DENOISE_MATH_FUSED)
+
+
+@pytest.mark.parametrize("passes", [4, 7])
 @pytest.mark.parametrize("stride_mode", [0, 1])
-def test_denoise_matches_oracle(renderer, stride_mode):
+def test_denoise_matches_oracle(renderer, stride_mode, passes, denoise_math):
+    """Prepass, a-trous passes and composite against the oracle on a frame
+    whose size is no multiple of the tiles; 7 passes reach strides of 64,
+    wider than the frame's sub-lattice tiles.  EXACT evaluates each weight as
+    the oracle does; FUSED is held to the same 1e-5 relative."""
     scene = SceneBuilder.config(1, seed=71)
     w, h = 72, 56
     _setup(renderer, scene, w, h, gbuf=True)
@@ -333,13 +345,50 @@ def test_denoise_matches_oracle(renderer, stride_mode):
     renderer.write_accum(acc)
     gb = renderer.read_gbuffer()
     dn = Denoiser(renderer)
+    dn.passes = passes
     out8 = dn.execute(stride_mode)
     res = dn.read_result()
     u = dn.uniforms()
     rres, r8 = ref_denoise(u, acc, gb, stride_mode)
+    err = np.abs(res - rres) / np.maximum(np.abs(rres), 1e-3)
+    print("denoise max rel err", float(err.max()))
     assert np.all(np.isfinite(res))
     assert np.allclose(res, rres, rtol=1e-5, atol=1e-6)
     assert np.mean(np.abs(out8.astype(int) - r8.astype(int)) <= 1) >= 0.999
+
+
+def test_denoise_1080p_frame_matches_oracle(renderer):
+    """The configuration-5 frame: 1920x1080, a 4-spp render of a seeded
+    small-analytic scene with its G-buffer, the reference's default sigmas
+    and 5 a-trous passes (strides 1..16), against the oracle over the whole
+    frame; the fused weights within 5e-5 of the exact ones at worst (their
+    mean difference is ~1e-7)."""
+    scene = SceneBuilder.config(1, seed=5)
+    w, h = 1920, 1080
+    _setup(renderer, scene, w, h, gbuf=True)
+    p = scene.params(w, h, 0, 1)
+    renderer.gbuffer(p)
+    renderer.render_section(p, passes=4, seeds=[11, 12, 13, 14])
+    acc, gb = renderer.read_accum(), renderer.read_gbuffer()
+    dn = Denoiser(renderer)
+    dn.passes = 5
+    out = {}
+    try:
+        for m in (A.PT_DENOISE_MATH_EXACT, A.PT_DENOISE_MATH_FUSED):
+            renderer.set_option(A.PT_OPT_DENOISE_MATH, m)
+            out8 = dn.execute(0)
+            out[m] = (dn.read_result(), out8)
+    finally:
+        renderer.set_option(A.PT_OPT_DENOISE_MATH, A.PT_DENOISE_MATH_FUSED)
+    rres, r8 = ref_denoise(dn.uniforms(), acc, gb, 0)
+    for m, (res, out8) in out.items():
+        err = np.abs(res - rres) / np.maximum(np.abs(rres), 1e-3)
+        print("1080p denoise math", m, "max rel err", float(err.max()), "mean", float(err.mean()))
+        assert np.allclose(res, rres, rtol=2e-5, atol=2e-6)
+        assert np.mean(np.abs(out8.astype(int) - r8.astype(int)) <= 1) >= 0.9999
+    d = np.abs(out[0][0] - out[1][0]) / np.maximum(np.abs(out[1][0]), 1e-3)
+    print("fused vs exact max rel", float(d.max()), "mean", float(d.mean()))
+    assert d.max() <= 5e-5 and d.mean() <= 1e-6
 
 
 @pytest.mark.parametrize("name", ["c1_64x48", "mesh_48x32"])

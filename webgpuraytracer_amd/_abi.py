@@ -131,7 +131,8 @@ ERR_NAMES = {0: "PT_OK", -1: "PT_ERR_ARG", -2: "PT_ERR_LAYOUT", -3: "PT_ERR_CAPA
 
 PT_OPT_BACKEND, PT_OPT_COUNT_STATS, PT_OPT_WAVEFRONT_POOL, PT_OPT_SCHEDULE, PT_OPT_TRAVERSAL, PT_OPT_LDS_STACK, \
     PT_OPT_WAVEFRONT_REFILL, PT_OPT_BLAS_BUILD, PT_OPT_TLAS_BUILD = 1, 2, 3, 4, 5, 6, 7, 8, 9
-PT_OPT_KERNEL_FLAGS, PT_OPT_SPECIALISED = 10, 11
+PT_OPT_KERNEL_FLAGS, PT_OPT_SPECIALISED, PT_OPT_DENOISE_MATH = 10, 11, 12
+PT_DENOISE_MATH_FUSED, PT_DENOISE_MATH_EXACT = 0, 1
 KERNEL_TEXTURES_BIT, KERNEL_TRANSLUCENT_BIT = 1 << 20, 1 << 21
 PT_BLAS_BUILD_REFERENCE, PT_BLAS_BUILD_DEVICE = 0, 1
 PT_TLAS_BUILD_REFERENCE, PT_TLAS_BUILD_DEVICE = 0, 1

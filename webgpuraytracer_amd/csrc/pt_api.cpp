@@ -476,6 +476,13 @@ int pt_set_option(pt_ctx* c, int key, int64_t value) {
                 return c->fail(PT_ERR_ARG, "unknown schedule %lld", (long long)value);
             c->schedule = int(value);
             return PT_OK;
+        case PT_OPT_DENOISE_MATH:
+            if (value != PT_DENOISE_MATH_FUSED && value != PT_DENOISE_MATH_EXACT)
+                return c->fail(PT_ERR_ARG, "unknown denoise math %lld", (long long)value);
+            c->denoise_math = int(value);
+            return PT_OK;
+        case PT_OPT_KERNEL_FLAGS:
+        case PT_OPT_SPECIALISED: return c->fail(PT_ERR_ARG, "option %d is read-only", key);
         default: return c->fail(PT_ERR_ARG, "unknown option %d", key);
     }
 }
@@ -493,6 +500,7 @@ int pt_get_option(pt_ctx* c, int key, int64_t* value) {
         case PT_OPT_WAVEFRONT_REFILL: *value = c->wf_refill; return PT_OK;
         case PT_OPT_BLAS_BUILD: *value = c->blas_build; return PT_OK;
         case PT_OPT_TLAS_BUILD: *value = c->tlas_build; return PT_OK;
+        case PT_OPT_DENOISE_MATH: *value = c->denoise_math; return PT_OK;
         case PT_OPT_KERNEL_FLAGS:
         case PT_OPT_SPECIALISED:
             if (!c->has_scene) return c->fail(PT_ERR_STATE, "no scene uploaded");
@@ -899,7 +907,10 @@ int pt_denoise(pt_ctx* c, const pt_denoise_params* u, int stride_mode) {
     a.u = *u;
     a.accum = static_cast<const float4*>(c->accum.p);
     a.gbuf = static_cast<const pt_gbuffer_texel*>(c->gbuf.p);
-    a.guide = static_cast<float4*>(c->dn_guide.p);
+    a.gn = static_cast<float4*>(c->dn_guide.p);
+    a.ga = a.gn + px;
+    a.gm = a.ga + px;
+    a.exact = c->denoise_math == PT_DENOISE_MATH_EXACT ? 1 : 0;
     a.a = static_cast<float4*>(c->dn_a.p);
     a.b = static_cast<float4*>(c->dn_b.p);
     a.fin = static_cast<float4*>(c->dn_fin.p);

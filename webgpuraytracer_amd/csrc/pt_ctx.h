@@ -42,6 +42,7 @@ struct pt_ctx {
     int schedule = PT_SCHEDULE_REGEN;
     int traversal = PT_TRAVERSAL_WIDE;
     bool count_stats = true;
+    int denoise_math = PT_DENOISE_MATH_FUSED;
     // scene
     bool has_scene = false;
     uint32_t flags = 0;

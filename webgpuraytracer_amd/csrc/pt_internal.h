@@ -48,11 +48,14 @@ struct DenoiseArgs {
     pt_denoise_params u;
     const float4* accum;
     const pt_gbuffer_texel* gbuf;
-    float4* guide;               // 3 float4 per pixel (see dn_guide)
+    float4* gn;                  // guide planes, one float4 per pixel each (see dn_guide):
+    float4* ga;                  //   (normal, |normal| > 0.1), (tonemap(albedo), dist/(dist+3)),
+    float4* gm;                  //   (transmission, roughness, metallic, 0)
     float4* a;
     float4* b;
     float4* fin;                 // final fp32 result
     uchar4* out;
+    int exact;                   // 1: the a-trous weights with the library exp/pow, one per factor
 };
 
 // Wavefront batch (pt_wavefront.hip): device state of `pool` path slots;

@@ -367,9 +367,9 @@ __device__ __forceinline__ v3 dn_tonemap(v3 c, float e) {
 }
 
 // Per-pixel guide values the a-trous taps compare, computed once instead of
-// once per tap: guide[3i] = (normal, |normal| > 0.1), guide[3i+1] =
-// (tonemap(albedo), dist/(dist+3)), guide[3i+2] = (transmission, roughness,
-// metallic, 0).
+// once per tap, in three structure-of-arrays planes: gn = (normal,
+// |normal| > 0.1), ga = (tonemap(albedo), dist/(dist+3)), gm = (transmission,
+// roughness, metallic, 0).
 __global__ void dn_guide(DenoiseArgs A) {
     const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
     if (x >= A.u.width || y >= A.u.height) return;
@@ -377,9 +377,9 @@ __global__ void dn_guide(DenoiseArgs A) {
     const pt_gbuffer_texel g = A.gbuf[i];
     const v3 n = ld3(g.normal);
     const v3 ta = dn_tonemap(ld3(g.albedo), A.u.exposure);
-    A.guide[3 * i] = make_float4(n.x, n.y, n.z, len(n) > 0.1f ? 1.0f : 0.0f);
-    A.guide[3 * i + 1] = make_float4(ta.x, ta.y, ta.z, g.dist / (g.dist + 3.0f));
-    A.guide[3 * i + 2] = make_float4(g.transmission, g.roughness, g.metallic, 0.0f);
+    A.gn[i] = make_float4(n.x, n.y, n.z, len(n) > 0.1f ? 1.0f : 0.0f);
+    A.ga[i] = make_float4(ta.x, ta.y, ta.z, g.dist / (g.dist + 3.0f));
+    A.gm[i] = make_float4(g.transmission, g.roughness, g.metallic, 0.0f);
 }
 
 // prepass_main, denoiser.wgsl:55-112
@@ -418,7 +418,7 @@ __global__ void dn_prepass(DenoiseArgs A) {
                        cc.z <= 10000.0f;
     if (!valid) { cc.x = 0.0f; cc.y = 0.0f; cc.z = 0.0f; }
     v3 mc = dn_tonemap(mk(cc.x, cc.y, cc.z), A.u.exposure);
-    const float4 g0 = A.guide[3 * i], g1 = A.guide[3 * i + 1];
+    const float4 g0 = A.gn[i], g1 = A.ga[i];
     const bool sky = len(mk(g0.x, g0.y, g0.z)) < 0.1f;
     if ((A.u.flags & 2u) && !sky) {
         const v3 ma = vmax(mk(g1.x, g1.y, g1.z), splat(0.001f));
@@ -428,14 +428,50 @@ __global__ void dn_prepass(DenoiseArgs A) {
     A.a[i] = make_float4(mc.x, mc.y, mc.z, cc.w);
 }
 
-// atrous_main, denoiser.wgsl:117-189
-__global__ void dn_atrous(DenoiseArgs A, const float4* __restrict__ in, float4* __restrict__ outb, int step) {
+// atrous_main, denoiser.wgsl:117-189.
+//
+// A pass with stride s visits the 5x5 taps (x + dx*s, y + dy*s): on the
+// sub-lattice of pixels congruent to (x, y) modulo s they are a dense 5x5
+// stencil.  So each workgroup takes a 32x8 tile of one such sub-lattice (one
+// phase (x mod s, y mod s)), stages it with a 2-point halo (36x12 points,
+// clamped to the image edge as the taps are) in LDS — colour and the three
+// guide planes, 27.6 KB — and every tap reads LDS.  The phases of one image
+// region are consecutive workgroups of one XCD, so the strided global loads
+// of the tile fill share that XCD's L2.
+//
+// EXACT: the reference's weights factor by factor, with the library exp and
+// pow (one correctly rounded division per Gaussian).  Otherwise the four
+// Gaussians are summed into one base-2 exponent with precomputed reciprocals
+// and the normal weight pow(d, sigma_n) joins it as sigma_n * log2(d), so a
+// tap costs one hardware exp2 and one log2: within a few ulp of each factor,
+// inside what WGSL allows its exp and pow.
+constexpr int kDnTx = 32, kDnTy = 8, kDnHx = kDnTx + 4, kDnHy = kDnTy + 4, kDnPts = kDnHx * kDnHy;
+
+template <bool EXACT>
+__global__ void __launch_bounds__(256) dn_atrous(DenoiseArgs A, const float4* __restrict__ in,
+                                                 float4* __restrict__ outb, int step, uint32_t tiles_x, uint32_t phx,
+                                                 uint32_t nph) {
+    __shared__ float4 s_col[kDnPts], s_n[kDnPts], s_a[kDnPts], s_m[kDnPts];
     const int W = int(A.u.width), H = int(A.u.height);
-    const int x = int(blockIdx.x * 16 + threadIdx.x), y = int(blockIdx.y * 16 + threadIdx.y);
+    const uint32_t L = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t phase = L % nph, tile = L / nph;
+    const int ph_x = int(phase % phx), ph_y = int(phase / phx);
+    const int lx0 = int(tile % tiles_x) * kDnTx - 2, ly0 = int(tile / tiles_x) * kDnTy - 2;
+    for (int k = int(threadIdx.x); k < kDnPts; k += 256) {
+        const int px = min(max(ph_x + step * (lx0 + k % kDnHx), 0), W - 1);
+        const int py = min(max(ph_y + step * (ly0 + k / kDnHx), 0), H - 1);
+        const uint32_t si = uint32_t(py * W + px);
+        s_col[k] = in[si];
+        s_n[k] = A.gn[si];
+        s_a[k] = A.ga[si];
+        s_m[k] = A.gm[si];
+    }
+    __syncthreads();
+    const int li = int(threadIdx.x) % kDnTx, lj = int(threadIdx.x) / kDnTx;
+    const int x = ph_x + step * (lx0 + 2 + li), y = ph_y + step * (ly0 + 2 + lj);
     if (x >= W || y >= H) return;
-    const uint32_t ci = uint32_t(y * W + x);
-    const float4 cc = in[ci];
-    const float4 cg0 = A.guide[3 * ci], cg1 = A.guide[3 * ci + 1], cg2 = A.guide[3 * ci + 2];
+    const int c = (lj + 2) * kDnHx + li + 2;
+    const float4 cc = s_col[c], cg0 = s_n[c], cg1 = s_a[c], cg2 = s_m[c];
     const v3 cn = mk(cg0.x, cg0.y, cg0.z);
     const float kern[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
     float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -446,39 +482,61 @@ __global__ void dn_atrous(DenoiseArgs A, const float4* __restrict__ in, float4* 
     const float sd2 = A.u.sigma_depth * A.u.sigma_depth + 1e-5f;
     const float sa2 = A.u.sigma_albedo * A.u.sigma_albedo + 1e-5f;
     const float sm2 = A.u.sigma_material * A.u.sigma_material + 1e-5f;
-    // one row of taps per iteration: unrolling all 25 taps hoists their loads
-    // past the register file (scratch spills)
-#pragma unroll 1
+    constexpr float kLog2e = 1.4426950408889634f;
+    const float ic = kLog2e / sc2, id = kLog2e / sd2, ia = kLog2e / sa2, im = kLog2e / sm2;
+    const float sn = A.u.sigma_normal;
+    // EXACT: one row of taps per iteration (unrolled, the library exp/pow of
+    // 25 taps do not fit the register file)
+    constexpr int kRowUnroll = EXACT ? 1 : 5;
+#pragma unroll kRowUnroll
     for (int dy = -2; dy <= 2; ++dy) {
-        const int syp = min(max(y + dy * step, 0), H - 1);
 #pragma unroll
         for (int dx = -2; dx <= 2; ++dx) {
-            const int sxp = min(max(x + dx * step, 0), W - 1);
-            const uint32_t si = uint32_t(syp * W + sxp);
-            const float4 s = in[si];
-            const float4 g0 = A.guide[3 * si], g1 = A.guide[3 * si + 1], g2 = A.guide[3 * si + 2];
+            const int t = c + dy * kDnHx + dx;
+            const float4 s = s_col[t], g0 = s_n[t], g1 = s_a[t], g2 = s_m[t];
             const float hk = kern[dx + 2] * kern[dy + 2];
             const v3 cd = mk(cc.x, cc.y, cc.z) - mk(s.x, s.y, s.z);
-            const float wc = expf(-dot(cd, cd) / sc2);
-            float wn = 1.0f;
-            if (cg0.w != 0.0f || g0.w != 0.0f) wn = powf(mx(0.0f, dot(cn, mk(g0.x, g0.y, g0.z))), A.u.sigma_normal);
             const float dd = cg1.w - g1.w;
-            const float wd = expf(-(dd * dd) / sd2);
             const v3 ad = mk(cg1.x, cg1.y, cg1.z) - mk(g1.x, g1.y, g1.z);
-            const float wa = expf(-dot(ad, ad) / sa2);
             const v3 md_ = mk(cg2.x, cg2.y, cg2.z) - mk(g2.x, g2.y, g2.z);
-            const float wm = expf(-dot(md_, md_) / sm2);
-            float w = hk * wc * wn * wd * wa * wm;
+            const bool normals = cg0.w != 0.0f || g0.w != 0.0f;
+            float w;
+            if (EXACT) {
+                const float wc = expf(-dot(cd, cd) / sc2);
+                float wn = 1.0f;
+                if (normals) wn = powf(mx(0.0f, dot(cn, mk(g0.x, g0.y, g0.z))), sn);
+                const float wd = expf(-(dd * dd) / sd2);
+                const float wa = expf(-dot(ad, ad) / sa2);
+                const float wm = expf(-dot(md_, md_) / sm2);
+                w = hk * wc * wn * wd * wa * wm;
+            } else {
+                float e = fmaf(cd.x, cd.x, fmaf(cd.y, cd.y, cd.z * cd.z)) * ic;
+                e = fmaf(dd * dd, id, e);
+                e = fmaf(fmaf(ad.x, ad.x, fmaf(ad.y, ad.y, ad.z * ad.z)), ia, e);
+                e = fmaf(fmaf(md_.x, md_.x, fmaf(md_.y, md_.y, md_.z * md_.z)), im, e);
+                if (normals && sn != 0.0f) {
+                    const float nd = mx(0.0f, fmaf(cn.x, g0.x, fmaf(cn.y, g0.y, cn.z * g0.z)));
+                    e = fmaf(-sn, __builtin_amdgcn_logf(nd), e);  // pow(nd, sn) = 2^(sn log2 nd)
+                }
+                w = hk * __builtin_amdgcn_exp2f(-e);
+            }
             if (dx != 0 || dy != 0) w *= att;
-            sum.x = sum.x + s.x * w;
-            sum.y = sum.y + s.y * w;
-            sum.z = sum.z + s.z * w;
-            sum.w = sum.w + s.w * w;
+            if (EXACT) {
+                sum.x = sum.x + s.x * w;
+                sum.y = sum.y + s.y * w;
+                sum.z = sum.z + s.z * w;
+                sum.w = sum.w + s.w * w;
+            } else {
+                sum.x = fmaf(s.x, w, sum.x);
+                sum.y = fmaf(s.y, w, sum.y);
+                sum.z = fmaf(s.z, w, sum.z);
+                sum.w = fmaf(s.w, w, sum.w);
+            }
             sw += w;
         }
     }
     const float den = sw + 1e-5f;
-    outb[ci] = make_float4(sum.x / den, sum.y / den, sum.z / den, sum.w / den);
+    outb[uint32_t(y * W + x)] = make_float4(sum.x / den, sum.y / den, sum.z / den, sum.w / den);
 }
 
 // composite_main, denoiser.wgsl:194-214
@@ -487,7 +545,7 @@ __global__ void dn_composite(DenoiseArgs A, const float4* __restrict__ in) {
     if (x >= A.u.width || y >= A.u.height) return;
     const uint32_t i = y * A.u.width + x;
     float4 f = in[i];
-    const float4 g0 = A.guide[3 * i], g1 = A.guide[3 * i + 1];
+    const float4 g0 = A.gn[i], g1 = A.ga[i];
     const bool sky = len(mk(g0.x, g0.y, g0.z)) < 0.1f;
     if ((A.u.flags & 2u) && !sky) {
         const v3 ma = vmax(mk(g1.x, g1.y, g1.z), splat(0.001f));
@@ -652,11 +710,23 @@ hipError_t launch_denoise(const DenoiseArgs& a, int stride_mode, hipStream_t s) 
     const dim3 blk(16, 16), grd((a.u.width + 15) / 16, (a.u.height + 15) / 16);
     hipLaunchKernelGGL(dn_guide, grd, blk, 0, s, a);
     hipLaunchKernelGGL(dn_prepass, grd, blk, 0, s, a);
+    const uint32_t W = a.u.width, H = a.u.height;
     for (uint32_t i = 0; i < a.u.passes; ++i) {
-        const int step = stride_mode == 0 ? int(1u << i) : int(1u << (a.u.passes - 1));
+        const uint32_t step = stride_mode == 0 ? 1u << i : 1u << (a.u.passes - 1);
         const float4* in = (i % 2 == 0) ? a.a : a.b;
         float4* outb = (i % 2 == 0) ? a.b : a.a;
-        hipLaunchKernelGGL(dn_atrous, grd, blk, 0, s, a, in, outb, step);
+        // sub-lattice tiles: phases present in the image x tiles of the largest sub-lattice
+        const uint32_t phx = std::min(step, W), phy = std::min(step, H);
+        const uint32_t tiles_x = ((W + step - 1) / step + kDnTx - 1) / kDnTx;
+        const uint32_t tiles_y = ((H + step - 1) / step + kDnTy - 1) / kDnTy;
+        const uint64_t blocks = uint64_t(tiles_x) * tiles_y * phx * phy;
+        if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+        if (a.exact)
+            hipLaunchKernelGGL(dn_atrous<true>, dim3(uint32_t(blocks)), dim3(256), 0, s, a, in, outb, int(step),
+                               tiles_x, phx, phx * phy);
+        else
+            hipLaunchKernelGGL(dn_atrous<false>, dim3(uint32_t(blocks)), dim3(256), 0, s, a, in, outb, int(step),
+                               tiles_x, phx, phx * phy);
     }
     hipLaunchKernelGGL(dn_composite, grd, blk, 0, s, a, (a.u.passes % 2 == 0) ? a.a : a.b);
     return hipGetLastError();
