@@ -209,6 +209,20 @@ int pt_trace(pt_ctx* ctx, const pt_scene_params* params, uint32_t section_w,
              uint32_t section_h, uint32_t n_passes, const uint32_t* seeds,
              pt_stats* stats);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): pt_trace with the same arguments and results, calling no
+ * hipMalloc, hipFree or allocating routine.  It always runs the persistent
+ * megakernel (bit-identical to either backend) with its scratch — work
+ * counters, tile counters, the seed list, per-lane medium stacks and
+ * traversal-stack spill — in a static 64 MiB device array per GPU, held until
+ * the process ends; the persistent grid shrinks to what the array holds.  The
+ * scene and frame must already be uploaded and sized (pt_upload_scene,
+ * pt_resize).  At most 65536 passes per call (PT_ERR_CAPACITY beyond).  Calls
+ * on one GPU share the array: calls that overlap on one GPU race. */
+int pt_trace_noalloc(pt_ctx* ctx, const pt_scene_params* params, uint32_t section_w, uint32_t section_h,
+                     uint32_t n_passes, const uint32_t* seeds, pt_stats* stats);
+
 /* gbuf_main (shader.wgsl:2510-2603) over the whole frame: fills the G-buffer
  * and writes the four debug strips to the rgba8 output. */
 int pt_gbuffer(pt_ctx* ctx, const pt_scene_params* params);

@@ -164,6 +164,7 @@ PT_FUNCS = {
     "pt_trace": (C.c_int, [vp, P(SceneParams), u32, u32, u32, P(u32), P(Stats)]),
     "pt_trace_hits": (C.c_int, [vp, P(SceneParams), u32, P(Hit)]),
     "pt_get_option": (C.c_int, [vp, C.c_int, P(C.c_int64)]),
+    "pt_trace_noalloc": (C.c_int, [vp, P(SceneParams), u32, u32, u32, P(u32), P(Stats)]),
     "pt_trace_hits_rays": (C.c_int, [vp, P(SceneParams), u32, P(Hit), vp]),
     "pt_gbuffer": (C.c_int, [vp, P(SceneParams)]),
     "pt_denoise": (C.c_int, [vp, P(DenoiseParams), C.c_int]),

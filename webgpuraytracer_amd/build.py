@@ -64,7 +64,7 @@ def build_pt():
     out = os.path.join(LIB, "libpt.so")
     with tempfile.TemporaryDirectory(prefix="ptbuild") as td:
         srcs = ("pt_kernels.hip", "pt_wavefront.hip", "nn_kernels.hip", "pt_build.hip", "pt_api.cpp", "pt_layout.cpp",
-                "pt_unet.cpp", "pt_batch.cpp")
+                "pt_unet.cpp", "pt_batch.cpp", "pt_noalloc.cpp")
         objs = [os.path.join(td, src.split(".")[0] + ".o") for src in srcs]
         with concurrent.futures.ThreadPoolExecutor(max_workers=len(srcs)) as ex:
             jobs = [ex.submit(_run, [hipcc] + HIP_FLAGS + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", o])

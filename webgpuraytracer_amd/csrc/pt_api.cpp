@@ -414,7 +414,8 @@ int pt_create(pt_ctx** out, int device) {
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
-    if (e != hipSuccess || c->n_cu <= 0) { delete c; return PT_ERR_HIP; }
+    if (e != hipSuccess || c->n_cu <= 0) { pt_destroy(c); return PT_ERR_HIP; }
+    if (pt::noalloc_setup(c) != PT_OK) { pt_destroy(c); return PT_ERR_HIP; }
     *out = c;
     return PT_OK;
 }

@@ -148,6 +148,7 @@ int prepare_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t sh,
                   const uint32_t* seeds, pt_hit* hits_dev, uint32_t n_rec, TracePlan& plan);
 hipError_t enqueue_trace(pt_ctx* c, TracePlan& plan, hipStream_t s, bool timed);
 int read_stats(pt_ctx* c, pt_stats* stats, float ms, uint32_t launches);
+int noalloc_setup(pt_ctx* c);  // pt_noalloc.cpp: the device address of this device's scratch array
 
 }  // namespace pt
 
