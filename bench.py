@@ -40,6 +40,18 @@ def algorithmic_bytes(st, sky_w, sky_h):
             shade * st.shade_events + 36 * st.paths)
 
 
+def cpu_model():
+    """The host CPU's model name (lscpu's "Model name", read from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.lower().startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(scene, params, budget_s):
     """The oracle (a CPU restatement of shader.wgsl `main`) on the same scene,
     1 spp on every 8th row, repeated until budget_s of work; all host threads."""
@@ -58,7 +70,7 @@ def cpu_baseline(scene, params, budget_s):
         if time.perf_counter() - t0 >= budget_s:
             break
     dt = time.perf_counter() - t0
-    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
             "sample": "oracle FP32 restatement, same scene and camera, 1 spp x %d passes over every 8th row "
                       "of 1920x1080 (%d rays in %.1f s)" % (passes, rays, dt)}
 
