@@ -72,8 +72,10 @@ int pt_unet_denoise(pt_ctx* ctx, float exposure);
  * as pt_unet_denoise reads it) and from then on runs the 17 inner
  * convolutions on fp8 products (OCP e4m3, fp32 accumulation; the first and
  * the output convolution stay bf16): each layer input is quantized
- * per channel as x / s_a[ci], s_a = the calibration frame's max |x| / 224
- * (one binade of headroom; larger values saturate at 448); the weights are
+ * per channel as x / s_a[ci], s_a = a / 224 where a is the calibration
+ * frame's 99.995th percentile of |x| (two bf16 forward passes: the maxima,
+ * then 1024-bin histograms up to them); one binade of headroom, larger values
+ * saturate at 448; the weights are
  * stored in e4m3 as w * s_a[ci] / s_w[co], s_w = that row's max / 448.
  * Activations stay bf16 between layers.  Training (pt_unet_train,
  * pt_unet_loss_grad) always runs in bf16; pt_unet_init, pt_unet_set_params

@@ -185,3 +185,91 @@ def test_node_binding_textured_scene(tmp_path):
     assert np.array_equal(np.frombuffer((tmp_path / "js_accum.bin").read_bytes(), np.float32), acc.ravel())
     assert float(acc[..., :3].mean()) > 0.0
     r.close()
+
+
+JS_ASYNC = r"""
+const fs = require('fs');
+const t = require(process.argv[1]);
+const dir = process.argv[2];
+const cfg = JSON.parse(fs.readFileSync(dir + '/cfg.json'));
+const rd = (name, T) => { const b = fs.readFileSync(dir + '/' + name + '.bin'); return new T(b.buffer, b.byteOffset, b.byteLength / T.BYTES_PER_ELEMENT); };
+const sceneData = { mat: rd('materials', Uint8Array), object: rd('objects', Uint8Array), tlas: rd('tlas', Uint8Array),
+  mesh: rd('meshes', Uint8Array), bvh: rd('bvh', Uint8Array), triangle: rd('triangles', Uint8Array),
+  plane: rd('planes', Uint8Array), light: rd('lights', Uint8Array), flags: cfg.flags };
+const sky = { data: rd('sky', Uint16Array), width: cfg.skyWidth, height: cfg.skyHeight, exposure: cfg.skyExposure };
+const cdf = { skyCDF: rd('cdf', Float32Array), totalWeight: cfg.skyTotalLum };
+const same = (a, b) => Buffer.compare(Buffer.from(a.buffer, a.byteOffset, a.byteLength),
+                                      Buffer.from(b.buffer, b.byteOffset, b.byteLength)) === 0;
+(async () => {
+  const res = {};
+  const r = new t.NativeRenderer(0);
+  r.setScene(sceneData, sky, cdf, cfg.bounces, true);
+  r.resize(cfg.width, cfg.height);
+  r.gbuffer(cfg.camera);
+  const s1 = r.render(cfg.camera, 64, 3);
+  const accSync = r.readAccum();
+  const dnSync = new t.NativeDenoiser(r).execute(0);
+  const ln = new t.NativeLearnedDenoiser(r);
+  const unSync = ln.execute();
+  r.resize(cfg.width, cfg.height);
+  r.gbuffer(cfg.camera);
+  // the event loop runs while the GPU renders
+  let ticks = 0;
+  const timer = setInterval(() => { ticks++; }, 1);
+  const pr = r.renderAsync(cfg.camera, 64, 3);
+  try { r.readAccum(); res.busy = false; } catch (e) { res.busy = /busy/.test(e.message); }
+  const s2 = await pr;
+  clearInterval(timer);
+  res.ticks = ticks;
+  res.traceSame = same(accSync, r.readAccum()) && s1.extensionRays === s2.extensionRays && s1.paths === s2.paths;
+  res.denoiseSame = same(dnSync, await new t.NativeDenoiser(r).executeAsync(0));
+  res.unetSame = same(unSync, await ln.executeAsync());
+  // a batch frame, synchronously and asynchronously, on fresh renderers
+  const mk = (w, h) => { const x = new t.NativeRenderer(0); x.setScene(sceneData, sky, cdf, cfg.bounces); x.resize(w, h); return x; };
+  const a2 = mk(40, 24), a3 = mk(24, 40), b2 = mk(40, 24), b3 = mk(24, 40);
+  const bs = new t.NativeBatch(0); bs.add(a2); bs.add(a3);
+  const ba = new t.NativeBatch(0); ba.add(b2); ba.add(b3);
+  const stS = bs.frame([cfg.camera2, cfg.camera3], [2, 3], 0);
+  const prB = ba.frameAsync([cfg.camera2, cfg.camera3], [2, 3], 0);
+  try { b2.readAccum(); res.batchBusy = false; } catch (e) { res.batchBusy = /busy/.test(e.message); }
+  const stA = await prB;
+  res.batchSame = same(a2.readAccum(), b2.readAccum()) && same(a3.readAccum(), b3.readAccum()) &&
+                  stS[1].extensionRays === stA[1].extensionRays;
+  // a failing call rejects its Promise with the error message, and frees the context
+  const plain = mk(16, 16);  // no G-buffer: pt_denoise refuses
+  try { await new t.NativeDenoiser(plain).executeAsync(0); res.rejected = false; }
+  catch (e) { res.rejected = /pt_denoise failed/.test(e.message); }
+  try { plain.readAccum(); res.freed = true; } catch (e) { res.freed = false; }
+  console.log(JSON.stringify(res));
+})().catch((e) => { console.error(e); process.exit(1); });
+"""
+
+
+def test_node_async_calls(tmp_path):
+    """traceAsync / denoiseAsync / unetDenoiseAsync / batchFrameAsync: the
+    Promise resolves with the synchronous call's bits while the JavaScript
+    event loop keeps running; the context is busy (calls throw) until it
+    settles; a failing call rejects with the C ABI's error."""
+    if not shutil.which("node") or not os.path.exists(ADDON):
+        pytest.skip("node or the addon is not available")
+    scene = SceneBuilder.config(1, seed=23)
+    d = scene.desc()
+    for name in ("materials", "objects", "tlas", "meshes", "bvh", "triangles", "planes", "lights"):
+        a = getattr(d, name)
+        (tmp_path / (name + ".bin")).write_bytes(C.string_at(a.ptr, a.count * a.stride) if a.count else b"")
+    (tmp_path / "sky.bin").write_bytes(C.string_at(d.sky_rgba16f, d.sky_width * d.sky_height * 8))
+    (tmp_path / "cdf.bin").write_bytes(C.string_at(d.sky_cdf, d.sky_cdf_count * 4))
+    w, h = 512, 384
+    cfg = {"flags": d.flags, "bounces": d.bounce_limit, "skyWidth": d.sky_width, "skyHeight": d.sky_height,
+           "skyExposure": d.sky_exposure, "skyTotalLum": d.sky_total_lum, "width": w, "height": h,
+           "camera": _camera(scene.params(w, h, 0, 0)), "camera2": _camera(scene.params(40, 24, 0, 0)),
+           "camera3": _camera(scene.params(24, 40, 0, 0))}
+    (tmp_path / "cfg.json").write_text(json.dumps(cfg))
+    js = subprocess.run(["node", "-e", JS_ASYNC, SHIM, str(tmp_path)], stdout=subprocess.PIPE,
+                        stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert js.returncode == 0, js.stdout
+    res = json.loads(js.stdout.strip().splitlines()[-1])
+    print(res)
+    assert res == {"busy": True, "ticks": res["ticks"], "traceSame": True, "denoiseSame": True, "unetSame": True,
+                   "batchBusy": True, "batchSame": True, "rejected": True, "freed": True}
+    assert res["ticks"] >= 2  # the event loop ran during a ~20 ms render

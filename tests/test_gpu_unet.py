@@ -288,3 +288,38 @@ def test_fp8_forward_matches_fp8_restatement(renderer):
     print("fp8 vs restatement %.2e, fp8 vs bf16 %.2e (other-frame calibration %.2e)" % (e_emu, e_16, e_cal))
     assert e_emu < 0.5 * e_16 and e_emu < 0.025  # measured: 1.5e-2 against 3.7e-2
     assert e_16 < 0.05 and e_cal < 0.05
+
+
+def test_c5_1080p_unet_one_spp_beats_atrous_four_spp(renderer):
+    """The equal-time comparison's frame: 1920x1080, a seeded small-analytic
+    scene outside the training seeds, a 4096-spp reference.  The learned
+    denoiser on a 1-spp frame is closer to the reference than 5-pass a-trous
+    on a 4-spp frame (relMSE of the tonemapped images), and its fp8 variant
+    stays within 15% of bf16."""
+    from webgpuraytracer_amd import Denoiser
+    from webgpuraytracer_amd.unet_train import rel_mse, tonemap
+    scene = SceneBuilder.config(1, seed=7)
+    renderer.set_scene(scene, isgbuf=True)
+    renderer.resize(1920, 1080)
+    renderer.gbuffer()
+    renderer.reset()
+    renderer.render(spp=4096, base_seed=123)
+    ref = tonemap(renderer.read_accum()[..., :3])
+    dn = Denoiser(renderer)
+    dn.passes = 5
+    renderer.reset()
+    renderer.render(spp=4, base_seed=5)
+    dn.execute(0)
+    e_at4 = rel_mse(dn.read_result()[..., :3], ref)
+    renderer.reset()
+    renderer.render(spp=1, base_seed=5)
+    net = LearnedDenoiser(renderer, pretrained=True)
+    net.execute()
+    e_un = rel_mse(tonemap(net.read_result()[..., :3]), ref)
+    net.set_precision(True)
+    net.execute()
+    e_8 = rel_mse(tonemap(net.read_result()[..., :3]), ref)
+    net.set_precision(False)
+    print("1080p relMSE vs 4096 spp: U-Net 1 spp %.5f (fp8 %.5f), a-trous 4 spp %.5f" % (e_un, e_8, e_at4))
+    assert e_un < e_at4
+    assert e_8 < 1.15 * e_un

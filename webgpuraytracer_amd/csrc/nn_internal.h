@@ -49,6 +49,10 @@ hipError_t conv3x3_fp8(const Tens& in, uint32_t cin, const uint8_t* w, const flo
 // amax[ch] = max(amax[ch], max over P pixels of |x[p][ch]|) for ch < c
 // (c a multiple of 8, at most 256), the maxima as float bit patterns.
 hipError_t chan_amax(const Tens& x, uint32_t c, uint32_t P, uint32_t* amax, hipStream_t s);
+// Per-channel histograms of |x| in kHistBins bins over [0, amax[ch]] (amax as
+// float bits, from chan_amax), added to hist[ch * kHistBins + bin].
+constexpr uint32_t kHistBins = 1024;
+hipError_t chan_hist(const Tens& x, uint32_t c, uint32_t P, const uint32_t* amax, uint32_t* hist, hipStream_t s);
 
 // Backward staging for a layer with c output channels and output gradient
 // ga: dz = ga * (a > 0) when a is given (ReLU output), else ga; written NHWC

@@ -543,6 +543,43 @@ hipError_t chan_amax(const Tens& x, uint32_t c, uint32_t P, uint32_t* amax, hipS
     return hipGetLastError();
 }
 
+// Per-channel histograms of |x| over [0, amax[ch]] in kHistBins equal bins
+// (values at amax land in the last bin): one workgroup per 8-channel group
+// and pixel chunk, privatised in LDS, then added to hist[ch][bin].
+__global__ void __launch_bounds__(kBlock) chan_hist_k(Tens x, uint32_t P, uint32_t chunk,
+                                                      const uint32_t* __restrict__ amax, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t sm[8 * kHistBins];
+    for (uint32_t i = threadIdx.x; i < 8u * kHistBins; i += kBlock) sm[i] = 0u;
+    __syncthreads();
+    const uint32_t g = blockIdx.y;
+    float scale[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const float a = __uint_as_float(amax[g * 8u + uint32_t(j)]);
+        scale[j] = a > 0.0f ? float(kHistBins) / a : 0.0f;
+    }
+    const uint32_t p0 = blockIdx.x * chunk, p1 = min(P, p0 + chunk);
+    for (uint32_t p = p0 + threadIdx.x; p < p1; p += kBlock) {
+        const bf16x8 v = load8(x.p + size_t(p) * x.stride + x.off + g * 8u);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t b = min(kHistBins - 1u, uint32_t(fabsf(f(v[j])) * scale[j]));
+            atomicAdd(&sm[uint32_t(j) * kHistBins + b], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 8u * kHistBins; i += kBlock)
+        if (sm[i]) atomicAdd(&hist[size_t(g * 8u + i / kHistBins) * kHistBins + i % kHistBins], sm[i]);
+}
+
+hipError_t chan_hist(const Tens& x, uint32_t c, uint32_t P, const uint32_t* amax, uint32_t* hist, hipStream_t s) {
+    if (c == 0 || c % 8u || x.stride % 8u || x.off % 8u) return hipErrorInvalidValue;
+    if (P == 0) return hipSuccess;
+    const uint32_t chunk = 8192;
+    hipLaunchKernelGGL(chan_hist_k, dim3((P + chunk - 1) / chunk, c / 8u), dim3(kBlock), 0, s, x, P, chunk, amax, hist);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------- backward staging --
 // The weight gradient reduces over pixels, so its MFMA operands need 8
 // consecutive pixels of one channel per lane: the backward pass stages dz and

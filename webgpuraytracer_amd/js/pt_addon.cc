@@ -7,12 +7,16 @@
 // layouts; a nonzero pt_* status becomes a thrown JS Error carrying
 // pt_last_error().  Calls are synchronous (the C ABI returns after the
 // device work is done), which is what `await onSubmittedWorkDone()` gives the
-// reference.
+// reference.  The long calls (trace, denoise, unetDenoise, batchFrame) also
+// come as *Async variants returning a Promise (napi_async_work), for hosts that
+// must keep their event loop running while the GPU works.
 #include <node_api.h>
 
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "pt_abi.h"
@@ -44,12 +48,24 @@ bool get_args(napi_env env, napi_callback_info info, size_t want, napi_value* ar
     return true;
 }
 
+// Handles (contexts, batches) with an asynchronous call in flight.  A context
+// is not re-entrant, so any other call on a busy handle throws.  Touched only
+// on the JavaScript thread.
+std::unordered_set<const void*> g_busy;
+
+bool not_busy(napi_env env, const void* h) {
+    if (!g_busy.count(h)) return true;
+    napi_throw_error(env, nullptr, "busy: an asynchronous call on this context is still in flight");
+    return false;
+}
+
 pt_ctx* get_ctx(napi_env env, napi_value v) {
     void* p = nullptr;
     if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
         napi_throw_type_error(env, nullptr, "first argument must be a pt context");
         return nullptr;
     }
+    if (!not_busy(env, p)) return nullptr;
     return static_cast<pt_ctx*>(p);
 }
 
@@ -234,31 +250,7 @@ napi_value set_num(napi_env env, napi_value obj, const char* k, double v) {
     return obj;
 }
 
-// trace(ctx, params, sectionW, sectionH, nPasses, seeds|null) -> stats
-napi_value trace(napi_env env, napi_callback_info info) {
-    napi_value argv[6];
-    if (!get_args(env, info, 6, argv)) return nullptr;
-    pt_ctx* ctx = get_ctx(env, argv[0]);
-    if (!ctx) return nullptr;
-    pt_scene_params p;
-    if (!get_params(env, argv[1], &p)) return nullptr;
-    uint32_t sw = 0, sh = 0, n = 1;
-    napi_get_value_uint32(env, argv[2], &sw);
-    napi_get_value_uint32(env, argv[3], &sh);
-    napi_get_value_uint32(env, argv[4], &n);
-    const uint32_t* seeds = nullptr;
-    void* sp;
-    size_t sn;
-    if (get_bytes(env, argv[5], &sp, &sn)) {
-        if (sn < size_t(n) * 4) {
-            napi_throw_type_error(env, nullptr, "seeds must hold one u32 per pass");
-            return nullptr;
-        }
-        seeds = static_cast<const uint32_t*>(sp);
-    }
-    pt_stats st;
-    int rc = pt_trace(ctx, &p, sw, sh, n, seeds, &st);
-    if (rc) return throw_pt(env, ctx, rc, "pt_trace");
+napi_value stats_object(napi_env env, const pt_stats& st) {
     napi_value o;
     napi_create_object(env, &o);
     set_num(env, o, "extensionRays", double(st.extension_rays));
@@ -266,6 +258,48 @@ napi_value trace(napi_env env, napi_callback_info info) {
     set_num(env, o, "paths", double(st.paths));
     set_num(env, o, "kernelMs", st.kernel_ms);
     return o;
+}
+
+// The arguments of one trace call, copied off the JavaScript values.
+struct TraceCall {
+    pt_ctx* ctx = nullptr;
+    pt_scene_params p{};
+    uint32_t sw = 0, sh = 0, n = 1;
+    std::vector<uint32_t> seeds;
+    bool has_seeds = false;
+};
+
+// trace(ctx, params, sectionW, sectionH, nPasses, seeds|null)
+bool parse_trace(napi_env env, napi_callback_info info, TraceCall* t) {
+    napi_value argv[6];
+    if (!get_args(env, info, 6, argv)) return false;
+    if (!(t->ctx = get_ctx(env, argv[0]))) return false;
+    if (!get_params(env, argv[1], &t->p)) return false;
+    napi_get_value_uint32(env, argv[2], &t->sw);
+    napi_get_value_uint32(env, argv[3], &t->sh);
+    napi_get_value_uint32(env, argv[4], &t->n);
+    void* sp;
+    size_t sn;
+    if (get_bytes(env, argv[5], &sp, &sn)) {
+        if (sn < size_t(t->n) * 4) {
+            napi_throw_type_error(env, nullptr, "seeds must hold one u32 per pass");
+            return false;
+        }
+        const uint32_t* s = static_cast<const uint32_t*>(sp);
+        t->seeds.assign(s, s + t->n);
+        t->has_seeds = true;
+    }
+    return true;
+}
+
+// trace(ctx, params, sectionW, sectionH, nPasses, seeds|null) -> stats
+napi_value trace(napi_env env, napi_callback_info info) {
+    TraceCall t;
+    if (!parse_trace(env, info, &t)) return nullptr;
+    pt_stats st;
+    int rc = pt_trace(t.ctx, &t.p, t.sw, t.sh, t.n, t.has_seeds ? t.seeds.data() : nullptr, &st);
+    if (rc) return throw_pt(env, t.ctx, rc, "pt_trace");
+    return stats_object(env, st);
 }
 
 napi_value gbuffer(napi_env env, napi_callback_info info) {
@@ -280,21 +314,28 @@ napi_value gbuffer(napi_env env, napi_callback_info info) {
     return nullptr;
 }
 
-napi_value denoise(napi_env env, napi_callback_info info) {
+// denoise(ctx, uniforms48, strideMode)
+bool parse_denoise(napi_env env, napi_callback_info info, pt_ctx** ctx, pt_denoise_params* u, int32_t* mode) {
     napi_value argv[3];
-    if (!get_args(env, info, 3, argv)) return nullptr;
-    pt_ctx* ctx = get_ctx(env, argv[0]);
-    if (!ctx) return nullptr;
+    if (!get_args(env, info, 3, argv)) return false;
+    if (!(*ctx = get_ctx(env, argv[0]))) return false;
     void* data;
     size_t n;
     if (!get_bytes(env, argv[1], &data, &n) || n != sizeof(pt_denoise_params)) {
         napi_throw_type_error(env, nullptr, "uniforms must be the 48-byte denoiser uniform");
-        return nullptr;
+        return false;
     }
+    std::memcpy(u, data, sizeof(*u));
+    *mode = 0;
+    napi_get_value_int32(env, argv[2], mode);
+    return true;
+}
+
+napi_value denoise(napi_env env, napi_callback_info info) {
+    pt_ctx* ctx;
     pt_denoise_params u;
-    std::memcpy(&u, data, sizeof(u));
-    int32_t mode = 0;
-    napi_get_value_int32(env, argv[2], &mode);
+    int32_t mode;
+    if (!parse_denoise(env, info, &ctx, &u, &mode)) return nullptr;
     int rc = pt_denoise(ctx, &u, mode);
     if (rc) return throw_pt(env, ctx, rc, "pt_denoise");
     return nullptr;
@@ -391,13 +432,20 @@ napi_value unet_set_params(napi_env env, napi_callback_info info) {
     return nullptr;
 }
 
-napi_value unet_denoise(napi_env env, napi_callback_info info) {
+// unetDenoise(ctx, exposure)
+bool parse_unet(napi_env env, napi_callback_info info, pt_ctx** ctx, double* exposure) {
     napi_value argv[2];
-    if (!get_args(env, info, 2, argv)) return nullptr;
-    pt_ctx* ctx = get_ctx(env, argv[0]);
-    if (!ctx) return nullptr;
-    double exposure = 1.0;
-    napi_get_value_double(env, argv[1], &exposure);
+    if (!get_args(env, info, 2, argv)) return false;
+    if (!(*ctx = get_ctx(env, argv[0]))) return false;
+    *exposure = 1.0;
+    napi_get_value_double(env, argv[1], exposure);
+    return true;
+}
+
+napi_value unet_denoise(napi_env env, napi_callback_info info) {
+    pt_ctx* ctx;
+    double exposure;
+    if (!parse_unet(env, info, &ctx, &exposure)) return nullptr;
     int rc = pt_unet_denoise(ctx, float(exposure));
     if (rc) return throw_pt(env, ctx, rc, "pt_unet_denoise");
     return nullptr;
@@ -420,12 +468,17 @@ napi_value unet_set_precision(napi_env env, napi_callback_info info) {
 // -> [{extensionRays, shadowRays, paths, kernelMs}...]
 void batch_cleanup(napi_env, void* data, void*) { pt_batch_destroy(static_cast<pt_batch*>(data)); }
 
+// The contexts of each batch (their jobs), so an asynchronous batch frame can
+// mark them busy too.
+std::unordered_map<const void*, std::vector<const void*>> g_batch_jobs;
+
 pt_batch* get_batch(napi_env env, napi_value v) {
     void* p = nullptr;
     if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
         napi_throw_type_error(env, nullptr, "first argument must be a pt batch");
         return nullptr;
     }
+    if (!not_busy(env, p)) return nullptr;
     return static_cast<pt_batch*>(p);
 }
 
@@ -462,21 +515,34 @@ napi_value batch_add(napi_env env, napi_callback_info info) {
     uint32_t idx = 0;
     int rc = pt_batch_add(b, ctx, &idx);
     if (rc) return throw_batch(env, b, rc, "pt_batch_add");
+    g_batch_jobs[b].push_back(ctx);
     napi_value r;
     napi_create_uint32(env, idx, &r);
     return r;
 }
 
-napi_value batch_frame(napi_env env, napi_callback_info info) {
+// The arguments of one batch frame, copied off the JavaScript values.
+struct BatchCall {
+    pt_batch* b = nullptr;
+    std::vector<pt_scene_params> params;
+    std::vector<uint32_t> passes;
+    std::vector<std::vector<uint32_t>> seeds;
+    std::vector<char> has_seeds;
+};
+
+// batchFrame(batch, [params ArrayBuffer...], [passes...], [seeds Uint32Array|null...])
+bool parse_batch(napi_env env, napi_callback_info info, BatchCall* c) {
     napi_value argv[4];
-    if (!get_args(env, info, 4, argv)) return nullptr;
-    pt_batch* b = get_batch(env, argv[0]);
-    if (!b) return nullptr;
+    if (!get_args(env, info, 4, argv)) return false;
+    if (!(c->b = get_batch(env, argv[0]))) return false;
     uint32_t n = 0;
-    pt_batch_size(b, &n);
-    std::vector<pt_scene_params> params(n);
-    std::vector<uint32_t> passes(n, 0);
-    std::vector<const uint32_t*> seeds(n, nullptr);
+    pt_batch_size(c->b, &n);
+    for (const void* job : g_batch_jobs[c->b])
+        if (!not_busy(env, job)) return false;
+    c->params.resize(n);
+    c->passes.assign(n, 0);
+    c->seeds.resize(n);
+    c->has_seeds.assign(n, 0);
     for (uint32_t i = 0; i < n; ++i) {
         napi_value pv, nv, sv;
         void* data;
@@ -484,36 +550,178 @@ napi_value batch_frame(napi_env env, napi_callback_info info) {
         if (napi_get_element(env, argv[1], i, &pv) != napi_ok || !get_bytes(env, pv, &data, &len) ||
             len != sizeof(pt_scene_params)) {
             napi_throw_type_error(env, nullptr, "params must hold one 112-byte SceneParams per job");
-            return nullptr;
+            return false;
         }
-        std::memcpy(&params[i], data, sizeof(pt_scene_params));
-        if (napi_get_element(env, argv[2], i, &nv) != napi_ok || napi_get_value_uint32(env, nv, &passes[i]) != napi_ok) {
+        std::memcpy(&c->params[i], data, sizeof(pt_scene_params));
+        if (napi_get_element(env, argv[2], i, &nv) != napi_ok ||
+            napi_get_value_uint32(env, nv, &c->passes[i]) != napi_ok) {
             napi_throw_type_error(env, nullptr, "passes must hold one count per job");
-            return nullptr;
+            return false;
         }
         if (napi_get_element(env, argv[3], i, &sv) == napi_ok && get_bytes(env, sv, &data, &len)) {
-            if (len < size_t(passes[i]) * 4) {
+            if (len < size_t(c->passes[i]) * 4) {
                 napi_throw_type_error(env, nullptr, "seeds must hold one u32 per pass");
-                return nullptr;
+                return false;
             }
-            seeds[i] = static_cast<const uint32_t*>(data);
+            const uint32_t* s = static_cast<const uint32_t*>(data);
+            c->seeds[i].assign(s, s + c->passes[i]);
+            c->has_seeds[i] = 1;
         }
     }
-    std::vector<pt_stats> st(n);
-    int rc = pt_batch_frame(b, params.data(), passes.data(), seeds.data(), st.data());
-    if (rc) return throw_batch(env, b, rc, "pt_batch_frame");
+    return true;
+}
+
+int run_batch(BatchCall& c, std::vector<pt_stats>& st) {
+    std::vector<const uint32_t*> sp(c.passes.size(), nullptr);
+    for (size_t i = 0; i < sp.size(); ++i)
+        if (c.has_seeds[i]) sp[i] = c.seeds[i].data();
+    st.assign(c.passes.size(), pt_stats{});
+    return pt_batch_frame(c.b, c.params.data(), c.passes.data(), sp.data(), st.data());
+}
+
+napi_value stats_array(napi_env env, const std::vector<pt_stats>& st) {
     napi_value arr;
-    napi_create_array_with_length(env, n, &arr);
-    for (uint32_t i = 0; i < n; ++i) {
-        napi_value o;
-        napi_create_object(env, &o);
-        set_num(env, o, "extensionRays", double(st[i].extension_rays));
-        set_num(env, o, "shadowRays", double(st[i].shadow_rays));
-        set_num(env, o, "paths", double(st[i].paths));
-        set_num(env, o, "kernelMs", st[i].kernel_ms);
-        napi_set_element(env, arr, i, o);
-    }
+    napi_create_array_with_length(env, st.size(), &arr);
+    for (uint32_t i = 0; i < st.size(); ++i) napi_set_element(env, arr, i, stats_object(env, st[i]));
     return arr;
+}
+
+napi_value batch_frame(napi_env env, napi_callback_info info) {
+    BatchCall c;
+    if (!parse_batch(env, info, &c)) return nullptr;
+    std::vector<pt_stats> st;
+    int rc = run_batch(c, st);
+    if (rc) return throw_batch(env, c.b, rc, "pt_batch_frame");
+    return stats_array(env, st);
+}
+
+// ---- asynchronous variants --------------------------------------------------
+// traceAsync / denoiseAsync / unetDenoiseAsync / batchFrameAsync take the same
+// arguments as the synchronous calls and return a Promise, as the reference's
+// host awaits the GPU (await device.queue.onSubmittedWorkDone(),
+// tracer.js:2280-2281, 2294).  The arguments are copied on the JavaScript
+// thread, the C call runs on a libuv worker thread, and the Promise settles on
+// the JavaScript thread with the synchronous call's result, or rejects with
+// its Error.  The handles involved are marked busy (and kept alive by
+// references) until then.
+enum class Op { Trace, Denoise, Unet, Batch };
+
+struct AsyncCall {
+    Op op;
+    napi_async_work work = nullptr;
+    napi_deferred deferred = nullptr;
+    std::vector<napi_ref> refs;           // the handle Externals, alive while in flight
+    std::vector<const void*> claimed;     // busy handles
+    TraceCall t;
+    pt_ctx* ctx = nullptr;
+    pt_denoise_params u{};
+    int32_t mode = 0;
+    double exposure = 1.0;
+    BatchCall bc;
+    int rc = 0;
+    std::string err;
+    pt_stats st{};
+    std::vector<pt_stats> bst;
+};
+
+void async_execute(napi_env, void* data) {
+    AsyncCall* w = static_cast<AsyncCall*>(data);
+    const char* what = "";
+    switch (w->op) {
+        case Op::Trace:
+            what = "pt_trace";
+            w->rc = pt_trace(w->t.ctx, &w->t.p, w->t.sw, w->t.sh, w->t.n, w->t.has_seeds ? w->t.seeds.data() : nullptr,
+                             &w->st);
+            break;
+        case Op::Denoise:
+            what = "pt_denoise";
+            w->rc = pt_denoise(w->ctx, &w->u, w->mode);
+            break;
+        case Op::Unet:
+            what = "pt_unet_denoise";
+            w->rc = pt_unet_denoise(w->ctx, float(w->exposure));
+            break;
+        case Op::Batch:
+            what = "pt_batch_frame";
+            w->rc = run_batch(w->bc, w->bst);
+            break;
+    }
+    if (w->rc) {
+        const char* msg = w->op == Op::Batch ? pt_batch_last_error(w->bc.b)
+                                             : pt_last_error(w->op == Op::Trace ? w->t.ctx : w->ctx);
+        w->err = std::string(what) + " failed (" + std::to_string(w->rc) + "): " + msg;
+    }
+}
+
+void async_complete(napi_env env, napi_status status, void* data) {
+    AsyncCall* w = static_cast<AsyncCall*>(data);
+    for (const void* h : w->claimed) g_busy.erase(h);
+    if (status != napi_ok || w->rc) {
+        napi_value m, e;
+        napi_create_string_utf8(env, status != napi_ok ? "asynchronous call cancelled" : w->err.c_str(),
+                                NAPI_AUTO_LENGTH, &m);
+        napi_create_error(env, nullptr, m, &e);
+        napi_reject_deferred(env, w->deferred, e);
+    } else {
+        napi_value r;
+        if (w->op == Op::Trace) r = stats_object(env, w->st);
+        else if (w->op == Op::Batch) r = stats_array(env, w->bst);
+        else napi_get_undefined(env, &r);
+        napi_resolve_deferred(env, w->deferred, r);
+    }
+    for (napi_ref ref : w->refs) napi_delete_reference(env, ref);
+    napi_delete_async_work(env, w->work);
+    delete w;
+}
+
+napi_value queue_async(napi_env env, napi_callback_info info, AsyncCall* w, const char* name) {
+    // keep the handle Externals (the first argument) alive while in flight
+    napi_value argv[1];
+    size_t argc = 1;
+    napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr);
+    napi_ref ref;
+    if (napi_create_reference(env, argv[0], 1, &ref) == napi_ok) w->refs.push_back(ref);
+    napi_value promise, res_name;
+    if (napi_create_promise(env, &w->deferred, &promise) != napi_ok ||
+        napi_create_string_utf8(env, name, NAPI_AUTO_LENGTH, &res_name) != napi_ok ||
+        napi_create_async_work(env, nullptr, res_name, async_execute, async_complete, w, &w->work) != napi_ok ||
+        napi_queue_async_work(env, w->work) != napi_ok) {
+        for (napi_ref r : w->refs) napi_delete_reference(env, r);
+        delete w;
+        napi_throw_error(env, nullptr, "could not queue the asynchronous call");
+        return nullptr;
+    }
+    for (const void* h : w->claimed) g_busy.insert(h);
+    return promise;
+}
+
+napi_value trace_async(napi_env env, napi_callback_info info) {
+    auto* w = new AsyncCall{Op::Trace};
+    if (!parse_trace(env, info, &w->t)) { delete w; return nullptr; }
+    w->claimed = {w->t.ctx};
+    return queue_async(env, info, w, "pt_trace");
+}
+
+napi_value denoise_async(napi_env env, napi_callback_info info) {
+    auto* w = new AsyncCall{Op::Denoise};
+    if (!parse_denoise(env, info, &w->ctx, &w->u, &w->mode)) { delete w; return nullptr; }
+    w->claimed = {w->ctx};
+    return queue_async(env, info, w, "pt_denoise");
+}
+
+napi_value unet_denoise_async(napi_env env, napi_callback_info info) {
+    auto* w = new AsyncCall{Op::Unet};
+    if (!parse_unet(env, info, &w->ctx, &w->exposure)) { delete w; return nullptr; }
+    w->claimed = {w->ctx};
+    return queue_async(env, info, w, "pt_unet_denoise");
+}
+
+napi_value batch_frame_async(napi_env env, napi_callback_info info) {
+    auto* w = new AsyncCall{Op::Batch};
+    if (!parse_batch(env, info, &w->bc)) { delete w; return nullptr; }
+    w->claimed = g_batch_jobs[w->bc.b];
+    w->claimed.push_back(w->bc.b);
+    return queue_async(env, info, w, "pt_batch_frame");
 }
 
 napi_value last_error(napi_env env, napi_callback_info info) {
@@ -550,6 +758,10 @@ napi_value init(napi_env env, napi_value exports) {
         {"batchCreate", nullptr, batch_create, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"batchAdd", nullptr, batch_add, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
         {"batchFrame", nullptr, batch_frame, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"traceAsync", nullptr, trace_async, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"denoiseAsync", nullptr, denoise_async, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"unetDenoiseAsync", nullptr, unet_denoise_async, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
+        {"batchFrameAsync", nullptr, batch_frame_async, nullptr, nullptr, nullptr, napi_enumerable, nullptr},
     };
     napi_define_properties(env, exports, sizeof(props) / sizeof(props[0]), props);
     return exports;

@@ -144,6 +144,24 @@ class NativeRenderer {
     return this.renderSection(camera, null, seeds);
   }
 
+  // The same, returning a Promise that resolves with the stats once the GPU is
+  // done, like the reference's `await onSubmittedWorkDone()` (tracer.js:2280-2281,
+  // 2294); the event loop keeps running meanwhile.  Other calls on this
+  // renderer throw until it settles.
+  renderSectionAsync(camera, section, seeds) {
+    const p = this.params(camera, seeds[0], section);
+    const pr = addon.traceAsync(this.ctx, p, section ? section.width : 0, section ? section.height : 0,
+                                seeds.length, seeds);
+    this.frame += seeds.length;
+    return pr;
+  }
+
+  renderAsync(camera, reps = 1, seedBase = 0) {
+    const seeds = new Uint32Array(reps);
+    for (let i = 0; i < reps; i++) seeds[i] = (Math.imul(this.frame + i, 0x85EBCA77) + seedBase) >>> 0;
+    return this.renderSectionAsync(camera, null, seeds);
+  }
+
   gbuffer(camera) { addon.gbuffer(this.ctx, this.params(camera, 0)); }
 
   setOption(key, value) { addon.setOption(this.ctx, key, value); }
@@ -187,6 +205,11 @@ class NativeDenoiser {
     addon.denoise(this.r.ctx, this.uniforms(), strideMode);
     return this.r.readOutput();
   }
+
+  // Promise of the rgba8 output
+  executeAsync(strideMode = 0) {
+    return addon.denoiseAsync(this.r.ctx, this.uniforms(), strideMode).then(() => this.r.readOutput());
+  }
 }
 
 // A little-endian float32 .npy file (the package's trained weights) as a Float32Array.
@@ -223,6 +246,11 @@ class NativeLearnedDenoiser {
     return this.r.readOutput();
   }
 
+  // Promise of the rgba8 output
+  executeAsync() {
+    return addon.unetDenoiseAsync(this.r.ctx, this.exposure).then(() => this.r.readOutput());
+  }
+
   readResult() { return this.r.readDenoised(); }
 
   // fp8 inference (true: calibrated on the renderer's current frame) or bf16
@@ -243,7 +271,7 @@ class NativeBatch {
   }
 
   // cameras[i], passes[i] for job i (0 skips it); seeds as NativeRenderer.render derives them
-  frame(cameras, passes, seedBase = 0) {
+  _args(cameras, passes, seedBase) {
     const params = [], seeds = [];
     this.jobs.forEach((r, i) => {
       const s = new Uint32Array(passes[i]);
@@ -251,9 +279,22 @@ class NativeBatch {
       params.push(r.params(cameras[i], passes[i] ? s[0] : 0));
       seeds.push(passes[i] ? s : null);
     });
+    return [params, seeds];
+  }
+
+  frame(cameras, passes, seedBase = 0) {
+    const [params, seeds] = this._args(cameras, passes, seedBase);
     const st = addon.batchFrame(this.h, params, passes, seeds);
     this.jobs.forEach((r, i) => { r.frame += passes[i]; });
     return st;
+  }
+
+  // Promise of the per-job stats; the batch and its renderers are busy until it settles
+  frameAsync(cameras, passes, seedBase = 0) {
+    const [params, seeds] = this._args(cameras, passes, seedBase);
+    const pr = addon.batchFrameAsync(this.h, params, passes, seeds);
+    this.jobs.forEach((r, i) => { r.frame += passes[i]; });
+    return pr;
   }
 }
 
