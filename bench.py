@@ -88,8 +88,9 @@ def main():
     ap.add_argument("--backend", choices=["megakernel", "wavefront"], default="wavefront")
     ap.add_argument("--schedule", choices=["regen", "sync"], default="regen",
                     help="megakernel pass schedule (same bits either way)")
-    ap.add_argument("--traversal", choices=["wide", "reference"], default="wide",
-                    help="BVH traversal: 4-wide nodes, or the reference's child-pair order")
+    ap.add_argument("--traversal", choices=["wide", "quant", "reference"], default="wide",
+                    help="BVH traversal: 4-wide nodes, 4-wide nodes with 8-bit quantised boxes, or the "
+                         "reference's child-pair order")
     ap.add_argument("--blas-build", choices=["reference", "device"], default="reference",
                     help="bottom-level trees of the wide traversal: the uploaded BVH, or rebuilt on the GPU")
     ap.add_argument("--tlas-build", choices=["reference", "device"], default="reference",
@@ -125,19 +126,23 @@ def main():
     d = scene.desc()
     mk = (A.PT_OPT_BACKEND, A.PT_BACKEND_MEGAKERNEL)
     regen, sync = (A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_REGEN), (A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_SYNC)
-    wide, refo = (A.PT_OPT_TRAVERSAL, A.PT_TRAVERSAL_WIDE), (A.PT_OPT_TRAVERSAL, A.PT_TRAVERSAL_REFERENCE)
+    trav = {"wide": (A.PT_OPT_TRAVERSAL, A.PT_TRAVERSAL_WIDE),
+            "quant": (A.PT_OPT_TRAVERSAL, A.PT_TRAVERSAL_WIDE_QUANTIZED),
+            "reference": (A.PT_OPT_TRAVERSAL, A.PT_TRAVERSAL_REFERENCE)}
     wf = (A.PT_OPT_BACKEND, A.PT_BACKEND_WAVEFRONT)
-    variants = {"wavefront/wide": [wf, wide],
-                "wavefront/reference": [wf, refo],
-                "megakernel/regen/wide": [mk, regen, wide],
-                "megakernel/regen/reference": [mk, regen, refo],
-                "megakernel/sync/wide": [mk, sync, wide]}
+    variants = {"wavefront/wide": [wf, trav["wide"]],
+                "wavefront/quant": [wf, trav["quant"]],
+                "wavefront/reference": [wf, trav["reference"]],
+                "megakernel/regen/wide": [mk, regen, trav["wide"]],
+                "megakernel/regen/quant": [mk, regen, trav["quant"]],
+                "megakernel/regen/reference": [mk, regen, trav["reference"]],
+                "megakernel/sync/wide": [mk, sync, trav["wide"]]}
     if args.backend == "wavefront":
         chosen = "wavefront/" + args.traversal
     else:
         chosen = "megakernel/%s/%s" % (args.schedule, args.traversal)
     if chosen not in variants:
-        variants[chosen] = [mk, sync if args.schedule == "sync" else regen, wide if args.traversal == "wide" else refo]
+        variants[chosen] = [mk, sync if args.schedule == "sync" else regen, trav[args.traversal]]
 
     def step(k):
         seeds = [(args.seed * 0x9E3779B1 + (k * SPP + i) * 0x85EBCA77) & 0xFFFFFFFF for i in range(SPP)]

@@ -166,9 +166,13 @@ enum { PT_SCHEDULE_SYNC = 0, PT_SCHEDULE_REGEN = 1 };
    per-hit node counts match it bit for bit.  WIDE: the same boxes regrouped
    into 4-wide nodes, nearest child first; the same closest hit except where
    two candidates tie within rounding (then the visit order picks one), and
-   node counts are of wide nodes.  Work counters (PT_OPT_COUNT_STATS) and
-   pt_trace_hits always use REFERENCE. */
-enum { PT_TRAVERSAL_REFERENCE = 0, PT_TRAVERSAL_WIDE = 1 };
+   node counts are of wide nodes.  WIDE_QUANTIZED: the same 4-wide trees
+   with each child box stored as 8-bit offsets from the node's origin in
+   power-of-two steps, rounded outward (64 B nodes instead of 128 B); every
+   decoded box contains the exact one, so the same closest hits as WIDE up to
+   ties.  Work counters (PT_OPT_COUNT_STATS) and pt_trace_hits always use
+   REFERENCE. */
+enum { PT_TRAVERSAL_REFERENCE = 0, PT_TRAVERSAL_WIDE = 1, PT_TRAVERSAL_WIDE_QUANTIZED = 2 };
 
 /* Lifetime: `new Renderer` + Renderer.init (tracer.js:1719-1770). */
 int pt_create(pt_ctx** out, int device);

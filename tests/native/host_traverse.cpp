@@ -1,7 +1,8 @@
 // host_traverse.cpp — the kernels' traversal code (dev/traverse.h) compiled
 // for the CPU, over the scene layout pt_layout.cpp builds for the device.
-// Test-only: lets the CPU test suite compare the child-pair and the wide
-// traversal with the oracle, and check the wide stack bound, without a GPU.
+// Test-only: lets the CPU test suite compare the child-pair, the wide and the
+// quantised wide traversal with the oracle, and check the wide stack bound,
+// without a GPU.
 #include <algorithm>
 #include <cstring>
 #include <string>
@@ -17,6 +18,7 @@ namespace {
 struct HostScene {
     HostLayout L;
     std::vector<float4> geo;
+    std::vector<QNode> qtn, qbn;  // the wide nodes quantised as the device builder does
     DevScene S{};
 };
 
@@ -31,6 +33,10 @@ bool make_scene(const pt_scene_desc* d, HostScene& h, std::string& err) {
     S.bnodes = h.L.bn.data();
     S.wtnodes = h.L.wtn.data();
     S.wbnodes = h.L.wbn.data();
+    for (const WNode& w : h.L.wtn) h.qtn.push_back(quantize_node(w));
+    for (const WNode& w : h.L.wbn) h.qbn.push_back(quantize_node(w));
+    S.qtnodes = h.qtn.data();
+    S.qbnodes = h.qbn.data();
     S.roots = h.L.roots.data();
     S.tlas_raw = static_cast<const pt_bvh_node*>(d->tlas.ptr);
     S.bvh_raw = static_cast<const pt_bvh_node*>(d->bvh.ptr);
@@ -84,8 +90,27 @@ long ht_wide_nodes(const pt_scene_desc* d, void* out_tlas, void* out_blas) {
     return long(h.L.wtn.size() + h.L.wbn.size());
 }
 
-// Closest hits of n rays (6 floats each).  wide selects the traversal;
-// stack_cap 0 uses the layout's bound, otherwise the given capacity.
+// The quantised nodes (TLAS then BLAS, 64 B each), as ht_wide_nodes.
+long ht_quant_nodes(const pt_scene_desc* d, void* out_tlas, void* out_blas) {
+    HostScene h;
+    std::string err;
+    if (!make_scene(d, h, err)) return -1;
+    if (out_tlas) std::memcpy(out_tlas, h.qtn.data(), h.qtn.size() * sizeof(QNode));
+    if (out_blas) std::memcpy(out_blas, h.qbn.data(), h.qbn.size() * sizeof(QNode));
+    return long(h.qtn.size() + h.qbn.size());
+}
+
+// One node quantised: a WNode (128 B) in, a QNode (64 B) out.
+void ht_quantize_node(const void* wnode, void* qnode) {
+    WNode w;
+    std::memcpy(&w, wnode, sizeof(w));
+    const QNode q = quantize_node(w);
+    std::memcpy(qnode, &q, sizeof(q));
+}
+
+// Closest hits of n rays (6 floats each).  wide selects the traversal (0
+// child pairs, 1 wide, 2 quantised wide); stack_cap 0 uses the layout's
+// bound, otherwise the given capacity.
 int ht_trace_rays(const pt_scene_desc* d, const float* rays, uint32_t n, int wide, uint32_t stack_cap, pt_hit* out) {
     HostScene h;
     std::string err;
@@ -100,10 +125,12 @@ int ht_trace_rays(const pt_scene_desc* d, const float* rays, uint32_t n, int wid
         Counters c;
         c.clear();
         HitRec hr;
-        if (wide)
-            closest_hit<kDynamicFlags, false, true>(h.S, fl, o, dd, st, hr, c);
+        if (wide == kTravQuant)
+            closest_hit<kDynamicFlags, false, kTravQuant>(h.S, fl, o, dd, st, hr, c);
+        else if (wide)
+            closest_hit<kDynamicFlags, false, kTravWide>(h.S, fl, o, dd, st, hr, c);
         else
-            closest_hit<kDynamicFlags, false, false>(h.S, fl, o, dd, st, hr, c);
+            closest_hit<kDynamicFlags, false, kTravPairs>(h.S, fl, o, dd, st, hr, c);
         const SurfHit s = resolve_hit<false>(h.S, o, dd, hr);
         pt_hit& x = out[i];
         std::memset(&x, 0, sizeof(x));
@@ -133,8 +160,10 @@ int ht_trace_shadow_rays(const pt_scene_desc* d, const float* rays, const int32_
         LdsStack st = lds_stack_of(stack.data(), 1, cap);
         Counters c;
         c.clear();
-        const bool b = wide ? occluded<kDynamicFlags, false, true>(h.S, fl, o, dd, target[i], dist[i], st, c)
-                            : occluded<kDynamicFlags, false, false>(h.S, fl, o, dd, target[i], dist[i], st, c);
+        const bool b = wide == kTravQuant
+                           ? occluded<kDynamicFlags, false, kTravQuant>(h.S, fl, o, dd, target[i], dist[i], st, c)
+                       : wide ? occluded<kDynamicFlags, false, kTravWide>(h.S, fl, o, dd, target[i], dist[i], st, c)
+                              : occluded<kDynamicFlags, false, kTravPairs>(h.S, fl, o, dd, target[i], dist[i], st, c);
         occ[i] = b ? 1 : 0;
     }
     return 0;
@@ -159,8 +188,10 @@ int ht_shadow_transmittance(const pt_scene_desc* d, const float* rays, const int
         LdsStack st = lds_stack_of(stack.data(), 1, cap);
         Counters c;
         c.clear();
-        out[i] = wide ? transmittance<kDynamicFlags, false, true>(h.S, fl, o, dd, target[i], dist[i], st, c)
-                      : transmittance<kDynamicFlags, false, false>(h.S, fl, o, dd, target[i], dist[i], st, c);
+        out[i] = wide == kTravQuant
+                     ? transmittance<kDynamicFlags, false, kTravQuant>(h.S, fl, o, dd, target[i], dist[i], st, c)
+                 : wide ? transmittance<kDynamicFlags, false, kTravWide>(h.S, fl, o, dd, target[i], dist[i], st, c)
+                        : transmittance<kDynamicFlags, false, kTravPairs>(h.S, fl, o, dd, target[i], dist[i], st, c);
     }
     return 0;
 }

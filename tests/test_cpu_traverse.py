@@ -4,8 +4,11 @@ the oracle without a GPU:
 
 * the reference-order traversal over the child-pair nodes must reproduce the
   oracle's hits bit for bit, node-pop counts included;
-* the wide traversal must find the same closest hit, except where the FP64
-  arbiter finds two candidates tied, and the same shadow answers;
+* the wide traversal, over exact or quantised boxes, must find the same
+  closest hit, except where the FP64 arbiter finds two candidates tied, and
+  the same shadow answers;
+* every quantised box must contain its exact box and stay within two
+  quantisation steps of it;
 * the wide stack bound computed at upload must never drop a push.
 """
 import ctypes as C
@@ -35,6 +38,9 @@ def ht(tmp_path_factory):
     L.ht_layout_info.argtypes = [C.c_void_p, C.POINTER(C.c_uint32)]
     L.ht_wide_nodes.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.ht_wide_nodes.restype = C.c_long
+    L.ht_quant_nodes.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.ht_quant_nodes.restype = C.c_long
+    L.ht_quantize_node.argtypes = [C.c_void_p, C.c_void_p]
     L.ht_trace_rays.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_uint32, C.c_void_p]
     L.ht_trace_shadow_rays.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int,
                                        C.c_void_p]
@@ -94,32 +100,38 @@ def test_reference_order_matches_oracle_bitwise(ht, which):
     assert np.array_equal(g["t"].view(np.uint32), r["t"].view(np.uint32))
 
 
+WIDE = {"wide": A.PT_TRAVERSAL_WIDE, "quant": A.PT_TRAVERSAL_WIDE_QUANTIZED}
+
+
+@pytest.mark.parametrize("kind", sorted(WIDE))
 @pytest.mark.parametrize("which", ["c1", "mesh"])
-def test_wide_matches_oracle(ht, which):
+def test_wide_matches_oracle(ht, which, kind):
     scene = SCENES[which]()
     desc = scene.desc()
     ref = RefScene(desc)
     rays = np.concatenate([random_rays(6000, seed=7, box=4.0 if which == "c1" else 9.0,
                                        spread=3.0 if which == "c1" else 8.0), _camera_rays(scene, 48, 32)])
-    g = _trace(ht, desc, rays, wide=True)
+    g = _trace(ht, desc, rays, wide=WIDE[kind])
     r, _ = ref.trace_rays(rays)
     s = compare_hits(g, r, rays, ref)
-    print(which, s)
+    print(which, kind, s)
     assert s["exact_t"] >= 0.999 * s["n"]
 
 
 def test_wide_matches_oracle_large_mesh_scene(ht):
-    """The benchmark scene (808,500 triangles, 64 instances), 3000 rays."""
+    """The benchmark scene (808,500 triangles, 64 instances), 3000 rays, both
+    wide traversals."""
     scene = SceneBuilder.config(2, seed=7, sky=(64, 32))
     desc = scene.desc()
     ref = RefScene(desc)
     rays = random_rays(3000, seed=3, box=9.0, spread=8.0)
-    g = _trace(ht, desc, rays, wide=True)
     r, _ = ref.trace_rays(rays)
-    s = compare_hits(g, r, rays, ref)
     info = _info(ht, desc)
-    print(s, info)
-    assert s["exact_t"] >= 0.999 * s["n"]
+    for kind in sorted(WIDE):
+        g = _trace(ht, desc, rays, wide=WIDE[kind])
+        s = compare_hits(g, r, rays, ref)
+        print(kind, s, info)
+        assert s["exact_t"] >= 0.999 * s["n"]
     # two levels per wide node: about half the depth, at most 3 pushes a level
     assert info["wstack_cap"] <= 3 * (info["tlas_depth"] + info["blas_depth"]) + 2
 
@@ -135,7 +147,7 @@ def test_wide_shadow_equals_reference(ht, which):
     target = rng.integers(-1, max(n_obj, 1), size=len(rays)).astype(np.int32)
     dist = rng.uniform(0.5, 20.0, size=len(rays)).astype(np.float32)
     occ = {}
-    for wide in (0, 1):
+    for wide in (0, 1, 2):
         o = np.zeros(len(rays), np.uint8)
         assert ht.ht_trace_shadow_rays(C.byref(desc), rays.ctypes.data, target.ctypes.data, dist.ctypes.data,
                                        len(rays), wide, o.ctypes.data) == 0
@@ -143,19 +155,21 @@ def test_wide_shadow_equals_reference(ht, which):
     oracle = ref.trace_shadow_rays(rays, target, dist)[:, 0] == 0.0
     assert np.array_equal(occ[0].astype(bool), oracle)
     assert np.array_equal(occ[1], occ[0])
+    assert np.array_equal(occ[2], occ[0])
     assert 0.05 < oracle.mean() < 0.95
 
 
+@pytest.mark.parametrize("kind", sorted(WIDE))
 @pytest.mark.parametrize("which", ["c1", "mesh"])
-def test_wide_stack_bound_drops_nothing(ht, which):
+def test_wide_stack_bound_drops_nothing(ht, which, kind):
     """Results with the computed stack capacity equal those with a 128-entry
     stack (a dropped push would lose a subtree)."""
     scene = SCENES[which]()
     desc = scene.desc()
     info = _info(ht, desc)
     rays = random_rays(6000, seed=21, box=4.0 if which == "c1" else 9.0, spread=3.0 if which == "c1" else 8.0)
-    a = hits_to_np(_trace(ht, desc, rays, wide=True))
-    b = hits_to_np(_trace(ht, desc, rays, wide=True, cap=128))
+    a = hits_to_np(_trace(ht, desc, rays, wide=WIDE[kind]))
+    b = hits_to_np(_trace(ht, desc, rays, wide=WIDE[kind], cap=128))
     for k in ("o", "m", "prim", "plane", "t"):
         assert np.array_equal(a[k], b[k]), k
     assert 2 <= info["wstack_cap"] <= 128
@@ -193,6 +207,120 @@ def test_wide_nodes_regroup_the_reference_tree(ht):
     bvh_leaves = int(np.sum(bvh[:, 3] > 0))
     assert leaves(wt) == tlas_leaves
     assert leaves(wb) == bvh_leaves  # each distinct BLAS converted once
+
+
+def _decode_quant(q):
+    """Child boxes of QNodes (n x 16 uint32) as the traversal decodes them:
+    org + float(q) * 2^e per axis in float32.  Returns (lo, hi, count, full)
+    with lo, hi of shape (n, 3, 4)."""
+    org = q[:, 0:3].view(np.float32)
+    meta = q[:, 3]
+    full = (meta & 0x80000000) != 0
+    cnt = (meta >> 24) & 7
+    words_lo, words_hi = q[:, [4, 5, 6]], q[:, [7, 8, 9]]
+    lo = np.zeros((len(q), 3, 4), np.float32)
+    hi = np.zeros((len(q), 3, 4), np.float32)
+    for a in range(3):
+        step = np.ldexp(np.float32(1.0), ((meta >> (8 * a)) & 255).astype(np.int32) - 128).astype(np.float32)
+        for i in range(4):
+            ql = ((words_lo[:, a] >> (8 * i)) & 255).astype(np.float32)
+            qh = ((words_hi[:, a] >> (8 * i)) & 255).astype(np.float32)
+            lo[:, a, i] = org[:, a] + ql * step
+            hi[:, a, i] = org[:, a] + qh * step
+    return lo, hi, cnt, full, step
+
+
+def _check_quant(w, q):
+    """Every decoded box contains the WNode's box and lies within two steps
+    of it; counts and refs carry over; nothing is flagged in a finite tree."""
+    lo, hi, cnt, full, _ = _decode_quant(q)
+    assert not full.any()
+    assert np.array_equal(cnt, w[:, 28])
+    assert np.array_equal(q[:, 12:16], w[:, 24:28])
+    boxes = w[:, :24].view(np.float32).reshape(-1, 6, 4)
+    org = q[:, 0:3].view(np.float32)
+    meta = q[:, 3]
+    for a in range(3):
+        e = ((meta >> (8 * a)) & 255).astype(np.int64) - 128
+        step = np.ldexp(1.0, e)
+        for i in range(4):
+            used = cnt > i
+            wl, wh = boxes[used, a, i], boxes[used, 3 + a, i]
+            assert np.all(lo[used, a, i] <= wl) and np.all(hi[used, a, i] >= wh)
+            # one step, plus the rounding of org + q * step to float32
+            assert np.all(wl - lo[used, a, i] <= step[used] + np.spacing(np.abs(wl)))
+            assert np.all(hi[used, a, i] - wh <= step[used] + np.spacing(np.abs(wh)))
+        # the step is the smallest power of two giving 253 steps over the extent
+        ext = boxes[:, 3 + a, :].astype(np.float64).copy()
+        lo_ax = boxes[:, a, :].astype(np.float64).copy()
+        for i in range(4):
+            ext[cnt <= i, i] = -np.inf
+            lo_ax[cnt <= i, i] = np.inf
+        span = ext.max(axis=1) - lo_ax.min(axis=1)
+        assert np.array_equal(org[:, a].astype(np.float64), lo_ax.min(axis=1))
+        pos = span > 0
+        assert np.all(span[pos] <= 253 * step[pos])
+        assert np.all((span[pos] >= 253 * step[pos] / 2) | (e[pos] == -100))
+
+
+@pytest.mark.parametrize("which", ["c1", "mesh"])
+def test_quantised_nodes_contain_exact_boxes(ht, which):
+    scene = SCENES[which]()
+    desc = scene.desc()
+    info = _info(ht, desc)
+    wt = np.zeros((info["tlas_wnodes"], 32), np.uint32)
+    wb = np.zeros((info["blas_wnodes"], 32), np.uint32)
+    qt = np.zeros((info["tlas_wnodes"], 16), np.uint32)
+    qb = np.zeros((info["blas_wnodes"], 16), np.uint32)
+    ht.ht_wide_nodes(C.byref(desc), wt.ctypes.data if len(wt) else None, wb.ctypes.data if len(wb) else None)
+    n = ht.ht_quant_nodes(C.byref(desc), qt.ctypes.data if len(qt) else None, qb.ctypes.data if len(qb) else None)
+    assert n == len(qt) + len(qb)
+    for w, q in ((wt, qt), (wb, qb)):
+        if len(w):
+            _check_quant(w, q)
+
+
+def test_quantiser_edge_cases(ht):
+    """Degenerate, far-from-origin, tiny and huge child boxes; a non-finite
+    bound or an extent beyond 2^120 flags the node for the full box."""
+    inf = np.float32(np.inf)
+
+    def node(boxes):
+        w = np.zeros(32, np.uint32)
+        f = w[:24].view(np.float32).reshape(6, 4)
+        f[:3, :] = inf
+        f[3:, :] = -inf
+        for i, (lo, hi) in enumerate(boxes):
+            f[:3, i] = lo
+            f[3:, i] = hi
+        w[24:28] = [0x80000000 | (7 + i) for i in range(4)]
+        w[28] = len(boxes)
+        return w
+
+    def quant(w):
+        q = np.zeros(16, np.uint32)
+        ht.ht_quantize_node(w.ctypes.data, q.ctypes.data)
+        return q
+
+    cases = [
+        [((0, 0, 0), (0, 0, 0)), ((0, 0, 0), (0, 0, 0))],                        # points: zero extent
+        [((1e6, 1e6, 1e6), (1e6 + 0.125, 1e6, 1e6 + 3)), ((1e6 - 1, 1e6, 1e6), (1e6, 1e6 + 1, 1e6))],
+        [((1e-30, 0, 0), (2e-30, 1, 1)), ((3e-30, 0, 0), (4e-30, 1, 1)), ((-1, -1, -1), (-1, -1, -1))],
+        [((-3e37, -1, -1), (3e37, 1, 1)), ((0, 0, 0), (1, 1, 1)), ((5, 5, 5), (6, 6, 6)),
+         ((-7, 2, 2), (-6.5, 2.25, 9))],
+        [((0.1, 0.2, 0.3), (0.1000001, 0.2000001, 0.3000001)), ((0.1, 0.2, 0.3), (0.1, 0.2, 0.3))],
+        [((1e-30, 0, 0), (2e-30, 0, 0)), ((3e-30, 0, 0), (4e-30, 0, 0))],       # step clamped at 2^-100
+    ]
+    for boxes in cases:
+        w = node([(np.float32(lo), np.float32(hi)) for lo, hi in
+                  [(np.array(l, np.float32), np.array(h, np.float32)) for l, h in boxes]])
+        _check_quant(w[None, :], quant(w)[None, :])
+    for bad in ([((0, 0, 0), (inf, 1, 1)), ((0, 0, 0), (1, 1, 1))],
+                [((-3e38, 0, 0), (3e38, 1, 1)), ((0, 0, 0), (1, 1, 1))]):
+        w = node([(np.array(l, np.float32), np.array(h, np.float32)) for l, h in bad])
+        q = quant(w)
+        assert q[3] & 0x80000000 and (q[3] >> 24) & 7 == 2
+        assert np.array_equal(q[12:16], w[24:28])
 
 
 _OCTA_OBJ = b"""v 1 0 0
@@ -252,7 +380,7 @@ def test_translucent_shadows_equal_oracle(ht):
     target = rng.integers(-1, desc.objects.count, size=len(rays)).astype(np.int32)
     dist = rng.uniform(0.5, 8.0, size=len(rays)).astype(np.float32)
     want = RefScene(desc).trace_shadow_rays(rays, target, dist)[:, 0]
-    for wide in (0, 1):
+    for wide in (0, 1, 2):
         got = np.zeros(len(rays), np.float32)
         assert ht.ht_shadow_transmittance(C.byref(desc), rays.ctypes.data, target.ctypes.data, dist.ctypes.data,
                                           len(rays), wide, got.ctypes.data) == 0

@@ -71,10 +71,11 @@ def traversal(renderer, request):
     renderer.set_option(A.PT_OPT_TRAVERSAL, A.PT_TRAVERSAL_WIDE)
 
 
-TRAVERSALS = [A.PT_TRAVERSAL_REFERENCE, A.PT_TRAVERSAL_WIDE]
+TRAVERSALS = [A.PT_TRAVERSAL_REFERENCE, A.PT_TRAVERSAL_WIDE, A.PT_TRAVERSAL_WIDE_QUANTIZED]
+TRAV_IDS = ["reference", "wide", "quant"]
 
 
-@pytest.mark.parametrize("traversal", TRAVERSALS, indirect=True, ids=["reference", "wide"])
+@pytest.mark.parametrize("traversal", TRAVERSALS, indirect=True, ids=TRAV_IDS)
 @pytest.mark.parametrize("which", ["c1", "mesh"])
 def test_trace_rays_match_oracle(renderer, which, traversal):
     scene = SceneBuilder.config(1, seed=11) if which == "c1" else small_mesh_scene(seed=5)
@@ -205,7 +206,7 @@ def test_c2_section_1spp_matches_oracle(renderer):
     assert abs(int(st.shadow_rays) - int(rst.shadow_rays)) <= 0.005 * rst.shadow_rays
 
 
-@pytest.mark.parametrize("traversal", TRAVERSALS, indirect=True, ids=["reference", "wide"])
+@pytest.mark.parametrize("traversal", TRAVERSALS, indirect=True, ids=TRAV_IDS)
 def test_shadow_rays_match_oracle(renderer, traversal):
     scene = SceneBuilder.config(1, seed=13)
     ref = _setup(renderer, scene, 16, 16)
@@ -475,9 +476,10 @@ def test_backends_and_schedules_bitwise(renderer, which, section):
 
 @pytest.mark.parametrize("which", ["c1", "mesh"])
 def test_wide_traversal_renders_like_reference_order(renderer, which):
-    """The wide traversal picks a different closest hit only on ties within
-    rounding; on these scenes a 5-pass render is pixel-identical to the
-    reference-order traversal's on at least 99.9% of pixels."""
+    """The wide traversals (exact and quantised boxes) pick a different
+    closest hit only on ties within rounding; on these scenes a 5-pass render
+    is pixel-identical to the reference-order traversal's on at least 99.9%
+    of pixels."""
     scene = SceneBuilder.config(1, seed=82) if which == "c1" else small_mesh_scene(seed=14)
     w, h = 96, 64
     renderer.set_scene(scene)
@@ -492,12 +494,13 @@ def test_wide_traversal_renders_like_reference_order(renderer, which):
             acc[t] = renderer.read_accum()
     finally:
         renderer.set_option(A.PT_OPT_TRAVERSAL, A.PT_TRAVERSAL_WIDE)
-    same = np.all(acc[0].view(np.uint32) == acc[1].view(np.uint32), axis=-1)
-    print(which, "identical pixels", same.mean())
-    assert same.mean() >= 0.999
+    for t in TRAVERSALS[1:]:
+        same = np.all(acc[0].view(np.uint32) == acc[t].view(np.uint32), axis=-1)
+        print(which, "traversal", t, "identical pixels", same.mean())
+        assert same.mean() >= 0.999
 
 
-@pytest.mark.parametrize("traversal", TRAVERSALS, indirect=True, ids=["reference", "wide"])
+@pytest.mark.parametrize("traversal", TRAVERSALS, indirect=True, ids=TRAV_IDS)
 def test_stack_spill_bitwise(renderer, traversal):
     """Traversal-stack entries beyond the LDS share spill to device memory:
     a 2-entry LDS share (most pushes spill) renders the same bits as a

@@ -141,7 +141,7 @@ PT_UNET_INPUT, PT_UNET_TARGET = 0, 1
 PT_OBJ_MESH, PT_OBJ_SPHERE, PT_OBJ_CUBE, PT_OBJ_CYLINDER, PT_OBJ_TORUS = 0, 1, 2, 3, 4
 PT_UNET_BF16, PT_UNET_FP8 = 0, 1
 PT_SCHEDULE_SYNC, PT_SCHEDULE_REGEN = 0, 1
-PT_TRAVERSAL_REFERENCE, PT_TRAVERSAL_WIDE = 0, 1
+PT_TRAVERSAL_REFERENCE, PT_TRAVERSAL_WIDE, PT_TRAVERSAL_WIDE_QUANTIZED = 0, 1, 2
 
 
 def dtype_of(struct):

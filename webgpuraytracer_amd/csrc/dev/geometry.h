@@ -45,6 +45,97 @@ struct alignas(16) WNode {
 };
 static_assert(sizeof(WNode) == 128, "WNode is 128 B");
 
+// The same 4-wide node with its child boxes quantised to 8 bits per bound,
+// 64 B: half a WNode, so twice the nodes per cache line and per byte of HBM.
+// Bounds are offsets from the node's origin (the children's per-axis
+// minimum) in steps of a power of two per axis: the decoded value is
+// org + q * 2^e in float (one exact product, one rounding), and the builder
+// rounds every lower bound down and every upper bound up in that same
+// arithmetic, so each decoded box contains the child's exact box.  The slab
+// test on a containing box accepts every ray the exact box accepts (the
+// test is monotone in the bounds), so no hit is lost; only entry distances,
+// and with them the visit order of ties, move.  QNode i is WNode i (same
+// refs); a node whose box is not finite is flagged and read from the WNode.
+struct alignas(16) QNode {
+    float4 org;  // xyz: origin; w (bits): byte k = exponent of axis k + 128 (k < 3), byte 3 = child count
+                 // | kQFullBit
+    uint4 q0;    // x, y, z: the children's lower bounds on x, y, z (child i in byte i); w: upper bounds on x
+    uint4 q1;    // x, y: upper bounds on y, z; z, w: 0
+    uint4 ref;
+};
+static_assert(sizeof(QNode) == 64, "QNode is 64 B");
+constexpr uint32_t kQFullBit = 0x80000000u;
+
+// QNode i from WNode i (the device builder, and the CPU tests).  Per axis
+// the origin is the children's minimum and the step 2^e the smallest with 253
+// steps covering the children's extent (two steps to spare for the outward
+// corrections); every bound is rounded outward in the decoder's own
+// arithmetic, org + float(q) * 2^e, so the decoded box contains the exact
+// one.  A node with a bound that is not finite, or an extent beyond 2^120,
+// is flagged for the full node.
+__host__ __device__ inline float qnode_bound(float org, uint32_t q, float s) { return org + float(q) * s; }
+
+__host__ __device__ inline QNode quantize_node(const WNode& w) {
+    const uint32_t cnt = w.meta.x < 4u ? w.meta.x : 4u;
+    const float lo[3][4] = {{w.lox.x, w.lox.y, w.lox.z, w.lox.w}, {w.loy.x, w.loy.y, w.loy.z, w.loy.w},
+                            {w.loz.x, w.loz.y, w.loz.z, w.loz.w}};
+    const float hi[3][4] = {{w.hix.x, w.hix.y, w.hix.z, w.hix.w}, {w.hiy.x, w.hiy.y, w.hiy.z, w.hiy.w},
+                            {w.hiz.x, w.hiz.y, w.hiz.z, w.hiz.w}};
+    bool ok = cnt >= 1u;
+    float org[3] = {0.0f, 0.0f, 0.0f};
+    int ex[3] = {0, 0, 0};
+    uint32_t qlo[3] = {0u, 0u, 0u}, qhi[3] = {0u, 0u, 0u};
+    for (int a = 0; a < 3 && ok; ++a) {
+        float mn = __builtin_inff(), mxv = -__builtin_inff();
+        for (uint32_t c = 0; c < cnt; ++c) {
+            ok = ok && __builtin_isfinite(lo[a][c]) && __builtin_isfinite(hi[a][c]) && lo[a][c] <= hi[a][c];
+            mn = lo[a][c] < mn ? lo[a][c] : mn;
+            mxv = hi[a][c] > mxv ? hi[a][c] : mxv;
+        }
+        if (!ok) break;
+        org[a] = mn;
+        const double ext = double(mxv) - double(mn);
+        int e = -100;
+        if (ext > 0.0) {
+            int k;
+            (void)frexp(ext / 253.0, &k);  // ext / 253 < 2^k
+            e = k > -100 ? k : -100;
+        }
+        if (e > 120) { ok = false; break; }
+        ex[a] = e;
+        const float s = ldexpf(1.0f, e);
+        for (uint32_t c = 0; c < cnt; ++c) {
+            double fl = floor((double(lo[a][c]) - double(mn)) / double(s));
+            double ce = ceil((double(hi[a][c]) - double(mn)) / double(s));
+            uint32_t ql = fl < 0.0 ? 0u : (fl > 255.0 ? 255u : uint32_t(fl));
+            uint32_t qh = ce < 0.0 ? 0u : (ce > 255.0 ? 255u : uint32_t(ce));
+            while (ql > 0u && qnode_bound(mn, ql, s) > lo[a][c]) --ql;
+            while (qh < 255u && qnode_bound(mn, qh, s) < hi[a][c]) ++qh;
+            if (qnode_bound(mn, ql, s) > lo[a][c] || qnode_bound(mn, qh, s) < hi[a][c]) ok = false;
+            qlo[a] |= ql << (8u * c);
+            qhi[a] |= qh << (8u * c);
+        }
+    }
+    QNode q;
+    uint32_t meta = (cnt & 7u) << 24;
+    if (ok) {
+        for (int a = 0; a < 3; ++a) meta |= uint32_t(ex[a] + 128) << (8 * a);
+    } else {
+        meta |= kQFullBit;
+        org[0] = org[1] = org[2] = 0.0f;
+        for (int a = 0; a < 3; ++a) qlo[a] = qhi[a] = 0u;
+    }
+    uint32_t mbits;
+    __builtin_memcpy(&mbits, &meta, 4);
+    float mf;
+    __builtin_memcpy(&mf, &mbits, 4);
+    q.org = make_float4(org[0], org[1], org[2], mf);
+    q.q0 = make_uint4(qlo[0], qlo[1], qlo[2], qhi[0]);
+    q.q1 = make_uint4(qhi[1], qhi[2], 0u, 0u);
+    q.ref = w.ref;
+    return q;
+}
+
 struct alignas(16) Root {
     float mn[3];
     uint32_t ref;   // CNode tree
@@ -61,6 +152,8 @@ struct DevScene {
     const CNode* bnodes;
     const WNode* wtnodes;
     const WNode* wbnodes;
+    const QNode* qtnodes;          // the wide trees with quantised boxes (PT_TRAVERSAL_WIDE_QUANTIZED)
+    const QNode* qbnodes;
     const Root* roots;             // slot 0: TLAS; then one per distinct BLAS root
     const pt_bvh_node* tlas_raw;
     const pt_bvh_node* bvh_raw;

@@ -117,18 +117,67 @@ __device__ __forceinline__ bool visit_wide(const WNode* nodes, uint32_t ref, v3 
     return k0 != inf;
 }
 
-// One inner visit of either tree: W selects the wide nodes.
-template <bool W, bool CN>
-__device__ __forceinline__ bool visit_node(const CNode* cn, const WNode* wn, uint32_t ref, v3 o, v3 inv, float limit,
-                                           LdsStack& st, Counters& c, uint32_t& next) {
-    if (W) return visit_wide<CN>(wn, ref, o, inv, limit, st, c, next);
+// Wide-node visit on the quantised nodes: the four child boxes decoded as
+// org + q * 2^e (the builder's arithmetic), then the reference's slab test,
+// sort and pushes as visit_wide.  A flagged node is read from its WNode.
+__device__ __forceinline__ float qdec(float org, uint32_t word, int i, float s) {
+    return qnode_bound(org, (word >> (8 * i)) & 255u, s);
+}
+
+template <bool CN>
+__device__ __forceinline__ bool visit_qwide(const QNode* qn, const WNode* wn, uint32_t ref, v3 o, v3 inv, float limit,
+                                            LdsStack& st, Counters& c, uint32_t& next) {
+    const QNode* p = qn + ref;
+    const float4 g = p->org;
+    const uint32_t meta = f2u(g.w);
+    if (meta & kQFullBit) return visit_wide<CN>(wn, ref, o, inv, limit, st, c, next);
+    if (CN) c.inner++;
+    const uint4 q0 = p->q0, q1 = p->q1, r = p->ref;
+    const float inf = __builtin_inff();
+    const float sx = ldexpf(1.0f, int(meta & 255u) - 128);
+    const float sy = ldexpf(1.0f, int((meta >> 8) & 255u) - 128);
+    const float sz = ldexpf(1.0f, int((meta >> 16) & 255u) - 128);
+    const uint32_t cnt = (meta >> 24) & 7u;
+    // one child at a time: its six bounds decoded, then the slab test
+    auto key = [&](int i) {
+        const v3 lo = mk(qdec(g.x, q0.x, i, sx), qdec(g.y, q0.y, i, sy), qdec(g.z, q0.z, i, sz));
+        const v3 hi = mk(qdec(g.x, q0.w, i, sx), qdec(g.y, q1.x, i, sy), qdec(g.z, q1.y, i, sz));
+        const float t = slab(o, inv, lo, hi);
+        return (t >= 0.0f && t < limit) ? t : inf;
+    };
+    float k0 = key(0), k1 = key(1), k2 = key(2), k3 = key(3);
+    k2 = cnt > 2u ? k2 : inf;
+    k3 = cnt > 3u ? k3 : inf;
+    uint32_t r0 = r.x, r1 = r.y, r2 = r.z, r3 = r.w;
+    order2(k0, r0, k1, r1);
+    order2(k2, r2, k3, r3);
+    order2(k0, r0, k2, r2);
+    order2(k1, r1, k3, r3);
+    order2(k1, r1, k2, r2);
+    if (k3 != inf) st.push(r3);
+    if (k2 != inf) st.push(r2);
+    if (k1 != inf) st.push(r1);
+    next = r0;
+    return k0 != inf;
+}
+
+// Node kinds of the traversal template parameter W.
+enum : int { kTravPairs = 0, kTravWide = 1, kTravQuant = 2 };
+
+// One inner visit of either tree: W selects the child-pair, wide or
+// quantised wide nodes.
+template <int W, bool CN>
+__device__ __forceinline__ bool visit_node(const CNode* cn, const WNode* wn, const QNode* qn, uint32_t ref, v3 o,
+                                           v3 inv, float limit, LdsStack& st, Counters& c, uint32_t& next) {
+    if (W == kTravQuant) return visit_qwide<CN>(qn, wn, ref, o, inv, limit, st, c, next);
+    if (W == kTravWide) return visit_wide<CN>(wn, ref, o, inv, limit, st, c, next);
     return visit_inner<CN>(cn, ref, o, inv, limit, st, c, next);
 }
-template <bool W>
+template <int W>
 __device__ __forceinline__ uint32_t root_ref(const Root& r) { return W ? r.wref : r.ref; }
 
 // BLAS-only closest-hit traversal (list meshes, shader.wgsl:767-771).
-template <bool CN, bool W>
+template <bool CN, int W>
 __device__ void blas_closest(const DevScene& S, uint32_t root_slot, const float* inv_m, uint32_t tri_off, v3 o, v3 d,
                              int idx, int kind, LdsStack& st, HitRec& h, Counters& c) {
     const v3 lo = xform_point(inv_m, o), ld = xform_dir(inv_m, d);
@@ -156,7 +205,7 @@ __device__ void blas_closest(const DevScene& S, uint32_t root_slot, const float*
                 }
             }
         } else {
-            have = visit_node<W, CN>(S.bnodes, S.wbnodes, ref, lo, linv, h.t, st, c, ref);
+            have = visit_node<W, CN>(S.bnodes, S.wbnodes, S.qbnodes, ref, lo, linv, h.t, st, c, ref);
         }
         if (!have) {
             if (st.sp == 0) break;
@@ -165,7 +214,7 @@ __device__ void blas_closest(const DevScene& S, uint32_t root_slot, const float*
     }
 }
 
-template <bool CN, bool W>
+template <bool CN, int W>
 __device__ bool blas_any(const DevScene& S, uint32_t root_slot, const float* inv_m, uint32_t tri_off, v3 o, v3 d,
                          float tdist, LdsStack& st, Counters& c, float* tr = nullptr, int mat = -1) {
     const v3 lo = xform_point(inv_m, o), ld = xform_dir(inv_m, d);
@@ -189,7 +238,7 @@ __device__ bool blas_any(const DevScene& S, uint32_t root_slot, const float* inv
                 if (t >= 0.0f && t < tdist && (!tr || tri_shadow(S, mat, f2u(g[0].w), u, v, *tr))) return true;
             }
         } else {
-            have = visit_node<W, CN>(S.bnodes, S.wbnodes, ref, lo, linv, tdist, st, c, ref);
+            have = visit_node<W, CN>(S.bnodes, S.wbnodes, S.qbnodes, ref, lo, linv, tdist, st, c, ref);
         }
         if (!have) {
             if (st.sp == 0) break;
@@ -214,7 +263,7 @@ __device__ bool blas_any(const DevScene& S, uint32_t root_slot, const float* inv
 // stops it (evaluate_shadow_attenuation, shader.wgsl:777-792), so the
 // transmittance is 0 or 1; with them (kTranslucentBit) a textured surface
 // multiplies `trans` by (1 - alpha) and the walk goes on.
-template <uint32_t F, bool CN, bool W>
+template <uint32_t F, bool CN, int W>
 struct Trav {
     v3 o, d, inv;     // world-space ray
     v3 lo, ld, linv;  // the ray in the current level's space
@@ -451,8 +500,8 @@ struct Trav {
             }
         } else {
             // one call site for both levels: (lo, linv) is the ray in the current level's space
-            have = visit_node<W, CN>(in_blas ? S.bnodes : S.tnodes, in_blas ? S.wbnodes : S.wtnodes, ref, lo, linv,
-                                     h.t, st, c, ref);
+            have = visit_node<W, CN>(in_blas ? S.bnodes : S.tnodes, in_blas ? S.wbnodes : S.wtnodes,
+                                     in_blas ? S.qbnodes : S.qtnodes, ref, lo, linv, h.t, st, c, ref);
         }
         return true;
     }
@@ -474,7 +523,7 @@ struct Trav {
     }
 };
 
-template <uint32_t F, bool CN, bool W>
+template <uint32_t F, bool CN, int W>
 __device__ void closest_hit(const DevScene& S, Flags<F> fl, v3 o, v3 d, LdsStack& st, HitRec& h, Counters& c) {
     Trav<F, CN, W> q;
     if (q.begin_closest(S, fl, o, d, st, h))
@@ -486,7 +535,7 @@ __device__ void closest_hit(const DevScene& S, Flags<F> fl, v3 o, v3 d, LdsStack
 // trace_scene_shadow (shader.wgsl:1206-1229): the light that reaches tdist
 // past the occluders (object `target` skipped): 0 when one stops it, else the
 // product of the translucent surfaces' (1 - alpha), 1 without any.
-template <uint32_t F, bool CN, bool W>
+template <uint32_t F, bool CN, int W>
 __device__ float transmittance(const DevScene& S, Flags<F> fl, v3 o, v3 d, int target, float tdist, LdsStack& st,
                                Counters& c) {
     Trav<F, CN, W> q;
@@ -500,7 +549,7 @@ __device__ float transmittance(const DevScene& S, Flags<F> fl, v3 o, v3 d, int t
 
 // trace_scene_shadow (shader.wgsl:1206-1229): true when an occluder lies
 // before tdist (object `target` skipped).
-template <uint32_t F, bool CN, bool W>
+template <uint32_t F, bool CN, int W>
 __device__ bool occluded(const DevScene& S, Flags<F> fl, v3 o, v3 d, int target, float tdist, LdsStack& st,
                          Counters& c) {
     Trav<F, CN, W> q;

@@ -123,8 +123,21 @@ int validate_desc(pt_ctx* c, const pt_scene_desc* d) {
     return PT_OK;
 }
 
-// Checks that every triangle range a BLAS leaf can reach stays inside the
-// triangle array, for a given tri_offset.
+// The quantised copies of both wide trees (PT_TRAVERSAL_WIDE_QUANTIZED), made
+// on the device after every upload, build and update.
+int quantize_trees(pt_ctx* c) {
+    int rc;
+    if ((rc = c->alloc(c->qtnodes, std::max<uint64_t>(c->n_wtn, 1) * sizeof(QNode), "quantised tlas nodes")) ||
+        (rc = c->alloc(c->qbnodes, std::max<uint64_t>(c->n_wbn, 1) * sizeof(QNode), "quantised bvh nodes")))
+        return rc;
+    hipError_t e = quantize_wide_nodes(static_cast<const WNode*>(c->wtnodes.p), static_cast<QNode*>(c->qtnodes.p),
+                                       uint32_t(c->n_wtn), c->stream);
+    if (e == hipSuccess)
+        e = quantize_wide_nodes(static_cast<const WNode*>(c->wbnodes.p), static_cast<QNode*>(c->qbnodes.p),
+                                uint32_t(c->n_wbn), c->stream);
+    return e == hipSuccess ? PT_OK : c->hip_fail(e, "node quantisation");
+}
+
 void refresh_dev(pt_ctx* c) {
     DevScene& S = c->dev;
     S.mats = static_cast<const pt_material*>(c->mats.p);
@@ -134,6 +147,8 @@ void refresh_dev(pt_ctx* c) {
     S.bnodes = static_cast<const CNode*>(c->bnodes.p);
     S.wtnodes = static_cast<const WNode*>(c->wtnodes.p);
     S.wbnodes = static_cast<const WNode*>(c->wbnodes.p);
+    S.qtnodes = static_cast<const QNode*>(c->qtnodes.p);
+    S.qbnodes = static_cast<const QNode*>(c->qbnodes.p);
     S.roots = static_cast<const Root*>(c->roots.p);
     S.tlas_raw = static_cast<const pt_bvh_node*>(c->tlas_raw.p);
     S.bvh_raw = static_cast<const pt_bvh_node*>(c->bvh_raw.p);
@@ -228,12 +243,14 @@ int upload_dynamic(pt_ctx* c, const pt_scene_desc* d, bool first) {
         c->blas_wneed = L.blas_wneed;
         if ((rc = c->upload(c->bnodes, L.bn.data(), L.bn.size() * sizeof(CNode), "bvh nodes"))) return rc;
         if ((rc = c->upload(c->wbnodes, L.wbn.data(), L.wbn.size() * sizeof(WNode), "wide bvh nodes"))) return rc;
+        c->n_wbn = L.wbn.size();
     }
     c->tlas_depth = td;
     c->tlas_wneed = L.tlas_wneed;
     c->host_roots = roots;
     if ((rc = c->upload(c->tnodes, L.tn.data(), L.tn.size() * sizeof(CNode), "tlas nodes"))) return rc;
     if ((rc = c->upload(c->wtnodes, L.wtn.data(), L.wtn.size() * sizeof(WNode), "wide tlas nodes"))) return rc;
+    c->n_wtn = L.wtn.size();
     if ((rc = c->upload(c->roots, roots.data(), roots.size() * sizeof(Root), "roots"))) return rc;
     if ((rc = c->upload(c->obj_root, L.obj_root.data(), L.obj_root.size() * 4, "object roots"))) return rc;
     if ((rc = c->upload(c->mesh_root, L.mesh_root.data(), L.mesh_root.size() * 4, "mesh roots"))) return rc;
@@ -350,6 +367,7 @@ int device_blas(pt_ctx* c, const pt_scene_desc* d) {
     }
     c->wbnodes.release();
     c->wbnodes = wn;
+    c->n_wbn = total;
     c->blas_wneed = need;
     c->wstack_cap = wneed;
     return c->upload(c->roots, c->host_roots.data(), c->host_roots.size() * sizeof(Root), "roots");
@@ -388,6 +406,7 @@ int device_tlas(pt_ctx* c, const pt_scene_desc* d) {
     if (need > kMaxStack || wneed > kMaxStack)
         return c->fail(PT_ERR_UNSUPPORTED, "device-built TLAS depth %u needs more than %u stack entries", res.depth,
                        kMaxStack);
+    c->n_wtn = nodes;
     c->host_roots[0] = res.root;
     c->tlas_depth = res.depth;
     c->tlas_wneed = res.need;
@@ -458,7 +477,7 @@ int pt_set_option(pt_ctx* c, int key, int64_t value) {
             c->lds_stack_opt = uint32_t(value);
             return PT_OK;
         case PT_OPT_TRAVERSAL:
-            if (value != PT_TRAVERSAL_REFERENCE && value != PT_TRAVERSAL_WIDE)
+            if (value != PT_TRAVERSAL_REFERENCE && value != PT_TRAVERSAL_WIDE && value != PT_TRAVERSAL_WIDE_QUANTIZED)
                 return c->fail(PT_ERR_ARG, "unknown traversal %lld", (long long)value);
             c->traversal = int(value);
             return PT_OK;
@@ -575,7 +594,7 @@ int pt_upload_scene(pt_ctx* c, const pt_scene_desc* d) {
         c->sky_total_weight = total;
         c->sky_total_lum = float(total);
     }
-    if ((rc = c->sync("upload"))) return rc;
+    if ((rc = quantize_trees(c)) || (rc = c->sync("upload"))) return rc;
     refresh_dev(c);
     c->has_scene = true;
     return PT_OK;
@@ -597,6 +616,7 @@ int pt_update_objects(pt_ctx* c, const pt_scene_desc* d) {
     c->has_textures = uses_textures(d);  // the texels stay those of the upload
     c->translucent = translucent_albedo(c, d);
     if (c->tlas_build == PT_TLAS_BUILD_DEVICE && (rc = device_tlas(c, d))) return rc;
+    if ((rc = quantize_trees(c))) return rc;
     refresh_dev(c);
     return c->sync("update");
 }
@@ -748,7 +768,7 @@ int prepare_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t sh,
     a.hits = hits_dev;
     a.n_rec = n_rec;
     a.schedule = c->schedule;
-    a.wide = c->traversal == PT_TRAVERSAL_WIDE;
+    a.wide = c->traversal;
     plan.wavefront = c->backend == PT_BACKEND_WAVEFRONT && !hits_dev;
     if (plan.wavefront) return prepare_wavefront(c, plan);
     plan.cn = c->count_stats && !hits_dev;
@@ -985,7 +1005,7 @@ int pt_trace_rays(pt_ctx* c, const float* rays, uint32_t n, pt_hit* hits) {
     a.rays = static_cast<const float*>(c->rays_in.p);
     a.n = n;
     a.hits = static_cast<pt_hit*>(c->rays_out.p);
-    a.wide = c->traversal == PT_TRAVERSAL_WIDE;
+    a.wide = c->traversal;
     hipError_t e = launch_rays(c->dev, a, false, c->stream);
     if (e != hipSuccess) return c->hip_fail(e, "rays launch");
     if ((rc = c->sync("rays"))) return rc;
@@ -1013,7 +1033,7 @@ int pt_trace_shadow_rays(pt_ctx* c, const float* rays, const int32_t* target_idx
     a.tdist = static_cast<const float*>(c->rays_aux2.p);
     a.n = n;
     a.trans = static_cast<float*>(c->rays_out.p);
-    a.wide = c->traversal == PT_TRAVERSAL_WIDE;
+    a.wide = c->traversal;
     hipError_t e = launch_rays(c->dev, a, true, c->stream);
     if (e != hipSuccess) return c->hip_fail(e, "shadow rays launch");
     if ((rc = c->sync("shadow rays"))) return rc;

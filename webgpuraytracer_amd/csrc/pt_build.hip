@@ -590,4 +590,16 @@ hipError_t build_tlas_device(const pt_object* objs, uint32_t n, const uint32_t* 
     return hipGetLastError();
 }
 
+// ------------------------------------------------------- quantised nodes ----
+__global__ void __launch_bounds__(kBlk) quantize_k(const WNode* __restrict__ in, QNode* __restrict__ out, uint32_t n) {
+    const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
+    if (i < n) out[i] = quantize_node(in[i]);
+}
+
+hipError_t quantize_wide_nodes(const WNode* in, QNode* out, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(quantize_k, dim3((n + kBlk - 1) / kBlk), dim3(kBlk), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
 }  // namespace pt

@@ -67,6 +67,8 @@ struct pt_ctx {
     bool translucent = false;        // some albedo texture has an alpha below 1
     std::vector<uint8_t> layer_alpha;  // per uploaded texture layer: some alpha below 255
     DevBuf tri_geo_w;                          // device-built trees: triangle records in their leaf order
+    DevBuf qtnodes, qbnodes;                   // the wide trees with quantised boxes
+    uint64_t n_wtn = 0, n_wbn = 0;             // wide nodes in wtnodes / wbnodes
     pt::DevScene dev{};
     // frame
     uint32_t width = 0, height = 0;
@@ -121,7 +123,7 @@ struct pt_ctx {
     void release_all() {
         DevBuf* all[] = {&mats, &objs, &obj_root, &tnodes, &bnodes, &wtnodes, &wbnodes, &roots, &tlas_raw, &bvh_raw, &tri_geo, &tris,
                          &planes, &meshes, &mesh_root, &lights, &sky, &cdf, &cdf_rows, &tri_geo_w, &accum, &out, &gbuf, &dn_a,
-                         &dn_b, &dn_fin, &dn_guide, &stats, &seeds, &rays_in, &rays_aux, &rays_aux2, &rays_out,
+                         &dn_b, &dn_fin, &dn_guide, &stats, &seeds, &rays_in, &rays_aux, &rays_aux2, &rays_out, &qtnodes, &qbnodes,
                          &wf_o, &wf_d, &wf_t, &wf_r, &wf_meta, &wf_ha, &wf_hb, &wf_media, &wf_q0, &wf_q1, &wf_sq,
                          &wf_so, &wf_sd, &wf_stp, &wf_sfi, &wf_cnt, &wf_em, &mk_media, &mk_tiles, &mk_spill, &wf_spill,
                          &tex};

@@ -25,7 +25,7 @@ struct TraceArgs {
     float* rec_rays;             // with hits: each recorded bounce's ray, 6 floats (NULL: not kept)
     uint32_t n_rec;
     int schedule;                // PT_SCHEDULE_*
-    int wide;                    // 1: wide traversal (PT_TRAVERSAL_WIDE)
+    int wide;                    // the traversal: PT_TRAVERSAL_* (0 child pairs, 1 wide, 2 wide quantised)
     uint32_t n_tiles;            // 8x8 tiles of the section
     uint32_t* tile_ctr;          // 8 per-XCD band counters, zero at launch
     float* media;                // kMediaFloats per lane of the grid, structure of arrays
@@ -41,7 +41,7 @@ struct RayArgs {
     uint32_t n;
     pt_hit* hits;
     float* trans;
-    int wide;                    // 1: wide traversal (PT_TRAVERSAL_WIDE)
+    int wide;                    // the traversal: PT_TRAVERSAL_* (0 child pairs, 1 wide, 2 wide quantised)
 };
 
 struct DenoiseArgs {
@@ -88,7 +88,7 @@ struct WfArgs {
     float4* sh_em;      // per path, height-mapped scenes only (else NULL): emission of a surface the height
                         // field blocked, to be added after the shadow slots; w = 1 while pending
     uint32_t* counters; // queue A length, queue B length, shadow queue length, two fetch cursors
-    int wide;           // 1: wide traversal (PT_TRAVERSAL_WIDE)
+    int wide;           // the traversal: PT_TRAVERSAL_* (0 child pairs, 1 wide, 2 wide quantised)
     uint32_t trace_grid;    // persistent traversal launches: workgroups (all resident)
     uint32_t refill;        // dynamic fetch: idle lanes of a wave that trigger a refill
     uint32_t stack_lds;     // traversal stack entries per lane in LDS
@@ -144,12 +144,14 @@ hipError_t build_tlas_device(const pt_object* objs, uint32_t n, const uint32_t* 
                              void* scratch, size_t scratch_bytes, CNode* cn, WNode* wn, uint32_t* ctr,
                              TlasBuildResult* res, hipStream_t stream);
 hipError_t launch_wavefront_batch(const DevScene& S, const WfArgs& a, bool count_nodes, hipStream_t stream);
+// QNodes from WNodes, index for index (pt_build.hip).
+hipError_t quantize_wide_nodes(const WNode* in, QNode* out, uint32_t n, hipStream_t stream);
 // True when the trace kernels have a variant compiled for this kernel flag
 // word (both backends dispatch on the same sets); otherwise the run-time-flags
 // variant runs.
 bool specialised_flag_set(uint32_t flags);
 // Resident workgroups per CU of the wavefront traversal kernels (the larger of the two).
-hipError_t wf_trace_blocks_per_cu(const DevScene& S, bool count_nodes, bool wide, uint32_t stack_lds,
+hipError_t wf_trace_blocks_per_cu(const DevScene& S, bool count_nodes, int wide, uint32_t stack_lds,
                                   int* blocks_per_cu);
 
 }  // namespace pt

@@ -76,7 +76,7 @@ __device__ void store_hit(pt_hit* o, const HitRec& h, const SurfHit& s) {
 // Megakernel NEE sink: traces the shadow ray at once and adds the
 // contribution, exactly where evaluate_nee's result is added
 // (shader.wgsl:2440, 2452).
-template <uint32_t F, bool CN, bool W>
+template <uint32_t F, bool CN, int W>
 struct ImmediateNee {
     const DevScene& S;
     Flags<F> fl;
@@ -101,7 +101,7 @@ struct ImmediateNee {
 // One path of shader.wgsl `main` (2279-2480).  rec: when non-null, the first
 // n_rec trace_scene hits of the path are written there, and when rec_rays is
 // non-null the ray each of them traced (origin, direction).
-template <uint32_t F, bool CN, bool REC, bool W>
+template <uint32_t F, bool CN, bool REC, int W>
 __device__ v3 trace_path(const DevScene& S, Flags<F> fl, const pt_scene_params& PP, const Params& P, uint32_t px,
                          uint32_t py, uint32_t sample_number, uint32_t seed, LdsStack& st, float* media,
                          uint32_t media_stride, Counters& c, pt_hit* rec, float* rec_rays, uint32_t n_rec) {
@@ -177,7 +177,7 @@ __device__ uint32_t next_tile(uint32_t* ctr, uint32_t n_tiles) {
 #ifndef PT_TRACE_WAVES_PER_EU
 #define PT_TRACE_WAVES_PER_EU 5
 #endif
-template <uint32_t F, bool CN, int MODE, bool W>
+template <uint32_t F, bool CN, int MODE, int W>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(PT_TRACE_WAVES_PER_EU)))
 trace_kernel(DevScene S, TraceArgs A) {
     extern __shared__ uint32_t lds_stack[];
@@ -272,7 +272,7 @@ __global__ void __launch_bounds__(kWave) gbuffer_kernel(DevScene S, pt_scene_par
     float rough = 0.0f, metal = 0.0f, trans = 0.0f, dist = 0.0f;
     for (int bounce = 0; bounce < S.bounce_limit; ++bounce) {
         HitRec h;
-        closest_hit<F, false, false>(S, fl, ro, rd, st, h, c);
+        closest_hit<F, false, kTravPairs>(S, fl, ro, rd, st, h, c);
         const SurfHit sh = resolve_for_shading<F>(S, fl, ro, rd, h);
         if (sh.m_idx == -1) {
             n = splat(0.0f);
@@ -338,7 +338,7 @@ __global__ void __launch_bounds__(kWave) gbuffer_kernel(DevScene S, pt_scene_par
     out[idx] = make_uchar4(unorm8(o.x), unorm8(o.y), unorm8(o.z), 255);
 }
 
-template <uint32_t F, bool W>
+template <uint32_t F, int W>
 __global__ void __launch_bounds__(kWave) rays_kernel(DevScene S, RayArgs A, int shadow) {
     extern __shared__ uint32_t lds_stack[];
     const Flags<F> fl{S.flags};
@@ -637,7 +637,7 @@ static size_t stack_bytes(const DevScene& S, uint32_t threads, bool wide) {
     return size_t(wide ? S.wstack_cap : S.stack_cap) * threads * 4u;
 }
 
-template <uint32_t F, bool CN, int MODE, bool W>
+template <uint32_t F, bool CN, int MODE, int W>
 static hipError_t trace_variant(const DevScene& S, const TraceArgs& a, uint32_t grid, int* blocks_per_cu,
                                 hipStream_t s) {
     const size_t lds = size_t(a.stack_lds) * kWave * 4u;
@@ -650,7 +650,7 @@ static hipError_t trace_variant(const DevScene& S, const TraceArgs& a, uint32_t 
 // Variants: hit recording and work counting use the reference-order
 // traversal (they report the reference's per-pop counts); rendering uses the
 // traversal the caller chose.
-template <uint32_t F, bool CN, bool W>
+template <uint32_t F, bool CN, int W>
 static hipError_t trace_sched(const DevScene& S, const TraceArgs& a, uint32_t grid, int* bpc, hipStream_t s) {
     if (a.schedule == PT_SCHEDULE_REGEN) return trace_variant<F, CN, kRegen, W>(S, a, grid, bpc, s);
     return trace_variant<F, CN, kSync, W>(S, a, grid, bpc, s);
@@ -658,10 +658,11 @@ static hipError_t trace_sched(const DevScene& S, const TraceArgs& a, uint32_t gr
 
 template <uint32_t F>
 static hipError_t trace_f(const DevScene& S, const TraceArgs& a, bool cn, uint32_t grid, int* bpc, hipStream_t s) {
-    if (a.hits) return trace_variant<F, false, kRecord, false>(S, a, grid, bpc, s);
-    if (cn) return trace_sched<F, true, false>(S, a, grid, bpc, s);
-    if (a.wide) return trace_sched<F, false, true>(S, a, grid, bpc, s);
-    return trace_sched<F, false, false>(S, a, grid, bpc, s);
+    if (a.hits) return trace_variant<F, false, kRecord, kTravPairs>(S, a, grid, bpc, s);
+    if (cn) return trace_sched<F, true, kTravPairs>(S, a, grid, bpc, s);
+    if (a.wide == kTravQuant) return trace_sched<F, false, kTravQuant>(S, a, grid, bpc, s);
+    if (a.wide == kTravWide) return trace_sched<F, false, kTravWide>(S, a, grid, bpc, s);
+    return trace_sched<F, false, kTravPairs>(S, a, grid, bpc, s);
 }
 
 static hipError_t trace_dispatch(const DevScene& S, const TraceArgs& a, bool cn, uint32_t grid, int* bpc,
@@ -697,12 +698,16 @@ hipError_t launch_gbuffer(const DevScene& S, const pt_scene_params& P, pt_gbuffe
 hipError_t launch_rays(const DevScene& S, const RayArgs& a, bool shadow, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     const uint32_t blocks = (a.n + kWave - 1) / kWave;
-    if (a.wide)
-        hipLaunchKernelGGL((rays_kernel<kDynamicFlags, true>), dim3(blocks), dim3(kWave), stack_bytes(S, kWave, true), s,
-                           S, a, shadow ? 1 : 0);
+    const size_t lds = stack_bytes(S, kWave, a.wide != kTravPairs);
+    if (a.wide == kTravQuant)
+        hipLaunchKernelGGL((rays_kernel<kDynamicFlags, kTravQuant>), dim3(blocks), dim3(kWave), lds, s, S, a,
+                           shadow ? 1 : 0);
+    else if (a.wide == kTravWide)
+        hipLaunchKernelGGL((rays_kernel<kDynamicFlags, kTravWide>), dim3(blocks), dim3(kWave), lds, s, S, a,
+                           shadow ? 1 : 0);
     else
-        hipLaunchKernelGGL((rays_kernel<kDynamicFlags, false>), dim3(blocks), dim3(kWave), stack_bytes(S, kWave, false),
-                           s, S, a, shadow ? 1 : 0);
+        hipLaunchKernelGGL((rays_kernel<kDynamicFlags, kTravPairs>), dim3(blocks), dim3(kWave), lds, s, S, a,
+                           shadow ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -88,7 +88,7 @@ extern "C" int pt_trace_noalloc(pt_ctx* c, const pt_scene_params* p, uint32_t sw
     a.stats = reinterpret_cast<unsigned long long*>(base + kOffStats);
     a.tile_ctr = reinterpret_cast<uint32_t*>(base + kOffTiles);
     a.schedule = c->schedule;
-    a.wide = c->traversal == PT_TRAVERSAL_WIDE;
+    a.wide = c->traversal;
     a.n_tiles = ((a.sec_w + 7u) / 8u) * ((a.sec_h + 7u) / 8u);
     // the same kernel variant pt_trace's megakernel path picks
     const bool cn = c->count_stats;

@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(kWfBlock) wf_raygen(WfArgs A) {
 #ifndef PT_WF_TRACE_WAVES_PER_EU
 #define PT_WF_TRACE_WAVES_PER_EU 6
 #endif
-template <uint32_t F, bool CN, bool W, bool SHADOW>
+template <uint32_t F, bool CN, int W, bool SHADOW>
 __global__ void __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(PT_WF_TRACE_WAVES_PER_EU)))
 wf_trace(DevScene S, WfArgs A, int cur) {
     extern __shared__ uint32_t lds_stack[];
@@ -372,7 +372,7 @@ __global__ void __launch_bounds__(kWfBlock) wf_accumulate(WfArgs A) {
     if (A.stats && lp == 0) atomicAdd(A.stats + kStPaths, (unsigned long long)A.n_items);
 }
 
-template <uint32_t F, bool CN, bool W>
+template <uint32_t F, bool CN, int W>
 static void wf_bounces(const DevScene& S, const WfArgs& a0, uint32_t blocks, hipStream_t s) {
     const size_t lds = size_t(a0.stack_lds) * kWfBlock * 4u;
     int cur = 0;
@@ -390,38 +390,36 @@ static hipError_t wf_run(const DevScene& S, const WfArgs& a0, bool cn, hipStream
     const uint32_t blocks = (a0.n_items + kWfBlock - 1) / kWfBlock;
     hipLaunchKernelGGL(wf_raygen, dim3(blocks), dim3(kWfBlock), 0, s, a0);
     if (cn)
-        wf_bounces<F, true, false>(S, a0, blocks, s);
-    else if (a0.wide)
-        wf_bounces<F, false, true>(S, a0, blocks, s);
+        wf_bounces<F, true, kTravPairs>(S, a0, blocks, s);
+    else if (a0.wide == kTravQuant)
+        wf_bounces<F, false, kTravQuant>(S, a0, blocks, s);
+    else if (a0.wide == kTravWide)
+        wf_bounces<F, false, kTravWide>(S, a0, blocks, s);
     else
-        wf_bounces<F, false, false>(S, a0, blocks, s);
+        wf_bounces<F, false, kTravPairs>(S, a0, blocks, s);
     const uint32_t npix = a0.sec_w * a0.sec_h;
     hipLaunchKernelGGL(wf_accumulate, dim3((npix + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, s, a0);
     return hipGetLastError();
 }
 
-template <uint32_t F>
-static hipError_t wf_occupancy_f(bool cn, bool wide, size_t lds, int* bpc) {
+template <uint32_t F, bool CN, int W>
+static hipError_t wf_occupancy_w(size_t lds, int* bpc) {
     int a = 0, b = 0;
-    hipError_t e;
-    if (cn) {
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, wf_trace<F, true, false, false>, kWfBlock, lds);
-        if (e == hipSuccess)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, wf_trace<F, true, false, true>, kWfBlock, lds);
-    } else if (wide) {
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, wf_trace<F, false, true, false>, kWfBlock, lds);
-        if (e == hipSuccess)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, wf_trace<F, false, true, true>, kWfBlock, lds);
-    } else {
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, wf_trace<F, false, false, false>, kWfBlock, lds);
-        if (e == hipSuccess)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, wf_trace<F, false, false, true>, kWfBlock, lds);
-    }
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, wf_trace<F, CN, W, false>, kWfBlock, lds);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, wf_trace<F, CN, W, true>, kWfBlock, lds);
     *bpc = std::max(a, b);
     return e;
 }
 
-hipError_t wf_trace_blocks_per_cu(const DevScene& S, bool count_nodes, bool wide, uint32_t stack_lds,
+template <uint32_t F>
+static hipError_t wf_occupancy_f(bool cn, int wide, size_t lds, int* bpc) {
+    if (cn) return wf_occupancy_w<F, true, kTravPairs>(lds, bpc);
+    if (wide == kTravQuant) return wf_occupancy_w<F, false, kTravQuant>(lds, bpc);
+    if (wide == kTravWide) return wf_occupancy_w<F, false, kTravWide>(lds, bpc);
+    return wf_occupancy_w<F, false, kTravPairs>(lds, bpc);
+}
+
+hipError_t wf_trace_blocks_per_cu(const DevScene& S, bool count_nodes, int wide, uint32_t stack_lds,
                                   int* blocks_per_cu) {
     const size_t lds = size_t(stack_lds) * kWfBlock * 4u;
     if (S.flags == kFlagsC1) return wf_occupancy_f<kFlagsC1>(count_nodes, wide, lds, blocks_per_cu);
