@@ -35,6 +35,14 @@ int pt_unet_param_count(pt_ctx* ctx, uint64_t* count);
 int pt_unet_get_params(pt_ctx* ctx, float* out, uint64_t count);
 int pt_unet_set_params(pt_ctx* ctx, const float* in, uint64_t count); /* also clears the Adam state */
 
+/* The Adam state, to resume training in another process: the first and
+ * second moments (param_count floats each, laid out as the parameters) and
+ * the number of steps taken.  pt_unet_set_adam after pt_unet_set_params
+ * restores a run exactly: the next pt_unet_train gives the same bits as the
+ * run that saved it. */
+int pt_unet_get_adam(pt_ctx* ctx, float* m, float* v, uint64_t count, uint32_t* steps);
+int pt_unet_set_adam(pt_ctx* ctx, const float* m, const float* v, uint64_t count, uint32_t steps);
+
 /* Inference on host features: pred [n][h][w][3] (log space). */
 int pt_unet_forward(pt_ctx* ctx, const float* feat, uint32_t n, uint32_t h, uint32_t w, float* pred);
 
@@ -69,10 +77,10 @@ int pt_unet_denoise(pt_ctx* ctx, float exposure);
 
 /* Inference precision of pt_unet_denoise and pt_unet_forward.  PT_UNET_FP8
  * calibrates on the context's current frame (accumulation buffer + G-buffer,
- * as pt_unet_denoise reads it) and from then on runs the 17 inner
- * convolutions on fp8 products (OCP e4m3, fp32 accumulation; the first and
- * the output convolution stay bf16): each layer input is quantized
- * per channel as x / s_a[ci], s_a = a / 224 where a is the calibration
+ * as pt_unet_denoise reads it) and from then on runs 16 of the 19
+ * convolutions on fp8 products (OCP e4m3, fp32 accumulation; the first, the
+ * last decoder and the output convolution stay bf16): each layer input is
+ * quantized per channel as x / s_a[ci], s_a = a / 224 where a is the calibration
  * frame's 99.995th percentile of |x| (two bf16 forward passes: the maxima,
  * then 1024-bin histograms up to them); one binade of headroom, larger values
  * saturate at 448; the weights are
@@ -84,7 +92,7 @@ enum { PT_UNET_BF16 = 0, PT_UNET_FP8 = 1 };
 int pt_unet_set_precision(pt_ctx* ctx, int precision);
 
 /* The fp8 state (PT_UNET_FP8 only), all 19 layers in network order (the
- * first and last layers' entries are computed but not used): the input
+ * entries of the three bf16 layers are computed but not used): the input
  * scales s_a (sum of the layers' input channels, 1472 floats), the weight
  * scales s_w (sum of output channels, 1232 floats) and the e4m3 weights
  * [9][cout][cin] per layer (1,020,672 bytes), with the padded channel counts

@@ -125,8 +125,8 @@ def forward_fp8(params, feat, act_scales, w_scales, w8):
     input (bf16) quantized per channel as e4m3(clamp(x * (1 / s_a), +-448)), the
     e4m3 weights (bytes, [9][cout][cin] per layer), the fp32 product times
     s_w[co] plus the bias, ReLU, stored as bf16 (pooling averages bf16 values
-    into bf16); the first and the output convolution run on bf16 weights and
-    the last layer's output stays fp32.  params: unflatten()
+    into bf16); the first, the last decoder and the output convolution run on
+    bf16 weights and the last layer's output stays fp32.  params: unflatten()
     output (for the biases); returns [n][3][h][w] (log space)."""
     table = layer_table()
     w8 = torch.from_numpy(np.asarray(w8, np.uint8).copy())
@@ -146,7 +146,7 @@ def forward_fp8(params, feat, act_scales, w_scales, w8):
         offs[1] += co
         offs[2] += 9 * co * ci
         last = li == len(table) - 1
-        if li == 0 or last:  # the first and the output convolution stay bf16
+        if li in (0, len(table) - 2) or last:  # the first, the last decoder and the output convolution stay bf16
             y = F.conv2d(t, _bf16(params[li][0]), params[li][1], padding=1)
         else:
             xq = _e4m3(t * inv[None, :, None, None])

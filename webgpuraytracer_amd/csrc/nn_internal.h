@@ -33,8 +33,27 @@ constexpr uint32_t kMaxCout = 128;
 // multiples of 16 (cout <= kMaxCout); w is bf16 [9][cout][cin].  relu: ReLU;
 // out_f32 (when non-null) receives the first `f32_channels` channels as fp32
 // [pixel][f32_channels] instead of `out`.  bias may be null.
+// Work done with a convolution's output (in its epilogue where the output
+// tile is staged, else by the separate kernel after it):
+//   pool.p: (bf16 output, d.h and d.w even) also the 2x2 average pool of the
+//           output, as pool2 writes it;
+//   upw.p:  (bf16 output) also its nearest 2x upsampling (dims 2d), as up2
+//           writes it;
+//   fin:    (fp32 output of 3 channels, one image) the denoised frame texels
+//           as unpack_frame writes them, for the fw x fh frame, instead of
+//           out_f32 (which is still required, and written when the texels
+//           cannot come from the kernel's stage).
+struct ConvFuse {
+    Tens pool;
+    Tens upw;
+    float4* fin;
+    uchar4* ldr;
+    uint32_t fw, fh;
+    float exposure;
+};
 hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bias, const Tens& out, uint32_t cout,
-                   Dims d, bool relu, float* out_f32, uint32_t f32_channels, hipStream_t s);
+                   Dims d, bool relu, float* out_f32, uint32_t f32_channels, hipStream_t s,
+                   const ConvFuse* fuse = nullptr);
 
 // The same convolution for inference on fp8 (OCP e4m3) operands with fp32
 // accumulation: the input (bf16, as above) is quantized per input channel as

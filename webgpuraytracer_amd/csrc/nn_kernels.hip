@@ -33,13 +33,31 @@ __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, u3
 __device__ __forceinline__ float f(bf16 x) { return float(x); }
 __device__ __forceinline__ bf16 b(float x) { return bf16(x); }
 
+__device__ __forceinline__ unsigned char unorm8n(float v) {
+    const float c = fminf(fmaxf(v, 0.0f), 1.0f);
+    return (unsigned char)(c * 255.0f + 0.5f);
+}
+
+// The denoised texel from the network's log-space prediction q[0..2]:
+// radiance expm1(q) (at least 0) into *fin and, when ldr is set, the display
+// tonemap (x e / (x e + 0.3))^0.4545 as rgba8.
+__device__ __forceinline__ void frame_texel(const float* q, float exposure, float4* fin, uchar4* ldr) {
+    const float r = fmaxf(expm1f(q[0]), 0.0f), g = fmaxf(expm1f(q[1]), 0.0f), bl = fmaxf(expm1f(q[2]), 0.0f);
+    *fin = make_float4(r, g, bl, 1.0f);
+    if (ldr) {
+        float o[3] = {r * exposure, g * exposure, bl * exposure};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) o[k] = powf(o[k] / (o[k] + 0.3f), 0.4545f);
+        *ldr = make_uchar4(unorm8n(o[0]), unorm8n(o[1]), unorm8n(o[2]), 255);
+    }
+}
+
 // ------------------------------------------------------------- conv fwd --
 // A block computes a 16 x 16 pixel tile of one image for a group of up to
 // NCT x 16 output channels.  Per 32-channel slice of the input it stages the
 // 18 x 18 halo and the slice's weights in LDS, both stored by 8-channel
 // group ([group][pixel], [group][tap][co]) so the 16 lanes of a fragment row
-// read 256 consecutive bytes; the next slice is fetched into registers while
-// the current one is multiplied.  Wave w owns tile rows 4w .. 4w + 3 (one
+// read 256 consecutive bytes.  Wave w owns tile rows 4w .. 4w + 3 (one
 // 16-pixel MFMA row tile each), so per tap it reads 4 input fragments and NCT
 // weight fragments from LDS for 4 x NCT products.
 constexpr uint32_t kTile = 16, kHalo = 18, kHaloPix = kHalo * kHalo;  // 324
@@ -56,10 +74,17 @@ struct ConvSmem {
 // and the frame is a multiple of the 16 x 16 tile, so no guards are needed;
 // the bf16 output then goes through LDS ([pixel][channel], rows padded by 16
 // bytes against bank conflicts) and leaves as 16-byte stores.
+// With a FULL bf16 output the epilogue can also write (fz) the 2x2 average
+// pool of the output tile and its nearest 2x upsampling; with a FULL fp32
+// output, the denoised frame texels.  A slice is fetched, staged and
+// multiplied in turn: the staging registers are free during the products, so
+// the kernel needs half the VGPRs of one that fetches the next slice while
+// multiplying (twice the resident waves, which overlap one another's loads;
+// measured 4% faster per 1080p frame than prefetching where cin > 32).
 template <int NCT, bool RELU, bool F32OUT, bool FULL>
 __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const bf16* __restrict__ w,
                                                     const float* __restrict__ bias, Tens out, uint32_t cout, Dims d,
-                                                    float* __restrict__ of32, uint32_t f32c) {
+                                                    float* __restrict__ of32, uint32_t f32c, ConvFuse fz) {
     __shared__ ConvSmem<NCT> sm;
     constexpr uint32_t kWPieces = 9u * NCT * 16u * 4u;
     constexpr uint32_t kHaloPer = (kHaloPieces + kBlock - 1u) / kBlock;  // 6
@@ -115,12 +140,13 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
     for (int m = 0; m < 4; ++m)
 #pragma unroll
         for (int t = 0; t < NCT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    fetch(0);
     for (uint32_t c0 = 0; c0 < cin; c0 += kSlice) {
+        fetch(c0);
         __syncthreads();  // the previous slice's reads are done
         stash();
         __syncthreads();
-        if (c0 + kSlice < cin) fetch(c0 + kSlice);
+        // (reading the wave's 6 halo rows once per horizontal offset instead of
+        // 4 rows per tap measured ~5% slower: the loads then wait together)
 #pragma unroll
         for (uint32_t tap = 0; tap < 9u; ++tap) {
             const uint32_t dy = tap / 3u, dx = tap % 3u;
@@ -142,6 +168,42 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
         }
     }
     // C[(lane / 16) * 4 + r][lane % 16]: tile column kq * 4 + r, channel lane % 16
+    if (FULL && F32OUT) {
+        // the first f32c (<= 4) channels as fp32 [pixel][f32c]: a tile row is
+        // 16 * f32c consecutive floats, written as 16-byte pieces
+        float* stage = reinterpret_cast<float*>(&sm);
+        static_assert(kTile * kTile * 4u * 4u <= sizeof(ConvSmem<NCT>), "fp32 output stage fits the LDS tile");
+        __syncthreads();  // every wave is done with the last slice
+        const uint32_t ch = lane & 15u;
+        if (ch < f32c) {
+            const float bv = bias ? bias[co0 + ch] : 0.0f;
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    float v = acc[m][0][r] + bv;
+                    if (RELU) v = fmaxf(v, 0.0f);
+                    stage[((wv * 4u + uint32_t(m)) * kTile + kq * 4u + uint32_t(r)) * f32c + ch] = v;
+                }
+        }
+        __syncthreads();
+        if (fz.fin) {  // the frame's denoised texels straight from the stage (f32c = 3, one image)
+            const uint32_t x = uint32_t(tx0) + tid % kTile, y = uint32_t(ty0) + tid / kTile;
+            if (x < fz.fw && y < fz.fh) {
+                const size_t i = size_t(y) * fz.fw + x;
+                frame_texel(stage + tid * f32c, fz.exposure, fz.fin + i, fz.ldr ? fz.ldr + i : nullptr);
+            }
+            return;
+        }
+        const uint32_t per_row = kTile * f32c / 4u;  // 16-byte pieces per tile row
+        for (uint32_t i = tid; i < kTile * per_row; i += kBlock) {
+            const uint32_t row = i / per_row, k = i % per_row;
+            const size_t op = (size_t(img) * d.h + uint32_t(ty0) + row) * d.w + uint32_t(tx0);
+            *reinterpret_cast<float4*>(of32 + op * f32c + k * 4u) =
+                *reinterpret_cast<const float4*>(stage + row * kTile * f32c + k * 4u);
+        }
+        return;
+    }
     if (FULL && !F32OUT) {
         constexpr uint32_t kCo = uint32_t(NCT) * 16u, kPitch = kCo * 2u + 16u;  // bytes per staged pixel
         static_assert(kTile * kTile * kPitch <= sizeof(ConvSmem<NCT>), "output stage fits the LDS tile");
@@ -169,6 +231,35 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
             const size_t op = (size_t(img) * d.h + uint32_t(ty0) + p / kTile) * d.w + uint32_t(tx0) + p % kTile;
             store8(out.p + op * out.stride + out.off + co0 + c8 * 8u,
                    *reinterpret_cast<const bf16x8*>(stage + p * kPitch + c8 * 16u));
+        }
+        if (fz.pool.p) {  // the 8 x 8 pooled tile, summed in pool2's order
+            constexpr uint32_t kQ = kTile / 2u;
+            for (uint32_t i = tid; i < kQ * kQ * (kCo / 8u); i += kBlock) {
+                const uint32_t q = i / (kCo / 8u), c8 = i % (kCo / 8u), qy = q / kQ, qx = q % kQ;
+                const uint32_t p00 = 2u * qy * kTile + 2u * qx;
+                const char* sb = stage + c8 * 16u;
+                const bf16x8 v0 = *reinterpret_cast<const bf16x8*>(sb + p00 * kPitch);
+                const bf16x8 v1 = *reinterpret_cast<const bf16x8*>(sb + (p00 + 1u) * kPitch);
+                const bf16x8 v2 = *reinterpret_cast<const bf16x8*>(sb + (p00 + kTile) * kPitch);
+                const bf16x8 v3 = *reinterpret_cast<const bf16x8*>(sb + (p00 + kTile + 1u) * kPitch);
+                bf16x8 o;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) o[k] = b(0.25f * (f(v0[k]) + f(v1[k]) + f(v2[k]) + f(v3[k])));
+                const size_t cp = (size_t(img) * (d.h / 2u) + uint32_t(ty0) / 2u + qy) * (d.w / 2u) +
+                                  uint32_t(tx0) / 2u + qx;
+                store8(fz.pool.p + cp * fz.pool.stride + fz.pool.off + co0 + c8 * 8u, o);
+            }
+        }
+        if (fz.upw.p) {  // each output pixel copied to its 2 x 2 block of the fine tensor (as up2)
+            const uint32_t W2 = 2u * d.w;
+            for (uint32_t i = tid; i < kTile * kTile * 4u * (kCo / 8u); i += kBlock) {
+                const uint32_t c8 = i % (kCo / 8u), f = i / (kCo / 8u);  // f: fine pixel of the 32 x 32 block
+                const uint32_t fy = f / (2u * kTile), fx = f % (2u * kTile);
+                const uint32_t p = (fy / 2u) * kTile + fx / 2u;
+                const size_t fp = (size_t(img) * 2u * d.h + 2u * uint32_t(ty0) + fy) * W2 + 2u * uint32_t(tx0) + fx;
+                store8(fz.upw.p + fp * fz.upw.stride + fz.upw.off + co0 + c8 * 8u,
+                       *reinterpret_cast<const bf16x8*>(stage + p * kPitch + c8 * 16u));
+            }
         }
         return;
     }
@@ -200,44 +291,73 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
 
 template <int NCT>
 static void conv_launch(const Tens& in, uint32_t cin, const bf16* w, const float* bias, const Tens& out,
-                        uint32_t cout, Dims d, bool relu, float* of32, uint32_t f32c, hipStream_t s) {
+                        uint32_t cout, Dims d, bool relu, float* of32, uint32_t f32c, const ConvFuse& fz,
+                        hipStream_t s) {
     const uint32_t tiles = d.n * ((d.h + kTile - 1u) / kTile) * ((d.w + kTile - 1u) / kTile);
     const uint32_t groups = (cout / 16u + NCT - 1u) / NCT;
     const dim3 grid(tiles, groups);
     const bool full = cout % (uint32_t(NCT) * 16u) == 0 && d.h % kTile == 0 && d.w % kTile == 0 &&
                       out.stride % 8u == 0 && out.off % 8u == 0;
-    if (of32)
-        hipLaunchKernelGGL((conv3x3_k<NCT, false, true, false>), grid, dim3(kBlock), 0, s, in, cin, w, bias, out,
-                           cout, d, of32, f32c);
+    // fp32 output through the LDS stage: one output group, whole 16-byte rows
+    const bool f32_full = NCT == 1 && cout == 16u && f32c <= 4u && d.h % kTile == 0 && d.w % kTile == 0 &&
+                          (kTile * f32c) % 4u == 0 && reinterpret_cast<uintptr_t>(of32) % 16u == 0;
+#define PT_CONV(...) hipLaunchKernelGGL((conv3x3_k<__VA_ARGS__>), grid, dim3(kBlock), 0, s, in, cin, w, bias, out, \
+                                        cout, d, of32, f32c, fz)
+    if (of32 && f32_full)
+        PT_CONV(NCT, false, true, true);
+    else if (of32)
+        PT_CONV(NCT, false, true, false);
     else if (relu && full)
-        hipLaunchKernelGGL((conv3x3_k<NCT, true, false, true>), grid, dim3(kBlock), 0, s, in, cin, w, bias, out, cout,
-                           d, of32, f32c);
+        PT_CONV(NCT, true, false, true);
     else if (relu)
-        hipLaunchKernelGGL((conv3x3_k<NCT, true, false, false>), grid, dim3(kBlock), 0, s, in, cin, w, bias, out,
-                           cout, d, of32, f32c);
+        PT_CONV(NCT, true, false, false);
     else if (full)
-        hipLaunchKernelGGL((conv3x3_k<NCT, false, false, true>), grid, dim3(kBlock), 0, s, in, cin, w, bias, out,
-                           cout, d, of32, f32c);
+        PT_CONV(NCT, false, false, true);
     else
-        hipLaunchKernelGGL((conv3x3_k<NCT, false, false, false>), grid, dim3(kBlock), 0, s, in, cin, w, bias, out,
-                           cout, d, of32, f32c);
+        PT_CONV(NCT, false, false, false);
+#undef PT_CONV
 }
 
 hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bias, const Tens& out, uint32_t cout,
-                   Dims d, bool relu, float* out_f32, uint32_t f32_channels, hipStream_t s) {
+                   Dims d, bool relu, float* out_f32, uint32_t f32_channels, hipStream_t s, const ConvFuse* fuse) {
     if (cin == 0 || cin % 16u || cout % 16u || cout == 0 || cout > kMaxCout || in.stride % 8u || in.off % 8u)
         return hipErrorInvalidValue;
     if (d.pixels() == 0) return hipSuccess;
+    ConvFuse fz{};
+    if (fuse) fz = *fuse;
     // output-channel tiles per block: all of them up to 4, else groups of 3 or 4
     const uint32_t ct = cout / 16u;
+    // (splitting the coarse levels' output channels over more workgroups, one
+    // 16-channel group each, measured slower: every group restages the halo)
     const uint32_t nct = ct <= 4u ? ct : (ct % 4u == 0 ? 4u : (ct % 3u == 0 ? 3u : 4u));
+    if (fz.fin && (!out_f32 || f32_channels != 3u || d.n != 1u || fz.fw > d.w || fz.fh > d.h))
+        return hipErrorInvalidValue;
+    // the frame texels come from the fp32 stage when the output conv takes that path
+    const bool texel_fused = fz.fin && nct == 1u && cout == 16u && d.h % kTile == 0 && d.w % kTile == 0 &&
+                             reinterpret_cast<uintptr_t>(out_f32) % 16u == 0;
+    const ConvFuse frame = fz;
+    if (!texel_fused) fz.fin = nullptr;
+    // pooling and upsampling fuse into the staged bf16 epilogue (full output
+    // groups and tiles); otherwise pool2 / up2 follow the convolution
+    const bool staged = !out_f32 && cout % (nct * 16u) == 0 && d.h % kTile == 0 && d.w % kTile == 0 &&
+                        out.stride % 8u == 0 && out.off % 8u == 0;
+    if ((fz.pool.p && (d.h % 2u || d.w % 2u || fz.pool.stride % 8u || fz.pool.off % 8u)) ||
+        (fz.upw.p && (out_f32 || fz.upw.stride % 8u || fz.upw.off % 8u)))
+        return hipErrorInvalidValue;
+    const Tens pool = fz.pool, upw = fz.upw;
+    if (!staged) fz.pool = fz.upw = Tens{nullptr, 0, 0};
     switch (nct) {
-        case 1: conv_launch<1>(in, cin, w, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
-        case 2: conv_launch<2>(in, cin, w, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
-        case 3: conv_launch<3>(in, cin, w, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
-        default: conv_launch<4>(in, cin, w, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
+        case 1: conv_launch<1>(in, cin, w, bias, out, cout, d, relu, out_f32, f32_channels, fz, s); break;
+        case 2: conv_launch<2>(in, cin, w, bias, out, cout, d, relu, out_f32, f32_channels, fz, s); break;
+        case 3: conv_launch<3>(in, cin, w, bias, out, cout, d, relu, out_f32, f32_channels, fz, s); break;
+        default: conv_launch<4>(in, cin, w, bias, out, cout, d, relu, out_f32, f32_channels, fz, s); break;
     }
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && pool.p && !staged) e = pool2(out, cout, d, pool, s);
+    if (e == hipSuccess && upw.p && !staged) e = up2(out, cout, d, upw, s);
+    if (e == hipSuccess && frame.fin && !texel_fused)
+        e = unpack_frame(out_f32, d, frame.fw, frame.fh, frame.exposure, frame.fin, frame.ldr, s);
+    return e;
 }
 
 // ------------------------------------------------------------- conv fp8 --
@@ -1117,21 +1237,24 @@ hipError_t adam(float* w, const float* g, float* m, float* v, size_t n, float lr
 }
 
 // ------------------------------------------------------- input / output --
+// The 16 packed input channels of a pixel (o 16-byte aligned), written as
+// two 16-byte stores.
 __device__ __forceinline__ void write_features(bf16* o, float r, float g, float bl, float ar, float ag, float ab,
                                                float nx, float ny, float nz, float dep) {
-    o[0] = b(log1pf(fmaxf(r, 0.0f)));
-    o[1] = b(log1pf(fmaxf(g, 0.0f)));
-    o[2] = b(log1pf(fmaxf(bl, 0.0f)));
-    o[3] = b(ar);
-    o[4] = b(ag);
-    o[5] = b(ab);
-    o[6] = b(nx);
-    o[7] = b(ny);
-    o[8] = b(nz);
+    bf16x8 lo, hi = zero8();
+    lo[0] = b(log1pf(fmaxf(r, 0.0f)));
+    lo[1] = b(log1pf(fmaxf(g, 0.0f)));
+    lo[2] = b(log1pf(fmaxf(bl, 0.0f)));
+    lo[3] = b(ar);
+    lo[4] = b(ag);
+    lo[5] = b(ab);
+    lo[6] = b(nx);
+    lo[7] = b(ny);
+    hi[0] = b(nz);
     const float dd = fmaxf(dep, 0.0f);
-    o[9] = b(dd / (dd + 1.0f));
-#pragma unroll
-    for (int k = 10; k < 16; ++k) o[k] = b(0.0f);
+    hi[1] = b(dd / (dd + 1.0f));
+    store8(o, lo);
+    store8(o + 8, hi);
 }
 
 __global__ void __launch_bounds__(kBlock) pack_input_k(const float* __restrict__ feat, uint32_t n,
@@ -1167,8 +1290,8 @@ __global__ void __launch_bounds__(kBlock) pack_frame_k(const float4* __restrict_
     const uint32_t x = x0 + p % d.w, y = y0 + p / d.w;
     bf16* o = out + size_t(p) * 16u;
     if (x >= fw || y >= fh) {
-#pragma unroll
-        for (int k = 0; k < 16; ++k) o[k] = b(0.0f);
+        store8(o, zero8());
+        store8(o + 8, zero8());
         return;
     }
     const uint32_t i = y * fw + x;
@@ -1262,26 +1385,13 @@ hipError_t l1_log_loss(const float* pred, const float* target, Dims d, bf16* dpr
     return hipGetLastError();
 }
 
-__device__ __forceinline__ unsigned char unorm8n(float v) {
-    const float c = fminf(fmaxf(v, 0.0f), 1.0f);
-    return (unsigned char)(c * 255.0f + 0.5f);
-}
-
 __global__ void __launch_bounds__(kBlock) unpack_frame_k(const float* __restrict__ pred, Dims d, uint32_t fw,
                                                          uint32_t fh, float exposure, float4* __restrict__ fin,
                                                          uchar4* __restrict__ out) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= fw * fh) return;
     const uint32_t x = i % fw, y = i / fw;
-    const float* q = pred + (size_t(y) * d.w + x) * 3u;
-    const float r = fmaxf(expm1f(q[0]), 0.0f), g = fmaxf(expm1f(q[1]), 0.0f), bl = fmaxf(expm1f(q[2]), 0.0f);
-    fin[i] = make_float4(r, g, bl, 1.0f);
-    if (out) {
-        float o[3] = {r * exposure, g * exposure, bl * exposure};
-#pragma unroll
-        for (int k = 0; k < 3; ++k) o[k] = powf(o[k] / (o[k] + 0.3f), 0.4545f);
-        out[i] = make_uchar4(unorm8n(o[0]), unorm8n(o[1]), unorm8n(o[2]), 255);
-    }
+    frame_texel(pred + (size_t(y) * d.w + x) * 3u, exposure, fin + i, out ? out + i : nullptr);
 }
 
 hipError_t unpack_frame(const float* pred, Dims d, uint32_t fw, uint32_t fh, float exposure, float4* fin,

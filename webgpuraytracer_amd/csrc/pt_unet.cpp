@@ -252,7 +252,7 @@ int alloc_acts(pt_ctx* c, UNet& u, Dims d0) {
 }
 
 int conv_fwd(pt_ctx* c, UNet& u, bool q8, uint32_t li, const Tens& in, const Tens& out, Dims d,
-             float* of32 = nullptr) {
+             float* of32 = nullptr, const ConvFuse* fuse = nullptr) {
     const Layer& l = u.L[li];
     if (u.calib) {
         const hipError_t e = chan_amax(in, l.cin, d.pixels(), u.calib + l.a_off, c->stream);
@@ -263,47 +263,61 @@ int conv_fwd(pt_ctx* c, UNet& u, bool q8, uint32_t li, const Tens& in, const Ten
                                        u.calib_hist + size_t(l.a_off) * kHistBins, c->stream);
         if (e != hipSuccess) return c->hip_fail(e, "unet calibration");
     }
-    // the first and last convolutions stay bf16: in e4m3 the input features
-    // and the output radiance lose the most (each alone measured at +12% and
-    // +3..9% relMSE of the denoised frame, the 17 others at +3% or less)
-    if (q8 && li != E0A && li != OUT)
-        return hip_rc(c,
-                      conv3x3_fp8(in, l.cin, static_cast<const uint8_t*>(u.w8.p) + l.wbf_off, fp(u.inv_sa) + l.a_off,
-                                  fp(u.sw) + l.s_off, fp(u.params) + l.b_off, out, l.cout, d, l.relu, of32,
-                                  of32 ? 3u : 0u, c->stream),
-                      "unet conv fp8");
+    // the first convolution, the last decoder convolution and the output
+    // convolution stay bf16: in e4m3 the input features, the full-resolution
+    // features the output is read from and the output radiance lose the most
+    // (first and output alone measured at +12% and +3..9% relMSE of the
+    // denoised frame; with the weights trained on 4096-spp targets the last
+    // decoder convolution in e4m3 took a 1080p frame from 1.13x to 1.93x the
+    // bf16 relMSE)
+    if (q8 && li != E0A && li != D0B && li != OUT) {
+        hipError_t e = conv3x3_fp8(in, l.cin, static_cast<const uint8_t*>(u.w8.p) + l.wbf_off, fp(u.inv_sa) + l.a_off,
+                                   fp(u.sw) + l.s_off, fp(u.params) + l.b_off, out, l.cout, d, l.relu, of32,
+                                   of32 ? 3u : 0u, c->stream);
+        // the fp8 kernel has no fused epilogue work: pooling and upsampling follow it
+        if (e == hipSuccess && fuse && fuse->pool.p) e = pool2(out, l.cout, d, fuse->pool, c->stream);
+        if (e == hipSuccess && fuse && fuse->upw.p) e = up2(out, l.cout, d, fuse->upw, c->stream);
+        return hip_rc(c, e, "unet conv fp8");
+    }
     return hip_rc(c,
                   conv3x3(in, l.cin, bfp(u.wbf) + l.wbf_off, fp(u.params) + l.b_off, out, l.cout, d, l.relu, of32,
-                          of32 ? 3u : 0u, c->stream),
+                          of32 ? 3u : 0u, c->stream, fuse),
                   "unet conv");
 }
 
 // The forward pass over u.in0; q8: the fp8 convolutions (inference only).
-int forward(pt_ctx* c, UNet& u, bool q8 = false) {
+// The encoder's second convolutions also write the pooled copy of their
+// output, and the convolution before each decoder level also writes the
+// upsampled copy of its output into that level's concatenation buffer (in
+// the convolution's epilogue where its output tile is staged).  out_fuse:
+// work for the output convolution (the frame texels of pt_unet_denoise).
+int forward(pt_ctx* c, UNet& u, bool q8 = false, const ConvFuse* out_fuse = nullptr) {
     const Dims* D = u.dims;
     int rc;
+    const Tens none{nullptr, 0, 0};
     // encoder
     Tens x = T(u.in0, kInC);
     for (int l = 0; l < 4; ++l) {
         const uint32_t cc = kC[l + 1] + kC[l];
         const Tens skip = T(u.cat[l], cc, kC[l + 1]);
+        const ConvFuse fz{T(u.pin[l + 1], kC[l]), none, nullptr, nullptr, 0u, 0u, 0.0f};
         if ((rc = conv_fwd(c, u, q8, E0A + 2 * l, x, T(u.tA[l], kC[l]), D[l]))) return rc;
-        if ((rc = conv_fwd(c, u, q8, E0B + 2 * l, T(u.tA[l], kC[l]), skip, D[l]))) return rc;
-        if ((rc = hip_rc(c, pool2(skip, kC[l], D[l], T(u.pin[l + 1], kC[l]), c->stream), "unet pool"))) return rc;
-        x = T(u.pin[l + 1], kC[l]);
+        if ((rc = conv_fwd(c, u, q8, E0B + 2 * l, T(u.tA[l], kC[l]), skip, D[l], nullptr, &fz))) return rc;
+        x = fz.pool;
     }
+    // the bottleneck's output goes up into cat[3], each decoder level's (but the last) into the next one's
     if ((rc = conv_fwd(c, u, q8, B4A, x, T(u.t4A, kC[4]), D[4]))) return rc;
-    if ((rc = conv_fwd(c, u, q8, B4B, T(u.t4A, kC[4]), T(u.b4, kC[4]), D[4]))) return rc;
-    // decoder
-    Tens low = T(u.b4, kC[4]);
+    const ConvFuse up3{none, T(u.cat[3], kC[4] + kC[3], 0), nullptr, nullptr, 0u, 0u, 0.0f};
+    if ((rc = conv_fwd(c, u, q8, B4B, T(u.t4A, kC[4]), T(u.b4, kC[4]), D[4], nullptr, &up3))) return rc;
     for (int l = 3; l >= 0; --l) {
         const uint32_t cc = kC[l + 1] + kC[l];
-        if ((rc = hip_rc(c, up2(low, kC[l + 1], D[l + 1], T(u.cat[l], cc, 0), c->stream), "unet up"))) return rc;
+        const ConvFuse upl{none, l > 0 ? T(u.cat[l - 1], kC[l] + kC[l - 1], 0) : none, nullptr, nullptr, 0u, 0u, 0.0f};
         if ((rc = conv_fwd(c, u, q8, D3A + 2 * (3 - l), T(u.cat[l], cc), T(u.dA[l], kC[l]), D[l]))) return rc;
-        if ((rc = conv_fwd(c, u, q8, D3B + 2 * (3 - l), T(u.dA[l], kC[l]), T(u.dd[l], kC[l]), D[l]))) return rc;
-        low = T(u.dd[l], kC[l]);
+        if ((rc = conv_fwd(c, u, q8, D3B + 2 * (3 - l), T(u.dA[l], kC[l]), T(u.dd[l], kC[l]), D[l], nullptr,
+                           l > 0 ? &upl : nullptr)))
+            return rc;
     }
-    return conv_fwd(c, u, q8, OUT, low, Tens{nullptr, 0, 0}, D[0], fp(u.pred));
+    return conv_fwd(c, u, q8, OUT, T(u.dd[0], kC[0]), none, D[0], fp(u.pred), out_fuse);
 }
 
 // Backward through layer li: x its input, a its output (for the ReLU mask),
@@ -392,14 +406,14 @@ int upload_batch_features(pt_ctx* c, UNet& u, const float* feat, const float* ta
 
 // The context's current frame (accumulation buffer + G-buffer), padded to
 // multiples of 16, through the network into u.pred.
-int frame_forward(pt_ctx* c, UNet& u, bool q8) {
+int frame_forward(pt_ctx* c, UNet& u, bool q8, const ConvFuse* out_fuse = nullptr) {
     const Dims d{1u, (c->height + 15u) & ~15u, (c->width + 15u) & ~15u};
     int rc;
     if ((rc = alloc_acts(c, u, d))) return rc;
     const hipError_t e = pack_frame(static_cast<const float4*>(c->accum.p), c->gbuf.p, c->width, c->height, 0u, 0u,
                                     d, bfp(u.in0), c->stream);
     if (e != hipSuccess) return c->hip_fail(e, "unet input");
-    return forward(c, u, q8);
+    return forward(c, u, q8, out_fuse);
 }
 
 // Calibrated amax of a layer input maps to this e4m3 value: one binade of
@@ -547,6 +561,33 @@ int pt_unet_set_params(pt_ctx* c, const float* in, uint64_t count) {
     u.fp8 = false;
     if ((rc = refresh_weights(c, u))) return rc;
     return c->sync("unet params");
+}
+
+int pt_unet_get_adam(pt_ctx* c, float* m, float* v, uint64_t count, uint32_t* steps) {
+    if (!c) return PT_ERR_ARG;
+    int rc = ensure_net(c);
+    if (rc) return rc;
+    UNet& u = *c->unet;
+    if (!m || !v || !steps || count != u.n_params)
+        return c->fail(PT_ERR_ARG, "expected two arrays of %llu floats and a step count", (unsigned long long)u.n_params);
+    if ((rc = c->sync("unet adam"))) return rc;
+    if ((rc = hip_rc(c, hipMemcpy(m, u.m.p, count * 4, hipMemcpyDeviceToHost), "unet adam m"))) return rc;
+    if ((rc = hip_rc(c, hipMemcpy(v, u.v.p, count * 4, hipMemcpyDeviceToHost), "unet adam v"))) return rc;
+    *steps = u.adam_t;
+    return PT_OK;
+}
+
+int pt_unet_set_adam(pt_ctx* c, const float* m, const float* v, uint64_t count, uint32_t steps) {
+    if (!c) return PT_ERR_ARG;
+    int rc = ensure_net(c);
+    if (rc) return rc;
+    UNet& u = *c->unet;
+    if (!m || !v || count != u.n_params)
+        return c->fail(PT_ERR_ARG, "expected two arrays of %llu floats", (unsigned long long)u.n_params);
+    if ((rc = c->upload(u.m, m, count * 4, "unet adam m")) || (rc = c->upload(u.v, v, count * 4, "unet adam v")))
+        return rc;
+    u.adam_t = steps;
+    return c->sync("unet adam");
 }
 
 int pt_unet_forward(pt_ctx* c, const float* feat, uint32_t n, uint32_t h, uint32_t w, float* pred) {
@@ -710,10 +751,10 @@ int pt_unet_denoise(pt_ctx* c, float exposure) {
     UNet& u = *c->unet;
     const uint32_t fw = c->width, fh = c->height;
     if ((rc = c->alloc(c->dn_fin, size_t(fw) * fh * 16, "denoise out"))) return rc;
-    if ((rc = frame_forward(c, u, u.fp8))) return rc;
-    const hipError_t e = unpack_frame(fp(u.pred), u.dims[0], fw, fh, exposure, static_cast<float4*>(c->dn_fin.p),
-                                      static_cast<uchar4*>(c->out.p), c->stream);
-    if (e != hipSuccess) return c->hip_fail(e, "unet output");
+    // the output convolution writes the denoised texels (fp32 and rgba8) itself
+    const ConvFuse texels{Tens{nullptr, 0, 0}, Tens{nullptr, 0, 0}, static_cast<float4*>(c->dn_fin.p),
+                          static_cast<uchar4*>(c->out.p), fw, fh, exposure};
+    if ((rc = frame_forward(c, u, u.fp8, &texels))) return rc;
     return c->sync("unet denoise");
 }
 

@@ -134,22 +134,36 @@ def cmd_train(a):
          % (a.scenes, a.variants, a.size, a.frame[0], a.frame[1], a.target_spp, a.input_spp))
     t_data = build_training_set(r, net, a.scenes, a.variants, a.size, a.frame, a.target_spp, a.input_spp,
                                 a.scene_seed)
+    # a run split over several processes: each renders its own scenes, resumes the
+    # parameters and Adam state of the previous part and continues its lr schedule
+    step0 = 0
+    if a.resume:
+        with np.load(a.resume, allow_pickle=False) as z:
+            st = {k: z[k] for k in ("params", "m", "v", "steps")}
+        net.set_state(st)
+        step0 = int(st["steps"][0])
+        _log("resumed from %s at step %d" % (a.resume, step0))
+    total = max(a.total_steps or (step0 + a.steps), 1)
     losses = []
     t0 = time.perf_counter()
     chunk = 200
     done = 0
     while done < a.steps:
         n = min(chunk, a.steps - done)
-        frac = done / max(1, a.steps)
+        frac = (step0 + done) / total
         lr = a.lr * (0.5 * (1.0 + np.cos(np.pi * frac)) * 0.9 + 0.1)  # cosine decay to lr / 10
-        losses.extend(net.train(steps=n, batch=a.batch, lr=float(lr), seed=a.seed * 1000003 + done).tolist())
+        losses.extend(net.train(steps=n, batch=a.batch, lr=float(lr),
+                                seed=a.seed * 1000003 + step0 + done).tolist())
         done += n
-        _log("  step %d/%d loss %.4f (lr %.2e, %.1f s)" % (done, a.steps, float(np.mean(losses[-n:])), lr,
+        _log("  step %d/%d loss %.4f (lr %.2e, %.1f s)" % (step0 + done, total, float(np.mean(losses[-n:])), lr,
                                                         time.perf_counter() - t0))
     t_train = time.perf_counter() - t0
     np.save(a.out, net.get_params())
-    log = {"scenes": a.scenes, "variants": a.variants, "size": a.size, "frame": a.frame, "target_spp": a.target_spp,
-           "input_spp": a.input_spp, "steps": a.steps, "batch": a.batch, "lr": a.lr,
+    if a.save_state:
+        np.savez(a.save_state, **net.get_state())
+    log = {"scenes": a.scenes, "scene_seed": a.scene_seed, "variants": a.variants, "size": a.size, "frame": a.frame,
+           "target_spp": a.target_spp, "input_spp": a.input_spp, "steps": a.steps, "step0": step0,
+           "total_steps": total, "batch": a.batch, "lr": a.lr,
            "render_s": t_data, "train_s": t_train, "ms_per_step": 1e3 * t_train / a.steps,
            "loss_first100": float(np.mean(losses[:100])), "loss_last100": float(np.mean(losses[-100:]))}
     _log(json.dumps(log))
@@ -289,6 +303,10 @@ def main(argv=None):
     t.add_argument("--seed", type=int, default=1)
     t.add_argument("--scene-seed", type=int, default=1000)
     t.add_argument("--out", default="unet.npy")
+    t.add_argument("--resume", default=None, help="an .npz written by --save-state: continue that run")
+    t.add_argument("--save-state", default=None, help="write parameters and Adam state (.npz) to resume from")
+    t.add_argument("--total-steps", type=int, default=0,
+                   help="steps of the whole (possibly split) run, for the lr schedule (default: this part's)")
     t.add_argument("--log", default=None)
     e = sub.add_parser("eval")
     e.add_argument("--weights", required=True)
