@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-count", action="store_true",
                     help="skip the extra counted step that measures node/triangle work (no roofline)")
+    ap.add_argument("--no-bounces", action="store_true",
+                    help="skip the extra step that times each bounce (wavefront backend)")
     ap.add_argument("--backend", choices=["megakernel", "wavefront"], default="wavefront")
     ap.add_argument("--schedule", choices=["regen", "sync"], default="regen",
                     help="megakernel pass schedule (same bits either way)")
@@ -183,6 +185,27 @@ def main():
                    "kernel_mrays_s": round(rays / (kern_ms / 1000) / 1e6, 2) if kern_ms > 0 else None,
                    "scene_build_s": round(build_s, 3), "upload_s": round(upload_s, 3)},
     }
+    if args.backend == "wavefront" and not args.no_bounces:
+        # The same launch as timed step 0 with an event before each bounce:
+        # rays and device time per bounce (bounce 0: the coherent camera rays
+        # and their NEE; later bounces: incoherent secondary rays).
+        r.set_option(A.PT_OPT_BOUNCE_TIMING, 1)
+        step(0)
+        r.set_option(A.PT_OPT_BOUNCE_TIMING, 0)
+        ext, sh, ms = r.bounce_stats(16)
+        rows = []
+        for b in range(16):
+            if ext[b] == 0 and sh[b] == 0:
+                continue
+            n = int(ext[b]) + int(sh[b])
+            rows.append({"bounce": b, "extension_rays": int(ext[b]), "shadow_rays": int(sh[b]),
+                         "ms": round(float(ms[b]), 3), "mrays_s": round(n / (ms[b] / 1000) / 1e6, 1) if ms[b] else None})
+        later = [x for x in rows if x["bounce"] > 0]
+        n_later = sum(x["extension_rays"] + x["shadow_rays"] for x in later)
+        ms_later = sum(x["ms"] for x in later)
+        out["config"]["per_bounce"] = rows
+        out["config"]["bounce0_mrays_s"] = rows[0]["mrays_s"] if rows else None
+        out["config"]["secondary_mrays_s"] = round(n_later / (ms_later / 1000) / 1e6, 1) if ms_later else None
     if not args.no_count:
         # The same launch as timed step 0 (same seeds) with work counters on.
         # Counting runs the reference-order traversal, so the node and

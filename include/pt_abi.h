@@ -121,7 +121,7 @@ PT_LAYOUT_ASSERT(sizeof(pt_hit) == 56, "pt_hit is 56 B");
 enum {
     PT_OPT_BACKEND = 1,     /* PT_BACKEND_* (default WAVEFRONT) */
     PT_OPT_COUNT_STATS = 2, /* 1: count node/triangle work (default 1) */
-    PT_OPT_WAVEFRONT_POOL = 3, /* paths per wavefront batch, whole passes (default 1<<27, ~344 B each) */
+    PT_OPT_WAVEFRONT_POOL = 3, /* paths per wavefront batch, whole passes (default 1<<27, ~312 B each) */
     PT_OPT_SCHEDULE = 4,       /* megakernel pass schedule, PT_SCHEDULE_* (default REGEN) */
     PT_OPT_TRAVERSAL = 5,      /* BVH traversal, PT_TRAVERSAL_* (default WIDE) */
     PT_OPT_LDS_STACK = 6,      /* traversal-stack entries per lane kept in LDS, the rest in device
@@ -133,6 +133,9 @@ enum {
     PT_OPT_TLAS_BUILD = 9,       /* PT_TLAS_BUILD_*: where the top-level tree comes from; read by
                                    every pt_upload_scene and pt_update_objects */
     PT_OPT_DENOISE_MATH = 12,    /* PT_DENOISE_MATH_*: how pt_denoise evaluates the a-trous weights */
+    PT_OPT_BOUNCE_TIMING = 13,   /* 1: the wavefront backend records an event before each bounce's
+                                   launches and waits after each batch, so that pt_get_bounce_stats
+                                   reports device time per bounce; for measurement (default 0) */
     /* read-only (pt_get_option), describing the uploaded scene: */
     PT_OPT_KERNEL_FLAGS = 10,    /* the flag word the trace kernels run with: desc->flags plus internal
                                    bits (bit 20 material textures, bit 21 albedo alpha below 1) */
@@ -182,6 +185,15 @@ int pt_set_option(pt_ctx* ctx, int key, int64_t value);
 /* The current value of an option, or of a read-only PT_OPT_KERNEL_FLAGS /
  * PT_OPT_SPECIALISED query (PT_ERR_STATE before the first upload). */
 int pt_get_option(pt_ctx* ctx, int key, int64_t* value);
+
+/* Per-bounce work of the last pt_trace call on the wavefront backend, for
+ * bounces 0 .. n-1 (n <= 16): extension rays traced at bounce b (the camera
+ * rays at b = 0), shadow rays traced for the NEE samples of bounce b's
+ * shading, and with PT_OPT_BOUNCE_TIMING the device time of bounce b's three
+ * launches (extension, shading, shadow) in ms, summed over the call's
+ * batches (0 without it).  The megakernel reports zeros.  No counterpart in
+ * the reference: a measurement entry point. */
+int pt_get_bounce_stats(pt_ctx* ctx, uint64_t* ext_rays, uint64_t* shadow_rays, double* ms, uint32_t n);
 
 /* Renderer.setScene (tracer.js:1955-2102): copy all scene arrays, the sky and
  * its CDF to the device and specialise the trace kernels on desc->flags. */

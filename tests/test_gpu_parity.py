@@ -474,6 +474,60 @@ def test_backends_and_schedules_bitwise(renderer, which, section):
             assert getattr(st_m, k) == getattr(st_o, k), (name, k)
 
 
+def test_bounce_stats_and_zero_bounces(renderer):
+    """Per-bounce ray counts add up to the call's totals (camera rays at
+    bounce 0, fewer rays at every later bounce), bounce times are reported
+    only with PT_OPT_BOUNCE_TIMING and do not change the image; a scene with
+    a bounce limit of 0 adds zero radiance on both backends, also after a
+    render that left other values in the wavefront's path slots."""
+    scene = SceneBuilder.config(1, seed=81)
+    w, h = 64, 48
+    renderer.set_scene(scene)
+    renderer.resize(w, h)
+    p = scene.params(w, h, sample_number=0, seed=5)
+    seeds = [5, 6, 7]
+    try:
+        renderer.set_option(A.PT_OPT_BACKEND, A.PT_BACKEND_WAVEFRONT)
+        st = renderer.render_section(p, passes=3, seeds=seeds)
+        plain = renderer.read_accum()
+        ext, sh, ms = renderer.bounce_stats(16)
+        assert not ms.any()
+        renderer.resize(w, h)
+        renderer.set_option(A.PT_OPT_BOUNCE_TIMING, 1)
+        st2 = renderer.render_section(p, passes=3, seeds=seeds)
+        ext2, sh2, ms2 = renderer.bounce_stats(16)
+        assert np.array_equal(renderer.read_accum().view(np.uint32), plain.view(np.uint32))
+    finally:
+        renderer.set_option(A.PT_OPT_BOUNCE_TIMING, 0)
+    bl = scene.desc().bounce_limit
+    assert 0 < bl <= 16
+    for e, s, t in ((ext, sh, st), (ext2, sh2, st2)):
+        assert int(e.sum()) == t.extension_rays and int(s.sum()) == t.shadow_rays
+        assert e[0] == 3 * w * h
+        assert np.all(np.diff(e[:bl].astype(np.int64)) <= 0) and not e[bl:].any() and not s[bl:].any()
+    assert np.array_equal(ext, ext2) and np.array_equal(sh, sh2)
+    assert np.all(ms2[:bl] > 0) and not ms2[bl:].any()
+    # bounce limit 0: no bounce runs, every path adds zero
+    d = scene.desc()
+    d.bounce_limit = 0
+    try:
+        for backend in (A.PT_BACKEND_WAVEFRONT, A.PT_BACKEND_MEGAKERNEL):
+            renderer.set_option(A.PT_OPT_BACKEND, backend)
+            renderer.set_scene(scene)
+            renderer.resize(w, h)
+            renderer.render_section(p, passes=3, seeds=seeds)  # leaves radiance in the path slots
+            renderer.set_scene(d)
+            renderer.write_accum(np.full((h, w, 4), 2.0, np.float32))
+            st0 = renderer.render_section(p, passes=3, seeds=seeds)
+            acc = renderer.read_accum()
+            assert st0.extension_rays == 0 and st0.shadow_rays == 0
+            # accum lerps toward zero: 2 * (1 - 1/1) after pass 0, then stays 0
+            assert not acc[..., :3].any(), backend
+    finally:
+        _restore_defaults(renderer)
+        renderer.set_scene(scene)
+
+
 @pytest.mark.parametrize("which", ["c1", "mesh"])
 def test_wide_traversal_renders_like_reference_order(renderer, which):
     """The wide traversals (exact and quantised boxes) pick a different

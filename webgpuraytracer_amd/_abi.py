@@ -131,7 +131,7 @@ ERR_NAMES = {0: "PT_OK", -1: "PT_ERR_ARG", -2: "PT_ERR_LAYOUT", -3: "PT_ERR_CAPA
 
 PT_OPT_BACKEND, PT_OPT_COUNT_STATS, PT_OPT_WAVEFRONT_POOL, PT_OPT_SCHEDULE, PT_OPT_TRAVERSAL, PT_OPT_LDS_STACK, \
     PT_OPT_WAVEFRONT_REFILL, PT_OPT_BLAS_BUILD, PT_OPT_TLAS_BUILD = 1, 2, 3, 4, 5, 6, 7, 8, 9
-PT_OPT_KERNEL_FLAGS, PT_OPT_SPECIALISED, PT_OPT_DENOISE_MATH = 10, 11, 12
+PT_OPT_KERNEL_FLAGS, PT_OPT_SPECIALISED, PT_OPT_DENOISE_MATH, PT_OPT_BOUNCE_TIMING = 10, 11, 12, 13
 PT_DENOISE_MATH_FUSED, PT_DENOISE_MATH_EXACT = 0, 1
 KERNEL_TEXTURES_BIT, KERNEL_TRANSLUCENT_BIT = 1 << 20, 1 << 21
 PT_BLAS_BUILD_REFERENCE, PT_BLAS_BUILD_DEVICE = 0, 1
@@ -165,6 +165,7 @@ PT_FUNCS = {
     "pt_trace_hits": (C.c_int, [vp, P(SceneParams), u32, P(Hit)]),
     "pt_get_option": (C.c_int, [vp, C.c_int, P(C.c_int64)]),
     "pt_trace_noalloc": (C.c_int, [vp, P(SceneParams), u32, u32, u32, P(u32), P(Stats)]),
+    "pt_get_bounce_stats": (C.c_int, [vp, vp, vp, vp, u32]),
     "pt_trace_hits_rays": (C.c_int, [vp, P(SceneParams), u32, P(Hit), vp]),
     "pt_gbuffer": (C.c_int, [vp, P(SceneParams)]),
     "pt_denoise": (C.c_int, [vp, P(DenoiseParams), C.c_int]),
@@ -183,6 +184,8 @@ PT_FUNCS = {
     "pt_unet_param_count": (C.c_int, [vp, P(u64)]),
     "pt_unet_get_params": (C.c_int, [vp, vp, u64]),
     "pt_unet_set_params": (C.c_int, [vp, vp, u64]),
+    "pt_unet_get_adam": (C.c_int, [vp, vp, vp, u64, P(u32)]),
+    "pt_unet_set_adam": (C.c_int, [vp, vp, vp, u64, u32]),
     "pt_unet_forward": (C.c_int, [vp, vp, u32, u32, u32, vp]),
     "pt_unet_loss_grad": (C.c_int, [vp, vp, vp, u32, u32, u32, P(f32), vp]),
     "pt_unet_set_dataset": (C.c_int, [vp, vp, vp, u32, u32, u32]),

@@ -448,6 +448,7 @@ void pt_destroy(pt_ctx* c) {
     c->release_all();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t ev : c->bounce_ev) (void)hipEventDestroy(ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -501,6 +502,7 @@ int pt_set_option(pt_ctx* c, int key, int64_t value) {
                 return c->fail(PT_ERR_ARG, "unknown denoise math %lld", (long long)value);
             c->denoise_math = int(value);
             return PT_OK;
+        case PT_OPT_BOUNCE_TIMING: c->bounce_timing = value != 0; return PT_OK;
         case PT_OPT_KERNEL_FLAGS:
         case PT_OPT_SPECIALISED: return c->fail(PT_ERR_ARG, "option %d is read-only", key);
         default: return c->fail(PT_ERR_ARG, "unknown option %d", key);
@@ -521,6 +523,7 @@ int pt_get_option(pt_ctx* c, int key, int64_t* value) {
         case PT_OPT_BLAS_BUILD: *value = c->blas_build; return PT_OK;
         case PT_OPT_TLAS_BUILD: *value = c->tlas_build; return PT_OK;
         case PT_OPT_DENOISE_MATH: *value = c->denoise_math; return PT_OK;
+        case PT_OPT_BOUNCE_TIMING: *value = c->bounce_timing ? 1 : 0; return PT_OK;
         case PT_OPT_KERNEL_FLAGS:
         case PT_OPT_SPECIALISED:
             if (!c->has_scene) return c->fail(PT_ERR_STATE, "no scene uploaded");
@@ -658,7 +661,7 @@ static int check_params(pt_ctx* c, const pt_scene_params* p) {
 }
 
 // Wavefront backend: the call's passes in batches of whole passes, each
-// batch one raygen + bounce_limit x (extend, shade, shadow) + accumulate.
+// batch bounce_limit x (extend, shade, shadow) + accumulate.
 // Allocates the path pool and fills plan.w; launches nothing.
 static int prepare_wavefront(pt_ctx* c, pt::TracePlan& plan) {
     const TraceArgs& a = plan.a;
@@ -666,7 +669,10 @@ static int prepare_wavefront(pt_ctx* c, pt::TracePlan& plan) {
     uint64_t per = std::max<uint64_t>(1, c->wf_pool_opt / std::max<uint64_t>(npix, 1));
     per = std::min<uint64_t>(per, a.n_passes);
     const uint64_t pool = per * npix;
-    if (pool > (1ull << 30)) return c->fail(PT_ERR_ARG, "wavefront pool too large");
+    // shadow-queue entries keep the path index below bit 29
+    if (pool >= (1ull << 29))
+        return c->fail(PT_ERR_ARG, "section of %llu pixels is too large for the wavefront backend (2^29 paths)",
+                       (unsigned long long)npix);
     const uint32_t P = uint32_t(pool);
     int rc;
     if (c->wf_pool < P) {
@@ -675,23 +681,24 @@ static int prepare_wavefront(pt_ctx* c, pt::TracePlan& plan) {
             {&c->wf_r, 16, "wf radiance"}, {&c->wf_meta, 4, "wf meta"}, {&c->wf_ha, 16, "wf hit"},
             {&c->wf_hb, 8, "wf bary"}, {&c->wf_media, 28 * 4, "wf media"}, {&c->wf_q0, 4, "wf queue"},
             {&c->wf_q1, 4, "wf queue"}, {&c->wf_sq, 4, "wf shadow queue"}, {&c->wf_so, 32, "wf shadow o"},
-            {&c->wf_sd, 32, "wf shadow d"}, {&c->wf_stp, 32, "wf shadow tp"}, {&c->wf_sfi, 32, "wf shadow L"}};
+            {&c->wf_sd, 32, "wf shadow d"}, {&c->wf_sc, 32, "wf shadow L"}};
         for (auto& b : bufs) {
             b.b->release();
             if ((rc = c->alloc(*b.b, size_t(P) * b.each, b.what))) { c->wf_pool = 0; return rc; }
         }
         if ((rc = c->alloc(c->wf_cnt, 64, "wf counters"))) return rc;
         c->wf_em.release();
+        c->wf_stp.release();
         c->wf_pool = P;
     }
-    // deferred emission of height-field-blocked surfaces: height-mapped scenes only, zeroed once
-    // (the shadow kernel clears each entry it applies)
-    if ((c->dev.flags & PT_HAS_HEIGHTMAPS) && c->wf_em.bytes < size_t(c->wf_pool) * 16u) {
-        c->wf_em.release();
-        if ((rc = c->alloc(c->wf_em, size_t(c->wf_pool) * 16u, "wf deferred emission"))) return rc;
-        hipError_t e0 = pt::launch_zero_u64(static_cast<unsigned long long*>(c->wf_em.p), c->wf_pool * 2u, c->stream);
-        if (e0 != hipSuccess) return c->hip_fail(e0, "wf deferred emission clear");
-    }
+    // the shadow slots' throughput (translucent occluders) and the deferred
+    // emission of height-field-blocked surfaces, for the scenes that have them
+    if ((c->dev.flags & pt::kTranslucentBit) && c->wf_stp.bytes < size_t(c->wf_pool) * 32u &&
+        (rc = c->alloc(c->wf_stp, size_t(c->wf_pool) * 32u, "wf shadow tp")))
+        return rc;
+    if ((c->dev.flags & PT_HAS_HEIGHTMAPS) && c->wf_em.bytes < size_t(c->wf_pool) * 16u &&
+        (rc = c->alloc(c->wf_em, size_t(c->wf_pool) * 16u, "wf deferred emission")))
+        return rc;
     WfArgs& w = plan.w;
     w = WfArgs{};
     w.P = a.P;
@@ -716,8 +723,8 @@ static int prepare_wavefront(pt_ctx* c, pt::TracePlan& plan) {
     w.shadow_queue = static_cast<uint32_t*>(c->wf_sq.p);
     w.sh_o = static_cast<float4*>(c->wf_so.p);
     w.sh_d = static_cast<float4*>(c->wf_sd.p);
-    w.sh_tp = static_cast<float4*>(c->wf_stp.p);
-    w.sh_fi = static_cast<float4*>(c->wf_sfi.p);
+    w.sh_c = static_cast<float4*>(c->wf_sc.p);
+    w.sh_tp = (c->dev.flags & pt::kTranslucentBit) ? static_cast<float4*>(c->wf_stp.p) : nullptr;
     w.sh_em = (c->dev.flags & PT_HAS_HEIGHTMAPS) ? static_cast<float4*>(c->wf_em.p) : nullptr;
     w.counters = static_cast<uint32_t*>(c->wf_cnt.p);
     // persistent traversal launches: grid, stack split, refill threshold
@@ -798,13 +805,28 @@ hipError_t enqueue_trace(pt_ctx* c, TracePlan& plan, hipStream_t s, bool timed) 
         WfArgs w = plan.w;
         const TraceArgs& a = plan.a;
         const uint64_t npix = uint64_t(a.sec_w) * a.sec_h;
+        // per-bounce timing (measurement only): events around the bounces, a wait per batch
+        const size_t nb = size_t(std::max(c->dev.bounce_limit, 0));
+        const bool per_bounce = timed && c->bounce_timing;
+        c->bounce_ms.assign(nb, 0.0);
+        while (per_bounce && e == hipSuccess && c->bounce_ev.size() < nb + 1) {
+            hipEvent_t ev = nullptr;
+            e = hipEventCreate(&ev);
+            if (e == hipSuccess) c->bounce_ev.push_back(ev);
+        }
         if (e == hipSuccess && timed) e = hipEventRecord(c->ev0, s);
         for (uint32_t first = 0; e == hipSuccess && first < a.n_passes; first += uint32_t(plan.per)) {
             w.first_pass = first;
             w.batch_passes = uint32_t(std::min<uint64_t>(plan.per, a.n_passes - first));
             w.n_items = uint32_t(w.batch_passes * npix);
             w.write_output = first + w.batch_passes >= a.n_passes ? 1 : 0;
-            e = launch_wavefront_batch(c->dev, w, c->count_stats, s);
+            e = launch_wavefront_batch(c->dev, w, c->count_stats, s, per_bounce ? c->bounce_ev.data() : nullptr);
+            if (e == hipSuccess && per_bounce) e = hipEventSynchronize(c->bounce_ev[nb]);
+            for (size_t b = 0; e == hipSuccess && per_bounce && b < nb; ++b) {
+                float ms = 0.0f;
+                e = hipEventElapsedTime(&ms, c->bounce_ev[b], c->bounce_ev[b + 1]);
+                c->bounce_ms[b] += ms;
+            }
         }
     } else {
         if (e == hipSuccess) e = launch_zero_u64(reinterpret_cast<unsigned long long*>(plan.a.tile_ctr), 4, s);
@@ -851,6 +873,23 @@ static int run_trace(pt_ctx* c, const pt_scene_params* p, uint32_t sw, uint32_t 
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
         return pt::read_stats(c, stats, ms, 1);
+    }
+    return PT_OK;
+}
+
+int pt_get_bounce_stats(pt_ctx* c, uint64_t* ext_rays, uint64_t* shadow_rays, double* ms, uint32_t n) {
+    if (!c) return PT_ERR_ARG;
+    if (n > uint32_t(pt::kStatBounces)) return c->fail(PT_ERR_ARG, "at most %d bounces are counted", pt::kStatBounces);
+    if (!c->stats.p) return c->fail(PT_ERR_STATE, "no trace call yet");
+    unsigned long long v[pt::kStCount];
+    int rc = c->sync("bounce stats");
+    if (rc) return rc;
+    const hipError_t e = hipMemcpy(v, c->stats.p, sizeof(v), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return c->hip_fail(e, "bounce stats readback");
+    for (uint32_t b = 0; b < n; ++b) {
+        if (ext_rays) ext_rays[b] = v[pt::kStExtB0 + b];
+        if (shadow_rays) shadow_rays[b] = v[pt::kStShadowB0 + b];
+        if (ms) ms[b] = b < c->bounce_ms.size() ? c->bounce_ms[b] : 0.0;
     }
     return PT_OK;
 }

@@ -36,6 +36,11 @@ struct pt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // PT_OPT_BOUNCE_TIMING: events around the wavefront bounces, and the last
+    // call's device time per bounce
+    bool bounce_timing = false;
+    std::vector<hipEvent_t> bounce_ev;
+    std::vector<double> bounce_ms;
     std::string err;
     // options
     int backend = PT_BACKEND_WAVEFRONT;
@@ -75,9 +80,9 @@ struct pt_ctx {
     bool with_gbuffer = false;
     DevBuf accum, out, gbuf, dn_a, dn_b, dn_fin, dn_guide, stats, seeds, rays_in, rays_aux, rays_aux2, rays_out;
     // wavefront path pool
-    uint64_t wf_pool_opt = 1ull << 27;  // paths per wavefront batch (~344 B each)
+    uint64_t wf_pool_opt = 1ull << 27;  // paths per wavefront batch (~312 B each)
     uint32_t wf_pool = 0;
-    DevBuf wf_o, wf_d, wf_t, wf_r, wf_meta, wf_ha, wf_hb, wf_media, wf_q0, wf_q1, wf_sq, wf_so, wf_sd, wf_stp, wf_sfi,
+    DevBuf wf_o, wf_d, wf_t, wf_r, wf_meta, wf_ha, wf_hb, wf_media, wf_q0, wf_q1, wf_sq, wf_so, wf_sd, wf_stp, wf_sc,
         wf_cnt, wf_em;
     // persistent megakernel: per-lane medium stacks, per-XCD tile counters
     DevBuf mk_media, mk_tiles, mk_spill, wf_spill;
@@ -125,7 +130,7 @@ struct pt_ctx {
                          &planes, &meshes, &mesh_root, &lights, &sky, &cdf, &cdf_rows, &tri_geo_w, &accum, &out, &gbuf, &dn_a,
                          &dn_b, &dn_fin, &dn_guide, &stats, &seeds, &rays_in, &rays_aux, &rays_aux2, &rays_out, &qtnodes, &qbnodes,
                          &wf_o, &wf_d, &wf_t, &wf_r, &wf_meta, &wf_ha, &wf_hb, &wf_media, &wf_q0, &wf_q1, &wf_sq,
-                         &wf_so, &wf_sd, &wf_stp, &wf_sfi, &wf_cnt, &wf_em, &mk_media, &mk_tiles, &mk_spill, &wf_spill,
+                         &wf_so, &wf_sd, &wf_stp, &wf_sc, &wf_cnt, &wf_em, &mk_media, &mk_tiles, &mk_spill, &wf_spill,
                          &tex};
         for (DevBuf* b : all) b->release();
     }

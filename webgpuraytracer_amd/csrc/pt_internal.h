@@ -11,7 +11,12 @@ namespace pt {
 constexpr uint32_t kLaneMediaFloats = 28;
 
 // Counter slots in the device stats buffer (uint64 each).
-enum { kStExt = 0, kStShadow, kStPaths, kStPops, kStInner, kStObjs, kStTris, kStShade, kStCount };
+// per-bounce ray counts (wavefront backend): kStatBounces slots each
+constexpr int kStatBounces = 16;
+enum {
+    kStExt = 0, kStShadow, kStPaths, kStPops, kStInner, kStObjs, kStTris, kStShade,
+    kStExtB0, kStShadowB0 = kStExtB0 + kStatBounces, kStCount = kStShadowB0 + kStatBounces
+};
 
 struct TraceArgs {
     pt_scene_params P;
@@ -83,8 +88,9 @@ struct WfArgs {
     uint32_t* shadow_queue;
     float4* sh_o;       // per slot x2: origin, distance
     float4* sh_d;       //              direction, ignored object bits
-    float4* sh_tp;      //              throughput, valid flag
-    float4* sh_fi;      //              clamped unoccluded radiance (unclamped, bounce in w, with translucent occluders)
+    float4* sh_c;       //              throughput x clamped unoccluded radiance, w: throughput x 0 bits
+                        //              (translucent occluders: unclamped radiance, bounce in w)
+    float4* sh_tp;      //              translucent occluders only (else NULL): throughput
     float4* sh_em;      // per path, height-mapped scenes only (else NULL): emission of a surface the height
                         // field blocked, to be added after the shadow slots; w = 1 while pending
     uint32_t* counters; // queue A length, queue B length, shadow queue length, two fetch cursors
@@ -143,7 +149,9 @@ size_t tlas_build_scratch_bytes(uint32_t n);
 hipError_t build_tlas_device(const pt_object* objs, uint32_t n, const uint32_t* obj_root, const Root* roots,
                              void* scratch, size_t scratch_bytes, CNode* cn, WNode* wn, uint32_t* ctr,
                              TlasBuildResult* res, hipStream_t stream);
-hipError_t launch_wavefront_batch(const DevScene& S, const WfArgs& a, bool count_nodes, hipStream_t stream);
+// bounce_ev: NULL, or bounce_limit + 1 events recorded around the bounces.
+hipError_t launch_wavefront_batch(const DevScene& S, const WfArgs& a, bool count_nodes, hipStream_t stream,
+                                  hipEvent_t* bounce_ev);
 // QNodes from WNodes, index for index (pt_build.hip).
 hipError_t quantize_wide_nodes(const WNode* in, QNode* out, uint32_t n, hipStream_t stream);
 // True when the trace kernels have a variant compiled for this kernel flag

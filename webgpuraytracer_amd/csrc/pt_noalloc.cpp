@@ -24,8 +24,8 @@ constexpr uint32_t kMaxPasses = 65536;              // seed-list capacity of one
 // Layout (byte offsets): work counters, 8 tile counters, the seed list, then
 // the lanes' medium stacks and stack-spill columns (structure of arrays).
 constexpr size_t kOffStats = 0;
-constexpr size_t kOffTiles = 128;
-constexpr size_t kOffSeeds = 256;
+constexpr size_t kOffTiles = 512;
+constexpr size_t kOffSeeds = 1024;
 constexpr size_t kOffLanes = kOffSeeds + 4 * size_t(kMaxPasses);
 static_assert(pt::kStCount * 8 <= kOffTiles, "work counters fit");
 static_assert(kOffTiles + 8 * 4 <= kOffSeeds, "tile counters fit");

@@ -18,6 +18,16 @@
 // kernel reads its queue length from device memory, so a whole batch is
 // launched without a host round trip.  The per-path RNG state and draw order
 // are the megakernel's, so both backends produce the same bits.
+//
+// Path state traffic: before the first bounce only the camera ray is stored
+// (32 bytes per path); the first shading derives the rest of the path's
+// state (PCG state, throughput, radiance) from the slot index again, as
+// path_begin does; a path that ends writes only its radiance; a shadow query
+// is 48 bytes (origin, direction, and the contribution already multiplied by
+// the throughput), and which slots a path queued travels in its shadow-queue
+// entry, so no slot is cleared or tested in memory.  (Computing the camera
+// ray inside the first traversal launch instead took 8 more VGPRs there and a
+// wave per SIMD on the small analytic scenes.)
 #include <algorithm>
 
 #include "pt_internal.h"
@@ -28,6 +38,32 @@ namespace pt {
 constexpr int kWfBlock = 256;
 // counters: queue lengths, and the fetch cursors of the two traversal kernels
 enum { kCntA = 0, kCntB = 1, kCntShadow = 2, kCntFetchExt = 3, kCntFetchShadow = 4 };
+// a shadow-queue entry: the path index and what its shading left pending
+// (light slot, env slot, deferred emission); the pool is below 2^29 paths
+constexpr uint32_t kSqSlot0 = 1u << 31, kSqSlot1 = 1u << 30, kSqEmit = 1u << 29, kSqPath = kSqEmit - 1u;
+
+// x * 0 for each factor x of the throughput, as the shadow kernel must add
+// it for an occluded slot (+0, -0, or NaN for an infinite or NaN factor), in
+// two bits per component: bit 0 sign, bit 1 NaN.
+__device__ __forceinline__ uint32_t zero_product_bits(v3 tp) {
+    uint32_t b = 0;
+    const float c[3] = {tp.x, tp.y, tp.z};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const float z = c[j] * 0.0f;
+        b |= (z != z ? 2u : (__float_as_uint(z) >> 31)) << (2 * j);
+    }
+    return b;
+}
+__device__ __forceinline__ v3 zero_product(uint32_t b) {
+    float c[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const uint32_t q = (b >> (2 * j)) & 3u;
+        c[j] = q & 2u ? __uint_as_float(0x7fc00000u) : __uint_as_float(q << 31);
+    }
+    return mk(c[0], c[1], c[2]);
+}
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -96,32 +132,31 @@ __device__ __forceinline__ void slot_pixel(const WfArgs& A, uint32_t i, uint32_t
     section_pixel(A, i - pass * npix, px, py);
 }
 
-__global__ void __launch_bounds__(kWfBlock) wf_raygen(WfArgs A) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        A.counters[kCntA] = A.n_items;
-        A.counters[kCntB] = 0u;
-        A.counters[kCntShadow] = 0u;
-        A.counters[kCntFetchExt] = 0u;
-        A.counters[kCntFetchShadow] = 0u;
-    }
-    if (i >= A.n_items) return;
+// Path of batch slot i at bounce 0: camera ray and PCG state as the
+// reference seeds them (shader.wgsl:2280-2287).
+__device__ __forceinline__ void wf_path_begin(const WfArgs& A, uint32_t i, PathState& ps) {
     uint32_t px, py, pass;
     slot_pixel(A, i, px, py, pass);
     const uint32_t sn = A.P.sample_number + A.first_pass + pass;
-    const uint32_t seed = A.seeds ? A.seeds[A.first_pass + pass] : A.P.seed;
-    PathState ps;
+    // the fallback as a register value: a select between the argument's and
+    // the list's addresses would compile to a flat load, whose counter waits
+    // then spread through the kernel
+    const uint32_t fallback = __builtin_amdgcn_readfirstlane(A.P.seed);
+    const uint32_t seed = A.seeds ? A.seeds[A.first_pass + pass] : fallback;
     path_begin(A.P, px, py, sn, seed, ps);
+}
+
+// The batch's camera rays (the first shading recomputes the rest of each
+// path's start); queue lengths and fetch cursors start at zero (bounce 0 reads
+// no queue: its paths are the batch slots 0 .. n_items - 1).
+__global__ void __launch_bounds__(kWfBlock) wf_raygen(WfArgs A) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 5u) A.counters[i] = 0u;
+    if (i >= A.n_items) return;
+    PathState ps;
+    wf_path_begin(A, i, ps);
     A.o_pdf[i] = make_float4(ps.ro.x, ps.ro.y, ps.ro.z, ps.last_pdf);
     A.d_wsum[i] = make_float4(ps.rd.x, ps.rd.y, ps.rd.z, ps.last_wsum);
-    A.tp_rng[i] = make_float4(ps.tp.x, ps.tp.y, ps.tp.z, __uint_as_float(ps.rng.s));
-    A.rad[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    A.meta[i] = 0u;
-    A.queue[0][i] = i;
-    // shadow slots read as empty until a bounce queues a query; wf_shadow
-    // empties them again after use
-    A.sh_tp[2u * i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    A.sh_tp[2u * i + 1u] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // Persistent traversal of a queue: the extension rays of queue[cur]
@@ -139,10 +174,15 @@ __global__ void __launch_bounds__(kWfBlock) wf_raygen(WfArgs A) {
 #endif
 template <uint32_t F, bool CN, int W, bool SHADOW>
 __global__ void __launch_bounds__(kWfBlock) __attribute__((amdgpu_waves_per_eu(PT_WF_TRACE_WAVES_PER_EU)))
-wf_trace(DevScene S, WfArgs A, int cur) {
+wf_trace(DevScene S, WfArgs A, int cur, int bounce) {
     extern __shared__ uint32_t lds_stack[];
     const Flags<F> fl{S.flags};
-    const uint32_t n = SHADOW ? A.counters[kCntShadow] : A.counters[cur];
+    const bool first = !SHADOW && bounce == 0;  // the batch's camera rays, slot by slot
+    // the counter is read even when unused so that n stays a scalar load (a
+    // select between the argument and the counter's address compiles to a
+    // flat load whose wait spreads through the kernel)
+    const uint32_t qn = A.counters[SHADOW ? int(kCntShadow) : cur];
+    const uint32_t n = first ? A.n_items : qn;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         // counters this launch does not read, reset for the launches after it
         if (SHADOW) {
@@ -162,7 +202,7 @@ wf_trace(DevScene S, WfArgs A, int cur) {
     c.clear();
     Trav<F, CN, W> q;
     HitRec h;
-    uint32_t p = 0;
+    uint32_t p = 0, pend = 0;  // path, and (shadow) its pending kSq* bits
     int k = 0;
     v3 rad = splat(0.0f);
     bool busy = false, drained = false;
@@ -172,27 +212,25 @@ wf_trace(DevScene S, WfArgs A, int cur) {
     };
     auto apply_slot = [&]() {
         const uint32_t sl = 2u * p + uint32_t(k);
-        const float4 tp = A.sh_tp[sl], fi = A.sh_fi[sl];
-        v3 add = q.blocked ? splat(0.0f) : mk(fi.x, fi.y, fi.z);
-        if (fl.has(kTranslucentBit)) {  // fi holds the unclamped contribution and the bounce
-            add = q.blocked || q.trans < 0.001f ? splat(0.0f)
-                                                : firefly(mk(fi.x, fi.y, fi.z) * q.trans, int(fi.w));
+        const float4 ct = A.sh_c[sl];
+        if (fl.has(kTranslucentBit)) {  // ct holds the unclamped radiance and the bounce
+            const float4 tp = A.sh_tp[sl];
+            const v3 add = q.blocked || q.trans < 0.001f ? splat(0.0f)
+                                                         : firefly(mk(ct.x, ct.y, ct.z) * q.trans, int(ct.w));
+            rad = rad + mk(tp.x, tp.y, tp.z) * add;
+        } else {  // ct holds throughput x clamped radiance; occluded: throughput x 0
+            rad = rad + (q.blocked ? zero_product(__float_as_uint(ct.w)) : mk(ct.x, ct.y, ct.z));
         }
-        rad = rad + mk(tp.x, tp.y, tp.z) * add;
-        A.sh_tp[sl].w = 0.0f;
         ++k;
     };
     // starts the next shadow slot of path p, or stores its radiance when none is left
     auto next_slot = [&]() {
         for (;;) {
-            while (k < 2 && A.sh_tp[2u * p + uint32_t(k)].w == 0.0f) ++k;
+            while (k < 2 && !(pend & (k == 0 ? kSqSlot0 : kSqSlot1))) ++k;
             if (k >= 2) {
-                if (A.sh_em) {
+                if (pend & kSqEmit) {
                     const float4 e = A.sh_em[p];
-                    if (e.w != 0.0f) {
-                        rad = rad + mk(e.x, e.y, e.z);
-                        A.sh_em[p].w = 0.0f;
-                    }
+                    rad = rad + mk(e.x, e.y, e.z);
                 }
                 A.rad[p] = make_float4(rad.x, rad.y, rad.z, 0.0f);
                 busy = false;
@@ -222,13 +260,15 @@ wf_trace(DevScene S, WfArgs A, int cur) {
                 const uint32_t i = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
                 if (i < n) {
                     if (SHADOW) {
-                        p = A.shadow_queue[i];
+                        const uint32_t e = A.shadow_queue[i];
+                        p = e & kSqPath;
+                        pend = e;
                         const float4 r = A.rad[p];
                         rad = mk(r.x, r.y, r.z);
                         k = 0;
                         next_slot();
                     } else {
-                        p = A.queue[cur][i];
+                        p = first ? i : A.queue[cur][i];
                         const float4 o = A.o_pdf[p], d = A.d_wsum[p];
                         c.ext++;
                         busy = q.begin_closest(S, fl, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), st, h);
@@ -256,6 +296,7 @@ wf_trace(DevScene S, WfArgs A, int cur) {
     }
     if (A.stats) {
         add_stat(A.stats, SHADOW ? kStShadow : kStExt, SHADOW ? c.shadow : c.ext);
+        if (bounce < kStatBounces) add_stat(A.stats, (SHADOW ? kStShadowB0 : kStExtB0) + bounce, SHADOW ? c.shadow : c.ext);
         if (CN) {
             add_stat(A.stats, kStPops, c.pops);
             add_stat(A.stats, kStInner, c.inner);
@@ -265,11 +306,12 @@ wf_trace(DevScene S, WfArgs A, int cur) {
     }
 }
 
-// Queues the NEE shadow query of slot k of path p (0: light, 1: env).
+// Queues the NEE shadow query of slot k of path p (0: light, 1: env); `pend`
+// collects the kSq* bits of the path's shadow-queue entry.
 struct QueuedNee {
     const WfArgs& A;
     uint32_t p;
-    bool any;
+    uint32_t pend;
     bool att;  // translucent occluders possible
     __device__ void operator()(int k, bool need_ray, v3 o, v3 d, float dist, int ignore, v3 inc, v3 tp, int bounce,
                                v3& rad) {
@@ -277,47 +319,62 @@ struct QueuedNee {
             rad = rad + tp * firefly(inc, bounce);
             return;
         }
-        // clamped now, unless a translucent occluder may scale it first
-        const v3 fi = att ? inc : firefly(inc, bounce);
         const uint32_t s = 2u * p + uint32_t(k);
         A.sh_o[s] = make_float4(o.x, o.y, o.z, dist);
         A.sh_d[s] = make_float4(d.x, d.y, d.z, __int_as_float(ignore));
-        A.sh_tp[s] = make_float4(tp.x, tp.y, tp.z, 1.0f);
-        A.sh_fi[s] = make_float4(fi.x, fi.y, fi.z, att ? float(bounce) : 0.0f);
-        any = true;
+        if (att) {  // clamped after a translucent occluder scales it
+            A.sh_c[s] = make_float4(inc.x, inc.y, inc.z, float(bounce));
+            A.sh_tp[s] = make_float4(tp.x, tp.y, tp.z, 0.0f);
+        } else {  // the product rad + tp * L would add: the same rounding, done here
+            const v3 ct = tp * firefly(inc, bounce);
+            A.sh_c[s] = make_float4(ct.x, ct.y, ct.z, __uint_as_float(zero_product_bits(tp)));
+        }
+        pend |= k == 0 ? kSqSlot0 : kSqSlot1;
     }
     // radiance that must follow the queued NEE contributions: deferred to the
     // shadow kernel while a slot is pending
     __device__ void emit(v3 add, v3& rad) {
-        if (any && A.sh_em) A.sh_em[p] = make_float4(add.x, add.y, add.z, 1.0f);
-        else rad = rad + add;
+        if (pend && A.sh_em) {
+            A.sh_em[p] = make_float4(add.x, add.y, add.z, 0.0f);
+            pend |= kSqEmit;
+        } else {
+            rad = rad + add;
+        }
     }
 };
 
 template <uint32_t F, bool CN>
 __global__ void __launch_bounds__(kWfBlock) wf_shade(DevScene S, WfArgs A, int cur, int bounce) {
     const Flags<F> fl{S.flags};
-    const uint32_t n = A.counters[cur];
+    const uint32_t qn = A.counters[cur];  // read unconditionally: a scalar load (see wf_trace)
+    const uint32_t n = bounce == 0 ? A.n_items : qn;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x * blockDim.x >= n) return;
     const Params P{A.P.sky_total_lum, A.P.sky_width, A.P.sky_height, A.P.sky_cdf_index};
-    bool alive = false, shadow = false;
-    uint32_t p = 0;
+    bool alive = false;
+    uint32_t p = 0, pend = 0;
     Counters c;
     c.clear();
     if (i < n) {
-        p = A.queue[cur][i];
-        const float4 o = A.o_pdf[p], d = A.d_wsum[p], t = A.tp_rng[p], r = A.rad[p];
+        PathState ps;
+        uint32_t meta = 0;
+        if (bounce == 0) {  // batch slot i, straight from the camera
+            p = i;
+            wf_path_begin(A, i, ps);
+        } else {
+            p = A.queue[cur][i];
+            const float4 o = A.o_pdf[p], d = A.d_wsum[p], t = A.tp_rng[p], r = A.rad[p];
+            ps.ro = mk(o.x, o.y, o.z);
+            ps.last_pdf = o.w;
+            ps.rd = mk(d.x, d.y, d.z);
+            ps.last_wsum = d.w;
+            ps.tp = mk(t.x, t.y, t.z);
+            ps.rng.s = __float_as_uint(t.w);
+            ps.rad = mk(r.x, r.y, r.z);
+            meta = A.meta[p];
+        }
         const int4 ha = A.hit_a[p];
         const float2 hb = A.hit_b[p];
-        PathState ps;
-        ps.ro = mk(o.x, o.y, o.z);
-        ps.last_pdf = o.w;
-        ps.rd = mk(d.x, d.y, d.z);
-        ps.last_wsum = d.w;
-        ps.tp = mk(t.x, t.y, t.z);
-        ps.rng.s = __float_as_uint(t.w);
-        ps.rad = mk(r.x, r.y, r.z);
         HitRec h;
         h.t = __int_as_float(ha.x);
         h.kind = (ha.y >> 30) & 3;
@@ -327,23 +384,24 @@ __global__ void __launch_bounds__(kWfBlock) wf_shade(DevScene S, WfArgs A, int c
         h.count = uint32_t(ha.w);
         h.bu = hb.x;
         h.bv = hb.y;
-        const uint32_t meta = A.meta[p];
         Media md{A.media + p, A.pool, int(meta & 7u)};
         const SurfHit sh = resolve_for_shading<F>(S, fl, ps.ro, ps.rd, h);
-        QueuedNee sink{A, p, false, fl.has(kTranslucentBit)};
+        QueuedNee sink{A, p, 0u, fl.has(kTranslucentBit)};
         alive = shade_bounce<F, CN>(S, fl, P, bounce, h, sh, ps, md, c, sink);
         alive = alive && bounce + 1 < S.bounce_limit;
-        shadow = sink.any;
-        A.o_pdf[p] = make_float4(ps.ro.x, ps.ro.y, ps.ro.z, ps.last_pdf);
-        A.d_wsum[p] = make_float4(ps.rd.x, ps.rd.y, ps.rd.z, ps.last_wsum);
-        A.tp_rng[p] = make_float4(ps.tp.x, ps.tp.y, ps.tp.z, __uint_as_float(ps.rng.s));
+        pend = sink.pend;
+        if (alive) {  // an ended path keeps only its radiance
+            A.o_pdf[p] = make_float4(ps.ro.x, ps.ro.y, ps.ro.z, ps.last_pdf);
+            A.d_wsum[p] = make_float4(ps.rd.x, ps.rd.y, ps.rd.z, ps.last_wsum);
+            A.tp_rng[p] = make_float4(ps.tp.x, ps.tp.y, ps.tp.z, __uint_as_float(ps.rng.s));
+            if (bounce == 0 || uint32_t(md.n) != meta) A.meta[p] = uint32_t(md.n);
+        }
         A.rad[p] = make_float4(ps.rad.x, ps.rad.y, ps.rad.z, 0.0f);
-        A.meta[p] = uint32_t(md.n);
     }
     const uint32_t qa = wave_push(&A.counters[cur ^ 1], alive);
     if (alive) A.queue[cur ^ 1][qa] = p;
-    const uint32_t qs = wave_push(&A.counters[kCntShadow], shadow);
-    if (shadow) A.shadow_queue[qs] = p;
+    const uint32_t qs = wave_push(&A.counters[kCntShadow], pend != 0u);
+    if (pend) A.shadow_queue[qs] = p | pend;
     if (A.stats && CN) add_stat(A.stats, kStShade, c.shade);
 }
 
@@ -372,31 +430,40 @@ __global__ void __launch_bounds__(kWfBlock) wf_accumulate(WfArgs A) {
     if (A.stats && lp == 0) atomicAdd(A.stats + kStPaths, (unsigned long long)A.n_items);
 }
 
+// bounce_ev (optional): bounce_limit + 1 events, recorded before each
+// bounce's launches and after the last.
 template <uint32_t F, bool CN, int W>
-static void wf_bounces(const DevScene& S, const WfArgs& a0, uint32_t blocks, hipStream_t s) {
+static void wf_bounces(const DevScene& S, const WfArgs& a0, uint32_t blocks, hipStream_t s, hipEvent_t* bounce_ev) {
     const size_t lds = size_t(a0.stack_lds) * kWfBlock * 4u;
     int cur = 0;
     for (int b = 0; b < S.bounce_limit; ++b) {
-        hipLaunchKernelGGL((wf_trace<F, CN, W, false>), dim3(a0.trace_grid), dim3(kWfBlock), lds, s, S, a0, cur);
+        if (bounce_ev) (void)hipEventRecord(bounce_ev[b], s);
+        hipLaunchKernelGGL((wf_trace<F, CN, W, false>), dim3(a0.trace_grid), dim3(kWfBlock), lds, s, S, a0, cur, b);
         hipLaunchKernelGGL((wf_shade<F, CN>), dim3(blocks), dim3(kWfBlock), 0, s, S, a0, cur, b);
-        hipLaunchKernelGGL((wf_trace<F, CN, W, true>), dim3(a0.trace_grid), dim3(kWfBlock), lds, s, S, a0, cur);
+        hipLaunchKernelGGL((wf_trace<F, CN, W, true>), dim3(a0.trace_grid), dim3(kWfBlock), lds, s, S, a0, cur, b);
         cur ^= 1;
     }
+    if (bounce_ev) (void)hipEventRecord(bounce_ev[S.bounce_limit], s);
 }
 
 // Work counting runs the reference-order traversal (its per-pop counts).
 template <uint32_t F>
-static hipError_t wf_run(const DevScene& S, const WfArgs& a0, bool cn, hipStream_t s) {
+static hipError_t wf_run(const DevScene& S, const WfArgs& a0, bool cn, hipStream_t s, hipEvent_t* bounce_ev) {
     const uint32_t blocks = (a0.n_items + kWfBlock - 1) / kWfBlock;
     hipLaunchKernelGGL(wf_raygen, dim3(blocks), dim3(kWfBlock), 0, s, a0);
+    // without a bounce no shading writes the radiance: the paths add zero
+    if (S.bounce_limit <= 0) {
+        const hipError_t e = launch_zero_u64(reinterpret_cast<unsigned long long*>(a0.rad), 2u * a0.n_items, s);
+        if (e != hipSuccess) return e;
+    }
     if (cn)
-        wf_bounces<F, true, kTravPairs>(S, a0, blocks, s);
+        wf_bounces<F, true, kTravPairs>(S, a0, blocks, s, bounce_ev);
     else if (a0.wide == kTravQuant)
-        wf_bounces<F, false, kTravQuant>(S, a0, blocks, s);
+        wf_bounces<F, false, kTravQuant>(S, a0, blocks, s, bounce_ev);
     else if (a0.wide == kTravWide)
-        wf_bounces<F, false, kTravWide>(S, a0, blocks, s);
+        wf_bounces<F, false, kTravWide>(S, a0, blocks, s, bounce_ev);
     else
-        wf_bounces<F, false, kTravPairs>(S, a0, blocks, s);
+        wf_bounces<F, false, kTravPairs>(S, a0, blocks, s, bounce_ev);
     const uint32_t npix = a0.sec_w * a0.sec_h;
     hipLaunchKernelGGL(wf_accumulate, dim3((npix + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, s, a0);
     return hipGetLastError();
@@ -437,15 +504,16 @@ bool specialised_flag_set(uint32_t flags) {
            flags == kFlagsC1TA || flags == kFlagsC1HA;
 }
 
-hipError_t launch_wavefront_batch(const DevScene& S, const WfArgs& a, bool count_nodes, hipStream_t s) {
-    if (S.flags == kFlagsC1) return wf_run<kFlagsC1>(S, a, count_nodes, s);
-    if (S.flags == kFlagsC2) return wf_run<kFlagsC2>(S, a, count_nodes, s);
-    if (S.flags == kFlagsC3) return wf_run<kFlagsC3>(S, a, count_nodes, s);
-    if (S.flags == kFlagsC1T) return wf_run<kFlagsC1T>(S, a, count_nodes, s);
-    if (S.flags == kFlagsC1H) return wf_run<kFlagsC1H>(S, a, count_nodes, s);
-    if (S.flags == kFlagsC1TA) return wf_run<kFlagsC1TA>(S, a, count_nodes, s);
-    if (S.flags == kFlagsC1HA) return wf_run<kFlagsC1HA>(S, a, count_nodes, s);
-    return wf_run<kDynamicFlags>(S, a, count_nodes, s);
+hipError_t launch_wavefront_batch(const DevScene& S, const WfArgs& a, bool count_nodes, hipStream_t s,
+                                  hipEvent_t* bounce_ev) {
+    if (S.flags == kFlagsC1) return wf_run<kFlagsC1>(S, a, count_nodes, s, bounce_ev);
+    if (S.flags == kFlagsC2) return wf_run<kFlagsC2>(S, a, count_nodes, s, bounce_ev);
+    if (S.flags == kFlagsC3) return wf_run<kFlagsC3>(S, a, count_nodes, s, bounce_ev);
+    if (S.flags == kFlagsC1T) return wf_run<kFlagsC1T>(S, a, count_nodes, s, bounce_ev);
+    if (S.flags == kFlagsC1H) return wf_run<kFlagsC1H>(S, a, count_nodes, s, bounce_ev);
+    if (S.flags == kFlagsC1TA) return wf_run<kFlagsC1TA>(S, a, count_nodes, s, bounce_ev);
+    if (S.flags == kFlagsC1HA) return wf_run<kFlagsC1HA>(S, a, count_nodes, s, bounce_ev);
+    return wf_run<kDynamicFlags>(S, a, count_nodes, s, bounce_ev);
 }
 
 }  // namespace pt

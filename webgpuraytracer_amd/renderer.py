@@ -121,6 +121,14 @@ class Renderer:
         self._check(self.lib.pt_trace_hits_rays(self.ctx, C.byref(params), n_bounces, out, r.ctypes.data))
         return out, r
 
+    def bounce_stats(self, n=16):
+        """Per-bounce work of the last render (wavefront backend): extension
+        rays, shadow rays, and device ms per bounce (with
+        PT_OPT_BOUNCE_TIMING set, else zeros), as three arrays of n."""
+        ext, sh, ms = np.zeros(n, np.uint64), np.zeros(n, np.uint64), np.zeros(n, np.float64)
+        self._check(self.lib.pt_get_bounce_stats(self.ctx, ext.ctypes.data, sh.ctypes.data, ms.ctypes.data, n))
+        return ext, sh, ms
+
     # -- readback --
     def read_accum(self):
         a = np.empty((self.height, self.width, 4), np.float32)
@@ -235,6 +243,22 @@ class LearnedDenoiser:
     def set_params(self, p):
         p = np.ascontiguousarray(p, np.float32)
         self.r._check(self.r.lib.pt_unet_set_params(self.r.ctx, p.ctypes.data, p.size))
+
+    def get_state(self):
+        """Parameters and Adam state: a dict of arrays (params, m, v, steps)
+        that set_state restores, so training resumes in another process."""
+        n = self.param_count()
+        m, v = np.empty(n, np.float32), np.empty(n, np.float32)
+        steps = C.c_uint32()
+        self.r._check(self.r.lib.pt_unet_get_adam(self.r.ctx, m.ctypes.data, v.ctypes.data, n, C.byref(steps)))
+        return {"params": self.get_params(), "m": m, "v": v, "steps": np.array([steps.value], np.uint32)}
+
+    def set_state(self, st):
+        self.set_params(st["params"])
+        m = np.ascontiguousarray(st["m"], np.float32)
+        v = np.ascontiguousarray(st["v"], np.float32)
+        self.r._check(self.r.lib.pt_unet_set_adam(self.r.ctx, m.ctypes.data, v.ctypes.data, m.size,
+                                                  int(np.asarray(st["steps"]).reshape(-1)[0])))
 
     @staticmethod
     def _nhwc(a, c):
