@@ -235,6 +235,17 @@ __device__ __forceinline__ float4 sample_texture(const DevScene& S, int idx, v2 
 
 
 // ---------------------------------------------------------------- stack ----
+// The stack's LDS part is addressed through an LDS-typed pointer in the device
+// compilation, so that its accesses are ds_read / ds_write: through a generic
+// pointer (and a select between it and the spill column) they compile to flat
+// accesses, each of which waits for every outstanding global load.  The
+// traversal code is also compiled for the CPU (tests), where it is a plain
+// pointer.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+#else
+typedef uint32_t lds_u32;
+#endif
 // Per-lane traversal stack.  The first `cap` entries live in LDS,
 // interleaved across the block's lanes (entry e of lane l at e*blockDim + l)
 // so that a wave's accesses fall in distinct banks; entries cap..total-1, if
@@ -243,7 +254,7 @@ __device__ __forceinline__ float4 sample_texture(const DevScene& S, int idx, v2 
 // still hold the deepest traversal.  Pushes beyond `total` are dropped (the
 // host sizes `total` from the tree depths so that none is).
 struct LdsStack {
-    uint32_t* base;
+    lds_u32* base;
     uint32_t stride;
     uint32_t cap;
     int sp;
@@ -264,12 +275,15 @@ struct LdsStack {
     __device__ __forceinline__ uint32_t pop() {
         --sp;
         const uint32_t e = uint32_t(sp);
-        return e < cap ? base[e * stride] : spill[(e - cap) * sstride];
+        if (e < cap) return base[e * stride];
+        return spill[(e - cap) * sstride];
     }
 };
+// The LDS view of lane column `p` of a kernel's shared stack array.
+__device__ __forceinline__ lds_u32* lds_column(uint32_t* p) { return (lds_u32*)p; }
 // A stack held entirely in LDS.
 __device__ __forceinline__ LdsStack lds_stack_of(uint32_t* base, uint32_t stride, uint32_t cap) {
-    return LdsStack{base, stride, cap, 0, nullptr, 0u, cap};
+    return LdsStack{lds_column(base), stride, cap, 0, nullptr, 0u, cap};
 }
 
 struct Counters {

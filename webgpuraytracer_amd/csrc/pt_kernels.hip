@@ -185,7 +185,7 @@ trace_kernel(DevScene S, TraceArgs A) {
     const uint32_t tiles_x = (A.sec_w + kTile - 1) / kTile;
     const uint32_t lane = threadIdx.x;
     const uint32_t gid = blockIdx.x * kWave + lane;
-    LdsStack st{lds_stack + lane, kWave, A.stack_lds, 0, A.stack_spill + gid, A.media_stride,
+    LdsStack st{lds_column(lds_stack + lane), kWave, A.stack_lds, 0, A.stack_spill + gid, A.media_stride,
                 W ? S.wstack_cap : S.stack_cap};
     float* media = A.media + gid;
     const Params P{A.P.sky_total_lum, A.P.sky_width, A.P.sky_height, A.P.sky_cdf_index};

@@ -196,7 +196,7 @@ wf_trace(DevScene S, WfArgs A, int cur, int bounce) {
     uint32_t* fetch = &A.counters[SHADOW ? kCntFetchShadow : kCntFetchExt];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = lane_id();
-    LdsStack st{lds_stack + threadIdx.x, blockDim.x, A.stack_lds, 0, A.stack_spill + gid, A.spill_stride,
+    LdsStack st{lds_column(lds_stack + threadIdx.x), blockDim.x, A.stack_lds, 0, A.stack_spill + gid, A.spill_stride,
                 W ? S.wstack_cap : S.stack_cap};
     Counters c;
     c.clear();
