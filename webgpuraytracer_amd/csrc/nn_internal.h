@@ -39,6 +39,9 @@ constexpr uint32_t kMaxCout = 128;
 //           output, as pool2 writes it;
 //   upw.p:  (bf16 output) also its nearest 2x upsampling (dims 2d), as up2
 //           writes it;
+//   up.p:   (ReLU, d even) input channels [0, cup) are read from `up` (dims
+//           d / 2) at the nearest coarse pixel, as up2 would have written
+//           them into `in`; channels [cup, cin) from `in`;
 //   fin:    (fp32 output of 3 channels, one image) the denoised frame texels
 //           as unpack_frame writes them, for the fw x fh frame, instead of
 //           out_f32 (which is still required, and written when the texels
@@ -46,6 +49,8 @@ constexpr uint32_t kMaxCout = 128;
 struct ConvFuse {
     Tens pool;
     Tens upw;
+    Tens up;
+    uint32_t cup;
     float4* fin;
     uchar4* ldr;
     uint32_t fw, fh;

@@ -80,8 +80,11 @@ struct ConvSmem {
 // multiplied in turn: the staging registers are free during the products, so
 // the kernel needs half the VGPRs of one that fetches the next slice while
 // multiplying (twice the resident waves, which overlap one another's loads;
-// measured 4% faster per 1080p frame than prefetching where cin > 32).
-template <int NCT, bool RELU, bool F32OUT, bool FULL>
+// measured 4% faster per 1080p frame than prefetching where cin > 32).  UP:
+// input channels below fz.cup are read from the half-resolution fz.up at the
+// nearest coarse pixel, the others from `in` (a decoder level's first
+// convolution over [upsampled | skip] without the upsampled copy).
+template <int NCT, bool RELU, bool F32OUT, bool FULL, bool UP = false>
 __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const bf16* __restrict__ w,
                                                     const float* __restrict__ bias, Tens out, uint32_t cout, Dims d,
                                                     float* __restrict__ of32, uint32_t f32c, ConvFuse fz) {
@@ -97,6 +100,7 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
     const uint32_t nct = FULL ? uint32_t(NCT) : min(uint32_t(NCT), (cout - co0) / 16u);
     // the pieces this thread stages: halo pixel j, channels (piece % 4) * 8
     const bf16* hsrc[kHaloPer];
+    uint32_t cpix[UP ? kHaloPer : 1];
     bool hval[kHaloPer];
 #pragma unroll
     for (uint32_t k = 0; k < kHaloPer; ++k) {
@@ -106,13 +110,19 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
         hval[k] = piece < kHaloPieces && gy >= 0 && gy < int(d.h) && gx >= 0 && gx < int(d.w);
         const size_t sp = hval[k] ? (size_t(img) * d.h + uint32_t(gy)) * d.w + uint32_t(gx) : 0;
         hsrc[k] = in.p + sp * in.stride + in.off + (piece % 4u) * 8u;
+        if (UP)
+            cpix[UP ? k : 0] =
+                hval[k] ? (img * (d.h / 2u) + uint32_t(gy) / 2u) * (d.w / 2u) + uint32_t(gx) / 2u : 0u;
     }
     bf16x8 hreg[kHaloPer], wreg[kWPer];
     auto fetch = [&](uint32_t c0) {
 #pragma unroll
         for (uint32_t k = 0; k < kHaloPer; ++k) {
             const uint32_t c = c0 + ((tid + k * kBlock) % 4u) * 8u;
-            hreg[k] = (hval[k] && c < cin) ? load8(hsrc[k] + c0) : zero8();
+            if (UP && c < fz.cup)
+                hreg[k] = hval[k] ? load8(fz.up.p + size_t(cpix[UP ? k : 0]) * fz.up.stride + fz.up.off + c) : zero8();
+            else
+                hreg[k] = (hval[k] && c < cin) ? load8(hsrc[k] + c0) : zero8();
         }
 #pragma unroll
         for (uint32_t k = 0; k < kWPer; ++k) {
@@ -307,6 +317,10 @@ static void conv_launch(const Tens& in, uint32_t cin, const bf16* w, const float
         PT_CONV(NCT, false, true, true);
     else if (of32)
         PT_CONV(NCT, false, true, false);
+    else if (fz.up.p && full)
+        PT_CONV(NCT, true, false, true, true);
+    else if (fz.up.p)
+        PT_CONV(NCT, true, false, false, true);  // the decoders' first convolutions have ReLU
     else if (relu && full)
         PT_CONV(NCT, true, false, true);
     else if (relu)
@@ -342,7 +356,9 @@ hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bia
     const bool staged = !out_f32 && cout % (nct * 16u) == 0 && d.h % kTile == 0 && d.w % kTile == 0 &&
                         out.stride % 8u == 0 && out.off % 8u == 0;
     if ((fz.pool.p && (d.h % 2u || d.w % 2u || fz.pool.stride % 8u || fz.pool.off % 8u)) ||
-        (fz.upw.p && (out_f32 || fz.upw.stride % 8u || fz.upw.off % 8u)))
+        (fz.upw.p && (out_f32 || fz.upw.stride % 8u || fz.upw.off % 8u)) ||
+        (fz.up.p && (d.h % 2u || d.w % 2u || !relu || out_f32 || fz.cup % 8u || fz.cup > cin ||
+                     fz.up.stride % 8u || fz.up.off % 8u)))
         return hipErrorInvalidValue;
     const Tens pool = fz.pool, upw = fz.upw;
     if (!staged) fz.pool = fz.upw = Tens{nullptr, 0, 0};
