@@ -91,7 +91,17 @@ def _target_with_margin(rng, params, feat, margin=0.25):
 
 def test_gradients_match_torch_fp32(net):
     rng = np.random.default_rng(12)
-    feat = _features(rng, 2, 64, 64)
+    _check_gradients(net, rng, _features(rng, 2, 64, 64))
+
+
+def test_gradients_match_torch_fp32_ragged_levels(net):
+    # 48 x 80 gives levels 48x80, 24x40, 12x20, 6x10 and 3x5: the weight
+    # gradient's pixel tiles (8, 16 or 32 wide) overhang the image at every level
+    rng = np.random.default_rng(15)
+    _check_gradients(net, rng, _features(rng, 2, 48, 80))
+
+
+def _check_gradients(net, rng, feat):
     target = _target_with_margin(rng, net.get_params(), feat)
     loss, grad = net.loss_grad(feat, target)
     rloss, rgrad = R.loss_and_grads(net.get_params(), feat, target)

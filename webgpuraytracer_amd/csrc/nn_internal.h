@@ -78,32 +78,19 @@ hipError_t chan_amax(const Tens& x, uint32_t c, uint32_t P, uint32_t* amax, hipS
 constexpr uint32_t kHistBins = 1024;
 hipError_t chan_hist(const Tens& x, uint32_t c, uint32_t P, const uint32_t* amax, uint32_t* hist, hipStream_t s);
 
-// Backward staging for a layer with c output channels and output gradient
-// ga: dz = ga * (a > 0) when a is given (ReLU output), else ga; written NHWC
-// contiguous (dz, [pixel][c], the data-gradient input) and channel-major
-// (dzc, [c][row of chw_elems(1, d) pixels, zero past the last], the
-// weight-gradient input); db[c] (fp32, overwritten) = sum over pixels of dz.
-// scratch holds dz_scratch_floats(c, d) floats.
-size_t chw_elems(uint32_t c, Dims d);
-size_t dz_scratch_floats(uint32_t c, Dims d);
-hipError_t dz_prepare(const Tens& ga, const Tens* a, uint32_t c, Dims d, bf16* dz, bf16* dzc, float* scratch,
-                      float* db, hipStream_t s);
-// The layer input channel-major (rows as dzc), x_copies(d) copies of
-// chw_elems(c, d) elements: with w a multiple of 8 one copy per horizontal tap
-// offset, xs[dx][ch][p] = x[p + dx - 1][ch] within the pixel's image row, else
-// 0 (dx = 0..2); otherwise one per tap, xs[tap][ch][p] = x[p + offset(tap)][ch]
-// inside the image, else 0.  c <= kMaxCout.
-uint32_t x_copies(Dims d);
-hipError_t x_prepare(const Tens& x, uint32_t c, Dims d, bf16* xs, hipStream_t s);
-
-// Weight gradient of a 3x3 convolution over an input-channel slice:
-// dw[tap][co][ci_off + ci] (fp32, rows of cin_total, overwritten) = sum over
-// pixels of dz[p][co] * x[p + offset(tap)][ci] for ci < cin (cin <= kMaxCout),
-// from dzc and xs as dz_prepare and x_prepare write them; partial holds
-// wgrad_partial_floats(cin, cout, d) floats of scratch.
+// Backward of a 3x3 convolution (cout output channels) over an
+// input-channel slice, except the data gradient: with dz = ga * (a > 0)
+// when a is given (ReLU output), else ga,
+//   dw[tap][co][ci_off + ci] (fp32, rows of cin_total, overwritten) = sum over
+//     pixels of dz[p][co] * x[p + offset(tap)][ci] for ci < cin (x the
+//     slice's NHWC view, zero outside the image; cin <= kMaxCout),
+//   db[co] (fp32, overwritten; when db is non-null) = sum over pixels of dz,
+//   dz_out (when non-null) = dz, NHWC contiguous [pixel][cout].
+// partial holds wgrad_partial_floats(cin, cout, d) floats of scratch.
 size_t wgrad_partial_floats(uint32_t cin, uint32_t cout, Dims d);
-hipError_t conv3x3_wgrad(const bf16* dzc, uint32_t cout, const bf16* xs, uint32_t cin, Dims d, float* partial,
-                         float* dw, uint32_t ci_off, uint32_t cin_total, hipStream_t s);
+hipError_t conv3x3_wgrad(const Tens& ga, const Tens* a, uint32_t cout, const Tens& x, uint32_t cin, Dims d,
+                         bf16* dz_out, float* partial, float* dw, uint32_t ci_off, uint32_t cin_total, float* db,
+                         hipStream_t s);
 
 // 2x2 average pooling (d: the input's dims; h and w even).
 hipError_t pool2(const Tens& in, uint32_t c, Dims d, const Tens& out, hipStream_t s);

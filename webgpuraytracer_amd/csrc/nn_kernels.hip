@@ -10,9 +10,9 @@
 //               rows = 16 pixels per wave, columns = output channels, K = the
 //               9 taps x input channels in chunks of 32; bias + ReLU fused
 //   wgrad_k     weight gradient: rows = output channels, columns = input
-//               channels, K = pixels, both operands read channel-major
-//               (dz_prepare_k, x_prepare_k stage them); split over pixel
-//               chunks, partial sums reduced in a fixed order
+//               channels, K = pixels; NHWC tiles staged in LDS, fragments
+//               read with LDS transpose loads; split over pixel chunks,
+//               partial sums reduced in a fixed order
 //   the rest    pooling, upsampling, Adam, loss, packing (16-byte vectors)
 #include "nn_internal.h"
 
@@ -716,124 +716,11 @@ hipError_t chan_hist(const Tens& x, uint32_t c, uint32_t P, const uint32_t* amax
     return hipGetLastError();
 }
 
-// ------------------------------------------------------- backward staging --
-// The weight gradient reduces over pixels, so its MFMA operands need 8
-// consecutive pixels of one channel per lane: the backward pass stages dz and
-// the layer input channel-major.  The input gets one copy per horizontal tap
-// offset, so every tap's fragment is one aligned 16-byte load (a vertical
-// offset moves by whole rows, and w is a multiple of 16).
-constexpr uint32_t kStagePix = 64;  // pixels per staging block
-
-
-// Channel-major arrays have rows of chw_stride(d) = P rounded up to 64
-// pixels; the pixels past P are zero.
-static __host__ __device__ uint32_t chw_stride_of(uint32_t P) { return (P + kStagePix - 1u) / kStagePix * kStagePix; }
-
-// xs[dx][ch][p] = x[p + dx - 1][ch] when that pixel is in the same image row,
-// else 0 (dx = 0, 1, 2); rows of Ps pixels.  For w a multiple of 8.
-__global__ void __launch_bounds__(kBlock) x_prepare_k(Tens x, uint32_t c, Dims d, uint32_t Ps,
-                                                      bf16* __restrict__ xs) {
-    __shared__ bf16 t[kMaxCout][kStagePix + 8];  // pixels p0 - 1 .. p0 + 64 at index 0 .. 65
-    const uint32_t P = d.pixels();
-    const uint32_t p0 = blockIdx.x * kStagePix;
-    const uint32_t c8 = c / 8u;
-    for (uint32_t piece = threadIdx.x; piece < (kStagePix + 2u) * c8; piece += kBlock) {
-        const uint32_t j = piece / c8, ch = (piece % c8) * 8u;
-        const int p = int(p0) + int(j) - 1;
-        bf16x8 v = zero8();
-        if (p >= 0 && uint32_t(p) < P) v = load8(x.p + size_t(p) * x.stride + x.off + ch);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) t[ch + i][j] = v[i];
-    }
-    __syncthreads();
-    constexpr uint32_t groups = kStagePix / 8u;
-    for (uint32_t item = threadIdx.x; item < 3u * c * groups; item += kBlock) {
-        const uint32_t g = item % groups, ch = (item / groups) % c, dx = item / (groups * c);
-        const uint32_t pb = p0 + g * 8u;
-        bf16x8 v;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int sc = int((pb + uint32_t(i)) % d.w) + int(dx) - 1;
-            v[i] = (pb + uint32_t(i) < P && sc >= 0 && sc < int(d.w)) ? t[ch][g * 8u + uint32_t(i) + dx] : b(0.0f);
-        }
-        store8(xs + (size_t(dx) * c + ch) * Ps + pb, v);
-    }
-}
-
-// Any w: one copy per tap, xs[tap][ch][p] = x[p + offset(tap)][ch] inside the
-// image, else 0 (the small coarse levels whose width is not a multiple of 8).
-__global__ void __launch_bounds__(kBlock) x_prepare9_k(Tens x, uint32_t c, Dims d, uint32_t Ps,
-                                                       bf16* __restrict__ xs) {
-    const uint32_t groups = Ps / 8u;
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= 9u * c * groups) return;
-    const uint32_t g = i % groups, ch = (i / groups) % c, tap = i / (groups * c);
-    const int dy = int(tap / 3u) - 1, dx = int(tap % 3u) - 1;
-    const uint32_t P = d.pixels();
-    bf16x8 v;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const uint32_t q = g * 8u + uint32_t(j);
-        bf16 val = b(0.0f);
-        if (q < P) {
-            const int sx = int(q % d.w) + dx, sy = int((q / d.w) % d.h) + dy;
-            const uint32_t img = q / (d.w * d.h);
-            if (sx >= 0 && sx < int(d.w) && sy >= 0 && sy < int(d.h))
-                val = x.p[((size_t(img) * d.h + uint32_t(sy)) * d.w + uint32_t(sx)) * x.stride + x.off + ch];
-        }
-        v[j] = val;
-    }
-    store8(xs + (size_t(tap) * c + ch) * Ps + g * 8u, v);
-}
-
-// dz = ga * (a > 0) (relu) or ga, for 64 pixels per block: NHWC [p][c] (dz),
-// channel-major [c][p] (dzc) and the block's channel sums bpart[block][c].
-template <bool RELU>
-__global__ void __launch_bounds__(kBlock) dz_prepare_k(Tens ga, Tens a, uint32_t c, uint32_t P, uint32_t Ps,
-                                                       bf16* __restrict__ dz, bf16* __restrict__ dzc,
-                                                       float* __restrict__ bpart) {
-    __shared__ bf16 t[kMaxCout][kStagePix + 8];
-    const uint32_t p0 = blockIdx.x * kStagePix;
-    const uint32_t c8 = c / 8u;
-    for (uint32_t piece = threadIdx.x; piece < kStagePix * c8; piece += kBlock) {
-        const uint32_t j = piece / c8, ch = (piece % c8) * 8u;
-        const uint32_t p = p0 + j;
-        bf16x8 v = zero8();
-        if (p < P) {
-            v = load8(ga.p + size_t(p) * ga.stride + ga.off + ch);
-            if (RELU) {
-                const bf16x8 av = load8(a.p + size_t(p) * a.stride + a.off + ch);
-#pragma unroll
-                for (int i = 0; i < 8; ++i)
-                    if (!(f(av[i]) > 0.0f)) v[i] = b(0.0f);
-            }
-            store8(dz + size_t(p) * c + ch, v);
-        }
-#pragma unroll
-        for (int i = 0; i < 8; ++i) t[ch + i][j] = v[i];
-    }
-    __syncthreads();
-    constexpr uint32_t groups = kStagePix / 8u;
-    for (uint32_t item = threadIdx.x; item < c * groups; item += kBlock) {
-        const uint32_t g = item % groups, ch = item / groups;
-        const uint32_t pb = p0 + g * 8u;
-        bf16x8 v;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) v[i] = t[ch][g * 8u + uint32_t(i)];
-        store8(dzc + size_t(ch) * Ps + pb, v);
-    }
-    for (uint32_t ch = threadIdx.x; ch < c; ch += kBlock) {
-        float sum = 0.0f;
-        for (uint32_t j = 0; j < kStagePix; ++j) sum += f(t[ch][j]);
-        bpart[size_t(blockIdx.x) * c + ch] = sum;
-    }
-}
-
-// ----------------------------------------------------------- reductions --
+// ------------------------------------------------------------ reductions --
 // Deterministic sums over chunks: out[row * out_cols + col_off + col] =
-// sum over k < chunks of partial[k * n + row * cols + col].  Pass 1 sums
-// segments of at most kRedSeg chunks into tmp[segment][n]; pass 2 sums the
-// segments.
+// sum over k < chunks of partial[k * n + row * cols + col] for the first nw
+// of the n elements, out2[i - nw] for the rest.  Pass 1 sums segments of at
+// most kRedSeg chunks into tmp[segment][n]; pass 2 sums the segments.
 constexpr uint32_t kRedSeg = 32;
 
 __global__ void __launch_bounds__(kBlock) reduce_seg_k(const float* __restrict__ partial, uint32_t chunks, uint32_t n,
@@ -854,8 +741,8 @@ __global__ void __launch_bounds__(kBlock) reduce_seg_k(const float* __restrict__
 }
 
 __global__ void __launch_bounds__(kBlock) reduce_fin_k(const float* __restrict__ tmp, uint32_t segs, uint32_t n,
-                                                       uint32_t cols, uint32_t out_cols, uint32_t col_off,
-                                                       float* __restrict__ out) {
+                                                       uint32_t nw, uint32_t cols, uint32_t out_cols, uint32_t col_off,
+                                                       float* __restrict__ out, float* __restrict__ out2) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
@@ -867,216 +754,330 @@ __global__ void __launch_bounds__(kBlock) reduce_fin_k(const float* __restrict__
         s3 += tmp[size_t(k + 3u) * n + i];
     }
     for (; k < segs; ++k) s0 += tmp[size_t(k) * n + i];
+    const float v = (s0 + s1) + (s2 + s3);
+    if (i >= nw) {
+        out2[i - nw] = v;
+        return;
+    }
     const uint32_t row = i / cols, col = i - row * cols;
-    out[size_t(row) * out_cols + col_off + col] = (s0 + s1) + (s2 + s3);
+    out[size_t(row) * out_cols + col_off + col] = v;
 }
 
 static uint32_t red_segs(uint32_t chunks) { return (chunks + kRedSeg - 1u) / kRedSeg; }
 
-static hipError_t reduce_chunks(const float* partial, uint32_t chunks, uint32_t n, uint32_t cols, uint32_t out_cols,
-                                uint32_t col_off, float* tmp, float* out, hipStream_t s) {
+static hipError_t reduce_chunks(const float* partial, uint32_t chunks, uint32_t n, uint32_t nw, uint32_t cols,
+                                uint32_t out_cols, uint32_t col_off, float* tmp, float* out, float* out2,
+                                hipStream_t s) {
     const uint32_t segs = red_segs(chunks);
     hipLaunchKernelGGL(reduce_seg_k, dim3((n + kBlock - 1u) / kBlock, segs), dim3(kBlock), 0, s, partial, chunks, n,
                        tmp);
-    hipLaunchKernelGGL(reduce_fin_k, dim3((n + kBlock - 1u) / kBlock), dim3(kBlock), 0, s, tmp, segs, n, cols,
-                       out_cols, col_off, out);
-    return hipGetLastError();
-}
-
-size_t dz_scratch_floats(uint32_t c, Dims d) {
-    const uint32_t blocks = (d.pixels() + kStagePix - 1u) / kStagePix;
-    return size_t(blocks) * c + size_t(red_segs(blocks)) * c;
-}
-
-size_t chw_elems(uint32_t c, Dims d) { return size_t(chw_stride_of(d.pixels())) * c; }
-uint32_t x_copies(Dims d) { return d.w % 8u == 0 ? 3u : 9u; }
-
-hipError_t dz_prepare(const Tens& ga, const Tens* a, uint32_t c, Dims d, bf16* dz, bf16* dzc, float* scratch,
-                      float* db, hipStream_t s) {
-    const uint32_t P = d.pixels();
-    if (c == 0 || c % 16u || c > kMaxCout || ga.stride % 8u || ga.off % 8u || (a && (a->stride % 8u || a->off % 8u)) ||
-        P == 0)
-        return hipErrorInvalidValue;
-    const uint32_t Ps = chw_stride_of(P);
-    const uint32_t blocks = Ps / kStagePix;
-    float* bpart = scratch;
-    float* tmp = scratch + size_t(blocks) * c;
-    if (a)
-        hipLaunchKernelGGL(dz_prepare_k<true>, dim3(blocks), dim3(kBlock), 0, s, ga, *a, c, P, Ps, dz, dzc, bpart);
-    else
-        hipLaunchKernelGGL(dz_prepare_k<false>, dim3(blocks), dim3(kBlock), 0, s, ga, ga, c, P, Ps, dz, dzc, bpart);
-    return reduce_chunks(bpart, blocks, c, c, c, 0u, tmp, db, s);
-}
-
-hipError_t x_prepare(const Tens& x, uint32_t c, Dims d, bf16* xs, hipStream_t s) {
-    const uint32_t P = d.pixels();
-    if (c == 0 || c % 16u || c > kMaxCout || x.stride % 8u || x.off % 8u || P == 0) return hipErrorInvalidValue;
-    const uint32_t Ps = chw_stride_of(P);
-    if (x_copies(d) == 3u) {
-        hipLaunchKernelGGL(x_prepare_k, dim3(Ps / kStagePix), dim3(kBlock), 0, s, x, c, d, Ps, xs);
-    } else {
-        const uint32_t n = 9u * c * (Ps / 8u);
-        hipLaunchKernelGGL(x_prepare9_k, dim3((n + kBlock - 1u) / kBlock), dim3(kBlock), 0, s, x, c, d, Ps, xs);
-    }
+    hipLaunchKernelGGL(reduce_fin_k, dim3((n + kBlock - 1u) / kBlock), dim3(kBlock), 0, s, tmp, segs, n, nw, cols,
+                       out_cols, col_off, out, out2);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- wgrad --
-// Weight gradient as a GEMM over pixels: dW[tap][co][ci] = sum_p dzc[co][p] *
-// xs[tap % 3][ci][p + (tap / 3 - 1) w].  A wave owns one job (T taps, up to
-// NTX input-channel tiles, all cout / 16 <= MTX output-channel tiles) over a
-// chunk of pixels; the 4 waves of a block take 4 jobs of the same chunk, so
-// their dz fragments are shared through the cache.  Partial sums per chunk
-// are reduced by reduce_chunks.
-constexpr uint32_t kWgStep = 32;  // pixels per K step
+// Backward of a 3x3 convolution except the data gradient, in one kernel:
+// dz = ga * (a > 0) (ReLU layers) or ga, dW[tap][co][ci] = sum_p dz[p][co] *
+// x[p + offset(tap)][ci] (a GEMM over pixels), db[co] = sum_p dz[p][co], and
+// dz written NHWC for the data gradient.  All operands are read NHWC.
+//
+// A block walks tiles of 32 * steps pixels (th rows x tw columns of one
+// image, tw = 8, 16 or 32).  Per tile it stages dz (pixel-major, cout
+// channels contiguous) and the (th + 2) x (tw + 2) halo of x (the block's
+// input-channel span contiguous, zero outside the image) in LDS: every
+// thread issues all its global loads of the tile before its LDS stores (a
+// thread's pieces, <= kWgPieces per operand, are mapped once per launch).
+// The MFMA fragments come from ds_read_b64_tr_b16.  That instruction,
+// checked on the device: in each 16-lane group, lane 4e + g (e, g = 0..3)
+// reads 4 bf16 at its own address and lane 4g + r receives element r of
+// lanes 4e + g for e = 0..3.  So when lane L reads channels 4 (L % 4) .. +3
+// of pixel 16h + L / 4, lane 16b + i holds channel i of pixels 16h + 4b + e,
+// e = 0..3: two reads (h = 0, 1) give the 8 k-values 8b + 4h + e of an
+// operand fragment, with k <-> pixel 32s + 16h + 4b + e (K step s) the same
+// for dz and x.  A tap's shift is an address offset into the halo.  The
+// bias sums are the product with a fragment of ones.
+//
+// Fragments: B fragment f = 9 * (16-channel input tile) + tap.  Block group
+// blockIdx.y owns fragments [f0, f1) (a contiguous span of input tiles is
+// staged) and wave w of it f0 + w + 4j; every wave takes all cout / 16
+// output tiles; group 0 also writes dz and its wave 0 sums the bias.
+// blockIdx.x is a chunk of consecutive tiles (down a column of tiles, so
+// halo rows come again from the cache); its partial sums go to
+// partial[chunk] and reduce_chunks adds the chunks in order.  LDS pixel
+// pitches are odd multiples of 32 bytes: the 4 pixels of a 16-lane group then
+// fall in distinct banks.
+typedef short i16x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) char lds_char;
+typedef __attribute__((address_space(3))) i16x4 lds_i16x4;
+typedef __attribute__((address_space(3))) bf16x8 lds_bf16x8;
 
-template <int MTX, int NTX, int T, bool FULL9>
-__global__ void __launch_bounds__(kBlock) wgrad_k(const bf16* __restrict__ dzc, uint32_t cout,
-                                                  const bf16* __restrict__ xs, uint32_t cin, Dims d, uint32_t chunk,
-                                                  uint32_t ci_groups, uint32_t jobs, float* __restrict__ partial) {
-    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6, kq = lane >> 4;
-    const uint32_t job = blockIdx.y * 4u + wv;
-    if (job >= jobs) return;
-    const uint32_t tap0 = (job / ci_groups) * uint32_t(T);
-    const uint32_t cg = job % ci_groups;
-    const uint32_t ci0 = cg * uint32_t(NTX) * 16u;
-    const uint32_t nt = min(uint32_t(NTX), (cin - ci0) / 16u);
+constexpr uint32_t kWgLds = 65536;   // LDS per block (2 blocks per CU)
+constexpr uint32_t kWgBlocks = 512;  // blocks per launch to aim for
+constexpr int kWgPieces = 8;         // 16-byte pieces per thread and operand per tile
+constexpr int kWgBatch = 4;          // pieces per thread in flight at once
+
+struct WgGeom {
+    uint32_t tw, th, steps;            // tile: th x tw pixels = 32 * steps
+    uint32_t tiles_x, tiles_y, tiles;  // per image row, per image column, total
+    uint32_t per_chunk;                // tiles per block
+    uint32_t frags, per_group;         // B fragments in all, per block group
+    uint32_t dpitch;                   // dz tile pixel pitch, bytes
+};
+
+__device__ __forceinline__ bf16x8 tr_frag(lds_char* p, uint32_t second) {
+    const i16x4 lo = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_i16x4*)p);
+    const i16x4 hi = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_i16x4*)(p + second));
+    return __builtin_bit_cast(bf16x8, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+}
+
+// Piece k of thread t in a [rows][cols][c8] tile: (row, col, group) packed
+// in bytes 0, 1, 2; ~0u past the end.
+__device__ __forceinline__ uint32_t piece_of(uint32_t i, uint32_t rows, uint32_t cols, uint32_t c8) {
+    if (i >= rows * cols * c8) return ~0u;
+    const uint32_t q = i / c8, cg = i - q * c8;
+    const uint32_t r = q / cols, col = q - r * cols;
+    return r | (col << 8) | (cg << 16);
+}
+
+template <int MTX, int NB>
+__global__ void __launch_bounds__(kBlock) wgrad_k(Tens ga, Tens a, uint32_t cout, Tens x, uint32_t cin, Dims d,
+                                                  WgGeom g, bf16* __restrict__ dz_out, bool bias,
+                                                  float* __restrict__ partial) {
+    extern __shared__ __align__(16) char wg_smem[];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const uint32_t mt = cout / 16u;
-    const uint32_t P = d.pixels(), Ps = chw_stride_of(P);
-    const uint32_t pb = blockIdx.x * chunk, pe = min(pb + chunk, Ps);
-    f32x4 acc[T][NTX][MTX];
+    const uint32_t f0 = blockIdx.y * g.per_group, f1 = min(f0 + g.per_group, g.frags);
+    const uint32_t ct0 = f0 / 9u, span = (f1 - 1u) / 9u - ct0 + 1u;
+    const uint32_t xpitch = 32u * (span | 1u);
+    const uint32_t hw = g.tw + 2u, hpix = (g.th + 2u) * hw;
+    lds_char* xs = (lds_char*)wg_smem;
+    lds_char* dzs = xs + hpix * xpitch;
+    const uint32_t nb = f0 + wv < f1 ? (f1 - f0 - wv + 3u) / 4u : 0u;
+    const bool lead = blockIdx.y == 0;
+    const bool do_bias = bias && lead && wv == 0;
+    uint32_t boff[NB];
 #pragma unroll
-    for (int t = 0; t < T; ++t)
-#pragma unroll
-        for (int n = 0; n < NTX; ++n)
-#pragma unroll
-            for (int m = 0; m < MTX; ++m) acc[t][n][m] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const bf16* arow[MTX];
-#pragma unroll
-    for (int m = 0; m < MTX; ++m) arow[m] = dzc + size_t(uint32_t(m) * 16u + (lane & 15u)) * Ps;
-    // fragments of one K step: this lane's 8 pixels (zero past P); with three
-    // copies they share one image row, because w is a multiple of 8
-    auto load_step = [&](uint32_t p, bf16x8(&a)[MTX], bf16x8(&bx)[T][NTX]) {
-        const uint32_t pk = p + kq * 8u;
-        const int row = int((pk / d.w) % d.h);
-#pragma unroll
-        for (int m = 0; m < MTX; ++m) a[m] = uint32_t(m) < mt ? load8(arow[m] + pk) : zero8();
-#pragma unroll
-        for (int t = 0; t < T; ++t) {
-            const uint32_t tap = tap0 + uint32_t(t);
-            const bf16* xrow;
-            bool rv;
-            if (FULL9) {
-                rv = true;
-                xrow = xs + size_t(tap * cin + ci0 + (lane & 15u)) * Ps + pk;
-            } else {
-                const int dy = int(tap / 3u) - 1;
-                rv = pk < P && row + dy >= 0 && row + dy < int(d.h);
-                const uint32_t src = rv ? uint32_t(int(pk) + dy * int(d.w)) : 0u;
-                xrow = xs + size_t((tap % 3u) * cin + ci0 + (lane & 15u)) * Ps + src;
-            }
-#pragma unroll
-            for (int n = 0; n < NTX; ++n) bx[t][n] = (rv && uint32_t(n) < nt) ? load8(xrow + size_t(n) * 16u * Ps) : zero8();
-        }
-    };
-    auto mma = [&](const bf16x8(&a)[MTX], const bf16x8(&bx)[T][NTX]) {
-#pragma unroll
-        for (int t = 0; t < T; ++t)
-#pragma unroll
-            for (int n = 0; n < NTX; ++n)
-#pragma unroll
-                for (int m = 0; m < MTX; ++m)
-                    if (uint32_t(n) < nt && uint32_t(m) < mt)
-                        acc[t][n][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], bx[t][n], acc[t][n][m], 0, 0, 0);
-    };
-    // two register sets: the next step's loads are in flight during this step's products
-    bf16x8 a0[MTX], b0[T][NTX], a1[MTX], b1[T][NTX];
-    uint32_t p = pb;
-    if (p < pe) load_step(p, a0, b0);
-    while (p < pe) {
-        const uint32_t pn = p + kWgStep;
-        if (pn < pe) load_step(pn, a1, b1);
-        mma(a0, b0);
-        if (pn >= pe) break;
-        const uint32_t pn2 = pn + kWgStep;
-        if (pn2 < pe) load_step(pn2, a0, b0);
-        mma(a1, b1);
-        p = pn2;
+    for (int j = 0; j < NB; ++j) {
+        const uint32_t fj = f0 + wv + 4u * uint32_t(j), tap = fj % 9u;
+        boff[j] = ((tap / 3u) * hw + tap % 3u) * xpitch + (fj / 9u - ct0) * 32u;
     }
-    float* out = partial + size_t(blockIdx.x) * 9u * cout * cin;
+    const uint32_t ql = lane >> 2, cq = (lane & 3u) * 8u;
+    const uint32_t twq = min(g.tw, 16u);
+    const uint32_t lx = ((ql / twq) * hw + ql % twq) * xpitch + cq;
+    const uint32_t ld = ql * g.dpitch + cq;
+    const uint32_t xhalf = ((16u / g.tw) * hw + 16u % g.tw) * xpitch;  // pixel + 16 in the halo
+    const uint32_t xstep = (32u / g.tw) * hw * xpitch;                  // pixel + 32
+    const uint32_t c8x = span * 2u, c8d = cout / 8u;
+    uint32_t px[kWgPieces], pd[kWgPieces];
 #pragma unroll
-    for (int t = 0; t < T; ++t)
+    for (int k = 0; k < kWgPieces; ++k) {
+        px[k] = piece_of(threadIdx.x + uint32_t(k) * kBlock, g.th + 2u, hw, c8x);
+        pd[k] = piece_of(threadIdx.x + uint32_t(k) * kBlock, g.th, g.tw, c8d);
+    }
+    f32x4 acc[NB][MTX], bacc[MTX];
 #pragma unroll
-        for (int n = 0; n < NTX; ++n) {
-            if (uint32_t(n) >= nt) continue;
-            const uint32_t ci = ci0 + uint32_t(n) * 16u + (lane & 15u);
+    for (int m = 0; m < MTX; ++m) {
+        bacc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int m = 0; m < MTX; ++m) {
-                if (uint32_t(m) >= mt) continue;
+        for (int j = 0; j < NB; ++j) acc[j][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+    const bf16x8 ones = __builtin_bit_cast(bf16x8, u32x4{0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u});
+    const uint32_t t0 = blockIdx.x * g.per_chunk, t1 = min(t0 + g.per_chunk, g.tiles);
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint32_t tyi = t % g.tiles_y, rest = t / g.tiles_y;
+        const uint32_t txi = rest % g.tiles_x, img = rest / g.tiles_x;
+        const int y0 = int(tyi * g.th), x0 = int(txi * g.tw);
+        const size_t pix0 = size_t(img) * d.h * d.w;
+        // the piece maps are decoded again per tile rather than held decoded
+        // in registers across the loop
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const uint32_t co = uint32_t(m) * 16u + kq * 4u + uint32_t(r);
-                    out[(size_t(tap0 + uint32_t(t)) * cout + co) * cin + ci] = acc[t][n][m][r];
+        for (int k = 0; k < kWgPieces; ++k) asm volatile("" : "+v"(px[k]), "+v"(pd[k]));
+        __syncthreads();  // the previous tile's fragments are read
+        // global loads in batches of kWgBatch pieces per thread, each batch
+        // issued before its LDS stores
+#pragma unroll
+        for (int k0 = 0; k0 < kWgPieces; k0 += kWgBatch) {
+            bf16x8 v[kWgBatch];
+#pragma unroll
+            for (int k = 0; k < kWgBatch; ++k) {
+                const uint32_t pc = px[k0 + k];
+                v[k] = zero8();
+                if (pc != ~0u) {
+                    const int iy = y0 + int(pc & 255u) - 1, ix = x0 + int((pc >> 8) & 255u) - 1;
+                    if (iy >= 0 && iy < int(d.h) && ix >= 0 && ix < int(d.w))
+                        v[k] = load8(x.p + (pix0 + uint32_t(iy) * d.w + uint32_t(ix)) * x.stride + x.off +
+                                     ct0 * 16u + (pc >> 16) * 8u);
                 }
             }
+#pragma unroll
+            for (int k = 0; k < kWgBatch; ++k) {
+                const uint32_t pc = px[k0 + k];
+                if (pc != ~0u)
+                    *(lds_bf16x8*)(xs + (((pc & 255u) * hw + ((pc >> 8) & 255u)) * xpitch + (pc >> 16) * 16u)) = v[k];
+            }
         }
+#pragma unroll
+        for (int k0 = 0; k0 < kWgPieces; k0 += kWgBatch) {
+            bf16x8 v[kWgBatch], m[kWgBatch];
+#pragma unroll
+            for (int k = 0; k < kWgBatch; ++k) {
+                const uint32_t pc = pd[k0 + k];
+                v[k] = zero8();
+                m[k] = zero8();
+                if (pc != ~0u) {
+                    const uint32_t iy = uint32_t(y0) + (pc & 255u), ix = uint32_t(x0) + ((pc >> 8) & 255u);
+                    if (iy < d.h && ix < d.w) {
+                        const size_t p = pix0 + iy * d.w + ix;
+                        const uint32_t ch = (pc >> 16) * 8u;
+                        v[k] = load8(ga.p + p * ga.stride + ga.off + ch);
+                        if (a.p) m[k] = load8(a.p + p * a.stride + a.off + ch);
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kWgBatch; ++k) {
+                const uint32_t pc = pd[k0 + k];
+                if (pc == ~0u) continue;
+                if (a.p) {
+#pragma unroll
+                    for (int i = 0; i < 8; ++i)
+                        if (!(f(m[k][i]) > 0.0f)) v[k][i] = b(0.0f);
+                }
+                const uint32_t ty = pc & 255u, tx = (pc >> 8) & 255u, ch = (pc >> 16) * 8u;
+                *(lds_bf16x8*)(dzs + (ty * g.tw + tx) * g.dpitch + ch * 2u) = v[k];
+                const uint32_t iy = uint32_t(y0) + ty, ix = uint32_t(x0) + tx;
+                if (dz_out && lead && iy < d.h && ix < d.w) store8(dz_out + (pix0 + iy * d.w + ix) * cout + ch, v[k]);
+            }
+        }
+        __syncthreads();
+        for (uint32_t s = 0; s < g.steps; ++s) {
+            bf16x8 af[MTX], bx[NB];
+#pragma unroll
+            for (int m = 0; m < MTX; ++m)
+                if (uint32_t(m) < mt) af[m] = tr_frag(dzs + ld + s * 32u * g.dpitch + uint32_t(m) * 32u, 16u * g.dpitch);
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                if (uint32_t(j) < nb) bx[j] = tr_frag(xs + lx + s * xstep + boff[j], xhalf);
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+#pragma unroll
+                for (int m = 0; m < MTX; ++m)
+                    if (uint32_t(j) < nb && uint32_t(m) < mt)
+                        acc[j][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], bx[j], acc[j][m], 0, 0, 0);
+            if (do_bias) {
+#pragma unroll
+                for (int m = 0; m < MTX; ++m)
+                    if (uint32_t(m) < mt) bacc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[m], ones, bacc[m], 0, 0, 0);
+            }
+        }
+    }
+    const uint32_t nw = 9u * cout * cin;
+    float* out = partial + size_t(blockIdx.x) * (nw + (bias ? cout : 0u));
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        if (uint32_t(j) >= nb) continue;
+        const uint32_t fj = f0 + wv + 4u * uint32_t(j), tap = fj % 9u;
+        const uint32_t ci = (fj / 9u) * 16u + (lane & 15u);
+#pragma unroll
+        for (int m = 0; m < MTX; ++m) {
+            if (uint32_t(m) >= mt) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const uint32_t co = uint32_t(m) * 16u + (lane >> 4) * 4u + uint32_t(r);
+                out[(size_t(tap) * cout + co) * cin + ci] = acc[j][m][r];
+            }
+        }
+    }
+    if (do_bias && (lane & 15u) == 0u) {
+#pragma unroll
+        for (int m = 0; m < MTX; ++m) {
+            if (uint32_t(m) >= mt) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) out[nw + uint32_t(m) * 16u + (lane >> 4) * 4u + uint32_t(r)] = bacc[m][r];
+        }
+    }
 }
 
 struct WgPlan {
-    int kind;        // 0: <2, 2, 3>, 1: <4, 4, 1>, 2: <8, 2, 1>
-    uint32_t T, NT;  // taps per job, ci tiles per job
-    uint32_t jobs, ci_groups, chunks, chunk;
+    WgGeom g;
+    uint32_t mtx, nb;  // template: output tiles (at least cout / 16), fragments per wave
+    uint32_t groups, chunks;
+    size_t lds;
 };
 
-static WgPlan wgrad_plan(uint32_t cin, uint32_t cout, Dims d) {
-    WgPlan p{};
+static uint32_t pieces(uint32_t rows, uint32_t cols, uint32_t c8) { return (rows * cols * c8 + kBlock - 1u) / kBlock; }
+
+static bool wgrad_plan(uint32_t cin, uint32_t cout, Dims d, WgPlan& p) {
+    p = WgPlan{};
     const uint32_t mt = cout / 16u;
-    if (mt <= 2u) { p.kind = 0; p.T = 3; p.NT = 2; }
-    else if (mt <= 4u) { p.kind = 1; p.T = 1; p.NT = 4; }
-    else { p.kind = 2; p.T = 1; p.NT = 2; }
-    const uint32_t it = cin / 16u;
-    p.ci_groups = (it + p.NT - 1u) / p.NT;
-    p.jobs = (9u / p.T) * p.ci_groups;
-    const uint32_t Ps = chw_stride_of(d.pixels());
-    const uint32_t steps = Ps / kWgStep;
-    // about 2048 waves, at least 16 K steps per wave
-    uint32_t chunks = 2048u / p.jobs;
-    chunks = std::max(1u, std::min(chunks, steps / 16u));
-    const uint32_t per = (steps + chunks - 1u) / chunks;
-    p.chunk = per * kWgStep;
-    p.chunks = (Ps + p.chunk - 1u) / p.chunk;
-    return p;
+    p.mtx = mt <= 4u ? mt : (mt <= 6u ? 6u : 8u);
+    p.nb = p.mtx <= 3u ? 8u : p.mtx == 4u ? 6u : 2u;  // <= 96 accumulators
+    WgGeom& g = p.g;
+    g.frags = 9u * (cin / 16u);
+    g.tw = d.w >= 32u ? 32u : (d.w >= 16u ? 16u : 8u);
+    g.dpitch = 32u * (mt | 1u);
+    const uint32_t gmin = (g.frags + 4u * p.nb - 1u) / (4u * p.nb);
+    for (uint32_t steps = 8; steps >= 2u; steps /= 2u) {
+        g.steps = steps;
+        g.th = 32u * steps / g.tw;
+        if (pieces(g.th, g.tw, cout / 8u) > uint32_t(kWgPieces)) continue;
+        for (uint32_t groups = gmin; groups <= g.frags; ++groups) {
+            const uint32_t per = (g.frags + groups - 1u) / groups;
+            uint32_t span = 0;
+            for (uint32_t f0 = 0; f0 < g.frags; f0 += per)
+                span = std::max(span, (std::min(f0 + per, g.frags) - 1u) / 9u - f0 / 9u + 1u);
+            const size_t lds = size_t(g.th + 2u) * (g.tw + 2u) * 32u * (span | 1u) + size_t(32u * steps) * g.dpitch;
+            if (lds <= kWgLds && pieces(g.th + 2u, g.tw + 2u, 2u * span) <= uint32_t(kWgPieces)) {
+                g.per_group = per;
+                p.groups = (g.frags + per - 1u) / per;
+                p.lds = lds;
+                g.tiles_x = (d.w + g.tw - 1u) / g.tw;
+                g.tiles_y = (d.h + g.th - 1u) / g.th;
+                g.tiles = d.n * g.tiles_x * g.tiles_y;
+                const uint32_t want = std::max(1u, std::min(g.tiles, kWgBlocks / p.groups));
+                g.per_chunk = (g.tiles + want - 1u) / want;
+                p.chunks = (g.tiles + g.per_chunk - 1u) / g.per_chunk;
+                return true;
+            }
+        }
+    }
+    return false;
 }
 
 size_t wgrad_partial_floats(uint32_t cin, uint32_t cout, Dims d) {
-    const WgPlan p = wgrad_plan(cin, cout, d);
-    const size_t n = size_t(9) * cout * cin;
+    WgPlan p;
+    if (!wgrad_plan(cin, cout, d, p)) return 0;
+    const size_t n = size_t(9) * cout * cin + cout;
     return size_t(p.chunks) * n + size_t(red_segs(p.chunks)) * n;
 }
 
-hipError_t conv3x3_wgrad(const bf16* dzc, uint32_t cout, const bf16* xs, uint32_t cin, Dims d, float* partial,
-                         float* dw, uint32_t ci_off, uint32_t cin_total, hipStream_t s) {
+hipError_t conv3x3_wgrad(const Tens& ga, const Tens* a, uint32_t cout, const Tens& x, uint32_t cin, Dims d,
+                         bf16* dz_out, float* partial, float* dw, uint32_t ci_off, uint32_t cin_total, float* db,
+                         hipStream_t s) {
     if (cin == 0 || cin % 16u || cin > kMaxCout || cout % 16u || cout == 0 || cout > kMaxCout ||
-        ci_off + cin > cin_total || d.pixels() == 0)
+        ci_off + cin > cin_total || d.pixels() == 0 || x.stride % 8u || x.off % 8u || ga.stride % 8u ||
+        ga.off % 8u || (a && (a->stride % 8u || a->off % 8u)) || d.h > 65535u || d.w > 65535u)
         return hipErrorInvalidValue;
-    const WgPlan p = wgrad_plan(cin, cout, d);
-    const dim3 grid(p.chunks, (p.jobs + 3u) / 4u);
-    const bool full9 = x_copies(d) == 9u;
-#define PT_WGRAD_LAUNCH(MT_, NT_, T_)                                                                               \
-    if (full9)                                                                                                      \
-        hipLaunchKernelGGL((wgrad_k<MT_, NT_, T_, true>), grid, dim3(kBlock), 0, s, dzc, cout, xs, cin, d, p.chunk, \
-                           p.ci_groups, p.jobs, partial);                                                          \
-    else                                                                                                            \
-        hipLaunchKernelGGL((wgrad_k<MT_, NT_, T_, false>), grid, dim3(kBlock), 0, s, dzc, cout, xs, cin, d,        \
-                           p.chunk, p.ci_groups, p.jobs, partial);
-    switch (p.kind) {
-        case 0: PT_WGRAD_LAUNCH(2, 2, 3) break;
-        case 1: PT_WGRAD_LAUNCH(4, 4, 1) break;
-        default: PT_WGRAD_LAUNCH(8, 2, 1) break;
+    WgPlan p;
+    if (!wgrad_plan(cin, cout, d, p)) return hipErrorInvalidValue;
+    const dim3 grid(p.chunks, p.groups);
+    const Tens an = a ? *a : Tens{nullptr, 0u, 0u};
+    const bool bias = db != nullptr;
+#define PT_WGRAD_LAUNCH(MT_, NB_)                                                                                   \
+    hipLaunchKernelGGL((wgrad_k<MT_, NB_>), grid, dim3(kBlock), p.lds, s, ga, an, cout, x, cin, d, p.g, dz_out, bias, \
+                       partial)
+    switch (p.mtx) {
+        case 1: PT_WGRAD_LAUNCH(1, 8); break;
+        case 2: PT_WGRAD_LAUNCH(2, 8); break;
+        case 3: PT_WGRAD_LAUNCH(3, 8); break;
+        case 4: PT_WGRAD_LAUNCH(4, 6); break;
+        case 6: PT_WGRAD_LAUNCH(6, 2); break;
+        default: PT_WGRAD_LAUNCH(8, 2); break;
     }
 #undef PT_WGRAD_LAUNCH
-    const uint32_t n = 9u * cout * cin;
-    return reduce_chunks(partial, p.chunks, n, cin, cin_total, ci_off, partial + size_t(p.chunks) * n, dw, s);
+    const uint32_t nw = 9u * cout * cin, n = nw + (bias ? cout : 0u);
+    return reduce_chunks(partial, p.chunks, n, nw, cin, cin_total, ci_off, partial + size_t(p.chunks) * n, dw, db, s);
 }
 
 // --------------------------------------------------------- elementwise --

@@ -74,15 +74,15 @@ struct UNet {
     Dims dims[5] = {};
     bool have_acts = false;
     DevBuf in0, tA[4], cat[4], pin[5], t4A, b4, dA[4], dd[4], pred;
-    DevBuf g_tA[4], g_cat[4], g_pin[5], g_t4A, g_b4, g_dA[4], g_dd[4], g_pred, dz, dzc, xs;
-    DevBuf target, feat, loss, lpart, wpart, bpart;
+    DevBuf g_tA[4], g_cat[4], g_pin[5], g_t4A, g_b4, g_dA[4], g_dd[4], g_pred, dz;
+    DevBuf target, feat, loss, lpart, wpart;
     // dataset
     DevBuf ds_in, ds_tg;
     uint32_t ds_n = 0, ds_h = 0, ds_w = 0;
 
     std::vector<DevBuf*> all_bufs() {
         std::vector<DevBuf*> bufs{&params, &grads, &m, &v, &wbf, &wdt, &in0, &t4A, &b4, &pred, &g_t4A, &g_b4,
-                                  &g_pred, &dz, &dzc, &xs, &target, &feat, &loss, &wpart, &bpart, &ds_in,
+                                  &g_pred, &dz, &target, &feat, &loss, &wpart, &ds_in,
                                   &ds_tg, &w8, &inv_sa, &sw, &amax, &hist};
         for (int i = 0; i < 4; ++i) {
             for (DevBuf* b : {&tA[i], &cat[i], &dA[i], &dd[i], &g_tA[i], &g_cat[i], &g_dA[i], &g_dd[i]})
@@ -228,22 +228,18 @@ int alloc_acts(pt_ctx* c, UNet& u, Dims d0) {
         (rc = A(u.pred, P(0) * 3, 4, "unet pred")) || (rc = A(u.g_pred, P(0) * kOutC, 2, "unet grad")) ||
         (rc = A(u.target, P(0) * 3, 4, "unet target")) || (rc = A(u.feat, P(0) * kFeat, 4, "unet features")))
         return rc;
-    // backward scratch, sized for the largest layer: dz (NHWC and channel-major),
-    // the staged layer input, and the partial sums of the weight and bias gradients
-    size_t dz = 0, xs = 0, wp = 0, bp = 0;
+    // backward scratch, sized for the largest layer: dz and the partial sums of
+    // the weight and bias gradients
+    size_t dz = 0, wp = 0;
     const int lvl[kLayers] = {0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 3, 3, 2, 2, 1, 1, 0, 0, 0};
     for (uint32_t i = 0; i < kLayers; ++i) {
         const Layer& l = u.L[i];
         const Dims d = u.dims[lvl[i]];
-        dz = std::max(dz, chw_elems(l.cout, d));
-        for (uint32_t k = 0; k < slices(l); ++k) {
-            xs = std::max(xs, x_copies(d) * chw_elems(slice_count(l, k), d));
-            wp = std::max(wp, wgrad_partial_floats(slice_count(l, k), l.cout, d));
-        }
-        bp = std::max(bp, dz_scratch_floats(l.cout, d));
+        dz = std::max(dz, size_t(d.pixels()) * l.cout);
+        for (uint32_t k = 0; k < slices(l); ++k) wp = std::max(wp, wgrad_partial_floats(slice_count(l, k), l.cout, d));
     }
-    if ((rc = A(u.dz, dz, 2, "unet dz")) || (rc = A(u.dzc, dz, 2, "unet dz")) || (rc = A(u.xs, xs, 2, "unet xs")) ||
-        (rc = A(u.wpart, wp, 4, "unet wgrad scratch")) || (rc = A(u.bpart, bp, 4, "unet bias scratch")) ||
+    if ((rc = A(u.dz, dz, 2, "unet dz")) ||
+        (rc = A(u.wpart, wp, 4, "unet gradient scratch")) ||
         (rc = A(u.loss, 1024, 4, "unet loss")) ||
         (rc = A(u.lpart, loss_partials(uint32_t(P(0))), 4, "unet loss partials")))
         return rc;
@@ -338,17 +334,19 @@ int forward(pt_ctx* c, UNet& u, bool q8 = false, const ConvFuse* out_fuse = null
 int conv_back(pt_ctx* c, UNet& u, uint32_t li, const Tens& x, const Tens& a, const Tens& ga, const Tens& gx, Dims d) {
     const Layer& l = u.L[li];
     hipStream_t s = c->stream;
-    hipError_t e = dz_prepare(ga, l.relu ? &a : nullptr, l.cout, d, bfp(u.dz), bfp(u.dzc), fp(u.bpart),
-                              fp(u.grads) + l.b_off, s);
+    // the masked gradient dz goes to u.dz for the data gradient (a linear
+    // layer's is ga itself); the first slice's pass also sums the bias
+    const Tens dz = l.relu ? T(u.dz, l.cout) : ga;
+    hipError_t e = hipSuccess;
     size_t so = l.wbf_off;
     for (uint32_t k = 0; e == hipSuccess && k < slices(l); ++k) {
         const uint32_t c0 = slice_begin(k), nc = slice_count(l, k);
-        e = x_prepare(Tens{x.p, x.stride, x.off + c0}, nc, d, bfp(u.xs), s);
-        if (e == hipSuccess)
-            e = conv3x3_wgrad(bfp(u.dzc), l.cout, bfp(u.xs), nc, d, fp(u.wpart), fp(u.grads) + l.w_off, c0, l.cin, s);
+        e = conv3x3_wgrad(ga, l.relu ? &a : nullptr, l.cout, Tens{x.p, x.stride, x.off + c0}, nc, d,
+                          k == 0 && gx.p && l.relu ? bfp(u.dz) : nullptr, fp(u.wpart), fp(u.grads) + l.w_off, c0,
+                          l.cin, k == 0 ? fp(u.grads) + l.b_off : nullptr, s);
         if (e == hipSuccess && gx.p) {
             const Tens gxs{gx.p, gx.stride, gx.off + c0};
-            e = conv3x3(T(u.dz, l.cout), l.cout, bfp(u.wdt) + so, nullptr, gxs, nc, d, false, nullptr, 0, s);
+            e = conv3x3(dz, l.cout, bfp(u.wdt) + so, nullptr, gxs, nc, d, false, nullptr, 0, s);
         }
         so += size_t(9) * nc * l.cout;
     }
