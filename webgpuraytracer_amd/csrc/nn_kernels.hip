@@ -22,6 +22,8 @@ namespace pt {
 namespace nn {
 
 typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef short i16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
@@ -83,8 +85,10 @@ struct ConvSmem {
 // measured 4% faster per 1080p frame than prefetching where cin > 32).  UP:
 // input channels below fz.cup are read from the half-resolution fz.up at the
 // nearest coarse pixel, the others from `in` (a decoder level's first
-// convolution over [upsampled | skip] without the upsampled copy).
-template <int NCT, bool RELU, bool F32OUT, bool FULL, bool UP = false>
+// convolution over [upsampled | skip] without the upsampled copy).  PF
+// (launches of few blocks over several slices, where the extra registers
+// cost no occupancy): the next slice is fetched during this slice's products.
+template <int NCT, bool RELU, bool F32OUT, bool FULL, bool UP = false, bool PF = false>
 __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const bf16* __restrict__ w,
                                                     const float* __restrict__ bias, Tens out, uint32_t cout, Dims d,
                                                     float* __restrict__ of32, uint32_t f32c, ConvFuse fz) {
@@ -92,69 +96,96 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
     constexpr uint32_t kWPieces = 9u * NCT * 16u * 4u;
     constexpr uint32_t kHaloPer = (kHaloPieces + kBlock - 1u) / kBlock;  // 6
     constexpr uint32_t kWPer = (kWPieces + kBlock - 1u) / kBlock;
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6, kq = lane >> 4;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6, kq = lane >> 4, ln = lane & 15u;
     const uint32_t tiles_x = (d.w + kTile - 1u) / kTile, tiles_y = (d.h + kTile - 1u) / kTile;
-    const uint32_t img = blockIdx.x / (tiles_x * tiles_y), trem = blockIdx.x % (tiles_x * tiles_y);
-    const int ty0 = int(trem / tiles_x * kTile), tx0 = int(trem % tiles_x * kTile);
     const uint32_t co0 = blockIdx.y * uint32_t(NCT) * 16u;
     const uint32_t nct = FULL ? uint32_t(NCT) : min(uint32_t(NCT), (cout - co0) / 16u);
-    // the pieces this thread stages: halo pixel j, channels (piece % 4) * 8
-    const bf16* hsrc[kHaloPer];
-    uint32_t cpix[UP ? kHaloPer : 1];
-    bool hval[kHaloPer];
+    // This thread's halo pieces are the same in every tile: 8-channel group
+    // tid % 4 of halo pixels tid / 4 + 64 k (row << 8 | column, ~0u past the halo).
+    const uint32_t hc = (tid & 3u) * 8u;
+    uint32_t hpos[kHaloPer];
 #pragma unroll
     for (uint32_t k = 0; k < kHaloPer; ++k) {
-        const uint32_t piece = tid + k * kBlock;
-        const uint32_t j = piece / 4u;
-        const int gy = ty0 + int(j / kHalo) - 1, gx = tx0 + int(j % kHalo) - 1;
-        hval[k] = piece < kHaloPieces && gy >= 0 && gy < int(d.h) && gx >= 0 && gx < int(d.w);
-        const size_t sp = hval[k] ? (size_t(img) * d.h + uint32_t(gy)) * d.w + uint32_t(gx) : 0;
-        hsrc[k] = in.p + sp * in.stride + in.off + (piece % 4u) * 8u;
-        if (UP)
-            cpix[UP ? k : 0] =
-                hval[k] ? (img * (d.h / 2u) + uint32_t(gy) / 2u) * (d.w / 2u) + uint32_t(gx) / 2u : 0u;
+        const uint32_t j = (tid >> 2) + k * (kBlock / 4u);
+        hpos[k] = j < kHaloPix ? ((j / kHalo) << 8) | (j % kHalo) : ~0u;
     }
-    bf16x8 hreg[kHaloPer], wreg[kWPer];
-    auto fetch = [&](uint32_t c0) {
-#pragma unroll
-        for (uint32_t k = 0; k < kHaloPer; ++k) {
-            const uint32_t c = c0 + ((tid + k * kBlock) % 4u) * 8u;
-            if (UP && c < fz.cup)
-                hreg[k] = hval[k] ? load8(fz.up.p + size_t(cpix[UP ? k : 0]) * fz.up.stride + fz.up.off + c) : zero8();
-            else
-                hreg[k] = (hval[k] && c < cin) ? load8(hsrc[k] + c0) : zero8();
-        }
+    auto fetch_w = [&](uint32_t c0, bf16x8* wr) {
 #pragma unroll
         for (uint32_t k = 0; k < kWPer; ++k) {
             const uint32_t piece = tid + k * kBlock;
             const uint32_t c8 = piece % 4u, co = (piece / 4u) % (NCT * 16u), tap = piece / (4u * NCT * 16u);
             const uint32_t c = c0 + c8 * 8u;
             const bool ok = piece < kWPieces && co0 + co < cout && c < cin;
-            wreg[k] = ok ? load8(w + (size_t(tap) * cout + co0 + co) * cin + c) : zero8();
+            wr[k] = ok ? load8(w + ((tap * cout + co0 + co) * cin + c)) : zero8();
         }
     };
-    auto stash = [&]() {
-#pragma unroll
-        for (uint32_t k = 0; k < kHaloPer; ++k) {
-            const uint32_t piece = tid + k * kBlock;
-            if (piece < kHaloPieces) sm.halo[(piece % 4u) * kHaloPix + piece / 4u] = hreg[k];
-        }
+    auto stash_w = [&](const bf16x8* wr) {
 #pragma unroll
         for (uint32_t k = 0; k < kWPer; ++k) {
             const uint32_t piece = tid + k * kBlock;
-            if (piece < kWPieces) sm.w[(piece % 4u) * (kWPieces / 4u) + piece / 4u] = wreg[k];
+            if (piece < kWPieces) sm.w[(piece % 4u) * (kWPieces / 4u) + piece / 4u] = wr[k];
         }
+    };
+    // The products run as C[co][pixel] = W[co][ci] X[ci][pixel] (weights as
+    // the A operand), so lane (kq, ln) receives output channels 4 kq .. +3
+    // of pixel ln of a 16-pixel row: 4 consecutive channels, one 8-byte piece.
+    // The accumulators start at the bias.
+    float bv[NCT][4];
+#pragma unroll
+    for (int t = 0; t < NCT; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) bv[t][r] = 0.0f;
+    if (bias) {
+#pragma unroll
+        for (int t = 0; t < NCT; ++t)
+            if (uint32_t(t) < nct)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) bv[t][r] = bias[co0 + uint32_t(t) * 16u + kq * 4u + uint32_t(r)];
+    }
+    const uint32_t img = blockIdx.x / (tiles_x * tiles_y), trem = blockIdx.x % (tiles_x * tiles_y);
+    const uint32_t ty0 = trem / tiles_x * kTile, tx0 = trem % tiles_x * kTile;
+    const uint32_t ibase = img * d.h * d.w;  // element offsets fit 32 bits (checked by the host)
+    uint32_t hsrc[kHaloPer];
+    uint32_t cpix[UP ? kHaloPer : 1];
+    bool hval[kHaloPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kHaloPer; ++k) {
+        const uint32_t gy = ty0 + (hpos[k] >> 8) - 1u, gx = tx0 + (hpos[k] & 255u) - 1u;  // wraps below 0
+        hval[k] = hpos[k] != ~0u && gy < d.h && gx < d.w;
+        const uint32_t sp = hval[k] ? gy * d.w + gx : 0u;
+        hsrc[k] = (ibase + sp) * in.stride + in.off + hc;
+        if (UP) cpix[UP ? k : 0] = hval[k] ? (img * (d.h / 2u) + gy / 2u) * (d.w / 2u) + gx / 2u : 0u;
+    }
+    bf16x8 hreg[kHaloPer], wreg[kWPer];
+    auto fetch = [&](uint32_t c0) {
+        const uint32_t c = c0 + hc;
+#pragma unroll
+        for (uint32_t k = 0; k < kHaloPer; ++k) {
+            if (UP && c < fz.cup)
+                hreg[k] = hval[k] ? load8(fz.up.p + (cpix[UP ? k : 0] * fz.up.stride + fz.up.off + c)) : zero8();
+            else
+                hreg[k] = (hval[k] && c < cin) ? load8(in.p + (hsrc[k] + c0)) : zero8();
+        }
+        fetch_w(c0, wreg);
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (uint32_t k = 0; k < kHaloPer; ++k)
+            if (hpos[k] != ~0u) sm.halo[(tid & 3u) * kHaloPix + (tid >> 2) + k * (kBlock / 4u)] = hreg[k];
+        stash_w(wreg);
     };
     f32x4 acc[4][NCT];
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
-        for (int t = 0; t < NCT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < NCT; ++t) acc[m][t] = f32x4{bv[t][0], bv[t][1], bv[t][2], bv[t][3]};
+    if (PF) fetch(0u);
     for (uint32_t c0 = 0; c0 < cin; c0 += kSlice) {
-        fetch(c0);
+        if (!PF) fetch(c0);
         __syncthreads();  // the previous slice's reads are done
         stash();
         __syncthreads();
+        if (PF && c0 + kSlice < cin) fetch(c0 + kSlice);
         // (reading the wave's 6 halo rows once per horizontal offset instead of
         // 4 rows per tap measured ~5% slower: the loads then wait together)
 #pragma unroll
@@ -162,43 +193,38 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
             const uint32_t dy = tap / 3u, dx = tap % 3u;
             bf16x8 a[4], bw[NCT];
 #pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const uint32_t hp = (wv * 4u + uint32_t(m) + dy) * kHalo + (lane & 15u) + dx;
-                a[m] = sm.halo[kq * kHaloPix + hp];
-            }
+            for (int m = 0; m < 4; ++m) a[m] = sm.halo[kq * kHaloPix + (wv * 4u + uint32_t(m) + dy) * kHalo + ln + dx];
 #pragma unroll
-            for (int t = 0; t < NCT; ++t)
-                bw[t] = sm.w[kq * (kWPieces / 4u) + tap * NCT * 16u + uint32_t(t) * 16u + (lane & 15u)];
+            for (int t = 0; t < NCT; ++t) bw[t] = sm.w[kq * (kWPieces / 4u) + tap * NCT * 16u + uint32_t(t) * 16u + ln];
 #pragma unroll
             for (int m = 0; m < 4; ++m)
 #pragma unroll
                 for (int t = 0; t < NCT; ++t)
                     if (uint32_t(t) < nct)
-                        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], bw[t], acc[m][t], 0, 0, 0);
+                        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[t], a[m], acc[m][t], 0, 0, 0);
         }
     }
-    // C[(lane / 16) * 4 + r][lane % 16]: tile column kq * 4 + r, channel lane % 16
+    // acc[m][t][r]: channel co0 + 16 t + 4 kq + r of tile pixel (4 wv + m, ln)
     if (FULL && F32OUT) {
         // the first f32c (<= 4) channels as fp32 [pixel][f32c]: a tile row is
         // 16 * f32c consecutive floats, written as 16-byte pieces
         float* stage = reinterpret_cast<float*>(&sm);
-        static_assert(kTile * kTile * 4u * 4u <= sizeof(ConvSmem<NCT>), "fp32 output stage fits the LDS tile");
+        static_assert(kTile * kTile * 4u * 4u <= sizeof(sm.halo), "fp32 output stage fits the halo area");
         __syncthreads();  // every wave is done with the last slice
-        const uint32_t ch = lane & 15u;
-        if (ch < f32c) {
-            const float bv = bias ? bias[co0 + ch] : 0.0f;
+        if (kq * 4u < f32c) {
 #pragma unroll
             for (int m = 0; m < 4; ++m)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    float v = acc[m][0][r] + bv;
+                    const uint32_t ch = kq * 4u + uint32_t(r);
+                    float v = acc[m][0][r];
                     if (RELU) v = fmaxf(v, 0.0f);
-                    stage[((wv * 4u + uint32_t(m)) * kTile + kq * 4u + uint32_t(r)) * f32c + ch] = v;
+                    if (ch < f32c) stage[((wv * 4u + uint32_t(m)) * kTile + ln) * f32c + ch] = v;
                 }
         }
         __syncthreads();
         if (fz.fin) {  // the frame's denoised texels straight from the stage (f32c = 3, one image)
-            const uint32_t x = uint32_t(tx0) + tid % kTile, y = uint32_t(ty0) + tid / kTile;
+            const uint32_t x = tx0 + tid % kTile, y = ty0 + tid / kTile;
             if (x < fz.fw && y < fz.fh) {
                 const size_t i = size_t(y) * fz.fw + x;
                 frame_texel(stage + tid * f32c, fz.exposure, fz.fin + i, fz.ldr ? fz.ldr + i : nullptr);
@@ -208,8 +234,8 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
         const uint32_t per_row = kTile * f32c / 4u;  // 16-byte pieces per tile row
         for (uint32_t i = tid; i < kTile * per_row; i += kBlock) {
             const uint32_t row = i / per_row, k = i % per_row;
-            const size_t op = (size_t(img) * d.h + uint32_t(ty0) + row) * d.w + uint32_t(tx0);
-            *reinterpret_cast<float4*>(of32 + op * f32c + k * 4u) =
+            const uint32_t op = (img * d.h + ty0 + row) * d.w + tx0;
+            *reinterpret_cast<float4*>(of32 + (op * f32c + k * 4u)) =
                 *reinterpret_cast<const float4*>(stage + row * kTile * f32c + k * 4u);
         }
         return;
@@ -220,26 +246,27 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
         char* stage = reinterpret_cast<char*>(&sm);
         __syncthreads();  // every wave is done with the last slice
 #pragma unroll
-        for (int t = 0; t < NCT; ++t) {
-            const uint32_t ch = uint32_t(t) * 16u + (lane & 15u);
-            const float bv = bias ? bias[co0 + ch] : 0.0f;
+        for (int t = 0; t < NCT; ++t)
 #pragma unroll
-            for (int m = 0; m < 4; ++m)
+            for (int m = 0; m < 4; ++m) {
+                bf16x4 q;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    float v = acc[m][t][r] + bv;
-                    if (RELU) v = fmaxf(v, 0.0f);
-                    const uint32_t p = (wv * 4u + uint32_t(m)) * kTile + kq * 4u + uint32_t(r);
-                    *reinterpret_cast<bf16*>(stage + p * kPitch + ch * 2u) = b(v);
-                }
-        }
+                for (int r = 0; r < 4; ++r) q[r] = b(acc[m][t][r]);
+                // ReLU on the rounded values as signed 16-bit integers: a bf16
+                // is negative exactly when its bit pattern is (max with +0)
+                i16x4 qi = __builtin_bit_cast(i16x4, q);
+                if (RELU) qi = __builtin_elementwise_max(qi, i16x4{0, 0, 0, 0});
+                const uint32_t p = (wv * 4u + uint32_t(m)) * kTile + ln;
+                *reinterpret_cast<i16x4*>(stage + p * kPitch + (uint32_t(t) * 16u + kq * 4u) * 2u) = qi;
+            }
         __syncthreads();
         constexpr uint32_t kChunks = kTile * kTile * kCo / 8u;  // 16-byte pieces of the tile
+        const uint32_t obase = (img * d.h + ty0) * d.w + tx0;
 #pragma unroll
         for (uint32_t i = tid; i < kChunks; i += kBlock) {
             const uint32_t p = i / (kCo / 8u), c8 = i % (kCo / 8u);
-            const size_t op = (size_t(img) * d.h + uint32_t(ty0) + p / kTile) * d.w + uint32_t(tx0) + p % kTile;
-            store8(out.p + op * out.stride + out.off + co0 + c8 * 8u,
+            const uint32_t op = obase + (p / kTile) * d.w + p % kTile;
+            store8(out.p + (op * out.stride + out.off + co0 + c8 * 8u),
                    *reinterpret_cast<const bf16x8*>(stage + p * kPitch + c8 * 16u));
         }
         if (fz.pool.p) {  // the 8 x 8 pooled tile, summed in pool2's order
@@ -255,9 +282,8 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
                 bf16x8 o;
 #pragma unroll
                 for (int k = 0; k < 8; ++k) o[k] = b(0.25f * (f(v0[k]) + f(v1[k]) + f(v2[k]) + f(v3[k])));
-                const size_t cp = (size_t(img) * (d.h / 2u) + uint32_t(ty0) / 2u + qy) * (d.w / 2u) +
-                                  uint32_t(tx0) / 2u + qx;
-                store8(fz.pool.p + cp * fz.pool.stride + fz.pool.off + co0 + c8 * 8u, o);
+                const uint32_t cp = (img * (d.h / 2u) + ty0 / 2u + qy) * (d.w / 2u) + tx0 / 2u + qx;
+                store8(fz.pool.p + (cp * fz.pool.stride + fz.pool.off + co0 + c8 * 8u), o);
             }
         }
         if (fz.upw.p) {  // each output pixel copied to its 2 x 2 block of the fine tensor (as up2)
@@ -266,8 +292,8 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
                 const uint32_t c8 = i % (kCo / 8u), f = i / (kCo / 8u);  // f: fine pixel of the 32 x 32 block
                 const uint32_t fy = f / (2u * kTile), fx = f % (2u * kTile);
                 const uint32_t p = (fy / 2u) * kTile + fx / 2u;
-                const size_t fp = (size_t(img) * 2u * d.h + 2u * uint32_t(ty0) + fy) * W2 + 2u * uint32_t(tx0) + fx;
-                store8(fz.upw.p + fp * fz.upw.stride + fz.upw.off + co0 + c8 * 8u,
+                const uint32_t fp = (img * 2u * d.h + 2u * ty0 + fy) * W2 + 2u * tx0 + fx;
+                store8(fz.upw.p + (fp * fz.upw.stride + fz.upw.off + co0 + c8 * 8u),
                        *reinterpret_cast<const bf16x8*>(stage + p * kPitch + c8 * 16u));
             }
         }
@@ -276,18 +302,15 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
 #pragma unroll
     for (int t = 0; t < NCT; ++t) {
         if (uint32_t(t) >= nct) continue;
-        const uint32_t co = co0 + uint32_t(t) * 16u + (lane & 15u);
-        const float bv = bias ? bias[co] : 0.0f;
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
-            const int y = ty0 + int(wv * 4u) + m;
-            if (y >= int(d.h)) continue;
+            const uint32_t y = ty0 + wv * 4u + uint32_t(m), x = tx0 + ln;
+            if (y >= d.h || x >= d.w) continue;
+            const uint32_t op = (img * d.h + y) * d.w + x;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int x = tx0 + int(kq * 4u) + r;
-                if (x >= int(d.w)) continue;
-                const size_t op = (size_t(img) * d.h + uint32_t(y)) * d.w + uint32_t(x);
-                float v = acc[m][t][r] + bv;
+                const uint32_t co = co0 + uint32_t(t) * 16u + kq * 4u + uint32_t(r);
+                float v = acc[m][t][r];
                 if (RELU) v = fmaxf(v, 0.0f);
                 if (F32OUT) {
                     if (co < f32c) of32[op * f32c + co] = v;
@@ -298,6 +321,9 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
         }
     }
 }
+
+// Launches of fewer blocks than this prefetch the next input slice.
+constexpr uint32_t kConvPfBlocks = 1024;
 
 template <int NCT>
 static void conv_launch(const Tens& in, uint32_t cin, const bf16* w, const float* bias, const Tens& out,
@@ -313,22 +339,31 @@ static void conv_launch(const Tens& in, uint32_t cin, const bf16* w, const float
                           (kTile * f32c) % 4u == 0 && reinterpret_cast<uintptr_t>(of32) % 16u == 0;
 #define PT_CONV(...) hipLaunchKernelGGL((conv3x3_k<__VA_ARGS__>), grid, dim3(kBlock), 0, s, in, cin, w, bias, out, \
                                         cout, d, of32, f32c, fz)
+#define PT_CONV_PF(RELU_, FULL_, UP_)                 \
+    do {                                              \
+        if (pf)                                       \
+            PT_CONV(NCT, RELU_, false, FULL_, UP_, true); \
+        else                                          \
+            PT_CONV(NCT, RELU_, false, FULL_, UP_);   \
+    } while (0)
+    const bool pf = cin > kSlice && tiles * groups < kConvPfBlocks;
     if (of32 && f32_full)
         PT_CONV(NCT, false, true, true);
     else if (of32)
         PT_CONV(NCT, false, true, false);
     else if (fz.up.p && full)
-        PT_CONV(NCT, true, false, true, true);
+        PT_CONV_PF(true, true, true);
     else if (fz.up.p)
-        PT_CONV(NCT, true, false, false, true);  // the decoders' first convolutions have ReLU
+        PT_CONV_PF(true, false, true);  // the decoders' first convolutions have ReLU
     else if (relu && full)
-        PT_CONV(NCT, true, false, true);
+        PT_CONV_PF(true, true, false);
     else if (relu)
-        PT_CONV(NCT, true, false, false);
+        PT_CONV_PF(true, false, false);
     else if (full)
-        PT_CONV(NCT, false, false, true);
+        PT_CONV_PF(false, true, false);
     else
-        PT_CONV(NCT, false, false, false);
+        PT_CONV_PF(false, false, false);
+#undef PT_CONV_PF
 #undef PT_CONV
 }
 
@@ -339,11 +374,22 @@ hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bia
     if (d.pixels() == 0) return hipSuccess;
     ConvFuse fz{};
     if (fuse) fz = *fuse;
+    // the kernels index every tensor with 32-bit element offsets
+    const uint64_t P = d.pixels(), lim = 1ull << 31;
+    if (P * (in.stride + 8u) >= lim || P * (out.stride + 8u) >= lim || P * (f32_channels + 4u) >= lim ||
+        4u * P * (fz.upw.stride + 8u) >= lim || P * (fz.up.stride + 8u) >= lim || P * (fz.pool.stride + 8u) >= lim)
+        return hipErrorInvalidValue;
     // output-channel tiles per block: all of them up to 4, else groups of 3 or 4
     const uint32_t ct = cout / 16u;
-    // (splitting the coarse levels' output channels over more workgroups, one
-    // 16-channel group each, measured slower: every group restages the halo)
-    const uint32_t nct = ct <= 4u ? ct : (ct % 4u == 0 ? 4u : (ct % 3u == 0 ? 3u : 4u));
+    // (every output group restages the halo, so more, smaller groups pay only
+    // where the launch has fewer blocks than CUs: then they occupy idle CUs)
+    uint32_t nct = ct <= 4u ? ct : (ct % 4u == 0 ? 4u : (ct % 3u == 0 ? 3u : 4u));
+    const uint32_t tiles = d.n * ((d.h + kTile - 1u) / kTile) * ((d.w + kTile - 1u) / kTile);
+    while (nct > 1u && tiles * ((ct + nct - 1u) / nct) < 256u) {
+        uint32_t n = nct - 1u;
+        while (n > 1u && ct % n) --n;
+        nct = n;
+    }
     if (fz.fin && (!out_f32 || f32_channels != 3u || d.n != 1u || fz.fw > d.w || fz.fh > d.h))
         return hipErrorInvalidValue;
     // the frame texels come from the fp32 stage when the output conv takes that path
@@ -638,7 +684,14 @@ hipError_t conv3x3_fp8(const Tens& in, uint32_t cin, const uint8_t* w, const flo
         return hipErrorInvalidValue;
     if (d.pixels() == 0) return hipSuccess;
     const uint32_t ct = cout / 16u;
-    const uint32_t nct = ct <= 4u ? ct : (ct % 4u == 0 ? 4u : (ct % 3u == 0 ? 3u : 4u));
+    uint32_t nct = ct <= 4u ? ct : (ct % 4u == 0 ? 4u : (ct % 3u == 0 ? 3u : 4u));
+    // a launch of fewer blocks than CUs takes smaller output-channel groups
+    const uint32_t tiles = d.n * ((d.h + kTile - 1u) / kTile) * ((d.w + kTile - 1u) / kTile);
+    while (nct > 1u && tiles * ((ct + nct - 1u) / nct) < 256u) {
+        uint32_t n = nct - 1u;
+        while (n > 1u && ct % n) --n;
+        nct = n;
+    }
     switch (nct) {
         case 1: conv8_launch<1>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
         case 2: conv8_launch<2>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
@@ -807,7 +860,6 @@ static hipError_t reduce_chunks(const float* partial, uint32_t chunks, uint32_t 
 // partial[chunk] and reduce_chunks adds the chunks in order.  LDS pixel
 // pitches are odd multiples of 32 bytes: the 4 pixels of a 16-lane group then
 // fall in distinct banks.
-typedef short i16x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) char lds_char;
 typedef __attribute__((address_space(3))) i16x4 lds_i16x4;
 typedef __attribute__((address_space(3))) bf16x8 lds_bf16x8;
