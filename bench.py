@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--tlas-build", choices=["reference", "device"], default="reference",
                     help="top-level tree: the uploaded TLAS, or built on the GPU from the objects "
                          "(counted work is then of that tree)")
+    ap.add_argument("--lds-stack", type=int, default=None,
+                    help="traversal-stack entries per lane kept in LDS (PT_OPT_LDS_STACK; default the library's)")
     ap.add_argument("--compare", action="store_true",
                     help="also time the other backend/schedule variants (reported under alt)")
     args = ap.parse_args()
@@ -125,6 +127,8 @@ def main():
     # Timed steps run the production kernel; node/triangle counters (which
     # cost registers) are switched on only for one extra step afterwards.
     r.set_option(A.PT_OPT_COUNT_STATS, 0)
+    if args.lds_stack is not None:
+        r.set_option(A.PT_OPT_LDS_STACK, args.lds_stack)
     d = scene.desc()
     mk = (A.PT_OPT_BACKEND, A.PT_BACKEND_MEGAKERNEL)
     regen, sync = (A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_REGEN), (A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_SYNC)
