@@ -45,7 +45,10 @@ constexpr uint32_t kMaxCout = 128;
 //   fin:    (fp32 output of 3 channels, one image) the denoised frame texels
 //           as unpack_frame writes them, for the fw x fh frame, instead of
 //           out_f32 (which is still required, and written when the texels
-//           cannot come from the kernel's stage).
+//           cannot come from the kernel's stage);
+//   wup:    (with up.p) the coarse channels' merged weights (up_merge_weights):
+//           where the tiles are full, the coarse part runs as 2x2 products on
+//           the coarse map instead of 3x3 products on its upsampling.
 struct ConvFuse {
     Tens pool;
     Tens upw;
@@ -55,7 +58,18 @@ struct ConvFuse {
     uchar4* ldr;
     uint32_t fw, fh;
     float exposure;
+    const bf16* wup = nullptr;
 };
+
+// A 3x3 convolution over the nearest 2x upsampling of a coarse map, per
+// output pixel parity (py, px): its taps land on 2x2 coarse pixels, coarse
+// offset (a + py - 1, b + px - 1) for a, b in {0, 1}.  wup[(2 py + px) * 4 +
+// 2 a + b][co][ci] (bf16, 16 x cout x cup) = the sum, in fp32, of the taps
+// w[dy][dx][co][ci] (ci < cup of w [9][cout][cin]) whose upsampled pixel lies
+// in that coarse pixel.  (Equal to the 3x3 form up to the rounding of the
+// sums; the image borders agree too, since d.h and d.w are even.)
+size_t up_merged_elems(uint32_t cout, uint32_t cup);
+hipError_t up_merge_weights(const bf16* w, uint32_t cout, uint32_t cin, uint32_t cup, bf16* wup, hipStream_t s);
 hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bias, const Tens& out, uint32_t cout,
                    Dims d, bool relu, float* out_f32, uint32_t f32_channels, hipStream_t s,
                    const ConvFuse* fuse = nullptr);

@@ -40,6 +40,8 @@ struct Layer {
     size_t wbf_off;                // into the bf16 weight arrays (same size as the weights)
     size_t a_off, s_off;           // into the per-input-channel and per-output-channel fp8 scale arrays
     bool relu;
+    uint32_t cup = 0;              // a decoder level's first convolution: its upsampled (coarse) channels
+    size_t wup_off = 0;            // into the merged coarse weights (up_merge_weights)
 };
 
 enum {
@@ -61,6 +63,8 @@ struct UNet {
     size_t n_params = 0, n_wbf = 0;
     DevBuf params, grads, m, v;    // fp32, n_params
     DevBuf wbf, wdt;               // bf16 weights [9][cout][cin], transposed slices [9][ci][cout]
+    size_t n_wup = 0;
+    DevBuf wup;                    // the decoder-first layers' coarse weights merged per output parity (bf16)
     uint32_t adam_t = 0;
     // fp8 inference: e4m3 weights (layout of wbf), input-channel inverse scales, output-channel scales
     size_t n_act = 0, n_out = 0;   // sums of cin and cout over the layers
@@ -81,7 +85,7 @@ struct UNet {
     uint32_t ds_n = 0, ds_h = 0, ds_w = 0;
 
     std::vector<DevBuf*> all_bufs() {
-        std::vector<DevBuf*> bufs{&params, &grads, &m, &v, &wbf, &wdt, &in0, &t4A, &b4, &pred, &g_t4A, &g_b4,
+        std::vector<DevBuf*> bufs{&params, &grads, &m, &v, &wbf, &wdt, &wup, &in0, &t4A, &b4, &pred, &g_t4A, &g_b4,
                                   &g_pred, &dz, &target, &feat, &loss, &wpart, &ds_in,
                                   &ds_tg, &w8, &inv_sa, &sw, &amax, &hist};
         for (int i = 0; i < 4; ++i) {
@@ -96,7 +100,7 @@ struct UNet {
     }
     void release_acts() {
         for (DevBuf* b : all_bufs())
-            if (b != &params && b != &grads && b != &m && b != &v && b != &wbf && b != &wdt && b != &ds_in &&
+            if (b != &params && b != &grads && b != &m && b != &v && b != &wbf && b != &wdt && b != &wup && b != &ds_in &&
                 b != &ds_tg && b != &w8 && b != &inv_sa && b != &sw && b != &amax && b != &hist)
                 b->release();
         have_acts = false;
@@ -148,6 +152,15 @@ void build_layers(UNet& u) {
         l.wbf_off = woff;
         woff += size_t(9) * l.cout * l.cin;
     }
+    // decoder level l's first convolution reads kC[l + 1] upsampled channels
+    size_t uoff = 0;
+    for (int lv = 3; lv >= 0; --lv) {
+        Layer& l = u.L[D3A + 2 * (3 - lv)];
+        l.cup = kC[lv + 1];
+        l.wup_off = uoff;
+        uoff += up_merged_elems(l.cout, l.cup);
+    }
+    u.n_wup = uoff;
     u.n_params = off;
     u.n_wbf = woff;
     u.n_act = aoff;
@@ -186,6 +199,8 @@ int refresh_weights(pt_ctx* c, UNet& u) {
     for (uint32_t i = 0; i < kLayers; ++i) {
         const Layer& l = u.L[i];
         hipError_t e = to_bf16(fp(u.params) + l.w_off, bfp(u.wbf) + l.wbf_off, size_t(9) * l.cout * l.cin, s);
+        if (e == hipSuccess && l.cup)
+            e = up_merge_weights(bfp(u.wbf) + l.wbf_off, l.cout, l.cin, l.cup, bfp(u.wup) + l.wup_off, s);
         size_t so = l.wbf_off;
         for (uint32_t k = 0; e == hipSuccess && k < slices(l); ++k) {
             e = transpose_weights(bfp(u.wbf) + l.wbf_off, l.cout, l.cin, slice_begin(k), slice_count(l, k),
@@ -315,7 +330,8 @@ int forward(pt_ctx* c, UNet& u, bool q8 = false, const ConvFuse* out_fuse = null
     for (int l = 3; l >= 0; --l) {
         const uint32_t cc = kC[l + 1] + kC[l];
         const Tens low = l == 3 ? T(u.b4, kC[4]) : T(u.dd[l + 1], kC[l + 1]);
-        const ConvFuse rd{none, none, low, kC[l + 1], nullptr, nullptr, 0u, 0u, 0.0f};
+        const ConvFuse rd{none, none, low, kC[l + 1], nullptr, nullptr, 0u, 0u, 0.0f,
+                          bfp(u.wup) + u.L[D3A + 2 * (3 - l)].wup_off};
         const ConvFuse upl{none, l > 0 && !coarse ? T(u.cat[l - 1], kC[l] + kC[l - 1], 0) : none, none, 0u, nullptr,
                            nullptr, 0u, 0u, 0.0f};
         if ((rc = conv_fwd(c, u, q8, D3A + 2 * (3 - l), T(u.cat[l], cc), T(u.dA[l], kC[l]), D[l], nullptr,
@@ -528,7 +544,8 @@ int pt_unet_init(pt_ctx* c, uint64_t seed) {
     if ((rc = c->upload(u.params, p.data(), p.size() * 4, "unet params")) ||
         (rc = c->alloc(u.grads, u.n_params * 4, "unet grads")) || (rc = c->alloc(u.m, u.n_params * 4, "unet adam")) ||
         (rc = c->alloc(u.v, u.n_params * 4, "unet adam")) || (rc = c->alloc(u.wbf, u.n_wbf * 2, "unet bf16 weights")) ||
-        (rc = c->alloc(u.wdt, u.n_wbf * 2, "unet bf16 weights")))
+        (rc = c->alloc(u.wdt, u.n_wbf * 2, "unet bf16 weights")) ||
+        (rc = c->alloc(u.wup, u.n_wup * 2, "unet merged weights")))
         return rc;
     hipError_t e = fill_zero_f32(fp(u.m), u.n_params, c->stream);
     if (e == hipSuccess) e = fill_zero_f32(fp(u.v), u.n_params, c->stream);
