@@ -41,7 +41,7 @@ constexpr uint32_t kMaxCout = 128;
 //           writes it;
 //   up.p:   (ReLU, d even) input channels [0, cup) are read from `up` (dims
 //           d / 2) at the nearest coarse pixel, as up2 would have written
-//           them into `in`; channels [cup, cin) from `in`;
+//           them; channels [cup, cin) from channels [0, cin - cup) of `in`;
 //   fin:    (fp32 output of 3 channels, one image) the denoised frame texels
 //           as unpack_frame writes them, for the fw x fh frame, instead of
 //           out_f32 (which is still required, and written when the texels

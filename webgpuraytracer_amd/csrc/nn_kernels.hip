@@ -85,7 +85,8 @@ struct ConvSmem {
 // measured 4% faster per 1080p frame than prefetching where cin > 32).  UP:
 // input channels below fz.cup are read from the half-resolution fz.up at the
 // nearest coarse pixel, the others from `in` (a decoder level's first
-// convolution over [upsampled | skip] without the upsampled copy).  PF
+// convolution over [upsampled | skip] without the upsampled copy; `in`
+// then holds only the skip channels).  PF
 // (launches of few blocks over several slices, where the extra registers
 // cost no occupancy): the next slice is fetched during this slice's products.
 template <int NCT, bool RELU, bool F32OUT, bool FULL, bool UP = false, bool PF = false>
@@ -164,7 +165,7 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
             if (UP && c < fz.cup)
                 hreg[k] = hval[k] ? load8(fz.up.p + (cpix[UP ? k : 0] * fz.up.stride + fz.up.off + c)) : zero8();
             else
-                hreg[k] = (hval[k] && c < cin) ? load8(in.p + (hsrc[k] + c0)) : zero8();
+                hreg[k] = (hval[k] && c < cin) ? load8(in.p + (hsrc[k] + c0 - (UP ? fz.cup : 0u))) : zero8();
         }
         fetch_w(c0, wreg);
     };
@@ -369,7 +370,8 @@ static void conv_launch(const Tens& in, uint32_t cin, const bf16* w, const float
 
 // ------------------------------------------------ conv over [up2(C) | S] --
 // A decoder level's first convolution over [nearest 2x upsampling of the
-// coarse map C (channels [0, cup)) | skip S (channels [cup, cin) of `in`)],
+// coarse map C (channels [0, cup)) | skip S (channels [cup, cin), at
+// channels [0, cin - cup) of `in`)],
 // 16 x 16 tiles of full size.  Wave w computes the output pixels of parity
 // class (py, px) = (w / 2, w % 2): coarse positions Y, X = 0..7 of the tile,
 // as 4 MFMA tiles of 16 (tile m: Y = 2m + ln / 8, X = ln % 8).  The coarse
@@ -477,7 +479,7 @@ __global__ void __launch_bounds__(kBlock) conv_up_k(Tens in, uint32_t cin, const
         for (uint32_t k = 0; k < kHaloPer; ++k) {
             const uint32_t gy = ty0 + (hpos[k] >> 8) - 1u, gx = tx0 + (hpos[k] & 255u) - 1u;
             hr[k] = hpos[k] != ~0u && c < cin && gy < d.h && gx < d.w
-                        ? load8(in.p + (((img * d.h + gy) * d.w + gx) * in.stride + in.off + c))
+                        ? load8(in.p + (((img * d.h + gy) * d.w + gx) * in.stride + in.off + c - cup))
                         : zero8();
         }
 #pragma unroll

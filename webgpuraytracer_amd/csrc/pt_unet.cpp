@@ -308,11 +308,17 @@ int forward(pt_ctx* c, UNet& u, bool q8 = false, const ConvFuse* out_fuse = null
     const Dims* D = u.dims;
     int rc;
     const Tens none{nullptr, 0, 0};
+    // inference in bf16 (no backward, no calibration) skips the upsampled
+    // copies: the decoders' first convolutions read the coarse maps themselves
+    // (1080p frame 1.07 -> 1.02 ms against writing the copy in the epilogue of
+    // the convolution below), and the encoders write their skip outputs densely
+    // (into the idle gradient buffers) instead of into the concatenation
+    const bool coarse = infer && !q8 && !u.calib && !u.calib_hist;
     // encoder
     Tens x = T(u.in0, kInC);
     for (int l = 0; l < 4; ++l) {
         const uint32_t cc = kC[l + 1] + kC[l];
-        const Tens skip = T(u.cat[l], cc, kC[l + 1]);
+        const Tens skip = coarse ? T(u.g_tA[l], kC[l]) : T(u.cat[l], cc, kC[l + 1]);
         const ConvFuse fz{T(u.pin[l + 1], kC[l]), none, none, 0u, nullptr, nullptr, 0u, 0u, 0.0f};
         if ((rc = conv_fwd(c, u, q8, E0A + 2 * l, x, T(u.tA[l], kC[l]), D[l]))) return rc;
         if ((rc = conv_fwd(c, u, q8, E0B + 2 * l, T(u.tA[l], kC[l]), skip, D[l], nullptr, &fz))) return rc;
@@ -320,11 +326,6 @@ int forward(pt_ctx* c, UNet& u, bool q8 = false, const ConvFuse* out_fuse = null
     }
     // the bottleneck's output goes up into cat[3], each decoder level's (but the last) into the next one's
     if ((rc = conv_fwd(c, u, q8, B4A, x, T(u.t4A, kC[4]), D[4]))) return rc;
-    // inference in bf16 (no backward, no calibration) skips the upsampled
-    // copies: the decoders' first convolutions read the coarse maps themselves
-    // (1080p frame 1.07 -> 1.02 ms against writing the copy in the epilogue of
-    // the convolution below)
-    const bool coarse = infer && !q8 && !u.calib && !u.calib_hist;
     const ConvFuse up3{none, coarse ? none : T(u.cat[3], kC[4] + kC[3], 0), none, 0u, nullptr, nullptr, 0u, 0u, 0.0f};
     if ((rc = conv_fwd(c, u, q8, B4B, T(u.t4A, kC[4]), T(u.b4, kC[4]), D[4], nullptr, &up3))) return rc;
     for (int l = 3; l >= 0; --l) {
@@ -334,8 +335,8 @@ int forward(pt_ctx* c, UNet& u, bool q8 = false, const ConvFuse* out_fuse = null
                           bfp(u.wup) + u.L[D3A + 2 * (3 - l)].wup_off};
         const ConvFuse upl{none, l > 0 && !coarse ? T(u.cat[l - 1], kC[l] + kC[l - 1], 0) : none, none, 0u, nullptr,
                            nullptr, 0u, 0u, 0.0f};
-        if ((rc = conv_fwd(c, u, q8, D3A + 2 * (3 - l), T(u.cat[l], cc), T(u.dA[l], kC[l]), D[l], nullptr,
-                           coarse ? &rd : nullptr)))
+        if ((rc = conv_fwd(c, u, q8, D3A + 2 * (3 - l), coarse ? T(u.g_tA[l], kC[l]) : T(u.cat[l], cc),
+                           T(u.dA[l], kC[l]), D[l], nullptr, coarse ? &rd : nullptr)))
             return rc;
         if ((rc = conv_fwd(c, u, q8, D3B + 2 * (3 - l), T(u.dA[l], kC[l]), T(u.dd[l], kC[l]), D[l], nullptr,
                            l > 0 ? &upl : nullptr)))
