@@ -72,10 +72,10 @@ struct ConvSmem {
     bf16x8 w[4 * 9 * NCT * 16];   // [8-channel group][tap][co]
 };
 
-// FULL: every block has all NCT output tiles (cout a multiple of NCT x 16)
-// and the frame is a multiple of the 16 x 16 tile, so no guards are needed;
-// the bf16 output then goes through LDS ([pixel][channel], rows padded by 16
-// bytes against bank conflicts) and leaves as 16-byte stores.
+// FULL: every block has all NCT output tiles (cout a multiple of NCT x 16),
+// so the products need no guards; the bf16 output then goes through LDS
+// ([pixel][channel], rows padded by 16 bytes against bank conflicts) and
+// leaves as 16-byte stores, those of pixels past the image's edge dropped.
 // With a FULL bf16 output the epilogue can also write (fz) the 2x2 average
 // pool of the output tile and its nearest 2x upsampling; with a FULL fp32
 // output, the denoised frame texels.  A slice is fetched, staged and
@@ -266,6 +266,7 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
 #pragma unroll
         for (uint32_t i = tid; i < kChunks; i += kBlock) {
             const uint32_t p = i / (kCo / 8u), c8 = i % (kCo / 8u);
+            if (ty0 + p / kTile >= d.h || tx0 + p % kTile >= d.w) continue;
             const uint32_t op = obase + (p / kTile) * d.w + p % kTile;
             store8(out.p + (op * out.stride + out.off + co0 + c8 * 8u),
                    *reinterpret_cast<const bf16x8*>(stage + p * kPitch + c8 * 16u));
@@ -274,6 +275,7 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
             constexpr uint32_t kQ = kTile / 2u;
             for (uint32_t i = tid; i < kQ * kQ * (kCo / 8u); i += kBlock) {
                 const uint32_t q = i / (kCo / 8u), c8 = i % (kCo / 8u), qy = q / kQ, qx = q % kQ;
+                if (ty0 + 2u * qy >= d.h || tx0 + 2u * qx >= d.w) continue;  // (d.h and d.w are even)
                 const uint32_t p00 = 2u * qy * kTile + 2u * qx;
                 const char* sb = stage + c8 * 16u;
                 const bf16x8 v0 = *reinterpret_cast<const bf16x8*>(sb + p00 * kPitch);
@@ -292,6 +294,7 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
             for (uint32_t i = tid; i < kTile * kTile * 4u * (kCo / 8u); i += kBlock) {
                 const uint32_t c8 = i % (kCo / 8u), f = i / (kCo / 8u);  // f: fine pixel of the 32 x 32 block
                 const uint32_t fy = f / (2u * kTile), fx = f % (2u * kTile);
+                if (ty0 + fy / 2u >= d.h || tx0 + fx / 2u >= d.w) continue;
                 const uint32_t p = (fy / 2u) * kTile + fx / 2u;
                 const uint32_t fp = (img * 2u * d.h + 2u * ty0 + fy) * W2 + 2u * tx0 + fx;
                 store8(fz.upw.p + (fp * fz.upw.stride + fz.upw.off + co0 + c8 * 8u),
@@ -333,8 +336,7 @@ static void conv_launch(const Tens& in, uint32_t cin, const bf16* w, const float
     const uint32_t tiles = d.n * ((d.h + kTile - 1u) / kTile) * ((d.w + kTile - 1u) / kTile);
     const uint32_t groups = (cout / 16u + NCT - 1u) / NCT;
     const dim3 grid(tiles, groups);
-    const bool full = cout % (uint32_t(NCT) * 16u) == 0 && d.h % kTile == 0 && d.w % kTile == 0 &&
-                      out.stride % 8u == 0 && out.off % 8u == 0;
+    const bool full = cout % (uint32_t(NCT) * 16u) == 0 && out.stride % 8u == 0 && out.off % 8u == 0;
     // fp32 output through the LDS stage: one output group, whole 16-byte rows
     const bool f32_full = NCT == 1 && cout == 16u && f32c <= 4u && d.h % kTile == 0 && d.w % kTile == 0 &&
                           (kTile * f32c) % 4u == 0 && reinterpret_cast<uintptr_t>(of32) % 16u == 0;
@@ -610,9 +612,8 @@ hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bia
     const ConvFuse frame = fz;
     if (!texel_fused) fz.fin = nullptr;
     // pooling and upsampling fuse into the staged bf16 epilogue (full output
-    // groups and tiles); otherwise pool2 / up2 follow the convolution
-    const bool staged = !out_f32 && cout % (nct * 16u) == 0 && d.h % kTile == 0 && d.w % kTile == 0 &&
-                        out.stride % 8u == 0 && out.off % 8u == 0;
+    // groups); otherwise pool2 / up2 follow the convolution
+    const bool staged = !out_f32 && cout % (nct * 16u) == 0 && out.stride % 8u == 0 && out.off % 8u == 0;
     if ((fz.pool.p && (d.h % 2u || d.w % 2u || fz.pool.stride % 8u || fz.pool.off % 8u)) ||
         (fz.upw.p && (out_f32 || fz.upw.stride % 8u || fz.upw.off % 8u)) ||
         (fz.up.p && (d.h % 2u || d.w % 2u || !relu || out_f32 || fz.cup % 8u || fz.cup > cin ||
@@ -625,7 +626,7 @@ hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bia
     // the block's merged weights (16 taps) cut it to 2 blocks per CU by LDS,
     // and at 64 two output groups restage both halos (each measured slower
     // than the 3x3 form over the coarse reads)
-    if (fz.up.p && fz.wup && staged && !pool.p && !upw.p && cout == 32u) {
+    if (fz.up.p && fz.wup && staged && !pool.p && !upw.p && cout == 32u && d.h % kTile == 0 && d.w % kTile == 0) {
         conv_up_launch<2>(in, cin, w, fz.wup, bias, out, cout, d, fz.up, fz.cup, s);
         return hipGetLastError();
     }
