@@ -4,9 +4,11 @@ Tolerances (written here, per the measurement plan):
   * hit identities (object, material, triangle, plane) exact; t within 1e-5
     relative; a disagreement passes only when the FP64 brute-force arbiter
     shows the two candidates tied;
-  * 1-spp images: per-pixel agreement within 1e-4 relative on >= 97% of pixels
-    (the rest are paths that a last-ulp difference in sin/cos/pow/exp/log
-    between the device and host math libraries sent elsewhere);
+  * 1-spp images: per-pixel agreement within 1e-4 relative on >= 99.9% of
+    pixels (>= 99.5% within 1e-3 and >= 98% within 1e-4 with material
+    textures); the rest are paths that a last-ulp difference in
+    sin/cos/pow/exp/log between the device and host math libraries sent
+    elsewhere, and every ray of those paths is checked hit for hit;
   * converged images: relMSE <= 1e-3 per channel;
   * env CDF built on the device: within 1 f32 ulp of the oracle's;
   * G-buffer: exact identities, values within 1e-5; denoiser: 1e-5 relative.
