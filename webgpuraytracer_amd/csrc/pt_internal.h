@@ -93,7 +93,7 @@ struct WfArgs {
     float4* sh_tp;      //              translucent occluders only (else NULL): throughput
     float4* sh_em;      // per path, height-mapped scenes only (else NULL): emission of a surface the height
                         // field blocked, to be added after the shadow slots; w = 1 while pending
-    uint32_t* counters; // queue A length, queue B length, shadow queue length, two fetch cursors
+    uint32_t* counters; // per extension queue its length and its shadow queue's, two fetch cursors
     int wide;           // the traversal: PT_TRAVERSAL_* (0 child pairs, 1 wide, 2 wide quantised)
     uint32_t trace_grid;    // persistent traversal launches: workgroups (all resident)
     uint32_t refill;        // dynamic fetch: idle lanes of a wave that trigger a refill

@@ -14,7 +14,11 @@
 // The two traversal launches are persistent with dynamic fetch: a wave keeps
 // its lanes busy by taking new queue entries for its idle lanes (one atomic
 // per refill) instead of idling until its longest ray ends.
-// Queues are compacted with a wave64 ballot + one atomic per wave.  Every
+// The shading kernel appends each block's surviving paths and shadow
+// queries with one atomic per block (one atomic per wave on one counter
+// bounded that kernel at half its speed), grouped by the direction octant of
+// the ray, so that a traversal wave gets rays from one image region that
+// visit the tree's children in the same order.  Every
 // kernel reads its queue length from device memory, so a whole batch is
 // launched without a host round trip.  The per-path RNG state and draw order
 // are the megakernel's, so both backends produce the same bits.
@@ -36,8 +40,12 @@
 namespace pt {
 
 constexpr int kWfBlock = 256;
-// counters: queue lengths, and the fetch cursors of the two traversal kernels
-enum { kCntA = 0, kCntB = 1, kCntShadow = 2, kCntFetchExt = 3, kCntFetchShadow = 4 };
+// counters: per extension queue q (0, 1) the pair {its length, the length of
+// the shadow queue filled with it} at 2q, 2q + 1 (one 64-bit word), then the
+// fetch cursors of the two traversal kernels
+enum { kCntFetchExt = 4, kCntFetchShadow = 5, kCntWords = 6 };
+__device__ __forceinline__ int cnt_ext(int q) { return 2 * q; }
+__device__ __forceinline__ int cnt_shadow(int q) { return 2 * q + 1; }
 // a shadow-queue entry: the path index and what its shading left pending
 // (light slot, env slot, deferred emission); the pool is below 2^29 paths
 constexpr uint32_t kSqSlot0 = 1u << 31, kSqSlot1 = 1u << 30, kSqEmit = 1u << 29, kSqPath = kSqEmit - 1u;
@@ -67,17 +75,57 @@ __device__ __forceinline__ v3 zero_product(uint32_t b) {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Appends the calling lanes with pred set to a queue; returns each such
-// lane's slot.  Must be reached by every active lane of the wave.
-__device__ __forceinline__ uint32_t wave_push(uint32_t* counter, bool pred) {
-    const unsigned long long mask = __ballot(pred);
-    if (mask == 0ull) return 0u;
-    const uint32_t lane = lane_id();
-    const uint32_t leader = uint32_t(__ffsll(static_cast<long long>(mask)) - 1);
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
-    base = __shfl(base, int(leader), 64);
-    return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+// The shading kernel's block: its paths enter the next queues as one group
+// per block (one atomic), ordered by direction octant.
+constexpr int kShadeBlock = 256;
+constexpr uint32_t kShadeWaves = uint32_t(kShadeBlock) / 64u;
+
+// Appends the block's paths to the extension queue (pred pa, key ka) and to
+// the shadow queue (pred pb, key kb) with one 64-bit atomic on the pair
+// {extension length, shadow length}.  Each queue receives the block's
+// entries contiguously, grouped by key (0 .. 7, key 0's first), each key's
+// in thread order.  Every thread of the block must call it; sm holds 64
+// words.
+__device__ __forceinline__ void block_push2(unsigned long long* pair, bool pa, uint32_t ka, bool pb, uint32_t kb,
+                                            uint32_t* sm, uint32_t& qa, uint32_t& qb) {
+    static_assert(8u * kShadeWaves == 32u, "one scan lane per (queue, key, wave)");
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t ra = 0, rb = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; ++k) {
+        const bool ia = pa && ka == k, ib = pb && kb == k;
+        const unsigned long long ma = __ballot(ia), mb = __ballot(ib);
+        if (lane == 0u) {  // [queue][key][wave]: the block's order
+            sm[k * kShadeWaves + wv] = uint32_t(__popcll(ma));
+            sm[32u + k * kShadeWaves + wv] = uint32_t(__popcll(mb));
+        }
+        if (ia) ra = uint32_t(__popcll(ma & below));
+        if (ib) rb = uint32_t(__popcll(mb & below));
+    }
+    __syncthreads();
+    if (wv == 0u) {  // one scan over both queues' counts, then the block's atomic
+        const uint32_t v = sm[lane];
+        uint32_t s = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t t = __shfl_up(s, off, 64);
+            if (lane >= uint32_t(off)) s += t;
+        }
+        const uint32_t ta = __shfl(s, 31, 64), tb = __shfl(s, 63, 64) - ta;
+        unsigned long long base = 0;
+        if (lane == 0u && (ta | tb)) base = atomicAdd(pair, (static_cast<unsigned long long>(tb) << 32) | ta);
+        base = __shfl(base, 0, 64);
+        sm[lane] = lane < 32u ? uint32_t(base) + s - v : uint32_t(base >> 32) + s - v - ta;
+    }
+    __syncthreads();
+    qa = pa ? sm[ka * kShadeWaves + wv] + ra : 0u;
+    qb = pb ? sm[32u + kb * kShadeWaves + wv] + rb : 0u;
+}
+
+// Direction octant: the sign bits of d.
+__device__ __forceinline__ uint32_t octant(v3 d) {
+    return (__float_as_uint(d.x) >> 31) | ((__float_as_uint(d.y) >> 31) << 1) | ((__float_as_uint(d.z) >> 31) << 2);
 }
 
 __device__ __forceinline__ unsigned long long wsum64(uint32_t v) {
@@ -151,7 +199,7 @@ __device__ __forceinline__ void wf_path_begin(const WfArgs& A, uint32_t i, PathS
 // no queue: its paths are the batch slots 0 .. n_items - 1).
 __global__ void __launch_bounds__(kWfBlock) wf_raygen(WfArgs A) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 5u) A.counters[i] = 0u;
+    if (i < uint32_t(kCntWords)) A.counters[i] = 0u;
     if (i >= A.n_items) return;
     PathState ps;
     wf_path_begin(A, i, ps);
@@ -181,15 +229,15 @@ wf_trace(DevScene S, WfArgs A, int cur, int bounce) {
     // the counter is read even when unused so that n stays a scalar load (a
     // select between the argument and the counter's address compiles to a
     // flat load whose wait spreads through the kernel)
-    const uint32_t qn = A.counters[SHADOW ? int(kCntShadow) : cur];
+    const uint32_t qn = A.counters[SHADOW ? cnt_shadow(cur ^ 1) : cnt_ext(cur)];
     const uint32_t n = first ? A.n_items : qn;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         // counters this launch does not read, reset for the launches after it
         if (SHADOW) {
             A.counters[kCntFetchExt] = 0u;
         } else {
-            A.counters[cur ^ 1] = 0u;
-            A.counters[kCntShadow] = 0u;
+            A.counters[cnt_ext(cur ^ 1)] = 0u;
+            A.counters[cnt_shadow(cur ^ 1)] = 0u;
             A.counters[kCntFetchShadow] = 0u;
         }
     }
@@ -312,13 +360,15 @@ struct QueuedNee {
     const WfArgs& A;
     uint32_t p;
     uint32_t pend;
-    bool att;  // translucent occluders possible
+    bool att;          // translucent occluders possible
+    uint32_t key = 0;  // direction octant of the last queued ray
     __device__ void operator()(int k, bool need_ray, v3 o, v3 d, float dist, int ignore, v3 inc, v3 tp, int bounce,
                                v3& rad) {
         if (!need_ray) {
             rad = rad + tp * firefly(inc, bounce);
             return;
         }
+        key = octant(d);
         const uint32_t s = 2u * p + uint32_t(k);
         A.sh_o[s] = make_float4(o.x, o.y, o.z, dist);
         A.sh_d[s] = make_float4(d.x, d.y, d.z, __int_as_float(ignore));
@@ -344,15 +394,16 @@ struct QueuedNee {
 };
 
 template <uint32_t F, bool CN>
-__global__ void __launch_bounds__(kWfBlock) wf_shade(DevScene S, WfArgs A, int cur, int bounce) {
+__global__ void __launch_bounds__(kShadeBlock) wf_shade(DevScene S, WfArgs A, int cur, int bounce) {
     const Flags<F> fl{S.flags};
-    const uint32_t qn = A.counters[cur];  // read unconditionally: a scalar load (see wf_trace)
+    const uint32_t qn = A.counters[cnt_ext(cur)];  // read unconditionally: a scalar load (see wf_trace)
     const uint32_t n = bounce == 0 ? A.n_items : qn;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x * blockDim.x >= n) return;
     const Params P{A.P.sky_total_lum, A.P.sky_width, A.P.sky_height, A.P.sky_cdf_index};
+    __shared__ uint32_t grp[64];
     bool alive = false;
-    uint32_t p = 0, pend = 0;
+    uint32_t p = 0, pend = 0, ext_key = 0, sh_key = 0;
     Counters c;
     c.clear();
     if (i < n) {
@@ -390,6 +441,8 @@ __global__ void __launch_bounds__(kWfBlock) wf_shade(DevScene S, WfArgs A, int c
         alive = shade_bounce<F, CN>(S, fl, P, bounce, h, sh, ps, md, c, sink);
         alive = alive && bounce + 1 < S.bounce_limit;
         pend = sink.pend;
+        sh_key = sink.key;
+        ext_key = octant(ps.rd);
         if (alive) {  // an ended path keeps only its radiance
             A.o_pdf[p] = make_float4(ps.ro.x, ps.ro.y, ps.ro.z, ps.last_pdf);
             A.d_wsum[p] = make_float4(ps.rd.x, ps.rd.y, ps.rd.z, ps.last_wsum);
@@ -398,9 +451,10 @@ __global__ void __launch_bounds__(kWfBlock) wf_shade(DevScene S, WfArgs A, int c
         }
         A.rad[p] = make_float4(ps.rad.x, ps.rad.y, ps.rad.z, 0.0f);
     }
-    const uint32_t qa = wave_push(&A.counters[cur ^ 1], alive);
+    uint32_t qa, qs;
+    block_push2(reinterpret_cast<unsigned long long*>(A.counters + cnt_ext(cur ^ 1)), alive, ext_key, pend != 0u, sh_key,
+                grp, qa, qs);
     if (alive) A.queue[cur ^ 1][qa] = p;
-    const uint32_t qs = wave_push(&A.counters[kCntShadow], pend != 0u);
     if (pend) A.shadow_queue[qs] = p | pend;
     if (A.stats && CN) add_stat(A.stats, kStShade, c.shade);
 }
@@ -439,7 +493,7 @@ static void wf_bounces(const DevScene& S, const WfArgs& a0, uint32_t blocks, hip
     for (int b = 0; b < S.bounce_limit; ++b) {
         if (bounce_ev) (void)hipEventRecord(bounce_ev[b], s);
         hipLaunchKernelGGL((wf_trace<F, CN, W, false>), dim3(a0.trace_grid), dim3(kWfBlock), lds, s, S, a0, cur, b);
-        hipLaunchKernelGGL((wf_shade<F, CN>), dim3(blocks), dim3(kWfBlock), 0, s, S, a0, cur, b);
+        hipLaunchKernelGGL((wf_shade<F, CN>), dim3((a0.n_items + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, S, a0, cur, b);
         hipLaunchKernelGGL((wf_trace<F, CN, W, true>), dim3(a0.trace_grid), dim3(kWfBlock), lds, s, S, a0, cur, b);
         cur ^= 1;
     }
