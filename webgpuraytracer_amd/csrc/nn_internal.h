@@ -63,11 +63,14 @@ struct ConvFuse {
 
 // A 3x3 convolution over the nearest 2x upsampling of a coarse map, per
 // output pixel parity (py, px): its taps land on 2x2 coarse pixels, coarse
-// offset (a + py - 1, b + px - 1) for a, b in {0, 1}.  wup[(2 py + px) * 4 +
-// 2 a + b][co][ci] (bf16, 16 x cout x cup) = the sum, in fp32, of the taps
-// w[dy][dx][co][ci] (ci < cup of w [9][cout][cin]) whose upsampled pixel lies
-// in that coarse pixel.  (Equal to the 3x3 form up to the rounding of the
-// sums; the image borders agree too, since d.h and d.w are even.)
+// offset (a + py - 1, b + px - 1) for a, b in {0, 1}.  The merged weight of
+// class 2 py + px, tap 2 a + b, [co][ci] (bf16, 16 x cout x cup in all) = the
+// sum, in fp32, of the taps w[dy][dx][co][ci] (ci < cup of w [9][cout][cin])
+// whose upsampled pixel lies in that coarse pixel.  (Equal to the 3x3 form up
+// to the rounding of the sums; the image borders agree too, since d.h and d.w
+// are even.)  Stored in the order the convolution's MFMA fragments read them:
+// [class][K-step][16-channel output tile][lane][8] (cup a multiple of 8, cout
+// of 16).
 size_t up_merged_elems(uint32_t cout, uint32_t cup);
 hipError_t up_merge_weights(const bf16* w, uint32_t cout, uint32_t cin, uint32_t cup, bf16* wup, hipStream_t s);
 hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bias, const Tens& out, uint32_t cout,

@@ -372,160 +372,126 @@ static void conv_launch(const Tens& in, uint32_t cin, const bf16* w, const float
 
 // ------------------------------------------------ conv over [up2(C) | S] --
 // A decoder level's first convolution over [nearest 2x upsampling of the
-// coarse map C (channels [0, cup)) | skip S (channels [cup, cin), at
-// channels [0, cin - cup) of `in`)],
-// 16 x 16 tiles of full size.  Wave w computes the output pixels of parity
-// class (py, px) = (w / 2, w % 2): coarse positions Y, X = 0..7 of the tile,
-// as 4 MFMA tiles of 16 (tile m: Y = 2m + ln / 8, X = ln % 8).  The coarse
-// part takes the 4 merged taps of its class over a 10 x 10 coarse halo (per
-// 32-channel step), the skip part the 9 taps over the 18 x 18 halo (per
-// 32-channel slice), both into the same accumulators (started at the bias).
+// coarse map C (channels [0, cup)) | skip S (32 channels, [cup, cin), at
+// channels [0, 32) of `in`)] for 32 output channels, 16 x 16 tiles of full
+// size.  Wave w computes the output pixels of parity class (py, px) = (w / 2,
+// w % 2): coarse positions Y, X = 0..7 of the tile, as 4 MFMA tiles of 16
+// (tile m: Y = 2m + ln / 8, X = ln % 8).  The coarse part takes the 4 merged
+// taps of its class over a 10 x 10 coarse halo, the skip part the 9 taps over
+// the 18 x 18 halo at the class's pixels, both into the same accumulators
+// (started at the bias).  One staging round: both halos and the skip weights
+// go to LDS together; the merged weights come from global memory already in
+// MFMA fragment order (see up_merge_k), each wave reading its class's.  The
+// coarse part's K runs over (merged tap, 8-channel group) pieces: K-step s
+// gives lane group kq piece q = 4s + kq, tap q / GC, group q % GC.
 constexpr uint32_t kUpHalo = 10, kUpHaloPix = kUpHalo * kUpHalo;
+constexpr uint32_t kUpCo = 32;  // output channels (two MFMA column tiles)
 
-template <int NCT>
+template <uint32_t GC>
 union UpSmem {
     struct {
-        bf16x8 halo[4 * kUpHaloPix];  // [8-channel group][coarse pixel]
-        bf16x8 w[4 * 16 * NCT * 16];  // [8-channel group][class * 4 + merged tap][co]
-    } c;
-    struct {
-        bf16x8 halo[4 * kHaloPix];    // [8-channel group][pixel]
-        bf16x8 w[4 * 9 * NCT * 16];   // [8-channel group][tap][co]
-    } f;
+        bf16x8 hc[GC * kUpHaloPix];  // [8-channel group][coarse pixel]
+        bf16x8 hs[4 * kHaloPix];     // [8-channel group][pixel]
+    } m;
+    char stage[kTile * kTile * (kUpCo * 2u + 16u)];
 };
 
-template <int NCT>
+template <uint32_t GC>
 __global__ void __launch_bounds__(kBlock) conv_up_k(Tens in, uint32_t cin, const bf16* __restrict__ w,
-                                                    const bf16* __restrict__ wup, const float* __restrict__ bias,
-                                                    Tens out, uint32_t cout, Dims d, Tens up, uint32_t cup) {
-    __shared__ UpSmem<NCT> sm;
-    constexpr uint32_t kCo = uint32_t(NCT) * 16u;
-    constexpr uint32_t kCPer = (4u * kUpHaloPix + kBlock - 1u) / kBlock;      // 2
-    constexpr uint32_t kCWPer = 16u * kCo * 4u / kBlock;                       // 4 NCT
-    constexpr uint32_t kHaloPer = (kHaloPieces + kBlock - 1u) / kBlock;        // 6
-    constexpr uint32_t kWPer = (9u * kCo * 4u + kBlock - 1u) / kBlock;
+                                                    const bf16x8* __restrict__ wup, const float* __restrict__ bias,
+                                                    Tens out, Dims d, Tens up) {
+    __shared__ UpSmem<GC> sm;
+    constexpr uint32_t NCT = 2, cup = GC * 8u;
+    constexpr uint32_t kCP = GC * kUpHaloPix, kCPer = (kCP + kBlock - 1u) / kBlock;
+    constexpr uint32_t kSP = 4u * kHaloPix, kSPer = (kSP + kBlock - 1u) / kBlock;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6, kq = lane >> 4, ln = lane & 15u;
     const uint32_t py = wv >> 1, px = wv & 1u;
     const uint32_t tiles_x = d.w / kTile, tiles_y = d.h / kTile;
     const uint32_t img = blockIdx.x / (tiles_x * tiles_y), trem = blockIdx.x % (tiles_x * tiles_y);
     const uint32_t ty0 = trem / tiles_x * kTile, tx0 = trem % tiles_x * kTile;
-    const uint32_t co0 = blockIdx.y * kCo;
     const uint32_t H2 = d.h / 2u, W2 = d.w / 2u, Y0 = ty0 / 2u, X0 = tx0 / 2u;
-    const uint32_t g8 = (tid & 3u) * 8u;  // this thread's 8-channel group in every staged piece
+    // every global read of the block, issued together
+    bf16x8 hcr[kCPer], hsr[kSPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kCPer; ++k) {
+        const uint32_t piece = tid + k * kBlock, cp = piece / GC, g = piece % GC;
+        const uint32_t gy = Y0 + cp / kUpHalo - 1u, gx = X0 + cp % kUpHalo - 1u;  // wraps below 0
+        hcr[k] = piece < kCP && gy < H2 && gx < W2
+                     ? load8(up.p + (((img * H2 + gy) * W2 + gx) * up.stride + up.off + g * 8u))
+                     : zero8();
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kSPer; ++k) {
+        const uint32_t piece = tid + k * kBlock, sp = piece / 4u, g = piece % 4u;
+        const uint32_t gy = ty0 + sp / kHalo - 1u, gx = tx0 + sp % kHalo - 1u;
+        hsr[k] = piece < kSP && gy < d.h && gx < d.w
+                     ? load8(in.p + (((img * d.h + gy) * d.w + gx) * in.stride + in.off + g * 8u))
+                     : zero8();
+    }
     f32x4 acc[4][NCT];
 #pragma unroll
-    for (int t = 0; t < NCT; ++t) {
+    for (int t = 0; t < int(NCT); ++t) {
         float bv[4] = {0.f, 0.f, 0.f, 0.f};
         if (bias)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) bv[r] = bias[co0 + uint32_t(t) * 16u + kq * 4u + uint32_t(r)];
+            for (int r = 0; r < 4; ++r) bv[r] = bias[uint32_t(t) * 16u + kq * 4u + uint32_t(r)];
 #pragma unroll
         for (int m = 0; m < 4; ++m) acc[m][t] = f32x4{bv[0], bv[1], bv[2], bv[3]};
     }
-    // coarse part: the wave's 4 merged taps over the coarse halo
-    for (uint32_t c0 = 0; c0 < cup; c0 += kSlice) {
-        const uint32_t c = c0 + g8;
-        bf16x8 hr[kCPer], wr[kCWPer];
 #pragma unroll
-        for (uint32_t k = 0; k < kCPer; ++k) {
-            const uint32_t cp = (tid >> 2) + k * (kBlock / 4u);  // coarse halo pixel
-            const uint32_t gy = Y0 + cp / kUpHalo - 1u, gx = X0 + cp % kUpHalo - 1u;  // wraps below 0
-            hr[k] = cp < kUpHaloPix && c < cup && gy < H2 && gx < W2
-                        ? load8(up.p + (((img * H2 + gy) * W2 + gx) * up.stride + up.off + c))
-                        : zero8();
+    for (uint32_t k = 0; k < kCPer; ++k) {
+        const uint32_t piece = tid + k * kBlock;
+        if (piece < kCP) sm.m.hc[(piece % GC) * kUpHaloPix + piece / GC] = hcr[k];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kSPer; ++k) {
+        const uint32_t piece = tid + k * kBlock;
+        if (piece < kSP) sm.m.hs[(piece % 4u) * kHaloPix + piece / 4u] = hsr[k];
+    }
+    __syncthreads();
+    // coarse part: the class's merged taps, K over (tap, 8-channel group) pieces
+    const bf16x8* wf = wup + size_t(wv) * GC * NCT * 64u + lane;
+#pragma unroll
+    for (uint32_t s = 0; s < GC; ++s) {
+        const uint32_t q = 4u * s + kq, ab = q / GC, g = q % GC, a = ab >> 1, b2 = ab & 1u;
+        bf16x8 cf[4], bw[NCT];
+#pragma unroll
+        for (int t = 0; t < int(NCT); ++t) bw[t] = wf[(s * NCT + uint32_t(t)) * 64u];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const uint32_t Y = 2u * uint32_t(m) + (ln >> 3), X = ln & 7u;
+            cf[m] = sm.m.hc[g * kUpHaloPix + (Y + a + py) * kUpHalo + X + b2 + px];
         }
 #pragma unroll
-        for (uint32_t k = 0; k < kCWPer; ++k) {
-            const uint32_t rest = (tid >> 2) + k * (kBlock / 4u);  // (class * 4 + tap) * kCo + co
-            const uint32_t ct = rest / kCo, co = rest % kCo;
-            wr[k] = c < cup ? load8(wup + ((ct * cout + co0 + co) * cup + c)) : zero8();
-        }
-        __syncthreads();  // the previous step's reads are done
+        for (int m = 0; m < 4; ++m)
 #pragma unroll
-        for (uint32_t k = 0; k < kCPer; ++k) {
-            const uint32_t cp = (tid >> 2) + k * (kBlock / 4u);
-            if (cp < kUpHaloPix) sm.c.halo[(tid & 3u) * kUpHaloPix + cp] = hr[k];
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < kCWPer; ++k) sm.c.w[(tid & 3u) * (16u * kCo) + (tid >> 2) + k * (kBlock / 4u)] = wr[k];
-        __syncthreads();
-#pragma unroll
-        for (uint32_t ab = 0; ab < 4u; ++ab) {
-            const uint32_t a = ab >> 1, b2 = ab & 1u;
-            bf16x8 cf[4], bw[NCT];
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const uint32_t Y = 2u * uint32_t(m) + (ln >> 3), X = ln & 7u;
-                cf[m] = sm.c.halo[kq * kUpHaloPix + (Y + a + py) * kUpHalo + X + b2 + px];
-            }
-#pragma unroll
-            for (int t = 0; t < NCT; ++t) bw[t] = sm.c.w[kq * (16u * kCo) + (wv * 4u + ab) * kCo + uint32_t(t) * 16u + ln];
-#pragma unroll
-            for (int m = 0; m < 4; ++m)
-#pragma unroll
-                for (int t = 0; t < NCT; ++t)
-                    acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[t], cf[m], acc[m][t], 0, 0, 0);
-        }
+            for (int t = 0; t < int(NCT); ++t)
+                acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[t], cf[m], acc[m][t], 0, 0, 0);
     }
     // skip part: 9 taps over the full-resolution halo, at the class's pixels
-    uint32_t hpos[kHaloPer];
 #pragma unroll
-    for (uint32_t k = 0; k < kHaloPer; ++k) {
-        const uint32_t j = (tid >> 2) + k * (kBlock / 4u);
-        hpos[k] = j < kHaloPix ? ((j / kHalo) << 8) | (j % kHalo) : ~0u;
-    }
-    for (uint32_t c0 = cup; c0 < cin; c0 += kSlice) {
-        const uint32_t c = c0 + g8;
-        bf16x8 hr[kHaloPer], wr[kWPer];
+    for (uint32_t tap = 0; tap < 9u; ++tap) {
+        const uint32_t dy = tap / 3u, dx = tap % 3u;
+        bf16x8 a[4], bw[NCT];
 #pragma unroll
-        for (uint32_t k = 0; k < kHaloPer; ++k) {
-            const uint32_t gy = ty0 + (hpos[k] >> 8) - 1u, gx = tx0 + (hpos[k] & 255u) - 1u;
-            hr[k] = hpos[k] != ~0u && c < cin && gy < d.h && gx < d.w
-                        ? load8(in.p + (((img * d.h + gy) * d.w + gx) * in.stride + in.off + c - cup))
-                        : zero8();
+        for (int m = 0; m < 4; ++m) {
+            const uint32_t Y = 2u * uint32_t(m) + (ln >> 3), X = ln & 7u;
+            a[m] = sm.m.hs[kq * kHaloPix + (2u * Y + py + dy) * kHalo + 2u * X + px + dx];
         }
 #pragma unroll
-        for (uint32_t k = 0; k < kWPer; ++k) {
-            const uint32_t piece = tid + k * kBlock;
-            const uint32_t c8 = piece % 4u, co = (piece / 4u) % kCo, tap = piece / (4u * kCo);
-            const uint32_t cw = c0 + c8 * 8u;
-            wr[k] = piece < 9u * kCo * 4u && cw < cin ? load8(w + ((tap * cout + co0 + co) * cin + cw)) : zero8();
-        }
-        __syncthreads();
+        for (int t = 0; t < int(NCT); ++t) bw[t] = load8(w + ((tap * kUpCo + uint32_t(t) * 16u + ln) * cin + cup + kq * 8u));
 #pragma unroll
-        for (uint32_t k = 0; k < kHaloPer; ++k)
-            if (hpos[k] != ~0u) sm.f.halo[(tid & 3u) * kHaloPix + (tid >> 2) + k * (kBlock / 4u)] = hr[k];
+        for (int m = 0; m < 4; ++m)
 #pragma unroll
-        for (uint32_t k = 0; k < kWPer; ++k) {
-            const uint32_t piece = tid + k * kBlock;
-            if (piece < 9u * kCo * 4u) sm.f.w[(piece % 4u) * (9u * kCo) + piece / 4u] = wr[k];
-        }
-        __syncthreads();
-#pragma unroll
-        for (uint32_t tap = 0; tap < 9u; ++tap) {
-            const uint32_t dy = tap / 3u, dx = tap % 3u;
-            bf16x8 a[4], bw[NCT];
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const uint32_t Y = 2u * uint32_t(m) + (ln >> 3), X = ln & 7u;
-                a[m] = sm.f.halo[kq * kHaloPix + (2u * Y + py + dy) * kHalo + 2u * X + px + dx];
-            }
-#pragma unroll
-            for (int t = 0; t < NCT; ++t) bw[t] = sm.f.w[kq * (9u * kCo) + tap * kCo + uint32_t(t) * 16u + ln];
-#pragma unroll
-            for (int m = 0; m < 4; ++m)
-#pragma unroll
-                for (int t = 0; t < NCT; ++t)
-                    acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[t], a[m], acc[m][t], 0, 0, 0);
-        }
+            for (int t = 0; t < int(NCT); ++t)
+                acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[t], a[m], acc[m][t], 0, 0, 0);
     }
     // epilogue: ReLU, bf16, staged [pixel][channel] for 16-byte stores
-    constexpr uint32_t kPitch = kCo * 2u + 16u;
-    static_assert(kTile * kTile * kPitch <= sizeof(UpSmem<NCT>), "output stage fits the LDS tile");
-    char* stage = reinterpret_cast<char*>(&sm);
+    constexpr uint32_t kPitch = kUpCo * 2u + 16u;
+    char* stage = sm.stage;
     __syncthreads();
 #pragma unroll
-    for (int t = 0; t < NCT; ++t)
+    for (int t = 0; t < int(NCT); ++t)
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
             bf16x4 q;
@@ -537,23 +503,35 @@ __global__ void __launch_bounds__(kBlock) conv_up_k(Tens in, uint32_t cin, const
             *reinterpret_cast<i16x4*>(stage + p * kPitch + (uint32_t(t) * 16u + kq * 4u) * 2u) = qi;
         }
     __syncthreads();
-    constexpr uint32_t kChunks = kTile * kTile * kCo / 8u;
+    constexpr uint32_t kChunks = kTile * kTile * kUpCo / 8u;
     const uint32_t obase = (img * d.h + ty0) * d.w + tx0;
 #pragma unroll
     for (uint32_t i = tid; i < kChunks; i += kBlock) {
-        const uint32_t p = i / (kCo / 8u), c8 = i % (kCo / 8u);
+        const uint32_t p = i / (kUpCo / 8u), c8 = i % (kUpCo / 8u);
         const uint32_t op = obase + (p / kTile) * d.w + p % kTile;
-        store8(out.p + (op * out.stride + out.off + co0 + c8 * 8u),
+        store8(out.p + (op * out.stride + out.off + c8 * 8u),
                *reinterpret_cast<const bf16x8*>(stage + p * kPitch + c8 * 16u));
     }
 }
 
+// The merged weights of the coarse part, for each parity class (py, px) and
+// each of its 2 x 2 coarse taps (a, b): the sum of the 3 x 3 taps (dy, dx)
+// that land on coarse offset (a + py - 1, b + px - 1) (floor((py + dy - 1) / 2)
+// = a + py - 1, and likewise in x), summed in fp32 from the bf16 weights.
+// Stored in MFMA fragment order: [class][K-step s][column tile t][lane][8],
+// lane (kq, ln) holding output channel 16 t + ln, input channels 8 (q % G) ..
+// + 7 of merged tap q / G, q = 4 s + kq, G = cup / 8; a wave reads one
+// 16-byte piece per lane and step.
 __global__ void __launch_bounds__(kBlock) up_merge_k(const bf16* __restrict__ w, uint32_t cout, uint32_t cin,
                                                      uint32_t cup, bf16* __restrict__ wup) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= 16u * cout * cup) return;
-    const uint32_t ci = i % cup, co = (i / cup) % cout, ct = i / (cup * cout);
-    const int py = int(ct >> 3), px = int((ct >> 2) & 1u), a = int((ct >> 1) & 1u), bq = int(ct & 1u);
+    const uint32_t G = cup / 8u, nt = cout / 16u;
+    const uint32_t j = i % 8u, lane = (i / 8u) % 64u, t = (i / 512u) % nt, st = (i / (512u * nt)) % G;
+    const uint32_t cls = i / (512u * nt * G);
+    const uint32_t q = 4u * st + (lane >> 4), ab = q / G, g = q % G;
+    const int py = int(cls >> 1), px = int(cls & 1u), a = int(ab >> 1), bq = int(ab & 1u);
+    const uint32_t co = t * 16u + (lane & 15u), ci = g * 8u + j;
     float s = 0.0f;
     for (int dy = 0; dy < 3; ++dy) {
         if (((py + dy + 1) >> 1) - 1 != a + py - 1) continue;  // floor((py + dy - 1) / 2)
@@ -568,17 +546,20 @@ __global__ void __launch_bounds__(kBlock) up_merge_k(const bf16* __restrict__ w,
 size_t up_merged_elems(uint32_t cout, uint32_t cup) { return size_t(16) * cout * cup; }
 
 hipError_t up_merge_weights(const bf16* w, uint32_t cout, uint32_t cin, uint32_t cup, bf16* wup, hipStream_t s) {
-    if (cup == 0 || cup > cin || cup % 8u || cout == 0) return hipErrorInvalidValue;
+    if (cup == 0 || cup > cin || cup % 8u || cout == 0 || cout % 16u) return hipErrorInvalidValue;
     const uint32_t n = 16u * cout * cup;
     hipLaunchKernelGGL(up_merge_k, dim3((n + kBlock - 1u) / kBlock), dim3(kBlock), 0, s, w, cout, cin, cup, wup);
     return hipGetLastError();
 }
 
-template <int NCT>
 static void conv_up_launch(const Tens& in, uint32_t cin, const bf16* w, const bf16* wup, const float* bias,
-                           const Tens& out, uint32_t cout, Dims d, const Tens& up, uint32_t cup, hipStream_t s) {
-    const dim3 grid(d.n * (d.h / kTile) * (d.w / kTile), cout / (uint32_t(NCT) * 16u));
-    hipLaunchKernelGGL(conv_up_k<NCT>, grid, dim3(kBlock), 0, s, in, cin, w, wup, bias, out, cout, d, up, cup);
+                           const Tens& out, Dims d, const Tens& up, uint32_t cup, hipStream_t s) {
+    const dim3 grid(d.n * (d.h / kTile) * (d.w / kTile));
+    const bf16x8* wf = reinterpret_cast<const bf16x8*>(wup);
+    if (cup == 48u)
+        hipLaunchKernelGGL(conv_up_k<6>, grid, dim3(kBlock), 0, s, in, cin, w, wf, bias, out, d, up);
+    else
+        hipLaunchKernelGGL(conv_up_k<4>, grid, dim3(kBlock), 0, s, in, cin, w, wf, bias, out, d, up);
 }
 
 hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bias, const Tens& out, uint32_t cout,
@@ -626,8 +607,9 @@ hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bia
     // the block's merged weights (16 taps) cut it to 2 blocks per CU by LDS,
     // and at 64 two output groups restage both halos (each measured slower
     // than the 3x3 form over the coarse reads)
-    if (fz.up.p && fz.wup && staged && !pool.p && !upw.p && cout == 32u && d.h % kTile == 0 && d.w % kTile == 0) {
-        conv_up_launch<2>(in, cin, w, fz.wup, bias, out, cout, d, fz.up, fz.cup, s);
+    if (fz.up.p && fz.wup && staged && !pool.p && !upw.p && cout == kUpCo && (fz.cup == 48u || fz.cup == 32u) &&
+        cin == fz.cup + 32u && d.h % kTile == 0 && d.w % kTile == 0) {
+        conv_up_launch(in, cin, w, fz.wup, bias, out, d, fz.up, fz.cup, s);
         return hipGetLastError();
     }
     switch (nct) {
