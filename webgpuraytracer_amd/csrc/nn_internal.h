@@ -83,9 +83,11 @@ hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bia
 // [9][cout][cin] (cin-contiguous rows, 16-byte aligned, with the input
 // scales folded in), and the accumulated product is multiplied by
 // wscale[co] before the bias.  inv_scale has cin floats (16-byte aligned).
+// fuse: pool and up/cup as for conv3x3 (up needs a ReLU layer), upw as a
+// pass after the convolution; no frame texels.
 hipError_t conv3x3_fp8(const Tens& in, uint32_t cin, const uint8_t* w, const float* inv_scale, const float* wscale,
                        const float* bias, const Tens& out, uint32_t cout, Dims d, bool relu, float* out_f32,
-                       uint32_t f32_channels, hipStream_t s);
+                       uint32_t f32_channels, hipStream_t s, const ConvFuse* fuse = nullptr);
 
 // amax[ch] = max(amax[ch], max over P pixels of |x[p][ch]|) for ch < c
 // (c a multiple of 8, at most 256), the maxima as float bit patterns.

@@ -669,12 +669,17 @@ __device__ __forceinline__ uint32_t e4m3x4(float a, float b, float c, float d) {
     return uint32_t(__builtin_amdgcn_cvt_pk_fp8_f32(c, d, lo, true));
 }
 
-template <int NCT, int SL, bool RELU, bool F32OUT, bool FULL>
+// FULL (all NCT output tiles present) stages the bf16 output tile for 16-byte
+// stores, those past the image's edge dropped, and can write its 2x2 average
+// pool (fz.pool, as conv3x3_k).  UP: input channels below fz.cup come from
+// the half-resolution fz.up at the nearest coarse pixel, the others from `in`
+// (channels [0, cin - cup)), as in conv3x3_k.
+template <int NCT, int SL, bool RELU, bool F32OUT, bool FULL, bool UP = false>
 __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, const uint8_t* __restrict__ wq,
                                                        const float* __restrict__ inv_scale,
                                                        const float* __restrict__ wscale,
                                                        const float* __restrict__ bias, Tens out, uint32_t cout, Dims d,
-                                                       float* __restrict__ of32, uint32_t f32c) {
+                                                       float* __restrict__ of32, uint32_t f32c, ConvFuse fz) {
     __shared__ ConvLds8<NCT, SL> lds;
     ConvSmem8<NCT, SL>& sm = lds.m;
     constexpr uint32_t kG16 = uint32_t(SL) / 16u, kG8 = uint32_t(SL) / 8u;  // 16- and 8-channel groups per slice
@@ -692,6 +697,7 @@ __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, co
     const uint32_t nct = FULL ? uint32_t(NCT) : min(uint32_t(NCT), (cout - co0) / 16u);
     // halo piece k of this thread: pixel piece / 8, channels (piece % 8) * 8 .. + 7 of the slice
     const bf16* hsrc[kHaloPer];
+    const bf16* csrc[UP ? kHaloPer : 1];
     bool hval[kHaloPer];
 #pragma unroll
     for (uint32_t k = 0; k < kHaloPer; ++k) {
@@ -701,6 +707,10 @@ __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, co
         hval[k] = piece < kHalo8Pieces && gy >= 0 && gy < int(d.h) && gx >= 0 && gx < int(d.w);
         const size_t sp = hval[k] ? (size_t(img) * d.h + uint32_t(gy)) * d.w + uint32_t(gx) : 0;
         hsrc[k] = in.p + sp * in.stride + in.off + (piece % kG8) * 8u;
+        if (UP) {
+            const size_t cp = hval[k] ? (size_t(img) * (d.h / 2u) + uint32_t(gy) / 2u) * (d.w / 2u) + uint32_t(gx) / 2u : 0;
+            csrc[UP ? k : 0] = fz.up.p + cp * fz.up.stride + fz.up.off + (piece % kG8) * 8u;
+        }
     }
     u32x2 hreg[kHaloPer];
     i32x4 wreg[kWPer];
@@ -710,7 +720,8 @@ __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, co
             const uint32_t c = c0 + ((tid + k * kBlock) % kG8) * 8u;
             hreg[k] = u32x2{0u, 0u};
             if (hval[k] && c < cin) {
-                const bf16x8 v = load8(hsrc[k] + c0);
+                const bf16x8 v = UP && c < fz.cup ? load8(csrc[UP ? k : 0] + c0)
+                                                  : load8(hsrc[k] + c0 - (UP ? fz.cup : 0u));
                 const f32x4 s0 = *reinterpret_cast<const f32x4*>(inv_scale + c);
                 const f32x4 s1 = *reinterpret_cast<const f32x4*>(inv_scale + c + 4u);
                 hreg[k] = u32x2{e4m3x4(f(v[0]) * s0[0], f(v[1]) * s0[1], f(v[2]) * s0[2], f(v[3]) * s0[3]),
@@ -812,9 +823,28 @@ __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, co
 #pragma unroll
         for (uint32_t i = tid; i < kChunks; i += kBlock) {
             const uint32_t p = i / (kCo / 8u), c8 = i % (kCo / 8u);
+            if (uint32_t(ty0) + p / kTile >= d.h || uint32_t(tx0) + p % kTile >= d.w) continue;
             const size_t op = (size_t(img) * d.h + uint32_t(ty0) + p / kTile) * d.w + uint32_t(tx0) + p % kTile;
             store8(out.p + op * out.stride + out.off + co0 + c8 * 8u,
                    *reinterpret_cast<const bf16x8*>(stage + p * kPitch + c8 * 16u));
+        }
+        if (fz.pool.p) {  // the 8 x 8 pooled tile, summed in pool2's order
+            constexpr uint32_t kQ = kTile / 2u;
+            for (uint32_t i = tid; i < kQ * kQ * (kCo / 8u); i += kBlock) {
+                const uint32_t q = i / (kCo / 8u), c8 = i % (kCo / 8u), qy = q / kQ, qx = q % kQ;
+                if (uint32_t(ty0) + 2u * qy >= d.h || uint32_t(tx0) + 2u * qx >= d.w) continue;  // (d.h, d.w even)
+                const uint32_t p00 = 2u * qy * kTile + 2u * qx;
+                const char* sb = stage + c8 * 16u;
+                const bf16x8 v0 = *reinterpret_cast<const bf16x8*>(sb + p00 * kPitch);
+                const bf16x8 v1 = *reinterpret_cast<const bf16x8*>(sb + (p00 + 1u) * kPitch);
+                const bf16x8 v2 = *reinterpret_cast<const bf16x8*>(sb + (p00 + kTile) * kPitch);
+                const bf16x8 v3 = *reinterpret_cast<const bf16x8*>(sb + (p00 + kTile + 1u) * kPitch);
+                bf16x8 o;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) o[k] = b(0.25f * (f(v0[k]) + f(v1[k]) + f(v2[k]) + f(v3[k])));
+                const size_t cp = (size_t(img) * (d.h / 2u) + uint32_t(ty0) / 2u + qy) * (d.w / 2u) + uint32_t(tx0) / 2u + qx;
+                store8(fz.pool.p + (cp * fz.pool.stride + fz.pool.off + co0 + c8 * 8u), o);
+            }
         }
         return;
     }
@@ -847,44 +877,51 @@ __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, co
 template <int NCT, int SL>
 static void conv8_launch_sl(const Tens& in, uint32_t cin, const uint8_t* w, const float* inv_scale,
                             const float* wscale, const float* bias, const Tens& out, uint32_t cout, Dims d, bool relu,
-                            float* of32, uint32_t f32c, hipStream_t s) {
+                            float* of32, uint32_t f32c, bool full, const ConvFuse& fz, hipStream_t s) {
     const uint32_t tiles = d.n * ((d.h + kTile - 1u) / kTile) * ((d.w + kTile - 1u) / kTile);
     const uint32_t groups = (cout / 16u + NCT - 1u) / NCT;
     const dim3 grid(tiles, groups);
-    const bool full = cout % (uint32_t(NCT) * 16u) == 0 && d.h % kTile == 0 && d.w % kTile == 0 &&
-                      out.stride % 8u == 0 && out.off % 8u == 0;
+#define PT_CONV8(RELU_, F32_, FULL_, UP_)                                                                        \
+    hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, RELU_, F32_, FULL_, UP_>), grid, dim3(kBlock), 0, s, in, cin, w,   \
+                       inv_scale, wscale, bias, out, cout, d, of32, f32c, fz)
     if (of32)
-        hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, false, true, false>), grid, dim3(kBlock), 0, s, in, cin, w,
-                           inv_scale, wscale, bias, out, cout, d, of32, f32c);
+        PT_CONV8(false, true, false, false);
+    else if (fz.up.p && full)
+        PT_CONV8(true, false, true, true);
+    else if (fz.up.p)
+        PT_CONV8(true, false, false, true);
     else if (relu && full)
-        hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, true, false, true>), grid, dim3(kBlock), 0, s, in, cin, w,
-                           inv_scale, wscale, bias, out, cout, d, of32, f32c);
+        PT_CONV8(true, false, true, false);
     else if (relu)
-        hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, true, false, false>), grid, dim3(kBlock), 0, s, in, cin, w,
-                           inv_scale, wscale, bias, out, cout, d, of32, f32c);
+        PT_CONV8(true, false, false, false);
     else if (full)
-        hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, false, false, true>), grid, dim3(kBlock), 0, s, in, cin, w,
-                           inv_scale, wscale, bias, out, cout, d, of32, f32c);
+        PT_CONV8(false, false, true, false);
     else
-        hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, false, false, false>), grid, dim3(kBlock), 0, s, in, cin, w,
-                           inv_scale, wscale, bias, out, cout, d, of32, f32c);
+        PT_CONV8(false, false, false, false);
+#undef PT_CONV8
 }
 
 template <int NCT>
 static void conv8_launch(const Tens& in, uint32_t cin, const uint8_t* w, const float* inv_scale, const float* wscale,
                          const float* bias, const Tens& out, uint32_t cout, Dims d, bool relu, float* of32,
-                         uint32_t f32c, hipStream_t s) {
+                         uint32_t f32c, bool full, const ConvFuse& fz, hipStream_t s) {
     if (cin <= 32u)
-        conv8_launch_sl<NCT, 32>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, of32, f32c, s);
+        conv8_launch_sl<NCT, 32>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, of32, f32c, full, fz, s);
     else
-        conv8_launch_sl<NCT, 64>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, of32, f32c, s);
+        conv8_launch_sl<NCT, 64>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, of32, f32c, full, fz, s);
 }
 
 hipError_t conv3x3_fp8(const Tens& in, uint32_t cin, const uint8_t* w, const float* inv_scale, const float* wscale,
                        const float* bias, const Tens& out, uint32_t cout, Dims d, bool relu, float* out_f32,
-                       uint32_t f32_channels, hipStream_t s) {
+                       uint32_t f32_channels, hipStream_t s, const ConvFuse* fuse) {
     if (cin == 0 || cin % 16u || cout % 16u || cout == 0 || cout > kMaxCout || in.stride % 8u || in.off % 8u ||
         !w || !inv_scale || !wscale || (reinterpret_cast<uintptr_t>(w) | reinterpret_cast<uintptr_t>(inv_scale)) % 16u)
+        return hipErrorInvalidValue;
+    ConvFuse fz{};
+    if (fuse) fz = *fuse;
+    fz.fin = nullptr;  // (no frame texels: the output convolution stays bf16)
+    if (fz.up.p && (out_f32 || !relu || fz.cup == 0 || fz.cup % 16u || fz.cup >= cin || fz.up.stride % 8u ||
+                    fz.up.off % 8u || d.h % 2u || d.w % 2u))
         return hipErrorInvalidValue;
     if (d.pixels() == 0) return hipSuccess;
     const uint32_t ct = cout / 16u;
@@ -896,13 +933,21 @@ hipError_t conv3x3_fp8(const Tens& in, uint32_t cin, const uint8_t* w, const flo
         while (n > 1u && ct % n) --n;
         nct = n;
     }
+    const bool full = !out_f32 && cout % (nct * 16u) == 0 && out.stride % 8u == 0 && out.off % 8u == 0;
+    const Tens pool = fz.pool;
+    const Tens upw = fz.upw;
+    if (!full || d.h % 2u || d.w % 2u || pool.stride % 8u || pool.off % 8u) fz.pool = Tens{nullptr, 0, 0};
+    fz.upw = Tens{nullptr, 0, 0};
     switch (nct) {
-        case 1: conv8_launch<1>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
-        case 2: conv8_launch<2>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
-        case 3: conv8_launch<3>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
-        default: conv8_launch<4>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, out_f32, f32_channels, s); break;
+        case 1: conv8_launch<1>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, out_f32, f32_channels, full, fz, s); break;
+        case 2: conv8_launch<2>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, out_f32, f32_channels, full, fz, s); break;
+        case 3: conv8_launch<3>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, out_f32, f32_channels, full, fz, s); break;
+        default: conv8_launch<4>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, out_f32, f32_channels, full, fz, s); break;
     }
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && pool.p && !fz.pool.p) e = pool2(out, cout, d, pool, s);
+    if (e == hipSuccess && upw.p) e = up2(out, cout, d, upw, s);
+    return e;
 }
 
 // amax[ch] = max(amax[ch], max over pixels of |x[p][ch]|), as float bits (a

@@ -282,12 +282,9 @@ int conv_fwd(pt_ctx* c, UNet& u, bool q8, uint32_t li, const Tens& in, const Ten
     // decoder convolution in e4m3 took a 1080p frame from 1.13x to 1.93x the
     // bf16 relMSE)
     if (q8 && li != E0A && li != D0B && li != OUT) {
-        hipError_t e = conv3x3_fp8(in, l.cin, static_cast<const uint8_t*>(u.w8.p) + l.wbf_off, fp(u.inv_sa) + l.a_off,
-                                   fp(u.sw) + l.s_off, fp(u.params) + l.b_off, out, l.cout, d, l.relu, of32,
-                                   of32 ? 3u : 0u, c->stream);
-        // the fp8 kernel has no fused epilogue work: pooling and upsampling follow it
-        if (e == hipSuccess && fuse && fuse->pool.p) e = pool2(out, l.cout, d, fuse->pool, c->stream);
-        if (e == hipSuccess && fuse && fuse->upw.p) e = up2(out, l.cout, d, fuse->upw, c->stream);
+        const hipError_t e = conv3x3_fp8(in, l.cin, static_cast<const uint8_t*>(u.w8.p) + l.wbf_off,
+                                         fp(u.inv_sa) + l.a_off, fp(u.sw) + l.s_off, fp(u.params) + l.b_off, out,
+                                         l.cout, d, l.relu, of32, of32 ? 3u : 0u, c->stream, fuse);
         return hip_rc(c, e, "unet conv fp8");
     }
     return hip_rc(c,
@@ -299,21 +296,21 @@ int conv_fwd(pt_ctx* c, UNet& u, bool q8, uint32_t li, const Tens& in, const Ten
 // The forward pass over u.in0; q8: the fp8 convolutions (inference only);
 // infer: no backward follows.  The encoder's second convolutions also write
 // the pooled copy of their output (in the convolution's epilogue where its
-// output tile is staged).  For bf16 inference each decoder level's first
+// output tile is staged).  For inference each decoder level's first
 // convolution reads the upsampled half of its input straight from the coarse
-// map; otherwise (training, calibration, fp8) the convolution below writes
-// the upsampled copy into the level's concatenation buffer.  out_fuse: work
+// map; otherwise (training, calibration) the convolution below writes the
+// upsampled copy into the level's concatenation buffer.  out_fuse: work
 // for the output convolution (the frame texels of pt_unet_denoise).
 int forward(pt_ctx* c, UNet& u, bool q8 = false, const ConvFuse* out_fuse = nullptr, bool infer = false) {
     const Dims* D = u.dims;
     int rc;
     const Tens none{nullptr, 0, 0};
-    // inference in bf16 (no backward, no calibration) skips the upsampled
+    // inference (no backward, no calibration) skips the upsampled
     // copies: the decoders' first convolutions read the coarse maps themselves
     // (1080p frame 1.07 -> 1.02 ms against writing the copy in the epilogue of
     // the convolution below), and the encoders write their skip outputs densely
     // (into the idle gradient buffers) instead of into the concatenation
-    const bool coarse = infer && !q8 && !u.calib && !u.calib_hist;
+    const bool coarse = infer && !u.calib && !u.calib_hist;
     // encoder
     Tens x = T(u.in0, kInC);
     for (int l = 0; l < 4; ++l) {
