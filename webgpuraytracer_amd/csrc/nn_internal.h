@@ -59,6 +59,13 @@ struct ConvFuse {
     uint32_t fw, fh;
     float exposure;
     const bf16* wup = nullptr;
+    // (conv3x3_fp8 only) in8: `in` and up hold e4m3 bytes quantized with the
+    // layer's inv_scale (strides and offsets in bytes); q_out / q_pool: write
+    // the output / the pooled output as e4m3 bytes quantized with these
+    // per-channel inverse scales (16-byte aligned; strides and offsets in bytes)
+    bool in8 = false;
+    const float* q_out = nullptr;
+    const float* q_pool = nullptr;
 };
 
 // A 3x3 convolution over the nearest 2x upsampling of a coarse map, per

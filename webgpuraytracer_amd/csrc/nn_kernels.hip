@@ -673,8 +673,13 @@ __device__ __forceinline__ uint32_t e4m3x4(float a, float b, float c, float d) {
 // stores, those past the image's edge dropped, and can write its 2x2 average
 // pool (fz.pool, as conv3x3_k).  UP: input channels below fz.cup come from
 // the half-resolution fz.up at the nearest coarse pixel, the others from `in`
-// (channels [0, cin - cup)), as in conv3x3_k.
-template <int NCT, int SL, bool RELU, bool F32OUT, bool FULL, bool UP = false>
+// (channels [0, cin - cup)), as in conv3x3_k.  IN8: `in` (and fz.up) hold
+// e4m3 bytes already quantized with inv_scale (channel strides and offsets in
+// bytes); the halo is staged as read.  OUT8 (FULL, bf16 output): the staged
+// bf16 tile leaves as e4m3 bytes quantized with fz.q_out[co] (the consuming
+// layer's inverse input scales), the pooled tile likewise with fz.q_pool when
+// set: the bytes the consumer would compute from the bf16 values.
+template <int NCT, int SL, bool RELU, bool F32OUT, bool FULL, bool UP = false, bool IN8 = false, bool OUT8 = false>
 __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, const uint8_t* __restrict__ wq,
                                                        const float* __restrict__ inv_scale,
                                                        const float* __restrict__ wscale,
@@ -696,8 +701,9 @@ __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, co
     const uint32_t co0 = blockIdx.y * kCo;
     const uint32_t nct = FULL ? uint32_t(NCT) : min(uint32_t(NCT), (cout - co0) / 16u);
     // halo piece k of this thread: pixel piece / 8, channels (piece % 8) * 8 .. + 7 of the slice
-    const bf16* hsrc[kHaloPer];
-    const bf16* csrc[UP ? kHaloPer : 1];
+    constexpr uint32_t kE = IN8 ? 1u : 2u;  // bytes per staged input element in memory
+    const char* hsrc[kHaloPer];
+    const char* csrc[UP ? kHaloPer : 1];
     bool hval[kHaloPer];
 #pragma unroll
     for (uint32_t k = 0; k < kHaloPer; ++k) {
@@ -706,10 +712,11 @@ __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, co
         const int gy = ty0 + int(j / kHalo) - 1, gx = tx0 + int(j % kHalo) - 1;
         hval[k] = piece < kHalo8Pieces && gy >= 0 && gy < int(d.h) && gx >= 0 && gx < int(d.w);
         const size_t sp = hval[k] ? (size_t(img) * d.h + uint32_t(gy)) * d.w + uint32_t(gx) : 0;
-        hsrc[k] = in.p + sp * in.stride + in.off + (piece % kG8) * 8u;
+        hsrc[k] = reinterpret_cast<const char*>(in.p) + (sp * in.stride + in.off + (piece % kG8) * 8u) * kE;
         if (UP) {
             const size_t cp = hval[k] ? (size_t(img) * (d.h / 2u) + uint32_t(gy) / 2u) * (d.w / 2u) + uint32_t(gx) / 2u : 0;
-            csrc[UP ? k : 0] = fz.up.p + cp * fz.up.stride + fz.up.off + (piece % kG8) * 8u;
+            csrc[UP ? k : 0] =
+                reinterpret_cast<const char*>(fz.up.p) + (cp * fz.up.stride + fz.up.off + (piece % kG8) * 8u) * kE;
         }
     }
     u32x2 hreg[kHaloPer];
@@ -720,8 +727,13 @@ __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, co
             const uint32_t c = c0 + ((tid + k * kBlock) % kG8) * 8u;
             hreg[k] = u32x2{0u, 0u};
             if (hval[k] && c < cin) {
-                const bf16x8 v = UP && c < fz.cup ? load8(csrc[UP ? k : 0] + c0)
-                                                  : load8(hsrc[k] + c0 - (UP ? fz.cup : 0u));
+                const char* src = UP && c < fz.cup ? csrc[UP ? k : 0] + size_t(c0) * kE
+                                                   : hsrc[k] + (size_t(c0) - (UP ? fz.cup : 0u)) * kE;
+                if (IN8) {
+                    hreg[k] = *reinterpret_cast<const u32x2*>(src);
+                    continue;
+                }
+                const bf16x8 v = load8(reinterpret_cast<const bf16*>(src));
                 const f32x4 s0 = *reinterpret_cast<const f32x4*>(inv_scale + c);
                 const f32x4 s1 = *reinterpret_cast<const f32x4*>(inv_scale + c + 4u);
                 hreg[k] = u32x2{e4m3x4(f(v[0]) * s0[0], f(v[1]) * s0[1], f(v[2]) * s0[2], f(v[3]) * s0[3]),
@@ -819,14 +831,34 @@ __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, co
                 }
         }
         __syncthreads();
-        constexpr uint32_t kChunks = kTile * kTile * kCo / 8u;
+        if (OUT8) {  // 16 channels of a pixel per 16-byte store
+            constexpr uint32_t kC16 = kCo / 16u;
 #pragma unroll
-        for (uint32_t i = tid; i < kChunks; i += kBlock) {
-            const uint32_t p = i / (kCo / 8u), c8 = i % (kCo / 8u);
-            if (uint32_t(ty0) + p / kTile >= d.h || uint32_t(tx0) + p % kTile >= d.w) continue;
-            const size_t op = (size_t(img) * d.h + uint32_t(ty0) + p / kTile) * d.w + uint32_t(tx0) + p % kTile;
-            store8(out.p + op * out.stride + out.off + co0 + c8 * 8u,
-                   *reinterpret_cast<const bf16x8*>(stage + p * kPitch + c8 * 16u));
+            for (uint32_t i = tid; i < kTile * kTile * kC16; i += kBlock) {
+                const uint32_t p = i / kC16, c16 = i % kC16;
+                if (uint32_t(ty0) + p / kTile >= d.h || uint32_t(tx0) + p % kTile >= d.w) continue;
+                const size_t op = (size_t(img) * d.h + uint32_t(ty0) + p / kTile) * d.w + uint32_t(tx0) + p % kTile;
+                const bf16x8 v0 = *reinterpret_cast<const bf16x8*>(stage + p * kPitch + c16 * 32u);
+                const bf16x8 v1 = *reinterpret_cast<const bf16x8*>(stage + p * kPitch + c16 * 32u + 16u);
+                const f32x4* sc = reinterpret_cast<const f32x4*>(fz.q_out + co0 + c16 * 16u);
+                const f32x4 s0 = sc[0], s1 = sc[1], s2 = sc[2], s3 = sc[3];
+                const u32x4 q{e4m3x4(f(v0[0]) * s0[0], f(v0[1]) * s0[1], f(v0[2]) * s0[2], f(v0[3]) * s0[3]),
+                              e4m3x4(f(v0[4]) * s1[0], f(v0[5]) * s1[1], f(v0[6]) * s1[2], f(v0[7]) * s1[3]),
+                              e4m3x4(f(v1[0]) * s2[0], f(v1[1]) * s2[1], f(v1[2]) * s2[2], f(v1[3]) * s2[3]),
+                              e4m3x4(f(v1[4]) * s3[0], f(v1[5]) * s3[1], f(v1[6]) * s3[2], f(v1[7]) * s3[3])};
+                *reinterpret_cast<u32x4*>(reinterpret_cast<uint8_t*>(out.p) + op * out.stride + out.off + co0 +
+                                          c16 * 16u) = q;
+            }
+        } else {
+            constexpr uint32_t kChunks = kTile * kTile * kCo / 8u;
+#pragma unroll
+            for (uint32_t i = tid; i < kChunks; i += kBlock) {
+                const uint32_t p = i / (kCo / 8u), c8 = i % (kCo / 8u);
+                if (uint32_t(ty0) + p / kTile >= d.h || uint32_t(tx0) + p % kTile >= d.w) continue;
+                const size_t op = (size_t(img) * d.h + uint32_t(ty0) + p / kTile) * d.w + uint32_t(tx0) + p % kTile;
+                store8(out.p + op * out.stride + out.off + co0 + c8 * 8u,
+                       *reinterpret_cast<const bf16x8*>(stage + p * kPitch + c8 * 16u));
+            }
         }
         if (fz.pool.p) {  // the 8 x 8 pooled tile, summed in pool2's order
             constexpr uint32_t kQ = kTile / 2u;
@@ -843,7 +875,16 @@ __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, co
 #pragma unroll
                 for (int k = 0; k < 8; ++k) o[k] = b(0.25f * (f(v0[k]) + f(v1[k]) + f(v2[k]) + f(v3[k])));
                 const size_t cp = (size_t(img) * (d.h / 2u) + uint32_t(ty0) / 2u + qy) * (d.w / 2u) + uint32_t(tx0) / 2u + qx;
-                store8(fz.pool.p + (cp * fz.pool.stride + fz.pool.off + co0 + c8 * 8u), o);
+                if (OUT8 && fz.q_pool) {  // e4m3 with the pooled map's consumer's scales
+                    const f32x4* sc = reinterpret_cast<const f32x4*>(fz.q_pool + co0 + c8 * 8u);
+                    const f32x4 s0 = sc[0], s1 = sc[1];
+                    const u32x2 q{e4m3x4(f(o[0]) * s0[0], f(o[1]) * s0[1], f(o[2]) * s0[2], f(o[3]) * s0[3]),
+                                  e4m3x4(f(o[4]) * s1[0], f(o[5]) * s1[1], f(o[6]) * s1[2], f(o[7]) * s1[3])};
+                    *reinterpret_cast<u32x2*>(reinterpret_cast<uint8_t*>(fz.pool.p) + cp * fz.pool.stride +
+                                              fz.pool.off + co0 + c8 * 8u) = q;
+                } else {
+                    store8(fz.pool.p + (cp * fz.pool.stride + fz.pool.off + co0 + c8 * 8u), o);
+                }
             }
         }
         return;
@@ -881,9 +922,22 @@ static void conv8_launch_sl(const Tens& in, uint32_t cin, const uint8_t* w, cons
     const uint32_t tiles = d.n * ((d.h + kTile - 1u) / kTile) * ((d.w + kTile - 1u) / kTile);
     const uint32_t groups = (cout / 16u + NCT - 1u) / NCT;
     const dim3 grid(tiles, groups);
-#define PT_CONV8(RELU_, F32_, FULL_, UP_)                                                                        \
-    hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, RELU_, F32_, FULL_, UP_>), grid, dim3(kBlock), 0, s, in, cin, w,   \
-                       inv_scale, wscale, bias, out, cout, d, of32, f32c, fz)
+#define PT_CONV8(RELU_, F32_, FULL_, UP_, ...)                                                                   \
+    hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, RELU_, F32_, FULL_, UP_, ##__VA_ARGS__>), grid, dim3(kBlock), 0, s, in, \
+                       cin, w, inv_scale, wscale, bias, out, cout, d, of32, f32c, fz)
+    // the e4m3-in-memory forms (full ReLU layers): E0B reads bf16 and writes
+    // e4m3, the level-0 decoder reads e4m3 and writes bf16, the others both
+    if (fz.in8 || fz.q_out) {
+        if (fz.up.p && fz.q_out)
+            PT_CONV8(true, false, true, true, true, true);
+        else if (fz.up.p)
+            PT_CONV8(true, false, true, true, true, false);
+        else if (fz.in8)
+            PT_CONV8(true, false, true, false, true, true);
+        else
+            PT_CONV8(true, false, true, false, false, true);
+        return;
+    }
     if (of32)
         PT_CONV8(false, true, false, false);
     else if (fz.up.p && full)
@@ -923,6 +977,15 @@ hipError_t conv3x3_fp8(const Tens& in, uint32_t cin, const uint8_t* w, const flo
     if (fz.up.p && (out_f32 || !relu || fz.cup == 0 || fz.cup % 16u || fz.cup >= cin || fz.up.stride % 8u ||
                     fz.up.off % 8u || d.h % 2u || d.w % 2u))
         return hipErrorInvalidValue;
+    // the e4m3 forms: ReLU layers with bf16-free full output groups; an e4m3
+    // input without coarse reads also has e4m3 output, e4m3 pooling needs
+    // e4m3 output
+    const bool e8 = fz.in8 || fz.q_out;
+    if (e8 && (out_f32 || !relu || fz.upw.p || (fz.in8 && !fz.up.p && !fz.q_out) || (fz.up.p && !fz.in8) ||
+               (fz.q_pool && !fz.q_out) || (fz.pool.p && fz.q_out && !fz.q_pool) ||
+               (fz.q_out && (out.stride % 16u || out.off % 16u || reinterpret_cast<uintptr_t>(fz.q_out) % 16u)) ||
+               (fz.q_pool && reinterpret_cast<uintptr_t>(fz.q_pool) % 16u)))
+        return hipErrorInvalidValue;
     if (d.pixels() == 0) return hipSuccess;
     const uint32_t ct = cout / 16u;
     uint32_t nct = ct <= 4u ? ct : (ct % 4u == 0 ? 4u : (ct % 3u == 0 ? 3u : 4u));
@@ -934,6 +997,8 @@ hipError_t conv3x3_fp8(const Tens& in, uint32_t cin, const uint8_t* w, const flo
         nct = n;
     }
     const bool full = !out_f32 && cout % (nct * 16u) == 0 && out.stride % 8u == 0 && out.off % 8u == 0;
+    if (e8 && (!full || (fz.pool.p && (d.h % 2u || d.w % 2u || fz.pool.stride % 8u || fz.pool.off % 8u))))
+        return hipErrorInvalidValue;  // (no separate pass can take e4m3 data)
     const Tens pool = fz.pool;
     const Tens upw = fz.upw;
     if (!full || d.h % 2u || d.w % 2u || pool.stride % 8u || pool.off % 8u) fz.pool = Tens{nullptr, 0, 0};

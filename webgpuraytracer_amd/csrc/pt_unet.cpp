@@ -262,8 +262,13 @@ int alloc_acts(pt_ctx* c, UNet& u, Dims d0) {
     return PT_OK;
 }
 
+// e8 (fp8 inference with the coarse-read layout): activations between two
+// fp8 layers are e4m3 bytes in memory, quantized by the producing layer with
+// the inverse input scales of the layer that reads them (the same bytes the
+// reader would compute from the bf16 values); the buffers are the bf16 ones,
+// used at one byte per channel.
 int conv_fwd(pt_ctx* c, UNet& u, bool q8, uint32_t li, const Tens& in, const Tens& out, Dims d,
-             float* of32 = nullptr, const ConvFuse* fuse = nullptr) {
+             float* of32 = nullptr, const ConvFuse* fuse = nullptr, bool e8 = false) {
     const Layer& l = u.L[li];
     if (u.calib) {
         const hipError_t e = chan_amax(in, l.cin, d.pixels(), u.calib + l.a_off, c->stream);
@@ -282,9 +287,20 @@ int conv_fwd(pt_ctx* c, UNet& u, bool q8, uint32_t li, const Tens& in, const Ten
     // decoder convolution in e4m3 took a 1080p frame from 1.13x to 1.93x the
     // bf16 relMSE)
     if (q8 && li != E0A && li != D0B && li != OUT) {
+        ConvFuse f8{};
+        if (fuse) f8 = *fuse;
+        if (e8) {
+            const bool enc_b = li <= E3B && li % 2u == 1u;  // its output is a skip; its pooled copy feeds li + 1
+            const uint32_t lvl = (li - 1u) / 2u;
+            const uint32_t next = enc_b ? D3A + 2u * (3u - lvl) : li + 1u;  // the layer reading the output
+            f8.in8 = li != E0B;                                             // E0A's output stays bf16
+            if (next != D0B && next != OUT)
+                f8.q_out = fp(u.inv_sa) + u.L[next].a_off + (enc_b ? kC[lvl + 1] : 0u);
+            if (enc_b) f8.q_pool = fp(u.inv_sa) + u.L[li + 1u].a_off;
+        }
         const hipError_t e = conv3x3_fp8(in, l.cin, static_cast<const uint8_t*>(u.w8.p) + l.wbf_off,
                                          fp(u.inv_sa) + l.a_off, fp(u.sw) + l.s_off, fp(u.params) + l.b_off, out,
-                                         l.cout, d, l.relu, of32, of32 ? 3u : 0u, c->stream, fuse);
+                                         l.cout, d, l.relu, of32, of32 ? 3u : 0u, c->stream, &f8);
         return hip_rc(c, e, "unet conv fp8");
     }
     return hip_rc(c,
@@ -311,20 +327,21 @@ int forward(pt_ctx* c, UNet& u, bool q8 = false, const ConvFuse* out_fuse = null
     // the convolution below), and the encoders write their skip outputs densely
     // (into the idle gradient buffers) instead of into the concatenation
     const bool coarse = infer && !u.calib && !u.calib_hist;
+    const bool e8 = q8 && coarse;
     // encoder
     Tens x = T(u.in0, kInC);
     for (int l = 0; l < 4; ++l) {
         const uint32_t cc = kC[l + 1] + kC[l];
         const Tens skip = coarse ? T(u.g_tA[l], kC[l]) : T(u.cat[l], cc, kC[l + 1]);
         const ConvFuse fz{T(u.pin[l + 1], kC[l]), none, none, 0u, nullptr, nullptr, 0u, 0u, 0.0f};
-        if ((rc = conv_fwd(c, u, q8, E0A + 2 * l, x, T(u.tA[l], kC[l]), D[l]))) return rc;
-        if ((rc = conv_fwd(c, u, q8, E0B + 2 * l, T(u.tA[l], kC[l]), skip, D[l], nullptr, &fz))) return rc;
+        if ((rc = conv_fwd(c, u, q8, E0A + 2 * l, x, T(u.tA[l], kC[l]), D[l], nullptr, nullptr, e8))) return rc;
+        if ((rc = conv_fwd(c, u, q8, E0B + 2 * l, T(u.tA[l], kC[l]), skip, D[l], nullptr, &fz, e8))) return rc;
         x = fz.pool;
     }
     // the bottleneck's output goes up into cat[3], each decoder level's (but the last) into the next one's
-    if ((rc = conv_fwd(c, u, q8, B4A, x, T(u.t4A, kC[4]), D[4]))) return rc;
+    if ((rc = conv_fwd(c, u, q8, B4A, x, T(u.t4A, kC[4]), D[4], nullptr, nullptr, e8))) return rc;
     const ConvFuse up3{none, coarse ? none : T(u.cat[3], kC[4] + kC[3], 0), none, 0u, nullptr, nullptr, 0u, 0u, 0.0f};
-    if ((rc = conv_fwd(c, u, q8, B4B, T(u.t4A, kC[4]), T(u.b4, kC[4]), D[4], nullptr, &up3))) return rc;
+    if ((rc = conv_fwd(c, u, q8, B4B, T(u.t4A, kC[4]), T(u.b4, kC[4]), D[4], nullptr, &up3, e8))) return rc;
     for (int l = 3; l >= 0; --l) {
         const uint32_t cc = kC[l + 1] + kC[l];
         const Tens low = l == 3 ? T(u.b4, kC[4]) : T(u.dd[l + 1], kC[l + 1]);
@@ -333,10 +350,10 @@ int forward(pt_ctx* c, UNet& u, bool q8 = false, const ConvFuse* out_fuse = null
         const ConvFuse upl{none, l > 0 && !coarse ? T(u.cat[l - 1], kC[l] + kC[l - 1], 0) : none, none, 0u, nullptr,
                            nullptr, 0u, 0u, 0.0f};
         if ((rc = conv_fwd(c, u, q8, D3A + 2 * (3 - l), coarse ? T(u.g_tA[l], kC[l]) : T(u.cat[l], cc),
-                           T(u.dA[l], kC[l]), D[l], nullptr, coarse ? &rd : nullptr)))
+                           T(u.dA[l], kC[l]), D[l], nullptr, coarse ? &rd : nullptr, e8)))
             return rc;
         if ((rc = conv_fwd(c, u, q8, D3B + 2 * (3 - l), T(u.dA[l], kC[l]), T(u.dd[l], kC[l]), D[l], nullptr,
-                           l > 0 ? &upl : nullptr)))
+                           l > 0 ? &upl : nullptr, e8)))
             return rc;
     }
     return conv_fwd(c, u, q8, OUT, T(u.dd[0], kC[0]), none, D[0], fp(u.pred), out_fuse);
