@@ -54,6 +54,18 @@ __device__ __forceinline__ void frame_texel(const float* q, float exposure, floa
     }
 }
 
+// Four floats as OCP e4m3 bytes, clamped to +-448 first (v_cvt_pk_fp8_f32
+// turns larger magnitudes into NaN rather than saturating), rounded to
+// nearest even.
+__device__ __forceinline__ uint32_t e4m3x4(float a, float b, float c, float d) {
+    a = fminf(fmaxf(a, -448.0f), 448.0f);
+    b = fminf(fmaxf(b, -448.0f), 448.0f);
+    c = fminf(fmaxf(c, -448.0f), 448.0f);
+    d = fminf(fmaxf(d, -448.0f), 448.0f);
+    const int lo = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+    return uint32_t(__builtin_amdgcn_cvt_pk_fp8_f32(c, d, lo, true));
+}
+
 // ------------------------------------------------------------- conv fwd --
 // A block computes a 16 x 16 pixel tile of one image for a group of up to
 // NCT x 16 output channels.  Per 32-channel slice of the input it stages the
@@ -89,7 +101,10 @@ struct ConvSmem {
 // then holds only the skip channels).  PF
 // (launches of few blocks over several slices, where the extra registers
 // cost no occupancy): the next slice is fetched during this slice's products.
-template <int NCT, bool RELU, bool F32OUT, bool FULL, bool UP = false, bool PF = false>
+// OUT8 (FULL bf16 output, no pooling or upsampling): the staged bf16 tile
+// leaves as e4m3 bytes quantized with fz.q_out[co] (the next layer's fp8
+// input scales; strides and offsets in bytes).
+template <int NCT, bool RELU, bool F32OUT, bool FULL, bool UP = false, bool PF = false, bool OUT8 = false>
 __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const bf16* __restrict__ w,
                                                     const float* __restrict__ bias, Tens out, uint32_t cout, Dims d,
                                                     float* __restrict__ of32, uint32_t f32c, ConvFuse fz) {
@@ -261,8 +276,28 @@ __global__ void __launch_bounds__(kBlock) conv3x3_k(Tens in, uint32_t cin, const
                 *reinterpret_cast<i16x4*>(stage + p * kPitch + (uint32_t(t) * 16u + kq * 4u) * 2u) = qi;
             }
         __syncthreads();
-        constexpr uint32_t kChunks = kTile * kTile * kCo / 8u;  // 16-byte pieces of the tile
         const uint32_t obase = (img * d.h + ty0) * d.w + tx0;
+        if (OUT8) {  // 16 channels of a pixel per 16-byte store
+            constexpr uint32_t kC16 = kCo / 16u;
+#pragma unroll
+            for (uint32_t i = tid; i < kTile * kTile * kC16; i += kBlock) {
+                const uint32_t p = i / kC16, c16 = i % kC16;
+                if (ty0 + p / kTile >= d.h || tx0 + p % kTile >= d.w) continue;
+                const uint32_t op = obase + (p / kTile) * d.w + p % kTile;
+                const bf16x8 v0 = *reinterpret_cast<const bf16x8*>(stage + p * kPitch + c16 * 32u);
+                const bf16x8 v1 = *reinterpret_cast<const bf16x8*>(stage + p * kPitch + c16 * 32u + 16u);
+                const f32x4* sc = reinterpret_cast<const f32x4*>(fz.q_out + co0 + c16 * 16u);
+                const f32x4 s0 = sc[0], s1 = sc[1], s2 = sc[2], s3 = sc[3];
+                const u32x4 q{e4m3x4(f(v0[0]) * s0[0], f(v0[1]) * s0[1], f(v0[2]) * s0[2], f(v0[3]) * s0[3]),
+                              e4m3x4(f(v0[4]) * s1[0], f(v0[5]) * s1[1], f(v0[6]) * s1[2], f(v0[7]) * s1[3]),
+                              e4m3x4(f(v1[0]) * s2[0], f(v1[1]) * s2[1], f(v1[2]) * s2[2], f(v1[3]) * s2[3]),
+                              e4m3x4(f(v1[4]) * s3[0], f(v1[5]) * s3[1], f(v1[6]) * s3[2], f(v1[7]) * s3[3])};
+                *reinterpret_cast<u32x4*>(reinterpret_cast<uint8_t*>(out.p) + op * out.stride + out.off + co0 +
+                                          c16 * 16u) = q;
+            }
+            return;
+        }
+        constexpr uint32_t kChunks = kTile * kTile * kCo / 8u;  // 16-byte pieces of the tile
 #pragma unroll
         for (uint32_t i = tid; i < kChunks; i += kBlock) {
             const uint32_t p = i / (kCo / 8u), c8 = i % (kCo / 8u);
@@ -350,7 +385,9 @@ static void conv_launch(const Tens& in, uint32_t cin, const bf16* w, const float
             PT_CONV(NCT, RELU_, false, FULL_, UP_);   \
     } while (0)
     const bool pf = cin > kSlice && tiles * groups < kConvPfBlocks;
-    if (of32 && f32_full)
+    if (fz.q_out)  // (checked by the caller: a full ReLU layer without coarse reads, pooling or upsampling)
+        PT_CONV(NCT, true, false, true, false, false, true);
+    else if (of32 && f32_full)
         PT_CONV(NCT, false, true, true);
     else if (of32)
         PT_CONV(NCT, false, true, false);
@@ -600,6 +637,10 @@ hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bia
         (fz.up.p && (d.h % 2u || d.w % 2u || !relu || out_f32 || fz.cup % 8u || fz.cup > cin ||
                      fz.up.stride % 8u || fz.up.off % 8u)))
         return hipErrorInvalidValue;
+    // e4m3 output (for an fp8 layer reading it): a full ReLU layer only
+    if (fz.q_out && (!staged || !relu || fz.pool.p || fz.upw.p || fz.up.p || fz.in8 || out.stride % 16u ||
+                     out.off % 16u || reinterpret_cast<uintptr_t>(fz.q_out) % 16u))
+        return hipErrorInvalidValue;
     const Tens pool = fz.pool, upw = fz.upw;
     if (!staged) fz.pool = fz.upw = Tens{nullptr, 0, 0};
     // a decoder level's first convolution with merged weights: the coarse
@@ -660,14 +701,6 @@ union ConvLds8 {
     char stage[kTile * kTile * (NCT * 32 + 16)];
 };
 
-__device__ __forceinline__ uint32_t e4m3x4(float a, float b, float c, float d) {
-    a = fminf(fmaxf(a, -448.0f), 448.0f);
-    b = fminf(fmaxf(b, -448.0f), 448.0f);
-    c = fminf(fmaxf(c, -448.0f), 448.0f);
-    d = fminf(fmaxf(d, -448.0f), 448.0f);
-    const int lo = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
-    return uint32_t(__builtin_amdgcn_cvt_pk_fp8_f32(c, d, lo, true));
-}
 
 // FULL (all NCT output tiles present) stages the bf16 output tile for 16-byte
 // stores, those past the image's edge dropped, and can write its 2x2 average
@@ -925,8 +958,8 @@ static void conv8_launch_sl(const Tens& in, uint32_t cin, const uint8_t* w, cons
 #define PT_CONV8(RELU_, F32_, FULL_, UP_, ...)                                                                   \
     hipLaunchKernelGGL((conv3x3_f8_k<NCT, SL, RELU_, F32_, FULL_, UP_, ##__VA_ARGS__>), grid, dim3(kBlock), 0, s, in, \
                        cin, w, inv_scale, wscale, bias, out, cout, d, of32, f32c, fz)
-    // the e4m3-in-memory forms (full ReLU layers): E0B reads bf16 and writes
-    // e4m3, the level-0 decoder reads e4m3 and writes bf16, the others both
+    // the e4m3-in-memory forms (full ReLU layers): e4m3 in and out; e4m3 in
+    // with coarse reads and bf16 out (the level-0 decoder); bf16 in, e4m3 out
     if (fz.in8 || fz.q_out) {
         if (fz.up.p && fz.q_out)
             PT_CONV8(true, false, true, true, true, true);

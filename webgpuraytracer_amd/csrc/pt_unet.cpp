@@ -293,7 +293,7 @@ int conv_fwd(pt_ctx* c, UNet& u, bool q8, uint32_t li, const Tens& in, const Ten
             const bool enc_b = li <= E3B && li % 2u == 1u;  // its output is a skip; its pooled copy feeds li + 1
             const uint32_t lvl = (li - 1u) / 2u;
             const uint32_t next = enc_b ? D3A + 2u * (3u - lvl) : li + 1u;  // the layer reading the output
-            f8.in8 = li != E0B;                                             // E0A's output stays bf16
+            f8.in8 = true;  // (the first layer writes e4m3 for the second, see below)
             if (next != D0B && next != OUT)
                 f8.q_out = fp(u.inv_sa) + u.L[next].a_off + (enc_b ? kC[lvl + 1] : 0u);
             if (enc_b) f8.q_pool = fp(u.inv_sa) + u.L[li + 1u].a_off;
@@ -302,6 +302,14 @@ int conv_fwd(pt_ctx* c, UNet& u, bool q8, uint32_t li, const Tens& in, const Ten
                                          fp(u.inv_sa) + l.a_off, fp(u.sw) + l.s_off, fp(u.params) + l.b_off, out,
                                          l.cout, d, l.relu, of32, of32 ? 3u : 0u, c->stream, &f8);
         return hip_rc(c, e, "unet conv fp8");
+    }
+    // the first convolution stays bf16 but writes its output as e4m3 for the
+    // second, an fp8 layer
+    ConvFuse fq{};
+    if (e8 && li == E0A) {
+        if (fuse) fq = *fuse;
+        fq.q_out = fp(u.inv_sa) + u.L[E0B].a_off;
+        fuse = &fq;
     }
     return hip_rc(c,
                   conv3x3(in, l.cin, bfp(u.wbf) + l.wbf_off, fp(u.params) + l.b_off, out, l.cout, d, l.relu, of32,
