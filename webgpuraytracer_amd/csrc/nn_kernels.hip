@@ -735,8 +735,11 @@ __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, co
     const uint32_t nct = FULL ? uint32_t(NCT) : min(uint32_t(NCT), (cout - co0) / 16u);
     // halo piece k of this thread: pixel piece / 8, channels (piece % 8) * 8 .. + 7 of the slice
     constexpr uint32_t kE = IN8 ? 1u : 2u;  // bytes per staged input element in memory
-    const char* hsrc[kHaloPer];
-    const char* csrc[UP ? kHaloPer : 1];
+    // byte offsets of this thread's halo pieces (32-bit: checked by the host)
+    const char* inb = reinterpret_cast<const char*>(in.p);
+    const char* upb = reinterpret_cast<const char*>(fz.up.p);
+    uint32_t hsrc[kHaloPer];
+    uint32_t csrc[UP ? kHaloPer : 1];
     bool hval[kHaloPer];
 #pragma unroll
     for (uint32_t k = 0; k < kHaloPer; ++k) {
@@ -744,12 +747,11 @@ __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, co
         const uint32_t j = piece / kG8;
         const int gy = ty0 + int(j / kHalo) - 1, gx = tx0 + int(j % kHalo) - 1;
         hval[k] = piece < kHalo8Pieces && gy >= 0 && gy < int(d.h) && gx >= 0 && gx < int(d.w);
-        const size_t sp = hval[k] ? (size_t(img) * d.h + uint32_t(gy)) * d.w + uint32_t(gx) : 0;
-        hsrc[k] = reinterpret_cast<const char*>(in.p) + (sp * in.stride + in.off + (piece % kG8) * 8u) * kE;
+        const uint32_t sp = hval[k] ? (img * d.h + uint32_t(gy)) * d.w + uint32_t(gx) : 0u;
+        hsrc[k] = (sp * in.stride + in.off + (piece % kG8) * 8u) * kE;
         if (UP) {
-            const size_t cp = hval[k] ? (size_t(img) * (d.h / 2u) + uint32_t(gy) / 2u) * (d.w / 2u) + uint32_t(gx) / 2u : 0;
-            csrc[UP ? k : 0] =
-                reinterpret_cast<const char*>(fz.up.p) + (cp * fz.up.stride + fz.up.off + (piece % kG8) * 8u) * kE;
+            const uint32_t cp = hval[k] ? (img * (d.h / 2u) + uint32_t(gy) / 2u) * (d.w / 2u) + uint32_t(gx) / 2u : 0u;
+            csrc[UP ? k : 0] = (cp * fz.up.stride + fz.up.off + (piece % kG8) * 8u) * kE;
         }
     }
     u32x2 hreg[kHaloPer];
@@ -760,8 +762,9 @@ __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, co
             const uint32_t c = c0 + ((tid + k * kBlock) % kG8) * 8u;
             hreg[k] = u32x2{0u, 0u};
             if (hval[k] && c < cin) {
-                const char* src = UP && c < fz.cup ? csrc[UP ? k : 0] + size_t(c0) * kE
-                                                   : hsrc[k] + (size_t(c0) - (UP ? fz.cup : 0u)) * kE;
+                // (the skip offset wraps modulo 2^32 to the right channel)
+                const char* src = UP && c < fz.cup ? upb + (csrc[UP ? k : 0] + c0 * kE)
+                                                   : inb + (hsrc[k] + (c0 - (UP ? fz.cup : 0u)) * kE);
                 if (IN8) {
                     hreg[k] = *reinterpret_cast<const u32x2*>(src);
                     continue;
@@ -809,12 +812,13 @@ __global__ void __launch_bounds__(kBlock) conv3x3_f8_k(Tens in, uint32_t cin, co
 #pragma unroll
         for (int t = 0; t < NCT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
     const i32x8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
-    fetch(0);
+    // slices are fetched, staged and multiplied in turn (as conv3x3_k: the
+    // staging registers are free during the products)
     for (uint32_t c0 = 0; c0 < cin; c0 += uint32_t(SL)) {
+        fetch(c0);
         __syncthreads();  // the previous slice's reads are done
         stash();
         __syncthreads();
-        if (c0 + uint32_t(SL) < cin) fetch(c0 + uint32_t(SL));
 #pragma unroll
         for (uint32_t s = 0; s < kSteps; ++s) {
             const uint32_t q = s * 4u + kq;                 // this lane group's slot
@@ -1020,6 +1024,9 @@ hipError_t conv3x3_fp8(const Tens& in, uint32_t cin, const uint8_t* w, const flo
                (fz.q_pool && reinterpret_cast<uintptr_t>(fz.q_pool) % 16u)))
         return hipErrorInvalidValue;
     if (d.pixels() == 0) return hipSuccess;
+    // the kernel's halo offsets are 32-bit bytes
+    const uint64_t P = d.pixels(), lim = 1ull << 31;
+    if (2u * P * (in.stride + 8u) >= lim || 2u * P * (fz.up.stride + 8u) >= lim) return hipErrorInvalidValue;
     const uint32_t ct = cout / 16u;
     uint32_t nct = ct <= 4u ? ct : (ct % 4u == 0 ? 4u : (ct % 3u == 0 ? 3u : 4u));
     // a launch of fewer blocks than CUs takes smaller output-channel groups
