@@ -85,7 +85,9 @@ int pt_unet_denoise(pt_ctx* ctx, float exposure);
  * then 1024-bin histograms up to them); one binade of headroom, larger values
  * saturate at 448; the weights are
  * stored in e4m3 as w * s_a[ci] / s_w[co], s_w = that row's max / 448.
- * Activations stay bf16 between layers.  Training (pt_unet_train,
+ * Between two fp8 layers the activations are held as those e4m3 values (the
+ * bf16 output quantized by its producer with the reader's s_a), elsewhere as
+ * bf16.  Training (pt_unet_train,
  * pt_unet_loss_grad) always runs in bf16; pt_unet_init, pt_unet_set_params
  * and pt_unet_train return inference to PT_UNET_BF16. */
 enum { PT_UNET_BF16 = 0, PT_UNET_FP8 = 1 };
