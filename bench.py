@@ -100,6 +100,9 @@ def main():
                          "(counted work is then of that tree)")
     ap.add_argument("--lds-stack", type=int, default=None,
                     help="traversal-stack entries per lane kept in LDS (PT_OPT_LDS_STACK; default the library's)")
+    ap.add_argument("--refill", type=int, default=None,
+                    help="idle lanes that make a traversal wave fetch new rays (PT_OPT_WAVEFRONT_REFILL; "
+                         "default the library's)")
     ap.add_argument("--compare", action="store_true",
                     help="also time the other backend/schedule variants (reported under alt)")
     args = ap.parse_args()
@@ -129,6 +132,8 @@ def main():
     r.set_option(A.PT_OPT_COUNT_STATS, 0)
     if args.lds_stack is not None:
         r.set_option(A.PT_OPT_LDS_STACK, args.lds_stack)
+    if args.refill is not None:
+        r.set_option(A.PT_OPT_WAVEFRONT_REFILL, args.refill)
     d = scene.desc()
     mk = (A.PT_OPT_BACKEND, A.PT_BACKEND_MEGAKERNEL)
     regen, sync = (A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_REGEN), (A.PT_OPT_SCHEDULE, A.PT_SCHEDULE_SYNC)
