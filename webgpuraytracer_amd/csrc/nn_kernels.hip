@@ -996,7 +996,9 @@ template <int NCT>
 static void conv8_launch(const Tens& in, uint32_t cin, const uint8_t* w, const float* inv_scale, const float* wscale,
                          const float* bias, const Tens& out, uint32_t cout, Dims d, bool relu, float* of32,
                          uint32_t f32c, bool full, const ConvFuse& fz, hipStream_t s) {
-    if (cin <= 32u)
+    // 32-channel slices also where 64 would leave most of the last slice
+    // empty (80 = 32 + 32 + 16 channels: 9 K-steps of 128 instead of 10)
+    if (cin <= 32u || cin % 64u == 16u)
         conv8_launch_sl<NCT, 32>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, of32, f32c, full, fz, s);
     else
         conv8_launch_sl<NCT, 64>(in, cin, w, inv_scale, wscale, bias, out, cout, d, relu, of32, f32c, full, fz, s);
