@@ -487,13 +487,14 @@ __global__ void __launch_bounds__(kWfBlock) wf_accumulate(WfArgs A) {
 // bounce_ev (optional): bounce_limit + 1 events, recorded before each
 // bounce's launches and after the last.
 template <uint32_t F, bool CN, int W>
-static void wf_bounces(const DevScene& S, const WfArgs& a0, uint32_t blocks, hipStream_t s, hipEvent_t* bounce_ev) {
+static void wf_bounces(const DevScene& S, const WfArgs& a0, uint32_t shade_blocks, hipStream_t s,
+                       hipEvent_t* bounce_ev) {
     const size_t lds = size_t(a0.stack_lds) * kWfBlock * 4u;
     int cur = 0;
     for (int b = 0; b < S.bounce_limit; ++b) {
         if (bounce_ev) (void)hipEventRecord(bounce_ev[b], s);
         hipLaunchKernelGGL((wf_trace<F, CN, W, false>), dim3(a0.trace_grid), dim3(kWfBlock), lds, s, S, a0, cur, b);
-        hipLaunchKernelGGL((wf_shade<F, CN>), dim3((a0.n_items + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, S, a0, cur, b);
+        hipLaunchKernelGGL((wf_shade<F, CN>), dim3(shade_blocks), dim3(kShadeBlock), 0, s, S, a0, cur, b);
         hipLaunchKernelGGL((wf_trace<F, CN, W, true>), dim3(a0.trace_grid), dim3(kWfBlock), lds, s, S, a0, cur, b);
         cur ^= 1;
     }
@@ -510,14 +511,15 @@ static hipError_t wf_run(const DevScene& S, const WfArgs& a0, bool cn, hipStream
         const hipError_t e = launch_zero_u64(reinterpret_cast<unsigned long long*>(a0.rad), 2u * a0.n_items, s);
         if (e != hipSuccess) return e;
     }
+    const uint32_t sb = (a0.n_items + uint32_t(kShadeBlock) - 1u) / uint32_t(kShadeBlock);
     if (cn)
-        wf_bounces<F, true, kTravPairs>(S, a0, blocks, s, bounce_ev);
+        wf_bounces<F, true, kTravPairs>(S, a0, sb, s, bounce_ev);
     else if (a0.wide == kTravQuant)
-        wf_bounces<F, false, kTravQuant>(S, a0, blocks, s, bounce_ev);
+        wf_bounces<F, false, kTravQuant>(S, a0, sb, s, bounce_ev);
     else if (a0.wide == kTravWide)
-        wf_bounces<F, false, kTravWide>(S, a0, blocks, s, bounce_ev);
+        wf_bounces<F, false, kTravWide>(S, a0, sb, s, bounce_ev);
     else
-        wf_bounces<F, false, kTravPairs>(S, a0, blocks, s, bounce_ev);
+        wf_bounces<F, false, kTravPairs>(S, a0, sb, s, bounce_ev);
     const uint32_t npix = a0.sec_w * a0.sec_h;
     hipLaunchKernelGGL(wf_accumulate, dim3((npix + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, s, a0);
     return hipGetLastError();
