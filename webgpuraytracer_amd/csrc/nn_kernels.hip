@@ -409,45 +409,47 @@ static void conv_launch(const Tens& in, uint32_t cin, const bf16* w, const float
 
 // ------------------------------------------------ conv over [up2(C) | S] --
 // A decoder level's first convolution over [nearest 2x upsampling of the
-// coarse map C (channels [0, cup)) | skip S (32 channels, [cup, cin), at
-// channels [0, 32) of `in`)] for 32 output channels, 16 x 16 tiles of full
-// size.  Wave w computes the output pixels of parity class (py, px) = (w / 2,
-// w % 2): coarse positions Y, X = 0..7 of the tile, as 4 MFMA tiles of 16
-// (tile m: Y = 2m + ln / 8, X = ln % 8).  The coarse part takes the 4 merged
-// taps of its class over a 10 x 10 coarse halo, the skip part the 9 taps over
-// the 18 x 18 halo at the class's pixels, both into the same accumulators
-// (started at the bias).  One staging round: both halos and the skip weights
+// coarse map C (channels [0, cup), cup = 8 GC) | skip S (channels [cup, cin),
+// 8 GS of them, at channels [0, 8 GS) of `in`)] for NCT x 16 output channels,
+// 16 x 16 tiles of full size.  Wave w computes the output pixels of parity
+// class (py, px) = (w / 2, w % 2): coarse positions Y, X = 0..7 of the tile,
+// as 4 MFMA tiles of 16 (tile m: Y = 2m + ln / 8, X = ln % 8).  The coarse
+// part takes the 4 merged taps of its class over a 10 x 10 coarse halo, the
+// skip part the 9 taps over the 18 x 18 halo at the class's pixels, both into
+// the same accumulators (started at the bias).  One staging round: both halos
 // go to LDS together; the merged weights come from global memory already in
-// MFMA fragment order (see up_merge_k), each wave reading its class's.  The
-// coarse part's K runs over (merged tap, 8-channel group) pieces: K-step s
-// gives lane group kq piece q = 4s + kq, tap q / GC, group q % GC.
+// MFMA fragment order (see up_merge_k), each wave reading its class's, and the
+// skip weights from global memory as well.  Both parts run K over (tap,
+// 8-channel group) pieces: K-step s gives lane group kq piece q = 4s + kq, tap
+// q / G, group q % G (G = GC or GS), so no step carries empty channels but the
+// skip part's last (9 GS a multiple of 4 or not).
 constexpr uint32_t kUpHalo = 10, kUpHaloPix = kUpHalo * kUpHalo;
-constexpr uint32_t kUpCo = 32;  // output channels (two MFMA column tiles)
 
-template <uint32_t GC>
+template <uint32_t GC, uint32_t GS, int NCT>
 union UpSmem {
     struct {
         bf16x8 hc[GC * kUpHaloPix];  // [8-channel group][coarse pixel]
-        bf16x8 hs[4 * kHaloPix];     // [8-channel group][pixel]
+        bf16x8 hs[GS * kHaloPix];    // [8-channel group][pixel]
     } m;
-    char stage[kTile * kTile * (kUpCo * 2u + 16u)];
+    char stage[kTile * kTile * (uint32_t(NCT) * 32u + 16u)];
 };
 
-template <uint32_t GC>
+template <uint32_t GC, uint32_t GS, int NCT>
 __global__ void __launch_bounds__(kBlock) conv_up_k(Tens in, uint32_t cin, const bf16* __restrict__ w,
                                                     const bf16x8* __restrict__ wup, const float* __restrict__ bias,
                                                     Tens out, Dims d, Tens up) {
-    __shared__ UpSmem<GC> sm;
-    constexpr uint32_t NCT = 2, cup = GC * 8u;
+    __shared__ UpSmem<GC, GS, NCT> sm;
+    constexpr uint32_t cup = GC * 8u, kCo = uint32_t(NCT) * 16u;
     constexpr uint32_t kCP = GC * kUpHaloPix, kCPer = (kCP + kBlock - 1u) / kBlock;
-    constexpr uint32_t kSP = 4u * kHaloPix, kSPer = (kSP + kBlock - 1u) / kBlock;
+    constexpr uint32_t kSP = GS * kHaloPix, kSPer = (kSP + kBlock - 1u) / kBlock;
+    constexpr uint32_t kSkipSteps = (9u * GS + 3u) / 4u;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6, kq = lane >> 4, ln = lane & 15u;
     const uint32_t py = wv >> 1, px = wv & 1u;
     const uint32_t tiles_x = d.w / kTile, tiles_y = d.h / kTile;
     const uint32_t img = blockIdx.x / (tiles_x * tiles_y), trem = blockIdx.x % (tiles_x * tiles_y);
     const uint32_t ty0 = trem / tiles_x * kTile, tx0 = trem % tiles_x * kTile;
     const uint32_t H2 = d.h / 2u, W2 = d.w / 2u, Y0 = ty0 / 2u, X0 = tx0 / 2u;
-    // every global read of the block, issued together
+    // every halo read of the block, issued together
     bf16x8 hcr[kCPer], hsr[kSPer];
 #pragma unroll
     for (uint32_t k = 0; k < kCPer; ++k) {
@@ -459,7 +461,7 @@ __global__ void __launch_bounds__(kBlock) conv_up_k(Tens in, uint32_t cin, const
     }
 #pragma unroll
     for (uint32_t k = 0; k < kSPer; ++k) {
-        const uint32_t piece = tid + k * kBlock, sp = piece / 4u, g = piece % 4u;
+        const uint32_t piece = tid + k * kBlock, sp = piece / GS, g = piece % GS;
         const uint32_t gy = ty0 + sp / kHalo - 1u, gx = tx0 + sp % kHalo - 1u;
         hsr[k] = piece < kSP && gy < d.h && gx < d.w
                      ? load8(in.p + (((img * d.h + gy) * d.w + gx) * in.stride + in.off + g * 8u))
@@ -467,7 +469,7 @@ __global__ void __launch_bounds__(kBlock) conv_up_k(Tens in, uint32_t cin, const
     }
     f32x4 acc[4][NCT];
 #pragma unroll
-    for (int t = 0; t < int(NCT); ++t) {
+    for (int t = 0; t < NCT; ++t) {
         float bv[4] = {0.f, 0.f, 0.f, 0.f};
         if (bias)
 #pragma unroll
@@ -483,17 +485,17 @@ __global__ void __launch_bounds__(kBlock) conv_up_k(Tens in, uint32_t cin, const
 #pragma unroll
     for (uint32_t k = 0; k < kSPer; ++k) {
         const uint32_t piece = tid + k * kBlock;
-        if (piece < kSP) sm.m.hs[(piece % 4u) * kHaloPix + piece / 4u] = hsr[k];
+        if (piece < kSP) sm.m.hs[(piece % GS) * kHaloPix + piece / GS] = hsr[k];
     }
     __syncthreads();
-    // coarse part: the class's merged taps, K over (tap, 8-channel group) pieces
-    const bf16x8* wf = wup + size_t(wv) * GC * NCT * 64u + lane;
+    // coarse part: the class's merged taps
+    const bf16x8* wf = wup + size_t(wv) * GC * uint32_t(NCT) * 64u + lane;
 #pragma unroll
     for (uint32_t s = 0; s < GC; ++s) {
         const uint32_t q = 4u * s + kq, ab = q / GC, g = q % GC, a = ab >> 1, b2 = ab & 1u;
         bf16x8 cf[4], bw[NCT];
 #pragma unroll
-        for (int t = 0; t < int(NCT); ++t) bw[t] = wf[(s * NCT + uint32_t(t)) * 64u];
+        for (int t = 0; t < NCT; ++t) bw[t] = wf[(s * uint32_t(NCT) + uint32_t(t)) * 64u];
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
             const uint32_t Y = 2u * uint32_t(m) + (ln >> 3), X = ln & 7u;
@@ -502,33 +504,36 @@ __global__ void __launch_bounds__(kBlock) conv_up_k(Tens in, uint32_t cin, const
 #pragma unroll
         for (int m = 0; m < 4; ++m)
 #pragma unroll
-            for (int t = 0; t < int(NCT); ++t)
+            for (int t = 0; t < NCT; ++t)
                 acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[t], cf[m], acc[m][t], 0, 0, 0);
     }
     // skip part: 9 taps over the full-resolution halo, at the class's pixels
 #pragma unroll
-    for (uint32_t tap = 0; tap < 9u; ++tap) {
-        const uint32_t dy = tap / 3u, dx = tap % 3u;
+    for (uint32_t s = 0; s < kSkipSteps; ++s) {
+        const uint32_t q = 4u * s + kq;
+        const bool valid = q < 9u * GS;
+        const uint32_t qa = valid ? q : 0u, tap = qa / GS, g = qa % GS, dy = tap / 3u, dx = tap % 3u;
         bf16x8 a[4], bw[NCT];
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
             const uint32_t Y = 2u * uint32_t(m) + (ln >> 3), X = ln & 7u;
-            a[m] = sm.m.hs[kq * kHaloPix + (2u * Y + py + dy) * kHalo + 2u * X + px + dx];
+            a[m] = valid ? sm.m.hs[g * kHaloPix + (2u * Y + py + dy) * kHalo + 2u * X + px + dx] : zero8();
         }
 #pragma unroll
-        for (int t = 0; t < int(NCT); ++t) bw[t] = load8(w + ((tap * kUpCo + uint32_t(t) * 16u + ln) * cin + cup + kq * 8u));
+        for (int t = 0; t < NCT; ++t)
+            bw[t] = valid ? load8(w + ((tap * kCo + uint32_t(t) * 16u + ln) * cin + cup + g * 8u)) : zero8();
 #pragma unroll
         for (int m = 0; m < 4; ++m)
 #pragma unroll
-            for (int t = 0; t < int(NCT); ++t)
+            for (int t = 0; t < NCT; ++t)
                 acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[t], a[m], acc[m][t], 0, 0, 0);
     }
     // epilogue: ReLU, bf16, staged [pixel][channel] for 16-byte stores
-    constexpr uint32_t kPitch = kUpCo * 2u + 16u;
+    constexpr uint32_t kPitch = kCo * 2u + 16u;
     char* stage = sm.stage;
     __syncthreads();
 #pragma unroll
-    for (int t = 0; t < int(NCT); ++t)
+    for (int t = 0; t < NCT; ++t)
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
             bf16x4 q;
@@ -540,11 +545,11 @@ __global__ void __launch_bounds__(kBlock) conv_up_k(Tens in, uint32_t cin, const
             *reinterpret_cast<i16x4*>(stage + p * kPitch + (uint32_t(t) * 16u + kq * 4u) * 2u) = qi;
         }
     __syncthreads();
-    constexpr uint32_t kChunks = kTile * kTile * kUpCo / 8u;
+    constexpr uint32_t kChunks = kTile * kTile * kCo / 8u;
     const uint32_t obase = (img * d.h + ty0) * d.w + tx0;
 #pragma unroll
     for (uint32_t i = tid; i < kChunks; i += kBlock) {
-        const uint32_t p = i / (kUpCo / 8u), c8 = i % (kUpCo / 8u);
+        const uint32_t p = i / (kCo / 8u), c8 = i % (kCo / 8u);
         const uint32_t op = obase + (p / kTile) * d.w + p % kTile;
         store8(out.p + (op * out.stride + out.off + c8 * 8u),
                *reinterpret_cast<const bf16x8*>(stage + p * kPitch + c8 * 16u));
@@ -589,14 +594,24 @@ hipError_t up_merge_weights(const bf16* w, uint32_t cout, uint32_t cin, uint32_t
     return hipGetLastError();
 }
 
+// The (coarse, skip, output) channel counts conv_up_k is built for: the
+// level-0 and level-1 decoders' first convolutions (and a 32 + 32 form).
+static bool conv_up_shape(uint32_t cup, uint32_t cskip, uint32_t cout) {
+    return (cup == 48u && cskip == 32u && cout == 32u) || (cup == 32u && cskip == 32u && cout == 32u) ||
+           (cup == 64u && cskip == 48u && cout == 48u);
+}
+
 static void conv_up_launch(const Tens& in, uint32_t cin, const bf16* w, const bf16* wup, const float* bias,
-                           const Tens& out, Dims d, const Tens& up, uint32_t cup, hipStream_t s) {
+                           const Tens& out, uint32_t cout, Dims d, const Tens& up, uint32_t cup, hipStream_t s) {
     const dim3 grid(d.n * (d.h / kTile) * (d.w / kTile));
     const bf16x8* wf = reinterpret_cast<const bf16x8*>(wup);
     if (cup == 48u)
-        hipLaunchKernelGGL(conv_up_k<6>, grid, dim3(kBlock), 0, s, in, cin, w, wf, bias, out, d, up);
+        hipLaunchKernelGGL((conv_up_k<6, 4, 2>), grid, dim3(kBlock), 0, s, in, cin, w, wf, bias, out, d, up);
+    else if (cup == 32u)
+        hipLaunchKernelGGL((conv_up_k<4, 4, 2>), grid, dim3(kBlock), 0, s, in, cin, w, wf, bias, out, d, up);
     else
-        hipLaunchKernelGGL(conv_up_k<4>, grid, dim3(kBlock), 0, s, in, cin, w, wf, bias, out, d, up);
+        hipLaunchKernelGGL((conv_up_k<8, 6, 3>), grid, dim3(kBlock), 0, s, in, cin, w, wf, bias, out, d, up);
+    (void)cout;
 }
 
 hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bias, const Tens& out, uint32_t cout,
@@ -644,13 +659,10 @@ hipError_t conv3x3(const Tens& in, uint32_t cin, const bf16* w, const float* bia
     const Tens pool = fz.pool, upw = fz.upw;
     if (!staged) fz.pool = fz.upw = Tens{nullptr, 0, 0};
     // a decoder level's first convolution with merged weights: the coarse
-    // part on the coarse map.  Taken for 32 output channels (full tiles): at 48
-    // the block's merged weights (16 taps) cut it to 2 blocks per CU by LDS,
-    // and at 64 two output groups restage both halos (each measured slower
-    // than the 3x3 form over the coarse reads)
-    if (fz.up.p && fz.wup && staged && !pool.p && !upw.p && cout == kUpCo && (fz.cup == 48u || fz.cup == 32u) &&
-        cin == fz.cup + 32u && d.h % kTile == 0 && d.w % kTile == 0) {
-        conv_up_launch(in, cin, w, fz.wup, bias, out, d, fz.up, fz.cup, s);
+    // part on the coarse map (full tiles, the shapes conv_up_shape lists)
+    if (fz.up.p && fz.wup && staged && !pool.p && !upw.p && conv_up_shape(fz.cup, cin - fz.cup, cout) &&
+        d.h % kTile == 0 && d.w % kTile == 0) {
+        conv_up_launch(in, cin, w, fz.wup, bias, out, cout, d, fz.up, fz.cup, s);
         return hipGetLastError();
     }
     switch (nct) {
